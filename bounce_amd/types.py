@@ -1,0 +1,132 @@
+"""NumPy mirrors of the C records in include/bounce_amd.h.
+
+Each dtype is byte-compatible with the reference type named beside it, so arrays of
+these records can be handed to the C ABI (and to the reference) without conversion.
+"""
+import ctypes
+
+import numpy as np
+
+# b3Transform (transform.h:56-57): rotation columns x, y, z then position.
+XF_FLOATS = 12
+
+# b3amd_shape: b3SphereShape / b3CapsuleShape / b3HullShape fields read by the narrowphase.
+SHAPE_DTYPE = np.dtype([
+    ("type", "<u4"), ("hull", "<u4"), ("radius", "<f4"), ("v", "<f4", (6,)), ("pad", "<u4", (3,)),
+])
+assert SHAPE_DTYPE.itemsize == 48
+
+# b3ManifoldPoint fields written by the narrowphase (manifold.h:66-76).
+POINT_DTYPE = np.dtype([
+    ("localNormal1", "<f4", (3,)), ("localPoint1", "<f4", (3,)), ("localPoint2", "<f4", (3,)),
+    ("triangleKey", "<u4"), ("key", "<u4", (4,)),
+])
+assert POINT_DTYPE.itemsize == 56
+
+# b3Manifold (manifold.h:79-93) without solver impulses.
+MANIFOLD_DTYPE = np.dtype([
+    ("pointCount", "<u4"), ("flags", "<u4"), ("reserved", "<u4", (2,)), ("points", POINT_DTYPE, (4,)),
+])
+assert MANIFOLD_DTYPE.itemsize == 240
+
+# b3ConvexCache = b3SimplexCache (gjk.h:85-92) + b3SATFeaturePair (sat.h:74-80).
+CACHE_DTYPE = np.dtype({
+    "names": ["metric", "iterations", "count", "sindex1", "sindex2",
+              "state", "ftype", "index1", "index2"],
+    "formats": ["<f4", "<u4", "<u2", ("u1", (4,)), ("u1", (4,)), "<u4", "<u4", "<u4", "<u4"],
+    "offsets": [0, 4, 8, 10, 14, 20, 24, 28, 32],
+    "itemsize": 36,
+})
+
+SPHERE, CAPSULE, HULL, MESH = 0, 1, 2, 3
+CACHE_SEPARATION, CACHE_OVERLAP, CACHE_EMPTY = 0, 1, 2
+FEATURE_EDGE1, FEATURE_FACE1, FEATURE_FACE2 = 0, 1, 2
+NULL_KEY = 0xFFFFFFFF
+
+
+def empty_caches(n):
+    """n fresh b3ConvexCache records (count 0, feature state empty)."""
+    c = np.zeros(n, CACHE_DTYPE)
+    c["state"] = CACHE_EMPTY
+    return c
+
+
+class HullPoolStruct(ctypes.Structure):
+    """b3amd_hull_pool / b3o_hull_pool."""
+    _fields_ = [
+        ("count", ctypes.c_uint32),
+        ("desc", ctypes.c_void_p),
+        ("centroids", ctypes.c_void_p),
+        ("vertices", ctypes.c_void_p),
+        ("edges", ctypes.c_void_p),
+        ("faces", ctypes.c_void_p),
+        ("planes", ctypes.c_void_p),
+    ]
+
+
+class HullPool:
+    """A flat pool of b3Hull records (hull.h:37-64) held in NumPy arrays."""
+
+    def __init__(self, desc, centroids, vertices, edges, faces, planes):
+        self.desc = np.ascontiguousarray(desc, dtype=np.uint32).reshape(-1, 6)
+        self.centroids = np.ascontiguousarray(centroids, dtype=np.float32).reshape(-1, 3)
+        self.vertices = np.ascontiguousarray(vertices, dtype=np.float32).reshape(-1, 3)
+        self.edges = np.ascontiguousarray(edges, dtype=np.uint32).reshape(-1, 4)
+        self.faces = np.ascontiguousarray(faces, dtype=np.uint32).reshape(-1)
+        self.planes = np.ascontiguousarray(planes, dtype=np.float32).reshape(-1, 4)
+
+    @property
+    def count(self):
+        return self.desc.shape[0]
+
+    def struct(self):
+        s = HullPoolStruct()
+        s.count = self.count
+        s.desc = self.desc.ctypes.data
+        s.centroids = self.centroids.ctypes.data
+        s.vertices = self.vertices.ctypes.data
+        s.edges = self.edges.ctypes.data
+        s.faces = self.faces.ctypes.data
+        s.planes = self.planes.ctypes.data
+        return s
+
+    def hull(self, h):
+        """(vertices, edges, faces, planes, centroid) of hull h as views."""
+        vo, vn, eo, en, fo, fn = (int(x) for x in self.desc[h])
+        return (self.vertices[vo:vo + vn], self.edges[eo:eo + en], self.faces[fo:fo + fn],
+                self.planes[fo:fo + fn], self.centroids[h])
+
+    @staticmethod
+    def concat(hulls):
+        """Pool from a list of (vertices, edges, faces, planes, centroid) tuples."""
+        desc, cs, vs, es, fs, ps = [], [], [], [], [], []
+        vo = eo = fo = 0
+        for v, e, f, p, c in hulls:
+            v = np.asarray(v, np.float32).reshape(-1, 3)
+            e = np.asarray(e, np.uint32).reshape(-1, 4)
+            f = np.asarray(f, np.uint32).reshape(-1)
+            p = np.asarray(p, np.float32).reshape(-1, 4)
+            desc.append([vo, len(v), eo, len(e), fo, len(f)])
+            cs.append(np.asarray(c, np.float32).reshape(3))
+            vs.append(v), es.append(e), fs.append(f), ps.append(p)
+            vo += len(v)
+            eo += len(e)
+            fo += len(f)
+        return HullPool(np.array(desc, np.uint32), np.array(cs, np.float32), np.concatenate(vs),
+                        np.concatenate(es), np.concatenate(fs), np.concatenate(ps))
+
+    def shapes(self, radius=0.0):
+        """One b3HullShape record per pool hull (shape index == hull index)."""
+        s = np.zeros(self.count, SHAPE_DTYPE)
+        s["type"] = HULL
+        s["hull"] = np.arange(self.count, dtype=np.uint32)
+        s["radius"] = radius
+        return s
+
+
+def ptr(a):
+    """ctypes void pointer to a NumPy array's data (None for None)."""
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"]
+    return ctypes.c_void_p(a.ctypes.data)

@@ -1,0 +1,182 @@
+/*
+ * bounce_amd - MI355X (gfx950) contact narrowphase for the Bounce physics engine.
+ *
+ * C ABI of the drop-in boundary.  Every entry point returns an int status
+ * (B3AMD_OK = 0, negative = error; b3amd_last_error() gives the message) and never
+ * aborts: the reference has no status returns (asserts only, settings.h:115), so an
+ * error here corresponds to what the reference would have asserted on or silently
+ * mis-handled.  All records are plain C structs byte-compatible with the reference
+ * types they replace; there are no GPU-runtime types in any signature.
+ *
+ * Reference call surface replaced (see the per-function notes):
+ *   b3CollideShapeAndShape      collide.h:63-66, collide.cpp:162-188
+ *   b3Collide{Sphere,Capsule,Hull}And*   collide.h:68-97
+ *   b3ConvexCache               collide.h:38-42 (per-contact, persistent)
+ *   b3Manifold / b3ManifoldPoint manifold.h:66-93 (output)
+ *   b3Hull                      hull.h:37-64 (borrowed, immutable geometry)
+ *   b3MeshContact::Collide      mesh_contact.cpp:217-275 (mesh entry)
+ * Caller: b3Contact::Update (contact.cpp:41-140) from
+ * b3ContactManager::UpdateContacts (contact_manager.cpp:167-231).
+ */
+#ifndef BOUNCE_AMD_H
+#define BOUNCE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes ---- */
+#define B3AMD_OK 0
+#define B3AMD_ERR_ARGUMENT (-1)   /* bad pointer, size or index */
+#define B3AMD_ERR_DEVICE (-2)     /* HIP runtime error */
+#define B3AMD_ERR_BUDGET (-3)     /* HBM budget of the context exceeded (admission refused) */
+#define B3AMD_ERR_UNSUPPORTED (-4)/* record outside the supported limits */
+#define B3AMD_ERR_NO_DEVICE (-5)  /* no GPU visible */
+#define B3AMD_ERR_OVERFLOW (-6)   /* a per-pair bounded buffer overflowed (see flags) */
+
+/* ---- records (byte-compatible with the reference) ---- */
+typedef struct { float x, y, z; } b3amd_vec3;                       /* b3Vec3 vec3.h:106 */
+typedef struct { b3amd_vec3 rotation[3]; b3amd_vec3 position; } b3amd_transform; /* b3Transform transform.h:56-57 (rotation columns) */
+typedef struct { float x, y, z, w; } b3amd_quat;                   /* b3Quat quat.h:141 */
+
+/* Flat pool of b3Hull records.  Hull h: desc[6h..6h+5] = vertexOffset, vertexCount,
+ * edgeOffset, edgeCount (half-edges), faceOffset, faceCount; indices inside a hull are
+ * hull-local, exactly as in b3Hull.  Half-edges 2k and 2k+1 must be twins
+ * (sat.cpp:123-128).  Limits: vertexCount <= 255, faceCount <= 255, edgeCount <= 4096. */
+typedef struct {
+    uint32_t count;
+    const uint32_t* desc;
+    const float* centroids;   /* [count][3]   b3Hull::centroid */
+    const float* vertices;    /* [*][3]       b3Vec3 */
+    const uint32_t* edges;    /* [*][4]       b3HalfEdge origin, twin, face, next */
+    const uint32_t* faces;    /* [*]          b3Face::edge */
+    const float* planes;      /* [*][4]       b3Plane normal, offset */
+} b3amd_hull_pool;
+
+/* Shape types follow b3ShapeType (shape.h:32-39). */
+enum { B3AMD_SPHERE = 0, B3AMD_CAPSULE = 1, B3AMD_HULL = 2, B3AMD_MESH = 3 };
+
+/* One convex shape: b3SphereShape (m_center, m_radius), b3CapsuleShape (m_centers[2],
+ * m_radius) or b3HullShape (m_hull -> pool index, m_radius). */
+typedef struct {
+    uint32_t type;
+    uint32_t hull;
+    float radius;
+    float v[6];
+    uint32_t pad[3];
+} b3amd_shape;
+
+/* b3ManifoldPoint fields written by the narrowphase (manifold.h:66-76); key[] holds the
+ * b3FeaturePair words in struct order inEdge1, outEdge1, inEdge2, outEdge2 (clip.h:28-34),
+ * i.e. the two u64 of b3ManifoldPointKey::key1/key2 (manifold.h:41-63). */
+typedef struct {
+    b3amd_vec3 localNormal1;
+    b3amd_vec3 localPoint1;
+    b3amd_vec3 localPoint2;
+    uint32_t triangleKey;
+    uint32_t key[4];
+} b3amd_manifold_point;
+
+/* b3Manifold (manifold.h:79-93) without the solver-owned impulses; flags carries
+ * per-pair status bits (B3AMD_FLAG_*). */
+typedef struct {
+    uint32_t pointCount;
+    uint32_t flags;
+    uint32_t reserved[2];
+    b3amd_manifold_point points[4];
+} b3amd_manifold;
+
+#define B3AMD_FLAG_POLY_OVERFLOW 1u   /* clip polygon exceeded the device bound */
+
+typedef struct { uint32_t state, type, index1, index2; } b3amd_feature_cache; /* b3SATFeaturePair sat.h:74-80 */
+typedef struct {
+    float metric;
+    uint32_t iterations;
+    uint16_t count;
+    uint8_t index1[4];
+    uint8_t index2[4];
+} b3amd_simplex_cache;                                                      /* b3SimplexCache gjk.h:85-92 */
+typedef struct { b3amd_simplex_cache simplex; b3amd_feature_cache feature; } b3amd_convex_cache; /* b3ConvexCache collide.h:38-42 */
+
+/* b3SATCacheType / b3SATFeatureType (sat.h:58-72). */
+enum { B3AMD_CACHE_SEPARATION = 0, B3AMD_CACHE_OVERLAP = 1, B3AMD_CACHE_EMPTY = 2 };
+enum { B3AMD_FEATURE_EDGE1 = 0, B3AMD_FEATURE_FACE1 = 1, B3AMD_FEATURE_FACE2 = 2 };
+
+/* ---- context ---- */
+typedef struct b3amd_context b3amd_context;
+
+/* One context per caller thread; owns one HIP stream on `device`.  hbm_budget_bytes
+ * caps the device memory the context may hold (0 = 64 GiB). */
+int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out);
+int b3amd_destroy(b3amd_context* ctx);
+/* Message of the last error on the calling thread. */
+const char* b3amd_last_error(void);
+/* Bytes of device memory currently held by the context. */
+uint64_t b3amd_memory_in_use(const b3amd_context* ctx);
+
+/* ---- geometry registry ----
+ * Replaces the borrowed `const b3Hull*` of b3HullShape (hull_shape.h:44): the pool is
+ * validated, converted to the device record layout and uploaded once; pairs refer to
+ * hulls by pool index.  A new upload replaces the previous pool. */
+int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool);
+/* The shape table that pair indices refer to (b3Shape records of the world). */
+int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* shapes);
+
+/* ---- batch narrowphase ----
+ * For each pair i: b3CollideShapeAndShape(out[i], xfs[2i], shapes[pairs[2i]],
+ * xfs[2i+1], shapes[pairs[2i+1]], &caches[i]) with the global convex cache on
+ * (collide_hulls.cpp:291).  Shape types must satisfy typeA <= typeB
+ * (contact_manager.cpp:238-242).  sweeps (may be NULL) gives per pair the
+ * orientation0/orientation of body A then body B (4 quats), read only by the cached
+ * face rebuild (collide_hulls_cache.cpp:117-126).  caches (may be NULL: fresh empty
+ * caches) are read and written back.  out[i] must not alias inputs. */
+int b3amd_collide(b3amd_context* ctx, uint32_t n, const uint32_t* pairs,
+                  const b3amd_transform* xfs, const b3amd_quat* sweeps,
+                  b3amd_convex_cache* caches, b3amd_manifold* out);
+
+/* Same on device-resident buffers (pointers from b3amd_device_alloc), enqueued on the
+ * context stream without synchronizing.  d_caches may be NULL only if d_sweeps is NULL. */
+int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs,
+                         const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
+                         b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
+
+int b3amd_synchronize(b3amd_context* ctx);
+
+/* ---- device memory helpers (callers need no other GPU runtime) ---- */
+int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);
+int b3amd_device_free(b3amd_context* ctx, void* ptr);
+int b3amd_copy_to_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
+int b3amd_copy_to_host(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* ---- kernel timing (HIP events on the context stream) ----
+ * While enabled, every kernel the context launches is bracketed by events; stats
+ * accumulate per kernel name until reset.  Names: "sat_hull", "contact_hull", ... */
+int b3amd_set_profiling(b3amd_context* ctx, int enabled);
+int b3amd_kernel_stats(b3amd_context* ctx, const char* name, uint32_t* launches, double* total_ms);
+int b3amd_reset_kernel_stats(b3amd_context* ctx);
+
+/* ---- workload generators (seeded, deterministic on every machine) ---- */
+
+/* `count` hulls from N ~ U{nmin..nmax} points on ellipsoids with semi-axes ~ U[amin,amax].
+ * Output capacities: vertices >= count*nmax*3 floats, edges >= count*(6nmax-12)*4,
+ * faces/planes >= count*(2nmax-4) (planes x4).  Writes totals to sizes[0..2]
+ * (vertices, half-edges, faces). */
+int b3amd_gen_hull_pool(uint64_t seed, uint32_t count, uint32_t nmin, uint32_t nmax,
+                        float amin, float amax, uint32_t* desc, float* centroids,
+                        float* vertices, uint32_t* edges, uint32_t* faces, float* planes,
+                        uint64_t* sizes);
+
+/* n hull-hull pairs over the pool: uniform random orientations; B is placed along a
+ * random unit direction u so that the gap between the supports along u is
+ * g ~ N(gap_mean, gap_sigma).  pairs[2i] < count, pairs[2i+1] < count (hull indices),
+ * xfs[2i], xfs[2i+1] the transforms. */
+int b3amd_gen_hull_pairs(uint64_t seed, const b3amd_hull_pool* pool, uint32_t n,
+                         float gap_mean, float gap_sigma, uint32_t* pairs,
+                         b3amd_transform* xfs);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,730 @@
+/*
+ * TEST INFRASTRUCTURE ONLY - plain-C restatement of the reference contact narrowphase.
+ *
+ * This file is the oracle: tests, the smoke check and bench.py's CPU-baseline leg use it
+ * as the checker.  The product (bounce_amd) never links, loads or calls it.
+ *
+ * Every function restates one reference function in the same FP32 operation order
+ * (compiled with -ffp-contract=off, no fast-math), citing the file:line it follows.
+ * Association order is written out explicitly: b3Dot = (x*x' + y*y') + z*z'
+ * (vec3.h:158-161); matrix*vector is the column combination (mat33.h:143-146).
+ *
+ * With -DB3O_COUNT_FLOPS every FP32 + - * / sqrt executed is counted (FL(n)), giving the
+ * algorithmic op count per pair used for the roofline.
+ *
+ * Pinning (what checks each function against something other than itself):
+ *   - all functions: bit-exact comparison with the compiled reference (oracle/_ref) on
+ *     seeded random hull pairs, boxes and degenerate placements (tests/test_oracle.py);
+ *   - golden fixtures written by the compiled reference (tests/golden/);
+ *   - face/edge queries: brute-force FP64 SAT separation (the separating-axis theorem);
+ *   - face contact: the Box Face Contact known answer (box_face_contact.h:25-49).
+ */
+#include "b3oracle.h"
+
+#include <float.h>
+#include <math.h>
+#include <string.h>
+
+/* ---------------------------------------------------------------- op counting */
+#ifdef B3O_COUNT_FLOPS
+static uint64_t g_flops;
+#define FL(n) (g_flops += (uint64_t)(n))
+uint64_t b3o_flop_count(void) { return g_flops; }
+void b3o_flop_reset(void) { g_flops = 0; }
+#else
+#define FL(n) ((void)0)
+uint64_t b3o_flop_count(void) { return 0; }
+void b3o_flop_reset(void) {}
+#endif
+
+/* ---------------------------------------------------------------- settings.h */
+#define LINEAR_SLOP 0.005f          /* settings.h:62 */
+#define MAX_FLOAT FLT_MAX           /* settings.h:41 */
+#define EPSILON FLT_EPSILON         /* settings.h:42 */
+#define NULL_EDGE 0xFFFFFFFFu       /* clip.h:25 */
+#define NULL_TRIANGLE 0xFFFFFFFFu   /* manifold.h:25 */
+#define MAX_POINTS 4                /* settings.h:81 */
+#define POLY_CAP 256                /* b3StackArray<...,32> grows on the heap; 256 bounds it here */
+
+typedef b3o_vec3 V3;
+typedef b3o_xf XF;
+typedef b3o_plane PL;
+typedef b3o_quat QT;
+typedef struct { V3 c[3]; } M33;
+
+/* ---------------------------------------------------------------- vec3.h */
+static V3 v3(float x, float y, float z) { V3 r = {x, y, z}; return r; }
+static V3 add(V3 a, V3 b) { FL(3); return v3(a.x + b.x, a.y + b.y, a.z + b.z); }        /* vec3.h:140-143 */
+static V3 sub(V3 a, V3 b) { FL(3); return v3(a.x - b.x, a.y - b.y, a.z - b.z); }        /* vec3.h:146-149 */
+static V3 neg(V3 a) { return v3(-a.x, -a.y, -a.z); }                                  /* vec3.h:134-137 */
+static V3 scale(float s, V3 v) { FL(3); return v3(s * v.x, s * v.y, s * v.z); }         /* vec3.h:128-131 */
+static float dot(V3 a, V3 b) { FL(5); return a.x * b.x + a.y * b.y + a.z * b.z; }      /* vec3.h:158-161 */
+static V3 cross(V3 a, V3 b) {                                                          /* vec3.h:164-167 */
+    FL(9);
+    return v3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+static float length(V3 v) { FL(6); return sqrtf(v.x * v.x + v.y * v.y + v.z * v.z); }  /* vec3.h:177-180 */
+static float length_sq(V3 v) { FL(5); return v.x * v.x + v.y * v.y + v.z * v.z; }      /* vec3.h:183-186 */
+static V3 normalize(V3 v) {                                                             /* vec3.h:189-198 */
+    float len = length(v);
+    if (len > EPSILON) { FL(1); float s = 1.0f / len; return scale(s, v); }
+    return v;
+}
+static float normalize_in_place(V3* v) {                                                /* vec3.h:93-104 */
+    FL(6);
+    float len = sqrtf(v->x * v->x + v->y * v->y + v->z * v->z);
+    if (len > EPSILON) {
+        FL(4);
+        float s = 1.0f / len;
+        v->x *= s; v->y *= s; v->z *= s;
+    }
+    return len;
+}
+
+/* ---------------------------------------------------------------- mat33.h / transform.h */
+static V3 mul_mv(const M33* A, V3 v) {                                                  /* mat33.h:143-146 */
+    return add(add(scale(v.x, A->c[0]), scale(v.y, A->c[1])), scale(v.z, A->c[2]));
+}
+static V3 mulT_mv(const M33* A, V3 v) {                                                 /* mat33.h:157-160 */
+    return v3(dot(A->c[0], v), dot(A->c[1], v), dot(A->c[2], v));
+}
+static M33 mulT_mm(const M33* A, const M33* B) {                                        /* mat33.h:163-169 */
+    M33 C;
+    for (int j = 0; j < 3; ++j) C.c[j] = v3(dot(A->c[0], B->c[j]), dot(A->c[1], B->c[j]), dot(A->c[2], B->c[j]));
+    return C;
+}
+static const M33* rot(const XF* T) { return (const M33*)&T->c[0]; }
+static V3 xf_mul(const XF* T, V3 v) { return add(mul_mv(rot(T), v), T->p); }           /* transform.h:74-77 */
+static V3 xf_mulT(const XF* T, V3 v) { return mulT_mv(rot(T), sub(v, T->p)); }        /* transform.h:106-110 */
+static XF xf_mulT_xf(const XF* A, const XF* B) {                                        /* transform.h:88-94 */
+    XF C;
+    M33 R = mulT_mm(rot(A), rot(B));
+    memcpy(&C.c[0], &R, sizeof(R));
+    C.p = mulT_mv(rot(A), sub(B->p, A->p));
+    return C;
+}
+
+/* ---------------------------------------------------------------- geometry.h */
+static PL plane_mul(const XF* T, PL p) {                                                /* geometry.h:77-81 */
+    PL r;
+    r.n = mul_mv(rot(T), p.n);
+    FL(1);
+    r.d = p.d + dot(r.n, T->p);
+    return r;
+}
+static float plane_dist(V3 P, PL p) { FL(1); return dot(p.n, P) - p.d; }               /* geometry.h:90-93 */
+static V3 closest_on_plane(V3 P, PL p) {                                                /* geometry.h:96-100 */
+    float f = plane_dist(P, p);
+    return sub(P, scale(f, p.n));
+}
+static void barycentric2(float out[3], V3 A, V3 B, V3 Q) {                             /* geometry.h:104-116 */
+    V3 AB = sub(B, A), QA = sub(A, Q), QB = sub(B, Q);
+    float divisor = dot(AB, AB);
+    out[0] = dot(QB, AB);
+    out[1] = -dot(QA, AB);
+    out[2] = divisor;
+}
+
+/* ---------------------------------------------------------------- quat.h */
+static QT q_mul(QT a, QT b) {                                                           /* quat.h:172-179 */
+    FL(28);
+    QT r;
+    r.x = a.w * b.x + a.x * b.w + a.y * b.z - a.z * b.y;
+    r.y = a.w * b.y + a.y * b.w + a.z * b.x - a.x * b.z;
+    r.z = a.w * b.z + a.z * b.w + a.x * b.y - a.y * b.x;
+    r.w = a.w * b.w - a.x * b.x - a.y * b.y - a.z * b.z;
+    return r;
+}
+static QT q_conj(QT q) { QT r = {-q.x, -q.y, -q.z, q.w}; return r; }                  /* quat.h:195-198 */
+
+/* ---------------------------------------------------------------- hull.h / hull.inl */
+typedef struct {
+    V3 centroid;
+    uint32_t vertexCount, edgeCount, faceCount;
+    const V3* vertices;
+    const b3o_halfedge* edges;
+    const uint32_t* faces;
+    const PL* planes;
+} Hull;
+
+static Hull hull_at(const b3o_hull_pool* pool, uint32_t h) {
+    const uint32_t* d = pool->desc + 6 * (size_t)h;
+    Hull r;
+    r.centroid = v3(pool->centroids[3 * h], pool->centroids[3 * h + 1], pool->centroids[3 * h + 2]);
+    r.vertexCount = d[1];
+    r.vertices = (const V3*)(pool->vertices + 3 * (size_t)d[0]);
+    r.edgeCount = d[3];
+    r.edges = (const b3o_halfedge*)(pool->edges + 4 * (size_t)d[2]);
+    r.faceCount = d[5];
+    r.faces = pool->faces + d[4];
+    r.planes = (const PL*)(pool->planes + 4 * (size_t)d[4]);
+    return r;
+}
+
+static uint32_t support_vertex(const Hull* h, V3 d) {                                  /* hull.inl:31-45 */
+    uint32_t maxIndex = 0;
+    float maxProjection = dot(d, h->vertices[0]);
+    for (uint32_t i = 1; i < h->vertexCount; ++i) {
+        float p = dot(d, h->vertices[i]);
+        if (p > maxProjection) { maxIndex = i; maxProjection = p; }
+    }
+    return maxIndex;
+}
+static uint32_t support_face(const Hull* h, V3 d) {                                    /* hull.inl:47-61 */
+    uint32_t maxIndex = 0;
+    float maxProjection = dot(d, h->planes[0].n);
+    for (uint32_t i = 1; i < h->faceCount; ++i) {
+        float p = dot(d, h->planes[i].n);
+        if (p > maxProjection) { maxIndex = i; maxProjection = p; }
+    }
+    return maxIndex;
+}
+static PL edge_side_plane(const Hull* h, uint32_t index) {                             /* hull.inl:63-78 */
+    const b3o_halfedge* e = h->edges + index;
+    const b3o_halfedge* t = h->edges + e->twin;
+    V3 P = h->vertices[e->origin], Q = h->vertices[t->origin];
+    V3 E = sub(Q, P);
+    V3 D = cross(E, h->planes[e->face].n);
+    PL r;
+    r.n = normalize(D);
+    r.d = dot(r.n, P);
+    return r;
+}
+
+/* ---------------------------------------------------------------- sat.cpp */
+typedef struct { uint32_t index; float separation; } FaceQuery;
+typedef struct { uint32_t index1, index2; float separation; } EdgeQuery;
+
+static float project_hull(const Hull* h, PL plane) {                                   /* sat.cpp:26-30 */
+    V3 s = h->vertices[support_vertex(h, neg(plane.n))];
+    return plane_dist(s, plane);
+}
+
+static FaceQuery query_face(const XF* xf1, const Hull* h1, const XF* xf2, const Hull* h2) { /* sat.cpp:33-58 */
+    XF xf = xf_mulT_xf(xf2, xf1);
+    FaceQuery q = {0, -MAX_FLOAT};
+    for (uint32_t i = 0; i < h1->faceCount; ++i) {
+        PL plane = plane_mul(&xf, h1->planes[i]);
+        float s = project_hull(h2, plane);
+        if (s > q.separation) { q.index = i; q.separation = s; }
+    }
+    return q;
+}
+
+static int is_minkowski_face(V3 A, V3 B, V3 BxA, V3 C, V3 D, V3 DxC) {                 /* sat.cpp:60-71 */
+    float ADC = dot(A, DxC);
+    float BDC = dot(B, DxC);
+    float CBA = dot(C, BxA);
+    float DBA = dot(D, BxA);
+    FL(1);
+    if (!(CBA * DBA < 0.0f)) return 0;
+    FL(1);
+    if (!(ADC * BDC < 0.0f)) return 0;
+    FL(1);
+    return CBA * BDC > 0.0f;
+}
+
+static float project_edges(V3 P1, V3 E1, V3 P2, V3 E2, V3 C1) {                        /* sat.cpp:73-108 */
+    float L1 = length(E1);
+    if (L1 < LINEAR_SLOP) return -MAX_FLOAT;
+    float L2 = length(E2);
+    if (L2 < LINEAR_SLOP) return -MAX_FLOAT;
+    const float kTol = 0.005f;
+    V3 E1xE2 = cross(E1, E2);
+    float L = length(E1xE2);
+    FL(2);
+    if (L < kTol * L1 * L2) return -MAX_FLOAT;
+    FL(1);
+    V3 N = scale(1.0f / L, E1xE2);
+    if (dot(N, sub(P1, C1)) < 0.0f) N = neg(N);
+    return dot(N, sub(P2, P1));
+}
+
+static EdgeQuery query_edge(const XF* xf1, const Hull* h1, const XF* xf2, const Hull* h2) { /* sat.cpp:110-173 */
+    XF xf = xf_mulT_xf(xf2, xf1);
+    V3 C1 = xf_mul(&xf, h1->centroid);
+    EdgeQuery q = {0, 0, -MAX_FLOAT};
+    for (uint32_t i = 0; i < h1->edgeCount; i += 2) {
+        const b3o_halfedge* e1 = h1->edges + i;
+        const b3o_halfedge* t1 = h1->edges + i + 1;
+        V3 P1 = xf_mul(&xf, h1->vertices[e1->origin]);
+        V3 Q1 = xf_mul(&xf, h1->vertices[t1->origin]);
+        V3 E1 = sub(Q1, P1);
+        V3 U1 = mul_mv(rot(&xf), h1->planes[e1->face].n);
+        V3 V1 = mul_mv(rot(&xf), h1->planes[t1->face].n);
+        for (uint32_t j = 0; j < h2->edgeCount; j += 2) {
+            const b3o_halfedge* e2 = h2->edges + j;
+            const b3o_halfedge* t2 = h2->edges + j + 1;
+            V3 P2 = h2->vertices[e2->origin];
+            V3 Q2 = h2->vertices[t2->origin];
+            V3 E2 = sub(Q2, P2);
+            V3 U2 = h2->planes[e2->face].n;
+            V3 V2 = h2->planes[t2->face].n;
+            if (is_minkowski_face(U1, V1, neg(E1), neg(U2), neg(V2), neg(E2))) {
+                float s = project_edges(P1, E1, P2, E2, C1);
+                if (s > q.separation) { q.separation = s; q.index1 = i; q.index2 = j; }
+            }
+        }
+    }
+    return q;
+}
+
+static int read_face(const b3o_feature_cache* c, const XF* xf1, const Hull* h1,       /* sat.cpp:209-222 */
+                     const XF* xf2, const Hull* h2, float totalRadius) {
+    XF xf = xf_mulT_xf(xf2, xf1);
+    PL plane = plane_mul(&xf, h1->planes[c->index1]);
+    float s = project_hull(h2, plane);
+    return s > totalRadius ? B3O_SEPARATION : B3O_OVERLAP;
+}
+
+static int read_edge(const b3o_feature_cache* c, const XF* xf1, const Hull* h1,       /* sat.cpp:224-278 */
+                     const XF* xf2, const Hull* h2, float totalRadius) {
+    uint32_t i = c->index1, j = c->index2;
+    XF xf = xf_mulT_xf(xf2, xf1);
+    V3 C1 = xf_mul(&xf, h1->centroid);
+    const b3o_halfedge* e1 = h1->edges + i;
+    const b3o_halfedge* t1 = h1->edges + i + 1;
+    V3 P1 = xf_mul(&xf, h1->vertices[e1->origin]);
+    V3 Q1 = xf_mul(&xf, h1->vertices[t1->origin]);
+    V3 E1 = sub(Q1, P1);
+    V3 U1 = mul_mv(rot(&xf), h1->planes[e1->face].n);
+    V3 V1 = mul_mv(rot(&xf), h1->planes[t1->face].n);
+    const b3o_halfedge* e2 = h2->edges + j;
+    const b3o_halfedge* t2 = h2->edges + j + 1;
+    V3 P2 = h2->vertices[e2->origin];
+    V3 Q2 = h2->vertices[t2->origin];
+    V3 E2 = sub(Q2, P2);
+    V3 U2 = h2->planes[e2->face].n;
+    V3 V2 = h2->planes[t2->face].n;
+    if (is_minkowski_face(U1, V1, neg(E1), neg(U2), neg(V2), neg(E2))) {
+        float s = project_edges(P1, E1, P2, E2, C1);
+        return s > totalRadius ? B3O_SEPARATION : B3O_OVERLAP;
+    }
+    return B3O_EMPTY;
+}
+
+static int read_state(b3o_feature_cache* c, const XF* xf1, const Hull* h1,            /* sat.cpp:177-206 */
+                      const XF* xf2, const Hull* h2, float totalRadius) {
+    if (c->state == B3O_EMPTY) {
+        c->state = B3O_SEPARATION; c->type = B3O_FACE1; c->index1 = 0; c->index2 = 0;
+    }
+    switch (c->type) {
+    case B3O_EDGE1: return read_edge(c, xf1, h1, xf2, h2, totalRadius);
+    case B3O_FACE1: return read_face(c, xf1, h1, xf2, h2, totalRadius);
+    case B3O_FACE2: return read_face(c, xf2, h2, xf1, h1, totalRadius);
+    default: return B3O_EMPTY;
+    }
+}
+
+/* ---------------------------------------------------------------- clip.h / clip.cpp */
+typedef struct { uint32_t inEdge1, outEdge1, inEdge2, outEdge2; } FPair;
+/* b3MakePair takes (inEdge1, inEdge2, outEdge1, outEdge2) (clip.h:36-44). */
+static FPair make_pair(uint32_t in1, uint32_t in2, uint32_t out1, uint32_t out2) {
+    FPair p; p.inEdge1 = in1; p.inEdge2 = in2; p.outEdge1 = out1; p.outEdge2 = out2; return p;
+}
+typedef struct { V3 position; FPair pair; } ClipVertex;
+typedef struct { PL plane; uint32_t id; } ClipPlane;
+typedef struct { uint32_t count; ClipVertex v[POLY_CAP]; } ClipPoly;
+static void copy_clip(ClipPoly* dst, const ClipPoly* src) {
+    dst->count = src->count;
+    memcpy(dst->v, src->v, src->count * sizeof(ClipVertex));
+}
+
+static void build_polygon(ClipPoly* out, const XF* xf, uint32_t index, const Hull* h) { /* clip.cpp:33-57 */
+    out->count = 0;
+    uint32_t begin = h->faces[index];
+    uint32_t e = begin;
+    do {
+        const b3o_halfedge* edge = h->edges + e;
+        const b3o_halfedge* twin = h->edges + edge->twin;
+        uint32_t edgeId = twin->twin;
+        uint32_t prevId = twin->next;
+        ClipVertex cv;
+        cv.position = xf_mul(xf, h->vertices[edge->origin]);
+        cv.pair = make_pair(prevId, NULL_EDGE, edgeId, NULL_EDGE);
+        if (out->count < POLY_CAP) out->v[out->count++] = cv;
+        e = edge->next;
+    } while (e != begin);
+}
+
+static void clip_polygon_to_plane(ClipPoly* out, const ClipPoly* in, ClipPlane plane) { /* clip.cpp:99-149 */
+    out->count = 0;
+    ClipVertex v1 = in->v[in->count - 1];
+    float d1 = plane_dist(v1.position, plane.plane);
+    for (uint32_t i = 0; i < in->count; ++i) {
+        ClipVertex v2 = in->v[i];
+        float d2 = plane_dist(v2.position, plane.plane);
+        if (d1 <= 0.0f && d2 <= 0.0f) {
+            if (out->count < POLY_CAP) out->v[out->count++] = v2;
+        } else if (d1 <= 0.0f && d2 > 0.0f) {
+            FL(2);
+            float f = d1 / (d1 - d2);
+            ClipVertex cv;
+            cv.position = add(v1.position, scale(f, sub(v2.position, v1.position)));
+            cv.pair = make_pair(v1.pair.inEdge1, NULL_EDGE, NULL_EDGE, plane.id);
+            if (out->count < POLY_CAP) out->v[out->count++] = cv;
+        } else if (d1 > 0.0f && d2 <= 0.0f) {
+            FL(2);
+            float f = d1 / (d1 - d2);
+            ClipVertex cv;
+            cv.position = add(v1.position, scale(f, sub(v2.position, v1.position)));
+            cv.pair = make_pair(v1.pair.inEdge1, plane.id, NULL_EDGE, NULL_EDGE);
+            if (out->count < POLY_CAP) out->v[out->count++] = cv;
+            if (out->count < POLY_CAP) out->v[out->count++] = v2;
+        }
+        v1 = v2;
+        d1 = d2;
+    }
+}
+
+static void clip_polygon_to_face(ClipPoly* out, const ClipPoly* in, const XF* xf,      /* clip.cpp:243-280 */
+                                 float r, uint32_t index, const Hull* h) {
+    static ClipPoly tmp;  /* single-threaded oracle */
+    copy_clip(out, in);
+    uint32_t begin = h->faces[index];
+    uint32_t e = begin;
+    do {
+        const b3o_halfedge* edge = h->edges + e;
+        const b3o_halfedge* twin = h->edges + edge->twin;
+        uint32_t edgeId = twin->twin;
+        PL plane = edge_side_plane(h, edgeId);
+        FL(1);
+        plane.d += r;
+        ClipPlane cp;
+        cp.id = edgeId;
+        cp.plane = plane_mul(xf, plane);
+        clip_polygon_to_plane(&tmp, out, cp);
+        copy_clip(out, &tmp);
+        if (out->count == 0) return;
+        e = edge->next;
+    } while (e != begin);
+}
+
+/* ---------------------------------------------------------------- contact_cluster.cpp */
+typedef struct { V3 position; uint32_t clipIndex; } CPV;
+typedef struct { uint32_t count; CPV v[POLY_CAP]; } CPoly;
+static void copy_cpoly(CPoly* dst, const CPoly* src) {
+    dst->count = src->count;
+    memcpy(dst->v, src->v, src->count * sizeof(CPV));
+}
+
+static int is_ccw3(V3 A, V3 B, V3 C, V3 N) {                                            /* contact_cluster.cpp:22-26 */
+    V3 n = cross(sub(B, A), sub(C, A));
+    return dot(n, N) > 0.0f;
+}
+
+static void sort_polygon(CPoly* out, const CPoly* in, V3 N) {                          /* contact_cluster.cpp:33-56 */
+    copy_cpoly(out, in);
+    V3 A = out->v[0].position;
+    for (uint32_t i = 1; i < out->count; ++i) {
+        V3 B = out->v[i].position;  /* stale pivot: not refreshed after swaps */
+        for (uint32_t j = i + 1; j < out->count; ++j) {
+            V3 C = out->v[j].position;
+            if (!is_ccw3(A, B, C, N)) {
+                CPV t = out->v[i]; out->v[i] = out->v[j]; out->v[j] = t;
+            }
+        }
+    }
+}
+
+static void reduce_polygon(CPoly* out, const CPoly* in, V3 N, uint32_t initial) {      /* contact_cluster.cpp:58-200 */
+    out->count = 0;
+    if (in->count <= MAX_POINTS) { sort_polygon(out, in, N); return; }
+    char chosen[POLY_CAP];
+    memset(chosen, 0, sizeof(chosen));
+    out->v[out->count++] = in->v[initial];
+    chosen[initial] = 1;
+    {
+        V3 A = out->v[0].position;
+        uint32_t index = 0;
+        float mx = -MAX_FLOAT;
+        for (uint32_t i = 0; i < in->count; ++i) {
+            if (chosen[i]) continue;
+            V3 d = sub(in->v[i].position, A);
+            float dd = dot(d, d);
+            if (dd > mx) { mx = dd; index = i; }
+        }
+        FL(1);
+        if (mx < EPSILON * EPSILON) return;
+        out->v[out->count++] = in->v[index];
+        chosen[index] = 1;
+    }
+    {
+        V3 A = out->v[0].position, B = out->v[1].position;
+        uint32_t index = 0;
+        float mx = -MAX_FLOAT;
+        for (uint32_t i = 0; i < in->count; ++i) {
+            if (chosen[i]) continue;
+            V3 C = in->v[i].position;
+            V3 n = cross(sub(B, A), sub(C, A));
+            float a2 = fabsf(dot(n, N));
+            if (a2 > mx) { mx = a2; index = i; }
+        }
+        const float kAreaTol = 0.01f;
+        FL(1);
+        if (mx < 2.0f * kAreaTol) return;
+        V3 C = in->v[index].position;
+        if (!is_ccw3(A, B, C, N)) { CPV t = out->v[0]; out->v[0] = out->v[1]; out->v[1] = t; }
+        out->v[out->count++] = in->v[index];
+        chosen[index] = 1;
+    }
+    {
+        V3 A = out->v[0].position, B = out->v[1].position;
+        uint32_t index = 0;
+        float mn = MAX_FLOAT;
+        for (uint32_t i = 0; i < in->count; ++i) {
+            if (chosen[i]) continue;
+            V3 D = in->v[i].position;
+            V3 n = cross(sub(B, A), sub(D, A));
+            float sa2 = dot(n, N);
+            if (sa2 < mn) { mn = sa2; index = i; }
+        }
+        const float kAreaTol = 0.01f;
+        FL(1);
+        if (fabsf(mn) < 2.0f * kAreaTol) return;
+        out->v[out->count++] = in->v[index];
+        chosen[index] = 1;
+    }
+    {
+        static CPoly quad;
+        sort_polygon(&quad, out, N);
+        copy_cpoly(out, &quad);
+    }
+}
+
+/* ---------------------------------------------------------------- manifold */
+static void set_key(b3o_mpoint* mp, FPair p) {                                         /* manifold.h:41-63 */
+    mp->triangleKey = NULL_TRIANGLE;
+    mp->key[0] = p.inEdge1; mp->key[1] = p.outEdge1; mp->key[2] = p.inEdge2; mp->key[3] = p.outEdge2;
+}
+
+/* ---------------------------------------------------------------- collide_hulls.cpp */
+static void build_edge_contact(b3o_manifold* m, const XF* xf1, uint32_t index1, const Hull* h1, /* collide_hulls.cpp:26-91 */
+                               const XF* xf2, uint32_t index2, const Hull* h2, int rebuild) {
+    const b3o_halfedge* e1 = h1->edges + index1;
+    const b3o_halfedge* t1 = h1->edges + index1 + 1;
+    V3 C1 = xf_mul(xf1, h1->centroid);
+    V3 P1 = xf_mul(xf1, h1->vertices[e1->origin]);
+    V3 Q1 = xf_mul(xf1, h1->vertices[t1->origin]);
+    V3 E1 = sub(Q1, P1);
+    V3 N1 = E1;
+    normalize_in_place(&N1);
+    const b3o_halfedge* e2 = h2->edges + index2;
+    const b3o_halfedge* t2 = h2->edges + index2 + 1;
+    V3 C2 = xf_mul(xf2, h2->centroid);
+    (void)C2;
+    V3 P2 = xf_mul(xf2, h2->vertices[e2->origin]);
+    V3 Q2 = xf_mul(xf2, h2->vertices[t2->origin]);
+    V3 E2 = sub(Q2, P2);
+    V3 N2 = E2;
+    normalize_in_place(&N2);
+    float b = dot(N1, N2);
+    FL(2);
+    float den = 1.0f - b * b;
+    if (den == 0.0f) return;
+    FL(1);
+    float inv_den = 1.0f / den;
+    V3 E3 = sub(P1, P2);
+    float d = dot(N1, E3);
+    float e = dot(N2, E3);
+    FL(6);
+    float s = inv_den * (b * e - d);
+    float t = inv_den * (e - b * d);
+    V3 c1 = add(P1, scale(s, N1));
+    V3 c2 = add(P2, scale(t, N2));
+    if (rebuild) {                                                                      /* collide_hulls_cache.cpp:82-93 */
+        float w2[3], w1[3];
+        barycentric2(w2, P1, Q1, c2);
+        barycentric2(w1, P2, Q2, c1);
+        if (!(w2[1] > 0.0f && w2[1] <= w2[2] && w1[1] > 0.0f && w1[1] <= w1[2])) return;
+    }
+    V3 N = cross(E1, E2);
+    normalize_in_place(&N);
+    if (dot(N, sub(P1, C1)) < 0.0f) N = neg(N);
+    FPair pair = make_pair(index1, index1 + 1, index2, index2 + 1);
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), N);
+    m->points[0].localPoint1 = xf_mulT(xf1, c1);
+    m->points[0].localPoint2 = xf_mulT(xf2, c2);
+    set_key(&m->points[0], pair);
+}
+
+static void build_face_contact(b3o_manifold* m, const XF* xf1, uint32_t index1, const Hull* h1, float r1, /* collide_hulls.cpp:93-204 */
+                               const XF* xf2, const Hull* h2, float r2, int flip) {
+    static ClipPoly polygon2, clipPolygon2;
+    static CPoly polygon1, reduced;
+    FL(1);
+    float totalRadius = r1 + r2;
+    PL localPlane1 = h1->planes[index1];
+    PL plane1 = plane_mul(xf1, localPlane1);
+    V3 normal1 = mulT_mv(rot(xf2), plane1.n);
+    uint32_t index2 = support_face(h2, neg(normal1));
+    build_polygon(&polygon2, xf2, index2, h2);
+    clip_polygon_to_face(&clipPolygon2, &polygon2, xf1, totalRadius, index1, h1);
+    if (clipPolygon2.count == 0) return;
+    polygon1.count = 0;
+    uint32_t minIndex = 0;
+    float minSep = MAX_FLOAT;
+    for (uint32_t i = 0; i < clipPolygon2.count; ++i) {
+        V3 p = clipPolygon2.v[i].position;
+        float sep = plane_dist(p, plane1);
+        if (sep <= totalRadius) {
+            if (sep < minSep) { minIndex = polygon1.count; minSep = sep; }
+            CPV v;
+            v.position = closest_on_plane(p, plane1);
+            v.clipIndex = i;
+            if (polygon1.count < POLY_CAP) polygon1.v[polygon1.count++] = v;
+        }
+    }
+    if (polygon1.count == 0) return;
+    V3 normal = plane1.n;
+    V3 s_normal = flip ? neg(normal) : normal;
+    reduce_polygon(&reduced, &polygon1, s_normal, minIndex);
+    uint32_t pointCount = reduced.count;
+    for (uint32_t i = 0; i < pointCount; ++i) {
+        ClipVertex v2 = clipPolygon2.v[reduced.v[i].clipIndex];
+        V3 v1 = closest_on_plane(v2.position, plane1);
+        b3o_mpoint* mp = m->points + i;
+        if (flip) {
+            FPair p = make_pair(v2.pair.inEdge2, v2.pair.inEdge1, v2.pair.outEdge2, v2.pair.outEdge1);
+            mp->localNormal1 = mulT_mv(rot(xf2), s_normal);
+            mp->localPoint1 = xf_mulT(xf2, v2.position);
+            mp->localPoint2 = xf_mulT(xf1, v1);
+            set_key(mp, p);
+        } else {
+            mp->localNormal1 = mulT_mv(rot(xf1), normal);
+            mp->localPoint1 = xf_mulT(xf1, v1);
+            mp->localPoint2 = xf_mulT(xf2, v2.position);
+            set_key(mp, v2.pair);
+        }
+    }
+    m->pointCount = pointCount;
+}
+
+/* ---------------------------------------------------------------- collide_hulls_cache.cpp */
+static void rebuild_face_contact(b3o_manifold* m, const XF* xf1, uint32_t index1, const Hull* h1, float r1, /* :113-153 */
+                                 const XF* xf2, const Hull* h2, float r2, int flip,
+                                 QT q10, QT q1, QT q20, QT q2) {
+    QT dq0 = q_mul(q_conj(q10), q20);
+    QT dq = q_mul(q_conj(q1), q2);
+    QT q = q_mul(q_conj(dq0), dq);
+    const float kTol = 0.995f;
+    if (fabsf(q.w) > kTol) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
+}
+
+static void make_fp(b3o_feature_cache* c, int state, int type, uint32_t i1, uint32_t i2) {
+    c->state = (uint32_t)state; c->type = (uint32_t)type; c->index1 = i1; c->index2 = i2;
+}
+
+static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1,   /* collide_hulls_cache.cpp:155-275 */
+                          const XF* xf2, const Hull* h2, float r2, b3o_feature_cache* c) {
+    FL(1);
+    float totalRadius = r1 + r2;
+    FaceQuery f1 = query_face(xf1, h1, xf2, h2);
+    if (f1.separation > totalRadius) { make_fp(c, B3O_SEPARATION, B3O_FACE1, f1.index, f1.index); return; }
+    FaceQuery f2 = query_face(xf2, h2, xf1, h1);
+    if (f2.separation > totalRadius) { make_fp(c, B3O_SEPARATION, B3O_FACE2, f2.index, f2.index); return; }
+    EdgeQuery eq = query_edge(xf1, h1, xf2, h2);
+    if (eq.separation > totalRadius) { make_fp(c, B3O_SEPARATION, B3O_EDGE1, eq.index1, eq.index2); return; }
+    const float kTol = 0.1f * LINEAR_SLOP;
+    FL(1);
+    float fmax12 = f1.separation > f2.separation ? f1.separation : f2.separation;      /* b3Max: a > b ? a : b */
+    FL(1);
+    if (eq.separation > fmax12 + kTol) {
+        build_edge_contact(m, xf1, eq.index1, h1, xf2, eq.index2, h2, 0);
+        if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_EDGE1, eq.index1, eq.index2); return; }
+    } else {
+        FL(1);
+        if (f1.separation + kTol > f2.separation) {
+            build_face_contact(m, xf1, f1.index, h1, r1, xf2, h2, r2, 0);
+            if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_FACE1, f1.index, f1.index); return; }
+        } else {
+            build_face_contact(m, xf2, f2.index, h2, r2, xf1, h1, r1, 1);
+            if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_FACE2, f2.index, f2.index); return; }
+        }
+    }
+    if (eq.separation > fmax12) {
+        build_edge_contact(m, xf1, eq.index1, h1, xf2, eq.index2, h2, 0);
+        if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_EDGE1, eq.index1, eq.index2); return; }
+    } else {
+        if (f1.separation > f2.separation) {
+            build_face_contact(m, xf1, f1.index, h1, r1, xf2, h2, r2, 0);
+            if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_FACE1, f1.index, f1.index); return; }
+        } else {
+            build_face_contact(m, xf2, f2.index, h2, r2, xf1, h1, r1, 1);
+            if (m->pointCount > 0) { make_fp(c, B3O_OVERLAP, B3O_FACE2, f2.index, f2.index); return; }
+        }
+    }
+    build_edge_contact(m, xf1, eq.index1, h1, xf2, eq.index2, h2, 0);
+    make_fp(c, B3O_OVERLAP, B3O_EDGE1, eq.index1, eq.index2);
+}
+
+static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1, /* collide_hulls_cache.cpp:279-344 */
+                                 const XF* xf2, const Hull* h2, float r2, b3o_feature_cache* c,
+                                 QT q10, QT q1, QT q20, QT q2) {
+    FL(1);
+    float totalRadius = r1 + r2;
+    int state0 = (int)c->state;
+    int state1 = read_state(c, xf1, h1, xf2, h2, totalRadius);
+    if (state0 == B3O_SEPARATION && state1 == B3O_SEPARATION) return;
+    if (state0 == B3O_OVERLAP && state1 == B3O_OVERLAP) {
+        switch (c->type) {
+        case B3O_EDGE1: build_edge_contact(m, xf1, c->index1, h1, xf2, c->index2, h2, 1); break;
+        case B3O_FACE1: rebuild_face_contact(m, xf1, c->index1, h1, r1, xf2, h2, r2, 0, q10, q1, q20, q2); break;
+        case B3O_FACE2: rebuild_face_contact(m, xf2, c->index1, h2, r2, xf1, h1, r1, 1, q20, q2, q10, q1); break;
+        default: break;
+        }
+        if (m->pointCount > 0) return;
+    }
+    c->state = B3O_EMPTY;
+    collide_cache(m, xf1, h1, r1, xf2, h2, r2, c);
+}
+
+/* ---------------------------------------------------------------- entry points */
+static void fresh_cache(b3o_convex_cache* c) {
+    memset(c, 0, sizeof(*c));
+    c->feature.state = B3O_EMPTY;
+}
+
+void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n,
+                       const uint32_t* pairs, const b3o_xf* xfs, const b3o_quat* sweeps,
+                       b3o_convex_cache* caches, b3o_manifold* out) {
+    static const QT qid = {0.0f, 0.0f, 0.0f, 1.0f};
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3o_shape* sa = shapes + pairs[2 * i];
+        const b3o_shape* sb = shapes + pairs[2 * i + 1];
+        const XF* xa = xfs + 2 * (size_t)i;
+        const XF* xb = xa + 1;
+        b3o_convex_cache local;
+        b3o_convex_cache* c = caches ? caches + i : &local;
+        if (!caches) fresh_cache(&local);
+        b3o_manifold* m = out + i;
+        memset(m, 0, sizeof(*m));
+        QT q10 = qid, q1 = qid, q20 = qid, q2 = qid;
+        if (sweeps) {
+            q10 = sweeps[4 * (size_t)i]; q1 = sweeps[4 * (size_t)i + 1];
+            q20 = sweeps[4 * (size_t)i + 2]; q2 = sweeps[4 * (size_t)i + 3];
+        }
+        if (sa->type == B3O_HULL && sb->type == B3O_HULL) {
+            Hull h1 = hull_at(pool, sa->hull), h2 = hull_at(pool, sb->hull);
+            collide_hulls_cached(m, xa, &h1, sa->radius, xb, &h2, sb->radius, &c->feature, q10, q1, q20, q2);
+        }
+    }
+}
+
+void b3o_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hullPairs,
+                      const b3o_xf* xfs, uint32_t* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Hull h1 = hull_at(pool, hullPairs[2 * i]), h2 = hull_at(pool, hullPairs[2 * i + 1]);
+        const XF* x1 = xfs + 2 * (size_t)i;
+        const XF* x2 = x1 + 1;
+        FaceQuery f1 = query_face(x1, &h1, x2, &h2);
+        FaceQuery f2 = query_face(x2, &h2, x1, &h1);
+        EdgeQuery e = query_edge(x1, &h1, x2, &h2);
+        uint32_t* o = out + 8 * (size_t)i;
+        o[0] = f1.index; memcpy(o + 1, &f1.separation, 4);
+        o[2] = f2.index; memcpy(o + 3, &f2.separation, 4);
+        o[4] = e.index1; o[5] = e.index2; memcpy(o + 6, &e.separation, 4);
+        o[7] = 0;
+    }
+}

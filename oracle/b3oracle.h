@@ -1,0 +1,121 @@
+/*
+ * TEST INFRASTRUCTURE ONLY - the oracle for the contact narrowphase.
+ *
+ * This header declares (1) the plain-C restatement of the reference narrowphase
+ * (b3oracle.c -> liboracle.so) and (2) the extern "C" driver over the compiled
+ * reference (ref_driver.cpp -> _ref/libb3ref.so).  Only tests, the smoke check and
+ * the CPU-baseline leg of bench.py load these libraries, and only as checkers.
+ *
+ * Exchange records are byte-compatible with the reference types:
+ *   b3o_vec3      == b3Vec3            (vec3.h:106)
+ *   b3o_xf        == b3Transform       (transform.h:56-57; rotation columns x,y,z then position)
+ *   b3o_quat      == b3Quat            (quat.h:141)
+ *   b3o_halfedge  == b3HalfEdge        (hull.h:29-35)
+ *   b3o_plane     == b3Plane           (geometry.h:72-73)
+ *   b3o_feature_cache == b3SATFeaturePair (sat.h:74-80)
+ *   b3o_simplex_cache == b3SimplexCache   (gjk.h:85-92)
+ * The manifold record keeps the fields the narrowphase writes (manifold.h:66-93):
+ * the three local vectors, the feature key (triangleKey + b3FeaturePair words in
+ * struct order inEdge1,outEdge1,inEdge2,outEdge2, clip.h:28-34) and pointCount.
+ */
+#ifndef B3ORACLE_H
+#define B3ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { float x, y, z; } b3o_vec3;
+typedef struct { b3o_vec3 c[3]; b3o_vec3 p; } b3o_xf;
+typedef struct { float x, y, z, w; } b3o_quat;
+typedef struct { b3o_vec3 n; float d; } b3o_plane;
+typedef struct { uint32_t origin, twin, face, next; } b3o_halfedge;
+
+/* Flat hull pool. Hull h: desc[6h..6h+5] = vertexOffset, vertexCount, edgeOffset,
+ * edgeCount (half-edges), faceOffset, faceCount. Edge/face indices are local to the hull. */
+typedef struct {
+    uint32_t count;
+    const uint32_t* desc;
+    const float* centroids;       /* [count][3] */
+    const float* vertices;        /* [*][3]     */
+    const uint32_t* edges;        /* [*][4]     */
+    const uint32_t* faces;        /* [*]        */
+    const float* planes;          /* [*][4]     */
+} b3o_hull_pool;
+
+/* Shape types follow b3ShapeType (shape.h:32-39). */
+enum { B3O_SPHERE = 0, B3O_CAPSULE = 1, B3O_HULL = 2, B3O_MESH = 3 };
+
+/* A convex shape: sphere (v[0..2] = centre), capsule (v[0..5] = centres), hull (hull index). */
+typedef struct {
+    uint32_t type;
+    uint32_t hull;
+    float radius;
+    float v[6];
+    uint32_t pad[3];
+} b3o_shape;
+
+typedef struct {
+    b3o_vec3 localNormal1;
+    b3o_vec3 localPoint1;
+    b3o_vec3 localPoint2;
+    uint32_t triangleKey;
+    uint32_t key[4];
+} b3o_mpoint;
+
+typedef struct {
+    uint32_t pointCount;
+    uint32_t pad[3];
+    b3o_mpoint points[4];
+} b3o_manifold;
+
+typedef struct { uint32_t state, type, index1, index2; } b3o_feature_cache;
+typedef struct {
+    float metric;
+    uint32_t iterations;
+    uint16_t count;
+    uint8_t index1[4];
+    uint8_t index2[4];
+} b3o_simplex_cache;
+typedef struct { b3o_simplex_cache simplex; b3o_feature_cache feature; } b3o_convex_cache;
+
+/* Feature cache enums (sat.h:58-72). */
+enum { B3O_SEPARATION = 0, B3O_OVERLAP = 1, B3O_EMPTY = 2 };
+enum { B3O_EDGE1 = 0, B3O_FACE1 = 1, B3O_FACE2 = 2 };
+
+/* ---- plain-C restatement (liboracle.so) ---- */
+
+/* Collide n shape pairs as b3CollideShapeAndShape (collide.cpp:162-188) with the
+ * global convex cache on (collide_hulls.cpp:291).  pairs[2i], pairs[2i+1] index
+ * shapes[]; xfs[2i], xfs[2i+1] are the transforms; sweeps (may be NULL) gives per pair
+ * orientation0/orientation of body A then body B (4 quats) for the cached face
+ * rebuild; caches (may be NULL: fresh empty caches) are read and updated in place. */
+void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
+                       uint32_t n, const uint32_t* pairs, const b3o_xf* xfs,
+                       const b3o_quat* sweeps, b3o_convex_cache* caches,
+                       b3o_manifold* out);
+
+/* Separation queries only (sat.cpp:33-58, 110-173) for hull pairs, for kernel-level checks.
+ * out[8i..8i+7] = face1.index, face1.sep(bits), face2.index, face2.sep(bits),
+ *                 edge.index1, edge.index2, edge.sep(bits), 0 */
+void b3o_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hullPairs,
+                      const b3o_xf* xfs, uint32_t* out);
+
+/* FP32 ops executed by the last b3o_collide_pairs call (liboracle_count.so only). */
+uint64_t b3o_flop_count(void);
+void b3o_flop_reset(void);
+
+/* ---- compiled reference driver (_ref/libb3ref.so) ---- */
+void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
+                         uint32_t n, const uint32_t* pairs, const b3o_xf* xfs,
+                         const b3o_quat* sweeps, b3o_convex_cache* caches,
+                         b3o_manifold* out);
+void b3ref_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hullPairs,
+                        const b3o_xf* xfs, uint32_t* out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

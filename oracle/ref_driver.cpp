@@ -1,0 +1,194 @@
+// TEST INFRASTRUCTURE ONLY.  extern "C" driver over the reference library, compiled
+// together with the reference sources in place (oracle/Makefile) into
+// oracle/_ref/libb3ref.so.  It builds b3Hull views over the flat pool, wraps them in
+// the reference's shape classes and calls b3CollideShapeAndShape (collide.cpp:162-188)
+// exactly as b3ConvexContact::Collide does (convex_contact.cpp:50-63).
+//
+// The body pointer of a shape is only read by the cached face rebuild
+// (collide_hulls_cache.cpp:117-126, GetBody()->GetSweep()); the driver points it at a
+// zeroed b3Body image whose m_sweep carries the pair's orientation0/orientation.
+#include <cstring>
+#include <cstdlib>
+#include <vector>
+#include <new>
+
+#define private public
+#define protected public
+#include <bounce/dynamics/body.h>
+#include <bounce/dynamics/shapes/sphere_shape.h>
+#include <bounce/dynamics/shapes/capsule_shape.h>
+#include <bounce/dynamics/shapes/hull_shape.h>
+#include <bounce/dynamics/contacts/manifold.h>
+#include <bounce/dynamics/contacts/collide/collide.h>
+#include <bounce/collision/shapes/hull.h>
+#include <bounce/common/draw.h>
+#undef private
+#undef protected
+
+#include "b3oracle.h"
+
+static_assert(sizeof(b3Transform) == sizeof(b3o_xf), "transform layout");
+static_assert(sizeof(b3HalfEdge) == sizeof(b3o_halfedge), "half-edge layout");
+static_assert(sizeof(b3Plane) == 16, "plane layout");
+static_assert(sizeof(b3ConvexCache) == sizeof(b3o_convex_cache), "cache layout");
+static_assert(sizeof(b3Quat) == sizeof(b3o_quat), "quat layout");
+
+// Profiling hooks the library expects its user to define (settings.h:147-151).
+void b3BeginProfileScope(const char*) {}
+void b3EndProfileScope() {}
+// Debug-draw globals normally defined by draw_world.cpp (not linked: it pulls in the cloth module).
+b3Draw* b3Draw_draw = nullptr;
+const b3Color b3Color_red(1.0f, 0.0f, 0.0f);
+const b3Color b3Color_green(0.0f, 1.0f, 0.0f);
+const b3Color b3Color_yellow(1.0f, 1.0f, 0.0f);
+const b3Color b3Color_pink(1.0f, 0.0f, 1.0f);
+
+namespace {
+
+struct Scene {
+    std::vector<b3Hull> hulls;
+    std::vector<b3Shape*> shapes;
+    ~Scene() {
+        for (b3Shape* s : shapes) {
+            switch (s->m_type) {
+            case e_sphereShape: delete (b3SphereShape*)s; break;
+            case e_capsuleShape: delete (b3CapsuleShape*)s; break;
+            default: delete (b3HullShape*)s; break;
+            }
+        }
+    }
+};
+
+void build_hulls(const b3o_hull_pool* pool, std::vector<b3Hull>& out) {
+    out.resize(pool->count);
+    for (uint32_t h = 0; h < pool->count; ++h) {
+        const uint32_t* d = pool->desc + 6 * h;
+        b3Hull& hull = out[h];
+        hull.centroid = b3Vec3(pool->centroids[3 * h], pool->centroids[3 * h + 1], pool->centroids[3 * h + 2]);
+        hull.vertexCount = d[1];
+        hull.vertices = (b3Vec3*)(pool->vertices + 3 * (size_t)d[0]);
+        hull.edgeCount = d[3];
+        hull.edges = (b3HalfEdge*)(pool->edges + 4 * (size_t)d[2]);
+        hull.faceCount = d[5];
+        hull.faces = (b3Face*)(pool->faces + d[4]);
+        hull.planes = (b3Plane*)(pool->planes + 4 * (size_t)d[4]);
+    }
+}
+
+b3Shape* make_shape(const b3o_shape& s, std::vector<b3Hull>& hulls) {
+    switch (s.type) {
+    case B3O_SPHERE: {
+        b3SphereShape* sp = new b3SphereShape();
+        sp->m_center = b3Vec3(s.v[0], s.v[1], s.v[2]);
+        sp->m_radius = s.radius;
+        return sp;
+    }
+    case B3O_CAPSULE: {
+        b3CapsuleShape* cp = new b3CapsuleShape();
+        cp->m_centers[0] = b3Vec3(s.v[0], s.v[1], s.v[2]);
+        cp->m_centers[1] = b3Vec3(s.v[3], s.v[4], s.v[5]);
+        cp->m_radius = s.radius;
+        return cp;
+    }
+    default: {
+        b3HullShape* hp = new b3HullShape();
+        hp->m_hull = &hulls[s.hull];
+        hp->m_radius = s.radius;
+        return hp;
+    }
+    }
+}
+
+struct BodyImage {
+    alignas(b3Body) unsigned char bytes[sizeof(b3Body)];
+    b3Body* body() { return reinterpret_cast<b3Body*>(bytes); }
+};
+
+void to_out(const b3Manifold& m, b3o_manifold* o) {
+    std::memset(o, 0, sizeof(*o));
+    o->pointCount = m.pointCount;
+    for (u32 i = 0; i < m.pointCount; ++i) {
+        const b3ManifoldPoint& p = m.points[i];
+        b3o_mpoint& q = o->points[i];
+        std::memcpy(&q.localNormal1, &p.localNormal1, 12);
+        std::memcpy(&q.localPoint1, &p.localPoint1, 12);
+        std::memcpy(&q.localPoint2, &p.localPoint2, 12);
+        q.triangleKey = p.key.triangleKey;
+        std::memcpy(q.key, &p.key.key1, 8);
+        std::memcpy(q.key + 2, &p.key.key2, 8);
+    }
+}
+
+}  // namespace
+
+extern "C" void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
+                                    uint32_t n, const uint32_t* pairs, const b3o_xf* xfs,
+                                    const b3o_quat* sweeps, b3o_convex_cache* caches,
+                                    b3o_manifold* out) {
+    Scene scene;
+    build_hulls(pool, scene.hulls);
+    // Shapes are created lazily so only referenced records are touched.
+    std::vector<b3Shape*> table;
+    BodyImage bodyA, bodyB;
+    std::memset(bodyA.bytes, 0, sizeof(bodyA.bytes));
+    std::memset(bodyB.bytes, 0, sizeof(bodyB.bytes));
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3o_shape& sa = shapes[pairs[2 * i]];
+        const b3o_shape& sb = shapes[pairs[2 * i + 1]];
+        b3Shape* A = make_shape(sa, scene.hulls);
+        b3Shape* B = make_shape(sb, scene.hulls);
+        scene.shapes.push_back(A);
+        scene.shapes.push_back(B);
+        A->m_body = bodyA.body();
+        B->m_body = bodyB.body();
+        if (sweeps) {
+            const b3o_quat* q = sweeps + 4 * (size_t)i;
+            std::memcpy(&bodyA.body()->m_sweep.orientation0, q + 0, 16);
+            std::memcpy(&bodyA.body()->m_sweep.orientation, q + 1, 16);
+            std::memcpy(&bodyB.body()->m_sweep.orientation0, q + 2, 16);
+            std::memcpy(&bodyB.body()->m_sweep.orientation, q + 3, 16);
+        }
+        b3ConvexCache cache;
+        if (caches) {
+            std::memcpy(&cache, caches + i, sizeof(cache));
+        } else {
+            std::memset(&cache, 0, sizeof(cache));
+            cache.simplexCache.count = 0;
+            cache.featureCache.m_featurePair.state = e_empty;
+        }
+        b3Transform xfA, xfB;
+        std::memcpy(&xfA, xfs + 2 * (size_t)i, sizeof(xfA));
+        std::memcpy(&xfB, xfs + 2 * (size_t)i + 1, sizeof(xfB));
+        b3Manifold m;
+        m.Initialize();
+        b3CollideShapeAndShape(m, xfA, A, xfB, B, &cache);
+        to_out(m, out + i);
+        if (caches) std::memcpy(caches + i, &cache, sizeof(cache));
+        // Keep memory bounded on large batches.
+        if (scene.shapes.size() > 4096) {
+            Scene tmp;
+            tmp.shapes.swap(scene.shapes);
+        }
+    }
+}
+
+extern "C" void b3ref_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hullPairs,
+                                   const b3o_xf* xfs, uint32_t* out) {
+    std::vector<b3Hull> hulls;
+    build_hulls(pool, hulls);
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3Hull* h1 = &hulls[hullPairs[2 * i]];
+        const b3Hull* h2 = &hulls[hullPairs[2 * i + 1]];
+        b3Transform xf1, xf2;
+        std::memcpy(&xf1, xfs + 2 * (size_t)i, sizeof(xf1));
+        std::memcpy(&xf2, xfs + 2 * (size_t)i + 1, sizeof(xf2));
+        b3FaceQuery f1 = b3QueryFaceSeparation(xf1, h1, xf2, h2);
+        b3FaceQuery f2 = b3QueryFaceSeparation(xf2, h2, xf1, h1);
+        b3EdgeQuery e = b3QueryEdgeSeparation(xf1, h1, xf2, h2);
+        uint32_t* o = out + 8 * (size_t)i;
+        o[0] = f1.index; std::memcpy(o + 1, &f1.separation, 4);
+        o[2] = f2.index; std::memcpy(o + 3, &f2.separation, 4);
+        o[4] = e.index1; o[5] = e.index2; std::memcpy(o + 6, &e.separation, 4);
+        o[7] = 0;
+    }
+}

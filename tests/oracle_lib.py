@@ -1,0 +1,86 @@
+"""ctypes access to the oracle (test infrastructure only).
+
+liboracle.so       - the plain-C restatement (oracle/b3oracle.c)
+liboracle_count.so - the same with FP32 op counting
+_ref/libb3ref.so   - the reference compiled in place (present only where it was built)
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from bounce_amd.types import CACHE_DTYPE, MANIFOLD_DTYPE, ptr
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+
+def _load(path):
+    if not os.path.exists(path):
+        return None
+    return ctypes.CDLL(path)
+
+
+def _bind(lib, prefix):
+    f = getattr(lib, prefix + "collide_pairs")
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                  ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    q = getattr(lib, prefix + "hull_queries")
+    q.restype = None
+    q.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    return f, q
+
+
+class Checker:
+    """Runs one implementation (restatement or reference) on NumPy inputs."""
+
+    def __init__(self, lib, prefix):
+        self.lib = lib
+        self._collide, self._queries = _bind(lib, prefix)
+
+    def collide(self, pool, shapes, pairs, xfs, sweeps=None, caches=None):
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = pairs.shape[0]
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        shapes = np.ascontiguousarray(shapes)
+        out = np.zeros(n, MANIFOLD_DTYPE)
+        ps = pool.struct()
+        if sweeps is not None:
+            sweeps = np.ascontiguousarray(sweeps, np.float32).reshape(n, 4, 4)
+        if caches is not None:
+            assert caches.dtype == CACHE_DTYPE and caches.flags["C_CONTIGUOUS"]
+        self._collide(ctypes.byref(ps), ptr(shapes), n, ptr(pairs), ptr(xfs), ptr(sweeps), ptr(caches), ptr(out))
+        return out
+
+    def queries(self, pool, pairs, xfs):
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = pairs.shape[0]
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        out = np.zeros((n, 8), np.uint32)
+        ps = pool.struct()
+        self._queries(ctypes.byref(ps), n, ptr(pairs), ptr(xfs), ptr(out))
+        return out
+
+
+def oracle():
+    lib = _load(os.path.join(ORACLE_DIR, "liboracle.so"))
+    if lib is None:
+        raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
+    return Checker(lib, "b3o_")
+
+
+def oracle_counting():
+    lib = _load(os.path.join(ORACLE_DIR, "liboracle_count.so"))
+    if lib is None:
+        raise RuntimeError("oracle/liboracle_count.so is not built")
+    c = Checker(lib, "b3o_")
+    lib.b3o_flop_count.restype = ctypes.c_uint64
+    lib.b3o_flop_reset.restype = None
+    return c
+
+
+def reference():
+    """The compiled reference, or None where oracle/_ref was not built."""
+    lib = _load(os.path.join(ORACLE_DIR, "_ref", "libb3ref.so"))
+    return None if lib is None else Checker(lib, "b3ref_")
