@@ -1,0 +1,166 @@
+"""bounce_amd: MI355X (gfx950) contact narrowphase for the Bounce physics engine.
+
+Python access to the C ABI in include/bounce_amd.h (libbounce_amd.so, built in-tree by
+bounce_amd/build.py).  The HIP library is the only implementation: loading fails loudly
+when it is missing, and there is no CPU fallback.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from .types import (CACHE_DTYPE, MANIFOLD_DTYPE, SHAPE_DTYPE, HullPool, empty_caches, ptr,
+                    SPHERE, CAPSULE, HULL, MESH)
+
+__all__ = ["Context", "HullPool", "load_library", "empty_caches", "BounceError",
+           "MANIFOLD_DTYPE", "CACHE_DTYPE", "SHAPE_DTYPE", "SPHERE", "CAPSULE", "HULL", "MESH"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbounce_amd.so")
+_lib = None
+
+OK, ERR_ARGUMENT, ERR_DEVICE, ERR_BUDGET, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6
+
+
+class BounceError(RuntimeError):
+    def __init__(self, code, message):
+        super().__init__("bounce_amd error %d: %s" % (code, message))
+        self.code = code
+
+
+def load_library():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libbounce_amd.so is not built: run `python bounce_amd/build.py` "
+                           "(or __graft_entry__.build()) first")
+    lib = ctypes.CDLL(LIB_PATH)
+    vp, u32, u64, i32 = ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint64, ctypes.c_int
+    sig = {
+        "b3amd_create": (i32, [i32, u64, ctypes.POINTER(vp)]),
+        "b3amd_destroy": (i32, [vp]),
+        "b3amd_last_error": (ctypes.c_char_p, []),
+        "b3amd_memory_in_use": (u64, [vp]),
+        "b3amd_upload_hulls": (i32, [vp, vp]),
+        "b3amd_upload_shapes": (i32, [vp, u32, vp]),
+        "b3amd_collide": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+        "b3amd_collide_device": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+        "b3amd_synchronize": (i32, [vp]),
+        "b3amd_device_alloc": (i32, [vp, u64, ctypes.POINTER(vp)]),
+        "b3amd_device_free": (i32, [vp, vp]),
+        "b3amd_copy_to_device": (i32, [vp, vp, vp, u64]),
+        "b3amd_copy_to_host": (i32, [vp, vp, vp, u64]),
+        "b3amd_set_profiling": (i32, [vp, i32]),
+        "b3amd_kernel_stats": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]),
+        "b3amd_reset_kernel_stats": (i32, [vp]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    _lib = lib
+    return lib
+
+
+def _check(rc):
+    if rc != OK:
+        raise BounceError(rc, load_library().b3amd_last_error().decode())
+
+
+class DeviceBuffer:
+    """Device memory owned by a Context (counts against its HBM budget)."""
+
+    def __init__(self, ctx, nbytes):
+        self.ctx = ctx
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(ctx.lib.b3amd_device_alloc(ctx.handle, self.nbytes, ctypes.byref(p)))
+        self.ptr = p
+
+    def upload(self, array):
+        a = np.ascontiguousarray(array)
+        assert a.nbytes <= self.nbytes
+        _check(self.ctx.lib.b3amd_copy_to_device(self.ctx.handle, self.ptr, ptr(a), a.nbytes))
+
+    def download(self, array):
+        assert array.flags["C_CONTIGUOUS"] and array.nbytes <= self.nbytes
+        _check(self.ctx.lib.b3amd_copy_to_host(self.ctx.handle, ptr(array), self.ptr, array.nbytes))
+        return array
+
+    def free(self):
+        if self.ptr is not None and self.ctx.handle is not None:
+            _check(self.ctx.lib.b3amd_device_free(self.ctx.handle, self.ptr))
+        self.ptr = None
+
+
+class Context:
+    """One narrowphase context on one GPU (b3amd_create)."""
+
+    def __init__(self, device=0, hbm_budget_bytes=0):
+        self.lib = load_library()
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_create(device, hbm_budget_bytes, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.lib.b3amd_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def memory_in_use(self):
+        return int(self.lib.b3amd_memory_in_use(self.handle))
+
+    def upload_hulls(self, pool):
+        s = pool.struct()
+        _check(self.lib.b3amd_upload_hulls(self.handle, ctypes.byref(s)))
+
+    def upload_shapes(self, shapes):
+        shapes = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)
+        _check(self.lib.b3amd_upload_shapes(self.handle, len(shapes), ptr(shapes)))
+
+    def collide(self, pairs, xfs, sweeps=None, caches=None):
+        """b3CollideShapeAndShape over a batch of host arrays; returns manifolds.
+        `caches` (CACHE_DTYPE) is updated in place when given."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = pairs.shape[0]
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        if sweeps is not None:
+            sweeps = np.ascontiguousarray(sweeps, np.float32).reshape(n, 4, 4)
+        if caches is not None:
+            assert caches.dtype == CACHE_DTYPE and caches.flags["C_CONTIGUOUS"] and len(caches) == n
+        out = np.zeros(n, MANIFOLD_DTYPE)
+        _check(self.lib.b3amd_collide(self.handle, n, ptr(pairs), ptr(xfs), ptr(sweeps), ptr(caches), ptr(out)))
+        return out
+
+    def alloc(self, nbytes):
+        return DeviceBuffer(self, nbytes)
+
+    def collide_device(self, n, d_pairs, d_xfs, d_out, d_sweeps=None, d_caches=None):
+        """Enqueue a batch on device-resident buffers (no synchronisation)."""
+        p = lambda b: None if b is None else b.ptr
+        _check(self.lib.b3amd_collide_device(self.handle, n, p(d_pairs), p(d_xfs), p(d_sweeps), p(d_caches),
+                                             p(d_out)))
+
+    def synchronize(self):
+        _check(self.lib.b3amd_synchronize(self.handle))
+
+    def set_profiling(self, on):
+        _check(self.lib.b3amd_set_profiling(self.handle, 1 if on else 0))
+
+    def kernel_stats(self, name):
+        """(launches, total_ms) recorded with HIP events on the context stream."""
+        n = ctypes.c_uint32()
+        ms = ctypes.c_double()
+        _check(self.lib.b3amd_kernel_stats(self.handle, name.encode(), ctypes.byref(n), ctypes.byref(ms)))
+        return n.value, ms.value
+
+    def reset_kernel_stats(self):
+        _check(self.lib.b3amd_reset_kernel_stats(self.handle))

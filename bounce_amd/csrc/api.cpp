@@ -1,0 +1,488 @@
+// C ABI of the narrowphase (include/bounce_amd.h): context, HBM-budgeted device memory,
+// hull/shape registry and the batch entry points that replace the per-contact
+// b3CollideShapeAndShape calls of b3ContactManager::UpdateContacts
+// (contact_manager.cpp:167-231).
+#include <hip/hip_runtime.h>
+
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "bounce_amd.h"
+#include "device_hull.h"
+#include "kernels.h"
+
+using namespace b3amd;
+
+namespace {
+
+thread_local std::string g_error;
+
+int fail(int code, const char* fmt, ...) {
+    char buf[512];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_error = buf;
+    return code;
+}
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
+    } while (0)
+
+struct KernelStat {
+    uint32_t launches = 0;
+    double total_ms = 0.0;
+};
+
+struct PendingTiming {
+    std::string name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace
+
+struct b3amd_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    uint64_t budget = 0;
+    uint64_t inUse = 0;
+    std::unordered_map<void*, uint64_t> allocs;
+
+    // Hull pool.
+    float4* dPool = nullptr;
+    uint32_t* dOffsets = nullptr;
+    uint32_t hullCount = 0;
+    int sizeClass = 0;
+    std::vector<HullLayout> layouts;
+
+    // Shape table.
+    ShapeRec* dShapes = nullptr;
+    uint32_t shapeCount = 0;
+    std::vector<ShapeRec> shapes;
+
+    // Per-batch scratch and host-API staging.
+    uint32_t* dStage = nullptr;
+    SatResult* dSat = nullptr;
+    uint32_t scratchCap = 0;
+    void* dIn = nullptr;
+    uint64_t dInCap = 0;
+
+    // Profiling.
+    bool profiling = false;
+    std::vector<hipEvent_t> freeEvents;
+    std::vector<PendingTiming> pending;
+    std::map<std::string, KernelStat> stats;
+
+    int alloc(uint64_t bytes, void** p) {
+        if (bytes == 0) bytes = 16;
+        if (inUse + bytes > budget)
+            return fail(B3AMD_ERR_BUDGET, "HBM budget exceeded: %llu in use + %llu requested > %llu",
+                        (unsigned long long)inUse, (unsigned long long)bytes, (unsigned long long)budget);
+        HIP_TRY(hipSetDevice(device));
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, bytes);
+        if (e != hipSuccess) return fail(B3AMD_ERR_BUDGET, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
+        allocs[q] = bytes;
+        inUse += bytes;
+        *p = q;
+        return B3AMD_OK;
+    }
+    int release(void* p) {
+        if (!p) return B3AMD_OK;
+        auto it = allocs.find(p);
+        if (it == allocs.end()) return fail(B3AMD_ERR_ARGUMENT, "pointer %p not owned by this context", p);
+        HIP_TRY(hipSetDevice(device));
+        HIP_TRY(hipFree(p));
+        inUse -= it->second;
+        allocs.erase(it);
+        return B3AMD_OK;
+    }
+    hipEvent_t event() {
+        if (!freeEvents.empty()) {
+            hipEvent_t e = freeEvents.back();
+            freeEvents.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return nullptr;
+        return e;
+    }
+    int resolve_timings() {
+        if (pending.empty()) return B3AMD_OK;
+        HIP_TRY(hipStreamSynchronize(stream));
+        for (PendingTiming& t : pending) {
+            float ms = 0.0f;
+            HIP_TRY(hipEventElapsedTime(&ms, t.start, t.stop));
+            KernelStat& s = stats[t.name];
+            s.launches += 1;
+            s.total_ms += ms;
+            freeEvents.push_back(t.start);
+            freeEvents.push_back(t.stop);
+        }
+        pending.clear();
+        return B3AMD_OK;
+    }
+};
+
+namespace {
+
+// Brackets one kernel launch with events while profiling is on.
+template <class F>
+int timed(b3amd_context* ctx, const char* name, F launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->profiling) {
+        a = ctx->event();
+        b = ctx->event();
+        if (!a || !b) return fail(B3AMD_ERR_DEVICE, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(a, ctx->stream));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
+    if (ctx->profiling) {
+        HIP_TRY(hipEventRecord(b, ctx->stream));
+        ctx->pending.push_back(PendingTiming{name, a, b});
+    }
+    return B3AMD_OK;
+}
+
+int ensure_scratch(b3amd_context* ctx, uint32_t n) {
+    if (n <= ctx->scratchCap) return B3AMD_OK;
+    uint32_t cap = n < 1024 ? 1024 : n;
+    int rc;
+    if ((rc = ctx->release(ctx->dStage)) || (rc = ctx->release(ctx->dSat))) return rc;
+    ctx->dStage = nullptr;
+    ctx->dSat = nullptr;
+    ctx->scratchCap = 0;
+    if ((rc = ctx->alloc(sizeof(uint32_t) * (uint64_t)cap, (void**)&ctx->dStage))) return rc;
+    if ((rc = ctx->alloc(sizeof(SatResult) * (uint64_t)cap, (void**)&ctx->dSat))) return rc;
+    ctx->scratchCap = cap;
+    return B3AMD_OK;
+}
+
+// Converts one b3Hull view into the device record (device_hull.h); returns an error
+// string or nullptr.
+const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint32_t>& words,
+                      HullLayout* layoutOut) {
+    const uint32_t* d = pool->desc + 6 * (size_t)h;
+    const uint32_t V = d[1], HE = d[3], F = d[5];
+    if (V < 3 || V > kMaxHullVertices) return "vertexCount outside 3..255";
+    if (F < 2 || F > kMaxHullFaces) return "faceCount outside 2..255";
+    if (HE < 6 || HE > kMaxHullHalfEdges || (HE & 1u)) return "edgeCount must be even and within 6..4096";
+    const float* verts = pool->vertices + 3 * (size_t)d[0];
+    const uint32_t* edges = pool->edges + 4 * (size_t)d[2];
+    const uint32_t* faces = pool->faces + d[4];
+    const float* planes = pool->planes + 4 * (size_t)d[4];
+    for (uint32_t e = 0; e < HE; ++e) {
+        const uint32_t* he = edges + 4 * e;
+        if (he[0] >= V || he[2] >= F || he[3] >= HE) return "half-edge index out of range";
+        if (he[1] != (e ^ 1u)) return "half-edges 2k and 2k+1 must be twins";
+    }
+    uint32_t maxDeg = 0;
+    for (uint32_t f = 0; f < F; ++f) {
+        if (faces[f] >= HE) return "face edge out of range";
+        uint32_t e = faces[f], deg = 0;
+        do {
+            e = edges[4 * e + 3];
+            if (++deg > HE) return "face loop does not close";
+        } while (e != faces[f]);
+        if (deg > maxDeg) maxDeg = deg;
+    }
+    HullLayout L{V, F, HE};
+    words.assign(4 * (size_t)L.units(), 0u);
+    float* fw = reinterpret_cast<float*>(words.data());
+    words[0] = V | (F << 16);
+    words[1] = HE | (maxDeg << 16);
+    words[2] = HE >> 1;
+    std::memcpy(fw + 4, pool->centroids + 3 * (size_t)h, 12);
+    for (uint32_t i = 0; i < V; ++i) std::memcpy(fw + 4 * (L.vertOff() + i), verts + 3 * i, 12);
+    for (uint32_t i = 0; i < F; ++i) std::memcpy(fw + 4 * (L.planeOff() + i), planes + 4 * i, 16);
+    uint32_t* ue = words.data() + 4 * L.uedgeOff();
+    for (uint32_t k = 0; k < L.E(); ++k) {
+        const uint32_t* a = edges + 8 * k;
+        const uint32_t* b = a + 4;
+        ue[k] = a[0] | (b[0] << 8) | (a[2] << 16) | (b[2] << 24);
+    }
+    uint32_t* hw = words.data() + 4 * L.halfOff();
+    for (uint32_t e = 0; e < HE; ++e) {
+        const uint32_t* he = edges + 4 * e;
+        hw[e] = he[0] | (he[2] << 8) | (he[3] << 16);
+    }
+    uint16_t* fe = reinterpret_cast<uint16_t*>(words.data() + 4 * L.faceOff());
+    for (uint32_t f = 0; f < F; ++f) fe[f] = (uint16_t)faces[f];
+    *layoutOut = L;
+    return nullptr;
+}
+
+int upload(b3amd_context* ctx, void** dptr, uint64_t* cap, const void* src, uint64_t bytes) {
+    if (bytes > *cap) {
+        int rc = ctx->release(*dptr);
+        if (rc) return rc;
+        *dptr = nullptr;
+        *cap = 0;
+        if ((rc = ctx->alloc(bytes, dptr))) return rc;
+        *cap = bytes;
+    }
+    HIP_TRY(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return B3AMD_OK;
+}
+
+int validate_pairs(const b3amd_context* ctx, uint32_t n, const uint32_t* pairs) {
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t a = pairs[2 * i], b = pairs[2 * i + 1];
+        if (a >= ctx->shapeCount || b >= ctx->shapeCount)
+            return fail(B3AMD_ERR_ARGUMENT, "pair %u refers to shape %u/%u of %u", i, a, b, ctx->shapeCount);
+        const ShapeRec& A = ctx->shapes[a];
+        const ShapeRec& B = ctx->shapes[b];
+        if (A.type > B.type) return fail(B3AMD_ERR_ARGUMENT, "pair %u: typeA > typeB (contact_manager.cpp:238-242)", i);
+        if (A.type != B3AMD_HULL || B.type != B3AMD_HULL)
+            return fail(B3AMD_ERR_UNSUPPORTED, "pair %u: shape types %u/%u not available in this build", i, A.type, B.type);
+    }
+    return B3AMD_OK;
+}
+
+int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs, const float* sweeps,
+              ConvexCache* caches, Manifold* out) {
+    if (n == 0) return B3AMD_OK;
+    if (!ctx->dPool) return fail(B3AMD_ERR_ARGUMENT, "no hull pool uploaded");
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    int rc = ensure_scratch(ctx, n);
+    if (rc) return rc;
+    PairInputs in;
+    in.n = n;
+    in.pairs = pairs;
+    in.shapes = ctx->dShapes;
+    in.xfs = xfs;
+    in.pool = ctx->dPool;
+    in.offsets = ctx->dOffsets;
+    in.shapeCount = ctx->shapeCount;
+    in.hullCount = ctx->hullCount;
+    CacheArgs ca;
+    static_cast<PairInputs&>(ca) = in;
+    ca.sweeps = sweeps;
+    ca.caches = caches;
+    ca.stage = ctx->dStage;
+    ca.out = out;
+    SatArgs sa;
+    static_cast<PairInputs&>(sa) = in;
+    sa.stage = ctx->dStage;
+    sa.out = ctx->dSat;
+    ContactArgs cta;
+    static_cast<PairInputs&>(cta) = in;
+    cta.sat = ctx->dSat;
+    cta.caches = caches;
+    cta.out = out;
+    hipStream_t s = ctx->stream;
+    int cls = ctx->sizeClass;
+    if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
+    if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_hull(sa, cls, s); }))) return rc;
+    if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
+    return B3AMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char* b3amd_last_error(void) { return g_error.c_str(); }
+
+int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
+    if (!out) return fail(B3AMD_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) return fail(B3AMD_ERR_NO_DEVICE, "no GPU visible");
+    if (device < 0 || device >= count) return fail(B3AMD_ERR_ARGUMENT, "device %d of %d", device, count);
+    HIP_TRY(hipSetDevice(device));
+    b3amd_context* ctx = new b3amd_context();
+    ctx->device = device;
+    ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete ctx;
+        return fail(B3AMD_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    *out = ctx;
+    return B3AMD_OK;
+}
+
+int b3amd_destroy(b3amd_context* ctx) {
+    if (!ctx) return B3AMD_OK;
+    hipSetDevice(ctx->device);
+    hipStreamSynchronize(ctx->stream);
+    for (auto& kv : ctx->allocs) hipFree(kv.first);
+    for (PendingTiming& t : ctx->pending) {
+        hipEventDestroy(t.start);
+        hipEventDestroy(t.stop);
+    }
+    for (hipEvent_t e : ctx->freeEvents) hipEventDestroy(e);
+    hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return B3AMD_OK;
+}
+
+uint64_t b3amd_memory_in_use(const b3amd_context* ctx) { return ctx ? ctx->inUse : 0; }
+
+int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
+    if (!ctx || !pool || !pool->desc || !pool->centroids || !pool->vertices || !pool->edges || !pool->faces ||
+        !pool->planes)
+        return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<uint32_t> all, words;
+    std::vector<uint32_t> offsets(pool->count);
+    std::vector<HullLayout> layouts(pool->count);
+    int cls = 0;
+    for (uint32_t h = 0; h < pool->count; ++h) {
+        const char* err = pack_hull(pool, h, words, &layouts[h]);
+        if (err) return fail(B3AMD_ERR_UNSUPPORTED, "hull %u: %s", h, err);
+        if ((all.size() / 4) > 0xFFFFFFFFull) return fail(B3AMD_ERR_UNSUPPORTED, "pool too large");
+        offsets[h] = (uint32_t)(all.size() / 4);
+        all.insert(all.end(), words.begin(), words.end());
+        const HullLayout& L = layouts[h];
+        if (L.V > 64 || L.F > 128 || L.E() > 192) cls = 1;
+    }
+    int rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = ctx->release(ctx->dPool)) || (rc = ctx->release(ctx->dOffsets))) return rc;
+    ctx->dPool = nullptr;
+    ctx->dOffsets = nullptr;
+    ctx->hullCount = 0;
+    if ((rc = ctx->alloc(4ull * all.size(), (void**)&ctx->dPool))) return rc;
+    if ((rc = ctx->alloc(4ull * offsets.size(), (void**)&ctx->dOffsets))) return rc;
+    HIP_TRY(hipMemcpy(ctx->dPool, all.data(), 4ull * all.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(ctx->dOffsets, offsets.data(), 4ull * offsets.size(), hipMemcpyHostToDevice));
+    ctx->hullCount = pool->count;
+    ctx->sizeClass = cls;
+    ctx->layouts.swap(layouts);
+    return B3AMD_OK;
+}
+
+int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* shapes) {
+    if (!ctx || (!shapes && count)) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    static_assert(sizeof(b3amd_shape) == sizeof(ShapeRec), "shape layout");
+    for (uint32_t i = 0; i < count; ++i) {
+        if (shapes[i].type > B3AMD_HULL) return fail(B3AMD_ERR_UNSUPPORTED, "shape %u: type %u", i, shapes[i].type);
+        if (shapes[i].type == B3AMD_HULL && shapes[i].hull >= ctx->hullCount)
+            return fail(B3AMD_ERR_ARGUMENT, "shape %u: hull %u of %u", i, shapes[i].hull, ctx->hullCount);
+    }
+    int rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    if ((rc = ctx->release(ctx->dShapes))) return rc;
+    ctx->dShapes = nullptr;
+    ctx->shapeCount = 0;
+    if ((rc = ctx->alloc(sizeof(ShapeRec) * (uint64_t)count, (void**)&ctx->dShapes))) return rc;
+    if (count) HIP_TRY(hipMemcpy(ctx->dShapes, shapes, sizeof(ShapeRec) * (uint64_t)count, hipMemcpyHostToDevice));
+    ctx->shapes.assign(reinterpret_cast<const ShapeRec*>(shapes), reinterpret_cast<const ShapeRec*>(shapes) + count);
+    ctx->shapeCount = count;
+    return B3AMD_OK;
+}
+
+int b3amd_collide(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const b3amd_transform* xfs,
+                  const b3amd_quat* sweeps, b3amd_convex_cache* caches, b3amd_manifold* out) {
+    if (!ctx || (n && (!pairs || !xfs || !out))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (n == 0) return B3AMD_OK;
+    int rc = validate_pairs(ctx, n, pairs);
+    if (rc) return rc;
+    const uint64_t bPairs = 8ull * n, bXf = 96ull * n, bSw = sweeps ? 64ull * n : 0, bC = caches ? 36ull * n : 0,
+                   bOut = 240ull * n;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t total = al(bPairs) + al(bXf) + al(bSw) + al(bC) + al(bOut);
+    if (total > ctx->dInCap) {
+        if ((rc = ctx->release(ctx->dIn))) return rc;
+        ctx->dIn = nullptr;
+        ctx->dInCap = 0;
+        if ((rc = ctx->alloc(total, &ctx->dIn))) return rc;
+        ctx->dInCap = total;
+    }
+    char* base = static_cast<char*>(ctx->dIn);
+    char* dPairs = base;
+    char* dXf = dPairs + al(bPairs);
+    char* dSw = dXf + al(bXf);
+    char* dC = dSw + al(bSw);
+    char* dOut = dC + al(bC);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dPairs, pairs, bPairs, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dXf, xfs, bXf, hipMemcpyHostToDevice, s));
+    if (sweeps) HIP_TRY(hipMemcpyAsync(dSw, sweeps, bSw, hipMemcpyHostToDevice, s));
+    if (caches) HIP_TRY(hipMemcpyAsync(dC, caches, bC, hipMemcpyHostToDevice, s));
+    rc = run_batch(ctx, n, reinterpret_cast<const uint32_t*>(dPairs), reinterpret_cast<const float*>(dXf),
+                   sweeps ? reinterpret_cast<const float*>(dSw) : nullptr,
+                   caches ? reinterpret_cast<ConvexCache*>(dC) : nullptr, reinterpret_cast<Manifold*>(dOut));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(out, dOut, bOut, hipMemcpyDeviceToHost, s));
+    if (caches) HIP_TRY(hipMemcpyAsync(caches, dC, bC, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ctx->resolve_timings();
+}
+
+int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                         const b3amd_quat* d_sweeps, b3amd_convex_cache* d_caches, b3amd_manifold* d_out) {
+    if (!ctx || (n && (!d_pairs || !d_xfs || !d_out))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
+    for (const ShapeRec& s : ctx->shapes)
+        if (s.type != B3AMD_HULL) return fail(B3AMD_ERR_UNSUPPORTED, "device batches need a hull-only shape table in this build");
+    return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
+                     reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
+}
+
+int b3amd_synchronize(b3amd_context* ctx) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ctx->resolve_timings();
+}
+
+int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr) {
+    if (!ctx || !ptr) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    return ctx->alloc(bytes, ptr);
+}
+
+int b3amd_device_free(b3amd_context* ctx, void* ptr) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ctx->release(ptr);
+}
+
+int b3amd_copy_to_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || !dst || !src) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_copy_to_host(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || !dst || !src) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_set_profiling(b3amd_context* ctx, int enabled) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    int rc = ctx->resolve_timings();
+    ctx->profiling = enabled != 0;
+    return rc;
+}
+
+int b3amd_kernel_stats(b3amd_context* ctx, const char* name, uint32_t* launches, double* total_ms) {
+    if (!ctx || !name || !launches || !total_ms) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    int rc = ctx->resolve_timings();
+    if (rc) return rc;
+    auto it = ctx->stats.find(name);
+    *launches = it == ctx->stats.end() ? 0 : it->second.launches;
+    *total_ms = it == ctx->stats.end() ? 0.0 : it->second.total_ms;
+    return B3AMD_OK;
+}
+
+int b3amd_reset_kernel_stats(b3amd_context* ctx) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    int rc = ctx->resolve_timings();
+    ctx->stats.clear();
+    return rc;
+}
+
+}  // extern "C"

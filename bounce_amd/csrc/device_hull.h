@@ -1,0 +1,101 @@
+// Device-resident hull records.
+//
+// One b3Hull (hull.h:37-64) becomes one contiguous block of 16-byte units in HBM:
+//   unit 0        header  x = V | F << 16, y = HE | maxFaceDegree << 16, z = HE / 2
+//   unit 1        centroid (x, y, z, 0)
+//   units 2..     V vertices (x, y, z, 0)
+//   then          F planes (nx, ny, nz, offset)
+//   then          HE/2 unique edges, one u32 each: origin(2k) | origin(2k+1) << 8 |
+//                 face(2k) << 16 | face(2k+1) << 24  (the SAT edge loop's working set)
+//   then          HE half-edges, one u32 each: origin | face << 8 | next << 16; the twin
+//                 of half-edge h is h ^ 1 (required by sat.cpp:123-128, checked at upload)
+//   then          F u16 first half-edge per face (b3Face::edge)
+// A kernel stages the float4 sections into LDS with one coalesced pass per wave.
+#ifndef B3AMD_DEVICE_HULL_H
+#define B3AMD_DEVICE_HULL_H
+
+#include <cstdint>
+
+#include "b3_math.h"
+
+namespace b3amd {
+
+constexpr uint32_t kMaxHullVertices = 255;
+constexpr uint32_t kMaxHullFaces = 255;
+constexpr uint32_t kMaxHullHalfEdges = 4096;
+
+struct HullLayout {
+    uint32_t V, F, HE;
+    B3_HD uint32_t E() const { return HE >> 1; }
+    B3_HD uint32_t vertOff() const { return 2; }
+    B3_HD uint32_t planeOff() const { return 2 + V; }
+    B3_HD uint32_t uedgeOff() const { return 2 + V + F; }
+    B3_HD uint32_t halfOff() const { return uedgeOff() + (E() + 3) / 4; }
+    B3_HD uint32_t faceOff() const { return halfOff() + (HE + 3) / 4; }
+    B3_HD uint32_t units() const { return faceOff() + (F + 7) / 8; }
+};
+
+// Read-only view of one hull record in global memory.
+struct DevHull {
+    const float4* base;
+    uint32_t V, F, HE;
+
+    __device__ static DevHull at(const float4* pool, const uint32_t* offsets, uint32_t h) {
+        DevHull r;
+        r.base = pool + offsets[h];
+        uint4 hd = reinterpret_cast<const uint4*>(r.base)[0];
+        r.V = hd.x & 0xFFFFu;
+        r.F = hd.x >> 16;
+        r.HE = hd.y & 0xFFFFu;
+        return r;
+    }
+    __device__ HullLayout layout() const { return HullLayout{V, F, HE}; }
+    __device__ V3 centroid() const {
+        float4 q = base[1];
+        return V3{q.x, q.y, q.z};
+    }
+    __device__ V3 vertex(uint32_t i) const {
+        float4 q = base[2 + i];
+        return V3{q.x, q.y, q.z};
+    }
+    __device__ Plane plane(uint32_t i) const {
+        float4 q = base[2 + V + i];
+        return Plane{V3{q.x, q.y, q.z}, q.w};
+    }
+    __device__ uint32_t uedge(uint32_t k) const {
+        return reinterpret_cast<const uint32_t*>(base + layout().uedgeOff())[k];
+    }
+    __device__ uint32_t half(uint32_t h) const {
+        return reinterpret_cast<const uint32_t*>(base + layout().halfOff())[h];
+    }
+    __device__ uint32_t origin(uint32_t h) const { return half(h) & 0xFFu; }
+    __device__ uint32_t face(uint32_t h) const { return (half(h) >> 8) & 0xFFu; }
+    __device__ uint32_t next(uint32_t h) const { return half(h) >> 16; }
+    __device__ uint32_t face_edge(uint32_t f) const {
+        return reinterpret_cast<const uint16_t*>(base + layout().faceOff())[f];
+    }
+};
+
+// Per-pair result of the separation queries (sat.cpp:33-173).
+struct SatResult {
+    uint32_t f1Index;
+    float f1Sep;
+    uint32_t f2Index;
+    float f2Sep;
+    uint32_t eIndex1;
+    uint32_t eIndex2;
+    float eSep;
+    uint32_t stage;  // kStage*
+};
+
+enum : uint32_t {
+    kStageSkip = 0,       // resolved by the feature cache, no SAT
+    kStageSepFace1 = 1,   // separated on a face of hull 1
+    kStageSepFace2 = 2,   // separated on a face of hull 2
+    kStageSepEdge = 3,    // separated on an edge pair
+    kStageOverlap = 4,    // overlapping: contact build needed
+};
+
+}  // namespace b3amd
+
+#endif

@@ -1,0 +1,104 @@
+// Device records and kernel launch interfaces shared by the kernels and the host API.
+#ifndef B3AMD_KERNELS_H
+#define B3AMD_KERNELS_H
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "b3_math.h"
+#include "device_hull.h"
+
+namespace b3amd {
+
+// b3amd_shape (include/bounce_amd.h).
+struct ShapeRec {
+    uint32_t type;
+    uint32_t hull;
+    float radius;
+    float v[6];
+    uint32_t pad[3];
+};
+static_assert(sizeof(ShapeRec) == 48, "shape record layout");
+
+// b3amd_manifold_point / b3amd_manifold.
+struct ManifoldPoint {
+    V3 localNormal1;
+    V3 localPoint1;
+    V3 localPoint2;
+    uint32_t triangleKey;
+    uint32_t key[4];
+};
+struct Manifold {
+    uint32_t pointCount;
+    uint32_t flags;
+    uint32_t reserved[2];
+    ManifoldPoint points[4];
+};
+static_assert(sizeof(ManifoldPoint) == 56, "manifold point layout");
+static_assert(sizeof(Manifold) == 240, "manifold layout");
+
+constexpr uint32_t kFlagPolyOverflow = 1u;
+
+// b3SATFeaturePair (sat.h:74-80) inside b3ConvexCache (collide.h:38-42).
+struct FeatureCache {
+    uint32_t state, type, index1, index2;
+};
+struct ConvexCache {
+    uint32_t simplex[5];  // b3SimplexCache, 20 bytes
+    FeatureCache feature;
+};
+static_assert(sizeof(ConvexCache) == 36, "convex cache layout");
+
+enum : uint32_t { kCacheSeparation = 0, kCacheOverlap = 1, kCacheEmpty = 2 };
+enum : uint32_t { kFeatureEdge1 = 0, kFeatureFace1 = 1, kFeatureFace2 = 2 };
+
+struct PairInputs {
+    uint32_t n;
+    const uint32_t* pairs;   // [n][2] shape indices
+    const ShapeRec* shapes;
+    const float* xfs;        // [n][2][12]
+    const float4* pool;      // hull records
+    const uint32_t* offsets; // hull -> first 16-byte unit
+    uint32_t shapeCount;
+    uint32_t hullCount;
+
+    // True when both shape indices and their hull indices are in range and both shapes
+    // are hulls; kernels skip (and flag) any other pair instead of reading out of bounds.
+    __device__ bool valid_hull_pair(uint32_t p) const {
+        uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
+        if (a >= shapeCount || b >= shapeCount) return false;
+        const ShapeRec& A = shapes[a];
+        const ShapeRec& B = shapes[b];
+        return A.type == 2u && B.type == 2u && A.hull < hullCount && B.hull < hullCount;
+    }
+};
+
+constexpr uint32_t kFlagInvalidPair = 2u;
+
+struct CacheArgs : PairInputs {
+    const float* sweeps;     // [n][4][4] or null
+    ConvexCache* caches;     // [n] or null
+    uint32_t* stage;         // [n] out
+    Manifold* out;           // [n]
+};
+
+struct SatArgs : PairInputs {
+    const uint32_t* stage;   // [n] from the cache kernel
+    SatResult* out;          // [n]
+};
+
+struct ContactArgs : PairInputs {
+    const SatResult* sat;    // [n]
+    ConvexCache* caches;     // [n] or null
+    Manifold* out;           // [n]
+};
+
+hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
+// sizeClass 0: every hull of the batch has V <= 64, F <= 128, HE/2 <= 192; 1: the limits.
+hipError_t launch_sat_hull(const SatArgs& a, int sizeClass, hipStream_t s);
+hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s);
+
+}  // namespace b3amd
+
+#endif

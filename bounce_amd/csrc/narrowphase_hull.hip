@@ -1,0 +1,847 @@
+// Hull-hull contact narrowphase for gfx950.
+//
+// Three kernels per batch (all on the context stream):
+//   cache_hull_kernel    one lane per pair: b3FeatureCache read and the cached rebuilds
+//                        (collide_hulls_cache.cpp:279-344); marks pairs that need a SAT.
+//   sat_hull_kernel      one 64-lane wave per pair: face queries in both directions and
+//                        the edge query (sat.cpp:33-173), staged through LDS.
+//   contact_hull_kernel  one lane per pair: the decision tree of b3CollideCache
+//                        (collide_hulls_cache.cpp:155-275), edge/face contact, clipping,
+//                        reduction, keys, manifold and feature-cache write.
+// All arithmetic keeps the reference's FP32 operation order (b3_math.h); the build uses
+// -ffp-contract=off and IEEE division/square root.
+#include <hip/hip_runtime.h>
+
+#include "b3_math.h"
+#include "device_hull.h"
+#include "kernels.h"
+
+namespace b3amd {
+
+// ------------------------------------------------------------------ wave reductions
+
+// (separation, key) arg-max: larger separation wins, equal separations keep the smaller
+// key.  Sequential strict-'>' scans in index order (sat.cpp:47,158) pick exactly this.
+__device__ __forceinline__ void wave_argmax(float& s, uint32_t& key) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float os = __shfl_xor(s, off, 64);
+        uint32_t ok = __shfl_xor(key, off, 64);
+        if (os > s || (os == s && ok < key)) {
+            s = os;
+            key = ok;
+        }
+    }
+}
+
+__device__ __forceinline__ V3 ld3(const float4& q) { return V3{q.x, q.y, q.z}; }
+
+__device__ __forceinline__ Xf load_xf(const float* p) {
+    Xf x;
+    x.R.c0 = V3{p[0], p[1], p[2]};
+    x.R.c1 = V3{p[3], p[4], p[5]};
+    x.R.c2 = V3{p[6], p[7], p[8]};
+    x.p = V3{p[9], p[10], p[11]};
+    return x;
+}
+
+// ------------------------------------------------------------------ SAT kernel
+
+// sat.cpp:73-108 with the per-edge lengths and P1 - C1 precomputed (same operations).
+__device__ __forceinline__ float project_edges(V3 P1, V3 E1, float L1, V3 D1, V3 P2, V3 E2, float L2) {
+    if (L1 < kLinearSlop) return -kMaxFloat;
+    if (L2 < kLinearSlop) return -kMaxFloat;
+    const float kTol = 0.005f;
+    V3 E1xE2 = cross(E1, E2);
+    float L = length(E1xE2);
+    if (L < kTol * L1 * L2) return -kMaxFloat;
+    V3 N = (1.0f / L) * E1xE2;
+    if (dot(N, D1) < 0.0f) N = -N;
+    return dot(N, P2 - P1);
+}
+
+template <int MAXV, int MAXF, int MAXE>
+struct SatLds {
+    float4 v1[MAXV];      // hull 1 vertices (hull-1 frame)
+    float4 v2[MAXV];      // hull 2 vertices (hull-2 frame)
+    float4 tv1[MAXV];     // hull 1 vertices in the hull-2 frame
+    float4 tn1[MAXF];     // hull 1 face normals in the hull-2 frame
+    float4 rec[MAXE * 4]; // per unique edge of hull 1: U1,V1,-E1 | L1,P1 | D1 (hull-2 frame)
+};
+
+template <int MAXV, int MAXF, int MAXE>
+__global__ __launch_bounds__(64) void sat_hull_kernel(SatArgs a) {
+    __shared__ SatLds<MAXV, MAXF, MAXE> lds;
+    const uint32_t lane = threadIdx.x;
+    const uint32_t p = blockIdx.x;
+    if (p >= a.n) return;
+    if (a.stage[p] == kStageSkip) {
+        if (lane == 0) a.out[p].stage = kStageSkip;
+        return;
+    }
+
+    const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
+    const uint32_t ha = a.shapes[sa].hull, hb = a.shapes[sb].hull;
+    const float totalRadius = a.shapes[sa].radius + a.shapes[sb].radius;
+    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
+    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const DevHull h1 = DevHull::at(a.pool, a.offsets, ha);
+    const DevHull h2 = DevHull::at(a.pool, a.offsets, hb);
+
+    for (uint32_t k = lane; k < h1.V; k += 64) lds.v1[k] = h1.base[2 + k];
+    for (uint32_t k = lane; k < h2.V; k += 64) lds.v2[k] = h2.base[2 + k];
+    __syncthreads();
+
+    SatResult r;
+    // ---- face query, hull 1 planes against hull 2 vertices (sat.cpp:33-58)
+    const Xf xf = mulT(xf2, xf1);
+    {
+        float best = -kMaxFloat;
+        uint32_t key = 0;
+        for (uint32_t i = lane; i < h1.F; i += 64) {
+            Plane pl = mul(xf, h1.plane(i));
+            lds.tn1[i] = make_float4(pl.n.x, pl.n.y, pl.n.z, 0.0f);
+            V3 d = -pl.n;
+            uint32_t mi = 0;
+            float mp = dot(d, ld3(lds.v2[0]));
+            for (uint32_t k = 1; k < h2.V; ++k) {
+                float q = dot(d, ld3(lds.v2[k]));
+                if (q > mp) {
+                    mp = q;
+                    mi = k;
+                }
+            }
+            float s = distance(ld3(lds.v2[mi]), pl);
+            if (s > best) {
+                best = s;
+                key = i;
+            }
+        }
+        wave_argmax(best, key);
+        r.f1Index = key;
+        r.f1Sep = best;
+    }
+    r.f2Index = 0;
+    r.f2Sep = -kMaxFloat;
+    r.eIndex1 = 0;
+    r.eIndex2 = 0;
+    r.eSep = -kMaxFloat;
+    if (r.f1Sep > totalRadius) {
+        r.stage = kStageSepFace1;
+        if (lane == 0) a.out[p] = r;
+        return;
+    }
+    // ---- face query, hull 2 planes against hull 1 vertices
+    {
+        const Xf xfb = mulT(xf1, xf2);
+        float best = -kMaxFloat;
+        uint32_t key = 0;
+        for (uint32_t i = lane; i < h2.F; i += 64) {
+            Plane pl = mul(xfb, h2.plane(i));
+            V3 d = -pl.n;
+            uint32_t mi = 0;
+            float mp = dot(d, ld3(lds.v1[0]));
+            for (uint32_t k = 1; k < h1.V; ++k) {
+                float q = dot(d, ld3(lds.v1[k]));
+                if (q > mp) {
+                    mp = q;
+                    mi = k;
+                }
+            }
+            float s = distance(ld3(lds.v1[mi]), pl);
+            if (s > best) {
+                best = s;
+                key = i;
+            }
+        }
+        wave_argmax(best, key);
+        r.f2Index = key;
+        r.f2Sep = best;
+    }
+    if (r.f2Sep > totalRadius) {
+        r.stage = kStageSepFace2;
+        if (lane == 0) a.out[p] = r;
+        return;
+    }
+    // ---- edge query (sat.cpp:110-173)
+    const V3 C1 = mul(xf, h1.centroid());
+    for (uint32_t k = lane; k < h1.V; k += 64) {
+        V3 t = mul(xf, ld3(lds.v1[k]));
+        lds.tv1[k] = make_float4(t.x, t.y, t.z, 0.0f);
+    }
+    __syncthreads();
+    const uint32_t E1n = h1.HE >> 1, E2n = h2.HE >> 1;
+    for (uint32_t i = lane; i < E1n; i += 64) {
+        uint32_t u = h1.uedge(i);
+        V3 P1 = ld3(lds.tv1[u & 0xFFu]);
+        V3 Q1 = ld3(lds.tv1[(u >> 8) & 0xFFu]);
+        V3 E1 = Q1 - P1;
+        V3 U1 = ld3(lds.tn1[(u >> 16) & 0xFFu]);
+        V3 V1 = ld3(lds.tn1[u >> 24]);
+        float L1 = length(E1);
+        V3 D1 = P1 - C1;
+        lds.rec[4 * i + 0] = make_float4(U1.x, U1.y, U1.z, V1.x);
+        lds.rec[4 * i + 1] = make_float4(V1.y, V1.z, -E1.x, -E1.y);
+        lds.rec[4 * i + 2] = make_float4(-E1.z, L1, P1.x, P1.y);
+        lds.rec[4 * i + 3] = make_float4(P1.z, D1.x, D1.y, D1.z);
+    }
+    constexpr int K = (MAXE + 63) / 64;
+    V3 P2[K], E2[K], nE2[K], nU2[K], nV2[K];
+    float L2[K];
+    bool valid[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        uint32_t j = lane + 64 * k;
+        valid[k] = j < E2n;
+        uint32_t u = valid[k] ? h2.uedge(j) : 0u;
+        P2[k] = ld3(lds.v2[u & 0xFFu]);
+        V3 Q2 = ld3(lds.v2[(u >> 8) & 0xFFu]);
+        E2[k] = Q2 - P2[k];
+        nE2[k] = -E2[k];
+        nU2[k] = -h2.plane((u >> 16) & 0xFFu).n;
+        nV2[k] = -h2.plane(u >> 24).n;
+        L2[k] = length(E2[k]);
+    }
+    __syncthreads();
+    float best = -kMaxFloat;
+    uint32_t key = 0;
+    for (uint32_t i = 0; i < E1n; ++i) {
+        const float4 q0 = lds.rec[4 * i + 0];
+        const float4 q1 = lds.rec[4 * i + 1];
+        const float4 q2 = lds.rec[4 * i + 2];
+        const V3 U1 = V3{q0.x, q0.y, q0.z};
+        const V3 V1 = V3{q0.w, q1.x, q1.y};
+        const V3 nE1 = V3{q1.z, q1.w, q2.x};
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            // b3IsMinkowskiFace(U1, V1, -E1, -U2, -V2, -E2) (sat.cpp:60-71)
+            float ADC = dot(U1, nE2[k]);
+            float BDC = dot(V1, nE2[k]);
+            float CBA = dot(nU2[k], nE1);
+            float DBA = dot(nV2[k], nE1);
+            bool pass = valid[k] && (CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f);
+            if (pass) {
+                const float4 q3 = lds.rec[4 * i + 3];
+                V3 E1 = -nE1;
+                V3 P1 = V3{q2.z, q2.w, q3.x};
+                V3 D1 = V3{q3.y, q3.z, q3.w};
+                float s = project_edges(P1, E1, q2.y, D1, P2[k], E2[k], L2[k]);
+                if (s > best) {
+                    best = s;
+                    key = (i << 16) | (lane + 64 * k);
+                }
+            }
+        }
+    }
+    wave_argmax(best, key);
+    r.eIndex1 = 2 * (key >> 16);
+    r.eIndex2 = 2 * (key & 0xFFFFu);
+    r.eSep = best;
+    r.stage = r.eSep > totalRadius ? kStageSepEdge : kStageOverlap;
+    if (lane == 0) a.out[p] = r;
+}
+
+// ------------------------------------------------------------------ contact building
+
+constexpr int kPolyCap = 32;  // b3StackArray<b3ClipVertex, 32> inline capacity (clip.cpp)
+
+struct ClipV {
+    V3 position;
+    uint32_t inEdge1, outEdge1, inEdge2, outEdge2;
+};
+
+struct Poly {
+    uint32_t count;
+    bool overflow;
+    ClipV v[kPolyCap];
+    __device__ void push(const ClipV& x) {
+        if (count < kPolyCap) v[count++] = x;
+        else overflow = true;
+    }
+};
+
+struct CPV {
+    V3 position;
+    uint32_t clipIndex;
+};
+struct CPoly {
+    uint32_t count;
+    CPV v[kPolyCap];
+};
+
+struct Contact {
+    uint32_t pointCount;
+    bool overflow;
+    V3 n[4], p1[4], p2[4];
+    uint32_t key[4][4];  // inEdge1, outEdge1, inEdge2, outEdge2
+};
+
+// b3MakePair argument order is (inEdge1, inEdge2, outEdge1, outEdge2) (clip.h:36-44).
+__device__ __forceinline__ ClipV clipv(V3 p, uint32_t in1, uint32_t in2, uint32_t out1, uint32_t out2) {
+    ClipV c;
+    c.position = p;
+    c.inEdge1 = in1;
+    c.inEdge2 = in2;
+    c.outEdge1 = out1;
+    c.outEdge2 = out2;
+    return c;
+}
+
+// hull.inl:63-78
+__device__ Plane edge_side_plane(const DevHull& h, uint32_t index) {
+    uint32_t he = h.half(index);
+    V3 P = h.vertex(he & 0xFFu);
+    V3 Q = h.vertex(h.origin(index ^ 1u));
+    V3 E = Q - P;
+    V3 D = cross(E, h.plane((he >> 8) & 0xFFu).n);
+    Plane r;
+    r.n = normalize(D);
+    r.d = dot(r.n, P);
+    return r;
+}
+
+// clip.cpp:33-57
+__device__ void build_polygon(Poly& out, const Xf& xf, uint32_t index, const DevHull& h) {
+    out.count = 0;
+    uint32_t begin = h.face_edge(index);
+    uint32_t e = begin;
+    uint32_t guard = 0;
+    do {
+        uint32_t he = h.half(e);
+        uint32_t twin = e ^ 1u;
+        uint32_t edgeId = twin ^ 1u;  // twin->twin
+        uint32_t prevId = h.next(twin);
+        out.push(clipv(mul(xf, h.vertex(he & 0xFFu)), prevId, kNullEdge, edgeId, kNullEdge));
+        e = he >> 16;
+    } while (e != begin && ++guard < 256);
+}
+
+// clip.cpp:99-149
+__device__ void clip_to_plane(Poly& out, const Poly& in, const Plane& pl, uint32_t id) {
+    out.count = 0;
+    ClipV v1 = in.v[in.count - 1];
+    float d1 = distance(v1.position, pl);
+    for (uint32_t i = 0; i < in.count; ++i) {
+        ClipV v2 = in.v[i];
+        float d2 = distance(v2.position, pl);
+        if (d1 <= 0.0f && d2 <= 0.0f) {
+            out.push(v2);
+        } else if (d1 <= 0.0f && d2 > 0.0f) {
+            float f = d1 / (d1 - d2);
+            V3 pos = v1.position + f * (v2.position - v1.position);
+            out.push(clipv(pos, v1.inEdge1, kNullEdge, kNullEdge, id));
+        } else if (d1 > 0.0f && d2 <= 0.0f) {
+            float f = d1 / (d1 - d2);
+            V3 pos = v1.position + f * (v2.position - v1.position);
+            out.push(clipv(pos, v1.inEdge1, id, kNullEdge, kNullEdge));
+            out.push(v2);
+        }
+        v1 = v2;
+        d1 = d2;
+    }
+}
+
+// clip.cpp:243-280; returns the clipped polygon in `a` (b is scratch).
+__device__ void clip_to_face(Poly& a, Poly& b, const Xf& xf, float r, uint32_t index, const DevHull& h) {
+    uint32_t begin = h.face_edge(index);
+    uint32_t e = begin;
+    uint32_t guard = 0;
+    do {
+        uint32_t edgeId = e;  // (edge->twin)->twin
+        Plane pl = edge_side_plane(h, edgeId);
+        pl.d += r;
+        Plane wp = mul(xf, pl);
+        clip_to_plane(b, a, wp, edgeId);
+        a.count = b.count;
+        a.overflow = a.overflow || b.overflow;
+        for (uint32_t i = 0; i < b.count; ++i) a.v[i] = b.v[i];
+        if (a.count == 0) return;
+        e = h.next(e);
+    } while (e != begin && ++guard < 256);
+}
+
+// contact_cluster.cpp:22-26
+__device__ __forceinline__ bool is_ccw(V3 A, V3 B, V3 C, V3 N) {
+    V3 n = cross(B - A, C - A);
+    return dot(n, N) > 0.0f;
+}
+
+// contact_cluster.cpp:33-56 (the pivot B is read once per i, before any swap)
+__device__ void sort_polygon(CPoly& p, V3 N) {
+    V3 A = p.v[0].position;
+    for (uint32_t i = 1; i < p.count; ++i) {
+        V3 B = p.v[i].position;
+        for (uint32_t j = i + 1; j < p.count; ++j) {
+            V3 C = p.v[j].position;
+            if (!is_ccw(A, B, C, N)) {
+                CPV t = p.v[i];
+                p.v[i] = p.v[j];
+                p.v[j] = t;
+            }
+        }
+    }
+}
+
+// contact_cluster.cpp:58-200: reduces `in` into `out`.
+__device__ void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_t initial) {
+    out.count = 0;
+    if (in.count <= 4) {
+        out.count = in.count;
+        for (uint32_t i = 0; i < in.count; ++i) out.v[i] = in.v[i];
+        sort_polygon(out, N);
+        return;
+    }
+    uint32_t chosen = 0;  // bit mask over in.v (count <= 32)
+    out.v[out.count++] = in.v[initial];
+    chosen |= 1u << initial;
+    {
+        V3 A = out.v[0].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 d = in.v[i].position - A;
+            float dd = dot(d, d);
+            if (dd > mx) {
+                mx = dd;
+                index = i;
+            }
+        }
+        if (mx < kEpsilon * kEpsilon) return;
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out.v[0].position, B = out.v[1].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 C = in.v[i].position;
+            V3 n = cross(B - A, C - A);
+            float a2 = fabsf(dot(n, N));
+            if (a2 > mx) {
+                mx = a2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (mx < 2.0f * kAreaTol) return;
+        V3 C = in.v[index].position;
+        if (!is_ccw(A, B, C, N)) {
+            CPV t = out.v[0];
+            out.v[0] = out.v[1];
+            out.v[1] = t;
+        }
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out.v[0].position, B = out.v[1].position;
+        uint32_t index = 0;
+        float mn = kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 D = in.v[i].position;
+            V3 n = cross(B - A, D - A);
+            float sa2 = dot(n, N);
+            if (sa2 < mn) {
+                mn = sa2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (fabsf(mn) < 2.0f * kAreaTol) return;
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    sort_polygon(out, N);
+}
+
+// collide_hulls.cpp:26-91, and with `rebuild` the segment check of
+// b3RebuildEdgeContact (collide_hulls_cache.cpp:35-111).
+__device__ void build_edge_contact(Contact& m, const Xf& xf1, uint32_t index1, const DevHull& h1,
+                                   const Xf& xf2, uint32_t index2, const DevHull& h2, bool rebuild) {
+    V3 C1 = mul(xf1, h1.centroid());
+    V3 P1 = mul(xf1, h1.vertex(h1.origin(index1)));
+    V3 Q1 = mul(xf1, h1.vertex(h1.origin(index1 + 1)));
+    V3 E1 = Q1 - P1;
+    V3 N1 = E1;
+    normalize_in_place(N1);
+    V3 P2 = mul(xf2, h2.vertex(h2.origin(index2)));
+    V3 Q2 = mul(xf2, h2.vertex(h2.origin(index2 + 1)));
+    V3 E2 = Q2 - P2;
+    V3 N2 = E2;
+    normalize_in_place(N2);
+    float b = dot(N1, N2);
+    float den = 1.0f - b * b;
+    if (den == 0.0f) return;
+    float inv_den = 1.0f / den;
+    V3 E3 = P1 - P2;
+    float d = dot(N1, E3);
+    float e = dot(N2, E3);
+    float s = inv_den * (b * e - d);
+    float t = inv_den * (e - b * d);
+    V3 c1 = P1 + s * N1;
+    V3 c2 = P2 + t * N2;
+    if (rebuild) {
+        float w2[3], w1[3];
+        barycentric(w2, P1, Q1, c2);
+        barycentric(w1, P2, Q2, c1);
+        if (!(w2[1] > 0.0f && w2[1] <= w2[2] && w1[1] > 0.0f && w1[1] <= w1[2])) return;
+    }
+    V3 N = cross(E1, E2);
+    normalize_in_place(N);
+    if (dot(N, P1 - C1) < 0.0f) N = -N;
+    m.pointCount = 1;
+    m.n[0] = mulT(xf1.R, N);
+    m.p1[0] = mulT(xf1, c1);
+    m.p2[0] = mulT(xf2, c2);
+    m.key[0][0] = index1;      // inEdge1
+    m.key[0][1] = index2;      // outEdge1
+    m.key[0][2] = index1 + 1;  // inEdge2
+    m.key[0][3] = index2 + 1;  // outEdge2
+}
+
+// collide_hulls.cpp:93-204
+__device__ void build_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const DevHull& h1, float r1,
+                                   const Xf& xf2, const DevHull& h2, float r2, bool flip) {
+    Poly poly, scratch;
+    poly.overflow = false;
+    scratch.overflow = false;
+    float totalRadius = r1 + r2;
+    Plane plane1 = mul(xf1, h1.plane(index1));
+    V3 normal1 = mulT(xf2.R, plane1.n);
+    // hull.inl:47-61 support face of hull 2 along -normal1
+    V3 dn = -normal1;
+    uint32_t index2 = 0;
+    float mp = dot(dn, h2.plane(0).n);
+    for (uint32_t i = 1; i < h2.F; ++i) {
+        float q = dot(dn, h2.plane(i).n);
+        if (q > mp) {
+            mp = q;
+            index2 = i;
+        }
+    }
+    build_polygon(poly, xf2, index2, h2);
+    clip_to_face(poly, scratch, xf1, totalRadius, index1, h1);
+    m.overflow = m.overflow || poly.overflow;
+    if (poly.count == 0) return;
+    CPoly p1, red;
+    p1.count = 0;
+    uint32_t minIndex = 0;
+    float minSep = kMaxFloat;
+    for (uint32_t i = 0; i < poly.count; ++i) {
+        V3 pos = poly.v[i].position;
+        float sep = distance(pos, plane1);
+        if (sep <= totalRadius) {
+            if (sep < minSep) {
+                minIndex = p1.count;
+                minSep = sep;
+            }
+            p1.v[p1.count].position = closest_on_plane(pos, plane1);
+            p1.v[p1.count].clipIndex = i;
+            ++p1.count;
+        }
+    }
+    if (p1.count == 0) return;
+    V3 normal = plane1.n;
+    V3 s_normal = flip ? -normal : normal;
+    reduce_polygon(red, p1, s_normal, minIndex);
+    for (uint32_t i = 0; i < red.count; ++i) {
+        const ClipV& v2 = poly.v[red.v[i].clipIndex];
+        V3 v1 = closest_on_plane(v2.position, plane1);
+        if (flip) {
+            // b3MakePair(in2, in1, out2, out1) swaps the hull roles of the key.
+            m.n[i] = mulT(xf2.R, s_normal);
+            m.p1[i] = mulT(xf2, v2.position);
+            m.p2[i] = mulT(xf1, v1);
+            m.key[i][0] = v2.inEdge2;
+            m.key[i][1] = v2.outEdge2;
+            m.key[i][2] = v2.inEdge1;
+            m.key[i][3] = v2.outEdge1;
+        } else {
+            m.n[i] = mulT(xf1.R, normal);
+            m.p1[i] = mulT(xf1, v1);
+            m.p2[i] = mulT(xf2, v2.position);
+            m.key[i][0] = v2.inEdge1;
+            m.key[i][1] = v2.outEdge1;
+            m.key[i][2] = v2.inEdge2;
+            m.key[i][3] = v2.outEdge2;
+        }
+    }
+    m.pointCount = red.count;
+}
+
+// collide_hulls_cache.cpp:113-153
+__device__ void rebuild_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const DevHull& h1, float r1,
+                                     const Xf& xf2, const DevHull& h2, float r2, bool flip,
+                                     Quat q10, Quat q1, Quat q20, Quat q2) {
+    Quat dq0 = qmul(conj(q10), q20);
+    Quat dq = qmul(conj(q1), q2);
+    Quat q = qmul(conj(dq0), dq);
+    const float kTol = 0.995f;
+    if (fabsf(q.w) > kTol) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
+}
+
+__device__ void write_manifold(Manifold* out, const Contact& c) {
+    Manifold m;
+    m.pointCount = c.pointCount;
+    m.flags = c.overflow ? kFlagPolyOverflow : 0u;
+    m.reserved[0] = 0;
+    m.reserved[1] = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        ManifoldPoint& q = m.points[i];
+        if ((uint32_t)i < c.pointCount) {
+            q.localNormal1 = c.n[i];
+            q.localPoint1 = c.p1[i];
+            q.localPoint2 = c.p2[i];
+            q.triangleKey = kNullTriangle;
+            q.key[0] = c.key[i][0];
+            q.key[1] = c.key[i][1];
+            q.key[2] = c.key[i][2];
+            q.key[3] = c.key[i][3];
+        } else {
+            q.localNormal1 = V3{0.0f, 0.0f, 0.0f};
+            q.localPoint1 = V3{0.0f, 0.0f, 0.0f};
+            q.localPoint2 = V3{0.0f, 0.0f, 0.0f};
+            q.triangleKey = 0;
+            q.key[0] = q.key[1] = q.key[2] = q.key[3] = 0;
+        }
+    }
+    *out = m;
+}
+
+__device__ __forceinline__ void set_feature(FeatureCache* c, uint32_t state, uint32_t type, uint32_t i1, uint32_t i2) {
+    if (!c) return;
+    c->state = state;
+    c->type = type;
+    c->index1 = i1;
+    c->index2 = i2;
+}
+
+// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275).
+__global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    const SatResult r = a.sat[p];
+    if (r.stage == kStageSkip) return;  // the cache kernel wrote this pair
+    FeatureCache* fc = a.caches ? &a.caches[p].feature : nullptr;
+    Contact m;
+    m.pointCount = 0;
+    m.overflow = false;
+    if (r.stage != kStageOverlap) {
+        if (r.stage == kStageSepFace1) set_feature(fc, kCacheSeparation, kFeatureFace1, r.f1Index, r.f1Index);
+        else if (r.stage == kStageSepFace2) set_feature(fc, kCacheSeparation, kFeatureFace2, r.f2Index, r.f2Index);
+        else set_feature(fc, kCacheSeparation, kFeatureEdge1, r.eIndex1, r.eIndex2);
+        write_manifold(a.out + p, m);
+        return;
+    }
+    const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
+    const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
+    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
+    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
+    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+
+    const float kTol = 0.1f * kLinearSlop;
+    const float fmax12 = r.f1Sep > r.f2Sep ? r.f1Sep : r.f2Sep;  // b3Max
+    if (r.eSep > fmax12 + kTol) {
+        build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    } else if (r.f1Sep + kTol > r.f2Sep) {
+        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    } else {
+        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    }
+    // Heuristic failed: fallback without the tolerance.
+    if (r.eSep > fmax12) {
+        build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    } else if (r.f1Sep > r.f2Sep) {
+        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    } else {
+        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
+        if (m.pointCount > 0) {
+            set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
+            write_manifold(a.out + p, m);
+            return;
+        }
+    }
+    // Last resort: a contact between the edge lines.
+    build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
+    set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
+    write_manifold(a.out + p, m);
+}
+
+// ------------------------------------------------------------------ feature-cache kernel
+
+// b3FeatureCache::ReadFace (sat.cpp:209-222)
+__device__ uint32_t read_face(uint32_t index1, const Xf& xf1, const DevHull& h1, const Xf& xf2, const DevHull& h2,
+                              float totalRadius) {
+    Xf xf = mulT(xf2, xf1);
+    Plane pl = mul(xf, h1.plane(index1));
+    V3 d = -pl.n;
+    uint32_t mi = 0;
+    float mp = dot(d, h2.vertex(0));
+    for (uint32_t k = 1; k < h2.V; ++k) {
+        float q = dot(d, h2.vertex(k));
+        if (q > mp) {
+            mp = q;
+            mi = k;
+        }
+    }
+    float s = distance(h2.vertex(mi), pl);
+    return s > totalRadius ? kCacheSeparation : kCacheOverlap;
+}
+
+// b3FeatureCache::ReadEdge (sat.cpp:224-278)
+__device__ uint32_t read_edge(uint32_t i, uint32_t j, const Xf& xf1, const DevHull& h1, const Xf& xf2,
+                              const DevHull& h2, float totalRadius) {
+    Xf xf = mulT(xf2, xf1);
+    V3 C1 = mul(xf, h1.centroid());
+    V3 P1 = mul(xf, h1.vertex(h1.origin(i)));
+    V3 Q1 = mul(xf, h1.vertex(h1.origin(i + 1)));
+    V3 E1 = Q1 - P1;
+    V3 U1 = mul(xf.R, h1.plane(h1.face(i)).n);
+    V3 V1 = mul(xf.R, h1.plane(h1.face(i + 1)).n);
+    V3 P2 = h2.vertex(h2.origin(j));
+    V3 Q2 = h2.vertex(h2.origin(j + 1));
+    V3 E2 = Q2 - P2;
+    V3 U2 = h2.plane(h2.face(j)).n;
+    V3 V2 = h2.plane(h2.face(j + 1)).n;
+    V3 nE1 = -E1, nE2 = -E2, nU2 = -U2, nV2 = -V2;
+    float ADC = dot(U1, nE2);
+    float BDC = dot(V1, nE2);
+    float CBA = dot(nU2, nE1);
+    float DBA = dot(nV2, nE1);
+    if ((CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f)) {
+        float s = project_edges(P1, E1, length(E1), P1 - C1, P2, E2, length(E2));
+        return s > totalRadius ? kCacheSeparation : kCacheOverlap;
+    }
+    return kCacheEmpty;
+}
+
+// b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush.
+__global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    if (!a.valid_hull_pair(p)) {
+        Contact m;
+        m.pointCount = 0;
+        m.overflow = false;
+        write_manifold(a.out + p, m);
+        a.out[p].flags = kFlagInvalidPair;
+        a.stage[p] = kStageSkip;
+        return;
+    }
+    FeatureCache* fc = a.caches ? &a.caches[p].feature : nullptr;
+    if (!fc || fc->state == kCacheEmpty) {
+        // ReadState on an empty cache evaluates face 0 of hull 1 and the result is
+        // discarded by the flush (sat.cpp:182-185): go straight to the SAT.
+        if (fc) fc->state = kCacheEmpty;
+        a.stage[p] = kStageOverlap;
+        return;
+    }
+    const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
+    const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
+    const float totalRadius = r1 + r2;
+    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
+    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
+    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+    const uint32_t state0 = fc->state, type = fc->type, i1 = fc->index1, i2 = fc->index2;
+    // A cached feature outside its hull cannot come from this pair: treat it as a miss.
+    const bool inRange = (type == kFeatureEdge1 && i1 + 1 < h1.HE && i2 + 1 < h2.HE && !(i1 & 1u) && !(i2 & 1u)) ||
+                         (type == kFeatureFace1 && i1 < h1.F) || (type == kFeatureFace2 && i1 < h2.F);
+    if (!inRange || state0 > kCacheEmpty) {
+        fc->state = kCacheEmpty;
+        a.stage[p] = kStageOverlap;
+        return;
+    }
+    uint32_t state1;
+    switch (type) {
+        case kFeatureEdge1: state1 = read_edge(i1, i2, xf1, h1, xf2, h2, totalRadius); break;
+        case kFeatureFace1: state1 = read_face(i1, xf1, h1, xf2, h2, totalRadius); break;
+        case kFeatureFace2: state1 = read_face(i1, xf2, h2, xf1, h1, totalRadius); break;
+        default: state1 = kCacheEmpty; break;
+    }
+    Contact m;
+    m.pointCount = 0;
+    m.overflow = false;
+    if (state0 == kCacheSeparation && state1 == kCacheSeparation) {
+        write_manifold(a.out + p, m);
+        a.stage[p] = kStageSkip;
+        return;
+    }
+    if (state0 == kCacheOverlap && state1 == kCacheOverlap) {
+        Quat q10{0, 0, 0, 1}, q1{0, 0, 0, 1}, q20{0, 0, 0, 1}, q2{0, 0, 0, 1};
+        if (a.sweeps) {
+            const float* s = a.sweeps + 16 * (size_t)p;
+            q10 = Quat{s[0], s[1], s[2], s[3]};
+            q1 = Quat{s[4], s[5], s[6], s[7]};
+            q20 = Quat{s[8], s[9], s[10], s[11]};
+            q2 = Quat{s[12], s[13], s[14], s[15]};
+        }
+        switch (type) {
+            case kFeatureEdge1: build_edge_contact(m, xf1, i1, h1, xf2, i2, h2, true); break;
+            case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2); break;
+            case kFeatureFace2: rebuild_face_contact(m, xf2, i1, h2, r2, xf1, h1, r1, true, q20, q2, q10, q1); break;
+            default: break;
+        }
+        if (m.pointCount > 0) {
+            write_manifold(a.out + p, m);
+            a.stage[p] = kStageSkip;
+            return;
+        }
+    }
+    fc->state = kCacheEmpty;
+    a.stage[p] = kStageOverlap;
+}
+
+// ------------------------------------------------------------------ launchers
+
+hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    cache_hull_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_sat_hull(const SatArgs& a, int sizeClass, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    if (sizeClass == 0) sat_hull_kernel<64, 128, 192><<<a.n, 64, 0, s>>>(a);
+    else sat_hull_kernel<255, 255, 512><<<a.n, 64, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    contact_hull_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace b3amd

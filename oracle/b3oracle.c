@@ -26,14 +26,21 @@
 #include <string.h>
 
 /* ---------------------------------------------------------------- op counting */
+/* Ops are counted per phase: 0 = feature-cache read and rebuilds, 1 = separation queries
+ * (face/edge SAT), 2 = contact building (clip, reduce, manifold). */
 #ifdef B3O_COUNT_FLOPS
-static uint64_t g_flops;
-#define FL(n) (g_flops += (uint64_t)(n))
-uint64_t b3o_flop_count(void) { return g_flops; }
-void b3o_flop_reset(void) { g_flops = 0; }
+static uint64_t g_flops[3];
+static int g_phase;
+#define FL(n) (g_flops[g_phase] += (uint64_t)(n))
+#define PHASE(p) (g_phase = (p))
+uint64_t b3o_flop_count(void) { return g_flops[0] + g_flops[1] + g_flops[2]; }
+uint64_t b3o_flop_count_phase(int p) { return (p >= 0 && p < 3) ? g_flops[p] : 0; }
+void b3o_flop_reset(void) { g_flops[0] = g_flops[1] = g_flops[2] = 0; g_phase = 0; }
 #else
 #define FL(n) ((void)0)
+#define PHASE(p) ((void)0)
 uint64_t b3o_flop_count(void) { return 0; }
+uint64_t b3o_flop_count_phase(int p) { (void)p; return 0; }
 void b3o_flop_reset(void) {}
 #endif
 
@@ -618,6 +625,7 @@ static void make_fp(b3o_feature_cache* c, int state, int type, uint32_t i1, uint
 
 static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1,   /* collide_hulls_cache.cpp:155-275 */
                           const XF* xf2, const Hull* h2, float r2, b3o_feature_cache* c) {
+    PHASE(1);
     FL(1);
     float totalRadius = r1 + r2;
     FaceQuery f1 = query_face(xf1, h1, xf2, h2);
@@ -626,6 +634,7 @@ static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float 
     if (f2.separation > totalRadius) { make_fp(c, B3O_SEPARATION, B3O_FACE2, f2.index, f2.index); return; }
     EdgeQuery eq = query_edge(xf1, h1, xf2, h2);
     if (eq.separation > totalRadius) { make_fp(c, B3O_SEPARATION, B3O_EDGE1, eq.index1, eq.index2); return; }
+    PHASE(2);
     const float kTol = 0.1f * LINEAR_SLOP;
     FL(1);
     float fmax12 = f1.separation > f2.separation ? f1.separation : f2.separation;      /* b3Max: a > b ? a : b */
@@ -662,6 +671,7 @@ static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float 
 static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1, /* collide_hulls_cache.cpp:279-344 */
                                  const XF* xf2, const Hull* h2, float r2, b3o_feature_cache* c,
                                  QT q10, QT q1, QT q20, QT q2) {
+    PHASE(0);
     FL(1);
     float totalRadius = r1 + r2;
     int state0 = (int)c->state;

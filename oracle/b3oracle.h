@@ -105,6 +105,8 @@ void b3o_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hul
 
 /* FP32 ops executed by the last b3o_collide_pairs call (liboracle_count.so only). */
 uint64_t b3o_flop_count(void);
+/* Per phase: 0 feature-cache read/rebuild, 1 separation queries, 2 contact building. */
+uint64_t b3o_flop_count_phase(int phase);
 void b3o_flop_reset(void);
 
 /* ---- compiled reference driver (_ref/libb3ref.so) ---- */

@@ -1,0 +1,88 @@
+"""Writes the golden fixtures for the narrowphase from the compiled reference.
+
+Run in a container where the reference tree is present and oracle/_ref is built
+(`make -C oracle`).  Each fixture holds the inputs (hull pool, shapes, pairs, transforms,
+sweeps, caches) and the outputs of b3CollideShapeAndShape as computed by the reference
+library compiled from its own sources (oracle/_ref/libb3ref.so).
+
+    python tests/golden/make_golden.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+sys.path.insert(0, ROOT)
+sys.path.insert(0, os.path.join(ROOT, "tests"))
+
+import oracle_lib  # noqa: E402
+from bounce_amd import load_library  # noqa: E402
+from bounce_amd.shapes import box_hull, identity_xf  # noqa: E402
+from bounce_amd.types import HullPool, empty_caches  # noqa: E402
+from util import hull_workload  # noqa: E402
+
+
+def save(name, pool, shapes, pairs, xfs, out, sweeps=None, caches_in=None, caches_out=None):
+    d = dict(desc=pool.desc, centroids=pool.centroids, vertices=pool.vertices, edges=pool.edges,
+             faces=pool.faces, planes=pool.planes, shapes=shapes, pairs=pairs, xfs=xfs, out=out)
+    if sweeps is not None:
+        d["sweeps"] = sweeps
+    if caches_in is not None:
+        d["caches_in"] = caches_in
+        d["caches_out"] = caches_out
+    np.savez_compressed(os.path.join(HERE, name), **d)
+
+
+def main():
+    ref = oracle_lib.reference()
+    if ref is None:
+        raise SystemExit("oracle/_ref/libb3ref.so not built")
+    lib = load_library()
+
+    # 1. Batched hull-hull pairs (8..64-point ellipsoid hulls, 60% overlapping), empty caches.
+    pool, shapes, pairs, xfs = hull_workload(lib, 1234, 48, 768)
+    out = ref.collide(pool, shapes, pairs, xfs)
+    save("hull_pairs.npz", pool, shapes, pairs, xfs, out)
+
+    # 2. The same pairs stepped once more with warm caches, perturbed transforms and sweeps.
+    caches = empty_caches(len(pairs))
+    ref.collide(pool, shapes, pairs, xfs, caches=caches)
+    rng = np.random.default_rng(7)
+    xfs2 = xfs.copy()
+    xfs2[:, 1, 9:12] += rng.normal(0.0, 0.004, size=(len(pairs), 3)).astype(np.float32)
+    sweeps = np.zeros((len(pairs), 4, 4), np.float32)
+    sweeps[:, :, 3] = 1.0
+    sweeps[::3, 3, :] = np.array([0.0, 0.1, 0.0, 0.99498744], np.float32)  # gate the face rebuild
+    c_in = caches.copy()
+    out2 = ref.collide(pool, shapes, pairs, xfs2, sweeps=sweeps, caches=caches)
+    save("hull_pairs_warm.npz", pool, shapes, pairs, xfs2, out2, sweeps=sweeps, caches_in=c_in, caches_out=caches)
+
+    # 3. Boxes: the Box Face Contact scene (box_face_contact.h:25-49) and random box pairs.
+    bpool = HullPool.concat([box_hull(1, 2, 1), box_hull(1, 1, 1), box_hull(0.5, 0.25, 1.5)])
+    bshapes = bpool.shapes()
+    rng = np.random.default_rng(11)
+    n = 256
+    bp = rng.integers(0, 3, size=(n, 2)).astype(np.uint32)
+    bx = np.zeros((n, 2, 12), np.float32)
+    for i in range(n):
+        for k in range(2):
+            q = rng.normal(size=4)
+            q /= np.linalg.norm(q)
+            x, y, z, w = q
+            R = np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)],
+                          [2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)],
+                          [2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)]])
+            pos = rng.uniform(-2.2, 2.2, size=3) if k else np.zeros(3)
+            bx[i, k] = identity_xf(pos, R)
+    bp[0] = [0, 1]
+    bx[0, 0] = identity_xf((0, 2, 0))
+    bx[0, 1] = identity_xf((0, 0, 0))
+    bout = ref.collide(bpool, bshapes, bp, bx)
+    save("box_pairs.npz", bpool, bshapes, bp, bx, bout)
+    print("wrote fixtures; overlap fractions:", (out["pointCount"] > 0).mean(), (bout["pointCount"] > 0).mean())
+
+
+if __name__ == "__main__":
+    main()

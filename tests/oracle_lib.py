@@ -76,6 +76,8 @@ def oracle_counting():
         raise RuntimeError("oracle/liboracle_count.so is not built")
     c = Checker(lib, "b3o_")
     lib.b3o_flop_count.restype = ctypes.c_uint64
+    lib.b3o_flop_count_phase.restype = ctypes.c_uint64
+    lib.b3o_flop_count_phase.argtypes = [ctypes.c_int]
     lib.b3o_flop_reset.restype = None
     return c
 

@@ -1,0 +1,138 @@
+"""GPU parity of the hull-hull narrowphase (HIP kernels, through the C ABI) against the
+oracle and the golden fixtures written by the compiled reference: bit-exact point
+counts, feature keys, manifold order, geometry and feature caches."""
+import sys
+
+import numpy as np
+import pytest
+
+import bounce_amd
+from bounce_amd.shapes import box_hull, identity_xf
+from bounce_amd.types import HullPool, MANIFOLD_DTYPE, empty_caches
+from util import assert_caches_equal, assert_manifolds_equal, hull_workload, load_golden
+
+pytestmark = pytest.mark.gpu
+
+
+def say(msg):
+    print(msg, flush=True)
+    sys.stdout.flush()
+
+
+def run(ctx, pool, shapes, pairs, xfs, sweeps=None, caches=None):
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    return ctx.collide(pairs, xfs, sweeps=sweeps, caches=caches)
+
+
+def test_gpu_box_face_contact(ctx, oracle):
+    say("case: box face contact")
+    pool = HullPool.concat([box_hull(1, 2, 1), box_hull(1, 1, 1)])
+    xfs = np.stack([identity_xf((0, 2, 0)), identity_xf((0, 0, 0))])[None]
+    pairs = np.array([[0, 1]], np.uint32)
+    got = run(ctx, pool, pool.shapes(), pairs, xfs)
+    assert got["pointCount"][0] == 4
+    assert_manifolds_equal(got, oracle.collide(pool, pool.shapes(), pairs, xfs))
+
+
+@pytest.mark.parametrize("name", ["hull_pairs.npz", "box_pairs.npz"])
+def test_gpu_golden(ctx, name):
+    say("case: golden %s" % name)
+    g = load_golden(name)
+    assert_manifolds_equal(run(ctx, g["pool"], g["shapes"], g["pairs"], g["xfs"]), g["out"], name)
+
+
+def test_gpu_golden_warm(ctx):
+    say("case: golden warm")
+    g = load_golden("hull_pairs_warm.npz")
+    caches = g["caches_in"].copy()
+    out = run(ctx, g["pool"], g["shapes"], g["pairs"], g["xfs"], sweeps=g["sweeps"], caches=caches)
+    assert_manifolds_equal(out, g["out"], "warm step")
+    assert_caches_equal(caches, g["caches_out"])
+
+
+@pytest.mark.parametrize("seed,nmin,nmax,gap", [(101, 8, 64, -0.0127), (202, 4, 16, 0.0), (303, 48, 64, -0.05)])
+def test_gpu_equals_oracle_random(ctx, lib, oracle, seed, nmin, nmax, gap):
+    say("case: random seed=%d n=%d..%d gap=%g" % (seed, nmin, nmax, gap))
+    pool, shapes, pairs, xfs = hull_workload(lib, seed, 300, 6000, gap_mean=gap, nmin=nmin, nmax=nmax)
+    got = run(ctx, pool, shapes, pairs, xfs)
+    assert_manifolds_equal(got, oracle.collide(pool, shapes, pairs, xfs))
+    caches = empty_caches(len(pairs))
+    want_c = empty_caches(len(pairs))
+    got2 = run(ctx, pool, shapes, pairs, xfs, caches=caches)
+    oracle.collide(pool, shapes, pairs, xfs, caches=want_c)
+    assert_manifolds_equal(got2, got)
+    assert_caches_equal(caches, want_c)
+
+
+def test_gpu_large_hulls(ctx, lib, oracle):
+    """Hulls beyond the 64-vertex class run on the large-record kernel."""
+    say("case: large hulls 70..128")
+    pool, shapes, pairs, xfs = hull_workload(lib, 404, 40, 400, nmin=70, nmax=128)
+    assert_manifolds_equal(run(ctx, pool, shapes, pairs, xfs), oracle.collide(pool, shapes, pairs, xfs))
+
+
+def test_gpu_warm_start_sequence(ctx, lib, oracle):
+    """Five steps with persistent caches, moving bodies and sweeps: every cache path."""
+    say("case: warm-start sequence")
+    pool, shapes, pairs, xfs = hull_workload(lib, 505, 200, 4000)
+    rng = np.random.default_rng(5)
+    cg, co = empty_caches(len(pairs)), empty_caches(len(pairs))
+    x = xfs.copy()
+    for step in range(5):
+        sweeps = np.zeros((len(pairs), 4, 4), np.float32)
+        sweeps[:, :, 3] = 1.0
+        tilt = rng.uniform(0, 0.15, size=len(pairs)).astype(np.float32)
+        sweeps[:, 3, 1] = tilt
+        sweeps[:, 3, 3] = np.sqrt(1 - tilt * tilt)
+        got = run(ctx, pool, shapes, pairs, x, sweeps=sweeps, caches=cg)
+        want = oracle.collide(pool, shapes, pairs, x, sweeps=sweeps, caches=co)
+        assert_manifolds_equal(got, want, "step %d" % step)
+        assert_caches_equal(cg, co, "step %d" % step)
+        x[:, 1, 9:12] += rng.normal(0, 0.003, size=(len(pairs), 3)).astype(np.float32)
+
+
+def test_gpu_device_api_equals_host_api(ctx, lib):
+    say("case: device-resident API")
+    pool, shapes, pairs, xfs = hull_workload(lib, 606, 100, 3000)
+    want = run(ctx, pool, shapes, pairs, xfs)
+    n = len(pairs)
+    dp, dx, do = ctx.alloc(pairs.nbytes), ctx.alloc(xfs.nbytes), ctx.alloc(n * MANIFOLD_DTYPE.itemsize)
+    dp.upload(pairs)
+    dx.upload(xfs)
+    ctx.collide_device(n, dp, dx, do)
+    ctx.synchronize()
+    got = do.download(np.zeros(n, MANIFOLD_DTYPE))
+    for b in (dp, dx, do):
+        b.free()
+    assert_manifolds_equal(got, want)
+
+
+def test_gpu_budget_admission(lib):
+    say("case: HBM budget admission")
+    with bounce_amd.Context(0, hbm_budget_bytes=1 << 20) as c:
+        with pytest.raises(bounce_amd.BounceError) as e:
+            c.alloc(2 << 20)
+        assert e.value.code == bounce_amd.ERR_BUDGET
+        b = c.alloc(1 << 19)
+        assert c.memory_in_use >= 1 << 19
+        b.free()
+        assert c.memory_in_use == 0
+
+
+def test_gpu_full_size_properties(ctx, lib, oracle):
+    """1,048,576 pairs: ~60% overlapping, manifolds well formed, a strided sample of
+    2,048 pairs bit-exact against the oracle, and the batch is deterministic."""
+    say("case: full size 1M pairs")
+    from bounce_amd.workloads import gen_hull_pool, gen_hull_pairs
+    pool = gen_hull_pool(lib, 2024, 65536)
+    pairs, xfs = gen_hull_pairs(lib, 2025, pool, 1 << 20, -0.0127, 0.05)
+    shapes = pool.shapes()
+    got = run(ctx, pool, shapes, pairs, xfs)
+    frac = (got["pointCount"] > 0).mean()
+    assert 0.57 < frac < 0.63
+    assert got["pointCount"].max() <= 4 and (got["flags"] == 0).all()
+    idx = np.arange(0, len(pairs), 512)
+    assert_manifolds_equal(got[idx], oracle.collide(pool, shapes, pairs[idx], xfs[idx]))
+    again = ctx.collide(pairs, xfs)
+    assert_manifolds_equal(again, got, "repeat")
