@@ -37,13 +37,17 @@ def workload(lib, n_pairs, n_pool):
 
 # ----------------------------------------------------------------- CPU baseline
 
-def _cpu_worker(args):
-    kind, pool, shapes, pairs, xfs = args
+_CPU_JOB = {}  # set before forking: children inherit the arrays instead of unpickling them
+
+
+def _cpu_worker(chunk):
+    j = _CPU_JOB
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
-    chk = oracle_lib.reference() if kind == "reference" else oracle_lib.oracle()
+    chk = oracle_lib.reference() if j["kind"] == "reference" else oracle_lib.oracle()
+    pairs, xfs = j["pairs"][chunk], j["xfs"][chunk]
     t = time.perf_counter()
-    m = chk.collide(pool, shapes, pairs, xfs)
+    m = chk.collide(j["pool"], j["shapes"], pairs, xfs)
     return time.perf_counter() - t, int((m["pointCount"] > 0).sum())
 
 
@@ -54,17 +58,17 @@ def cpu_baseline(pool, pairs, xfs, sample):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     kind = "reference" if oracle_lib.reference() is not None else "port"
-    cores = max(1, min(16, os.cpu_count() or 1))
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
     stride = max(1, len(pairs) // sample)
     idx = np.arange(0, len(pairs), stride)[:sample]
     chunks = np.array_split(idx, cores)
-    shapes = pool.shapes()
-    jobs = [(kind, pool, shapes, pairs[c], xfs[c]) for c in chunks]
+    _CPU_JOB.update(kind=kind, pool=pool, shapes=pool.shapes(), pairs=pairs, xfs=xfs)
     ctx = mp.get_context("fork")
-    t0 = time.perf_counter()
     with ctx.Pool(cores) as p:
-        res = p.map(_cpu_worker, jobs)
-    wall = time.perf_counter() - t0
+        p.map(_cpu_worker, [c[:64] for c in chunks])  # load the library in every worker
+        t0 = time.perf_counter()
+        res = p.map(_cpu_worker, chunks, chunksize=1)
+        wall = time.perf_counter() - t0
     cpu_s = sum(r[0] for r in res)
     return {
         "value": len(idx) / wall,
@@ -118,8 +122,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--pairs", type=int, default=1 << 20)
     ap.add_argument("--pool", type=int, default=65536)
-    ap.add_argument("--cpu-sample", type=int, default=131072)
+    ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="also run one device-to-device copy of BYTES (reference for HBM counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: the narrowphase has no cross-GPU exchange per step "
@@ -148,6 +154,11 @@ def main():
         dp, dx, do = ctx.alloc(pairs.nbytes), ctx.alloc(xfs.nbytes), ctx.alloc(n * MANIFOLD_DTYPE.itemsize)
         dp.upload(pairs)
         dx.upload(xfs)
+        if args.copy_calibration:
+            a, b = ctx.alloc(args.copy_calibration), ctx.alloc(args.copy_calibration)
+            ctx.copy_on_device(b, a, args.copy_calibration)
+            a.free()
+            b.free()
         for _ in range(args.warmup):
             ctx.collide_device(n, dp, dx, do)
         ctx.synchronize()

@@ -51,6 +51,7 @@ def load_library():
         "b3amd_device_free": (i32, [vp, vp]),
         "b3amd_copy_to_device": (i32, [vp, vp, vp, u64]),
         "b3amd_copy_to_host": (i32, [vp, vp, vp, u64]),
+        "b3amd_copy_on_device": (i32, [vp, vp, vp, u64]),
         "b3amd_set_profiling": (i32, [vp, i32]),
         "b3amd_kernel_stats": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]),
         "b3amd_reset_kernel_stats": (i32, [vp]),
@@ -142,6 +143,9 @@ class Context:
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)
+
+    def copy_on_device(self, dst, src, nbytes):
+        _check(self.lib.b3amd_copy_on_device(self.handle, dst.ptr, src.ptr, int(nbytes)))
 
     def collide_device(self, n, d_pairs, d_xfs, d_out, d_sweeps=None, d_caches=None):
         """Enqueue a batch on device-resident buffers (no synchronisation)."""

@@ -461,6 +461,13 @@ int b3amd_copy_to_host(b3amd_context* ctx, void* dst, const void* src, uint64_t 
     return B3AMD_OK;
 }
 
+int b3amd_copy_on_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    if (!ctx || !dst || !src) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    HIP_TRY(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return B3AMD_OK;
+}
+
 int b3amd_set_profiling(b3amd_context* ctx, int enabled) {
     if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
     int rc = ctx->resolve_timings();

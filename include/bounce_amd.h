@@ -149,6 +149,9 @@ int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);
 int b3amd_device_free(b3amd_context* ctx, void* ptr);
 int b3amd_copy_to_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
 int b3amd_copy_to_host(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
+/* Device-to-device copy on the context stream (synchronous).  Also the plain copy of
+ * known size that HBM traffic counters are calibrated against. */
+int b3amd_copy_on_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
 
 /* ---- kernel timing (HIP events on the context stream) ----
  * While enabled, every kernel the context launches is bracketed by events; stats
