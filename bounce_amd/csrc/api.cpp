@@ -67,6 +67,7 @@ struct b3amd_context {
     // Shape table.
     ShapeRec* dShapes = nullptr;
     uint32_t shapeCount = 0;
+    bool roundShapes = false;  // the table holds spheres or capsules
     std::vector<ShapeRec> shapes;
 
     // Per-batch scratch and host-API staging.
@@ -243,8 +244,6 @@ int validate_pairs(const b3amd_context* ctx, uint32_t n, const uint32_t* pairs) 
         const ShapeRec& A = ctx->shapes[a];
         const ShapeRec& B = ctx->shapes[b];
         if (A.type > B.type) return fail(B3AMD_ERR_ARGUMENT, "pair %u: typeA > typeB (contact_manager.cpp:238-242)", i);
-        if (A.type != B3AMD_HULL || B.type != B3AMD_HULL)
-            return fail(B3AMD_ERR_UNSUPPORTED, "pair %u: shape types %u/%u not available in this build", i, A.type, B.type);
     }
     return B3AMD_OK;
 }
@@ -283,6 +282,13 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     hipStream_t s = ctx->stream;
     int cls = ctx->sizeClass;
     if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
+    if (ctx->roundShapes) {
+        ConvexArgs cv;
+        static_cast<PairInputs&>(cv) = in;
+        cv.stage = ctx->dStage;
+        cv.out = out;
+        if ((rc = timed(ctx, "convex", [&] { return launch_convex(cv, s); }))) return rc;
+    }
     if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_hull(sa, cls, s); }))) return rc;
     if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
     return B3AMD_OK;
@@ -379,6 +385,8 @@ int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* s
     if (count) HIP_TRY(hipMemcpy(ctx->dShapes, shapes, sizeof(ShapeRec) * (uint64_t)count, hipMemcpyHostToDevice));
     ctx->shapes.assign(reinterpret_cast<const ShapeRec*>(shapes), reinterpret_cast<const ShapeRec*>(shapes) + count);
     ctx->shapeCount = count;
+    ctx->roundShapes = false;
+    for (uint32_t i = 0; i < count; ++i) ctx->roundShapes |= shapes[i].type != B3AMD_HULL;
     return B3AMD_OK;
 }
 
@@ -424,8 +432,6 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
                          const b3amd_quat* d_sweeps, b3amd_convex_cache* d_caches, b3amd_manifold* d_out) {
     if (!ctx || (n && (!d_pairs || !d_xfs || !d_out))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
-    for (const ShapeRec& s : ctx->shapes)
-        if (s.type != B3AMD_HULL) return fail(B3AMD_ERR_UNSUPPORTED, "device batches need a hull-only shape table in this build");
     return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
                      reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
 }

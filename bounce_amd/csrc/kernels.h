@@ -63,18 +63,22 @@ struct PairInputs {
     uint32_t shapeCount;
     uint32_t hullCount;
 
-    // True when both shape indices and their hull indices are in range and both shapes
-    // are hulls; kernels skip (and flag) any other pair instead of reading out of bounds.
-    __device__ bool valid_hull_pair(uint32_t p) const {
+    // 0: a pair kernels must not read (shape or hull index out of range, a type outside
+    // sphere/capsule/hull, or typeA > typeB, which b3ContactManager::Create never makes,
+    // contact_manager.cpp:238-242); 1: a hull-hull pair; 2: any other convex pair.
+    __device__ uint32_t classify(uint32_t p) const {
         uint32_t a = pairs[2 * p], b = pairs[2 * p + 1];
-        if (a >= shapeCount || b >= shapeCount) return false;
+        if (a >= shapeCount || b >= shapeCount) return 0u;
         const ShapeRec& A = shapes[a];
         const ShapeRec& B = shapes[b];
-        return A.type == 2u && B.type == 2u && A.hull < hullCount && B.hull < hullCount;
+        if (A.type > 2u || B.type > 2u || A.type > B.type) return 0u;
+        if ((A.type == 2u && A.hull >= hullCount) || (B.type == 2u && B.hull >= hullCount)) return 0u;
+        return (A.type == 2u && B.type == 2u) ? 1u : 2u;
     }
 };
 
 constexpr uint32_t kFlagInvalidPair = 2u;
+constexpr uint32_t kStageConvex = 5u;  // sphere/capsule pair: handled by the convex kernel
 
 struct CacheArgs : PairInputs {
     const float* sweeps;     // [n][4][4] or null
@@ -94,6 +98,12 @@ struct ContactArgs : PairInputs {
     Manifold* out;           // [n]
 };
 
+struct ConvexArgs : PairInputs {
+    const uint32_t* stage;   // [n] from the cache kernel
+    Manifold* out;           // [n]
+};
+
+hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // sizeClass 0: every hull of the batch has V <= 64, F <= 128, HE/2 <= 192; 1: the limits.
 hipError_t launch_sat_hull(const SatArgs& a, int sizeClass, hipStream_t s);

@@ -690,6 +690,766 @@ static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1,
     collide_cache(m, xf1, h1, r1, xf2, h2, r2, c);
 }
 
+/* ---------------------------------------------------------------- gjk.cpp (distance) */
+typedef struct { const V3* v; uint32_t count; } Proxy;  /* b3GJKProxy vertices (gjk_proxy.h:26-44) */
+typedef struct { V3 point1, point2, point; float weight; uint32_t index1, index2; } SVert;
+typedef struct { SVert v[4]; uint32_t count; } Simplex;
+typedef struct { V3 point1, point2; float distance; uint32_t iterations; } GJKOut;
+
+static uint32_t proxy_support(const Proxy* p, V3 d) {                                  /* gjk_proxy.h:57-71 */
+    uint32_t maxIndex = 0;
+    float maxProjection = dot(d, p->v[0]);
+    for (uint32_t i = 1; i < p->count; ++i) {
+        float q = dot(d, p->v[i]);
+        if (q > maxProjection) { maxIndex = i; maxProjection = q; }
+    }
+    return maxIndex;
+}
+
+static void bary2(float out[3], V3 A, V3 B, V3 Q) {                                    /* gjk.cpp:32-45 */
+    V3 AB = sub(B, A), QA = sub(A, Q), QB = sub(B, Q);
+    out[0] = dot(QB, AB);
+    out[1] = -dot(QA, AB);
+    FL(1);
+    out[2] = out[0] + out[1];
+}
+
+static void bary3(float out[4], V3 A, V3 B, V3 C, V3 Q) {                              /* gjk.cpp:47-72 */
+    V3 AB = sub(B, A), AC = sub(C, A);
+    V3 QA = sub(A, Q), QB = sub(B, Q), QC = sub(C, Q);
+    V3 QB_x_QC = cross(QB, QC), QC_x_QA = cross(QC, QA), QA_x_QB = cross(QA, QB);
+    V3 AB_x_AC = cross(AB, AC);
+    out[0] = dot(QB_x_QC, AB_x_AC);
+    out[1] = dot(QC_x_QA, AB_x_AC);
+    out[2] = dot(QA_x_QB, AB_x_AC);
+    FL(2);
+    out[3] = out[0] + out[1] + out[2];
+}
+
+static float det3(V3 a, V3 b, V3 c) { return dot(a, cross(b, c)); }                   /* vec3.h:170-173 */
+static float signf_(float x) { return (float)((0.0f < x) - (x < 0.0f)); }             /* math.h:88-92 */
+
+static void bary4(float out[5], V3 A, V3 B, V3 C, V3 D, V3 Q) {                        /* gjk.cpp:74-99 */
+    V3 AB = sub(B, A), AC = sub(C, A), AD = sub(D, A);
+    V3 QA = sub(A, Q), QB = sub(B, Q), QC = sub(C, Q), QD = sub(D, Q);
+    float divisor = det3(AB, AC, AD);
+    float sign = signf_(divisor);
+    FL(5);
+    out[0] = sign * det3(QB, QC, QD);
+    out[1] = sign * det3(QA, QD, QC);
+    out[2] = sign * det3(QA, QB, QD);
+    out[3] = sign * det3(QA, QC, QB);
+    out[4] = sign * divisor;
+}
+
+static V3 search_direction(const Simplex* s, V3 Q) {                                   /* gjk.cpp:101-146 */
+    switch (s->count) {
+    case 1: return sub(Q, s->v[0].point);
+    case 2: {
+        V3 A = s->v[0].point, B = s->v[1].point;
+        V3 AB = sub(B, A), AQ = sub(Q, A);
+        V3 AB_x_AQ = cross(AB, AQ);
+        return cross(AB_x_AQ, AB);
+    }
+    case 3: {
+        V3 A = s->v[0].point, B = s->v[1].point, C = s->v[2].point;
+        V3 AB = sub(B, A), AC = sub(C, A), AQ = sub(Q, A);
+        V3 AB_x_AC = cross(AB, AC);
+        float sign = dot(AB_x_AC, AQ);
+        return sign > 0.0f ? AB_x_AC : neg(AB_x_AC);
+    }
+    default: return v3(0.0f, 0.0f, 0.0f);
+    }
+}
+
+static V3 closest_point(const Simplex* s) {                                            /* gjk.cpp:148-166 */
+    switch (s->count) {
+    case 1: return s->v[0].point;
+    case 2: return add(scale(s->v[0].weight, s->v[0].point), scale(s->v[1].weight, s->v[1].point));
+    case 3: return add(add(scale(s->v[0].weight, s->v[0].point), scale(s->v[1].weight, s->v[1].point)),
+                       scale(s->v[2].weight, s->v[2].point));
+    default: return v3(0.0f, 0.0f, 0.0f);
+    }
+}
+
+static void closest_points(const Simplex* s, V3* p1, V3* p2) {                         /* gjk.cpp:168-198 */
+    const SVert* v = s->v;
+    switch (s->count) {
+    case 1: *p1 = v[0].point1; *p2 = v[0].point2; break;
+    case 2:
+        *p1 = add(scale(v[0].weight, v[0].point1), scale(v[1].weight, v[1].point1));
+        *p2 = add(scale(v[0].weight, v[0].point2), scale(v[1].weight, v[1].point2));
+        break;
+    case 3:
+        *p1 = add(add(scale(v[0].weight, v[0].point1), scale(v[1].weight, v[1].point1)), scale(v[2].weight, v[2].point1));
+        *p2 = add(add(scale(v[0].weight, v[0].point2), scale(v[1].weight, v[1].point2)), scale(v[2].weight, v[2].point2));
+        break;
+    case 4:
+        *p1 = add(add(add(scale(v[0].weight, v[0].point1), scale(v[1].weight, v[1].point1)),
+                      scale(v[2].weight, v[2].point1)), scale(v[3].weight, v[3].point1));
+        *p2 = *p1;
+        break;
+    default: break;
+    }
+}
+
+static void set1(Simplex* s, SVert a) { s->count = 1; s->v[0] = a; s->v[0].weight = 1.0f; }
+static void set2(Simplex* s, SVert a, SVert b, const float w[3]) {
+    FL(3);
+    float d = 1.0f / w[2];
+    s->count = 2;
+    s->v[0] = a; s->v[0].weight = d * w[0];
+    s->v[1] = b; s->v[1].weight = d * w[1];
+}
+static void set3(Simplex* s, SVert a, SVert b, SVert c, float w0, float w1, float w2, float divisor) {
+    FL(4);
+    float d = 1.0f / divisor;
+    s->count = 3;
+    s->v[0] = a; s->v[0].weight = d * w0;
+    s->v[1] = b; s->v[1].weight = d * w1;
+    s->v[2] = c; s->v[2].weight = d * w2;
+}
+
+static void solve2(Simplex* s, V3 Q) {                                                 /* gjk.cpp:201-234 */
+    SVert A = s->v[0], B = s->v[1];
+    float wAB[3];
+    bary2(wAB, A.point, B.point, Q);
+    if (wAB[1] <= 0.0f) { set1(s, A); return; }
+    if (wAB[0] <= 0.0f) { set1(s, B); return; }
+    set2(s, A, B, wAB);
+}
+
+static void solve3(Simplex* s, V3 Q) {                                                 /* gjk.cpp:236-332 */
+    SVert A = s->v[0], B = s->v[1], C = s->v[2];
+    float wAB[3], wBC[3], wCA[3];
+    bary2(wAB, A.point, B.point, Q);
+    bary2(wBC, B.point, C.point, Q);
+    bary2(wCA, C.point, A.point, Q);
+    if (wAB[1] <= 0.0f && wCA[0] <= 0.0f) { set1(s, A); return; }
+    if (wAB[0] <= 0.0f && wBC[1] <= 0.0f) { set1(s, B); return; }
+    if (wBC[0] <= 0.0f && wCA[1] <= 0.0f) { set1(s, C); return; }
+    float wABC[4];
+    bary3(wABC, A.point, B.point, C.point, Q);
+    float area = wABC[3];
+    FL(1);
+    if (wAB[0] > 0.0f && wAB[1] > 0.0f && area * wABC[2] <= 0.0f) { set2(s, A, B, wAB); return; }
+    FL(1);
+    if (wBC[0] > 0.0f && wBC[1] > 0.0f && area * wABC[0] <= 0.0f) { set2(s, B, C, wBC); return; }
+    FL(1);
+    if (wCA[0] > 0.0f && wCA[1] > 0.0f && area * wABC[1] <= 0.0f) { set2(s, C, A, wCA); return; }
+    float divisor = wABC[3];
+    if (divisor <= 0.0f) return;  /* give up */
+    set3(s, A, B, C, wABC[0], wABC[1], wABC[2], divisor);
+}
+
+static void solve4(Simplex* s, V3 Q) {                                                 /* gjk.cpp:334-575 */
+    SVert A = s->v[0], B = s->v[1], C = s->v[2], D = s->v[3];
+    float wAB[3], wAC[3], wAD[3], wBC[3], wCD[3], wDB[3];
+    bary2(wAB, A.point, B.point, Q);
+    bary2(wBC, B.point, C.point, Q);
+    bary2(wAC, A.point, C.point, Q);
+    bary2(wAD, A.point, D.point, Q);
+    bary2(wCD, C.point, D.point, Q);
+    bary2(wDB, D.point, B.point, Q);
+    if (wAB[1] <= 0.0f && wAC[1] <= 0.0f && wAD[1] <= 0.0f) { set1(s, A); return; }
+    if (wAB[0] <= 0.0f && wDB[0] <= 0.0f && wBC[1] <= 0.0f) { set1(s, B); return; }
+    if (wAC[0] <= 0.0f && wBC[0] <= 0.0f && wCD[1] <= 0.0f) { set1(s, C); return; }
+    if (wAD[0] <= 0.0f && wCD[0] <= 0.0f && wDB[1] <= 0.0f) { set1(s, D); return; }
+    float wACB[4], wABD[4], wADC[4], wBCD[4];
+    bary3(wACB, A.point, C.point, B.point, Q);
+    bary3(wABD, A.point, B.point, D.point, Q);
+    bary3(wADC, A.point, D.point, C.point, Q);
+    bary3(wBCD, B.point, C.point, D.point, Q);
+    if (wABD[2] <= 0.0f && wACB[1] <= 0.0f && wAB[0] > 0.0f && wAB[1] > 0.0f) { set2(s, A, B, wAB); return; }
+    if (wACB[2] <= 0.0f && wADC[1] <= 0.0f && wAC[0] > 0.0f && wAC[1] > 0.0f) { set2(s, A, C, wAC); return; }
+    if (wADC[2] <= 0.0f && wABD[1] <= 0.0f && wAD[0] > 0.0f && wAD[1] > 0.0f) { set2(s, A, D, wAD); return; }
+    if (wACB[0] <= 0.0f && wBCD[2] <= 0.0f && wBC[0] > 0.0f && wBC[1] > 0.0f) { set2(s, B, C, wBC); return; }
+    if (wADC[0] <= 0.0f && wBCD[0] <= 0.0f && wCD[0] > 0.0f && wCD[1] > 0.0f) { set2(s, C, D, wCD); return; }
+    if (wABD[0] <= 0.0f && wBCD[1] <= 0.0f && wDB[0] > 0.0f && wDB[1] > 0.0f) { set2(s, D, B, wDB); return; }
+    float wABCD[5];
+    bary4(wABCD, A.point, B.point, C.point, D.point, Q);
+    if (wABCD[3] <= 0.0f && wACB[0] > 0.0f && wACB[1] > 0.0f && wACB[2] > 0.0f) {
+        FL(2);
+        set3(s, A, C, B, wACB[0], wACB[1], wACB[2], wACB[0] + wACB[1] + wACB[2]);
+        return;
+    }
+    if (wABCD[2] <= 0.0f && wABD[0] > 0.0f && wABD[1] > 0.0f && wABD[2] > 0.0f) {
+        FL(2);
+        set3(s, A, B, D, wABD[0], wABD[1], wABD[2], wABD[0] + wABD[1] + wABD[2]);
+        return;
+    }
+    if (wABCD[1] <= 0.0f && wADC[0] > 0.0f && wADC[1] > 0.0f && wADC[2] > 0.0f) {
+        FL(2);
+        set3(s, A, D, C, wADC[0], wADC[1], wADC[2], wADC[0] + wADC[1] + wADC[2]);
+        return;
+    }
+    if (wABCD[0] <= 0.0f && wBCD[0] > 0.0f && wBCD[1] > 0.0f && wBCD[2] > 0.0f) {
+        FL(2);
+        set3(s, B, C, D, wBCD[0], wBCD[1], wBCD[2], wBCD[0] + wBCD[1] + wBCD[2]);
+        return;
+    }
+    FL(3);
+    float divisor = wABCD[0] + wABCD[1] + wABCD[2] + wABCD[3];
+    if (divisor <= 0.0f) return;  /* give up */
+    FL(5);
+    float d = 1.0f / divisor;
+    s->count = 4;
+    s->v[0].weight = d * wABCD[0];
+    s->v[1].weight = d * wABCD[1];
+    s->v[2].weight = d * wABCD[2];
+    s->v[3].weight = d * wABCD[3];
+}
+
+/* b3GJK with an empty simplex cache (gjk.cpp:579-737, 743-804, 845-851).  A vertex added
+ * to the simplex starts with weight 0; the reference leaves it unset, which is only
+ * observable after the degenerate give-up returns of Solve3/Solve4. */
+static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2) {
+    Simplex s;
+    memset(&s, 0, sizeof(s));
+    s.v[0].point1 = xf_mul(xf1, p1->v[0]);
+    s.v[0].point2 = xf_mul(xf2, p2->v[0]);
+    s.v[0].point = sub(s.v[0].point2, s.v[0].point1);
+    s.v[0].weight = 1.0f;
+    s.count = 1;
+    uint32_t save1[4], save2[4], saveCount;
+    const V3 origin = v3(0.0f, 0.0f, 0.0f);
+    uint32_t iter = 0;
+    while (iter < 20) {
+        saveCount = s.count;
+        for (uint32_t i = 0; i < saveCount; ++i) { save1[i] = s.v[i].index1; save2[i] = s.v[i].index2; }
+        switch (s.count) {
+        case 2: solve2(&s, origin); break;
+        case 3: solve3(&s, origin); break;
+        case 4: solve4(&s, origin); break;
+        default: break;
+        }
+        if (s.count == 4) break;
+        V3 p = closest_point(&s);
+        (void)dot(p, p);  /* distSq2: the "getting closer" exit is disabled (gjk.cpp:648-651) */
+        V3 d = search_direction(&s, origin);
+        if (dot(d, d) < EPSILON * EPSILON) break;
+        SVert* vx = s.v + s.count;
+        vx->index1 = proxy_support(p1, mulT_mv(rot(xf1), neg(d)));
+        vx->point1 = xf_mul(xf1, p1->v[vx->index1]);
+        vx->index2 = proxy_support(p2, mulT_mv(rot(xf2), d));
+        vx->point2 = xf_mul(xf2, p2->v[vx->index2]);
+        vx->point = sub(vx->point2, vx->point1);
+        vx->weight = 0.0f;
+        ++iter;
+        int dup = 0;
+        for (uint32_t i = 0; i < saveCount; ++i)
+            if (vx->index1 == save1[i] && vx->index2 == save2[i]) { dup = 1; break; }
+        if (dup) break;
+        ++s.count;
+    }
+    GJKOut o;
+    closest_points(&s, &o.point1, &o.point2);
+    o.distance = length(sub(o.point1, o.point2));
+    o.iterations = iter;
+    return o;
+}
+
+/* ---------------------------------------------------------------- convex shape records */
+typedef struct { V3 c[2]; uint32_t count; float radius; } Round;  /* sphere: 1 centre, capsule: 2 */
+
+static Round round_of(const b3o_shape* s) {
+    Round r;
+    r.c[0] = v3(s->v[0], s->v[1], s->v[2]);
+    r.c[1] = v3(s->v[3], s->v[4], s->v[5]);
+    r.count = s->type == B3O_SPHERE ? 1u : 2u;
+    r.radius = s->radius;
+    return r;
+}
+
+static void set_zero_key(b3o_mpoint* mp) {
+    mp->triangleKey = NULL_TRIANGLE;
+    mp->key[0] = mp->key[1] = mp->key[2] = mp->key[3] = 0;
+}
+
+/* collide_spheres.cpp:24-56 */
+static void collide_spheres(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, const Round* s2) {
+    V3 c1 = xf_mul(xf1, s1->c[0]);
+    float r1 = s1->radius;
+    V3 c2 = xf_mul(xf2, s2->c[0]);
+    float r2 = s2->radius;
+    V3 d = sub(c2, c1);
+    float dd = dot(d, d);
+    FL(2);
+    float totalRadius = r1 + r2;
+    if (dd > totalRadius * totalRadius) return;
+    float distance = length(d);
+    V3 normal = v3(0.0f, 1.0f, 0.0f);
+    if (distance > EPSILON) { FL(3); normal = v3(d.x / distance, d.y / distance, d.z / distance); }
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), normal);
+    m->points[0].localPoint1 = s1->c[0];
+    m->points[0].localPoint2 = s2->c[0];
+    set_zero_key(&m->points[0]);
+}
+
+/* collide_sphere_capsule.cpp:24-133 */
+static void collide_sphere_capsule(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, const Round* s2) {
+    V3 Q = xf_mul(xf1, s1->c[0]);
+    V3 A = xf_mul(xf2, s2->c[0]);
+    V3 B = xf_mul(xf2, s2->c[1]);
+    V3 AB = sub(B, A);
+    float u = dot(sub(B, Q), AB);
+    float v = dot(sub(Q, A), AB);
+    FL(1);
+    float radius = s1->radius + s2->radius;
+    if (v <= 0.0f || u <= 0.0f) {
+        V3 P = v <= 0.0f ? A : B;
+        V3 d = sub(P, Q);
+        float dd = dot(d, d);
+        FL(1);
+        if (dd > radius * radius) return;
+        V3 n = v3(0.0f, 1.0f, 0.0f);
+        float len = length(d);
+        if (len > EPSILON) { FL(3); n = v3(d.x / len, d.y / len, d.z / len); }
+        m->pointCount = 1;
+        m->points[0].localNormal1 = mulT_mv(rot(xf1), n);
+        m->points[0].localPoint1 = s1->c[0];
+        m->points[0].localPoint2 = v <= 0.0f ? s2->c[0] : s2->c[1];
+        set_zero_key(&m->points[0]);
+        return;
+    }
+    float s = dot(AB, AB);
+    V3 P;
+    FL(1);
+    if (s < LINEAR_SLOP * LINEAR_SLOP) {
+        P = A;
+    } else {
+        V3 w = add(scale(u, A), scale(v, B));
+        FL(3);
+        P = v3(w.x / s, w.y / s, w.z / s);
+    }
+    V3 d = sub(P, Q);
+    float dd = dot(d, d);
+    FL(1);
+    if (dd > radius * radius) return;
+    V3 QA = sub(A, Q);
+    V3 e = cross(AB, QA);
+    V3 n = cross(AB, e);
+    if (dot(n, QA) < 0.0f) n = neg(n);
+    normalize_in_place(&n);
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), n);
+    m->points[0].localPoint1 = s1->c[0];
+    m->points[0].localPoint2 = xf_mulT(xf2, P);
+    set_zero_key(&m->points[0]);
+}
+
+/* collide_capsules.cpp:27-57 */
+static V3 closest_on_segment(V3 Q, V3 A, V3 B) {
+    V3 AB = sub(B, A);
+    float u = dot(sub(B, Q), AB);
+    float v = dot(sub(Q, A), AB);
+    if (v <= 0.0f) return A;
+    if (u <= 0.0f) return B;
+    float w = dot(AB, AB);
+    FL(1);
+    if (w <= LINEAR_SLOP * LINEAR_SLOP) return A;
+    FL(1);
+    float den = 1.0f / w;
+    return scale(den, add(scale(u, A), scale(v, B)));
+}
+
+/* collide_capsules.cpp:59-152 */
+static void closest_points_segments(V3* C1, V3* C2, V3 P1, V3 Q1, V3 P2, V3 Q2) {
+    V3 E1 = sub(Q1, P1);
+    float L1 = length(E1);
+    V3 E2 = sub(Q2, P2);
+    float L2 = length(E2);
+    if (L1 < LINEAR_SLOP && L2 < LINEAR_SLOP) { *C1 = P1; *C2 = P2; return; }
+    if (L1 < LINEAR_SLOP) { *C1 = P1; *C2 = closest_on_segment(P1, P2, Q2); return; }
+    if (L2 < LINEAR_SLOP) { *C1 = closest_on_segment(P2, P1, Q1); *C2 = P2; return; }
+    FL(6);
+    V3 N1 = v3(E1.x / L1, E1.y / L1, E1.z / L1);
+    V3 N2 = v3(E2.x / L2, E2.y / L2, E2.z / L2);
+    float a11 = 1.0f, a12 = -dot(N1, N2);
+    float a21 = -a12, a22 = -1.0f;
+    /* b3Mat22 A: columns x = (a11, a21), y = (a12, a22) */
+    float Axx = a11, Axy = a21, Ayx = a12, Ayy = a22;
+    FL(6);
+    float n0 = fabsf(Axx) + fabsf(Axy), n1 = fabsf(Ayx) + fabsf(Ayy);
+    float norm_A = n0 > n1 ? n0 : n1;
+    float det_A = 1.0f / (a11 * a22 - a12 * a21);
+    FL(4);
+    float Ixx = det_A * a22, Ixy = -det_A * a21;
+    float Iyx = -det_A * a12, Iyy = det_A * a11;
+    FL(3);
+    float m0 = fabsf(Ixx) + fabsf(Ixy), m1 = fabsf(Iyx) + fabsf(Iyy);
+    float norm_inv_A = m0 > m1 ? m0 : m1;
+    float k_A = norm_A * norm_inv_A;
+    const float kMaxConditionNumber = 1000.0f;
+    if (!isinf(k_A) && k_A < kMaxConditionNumber) {
+        V3 E3 = sub(P1, P2);
+        float bx = -dot(N1, E3);
+        float by = -dot(N2, E3);
+        FL(6);
+        float xx = bx * Ixx + by * Iyx;  /* b3Mul(Mat22, Vec2) = v.x * A.x + v.y * A.y (mat22.h:56-59) */
+        float xy = bx * Ixy + by * Iyy;
+        *C1 = add(P1, scale(xx, N1));
+        *C2 = add(P2, scale(xy, N2));
+    } else {
+        *C1 = P1;
+        *C2 = P2;
+    }
+    *C1 = closest_on_segment(*C1, P1, Q1);
+    *C2 = closest_on_segment(*C1, P2, Q2);
+    *C1 = closest_on_segment(*C2, P1, Q1);
+}
+
+/* collide_capsules.cpp:154-174 */
+static int are_parallel(V3 P1, V3 Q1, V3 P2, V3 Q2) {
+    V3 E1 = sub(Q1, P1);
+    float L1 = length(E1);
+    if (L1 < LINEAR_SLOP) return 0;
+    V3 E2 = sub(Q2, P2);
+    float L2 = length(E2);
+    if (L2 < LINEAR_SLOP) return 0;
+    const float kTol = 0.005f;
+    V3 N = cross(E1, E2);
+    FL(2);
+    return length(N) < kTol * L1 * L2;
+}
+
+/* clip.cpp:59-97 (b3ClipEdgeToPlane) */
+static uint32_t clip_edge_to_plane(ClipVertex out[2], const ClipVertex in[2], ClipPlane plane) {
+    uint32_t n = 0;
+    float d1 = plane_dist(in[0].position, plane.plane);
+    float d2 = plane_dist(in[1].position, plane.plane);
+    if (d1 <= 0.0f) out[n++] = in[0];
+    if (d2 <= 0.0f) out[n++] = in[1];
+    if (d1 <= 0.0f && d2 > 0.0f) {
+        FL(2);
+        float f = d1 / (d1 - d2);
+        out[n].position = add(in[0].position, scale(f, sub(in[1].position, in[0].position)));
+        out[n].pair = make_pair(in[0].pair.inEdge1, NULL_EDGE, NULL_EDGE, plane.id);
+        ++n;
+    } else if (d1 > 0.0f && d2 <= 0.0f) {
+        FL(2);
+        float f = d1 / (d1 - d2);
+        out[n].position = add(in[0].position, scale(f, sub(in[1].position, in[0].position)));
+        out[n].pair = make_pair(in[1].pair.inEdge1, plane.id, NULL_EDGE, NULL_EDGE);
+        ++n;
+    }
+    return n;
+}
+
+/* clip.cpp:22-31 (b3BuildEdge) */
+static void build_edge(ClipVertex e[2], V3 A, V3 B) {
+    e[0].position = A;
+    e[0].pair = make_pair(0, NULL_EDGE, 0, NULL_EDGE);
+    e[1].position = B;
+    e[1].pair = make_pair(1, NULL_EDGE, 0, NULL_EDGE);
+}
+
+/* clip.cpp:152-198 (b3ClipEdgeToFace against a capsule segment) */
+static uint32_t clip_edge_to_segment(ClipVertex out[2], const ClipVertex in[2], V3 P1, V3 Q1) {
+    ClipVertex cur[2] = {in[0], in[1]};
+    V3 E1 = sub(Q1, P1);
+    ClipPlane cp1;
+    cp1.plane.n = normalize(E1);
+    cp1.plane.d = dot(cp1.plane.n, Q1);
+    cp1.id = 0;
+    ClipVertex tmp[2];
+    memset(tmp, 0, sizeof(tmp));
+    uint32_t n = clip_edge_to_plane(tmp, cur, cp1);
+    cur[0] = tmp[0]; cur[1] = tmp[1];
+    if (n < 2) { out[0] = cur[0]; out[1] = cur[1]; return n; }
+    ClipPlane cp2;
+    cp2.plane.n = neg(cp1.plane.n);
+    cp2.plane.d = dot(cp2.plane.n, P1);
+    cp2.id = 1;
+    ClipVertex tmp2[2];
+    memset(tmp2, 0, sizeof(tmp2));
+    n = clip_edge_to_plane(tmp2, cur, cp2);
+    out[0] = tmp2[0]; out[1] = tmp2[1];
+    return n;
+}
+
+/* clip.cpp:201-240 (b3ClipEdgeToFace against hull face side planes) */
+static uint32_t clip_edge_to_hull_face(ClipVertex out[2], const ClipVertex in[2], const XF* xf, float r,
+                                       uint32_t index, const Hull* h) {
+    ClipVertex cur[2] = {in[0], in[1]};
+    uint32_t n = 0;
+    uint32_t begin = h->faces[index];
+    uint32_t e = begin;
+    do {
+        const b3o_halfedge* edge = h->edges + e;
+        const b3o_halfedge* twin = h->edges + edge->twin;
+        uint32_t edgeId = twin->twin;
+        PL plane = edge_side_plane(h, edgeId);
+        FL(1);
+        plane.d += r;
+        ClipPlane cp;
+        cp.id = edgeId;
+        cp.plane = plane_mul(xf, plane);
+        ClipVertex tmp[2];
+        memset(tmp, 0, sizeof(tmp));
+        n = clip_edge_to_plane(tmp, cur, cp);
+        cur[0] = tmp[0]; cur[1] = tmp[1];
+        if (n == 0) break;
+        e = edge->next;
+    } while (e != begin);
+    out[0] = cur[0]; out[1] = cur[1];
+    return n;
+}
+
+static void set_pair_key(b3o_mpoint* mp, FPair p) { set_key(mp, p); }
+
+/* collide_capsules.cpp:176-259 */
+static void collide_capsules(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, const Round* s2) {
+    V3 A1 = xf_mul(xf1, s1->c[0]), B1 = xf_mul(xf1, s1->c[1]);
+    V3 A2 = xf_mul(xf2, s2->c[0]), B2 = xf_mul(xf2, s2->c[1]);
+    V3 point1, point2;
+    closest_points_segments(&point1, &point2, A1, B1, A2, B2);
+    float distance = length(sub(point1, point2));
+    float r1 = s1->radius, r2 = s2->radius;
+    FL(1);
+    float totalRadius = r1 + r2;
+    if (distance > totalRadius) return;
+    if (!(distance > EPSILON)) return;
+    if (are_parallel(A1, B1, A2, B2)) {
+        ClipVertex edge1[2], clip1[2];
+        build_edge(edge1, A1, B1);
+        uint32_t clipCount = clip_edge_to_segment(clip1, edge1, A2, B2);
+        if (clipCount == 2) {
+            V3 cp1 = closest_on_segment(clip1[0].position, A2, B2);
+            V3 cp2 = closest_on_segment(clip1[1].position, A2, B2);
+            float d1 = length(sub(clip1[0].position, cp1));
+            float d2 = length(sub(clip1[1].position, cp2));
+            if (d1 > EPSILON && d1 <= totalRadius && d2 > EPSILON && d2 <= totalRadius) {
+                V3 g1 = sub(cp1, clip1[0].position), g2 = sub(cp2, clip1[1].position);
+                FL(6);
+                V3 n1 = v3(g1.x / d1, g1.y / d1, g1.z / d1);
+                V3 n2 = v3(g2.x / d2, g2.y / d2, g2.z / d2);
+                m->pointCount = 2;
+                m->points[0].localNormal1 = mulT_mv(rot(xf1), n1);
+                m->points[0].localPoint1 = xf_mulT(xf1, clip1[0].position);
+                m->points[0].localPoint2 = xf_mulT(xf2, cp1);
+                set_pair_key(&m->points[0], clip1[0].pair);
+                m->points[1].localNormal1 = mulT_mv(rot(xf1), n2);
+                m->points[1].localPoint1 = xf_mulT(xf1, clip1[1].position);
+                m->points[1].localPoint2 = xf_mulT(xf2, cp2);
+                set_pair_key(&m->points[1], clip1[1].pair);
+                return;
+            }
+        }
+    }
+    V3 g = sub(point2, point1);
+    FL(3);
+    V3 normal = v3(g.x / distance, g.y / distance, g.z / distance);
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), normal);
+    m->points[0].localPoint1 = xf_mulT(xf1, point1);
+    m->points[0].localPoint2 = xf_mulT(xf2, point2);
+    set_zero_key(&m->points[0]);
+}
+
+/* collide_sphere_hull.cpp:26-87 with b3QueryFaceSeparation(sphere, hull) (sat_vertex_and_hull.cpp:29-54) */
+static void collide_sphere_hull(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, const Hull* h2, float r2) {
+    Proxy p1 = {s1->c, 1}, p2 = {h2->vertices, h2->vertexCount};
+    GJKOut g = gjk(xf1, &p1, xf2, &p2);
+    float r1 = s1->radius;
+    FL(1);
+    float totalRadius = r1 + r2;
+    if (g.distance > totalRadius) return;
+    if (g.distance > 0.0f) {
+        V3 c1 = g.point1, c2 = g.point2;
+        V3 dd = sub(c2, c1);
+        FL(3);
+        V3 normal = v3(dd.x / g.distance, dd.y / g.distance, dd.z / g.distance);
+        m->pointCount = 1;
+        m->points[0].localNormal1 = mulT_mv(rot(xf1), normal);
+        m->points[0].localPoint1 = s1->c[0];
+        m->points[0].localPoint2 = xf_mulT(xf2, c2);
+        set_zero_key(&m->points[0]);
+        return;
+    }
+    V3 support = xf_mulT(xf2, xf_mul(xf1, s1->c[0]));
+    uint32_t maxIndex = 0;
+    float maxSep = -MAX_FLOAT;
+    for (uint32_t i = 0; i < h2->faceCount; ++i) {
+        float sep = plane_dist(support, h2->planes[i]);
+        if (sep > maxSep) { maxIndex = i; maxSep = sep; }
+    }
+    if (maxSep > totalRadius) return;
+    PL plane2 = plane_mul(xf2, h2->planes[maxIndex]);
+    V3 c1 = xf_mul(xf1, s1->c[0]);
+    V3 c2 = closest_on_plane(c1, plane2);
+    V3 n1 = neg(plane2.n);
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), n1);
+    m->points[0].localPoint1 = s1->c[0];
+    m->points[0].localPoint2 = xf_mulT(xf2, c2);
+    set_zero_key(&m->points[0]);
+}
+
+/* collide_capsule_hull.cpp:29-87 */
+static void capsule_edge_contact(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, uint32_t index2, const Hull* h2) {
+    V3 P1 = xf_mul(xf1, s1->c[0]);
+    V3 Q1 = xf_mul(xf1, s1->c[1]);
+    V3 E1 = sub(Q1, P1);
+    V3 N1 = E1;
+    normalize_in_place(&N1);
+    const b3o_halfedge* e2 = h2->edges + index2;
+    const b3o_halfedge* t2 = h2->edges + index2 + 1;
+    V3 C2 = xf_mul(xf2, h2->centroid);
+    V3 P2 = xf_mul(xf2, h2->vertices[e2->origin]);
+    V3 Q2 = xf_mul(xf2, h2->vertices[t2->origin]);
+    V3 E2 = sub(Q2, P2);
+    V3 N2 = E2;
+    normalize_in_place(&N2);
+    float b = dot(N1, N2);
+    FL(2);
+    float den = 1.0f - b * b;
+    if (den == 0.0f) return;
+    FL(1);
+    float inv_den = 1.0f / den;
+    V3 E3 = sub(P1, P2);
+    float d = dot(N1, E3);
+    float e = dot(N2, E3);
+    FL(6);
+    float s = inv_den * (b * e - d);
+    float t = inv_den * (e - b * d);
+    V3 c1 = add(P1, scale(s, N1));
+    V3 c2 = add(P2, scale(t, N2));
+    V3 N = cross(E1, E2);
+    normalize_in_place(&N);
+    if (dot(N, sub(P2, C2)) > 0.0f) N = neg(N);
+    FPair pair = make_pair(0, 1, index2, index2 + 1);
+    m->pointCount = 1;
+    m->points[0].localNormal1 = mulT_mv(rot(xf1), N);
+    m->points[0].localPoint1 = xf_mulT(xf1, c1);
+    m->points[0].localPoint2 = xf_mulT(xf2, c2);
+    set_key(&m->points[0], pair);
+}
+
+/* collide_capsule_hull.cpp:89-140 */
+static void capsule_face_contact(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, uint32_t index2,
+                                 const Hull* h2, float r2) {
+    V3 A = xf_mul(xf1, s1->c[0]), B = xf_mul(xf1, s1->c[1]);
+    float r1 = s1->radius;
+    ClipVertex edge1[2], clip1[2];
+    build_edge(edge1, A, B);
+    uint32_t clipCount = clip_edge_to_hull_face(clip1, edge1, xf2, r2, index2, h2);
+    PL plane2 = plane_mul(xf2, h2->planes[index2]);
+    V3 n1 = neg(plane2.n);
+    FL(1);
+    float totalRadius = r1 + r2;
+    uint32_t pc = 0;
+    for (uint32_t i = 0; i < clipCount; ++i) {
+        V3 c1 = clip1[i].position;
+        float s = plane_dist(c1, plane2);
+        if (s <= totalRadius) {
+            V3 c2 = closest_on_plane(c1, plane2);
+            b3o_mpoint* mp = m->points + pc;
+            mp->localNormal1 = mulT_mv(rot(xf1), n1);
+            mp->localPoint1 = xf_mulT(xf1, c1);
+            mp->localPoint2 = xf_mulT(xf2, c2);
+            set_key(mp, clip1[i].pair);
+            ++pc;
+        }
+    }
+    m->pointCount = pc;
+}
+
+/* collide_capsule_hull.cpp:142-222 with the capsule SAT queries (sat_edge_and_hull.cpp:23-143) */
+static void collide_capsule_hull(b3o_manifold* m, const XF* xf1, const Round* s1, const XF* xf2, const Hull* h2, float r2) {
+    Proxy p1 = {s1->c, 2}, p2 = {h2->vertices, h2->vertexCount};
+    GJKOut g = gjk(xf1, &p1, xf2, &p2);
+    float r1 = s1->radius;
+    FL(1);
+    float totalRadius = r1 + r2;
+    if (g.distance > totalRadius) return;
+    if (g.distance > EPSILON) {
+        V3 dd = sub(g.point2, g.point1);
+        FL(3);
+        V3 N1 = v3(dd.x / g.distance, dd.y / g.distance, dd.z / g.distance);
+        V3 localN1 = mulT_mv(rot(xf2), N1);
+        uint32_t index2 = support_face(h2, neg(localN1));
+        V3 N2 = mul_mv(rot(xf2), h2->planes[index2].n);
+        const float kTol = 0.005f;
+        V3 N = cross(N1, N2);
+        float L = dot(N, N);
+        FL(1);
+        if (L < kTol * kTol) {
+            capsule_face_contact(m, xf1, s1, xf2, index2, h2, r2);
+            if (m->pointCount == 2) return;
+        }
+        m->pointCount = 1;
+        m->points[0].localNormal1 = mulT_mv(rot(xf1), N1);
+        m->points[0].localPoint1 = xf_mulT(xf1, g.point1);
+        m->points[0].localPoint2 = xf_mulT(xf2, g.point2);
+        set_zero_key(&m->points[0]);
+        return;
+    }
+    /* face query: hull 2 planes in the capsule frame (sat_edge_and_hull.cpp:29-52) */
+    XF xf = xf_mulT_xf(xf1, xf2);
+    uint32_t fIndex = 0;
+    float fSep = -MAX_FLOAT;
+    for (uint32_t i = 0; i < h2->faceCount; ++i) {
+        PL plane = plane_mul(&xf, h2->planes[i]);
+        V3 dn = neg(plane.n);
+        /* b3Capsule::GetSupportVertex (capsule.h:50-57) */
+        V3 sv = dot(dn, s1->c[0]) > dot(dn, s1->c[1]) ? s1->c[0] : s1->c[1];
+        float sep = plane_dist(sv, plane);
+        if (sep > fSep) { fIndex = i; fSep = sep; }
+    }
+    if (fSep > totalRadius) return;
+    /* edge query (sat_edge_and_hull.cpp:99-143) */
+    XF xe = xf_mulT_xf(xf2, xf1);
+    V3 P1 = xf_mul(&xe, s1->c[0]);
+    V3 Q1 = xf_mul(&xe, s1->c[1]);
+    V3 E1 = sub(Q1, P1);
+    V3 C2 = h2->centroid;
+    uint32_t eIndex = 0;
+    float eSep = -MAX_FLOAT;
+    for (uint32_t i = 0; i < h2->edgeCount; i += 2) {
+        const b3o_halfedge* e2 = h2->edges + i;
+        const b3o_halfedge* t2 = h2->edges + i + 1;
+        V3 P2 = h2->vertices[e2->origin];
+        V3 Q2 = h2->vertices[t2->origin];
+        V3 E2 = sub(Q2, P2);
+        V3 U2 = h2->planes[e2->face].n;
+        V3 V2 = h2->planes[t2->face].n;
+        FL(1);
+        if (dot(E1, U2) * dot(E1, V2) < 0.0f) {
+            float L1 = length(E1);
+            float sep;
+            if (L1 < LINEAR_SLOP) {
+                sep = -MAX_FLOAT;
+            } else {
+                float L2 = length(E2);
+                if (L2 < LINEAR_SLOP) {
+                    sep = -MAX_FLOAT;
+                } else {
+                    const float kTol = 0.005f;
+                    V3 E1xE2 = cross(E1, E2);
+                    float L = length(E1xE2);
+                    FL(2);
+                    if (L < kTol * L1 * L2) {
+                        sep = -MAX_FLOAT;
+                    } else {
+                        FL(1);
+                        V3 N = scale(1.0f / L, E1xE2);
+                        if (dot(N, sub(P2, C2)) > 0.0f) N = neg(N);
+                        sep = dot(N, sub(P2, P1));
+                    }
+                }
+            }
+            if (sep > eSep) { eSep = sep; eIndex = i; }
+        }
+    }
+    if (eSep > totalRadius) return;
+    const float kTol = 0.1f * LINEAR_SLOP;
+    FL(1);
+    if (eSep > fSep + kTol) capsule_edge_contact(m, xf1, s1, xf2, eIndex, h2);
+    else capsule_face_contact(m, xf1, s1, xf2, fIndex, h2, r2);
+}
+
 /* ---------------------------------------------------------------- entry points */
 static void fresh_cache(b3o_convex_cache* c) {
     memset(c, 0, sizeof(*c));
@@ -715,9 +1475,22 @@ void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes, uint3
             q10 = sweeps[4 * (size_t)i]; q1 = sweeps[4 * (size_t)i + 1];
             q20 = sweeps[4 * (size_t)i + 2]; q2 = sweeps[4 * (size_t)i + 3];
         }
+        /* s_CollideMatrix (collide.cpp:172-177); typeA <= typeB */
         if (sa->type == B3O_HULL && sb->type == B3O_HULL) {
             Hull h1 = hull_at(pool, sa->hull), h2 = hull_at(pool, sb->hull);
             collide_hulls_cached(m, xa, &h1, sa->radius, xb, &h2, sb->radius, &c->feature, q10, q1, q20, q2);
+        } else if (sb->type == B3O_HULL) {
+            Round r1 = round_of(sa);
+            Hull h2 = hull_at(pool, sb->hull);
+            PHASE(1);
+            if (sa->type == B3O_SPHERE) collide_sphere_hull(m, xa, &r1, xb, &h2, sb->radius);
+            else collide_capsule_hull(m, xa, &r1, xb, &h2, sb->radius);
+        } else {
+            Round r1 = round_of(sa), r2 = round_of(sb);
+            PHASE(1);
+            if (sa->type == B3O_SPHERE && sb->type == B3O_SPHERE) collide_spheres(m, xa, &r1, xb, &r2);
+            else if (sa->type == B3O_SPHERE) collide_sphere_capsule(m, xa, &r1, xb, &r2);
+            else collide_capsules(m, xa, &r1, xb, &r2);
         }
     }
 }

@@ -21,7 +21,7 @@ import oracle_lib  # noqa: E402
 from bounce_amd import load_library  # noqa: E402
 from bounce_amd.shapes import box_hull, identity_xf  # noqa: E402
 from bounce_amd.types import HullPool, empty_caches  # noqa: E402
-from util import hull_workload  # noqa: E402
+from util import hull_workload, mixed_workload, special_convex_cases  # noqa: E402
 
 
 def save(name, pool, shapes, pairs, xfs, out, sweeps=None, caches_in=None, caches_out=None):
@@ -81,7 +81,14 @@ def main():
     bx[0, 1] = identity_xf((0, 0, 0))
     bout = ref.collide(bpool, bshapes, bp, bx)
     save("box_pairs.npz", bpool, bshapes, bp, bx, bout)
-    print("wrote fixtures; overlap fractions:", (out["pointCount"] > 0).mean(), (bout["pointCount"] > 0).mean())
+    # 4. Sphere/capsule/hull pairs (GJK and closed forms), plus the hand-placed special cases.
+    mpool, mshapes, mpairs, mxfs = mixed_workload(lib, 4321, 1024)
+    mout = ref.collide(mpool, mshapes, mpairs, mxfs)
+    save("mixed_pairs.npz", mpool, mshapes, mpairs, mxfs, mout)
+    spool, sshapes, spairs, sxfs = special_convex_cases()
+    save("special_convex.npz", spool, sshapes, spairs, sxfs, ref.collide(spool, sshapes, spairs, sxfs))
+    print("wrote fixtures; overlap fractions:", (out["pointCount"] > 0).mean(), (bout["pointCount"] > 0).mean(),
+          (mout["pointCount"] > 0).mean())
 
 
 if __name__ == "__main__":

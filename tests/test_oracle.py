@@ -149,3 +149,47 @@ def test_manifold_invariants(workload, oracle):
         n = m["points"][i][:pc]["localNormal1"].astype(np.float64)
         np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-5)
         assert max(world_separations(m[i], xfs[i, 0], xfs[i, 1])) <= 1e-4
+
+
+def test_oracle_equals_reference_mixed_shapes(lib, oracle, reference):
+    """Sphere/capsule/hull pairs: closed forms, GJK distance, capsule clipping and the
+    deep SAT branches (collide_sphere_hull.cpp, collide_capsule_hull.cpp, ...)."""
+    from util import mixed_workload
+    pool, shapes, pairs, xfs = mixed_workload(lib, 21, 4000)
+    assert_manifolds_equal(oracle.collide(pool, shapes, pairs, xfs), reference.collide(pool, shapes, pairs, xfs))
+
+
+def test_oracle_equals_reference_special_convex(oracle, reference):
+    from util import special_convex_cases
+    pool, shapes, pairs, xfs = special_convex_cases()
+    want = reference.collide(pool, shapes, pairs, xfs)
+    assert (want["pointCount"] == 2).sum() >= 10  # parallel capsules and capsule-on-face
+    assert_manifolds_equal(oracle.collide(pool, shapes, pairs, xfs), want)
+
+
+def test_sphere_hull_gjk_distance_fp64(lib, oracle):
+    """Separated sphere-hull pairs: the GJK witness distance (world points of the
+    manifold, minus the radius) equals the FP64 point-to-polytope distance (closest
+    point over the hull's triangles)."""
+    from util import mixed_workload, point_hull_distance, world_separations
+    from bounce_amd.types import SPHERE, HULL
+    pool, shapes, pairs, xfs = mixed_workload(lib, 33, 400, gap_mean=0.05, gap_sigma=0.05, kinds=[(SPHERE, HULL)])
+    m = oracle.collide(pool, shapes, pairs, xfs)
+    checked = 0
+    for i in np.where(m["pointCount"] == 1)[0][:120]:
+        s, h = shapes[pairs[i, 0]], shapes[pairs[i, 1]]
+        Ra, ta = xf_matrix(xfs[i, 0])
+        Rb, tb = xf_matrix(xfs[i, 1])
+        c = Ra @ s["v"][:3].astype(np.float64) + ta
+        best = point_hull_distance(pool, int(h["hull"]), Rb, tb, c)
+        sep = world_separations(m[i], xfs[i, 0], xfs[i, 1], float(s["radius"]), 0.0)[0]
+        if best > 1e-3:  # shallow (GJK) branch
+            assert abs((sep + float(s["radius"])) - best) < 1e-4
+            checked += 1
+    assert checked >= 10
+
+
+@pytest.mark.parametrize("name", ["mixed_pairs.npz", "special_convex.npz"])
+def test_oracle_golden_convex(oracle, name):
+    g = load_golden(name)
+    assert_manifolds_equal(oracle.collide(g["pool"], g["shapes"], g["pairs"], g["xfs"]), g["out"], name)
