@@ -55,6 +55,9 @@ def load_library():
         "b3amd_set_profiling": (i32, [vp, i32]),
         "b3amd_kernel_stats": (i32, [vp, ctypes.c_char_p, ctypes.POINTER(u32), ctypes.POINTER(ctypes.c_double)]),
         "b3amd_reset_kernel_stats": (i32, [vp]),
+        "b3amd_upload_mesh": (i32, [vp, u32, vp, u32, vp, ctypes.POINTER(u32)]),
+        "b3amd_mesh_tree": (i32, [vp, u32, ctypes.POINTER(u32), vp]),
+        "b3amd_collide_mesh": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -140,6 +143,38 @@ class Context:
         out = np.zeros(n, MANIFOLD_DTYPE)
         _check(self.lib.b3amd_collide(self.handle, n, ptr(pairs), ptr(xfs), ptr(sweeps), ptr(caches), ptr(out)))
         return out
+
+    def upload_mesh(self, vertices, triangles):
+        """b3Mesh upload with its static tree; returns the mesh id."""
+        v = np.ascontiguousarray(vertices, np.float32).reshape(-1, 3)
+        t = np.ascontiguousarray(triangles, np.uint32).reshape(-1, 3)
+        mid = ctypes.c_uint32()
+        _check(self.lib.b3amd_upload_mesh(self.handle, len(v), ptr(v), len(t), ptr(t), ctypes.byref(mid)))
+        return mid.value
+
+    def mesh_tree(self, mesh_id):
+        """Static-tree nodes [2T-1][8] (float32 view; words 6-7 are u32 children)."""
+        n = ctypes.c_uint32()
+        _check(self.lib.b3amd_mesh_tree(self.handle, mesh_id, ctypes.byref(n), None))
+        nodes = np.zeros((n.value, 8), np.float32)
+        _check(self.lib.b3amd_mesh_tree(self.handle, mesh_id, ctypes.byref(n), ptr(nodes)))
+        return nodes
+
+    MESH_TRIANGLE_CAP = 64
+
+    def collide_mesh(self, mesh_id, shape_a, xfs):
+        """Fresh mesh contacts: (triangle counts, triangle lists [n][64], manifolds [n][3],
+        manifold counts)."""
+        shape_a = np.ascontiguousarray(shape_a, np.uint32)
+        n = len(shape_a)
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        counts = np.zeros(n, np.uint32)
+        lists = np.zeros((n, self.MESH_TRIANGLE_CAP), np.uint32)
+        out = np.zeros((n, 3), MANIFOLD_DTYPE)
+        mcount = np.zeros(n, np.uint32)
+        _check(self.lib.b3amd_collide_mesh(self.handle, mesh_id, n, ptr(shape_a), ptr(xfs), ptr(counts), ptr(lists),
+                                           ptr(out), ptr(mcount)))
+        return counts, lists, out, mcount
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)

@@ -20,7 +20,8 @@ LIB = os.path.join(HERE, "libbounce_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["narrowphase_hull.hip", "narrowphase_convex.hip", "api.cpp", "workload.cpp", "hullgen.cpp"]
+SOURCES = ["narrowphase_hull.hip", "narrowphase_convex.hip", "narrowphase_mesh.hip", "api.cpp", "workload.cpp",
+           "hullgen.cpp", "static_tree.cpp"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",

@@ -15,6 +15,7 @@
 #include "bounce_amd.h"
 #include "device_hull.h"
 #include "kernels.h"
+#include "static_tree.h"
 
 using namespace b3amd;
 
@@ -76,6 +77,17 @@ struct b3amd_context {
     uint32_t scratchCap = 0;
     void* dIn = nullptr;
     uint64_t dInCap = 0;
+
+    // Meshes (b3Mesh with its static tree) and mesh-contact scratch.
+    struct MeshRecord {
+        float4* verts = nullptr;
+        uint4* tris = nullptr;
+        TreeNode* nodes = nullptr;
+        uint32_t vertexCount = 0, triangleCount = 0, nodeCount = 0;
+    };
+    std::vector<MeshRecord> meshes;
+    void* dMeshScratch = nullptr;
+    uint64_t meshScratchCap = 0;
 
     // Profiling.
     bool profiling = false;
@@ -434,6 +446,137 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
     if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
     return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
                      reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
+}
+
+int b3amd_upload_mesh(b3amd_context* ctx, uint32_t vertexCount, const float* vertices, uint32_t triangleCount,
+                      const uint32_t* triangles, uint32_t* meshId) {
+    if (!ctx || !vertices || !triangles || !meshId || vertexCount == 0 || triangleCount == 0)
+        return fail(B3AMD_ERR_ARGUMENT, "null or empty mesh");
+    std::vector<TreeNode> nodes;
+    if (!build_static_tree(vertices, vertexCount, triangles, triangleCount, nodes))
+        return fail(B3AMD_ERR_ARGUMENT, "triangle refers to a vertex out of range");
+    std::vector<float4> v4(vertexCount);
+    for (uint32_t i = 0; i < vertexCount; ++i)
+        v4[i] = make_float4(vertices[3 * i], vertices[3 * i + 1], vertices[3 * i + 2], 0.0f);
+    std::vector<uint4> t4(triangleCount);
+    for (uint32_t i = 0; i < triangleCount; ++i)
+        t4[i] = make_uint4(triangles[3 * i], triangles[3 * i + 1], triangles[3 * i + 2], 0u);
+    b3amd_context::MeshRecord m;
+    int rc;
+    if ((rc = ctx->alloc(sizeof(float4) * (uint64_t)vertexCount, (void**)&m.verts))) return rc;
+    if ((rc = ctx->alloc(sizeof(uint4) * (uint64_t)triangleCount, (void**)&m.tris))) return rc;
+    if ((rc = ctx->alloc(sizeof(TreeNode) * (uint64_t)nodes.size(), (void**)&m.nodes))) return rc;
+    HIP_TRY(hipMemcpy(m.verts, v4.data(), sizeof(float4) * v4.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m.tris, t4.data(), sizeof(uint4) * t4.size(), hipMemcpyHostToDevice));
+    HIP_TRY(hipMemcpy(m.nodes, nodes.data(), sizeof(TreeNode) * nodes.size(), hipMemcpyHostToDevice));
+    m.vertexCount = vertexCount;
+    m.triangleCount = triangleCount;
+    m.nodeCount = (uint32_t)nodes.size();
+    ctx->meshes.push_back(m);
+    *meshId = (uint32_t)ctx->meshes.size() - 1;
+    return B3AMD_OK;
+}
+
+int b3amd_build_static_tree(uint32_t vertexCount, const float* vertices, uint32_t triangleCount,
+                            const uint32_t* triangles, float* nodesOut) {
+    if (!vertices || !triangles || !nodesOut || triangleCount == 0) return fail(B3AMD_ERR_ARGUMENT, "null or empty mesh");
+    std::vector<TreeNode> nodes;
+    if (!build_static_tree(vertices, vertexCount, triangles, triangleCount, nodes))
+        return fail(B3AMD_ERR_ARGUMENT, "triangle refers to a vertex out of range");
+    std::memcpy(nodesOut, nodes.data(), sizeof(TreeNode) * nodes.size());
+    return B3AMD_OK;
+}
+
+int b3amd_mesh_tree(b3amd_context* ctx, uint32_t meshId, uint32_t* nodeCount, float* nodesOut) {
+    if (!ctx || !nodeCount || meshId >= ctx->meshes.size()) return fail(B3AMD_ERR_ARGUMENT, "bad mesh id");
+    const auto& m = ctx->meshes[meshId];
+    *nodeCount = m.nodeCount;
+    if (nodesOut) HIP_TRY(hipMemcpy(nodesOut, m.nodes, sizeof(TreeNode) * (uint64_t)m.nodeCount, hipMemcpyDeviceToHost));
+    return B3AMD_OK;
+}
+
+int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
+                       const b3amd_transform* xfs, uint32_t* triangleCounts, uint32_t* triangleLists,
+                       b3amd_manifold* out, uint32_t* manifoldCounts) {
+    if (!ctx || (n && (!shapeA || !xfs || !out || !manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (meshId >= ctx->meshes.size()) return fail(B3AMD_ERR_ARGUMENT, "bad mesh id %u", meshId);
+    if (n == 0) return B3AMD_OK;
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (shapeA[i] >= ctx->shapeCount) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape %u of %u", i, shapeA[i], ctx->shapeCount);
+        if (ctx->shapes[shapeA[i]].type > B3AMD_HULL) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape A must be convex", i);
+    }
+    const uint32_t cap = kMeshTriangleCap;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    // Fixed part: inputs, per-contact outputs and the flat work list.
+    const uint64_t bShape = 4ull * n, bXf = 96ull * n, bCounts = 4ull * n, bLists = 4ull * n * cap,
+                   bFlags = 4ull * n, bOff = 4ull * n, bTotal = 256, bPairs = 4ull * n * cap, bOut = 720ull * n,
+                   bMc = 4ull * n;
+    const uint64_t fixed = al(bShape) + al(bXf) + al(bCounts) + al(bLists) + al(bFlags) + al(bOff) + al(bTotal) +
+                           al(bPairs) + al(bOut) + al(bMc);
+    int rc;
+    if (fixed > ctx->meshScratchCap) {
+        if ((rc = ctx->release(ctx->dMeshScratch))) return rc;
+        ctx->dMeshScratch = nullptr;
+        ctx->meshScratchCap = 0;
+        if ((rc = ctx->alloc(fixed, &ctx->dMeshScratch))) return rc;
+        ctx->meshScratchCap = fixed;
+    }
+    char* p = static_cast<char*>(ctx->dMeshScratch);
+    auto take = [&](uint64_t b) {
+        char* q = p;
+        p += al(b);
+        return q;
+    };
+    MeshArgs a;
+    a.n = n;
+    a.shapeA = reinterpret_cast<uint32_t*>(take(bShape));
+    a.xfs = reinterpret_cast<float*>(take(bXf));
+    a.triCounts = reinterpret_cast<uint32_t*>(take(bCounts));
+    a.triLists = reinterpret_cast<uint32_t*>(take(bLists));
+    a.contactFlags = reinterpret_cast<uint32_t*>(take(bFlags));
+    a.contactOffset = reinterpret_cast<uint32_t*>(take(bOff));
+    a.total = reinterpret_cast<uint32_t*>(take(bTotal));
+    a.pairContact = reinterpret_cast<uint32_t*>(take(bPairs));
+    a.out = reinterpret_cast<Manifold*>(take(bOut));
+    a.manifoldCounts = reinterpret_cast<uint32_t*>(take(bMc));
+    a.shapes = ctx->dShapes;
+    a.shapeCount = ctx->shapeCount;
+    a.hullCount = ctx->hullCount;
+    a.pool = ctx->dPool;
+    a.offsets = ctx->dOffsets;
+    const auto& mr = ctx->meshes[meshId];
+    a.mesh = MeshDev{mr.verts, mr.tris, reinterpret_cast<const uint32_t*>(mr.nodes), mr.nodeCount};
+    a.cap = cap;
+    a.temp = nullptr;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(const_cast<uint32_t*>(a.shapeA), shapeA, bShape, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(const_cast<float*>(a.xfs), xfs, bXf, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemsetAsync(a.total, 0, 4, s));
+    if ((rc = timed(ctx, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
+    uint32_t total = 0;
+    HIP_TRY(hipMemcpyAsync(&total, a.total, 4, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    // Per-triangle manifolds, sized by what the midphase found.
+    void* temp = nullptr;
+    if (total && (rc = ctx->alloc(240ull * total, &temp))) return rc;
+    a.temp = static_cast<Manifold*>(temp);
+    rc = timed(ctx, "mesh_narrow", [&] { return launch_mesh_narrow(a, total, s); });
+    if (!rc) rc = timed(ctx, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
+    if (!rc) {
+        hipError_t e = hipMemcpyAsync(out, a.out, bOut, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, a.manifoldCounts, bMc, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && triangleCounts) e = hipMemcpyAsync(triangleCounts, a.triCounts, bCounts, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess && triangleLists) e = hipMemcpyAsync(triangleLists, a.triLists, bLists, hipMemcpyDeviceToHost, s);
+        if (e == hipSuccess) e = hipStreamSynchronize(s);
+        if (e != hipSuccess) rc = fail(B3AMD_ERR_DEVICE, "mesh readback: %s", hipGetErrorString(e));
+    }
+    if (temp) {
+        int rc2 = ctx->release(temp);
+        if (!rc) rc = rc2;
+    }
+    if (!rc) rc = ctx->resolve_timings();
+    return rc;
 }
 
 int b3amd_synchronize(b3amd_context* ctx) {

@@ -1,0 +1,535 @@
+// Lane-per-pair contact building, templated over the hull accessor (a device hull record
+// or a triangle hull built on the fly): clipping (clip.cpp), polygon reduction
+// (contact_cluster.cpp:22-200), edge/face contacts and the cached rebuilds
+// (collide_hulls.cpp:26-204, collide_hulls_cache.cpp:35-153) and the feature-cache reads
+// (sat.cpp:177-278).  A hull accessor provides V, F, HE, vertex(i), plane(i), centroid(),
+// origin(h), face(h), next(h) and face_edge(f), with the twin of half-edge h at h ^ 1.
+#ifndef B3AMD_CONTACT_BUILD_H
+#define B3AMD_CONTACT_BUILD_H
+
+#include <hip/hip_runtime.h>
+
+#include "b3_math.h"
+#include "device_common.h"
+#include "device_hull.h"
+#include "kernels.h"
+
+namespace b3amd {
+
+// sat.cpp:73-108 with the per-edge lengths and P1 - C1 precomputed (same operations).
+__device__ inline float project_edges(V3 P1, V3 E1, float L1, V3 D1, V3 P2, V3 E2, float L2) {
+    if (L1 < kLinearSlop) return -kMaxFloat;
+    if (L2 < kLinearSlop) return -kMaxFloat;
+    const float kTol = 0.005f;
+    V3 E1xE2 = cross(E1, E2);
+    float L = length(E1xE2);
+    if (L < kTol * L1 * L2) return -kMaxFloat;
+    V3 N = (1.0f / L) * E1xE2;
+    if (dot(N, D1) < 0.0f) N = -N;
+    return dot(N, P2 - P1);
+}
+
+// ------------------------------------------------------------------ contact building
+
+constexpr int kPolyCap = 32;  // b3StackArray<b3ClipVertex, 32> inline capacity (clip.cpp)
+
+struct Poly {
+    uint32_t count;
+    bool overflow;
+    ClipV v[kPolyCap];
+    __device__ void push(const ClipV& x) {
+        if (count < kPolyCap) v[count++] = x;
+        else overflow = true;
+    }
+};
+
+struct CPV {
+    V3 position;
+    uint32_t clipIndex;
+};
+struct CPoly {
+    uint32_t count;
+    CPV v[kPolyCap];
+};
+
+// clip.cpp:33-57
+template <class H>
+__device__ inline void build_polygon(Poly& out, const Xf& xf, uint32_t index, const H& h) {
+    out.count = 0;
+    uint32_t begin = h.face_edge(index);
+    uint32_t e = begin;
+    uint32_t guard = 0;
+    do {
+        uint32_t twin = e ^ 1u;
+        uint32_t edgeId = twin ^ 1u;  // twin->twin
+        uint32_t prevId = h.next(twin);
+        out.push(clipv(mul(xf, h.vertex(h.origin(e))), prevId, kNullEdge, edgeId, kNullEdge));
+        e = h.next(e);
+    } while (e != begin && ++guard < 256);
+}
+
+// clip.cpp:99-149
+__device__ inline void clip_to_plane(Poly& out, const Poly& in, const Plane& pl, uint32_t id) {
+    out.count = 0;
+    ClipV v1 = in.v[in.count - 1];
+    float d1 = distance(v1.position, pl);
+    for (uint32_t i = 0; i < in.count; ++i) {
+        ClipV v2 = in.v[i];
+        float d2 = distance(v2.position, pl);
+        if (d1 <= 0.0f && d2 <= 0.0f) {
+            out.push(v2);
+        } else if (d1 <= 0.0f && d2 > 0.0f) {
+            float f = d1 / (d1 - d2);
+            V3 pos = v1.position + f * (v2.position - v1.position);
+            out.push(clipv(pos, v1.inEdge1, kNullEdge, kNullEdge, id));
+        } else if (d1 > 0.0f && d2 <= 0.0f) {
+            float f = d1 / (d1 - d2);
+            V3 pos = v1.position + f * (v2.position - v1.position);
+            out.push(clipv(pos, v1.inEdge1, id, kNullEdge, kNullEdge));
+            out.push(v2);
+        }
+        v1 = v2;
+        d1 = d2;
+    }
+}
+
+// clip.cpp:243-280; returns the clipped polygon in `a` (b is scratch).
+template <class H>
+__device__ inline void clip_to_face(Poly& a, Poly& b, const Xf& xf, float r, uint32_t index, const H& h) {
+    uint32_t begin = h.face_edge(index);
+    uint32_t e = begin;
+    uint32_t guard = 0;
+    do {
+        uint32_t edgeId = e;  // (edge->twin)->twin
+        Plane pl = edge_side_plane(h, edgeId);
+        pl.d += r;
+        Plane wp = mul(xf, pl);
+        clip_to_plane(b, a, wp, edgeId);
+        a.count = b.count;
+        a.overflow = a.overflow || b.overflow;
+        for (uint32_t i = 0; i < b.count; ++i) a.v[i] = b.v[i];
+        if (a.count == 0) return;
+        e = h.next(e);
+    } while (e != begin && ++guard < 256);
+}
+
+// contact_cluster.cpp:22-26
+__device__ __forceinline__ bool is_ccw(V3 A, V3 B, V3 C, V3 N) {
+    V3 n = cross(B - A, C - A);
+    return dot(n, N) > 0.0f;
+}
+
+// contact_cluster.cpp:33-56 (the pivot B is read once per i, before any swap)
+__device__ inline void sort_polygon(CPoly& p, V3 N) {
+    V3 A = p.v[0].position;
+    for (uint32_t i = 1; i < p.count; ++i) {
+        V3 B = p.v[i].position;
+        for (uint32_t j = i + 1; j < p.count; ++j) {
+            V3 C = p.v[j].position;
+            if (!is_ccw(A, B, C, N)) {
+                CPV t = p.v[i];
+                p.v[i] = p.v[j];
+                p.v[j] = t;
+            }
+        }
+    }
+}
+
+// contact_cluster.cpp:58-200: reduces `in` into `out`.
+__device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_t initial) {
+    out.count = 0;
+    if (in.count <= 4) {
+        out.count = in.count;
+        for (uint32_t i = 0; i < in.count; ++i) out.v[i] = in.v[i];
+        sort_polygon(out, N);
+        return;
+    }
+    uint32_t chosen = 0;  // bit mask over in.v (count <= 32)
+    out.v[out.count++] = in.v[initial];
+    chosen |= 1u << initial;
+    {
+        V3 A = out.v[0].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 d = in.v[i].position - A;
+            float dd = dot(d, d);
+            if (dd > mx) {
+                mx = dd;
+                index = i;
+            }
+        }
+        if (mx < kEpsilon * kEpsilon) return;
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out.v[0].position, B = out.v[1].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 C = in.v[i].position;
+            V3 n = cross(B - A, C - A);
+            float a2 = fabsf(dot(n, N));
+            if (a2 > mx) {
+                mx = a2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (mx < 2.0f * kAreaTol) return;
+        V3 C = in.v[index].position;
+        if (!is_ccw(A, B, C, N)) {
+            CPV t = out.v[0];
+            out.v[0] = out.v[1];
+            out.v[1] = t;
+        }
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out.v[0].position, B = out.v[1].position;
+        uint32_t index = 0;
+        float mn = kMaxFloat;
+        for (uint32_t i = 0; i < in.count; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 D = in.v[i].position;
+            V3 n = cross(B - A, D - A);
+            float sa2 = dot(n, N);
+            if (sa2 < mn) {
+                mn = sa2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (fabsf(mn) < 2.0f * kAreaTol) return;
+        out.v[out.count++] = in.v[index];
+        chosen |= 1u << index;
+    }
+    sort_polygon(out, N);
+}
+
+// collide_hulls.cpp:26-91, and with `rebuild` the segment check of
+// b3RebuildEdgeContact (collide_hulls_cache.cpp:35-111).
+template <class H1, class H2>
+__device__ inline void build_edge_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1,
+                                   const Xf& xf2, uint32_t index2, const H2& h2, bool rebuild) {
+    V3 C1 = mul(xf1, h1.centroid());
+    V3 P1 = mul(xf1, h1.vertex(h1.origin(index1)));
+    V3 Q1 = mul(xf1, h1.vertex(h1.origin(index1 + 1)));
+    V3 E1 = Q1 - P1;
+    V3 N1 = E1;
+    normalize_in_place(N1);
+    V3 P2 = mul(xf2, h2.vertex(h2.origin(index2)));
+    V3 Q2 = mul(xf2, h2.vertex(h2.origin(index2 + 1)));
+    V3 E2 = Q2 - P2;
+    V3 N2 = E2;
+    normalize_in_place(N2);
+    float b = dot(N1, N2);
+    float den = 1.0f - b * b;
+    if (den == 0.0f) return;
+    float inv_den = 1.0f / den;
+    V3 E3 = P1 - P2;
+    float d = dot(N1, E3);
+    float e = dot(N2, E3);
+    float s = inv_den * (b * e - d);
+    float t = inv_den * (e - b * d);
+    V3 c1 = P1 + s * N1;
+    V3 c2 = P2 + t * N2;
+    if (rebuild) {
+        float w2[3], w1[3];
+        barycentric(w2, P1, Q1, c2);
+        barycentric(w1, P2, Q2, c1);
+        if (!(w2[1] > 0.0f && w2[1] <= w2[2] && w1[1] > 0.0f && w1[1] <= w1[2])) return;
+    }
+    V3 N = cross(E1, E2);
+    normalize_in_place(N);
+    if (dot(N, P1 - C1) < 0.0f) N = -N;
+    m.pointCount = 1;
+    m.n[0] = mulT(xf1.R, N);
+    m.p1[0] = mulT(xf1, c1);
+    m.p2[0] = mulT(xf2, c2);
+    m.key[0][0] = index1;      // inEdge1
+    m.key[0][1] = index2;      // outEdge1
+    m.key[0][2] = index1 + 1;  // inEdge2
+    m.key[0][3] = index2 + 1;  // outEdge2
+}
+
+// collide_hulls.cpp:93-204
+template <class H1, class H2>
+__device__ inline void build_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1, float r1,
+                                   const Xf& xf2, const H2& h2, float r2, bool flip) {
+    Poly poly, scratch;
+    poly.overflow = false;
+    scratch.overflow = false;
+    float totalRadius = r1 + r2;
+    Plane plane1 = mul(xf1, h1.plane(index1));
+    V3 normal1 = mulT(xf2.R, plane1.n);
+    // hull.inl:47-61 support face of hull 2 along -normal1
+    V3 dn = -normal1;
+    uint32_t index2 = 0;
+    float mp = dot(dn, h2.plane(0).n);
+    for (uint32_t i = 1; i < h2.F; ++i) {
+        float q = dot(dn, h2.plane(i).n);
+        if (q > mp) {
+            mp = q;
+            index2 = i;
+        }
+    }
+    build_polygon(poly, xf2, index2, h2);
+    clip_to_face(poly, scratch, xf1, totalRadius, index1, h1);
+    m.overflow = m.overflow || poly.overflow;
+    if (poly.count == 0) return;
+    CPoly p1, red;
+    p1.count = 0;
+    uint32_t minIndex = 0;
+    float minSep = kMaxFloat;
+    for (uint32_t i = 0; i < poly.count; ++i) {
+        V3 pos = poly.v[i].position;
+        float sep = distance(pos, plane1);
+        if (sep <= totalRadius) {
+            if (sep < minSep) {
+                minIndex = p1.count;
+                minSep = sep;
+            }
+            p1.v[p1.count].position = closest_on_plane(pos, plane1);
+            p1.v[p1.count].clipIndex = i;
+            ++p1.count;
+        }
+    }
+    if (p1.count == 0) return;
+    V3 normal = plane1.n;
+    V3 s_normal = flip ? -normal : normal;
+    reduce_polygon(red, p1, s_normal, minIndex);
+    for (uint32_t i = 0; i < red.count; ++i) {
+        const ClipV& v2 = poly.v[red.v[i].clipIndex];
+        V3 v1 = closest_on_plane(v2.position, plane1);
+        if (flip) {
+            // b3MakePair(in2, in1, out2, out1) swaps the hull roles of the key.
+            m.n[i] = mulT(xf2.R, s_normal);
+            m.p1[i] = mulT(xf2, v2.position);
+            m.p2[i] = mulT(xf1, v1);
+            m.key[i][0] = v2.inEdge2;
+            m.key[i][1] = v2.outEdge2;
+            m.key[i][2] = v2.inEdge1;
+            m.key[i][3] = v2.outEdge1;
+        } else {
+            m.n[i] = mulT(xf1.R, normal);
+            m.p1[i] = mulT(xf1, v1);
+            m.p2[i] = mulT(xf2, v2.position);
+            m.key[i][0] = v2.inEdge1;
+            m.key[i][1] = v2.outEdge1;
+            m.key[i][2] = v2.inEdge2;
+            m.key[i][3] = v2.outEdge2;
+        }
+    }
+    m.pointCount = red.count;
+}
+
+// collide_hulls_cache.cpp:113-153
+template <class H1, class H2>
+__device__ inline void rebuild_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1, float r1,
+                                     const Xf& xf2, const H2& h2, float r2, bool flip,
+                                     Quat q10, Quat q1, Quat q20, Quat q2) {
+    Quat dq0 = qmul(conj(q10), q20);
+    Quat dq = qmul(conj(q1), q2);
+    Quat q = qmul(conj(dq0), dq);
+    const float kTol = 0.995f;
+    if (fabsf(q.w) > kTol) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
+}
+
+// b3FeatureCache::ReadFace (sat.cpp:209-222)
+template <class H1, class H2>
+__device__ inline uint32_t read_face(uint32_t index1, const Xf& xf1, const H1& h1, const Xf& xf2, const H2& h2,
+                              float totalRadius) {
+    Xf xf = mulT(xf2, xf1);
+    Plane pl = mul(xf, h1.plane(index1));
+    V3 d = -pl.n;
+    uint32_t mi = 0;
+    float mp = dot(d, h2.vertex(0));
+    for (uint32_t k = 1; k < h2.V; ++k) {
+        float q = dot(d, h2.vertex(k));
+        if (q > mp) {
+            mp = q;
+            mi = k;
+        }
+    }
+    float s = distance(h2.vertex(mi), pl);
+    return s > totalRadius ? kCacheSeparation : kCacheOverlap;
+}
+
+// b3FeatureCache::ReadEdge (sat.cpp:224-278)
+template <class H1, class H2>
+__device__ inline uint32_t read_edge(uint32_t i, uint32_t j, const Xf& xf1, const H1& h1, const Xf& xf2,
+                              const H2& h2, float totalRadius) {
+    Xf xf = mulT(xf2, xf1);
+    V3 C1 = mul(xf, h1.centroid());
+    V3 P1 = mul(xf, h1.vertex(h1.origin(i)));
+    V3 Q1 = mul(xf, h1.vertex(h1.origin(i + 1)));
+    V3 E1 = Q1 - P1;
+    V3 U1 = mul(xf.R, h1.plane(h1.face(i)).n);
+    V3 V1 = mul(xf.R, h1.plane(h1.face(i + 1)).n);
+    V3 P2 = h2.vertex(h2.origin(j));
+    V3 Q2 = h2.vertex(h2.origin(j + 1));
+    V3 E2 = Q2 - P2;
+    V3 U2 = h2.plane(h2.face(j)).n;
+    V3 V2 = h2.plane(h2.face(j + 1)).n;
+    V3 nE1 = -E1, nE2 = -E2, nU2 = -U2, nV2 = -V2;
+    float ADC = dot(U1, nE2);
+    float BDC = dot(V1, nE2);
+    float CBA = dot(nU2, nE1);
+    float DBA = dot(nV2, nE1);
+    if ((CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f)) {
+        float s = project_edges(P1, E1, length(E1), P1 - C1, P2, E2, length(E2));
+        return s > totalRadius ? kCacheSeparation : kCacheOverlap;
+    }
+    return kCacheEmpty;
+}
+
+// ------------------------------------------------------------------ sequential SAT
+
+// b3QueryFaceSeparation (sat.cpp:33-58) for one lane.
+template <class H1, class H2>
+__device__ inline void query_face_seq(const Xf& xf1, const H1& h1, const Xf& xf2, const H2& h2, uint32_t& index,
+                                      float& separation) {
+    Xf xf = mulT(xf2, xf1);
+    index = 0;
+    separation = -kMaxFloat;
+    for (uint32_t i = 0; i < h1.F; ++i) {
+        Plane pl = mul(xf, h1.plane(i));
+        V3 d = -pl.n;
+        uint32_t mi = 0;
+        float mp = dot(d, h2.vertex(0));
+        for (uint32_t k = 1; k < h2.V; ++k) {
+            float q = dot(d, h2.vertex(k));
+            if (q > mp) {
+                mp = q;
+                mi = k;
+            }
+        }
+        float s = distance(h2.vertex(mi), pl);
+        if (s > separation) {
+            index = i;
+            separation = s;
+        }
+    }
+}
+
+// b3QueryEdgeSeparation (sat.cpp:110-173) for one lane.
+template <class H1, class H2>
+__device__ inline void query_edge_seq(const Xf& xf1, const H1& h1, const Xf& xf2, const H2& h2, uint32_t& index1,
+                                      uint32_t& index2, float& separation) {
+    Xf xf = mulT(xf2, xf1);
+    V3 C1 = mul(xf, h1.centroid());
+    index1 = 0;
+    index2 = 0;
+    separation = -kMaxFloat;
+    for (uint32_t i = 0; i < h1.HE; i += 2) {
+        V3 P1 = mul(xf, h1.vertex(h1.origin(i)));
+        V3 Q1 = mul(xf, h1.vertex(h1.origin(i + 1)));
+        V3 E1 = Q1 - P1;
+        V3 U1 = mul(xf.R, h1.plane(h1.face(i)).n);
+        V3 V1 = mul(xf.R, h1.plane(h1.face(i + 1)).n);
+        V3 nE1 = -E1;
+        float L1 = length(E1);
+        V3 D1 = P1 - C1;
+        for (uint32_t j = 0; j < h2.HE; j += 2) {
+            V3 P2 = h2.vertex(h2.origin(j));
+            V3 Q2 = h2.vertex(h2.origin(j + 1));
+            V3 E2 = Q2 - P2;
+            V3 nE2 = -E2, nU2 = -h2.plane(h2.face(j)).n, nV2 = -h2.plane(h2.face(j + 1)).n;
+            float ADC = dot(U1, nE2);
+            float BDC = dot(V1, nE2);
+            float CBA = dot(nU2, nE1);
+            float DBA = dot(nV2, nE1);
+            if ((CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f)) {
+                float s = project_edges(P1, E1, L1, D1, P2, E2, length(E2));
+                if (s > separation) {
+                    separation = s;
+                    index1 = i;
+                    index2 = j;
+                }
+            }
+        }
+    }
+}
+
+__device__ inline void set_feature_cache(FeatureCache* c, uint32_t state, uint32_t type, uint32_t i1, uint32_t i2) {
+    if (!c) return;
+    c->state = state;
+    c->type = type;
+    c->index1 = i1;
+    c->index2 = i2;
+}
+
+// b3CollideCache (collide_hulls_cache.cpp:155-275) for one lane.
+template <class H1, class H2>
+__device__ inline void collide_cache_seq(Contact& m, const Xf& xf1, const H1& h1, float r1, const Xf& xf2,
+                                         const H2& h2, float r2, FeatureCache* fc) {
+    const float totalRadius = r1 + r2;
+    uint32_t f1i, f2i, ei, ej;
+    float f1s, f2s, es;
+    query_face_seq(xf1, h1, xf2, h2, f1i, f1s);
+    if (f1s > totalRadius) {
+        set_feature_cache(fc, kCacheSeparation, kFeatureFace1, f1i, f1i);
+        return;
+    }
+    query_face_seq(xf2, h2, xf1, h1, f2i, f2s);
+    if (f2s > totalRadius) {
+        set_feature_cache(fc, kCacheSeparation, kFeatureFace2, f2i, f2i);
+        return;
+    }
+    query_edge_seq(xf1, h1, xf2, h2, ei, ej, es);
+    if (es > totalRadius) {
+        set_feature_cache(fc, kCacheSeparation, kFeatureEdge1, ei, ej);
+        return;
+    }
+    const float kTol = 0.1f * kLinearSlop;
+    const float fmax12 = f1s > f2s ? f1s : f2s;
+    if (es > fmax12 + kTol) {
+        build_edge_contact(m, xf1, ei, h1, xf2, ej, h2, false);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureEdge1, ei, ej);
+            return;
+        }
+    } else if (f1s + kTol > f2s) {
+        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureFace1, f1i, f1i);
+            return;
+        }
+    } else {
+        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureFace2, f2i, f2i);
+            return;
+        }
+    }
+    if (es > fmax12) {
+        build_edge_contact(m, xf1, ei, h1, xf2, ej, h2, false);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureEdge1, ei, ej);
+            return;
+        }
+    } else if (f1s > f2s) {
+        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureFace1, f1i, f1i);
+            return;
+        }
+    } else {
+        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true);
+        if (m.pointCount > 0) {
+            set_feature_cache(fc, kCacheOverlap, kFeatureFace2, f2i, f2i);
+            return;
+        }
+    }
+    build_edge_contact(m, xf1, ei, h1, xf2, ej, h2, false);
+    set_feature_cache(fc, kCacheOverlap, kFeatureEdge1, ei, ej);
+}
+
+}  // namespace b3amd
+
+#endif

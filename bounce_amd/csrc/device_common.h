@@ -46,12 +46,12 @@ __device__ __forceinline__ ClipV clipv(V3 p, uint32_t in1, uint32_t in2, uint32_
 }
 
 // hull.inl:63-78
-__device__ inline Plane edge_side_plane(const DevHull& h, uint32_t index) {
-    uint32_t he = h.half(index);
-    V3 P = h.vertex(he & 0xFFu);
+template <class H>
+__device__ inline Plane edge_side_plane(const H& h, uint32_t index) {
+    V3 P = h.vertex(h.origin(index));
     V3 Q = h.vertex(h.origin(index ^ 1u));
     V3 E = Q - P;
-    V3 D = cross(E, h.plane((he >> 8) & 0xFFu).n);
+    V3 D = cross(E, h.plane(h.face(index)).n);
     Plane r;
     r.n = normalize(D);
     r.d = dot(r.n, P);

@@ -76,6 +76,38 @@ struct DevHull {
     }
 };
 
+
+// The 2-face hull of one mesh triangle, built in registers exactly as b3TriangleHull::Set
+// (triangle_hull.h:38-71): centroid (A + B + C) / 3, planes (N, dot(N, centroid)) and
+// (-N, dot(-N, centroid)) with N = normalize((B - A) x (C - A)), and the fixed half-edge
+// table (origin, twin, face, next): 0:(0,1,0,2) 1:(1,0,1,3) 2:(1,3,0,4) 3:(2,2,1,5)
+// 4:(2,5,0,0) 5:(0,4,1,1).
+struct TriHull {
+    V3 v0, v1, v2, c;
+    Plane p0, p1;
+    static constexpr uint32_t V = 3, F = 2, HE = 6;
+
+    __device__ static TriHull make(V3 A, V3 B, V3 C) {
+        TriHull t;
+        V3 s = (A + B) + C;
+        t.c = V3{s.x / 3.0f, s.y / 3.0f, s.z / 3.0f};
+        t.v0 = A;
+        t.v1 = B;
+        t.v2 = C;
+        V3 N = normalize(cross(B - A, C - A));
+        t.p0 = Plane{N, dot(N, t.c)};
+        t.p1 = Plane{-N, dot(-N, t.c)};
+        return t;
+    }
+    __device__ V3 vertex(uint32_t i) const { return i == 0 ? v0 : (i == 1 ? v1 : v2); }
+    __device__ Plane plane(uint32_t i) const { return i == 0 ? p0 : p1; }
+    __device__ V3 centroid() const { return c; }
+    __device__ uint32_t origin(uint32_t h) const { return (0x22110u >> (4u * h)) & 0xFu; }
+    __device__ uint32_t face(uint32_t h) const { return h & 1u; }
+    __device__ uint32_t next(uint32_t h) const { return h < 4u ? h + 2u : h - 4u; }
+    __device__ uint32_t face_edge(uint32_t f) const { return f; }
+};
+
 // Per-pair result of the separation queries (sat.cpp:33-173).
 struct SatResult {
     uint32_t f1Index;

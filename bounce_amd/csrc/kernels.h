@@ -103,6 +103,47 @@ struct ConvexArgs : PairInputs {
     Manifold* out;           // [n]
 };
 
+// A device-resident b3Mesh: vertices, triangles (v1, v2, v3, 0) and the static tree
+// (8 words per node: lower xyz, upper xyz, child1, child2|triangle; static_tree.h:62-77).
+struct MeshDev {
+    const float4* verts;
+    const uint4* tris;
+    const uint32_t* nodes;
+    uint32_t nodeCount;
+};
+
+// Triangles kept per mesh contact (the reference grows b3MeshContact::m_triangles without
+// bound, mesh_contact.cpp:166-177); contacts overlapping more are flagged.
+constexpr uint32_t kMeshTriangleCap = 64;
+constexpr uint32_t kFlagTriangleOverflow = 4u;
+constexpr uint32_t kFlagTreeStackOverflow = 8u;
+
+struct MeshArgs {
+    uint32_t n;                 // mesh contacts
+    const uint32_t* shapeA;     // [n] shape table index of the convex shape
+    const ShapeRec* shapes;
+    uint32_t shapeCount;
+    uint32_t hullCount;
+    const float* xfs;           // [n][2][12]: shape A body, mesh body
+    const float4* pool;
+    const uint32_t* offsets;
+    MeshDev mesh;
+    uint32_t cap;               // triangles per contact (<= kMeshTriangleCap)
+    uint32_t* triCounts;        // [n] triangles the midphase found (may exceed cap)
+    uint32_t* triLists;         // [n][cap] in DFS order
+    uint32_t* contactFlags;     // [n]
+    uint32_t* contactOffset;    // [n] first flat work item of the contact
+    uint32_t* total;            // flat work items (atomic counter, zeroed before the midphase)
+    uint32_t* pairContact;      // [n * cap] flat work item -> contact
+    Manifold* temp;             // [total] per-triangle manifolds
+    Manifold* out;              // [n][3] clustered manifolds
+    uint32_t* manifoldCounts;   // [n]
+};
+
+hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
+hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s);
+hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
+
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // sizeClass 0: every hull of the batch has V <= 64, F <= 128, HE/2 <= 192; 1: the limits.

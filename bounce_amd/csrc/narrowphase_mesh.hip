@@ -1,0 +1,410 @@
+// Shape-vs-triangle-mesh narrowphase for gfx950 (b3MeshContact, mesh_contact.cpp).
+//
+//   mesh_midphase_kernel  one lane per mesh contact: the fat AABB of shape A in the mesh
+//                         frame (mesh_contact.cpp:30-60) and the static-tree query in DFS
+//                         order, child2 first (static_tree.h:100-142); triangle lists are
+//                         compacted into one flat (contact, triangle) work list.
+//   mesh_narrow_kernel    one lane per (contact, triangle): b3TriangleHull built in
+//                         registers and b3CollideShapeAndShape with an empty cache, the
+//                         triangle index stamped into every point key (:238-266).
+//   mesh_cluster_kernel   one lane per mesh contact: b3ClusterSolver::Run over the
+//                         triangle manifolds in midphase order (contact_cluster.cpp:202-590).
+#include <hip/hip_runtime.h>
+
+#include "b3_math.h"
+#include "contact_build.h"
+#include "convex_build.h"
+#include "device_common.h"
+#include "device_hull.h"
+#include "kernels.h"
+
+namespace b3amd {
+namespace {
+
+struct Aabb {
+    V3 lo, hi;
+};
+
+__device__ __forceinline__ V3 vmin(V3 a, V3 b) {
+    return V3{a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z};
+}
+__device__ __forceinline__ V3 vmax(V3 a, V3 b) {
+    return V3{a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z};
+}
+
+// b3Shape::ComputeAABB: sphere_shape.cpp:53-59, capsule_shape.cpp:130-137,
+// hull_shape.cpp:162-166 (b3AABB3::Set over transformed vertices, then Extend(radius)).
+__device__ Aabb shape_aabb(const ShapeRec& s, const Xf& xf, const float4* pool, const uint32_t* offsets) {
+    Aabb a;
+    if (s.type == 0u) {
+        V3 c = mul(xf, V3{s.v[0], s.v[1], s.v[2]});
+        V3 r{s.radius, s.radius, s.radius};
+        a.lo = c - r;
+        a.hi = c + r;
+    } else if (s.type == 1u) {
+        V3 c1 = mul(xf, V3{s.v[0], s.v[1], s.v[2]});
+        V3 c2 = mul(xf, V3{s.v[3], s.v[4], s.v[5]});
+        V3 r{s.radius, s.radius, s.radius};
+        a.lo = vmin(c1, c2) - r;
+        a.hi = vmax(c1, c2) + r;
+    } else {
+        const DevHull h = DevHull::at(pool, offsets, s.hull);
+        a.lo = a.hi = mul(xf, h.vertex(0));
+        for (uint32_t i = 1; i < h.V; ++i) {
+            V3 v = mul(xf, h.vertex(i));
+            a.lo = vmin(a.lo, v);
+            a.hi = vmax(a.hi, v);
+        }
+        V3 r{s.radius, s.radius, s.radius};
+        a.lo = a.lo - r;
+        a.hi = a.hi + r;
+    }
+    return a;
+}
+
+// b3TestOverlap (aabb3.h:247-251) against one packed tree node.
+__device__ __forceinline__ bool overlaps(const float4& lo, const float4& hi, const Aabb& b) {
+    return (lo.x <= b.hi.x) && (lo.y <= b.hi.y) && (lo.z <= b.hi.z) && (lo.w >= b.lo.x) && (hi.x >= b.lo.y) &&
+           (hi.y >= b.lo.z);
+}
+
+constexpr uint32_t kTreeStack = 128;
+
+__global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    uint32_t flags = 0;
+    const uint32_t si = a.shapeA[c];
+    uint32_t found = 0;
+    if (si < a.shapeCount && a.shapes[si].type <= 2u &&
+        (a.shapes[si].type != 2u || a.shapes[si].hull < a.hullCount)) {
+        const ShapeRec s = a.shapes[si];
+        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+        Aabb box = shape_aabb(s, mulT(xfB, xfA), a.pool, a.offsets);
+        const V3 ext{0.2f, 0.2f, 0.2f};  // B3_AABB_EXTENSION (settings.h:54)
+        box.lo = box.lo - ext;
+        box.hi = box.hi + ext;
+        uint32_t stack[kTreeStack];
+        uint32_t top = 0;
+        if (a.mesh.nodeCount > 0) stack[top++] = 0;
+        uint32_t* list = a.triLists + (size_t)c * a.cap;
+        while (top) {
+            const uint32_t ni = stack[--top];
+            const float4* nd = reinterpret_cast<const float4*>(a.mesh.nodes + 8 * (size_t)ni);
+            const float4 q0 = nd[0], q1 = nd[1];  // lower xyz, upper x | upper yz, child1, child2
+            if (!overlaps(q0, q1, box)) continue;
+            const uint32_t child1 = __float_as_uint(q1.z), child2 = __float_as_uint(q1.w);
+            if (child1 == 0xFFFFFFFFu) {
+                if (found < a.cap) list[found] = child2;
+                ++found;
+            } else if (top + 2 <= kTreeStack) {
+                stack[top++] = child1;
+                stack[top++] = child2;
+            } else {
+                flags |= kFlagTreeStackOverflow;
+            }
+        }
+    } else {
+        flags |= kFlagInvalidPair;
+    }
+    if (found > a.cap) flags |= kFlagTriangleOverflow;
+    const uint32_t m = found < a.cap ? found : a.cap;
+    a.triCounts[c] = found;
+    a.contactFlags[c] = flags;
+    const uint32_t off = atomicAdd(a.total, m);
+    a.contactOffset[c] = off;
+    for (uint32_t k = 0; k < m; ++k) a.pairContact[off + k] = c;
+}
+
+__global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a, uint32_t total) {
+    const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= total) return;
+    const uint32_t c = a.pairContact[f];
+    const uint32_t k = f - a.contactOffset[c];
+    const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
+    const uint4 t = a.mesh.tris[ti];
+    const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
+    const TriHull tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
+    const ShapeRec s = a.shapes[a.shapeA[c]];
+    const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+    const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+    Contact m;
+    m.pointCount = 0;
+    m.overflow = false;
+    if (s.type == 2u) {
+        const DevHull h = DevHull::at(a.pool, a.offsets, s.hull);
+        collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
+    } else if (s.type == 0u) {
+        collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
+    } else {
+        collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
+    }
+    write_manifold(a.temp + f, m);
+    for (uint32_t j = 0; j < m.pointCount; ++j) a.temp[f].points[j].triangleKey = ti;
+}
+
+// ------------------------------------------------------------------ clustering
+
+constexpr uint32_t kMaxObs = 4 * kMeshTriangleCap;
+
+struct WPoint {
+    V3 point, normal;
+    float separation;
+};
+
+// b3WorldManifoldPoint::Initialize (manifold.cpp:57-69).
+__device__ WPoint world_point(const ManifoldPoint& p, float rA, const Xf& xfA, float rB, const Xf& xfB) {
+    WPoint w;
+    V3 nA = mul(xfA.R, p.localNormal1);
+    V3 cA = mul(xfA, p.localPoint1);
+    V3 cB = mul(xfB, p.localPoint2);
+    V3 pA = cA + rA * nA;
+    V3 pB = cB - rB * nA;
+    w.point = 0.5f * (pA + pB);
+    w.normal = nA;
+    w.separation = dot(cB - cA, nA) - rA - rB;
+    return w;
+}
+
+__device__ __forceinline__ float dist_sq(V3 a, V3 b) {
+    V3 d = a - b;
+    return d.x * d.x + d.y * d.y + d.z * d.z;  // b3DistanceSquared (vec3.h:207-211)
+}
+
+struct Clusters {
+    V3 c[4];  // InitializeClusters adds at most three
+    uint32_t n;
+    __device__ uint32_t find(V3 p) const {  // contact_cluster.cpp:361-377
+        uint32_t best = 0;
+        float bestValue = kMaxFloat;
+        for (uint32_t i = 0; i < n; ++i) {
+            float m = dist_sq(p, c[i]);
+            if (m < bestValue) {
+                bestValue = m;
+                best = i;
+            }
+        }
+        return best;
+    }
+    __device__ void add(V3 centroid) {  // contact_cluster.cpp:336-359
+        if (n == 0) {
+            c[n++] = centroid;
+            return;
+        }
+        uint32_t b = find(centroid);
+        const float kTol = 0.05f;
+        if (dist_sq(centroid, c[b]) <= kTol * kTol) {
+            c[b] = c[b] + centroid;
+            normalize_in_place(c[b]);
+            return;
+        }
+        c[n++] = centroid;
+    }
+};
+
+__global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
+    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
+    if (ci >= a.n) return;
+    Manifold* out = a.out + 3 * (size_t)ci;
+    Contact none;
+    none.pointCount = 0;
+    none.overflow = false;
+    const uint32_t flags = a.contactFlags[ci];
+    const uint32_t found = a.triCounts[ci];
+    const uint32_t m = found < a.cap ? found : a.cap;
+    uint32_t numOut = 0;
+    if (!(flags & kFlagInvalidPair) && m > 0) {
+        const Manifold* in = a.temp + a.contactOffset[ci];
+        const ShapeRec s = a.shapes[a.shapeA[ci]];
+        const Xf xfA = load_xf(a.xfs + 24 * (size_t)ci);
+        const Xf xfB = load_xf(a.xfs + 24 * (size_t)ci + 12);
+        const float rA = s.radius, rB = 0.0f;
+        // Observations: world normals of every point, in manifold order (:480-495).
+        V3 obs[kMaxObs];
+        uint32_t om[kMaxObs], cl[kMaxObs];
+        uint32_t nObs = 0;
+        for (uint32_t i = 0; i < m; ++i)
+            for (uint32_t j = 0; j < in[i].pointCount; ++j) {
+                obs[nObs] = world_point(in[i].points[j], rA, xfA, rB, xfB).normal;
+                om[nObs] = (i << 2) | j;
+                ++nObs;
+            }
+        Clusters C;
+        C.n = 0;
+        // InitializeClusters (:212-334).
+        if (nObs <= 3) {
+            for (uint32_t i = 0; i < nObs; ++i) C.add(obs[i]);
+        } else {
+            uint64_t chosenLo = 0, chosenHi[3] = {0, 0, 0};  // bit set over nObs <= 256
+            auto chosen = [&](uint32_t i) -> bool {
+                return i < 64 ? ((chosenLo >> i) & 1ull) : ((chosenHi[(i - 64) >> 6] >> ((i - 64) & 63)) & 1ull);
+            };
+            auto choose = [&](uint32_t i) {
+                if (i < 64) chosenLo |= 1ull << i;
+                else chosenHi[(i - 64) >> 6] |= 1ull << ((i - 64) & 63);
+            };
+            C.add(obs[0]);
+            choose(0);
+            {
+                V3 A = C.c[0];
+                uint32_t index = 0;
+                float mx = -kMaxFloat;
+                for (uint32_t i = 0; i < nObs; ++i) {
+                    if (chosen(i)) continue;
+                    float dd = dist_sq(A, obs[i]);
+                    if (dd > mx) {
+                        mx = dd;
+                        index = i;
+                    }
+                }
+                C.add(obs[index]);
+                choose(index);
+            }
+            if (C.n == 1) {
+                V3 A = C.c[0];
+                uint32_t index = 0;
+                float mx = -kMaxFloat;
+                for (uint32_t i = 0; i < nObs; ++i) {
+                    if (chosen(i)) continue;
+                    float dd = dist_sq(A, obs[i]);
+                    if (dd > mx) {
+                        mx = dd;
+                        index = i;
+                    }
+                }
+                C.add(obs[index]);
+                choose(index);
+            } else {
+                V3 A = C.c[0], B = C.c[1];
+                uint32_t index = 0;
+                float mx = -kMaxFloat;
+                for (uint32_t i = 0; i < nObs; ++i) {
+                    if (chosen(i)) continue;
+                    V3 Pt = obs[i];
+                    // b3ClosestPointOnSegment (geometry.h:174-193)
+                    float w[3];
+                    barycentric(w, A, B, Pt);
+                    V3 Q;
+                    if (w[1] <= 0.0f) Q = A;
+                    else if (w[0] <= 0.0f) Q = B;
+                    else {
+                        float sInv = 1.0f / w[2];
+                        float wA = sInv * w[0], wB = sInv * w[1];
+                        Q = wA * A + wB * B;
+                    }
+                    float dd = dist_sq(Pt, Q);
+                    if (dd > mx) {
+                        mx = dd;
+                        index = i;
+                    }
+                }
+                C.add(obs[index]);
+                choose(index);
+            }
+        }
+        // Ten Lloyd iterations with sequential sums (:404-444).
+        for (uint32_t iter = 0; iter < 10; ++iter) {
+            for (uint32_t i = 0; i < nObs; ++i) cl[i] = C.find(obs[i]);
+            for (uint32_t i = 0; i < C.n; ++i) {
+                V3 centroid{0.0f, 0.0f, 0.0f};
+                uint32_t count = 0;
+                for (uint32_t j = 0; j < nObs; ++j)
+                    if (cl[j] == i) {
+                        centroid = centroid + obs[j];
+                        ++count;
+                    }
+                if (count > 0) {
+                    float sInv = 1.0f / (float)count;  // b3Vec3::operator/= (vec3.h:70-76)
+                    C.c[i] = V3{centroid.x * sInv, centroid.y * sInv, centroid.z * sInv};
+                }
+            }
+        }
+        // Drop empty clusters, renumbering the rest in order (:449-472).
+        uint32_t used = 0;
+        for (uint32_t i = 0; i < C.n; ++i) {
+            uint32_t cnt = 0;
+            for (uint32_t j = 0; j < nObs; ++j) cnt += cl[j] == i;
+            if (cnt > 0) {
+                for (uint32_t j = 0; j < nObs; ++j)
+                    if (cl[j] == i) cl[j] = used;
+                C.c[used++] = C.c[i];
+            }
+        }
+        C.n = used;
+        // Per cluster: deepest point, offset polygon, reduction, original points (:497-587).
+        for (uint32_t i = 0; i < C.n && numOut < 3; ++i) {
+            CPoly poly, quad;
+            poly.count = 0;
+            V3 normal{0.0f, 0.0f, 0.0f};
+            for (uint32_t j = 0; j < nObs; ++j) {
+                if (cl[j] != i) continue;
+                const ManifoldPoint& mp = in[om[j] >> 2].points[om[j] & 3u];
+                if (poly.count < (uint32_t)kPolyCap) {
+                    poly.v[poly.count].position = world_point(mp, rA, xfA, rB, xfB).point;
+                    poly.v[poly.count].clipIndex = j;
+                    ++poly.count;
+                }
+                normal = normal + obs[j];
+            }
+            if (poly.count == 0) continue;
+            normalize_in_place(normal);
+            uint32_t minIndex = 0;
+            float minSep = kMaxFloat;
+            for (uint32_t j = 0; j < poly.count; ++j) {
+                const uint32_t o = poly.v[j].clipIndex;
+                WPoint w = world_point(in[om[o] >> 2].points[om[o] & 3u], rA, xfA, rB, xfB);
+                if (w.separation < minSep) {
+                    minIndex = j;
+                    minSep = w.separation;
+                }
+                poly.v[j].position = poly.v[j].position + w.separation * normal;
+            }
+            reduce_polygon(quad, poly, normal, minIndex);
+            Manifold mo;
+            mo.pointCount = quad.count;
+            mo.flags = flags;
+            mo.reserved[0] = mo.reserved[1] = 0;
+            for (uint32_t j = 0; j < 4; ++j) {
+                if (j < quad.count) {
+                    const uint32_t o = quad.v[j].clipIndex;
+                    mo.points[j] = in[om[o] >> 2].points[om[o] & 3u];
+                } else {
+                    mo.points[j].localNormal1 = V3{0.0f, 0.0f, 0.0f};
+                    mo.points[j].localPoint1 = V3{0.0f, 0.0f, 0.0f};
+                    mo.points[j].localPoint2 = V3{0.0f, 0.0f, 0.0f};
+                    mo.points[j].triangleKey = 0;
+                    mo.points[j].key[0] = mo.points[j].key[1] = mo.points[j].key[2] = mo.points[j].key[3] = 0;
+                }
+            }
+            out[numOut++] = mo;
+        }
+    }
+    for (uint32_t i = numOut; i < 3; ++i) {
+        write_manifold(out + i, none);
+        out[i].flags = flags;
+    }
+    a.manifoldCounts[ci] = numOut;
+}
+
+}  // namespace
+
+hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    mesh_midphase_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s) {
+    if (total == 0) return hipSuccess;
+    mesh_narrow_kernel<<<(total + 255) / 256, 256, 0, s>>>(a, total);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    mesh_cluster_kernel<<<(a.n + 63) / 64, 64, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+}  // namespace b3amd

@@ -144,6 +144,34 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
 
 int b3amd_synchronize(b3amd_context* ctx);
 
+/* ---- triangle meshes ----
+ * b3Mesh (mesh.h:41-99): vertices [V][3], triangles [T][3] (b3Triangle v1, v2, v3).  The
+ * static tree is built on the host with the node numbering of b3Mesh::BuildTree /
+ * b3StaticTree::Build (static_tree.cpp:42-180) and uploaded with the mesh. */
+int b3amd_upload_mesh(b3amd_context* ctx, uint32_t vertexCount, const float* vertices, uint32_t triangleCount,
+                      const uint32_t* triangles, uint32_t* meshId);
+/* The same tree built on the host only (no context, no GPU): nodesOut holds 2T - 1 nodes. */
+int b3amd_build_static_tree(uint32_t vertexCount, const float* vertices, uint32_t triangleCount,
+                            const uint32_t* triangles, float* nodesOut);
+/* Reads back the mesh's tree: nodeCount = 2T - 1 nodes of 8 words (lower xyz, upper xyz,
+ * child1, child2 or triangle; child1 = 0xFFFFFFFF for a leaf), nodesOut may be NULL. */
+int b3amd_mesh_tree(b3amd_context* ctx, uint32_t meshId, uint32_t* nodeCount, float* nodesOut);
+
+#define B3AMD_MESH_TRIANGLE_CAP 64u
+#define B3AMD_FLAG_INVALID_PAIR 2u
+#define B3AMD_FLAG_TRIANGLE_OVERFLOW 4u  /* more than B3AMD_MESH_TRIANGLE_CAP triangles overlapped */
+
+/* n freshly created mesh contacts between convex shapes[shapeA[i]] (body transform
+ * xfs[2i]) and mesh meshId (body transform xfs[2i+1]): what b3MeshContact does from
+ * construction to Collide (mesh_contact.cpp:30-275) - fat AABB in the mesh frame, tree
+ * query in DFS order, one narrowphase per triangle with an empty cache, clustering into at
+ * most 3 manifolds (contact_cluster.cpp:475-590).  out[3i..3i+2] receive the manifolds
+ * (manifoldCounts[i] of them); triangleCounts (may be NULL) the midphase hits and
+ * triangleLists (may be NULL, [n][B3AMD_MESH_TRIANGLE_CAP]) the triangle indices. */
+int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
+                       const b3amd_transform* xfs, uint32_t* triangleCounts, uint32_t* triangleLists,
+                       b3amd_manifold* out, uint32_t* manifoldCounts);
+
 /* ---- device memory helpers (callers need no other GPU runtime) ---- */
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);
 int b3amd_device_free(b3amd_context* ctx, void* ptr);

@@ -23,6 +23,7 @@
 
 #include <float.h>
 #include <math.h>
+#include <stdlib.h>
 #include <string.h>
 
 /* ---------------------------------------------------------------- op counting */
@@ -1450,6 +1451,469 @@ static void collide_capsule_hull(b3o_manifold* m, const XF* xf1, const Round* s1
     else capsule_face_contact(m, xf1, s1, xf2, fIndex, h2, r2);
 }
 
+/* ---------------------------------------------------------------- aabb3.h */
+typedef struct { V3 lo, hi; } AABB;
+
+static V3 vmin(V3 a, V3 b) { return v3(a.x < b.x ? a.x : b.x, a.y < b.y ? a.y : b.y, a.z < b.z ? a.z : b.z); }
+static V3 vmax(V3 a, V3 b) { return v3(a.x > b.x ? a.x : b.x, a.y > b.y ? a.y : b.y, a.z > b.z ? a.z : b.z); }
+static V3 aabb_centroid(const AABB* a) { return scale(0.5f, add(a->lo, a->hi)); }          /* aabb3.h:100-103 */
+static float comp(V3 v, uint32_t i) { return i == 0 ? v.x : (i == 1 ? v.y : v.z); }
+static uint32_t aabb_longest_axis(const AABB* a) {                                        /* aabb3.h:136-154 */
+    V3 c = aabb_centroid(a);
+    V3 r = sub(a->hi, c);
+    float mx = r.x;
+    uint32_t i = 0;
+    if (r.y > mx) { mx = r.y; i = 1; }
+    if (r.z > mx) { mx = r.z; i = 2; }
+    return i;
+}
+static int aabb_overlap(const AABB* a, const AABB* b) {                                   /* aabb3.h:247-251 */
+    return (a->lo.x <= b->hi.x) && (a->lo.y <= b->hi.y) && (a->lo.z <= b->hi.z) &&
+           (a->hi.x >= b->lo.x) && (a->hi.y >= b->lo.y) && (a->hi.z >= b->lo.z);
+}
+static void aabb_extend(AABB* a, float s) {                                               /* aabb3.h:85-90 */
+    V3 r = v3(s, s, s);
+    a->lo = sub(a->lo, r);
+    a->hi = add(a->hi, r);
+}
+
+/* ---------------------------------------------------------------- static_tree.cpp */
+typedef struct { AABB aabb; uint32_t child1, child2; } TNode;  /* child2 doubles as the leaf index */
+#define NULL_NODE 0xFFFFFFFFu
+
+static int sort_pred(const AABB* set, uint32_t axis, uint32_t a, uint32_t b) {             /* static_tree.cpp:35-40 */
+    V3 c1 = aabb_centroid(set + a), c2 = aabb_centroid(set + b);
+    return comp(c1, axis) < comp(c2, axis);
+}
+
+/* static_tree.cpp:42-67: Lomuto quicksort with the last element as pivot.  The recursion
+ * is unrolled onto an explicit stack; the two sub-ranges are disjoint, so the order of
+ * visiting them does not change the result. */
+static void tree_sort(const AABB* set, uint32_t axis, uint32_t* ids, uint32_t count, uint32_t* stack) {
+    uint32_t top = 0;
+    stack[top++] = 0;
+    stack[top++] = count;
+    while (top) {
+        uint32_t n = stack[--top];
+        uint32_t* p = ids + stack[--top];
+        if (n <= 1) continue;
+        uint32_t pivot = p[n - 1];
+        uint32_t low = 0;
+        for (uint32_t i = 0; i < n - 1; ++i) {
+            if (sort_pred(set, axis, p[i], pivot)) {
+                uint32_t t = p[i]; p[i] = p[low]; p[low] = t;
+                low++;
+            }
+        }
+        p[n - 1] = p[low];
+        p[low] = pivot;
+        stack[top++] = (uint32_t)(p - ids); stack[top++] = low;
+        stack[top++] = (uint32_t)(p - ids) + low + 1; stack[top++] = n - 1 - low;
+    }
+}
+
+static uint32_t tree_partition(const AABB* setAABB, const AABB* set, uint32_t* ids, uint32_t count,
+                               uint32_t* stack) {                                          /* static_tree.cpp:69-105 */
+    uint32_t axis = aabb_longest_axis(setAABB);
+    float splitPos = comp(aabb_centroid(setAABB), axis);
+    tree_sort(set, axis, ids, count, stack);
+    uint32_t left = 0, right = count - 1, middle = left;
+    while (middle < right) {
+        V3 c = aabb_centroid(set + ids[middle]);
+        if (comp(c, axis) > splitPos) break;
+        ++middle;
+    }
+    uint32_t c1 = middle, c2 = count - middle;
+    if (c1 == 0 || c2 == 0) middle = (left + right) / 2;
+    return middle;
+}
+
+/* b3StaticTree::Build (static_tree.cpp:107-180), one object per leaf; nodes must hold
+ * 2*count - 1 entries.  Recursion unrolled onto an explicit work list that preserves the
+ * node numbering (children are numbered when their parent is split). */
+void b3o_tree_build(const float* aabbs, uint32_t count, float* nodesOut /* [2n-1][8] */) {
+    const AABB* set = (const AABB*)aabbs;
+    TNode* nodes = (TNode*)nodesOut;
+    uint32_t* ids = (uint32_t*)malloc(sizeof(uint32_t) * count);
+    uint32_t* sortStack = (uint32_t*)malloc(sizeof(uint32_t) * (4 * (size_t)count + 8));
+    uint32_t* work = (uint32_t*)malloc(sizeof(uint32_t) * (6 * (size_t)count + 8));
+    for (uint32_t i = 0; i < count; ++i) ids[i] = i;
+    uint32_t nodeCount = 1, top = 0;
+    /* work item: node, first id, id count */
+    work[top++] = 0; work[top++] = 0; work[top++] = count;
+    while (top) {
+        uint32_t n = work[--top], first = work[--top], node = work[--top];
+        uint32_t* p = ids + first;
+        AABB setAABB = set[p[0]];
+        for (uint32_t i = 1; i < n; ++i) {
+            setAABB.lo = vmin(setAABB.lo, set[p[i]].lo);
+            setAABB.hi = vmax(setAABB.hi, set[p[i]].hi);
+        }
+        nodes[node].aabb = setAABB;
+        if (n <= 1) {
+            nodes[node].child1 = NULL_NODE;
+            nodes[node].child2 = p[0];
+        } else {
+            uint32_t middle = tree_partition(&setAABB, set, p, n, sortStack);
+            uint32_t c1 = nodeCount++, c2 = nodeCount++;
+            nodes[node].child1 = c1;
+            nodes[node].child2 = c2;
+            /* The reference builds child1's subtree (numbering its descendants) before
+             * child2's: process child1 first, so push child2 below it. */
+            work[top++] = c2; work[top++] = first + middle; work[top++] = n - middle;
+            work[top++] = c1; work[top++] = first; work[top++] = middle;
+        }
+    }
+    free(ids);
+    free(sortStack);
+    free(work);
+}
+
+/* b3StaticTree::QueryAABB (static_tree.h:100-142): DFS pushing child1 then child2. */
+static uint32_t tree_query(const TNode* nodes, uint32_t nodeCount, const AABB* box, uint32_t* out, uint32_t cap) {
+    uint32_t stack[512];
+    uint32_t top = 0, found = 0;
+    if (nodeCount == 0) return 0;
+    stack[top++] = 0;
+    while (top) {
+        uint32_t ni = stack[--top];
+        const TNode* nd = nodes + ni;
+        if (aabb_overlap(&nd->aabb, box)) {
+            if (nd->child1 == NULL_NODE) {
+                if (found < cap) out[found] = nd->child2;
+                ++found;
+            } else if (top + 2 <= 512) {
+                stack[top++] = nd->child1;
+                stack[top++] = nd->child2;
+            }
+        }
+    }
+    return found;
+}
+
+/* ---------------------------------------------------------------- shape AABBs */
+static AABB shape_aabb(const b3o_hull_pool* pool, const b3o_shape* s, const XF* xf) {
+    AABB a;
+    if (s->type == B3O_SPHERE) {                                                           /* sphere_shape.cpp:53-59 */
+        V3 c = xf_mul(xf, v3(s->v[0], s->v[1], s->v[2]));
+        V3 r = v3(s->radius, s->radius, s->radius);
+        a.lo = sub(c, r);
+        a.hi = add(c, r);
+    } else if (s->type == B3O_CAPSULE) {                                                   /* capsule_shape.cpp:130-137 */
+        V3 c1 = xf_mul(xf, v3(s->v[0], s->v[1], s->v[2]));
+        V3 c2 = xf_mul(xf, v3(s->v[3], s->v[4], s->v[5]));
+        V3 r = v3(s->radius, s->radius, s->radius);
+        a.lo = sub(vmin(c1, c2), r);
+        a.hi = add(vmax(c1, c2), r);
+    } else {                                                                               /* hull_shape.cpp:162-166 */
+        Hull h = hull_at(pool, s->hull);
+        a.lo = a.hi = xf_mul(xf, h.vertices[0]);
+        for (uint32_t i = 1; i < h.vertexCount; ++i) {
+            V3 v = xf_mul(xf, h.vertices[i]);
+            a.lo = vmin(a.lo, v);
+            a.hi = vmax(a.hi, v);
+        }
+        aabb_extend(&a, s->radius);
+    }
+    return a;
+}
+
+/* ---------------------------------------------------------------- triangle_hull.h:38-71 */
+typedef struct {
+    V3 v[3];
+    b3o_halfedge e[6];
+    uint32_t f[2];
+    PL p[2];
+} TriRec;
+
+static Hull triangle_hull(TriRec* t, V3 A, V3 B, V3 C, V3* centroid) {
+    V3 s = add(add(A, B), C);
+    FL(3);
+    *centroid = v3(s.x / 3.0f, s.y / 3.0f, s.z / 3.0f);
+    t->v[0] = A; t->v[1] = B; t->v[2] = C;
+    static const uint32_t tab[6][4] = {{0, 1, 0, 2}, {1, 0, 1, 3}, {1, 3, 0, 4}, {2, 2, 1, 5}, {2, 5, 0, 0}, {0, 4, 1, 1}};
+    for (int i = 0; i < 6; ++i) {
+        t->e[i].origin = tab[i][0]; t->e[i].twin = tab[i][1]; t->e[i].face = tab[i][2]; t->e[i].next = tab[i][3];
+    }
+    t->f[0] = 0;
+    t->f[1] = 1;
+    V3 N = normalize(cross(sub(B, A), sub(C, A)));
+    t->p[0].n = N;
+    t->p[0].d = dot(N, *centroid);
+    t->p[1].n = neg(N);
+    t->p[1].d = dot(neg(N), *centroid);
+    Hull h;
+    h.centroid = *centroid;
+    h.vertexCount = 3;
+    h.vertices = t->v;
+    h.edgeCount = 6;
+    h.edges = t->e;
+    h.faceCount = 2;
+    h.faces = t->f;
+    h.planes = t->p;
+    return h;
+}
+
+/* ---------------------------------------------------------------- manifold.cpp:57-69 */
+typedef struct { V3 point, normal; float separation; } WPoint;
+static WPoint world_point(const b3o_mpoint* p, float rA, const XF* xfA, float rB, const XF* xfB) {
+    WPoint w;
+    V3 nA = mul_mv(rot(xfA), p->localNormal1);
+    V3 cA = xf_mul(xfA, p->localPoint1);
+    V3 cB = xf_mul(xfB, p->localPoint2);
+    V3 pA = add(cA, scale(rA, nA));
+    V3 pB = sub(cB, scale(rB, nA));
+    w.point = scale(0.5f, add(pA, pB));
+    w.normal = nA;
+    FL(2);
+    w.separation = dot(sub(cB, cA), nA) - rA - rB;
+    return w;
+}
+
+/* ---------------------------------------------------------------- contact_cluster.cpp:202-590 */
+#define OBS_CAP 1024
+typedef struct { V3 point; uint32_t cluster, manifold, manifoldPoint; } Obs;
+typedef struct { Obs obs[OBS_CAP]; uint32_t nObs; V3 clusters[OBS_CAP]; uint32_t nClusters; } Cluster;
+
+static float dist_sq(V3 a, V3 b) { return length_sq(sub(a, b)); }                         /* vec3.h:207-211 */
+
+static uint32_t find_cluster(const Cluster* c, V3 p) {                                    /* :361-377 */
+    uint32_t best = 0;
+    float bestValue = MAX_FLOAT;
+    for (uint32_t i = 0; i < c->nClusters; ++i) {
+        float m = dist_sq(p, c->clusters[i]);
+        if (m < bestValue) { bestValue = m; best = i; }
+    }
+    return best;
+}
+
+static void add_cluster(Cluster* c, V3 centroid) {                                        /* :336-359 */
+    if (c->nClusters == 0) { c->clusters[c->nClusters++] = centroid; return; }
+    uint32_t b = find_cluster(c, centroid);
+    const float kTol = 0.05f;
+    FL(1);
+    if (dist_sq(centroid, c->clusters[b]) <= kTol * kTol) {
+        c->clusters[b] = add(c->clusters[b], centroid);
+        normalize_in_place(&c->clusters[b]);
+        return;
+    }
+    c->clusters[c->nClusters++] = centroid;
+}
+
+static void initialize_clusters(Cluster* c) {                                             /* :212-334 */
+    if (c->nObs <= 3) {
+        for (uint32_t i = 0; i < c->nObs; ++i) add_cluster(c, c->obs[i].point);
+        return;
+    }
+    static char chosen[OBS_CAP];
+    memset(chosen, 0, c->nObs);
+    add_cluster(c, c->obs[0].point);
+    chosen[0] = 1;
+    {
+        V3 A = c->clusters[0];
+        uint32_t index = 0;
+        float mx = -MAX_FLOAT;
+        for (uint32_t i = 0; i < c->nObs; ++i) {
+            if (chosen[i]) continue;
+            float dd = dist_sq(A, c->obs[i].point);
+            if (dd > mx) { mx = dd; index = i; }
+        }
+        add_cluster(c, c->obs[index].point);
+        chosen[index] = 1;
+    }
+    if (c->nClusters == 1) {
+        V3 A = c->clusters[0];
+        uint32_t index = 0;
+        float mx = -MAX_FLOAT;
+        for (uint32_t i = 0; i < c->nObs; ++i) {
+            if (chosen[i]) continue;
+            float dd = dist_sq(A, c->obs[i].point);
+            if (dd > mx) { mx = dd; index = i; }
+        }
+        add_cluster(c, c->obs[index].point);
+        chosen[index] = 1;
+        return;
+    }
+    {
+        V3 A = c->clusters[0], B = c->clusters[1];
+        uint32_t index = 0;
+        float mx = -MAX_FLOAT;
+        for (uint32_t i = 0; i < c->nObs; ++i) {
+            if (chosen[i]) continue;
+            V3 C = c->obs[i].point;
+            /* b3ClosestPointOnSegment (geometry.h:174-193) */
+            float w[3];
+            barycentric2(w, A, B, C);
+            V3 Q;
+            if (w[1] <= 0.0f) Q = A;
+            else if (w[0] <= 0.0f) Q = B;
+            else {
+                FL(1);
+                float s = 1.0f / w[2];
+                FL(2);
+                float wA = s * w[0], wB = s * w[1];
+                Q = add(scale(wA, A), scale(wB, B));
+            }
+            float dd = dist_sq(C, Q);
+            if (dd > mx) { mx = dd; index = i; }
+        }
+        add_cluster(c, c->obs[index].point);
+        chosen[index] = 1;
+    }
+}
+
+static void cluster_solve(Cluster* c) {                                                   /* :392-473 */
+    initialize_clusters(c);
+    for (uint32_t iter = 0; iter < 10; ++iter) {
+        for (uint32_t i = 0; i < c->nObs; ++i) c->obs[i].cluster = find_cluster(c, c->obs[i].point);
+        for (uint32_t i = 0; i < c->nClusters; ++i) {
+            V3 centroid = v3(0.0f, 0.0f, 0.0f);
+            uint32_t count = 0;
+            for (uint32_t j = 0; j < c->nObs; ++j)
+                if (c->obs[j].cluster == i) { centroid = add(centroid, c->obs[j].point); ++count; }
+            if (count > 0) {
+                /* centroid /= float(count): multiply by the reciprocal (vec3.h:70-76) */
+                FL(4);
+                float s = 1.0f / (float)count;
+                centroid = v3(centroid.x * s, centroid.y * s, centroid.z * s);
+                c->clusters[i] = centroid;
+            }
+        }
+    }
+    uint32_t used = 0;
+    for (uint32_t i = 0; i < c->nClusters; ++i) {
+        uint32_t cnt = 0;
+        for (uint32_t j = 0; j < c->nObs; ++j) if (c->obs[j].cluster == i) ++cnt;
+        if (cnt > 0) {
+            for (uint32_t j = 0; j < c->nObs; ++j) if (c->obs[j].cluster == i) c->obs[j].cluster = used;
+            c->clusters[used++] = c->clusters[i];
+        }
+    }
+    c->nClusters = used;
+}
+
+/* b3ClusterSolver::Run (contact_cluster.cpp:475-590).  Returns the manifold count. */
+static uint32_t cluster_run(b3o_manifold out[3], const b3o_manifold* in, uint32_t numIn,
+                            const XF* xfA, float rA, const XF* xfB, float rB) {
+    static Cluster c;
+    c.nObs = 0;
+    c.nClusters = 0;
+    for (uint32_t i = 0; i < numIn; ++i)
+        for (uint32_t j = 0; j < in[i].pointCount; ++j) {
+            WPoint w = world_point(in[i].points + j, rA, xfA, rB, xfB);
+            if (c.nObs < OBS_CAP) {
+                c.obs[c.nObs].point = w.normal;
+                c.obs[c.nObs].cluster = 0xFFFFFFFFu;
+                c.obs[c.nObs].manifold = i;
+                c.obs[c.nObs].manifoldPoint = j;
+                ++c.nObs;
+            }
+        }
+    cluster_solve(&c);
+    uint32_t numOut = 0;
+    static CPoly poly, quad;
+    for (uint32_t i = 0; i < c.nClusters; ++i) {
+        V3 normal = v3(0.0f, 0.0f, 0.0f);
+        poly.count = 0;
+        for (uint32_t j = 0; j < c.nObs; ++j) {
+            if (c.obs[j].cluster != i) continue;
+            WPoint w = world_point(in[c.obs[j].manifold].points + c.obs[j].manifoldPoint, rA, xfA, rB, xfB);
+            if (poly.count < POLY_CAP) {
+                poly.v[poly.count].position = w.point;
+                poly.v[poly.count].clipIndex = j;
+                ++poly.count;
+            }
+            normal = add(normal, c.obs[j].point);
+        }
+        if (poly.count == 0) continue;
+        normalize_in_place(&normal);
+        b3o_manifold* m = out + numOut++;
+        uint32_t minIndex = 0;
+        float minSep = MAX_FLOAT;
+        for (uint32_t j = 0; j < poly.count; ++j) {
+            const Obs* o = c.obs + poly.v[j].clipIndex;
+            WPoint w = world_point(in[o->manifold].points + o->manifoldPoint, rA, xfA, rB, xfB);
+            if (w.separation < minSep) { minIndex = j; minSep = w.separation; }
+            poly.v[j].position = add(poly.v[j].position, scale(w.separation, normal));
+        }
+        reduce_polygon(&quad, &poly, normal, minIndex);
+        for (uint32_t j = 0; j < quad.count; ++j) {
+            const Obs* o = c.obs + quad.v[j].clipIndex;
+            m->points[j] = in[o->manifold].points[o->manifoldPoint];
+        }
+        m->pointCount = quad.count;
+    }
+    return numOut;
+}
+
+/* ---------------------------------------------------------------- mesh_contact.cpp:30-275 */
+/* A freshly created b3MeshContact between shape A and the mesh: fat AABB of A in the mesh
+ * frame extended by B3_AABB_EXTENSION (mesh_contact.cpp:30-60), FindNewPairs through the
+ * static tree (:140-189, all caches empty), then Collide (:217-275). */
+void b3o_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh, uint32_t n,
+                      const uint32_t* shapeA, const b3o_xf* xfs, uint32_t cap, uint32_t* triCounts,
+                      uint32_t* triLists, b3o_manifold* out, uint32_t* manifoldCounts) {
+    const TNode* nodes = (const TNode*)mesh->nodes;
+    static uint32_t tris[65536];
+    static b3o_manifold temp[65536];
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3o_shape* sa = shapes + shapeA[i];
+        const XF* xfA = xfs + 2 * (size_t)i;
+        const XF* xfB = xfA + 1;
+        XF xf = xf_mulT_xf(xfB, xfA);
+        AABB box = shape_aabb(pool, sa, &xf);
+        aabb_extend(&box, 0.2f);  /* B3_AABB_EXTENSION (settings.h:54) */
+        uint32_t found = tree_query(nodes, mesh->nodeCount, &box, tris, 65536);
+        if (found > 65536) found = 65536;
+        triCounts[i] = found;
+        for (uint32_t k = 0; k < found && k < cap; ++k) triLists[(size_t)i * cap + k] = tris[k];
+        for (uint32_t k = 0; k < found; ++k) {
+            uint32_t ti = tris[k];
+            const uint32_t* t = mesh->triangles + 3 * (size_t)ti;
+            V3 a = v3(mesh->vertices[3 * t[0]], mesh->vertices[3 * t[0] + 1], mesh->vertices[3 * t[0] + 2]);
+            V3 b = v3(mesh->vertices[3 * t[1]], mesh->vertices[3 * t[1] + 1], mesh->vertices[3 * t[1] + 2]);
+            V3 c = v3(mesh->vertices[3 * t[2]], mesh->vertices[3 * t[2] + 1], mesh->vertices[3 * t[2] + 2]);
+            TriRec rec;
+            V3 centroid;
+            Hull th = triangle_hull(&rec, a, b, c, &centroid);
+            b3o_manifold* m = temp + k;
+            memset(m, 0, sizeof(*m));
+            static const QT qid = {0.0f, 0.0f, 0.0f, 1.0f};
+            if (sa->type == B3O_HULL) {
+                Hull h1 = hull_at(pool, sa->hull);
+                b3o_feature_cache fc;
+                fc.state = B3O_EMPTY;
+                fc.type = 0; fc.index1 = 0; fc.index2 = 0;
+                collide_hulls_cached(m, xfA, &h1, sa->radius, xfB, &th, 0.0f, &fc, qid, qid, qid, qid);
+            } else {
+                Round r1 = round_of(sa);
+                PHASE(1);
+                if (sa->type == B3O_SPHERE) collide_sphere_hull(m, xfA, &r1, xfB, &th, 0.0f);
+                else collide_capsule_hull(m, xfA, &r1, xfB, &th, 0.0f);
+            }
+            for (uint32_t j = 0; j < m->pointCount; ++j) m->points[j].triangleKey = ti;
+        }
+        PHASE(2);
+        b3o_manifold* o = out + 3 * (size_t)i;
+        memset(o, 0, 3 * sizeof(b3o_manifold));
+        manifoldCounts[i] = cluster_run(o, temp, found, xfA, sa->radius, xfB, 0.0f);
+    }
+}
+
+/* Triangle AABBs (b3Mesh::GetTriangleAABB, mesh.h:80-86) and the static tree over them. */
+void b3o_mesh_build_tree(const float* vertices, uint32_t triangleCount, const uint32_t* triangles, float* nodesOut) {
+    AABB* boxes = (AABB*)malloc(sizeof(AABB) * (size_t)triangleCount);
+    for (uint32_t i = 0; i < triangleCount; ++i) {
+        const uint32_t* t = triangles + 3 * (size_t)i;
+        V3 a = v3(vertices[3 * t[0]], vertices[3 * t[0] + 1], vertices[3 * t[0] + 2]);
+        V3 b = v3(vertices[3 * t[1]], vertices[3 * t[1] + 1], vertices[3 * t[1] + 2]);
+        V3 c = v3(vertices[3 * t[2]], vertices[3 * t[2] + 1], vertices[3 * t[2] + 2]);
+        boxes[i].lo = vmin(a, vmin(b, c));                                                 /* aabb3.h:64-68 */
+        boxes[i].hi = vmax(a, vmax(b, c));
+    }
+    b3o_tree_build((const float*)boxes, triangleCount, nodesOut);
+    free(boxes);
+}
 /* ---------------------------------------------------------------- entry points */
 static void fresh_cache(b3o_convex_cache* c) {
     memset(c, 0, sizeof(*c));

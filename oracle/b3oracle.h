@@ -103,6 +103,35 @@ void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
 void b3o_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hullPairs,
                       const b3o_xf* xfs, uint32_t* out);
 
+/* A b3Mesh (mesh.h:41-99): vertices [V][3], triangles [T][3] (v1, v2, v3) and the
+ * b3StaticTree nodes [2T-1][8] = aabb lower xyz, upper xyz, child1, child2|index
+ * (static_tree.h:62-77; child1 = 0xFFFFFFFF marks a leaf). */
+typedef struct {
+    uint32_t vertexCount;
+    const float* vertices;
+    uint32_t triangleCount;
+    const uint32_t* triangles;
+    uint32_t nodeCount;
+    const float* nodes;
+} b3o_mesh;
+
+/* Static tree over the triangle AABBs (static_tree.cpp:42-180); nodesOut holds 2T-1 nodes. */
+void b3o_mesh_build_tree(const float* vertices, uint32_t triangleCount, const uint32_t* triangles, float* nodesOut);
+
+/* n freshly created mesh contacts (b3MeshContact, mesh_contact.cpp): shape A = shapes[shapeA[i]],
+ * xfs[2i] = body A, xfs[2i+1] = mesh body.  Midphase triangle lists (DFS order) go to
+ * triLists[i*cap ...] with their full count in triCounts[i]; the clustered output is
+ * out[3i .. 3i+2] with manifoldCounts[i] manifolds. */
+void b3o_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh, uint32_t n,
+                      const uint32_t* shapeA, const b3o_xf* xfs, uint32_t cap, uint32_t* triCounts,
+                      uint32_t* triLists, b3o_manifold* out, uint32_t* manifoldCounts);
+void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh, uint32_t n,
+                        const uint32_t* shapeA, const b3o_xf* xfs, uint32_t cap, uint32_t* triCounts,
+                        uint32_t* triLists, b3o_manifold* out, uint32_t* manifoldCounts);
+/* The reference's own tree: b3Mesh::BuildTree, nodes copied out in the same 8-word layout. */
+void b3ref_mesh_build_tree(const float* vertices, uint32_t vertexCount, uint32_t triangleCount,
+                           const uint32_t* triangles, float* nodesOut);
+
 /* FP32 ops executed by the last b3o_collide_pairs call (liboracle_count.so only). */
 uint64_t b3o_flop_count(void);
 /* Per phase: 0 feature-cache read/rebuild, 1 separation queries, 2 contact building. */

@@ -21,6 +21,10 @@
 #include <bounce/dynamics/contacts/manifold.h>
 #include <bounce/dynamics/contacts/collide/collide.h>
 #include <bounce/collision/shapes/hull.h>
+#include <bounce/collision/shapes/mesh.h>
+#include <bounce/dynamics/shapes/mesh_shape.h>
+#include <bounce/dynamics/contacts/mesh_contact.h>
+#include <bounce/dynamics/world.h>
 #include <bounce/common/draw.h>
 #undef private
 #undef protected
@@ -191,4 +195,71 @@ extern "C" void b3ref_hull_queries(const b3o_hull_pool* pool, uint32_t n, const 
         o[4] = e.index1; o[5] = e.index2; std::memcpy(o + 6, &e.separation, 4);
         o[7] = 0;
     }
+}
+
+namespace {
+
+void make_mesh(b3Mesh& mesh, const float* vertices, uint32_t vertexCount, uint32_t triangleCount,
+               const uint32_t* triangles) {
+    mesh.vertexCount = vertexCount;
+    mesh.vertices = (b3Vec3*)vertices;
+    mesh.triangleCount = triangleCount;
+    mesh.triangles = (b3Triangle*)triangles;
+    mesh.BuildTree();
+}
+
+void copy_nodes(const b3Mesh& mesh, float* nodesOut) {
+    static_assert(sizeof(b3StaticTree::b3Node) == 32, "static tree node layout");
+    std::memcpy(nodesOut, mesh.tree.m_nodes, 32ull * mesh.tree.m_nodeCount);
+}
+
+}  // namespace
+
+extern "C" void b3ref_mesh_build_tree(const float* vertices, uint32_t vertexCount, uint32_t triangleCount,
+                                      const uint32_t* triangles, float* nodesOut) {
+    b3Mesh mesh;
+    make_mesh(mesh, vertices, vertexCount, triangleCount, triangles);
+    copy_nodes(mesh, nodesOut);
+}
+
+// Freshly created b3MeshContacts (contact_manager.cpp:246-268 + :130-141): construct,
+// FindNewPairs, Collide, and read the triangle cache list and clustered manifolds.
+extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* m,
+                                   uint32_t n, const uint32_t* shapeA, const b3o_xf* xfs, uint32_t cap,
+                                   uint32_t* triCounts, uint32_t* triLists, b3o_manifold* out,
+                                   uint32_t* manifoldCounts) {
+    Scene scene;
+    build_hulls(pool, scene.hulls);
+    b3Mesh mesh;
+    make_mesh(mesh, m->vertices, m->vertexCount, m->triangleCount, m->triangles);
+    b3World* world = new b3World();
+    b3BodyDef bdA;
+    bdA.type = e_dynamicBody;
+    b3Body* bodyA = world->CreateBody(bdA);
+    b3BodyDef bdB;
+    b3Body* bodyB = world->CreateBody(bdB);
+    b3MeshShape meshShape;
+    meshShape.m_mesh = &mesh;
+    meshShape.m_body = bodyB;
+    for (uint32_t i = 0; i < n; ++i) {
+        b3Shape* A = make_shape(shapes[shapeA[i]], scene.hulls);
+        scene.shapes.push_back(A);
+        A->m_body = bodyA;
+        std::memcpy(&bodyA->m_xf, xfs + 2 * (size_t)i, sizeof(b3Transform));
+        std::memcpy(&bodyB->m_xf, xfs + 2 * (size_t)i + 1, sizeof(b3Transform));
+        b3MeshContact* mc = new b3MeshContact(A, &meshShape);
+        mc->m_pair.shapeA = A;
+        mc->m_pair.shapeB = &meshShape;
+        mc->FindNewPairs();
+        triCounts[i] = mc->m_triangleCount;
+        for (uint32_t k = 0; k < mc->m_triangleCount && k < cap; ++k) triLists[(size_t)i * cap + k] = mc->m_triangles[k].index;
+        mc->Collide();
+        manifoldCounts[i] = mc->m_manifoldCount;
+        for (uint32_t k = 0; k < 3; ++k) {
+            if (k < mc->m_manifoldCount) to_out(mc->m_manifolds[k], out + 3 * (size_t)i + k);
+            else std::memset(out + 3 * (size_t)i + k, 0, sizeof(b3o_manifold));
+        }
+        delete mc;
+    }
+    delete world;
 }

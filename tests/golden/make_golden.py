@@ -91,5 +91,24 @@ def main():
           (mout["pointCount"] > 0).mean())
 
 
+def make_mesh_fixture():
+    """5. Shapes resting on a 48 x 48 heightfield (midphase, per-triangle narrowphase,
+    clustering), through b3MeshContact itself."""
+    from util import mesh_workload
+    ref = oracle_lib.reference()
+    lib = load_library()
+    pool, shapes, v, t, xfs = mesh_workload(lib, 77, 48, 48, 400)
+    nodes = oracle_lib.build_tree(ref.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    counts, lists, out, mcount = oracle_lib.collide_mesh(ref.lib, "b3ref_", pool, shapes, v, t, nodes, sa, xfs)
+    np.savez_compressed(os.path.join(HERE, "mesh_contacts.npz"), desc=pool.desc, centroids=pool.centroids,
+                        vertices=pool.vertices, edges=pool.edges, faces=pool.faces, planes=pool.planes,
+                        shapes=shapes, pairs=np.zeros((0, 2), np.uint32), xfs=xfs, shape_a=sa,
+                        mesh_vertices=v, mesh_triangles=t, mesh_nodes=nodes, tri_counts=counts,
+                        tri_lists=lists, out=out, manifold_counts=mcount)
+    print("mesh fixture: manifold counts", np.bincount(mcount))
+
+
 if __name__ == "__main__":
     main()
+    make_mesh_fixture()

@@ -86,3 +86,53 @@ def reference():
     """The compiled reference, or None where oracle/_ref was not built."""
     lib = _load(os.path.join(ORACLE_DIR, "_ref", "libb3ref.so"))
     return None if lib is None else Checker(lib, "b3ref_")
+
+
+class MeshStruct(ctypes.Structure):
+    """b3o_mesh."""
+    _fields_ = [
+        ("vertexCount", ctypes.c_uint32), ("vertices", ctypes.c_void_p),
+        ("triangleCount", ctypes.c_uint32), ("triangles", ctypes.c_void_p),
+        ("nodeCount", ctypes.c_uint32), ("nodes", ctypes.c_void_p),
+    ]
+
+
+def build_tree(lib, prefix, vertices, triangles):
+    """Static-tree nodes [2T-1][8] (float32 view; child words are u32)."""
+    vertices = np.ascontiguousarray(vertices, np.float32)
+    triangles = np.ascontiguousarray(triangles, np.uint32)
+    nodes = np.zeros((2 * len(triangles) - 1, 8), np.float32)
+    if prefix == "b3ref_":
+        f = lib.b3ref_mesh_build_tree
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+        f(ptr(vertices), len(vertices), len(triangles), ptr(triangles), ptr(nodes))
+    else:
+        f = lib.b3o_mesh_build_tree
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+        f(ptr(vertices), len(triangles), ptr(triangles), ptr(nodes))
+    return nodes
+
+
+def collide_mesh(lib, prefix, pool, shapes, vertices, triangles, nodes, shape_a, xfs, cap=256):
+    """Fresh mesh contacts: (triangle counts, triangle lists [n][cap], manifolds [n][3],
+    manifold counts)."""
+    vertices = np.ascontiguousarray(vertices, np.float32)
+    triangles = np.ascontiguousarray(triangles, np.uint32)
+    nodes = np.ascontiguousarray(nodes, np.float32)
+    shape_a = np.ascontiguousarray(shape_a, np.uint32)
+    n = len(shape_a)
+    xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+    m = MeshStruct(len(vertices), vertices.ctypes.data, len(triangles), triangles.ctypes.data, len(nodes),
+                   nodes.ctypes.data)
+    counts = np.zeros(n, np.uint32)
+    lists = np.zeros((n, cap), np.uint32)
+    out = np.zeros((n, 3), MANIFOLD_DTYPE)
+    mcount = np.zeros(n, np.uint32)
+    f = getattr(lib, prefix + "collide_mesh")
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32] + \
+        [ctypes.c_void_p] * 4
+    ps = pool.struct()
+    f(ctypes.byref(ps), ptr(np.ascontiguousarray(shapes)), ctypes.byref(m), n, ptr(shape_a), ptr(xfs), cap,
+      ptr(counts), ptr(lists), ptr(out), ptr(mcount))
+    return counts, lists, out, mcount

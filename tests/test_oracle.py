@@ -193,3 +193,79 @@ def test_sphere_hull_gjk_distance_fp64(lib, oracle):
 def test_oracle_golden_convex(oracle, name):
     g = load_golden(name)
     assert_manifolds_equal(oracle.collide(g["pool"], g["shapes"], g["pairs"], g["xfs"]), g["out"], name)
+
+
+# ---------------------------------------------------------------- mesh path
+
+def test_static_tree_equals_reference(lib, reference):
+    """The oracle's and the product's host tree builders reproduce b3StaticTree::Build node
+    for node (numbering, boxes, leaf triangles), including the full 256 x 256 heightfield."""
+    import ctypes
+    import oracle_lib
+    from bounce_amd.types import ptr
+    from util import grid_mesh
+    o = oracle_lib.oracle()
+    for H, W in ((4, 4), (32, 32), (64, 48), (256, 256)):
+        v, t = grid_mesh(H, W, seed=H)
+        want = oracle_lib.build_tree(reference.lib, "b3ref_", v, t).view(np.uint32)
+        assert np.array_equal(oracle_lib.build_tree(o.lib, "b3o_", v, t).view(np.uint32), want)
+        got = np.zeros((2 * len(t) - 1, 8), np.float32)
+        f = lib.b3amd_build_static_tree
+        f.argtypes = [ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p]
+        assert f(len(v), ptr(v), len(t), ptr(t), ptr(got)) == 0
+        assert np.array_equal(got.view(np.uint32), want), (H, W)
+
+
+def test_tree_query_set_is_brute_force_overlap(lib):
+    """The midphase triangle set equals the brute-force set of triangles whose AABB
+    overlaps the fat AABB (the order is the tree's DFS order)."""
+    import oracle_lib
+    from util import grid_mesh, mesh_workload
+    o = oracle_lib.oracle()
+    pool, shapes, v, t, xfs = mesh_workload(lib, 8, 32, 32, 60)
+    nodes = oracle_lib.build_tree(o.lib, "b3o_", v, t)
+    counts, lists, _, _ = oracle_lib.collide_mesh(o.lib, "b3o_", pool, shapes, v, t, nodes,
+                                                  np.arange(len(shapes), dtype=np.uint32), xfs)
+    tri_lo = v[t].min(1)
+    tri_hi = v[t].max(1)
+    for i in range(len(shapes)):
+        got = set(lists[i, :counts[i]].tolist())
+        assert len(got) == counts[i]
+        # shape AABB in the mesh frame (mesh at identity), fattened by 0.2
+        s = shapes[i]
+        R, p = xf_matrix(xfs[i, 0])
+        if s["type"] == 2:
+            pts = pool.hull(int(s["hull"]))[0].astype(np.float64) @ R.T + p
+            lo, hi = pts.min(0) - s["radius"], pts.max(0) + s["radius"]
+        else:
+            c = [R @ s["v"][:3] + p] + ([R @ s["v"][3:] + p] if s["type"] == 1 else [])
+            lo, hi = np.min(c, 0) - s["radius"], np.max(c, 0) + s["radius"]
+        lo, hi = lo - 0.2, hi + 0.2
+        want = set(np.where((tri_lo <= hi).all(1) & (tri_hi >= lo).all(1))[0].tolist())
+        # boundary ties in FP32 vs FP64 aside, the sets agree
+        assert len(got ^ want) <= 1, (i, got ^ want)
+
+
+def test_oracle_mesh_equals_reference(lib, oracle, reference):
+    import oracle_lib
+    from util import mesh_workload
+    pool, shapes, v, t, xfs = mesh_workload(lib, 9, 48, 48, 600)
+    nodes = oracle_lib.build_tree(reference.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs)
+    cr = oracle_lib.collide_mesh(reference.lib, "b3ref_", pool, shapes, v, t, nodes, sa, xfs)
+    np.testing.assert_array_equal(co[0], cr[0])
+    np.testing.assert_array_equal(co[1], cr[1])
+    np.testing.assert_array_equal(co[3], cr[3])
+    assert (cr[3] >= 2).sum() > 50  # several clustered manifolds per contact
+    assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "clustered mesh manifolds")
+
+
+def test_oracle_mesh_golden(oracle):
+    import oracle_lib
+    g = load_golden("mesh_contacts.npz")
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", g["pool"], g["shapes"], g["mesh_vertices"], g["mesh_triangles"],
+                                 g["mesh_nodes"], g["shape_a"], g["xfs"])
+    np.testing.assert_array_equal(co[0], g["tri_counts"])
+    np.testing.assert_array_equal(co[3], g["manifold_counts"])
+    assert_manifolds_equal(co[2].reshape(-1), g["out"].reshape(-1), "golden mesh manifolds")

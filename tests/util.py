@@ -262,3 +262,67 @@ def point_hull_distance(pool, h, R, t, p):
         q = closest_point_triangle(p, V[e[h0, 0]], V[e[h1, 0]], V[e[h2, 0]])
         best = min(best, np.linalg.norm(q - p))
     return best
+
+
+def grid_mesh(H, W, cell=0.5, amplitude=2.0, seed=0, lattice=8):
+    """Heightfield on a (H+1) x (W+1) vertex grid triangulated as b3GridMesh
+    (grid_mesh.h:57-77), heights from seeded value noise with smoothstep interpolation."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(-1.0, 1.0, size=(H // lattice + 2, W // lattice + 2))
+    verts = np.zeros(((H + 1) * (W + 1), 3), np.float32)
+    for i in range(H + 1):
+        for j in range(W + 1):
+            u, v = i / lattice, j / lattice
+            i0, j0 = int(u), int(v)
+            fu, fv = u - i0, v - j0
+            su, sv = fu * fu * (3 - 2 * fu), fv * fv * (3 - 2 * fv)
+            h = (g[i0, j0] * (1 - su) * (1 - sv) + g[i0 + 1, j0] * su * (1 - sv) +
+                 g[i0, j0 + 1] * (1 - su) * sv + g[i0 + 1, j0 + 1] * su * sv)
+            verts[i * (W + 1) + j] = ((j - 0.5 * W) * cell, amplitude * h, (i - 0.5 * H) * cell)
+    tris = []
+    for i in range(H):
+        for j in range(W):
+            v1 = i * (W + 1) + j
+            v2 = (i + 1) * (W + 1) + j
+            v3 = (i + 1) * (W + 1) + (j + 1)
+            v4 = i * (W + 1) + (j + 1)
+            tris.append((v3, v2, v1))
+            tris.append((v1, v4, v3))
+    return verts, np.array(tris, np.uint32)
+
+
+def mesh_workload(lib, seed, H, W, n_shapes, gap_sigma=0.05):
+    """Spheres, capsules and 8..32-point hulls dropped over a grid mesh, each resting near
+    the surface: (pool, shapes, vertices, triangles, xfs [n][2][12] (shape, mesh))."""
+    from bounce_amd.types import SHAPE_DTYPE, SPHERE, CAPSULE, HULL
+    from bounce_amd.workloads import gen_hull_pool
+    from bounce_amd.shapes import identity_xf
+    rng = np.random.default_rng(seed)
+    verts, tris = grid_mesh(H, W, seed=seed)
+    pool = gen_hull_pool(lib, seed, 64, 8, 32, 0.1, 0.5)
+    shapes = np.zeros(n_shapes, SHAPE_DTYPE)
+    xfs = np.zeros((n_shapes, 2, 12), np.float32)
+    half = 0.5 * 0.5 * np.array([W, H])
+    for i in range(n_shapes):
+        k = rng.random()
+        s = shapes[i]
+        R = random_rotation(rng)
+        if k < 0.4:
+            s["type"], s["radius"] = SPHERE, rng.uniform(0.1, 0.5)
+            low = float(s["radius"])
+        elif k < 0.7:
+            L = rng.uniform(0.2, 1.0)
+            s["type"], s["radius"] = CAPSULE, rng.uniform(0.1, 0.3)
+            s["v"][:3], s["v"][3:] = (0, -L / 2, 0), (0, L / 2, 0)
+            low = float(s["radius"]) + abs(R[1, 1]) * L / 2
+        else:
+            s["type"], s["hull"] = HULL, rng.integers(0, pool.count)
+            v = pool.hull(int(s["hull"]))[0].astype(np.float64)
+            low = -(v @ R.T)[:, 1].min()
+        x, z = rng.uniform(-half[0] + 1, half[0] - 1), rng.uniform(-half[1] + 1, half[1] - 1)
+        near = np.linalg.norm(verts[:, [0, 2]] - (x, z), axis=1) < 0.4
+        ground = verts[near, 1].mean() if near.any() else 0.0
+        y = ground + low + rng.normal(0.0, gap_sigma) - 0.03
+        xfs[i, 0] = identity_xf((x, y, z), R)
+        xfs[i, 1] = identity_xf((0, 0, 0))
+    return pool, shapes, verts, tris, xfs
