@@ -124,6 +124,8 @@ def main():
     ap.add_argument("--pool", type=int, default=65536)
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default="auto", metavar="auto|W,K",
+                    help="SAT launch shape: autotune on the batch, or waves-per-SIMD,register-slots")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (reference for HBM counters)")
     args = ap.parse_args()
@@ -159,6 +161,15 @@ def main():
             ctx.copy_on_device(b, a, args.copy_calibration)
             a.free()
             b.free()
+        if args.tuning == "auto":
+            t = time.perf_counter()
+            shape, tms = ctx.autotune(n, dp, dx, do, repeats=1)
+            log("autotune: waves/SIMD %d, register slots %d (%.3f ms per batch) in %.1f s"
+                % (shape[0], shape[1], tms, time.perf_counter() - t))
+        else:
+            w, k = (int(x) for x in args.tuning.split(","))
+            ctx.set_sat_tuning(w, k)
+        tuning = ctx.sat_tuning()
         for _ in range(args.warmup):
             ctx.collide_device(n, dp, dx, do)
         ctx.synchronize()
@@ -170,7 +181,7 @@ def main():
         ctx.synchronize()
         t1 = time.perf_counter()
         ctx.set_profiling(False)
-        stats = {k: ctx.kernel_stats(k) for k in ("cache_hull", "sat_hull", "contact_hull")}
+        stats = {k: ctx.kernel_stats(k) for k in ("reset_counter", "cache_hull", "sat_hull", "contact_hull")}
         out = do.download(np.zeros(n, MANIFOLD_DTYPE))
     ms = (t1 - t0) * 1000.0 / args.steps
     value = n / (ms / 1000.0)
@@ -199,6 +210,8 @@ def main():
             "hull_pool": pool.count,
             "overlap_fraction": float((out["pointCount"] > 0).mean()),
             "parallelism": "single GPU",
+            "sat_launch_shape": {"waves_per_simd": tuning[0], "register_slots": tuning[1],
+                                 "chosen_by": "autotune" if args.tuning == "auto" else "flag"},
         },
         "kernels_ms": {k: (v[1] / max(1, v[0])) for k, v in stats.items()},
         "roofline": {

@@ -58,6 +58,9 @@ def load_library():
         "b3amd_upload_mesh": (i32, [vp, u32, vp, u32, vp, ctypes.POINTER(u32)]),
         "b3amd_mesh_tree": (i32, [vp, u32, ctypes.POINTER(u32), vp]),
         "b3amd_collide_mesh": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
+        "b3amd_set_sat_tuning": (i32, [vp, u32, u32]),
+        "b3amd_get_sat_tuning": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "b3amd_autotune": (i32, [vp, u32, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -190,6 +193,22 @@ class Context:
 
     def synchronize(self):
         _check(self.lib.b3amd_synchronize(self.handle))
+
+    def set_sat_tuning(self, waves_per_simd, max_slots):
+        """Launch shape of the persistent SAT kernel (b3amd_set_sat_tuning)."""
+        _check(self.lib.b3amd_set_sat_tuning(self.handle, int(waves_per_simd), int(max_slots)))
+
+    def sat_tuning(self):
+        w, k = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self.lib.b3amd_get_sat_tuning(self.handle, ctypes.byref(w), ctypes.byref(k)))
+        return w.value, k.value
+
+    def autotune(self, n, d_pairs, d_xfs, d_out, repeats=2):
+        """Times every SAT launch shape on a device batch, keeps the fastest; returns
+        ((waves_per_simd, max_slots), mean ms per batch)."""
+        ms = ctypes.c_float()
+        _check(self.lib.b3amd_autotune(self.handle, n, d_pairs.ptr, d_xfs.ptr, d_out.ptr, repeats, ctypes.byref(ms)))
+        return self.sat_tuning(), ms.value
 
     def set_profiling(self, on):
         _check(self.lib.b3amd_set_profiling(self.handle, 1 if on else 0))

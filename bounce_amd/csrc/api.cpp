@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -74,7 +75,10 @@ struct b3amd_context {
     // Per-batch scratch and host-API staging.
     uint32_t* dStage = nullptr;
     SatResult* dSat = nullptr;
+    uint32_t* dCounter = nullptr;  // tail counter of the persistent SAT kernel
     uint32_t scratchCap = 0;
+    SatTuning tuning{0u, 3u};
+    bool legacySat = false;      // the one-wave-per-pair LDS kernel, kept for comparison
     void* dIn = nullptr;
     uint64_t dInCap = 0;
 
@@ -171,12 +175,15 @@ int ensure_scratch(b3amd_context* ctx, uint32_t n) {
     if (n <= ctx->scratchCap) return B3AMD_OK;
     uint32_t cap = n < 1024 ? 1024 : n;
     int rc;
-    if ((rc = ctx->release(ctx->dStage)) || (rc = ctx->release(ctx->dSat))) return rc;
+    if ((rc = ctx->release(ctx->dStage)) || (rc = ctx->release(ctx->dSat)) || (rc = ctx->release(ctx->dCounter)))
+        return rc;
     ctx->dStage = nullptr;
     ctx->dSat = nullptr;
+    ctx->dCounter = nullptr;
     ctx->scratchCap = 0;
     if ((rc = ctx->alloc(sizeof(uint32_t) * (uint64_t)cap, (void**)&ctx->dStage))) return rc;
     if ((rc = ctx->alloc(sizeof(SatResult) * (uint64_t)cap, (void**)&ctx->dSat))) return rc;
+    if ((rc = ctx->alloc(256, (void**)&ctx->dCounter))) return rc;
     ctx->scratchCap = cap;
     return B3AMD_OK;
 }
@@ -231,6 +238,22 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     }
     uint16_t* fe = reinterpret_cast<uint16_t*>(words.data() + 4 * L.faceOff());
     for (uint32_t f = 0; f < F; ++f) fe[f] = (uint16_t)faces[f];
+    // Negated Gauss map of the hull in its own frame (sat.cpp:131-156 for hull 2):
+    // -U, -V, -E = -(Q - P) and P per unique edge; the padding records stay zero.
+    float* gm = fw + 4 * L.gaussOff();
+    for (uint32_t k = 0; k < L.E(); ++k) {
+        const uint32_t* a = edges + 8 * k;
+        const uint32_t* b = a + 4;
+        const float* P = verts + 3 * a[0];
+        const float* Q = verts + 3 * b[0];
+        const float* U = planes + 4 * a[2];
+        const float* Vn = planes + 4 * b[2];
+        const float E[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
+        float* r = gm + 12 * k;
+        r[0] = -U[0], r[1] = -U[1], r[2] = -U[2], r[3] = -Vn[0];
+        r[4] = -Vn[1], r[5] = -Vn[2], r[6] = -E[0], r[7] = -E[1];
+        r[8] = -E[2], r[9] = P[0], r[10] = P[1], r[11] = P[2];
+    }
     *layoutOut = L;
     return nullptr;
 }
@@ -282,10 +305,12 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     ca.caches = caches;
     ca.stage = ctx->dStage;
     ca.out = out;
+    ca.sat = ctx->dSat;
     SatArgs sa;
     static_cast<PairInputs&>(sa) = in;
     sa.stage = ctx->dStage;
     sa.out = ctx->dSat;
+    sa.split = SatSplit{ctx->dCounter, 0u, 0u};
     ContactArgs cta;
     static_cast<PairInputs&>(cta) = in;
     cta.sat = ctx->dSat;
@@ -293,6 +318,7 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     cta.out = out;
     hipStream_t s = ctx->stream;
     int cls = ctx->sizeClass;
+    if ((rc = timed(ctx, "reset_counter", [&] { return launch_reset_counter(ctx->dCounter, s); }))) return rc;
     if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     if (ctx->roundShapes) {
         ConvexArgs cv;
@@ -301,7 +327,12 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
         cv.out = out;
         if ((rc = timed(ctx, "convex", [&] { return launch_convex(cv, s); }))) return rc;
     }
-    if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_hull(sa, cls, s); }))) return rc;
+    if (ctx->legacySat) {
+        if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_hull(sa, cls, s); }))) return rc;
+    } else {
+        const SatTuning t = ctx->tuning;
+        if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
+    }
     if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
     return B3AMD_OK;
 }
@@ -322,6 +353,8 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
     b3amd_context* ctx = new b3amd_context();
     ctx->device = device;
     ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
+    const char* legacy = getenv("B3AMD_SAT_LEGACY");
+    ctx->legacySat = legacy && legacy[0] == '1';
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(B3AMD_ERR_DEVICE, "hipStreamCreate failed");
@@ -446,6 +479,66 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
     if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
     return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
                      reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
+}
+
+int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t max_slots) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    if (waves_per_simd > 32) return fail(B3AMD_ERR_ARGUMENT, "waves_per_simd %u > 32", waves_per_simd);
+    if (max_slots < 1 || max_slots > 3) return fail(B3AMD_ERR_ARGUMENT, "max_slots %u outside 1..3", max_slots);
+    ctx->tuning.wavesPerSimd = waves_per_simd;
+    ctx->tuning.maxSlots = max_slots;
+    return B3AMD_OK;
+}
+
+int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* max_slots) {
+    if (!ctx || !waves_per_simd || !max_slots) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    *waves_per_simd = ctx->tuning.wavesPerSimd;
+    *max_slots = ctx->tuning.maxSlots;
+    return B3AMD_OK;
+}
+
+int b3amd_autotune(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                   b3amd_manifold* d_out, uint32_t repeats, float* best_ms) {
+    if (!ctx || !n || !d_pairs || !d_xfs || !d_out) return fail(B3AMD_ERR_ARGUMENT, "null or empty batch");
+    if (repeats == 0) repeats = 1;
+    static const uint32_t kWaves[] = {6, 5, 4, 3};
+    const SatTuning saved = ctx->tuning;
+    const bool savedProfiling = ctx->profiling;
+    ctx->profiling = false;
+    hipEvent_t e0 = ctx->event(), e1 = ctx->event();
+    if (!e0 || !e1) return fail(B3AMD_ERR_DEVICE, "hipEventCreate failed");
+    SatTuning best = saved;
+    float bestMs = -1.0f;
+    int rc = B3AMD_OK;
+    for (uint32_t slots = 1; slots <= 3 && rc == B3AMD_OK; ++slots) {
+        for (uint32_t w : kWaves) {
+            ctx->tuning = SatTuning{w, slots};
+            // One untimed batch, then the timed ones; no feature caches, so every hull
+            // pair runs the full SAT each time.
+            if ((rc = b3amd_collide_device(ctx, n, d_pairs, d_xfs, nullptr, nullptr, d_out))) break;
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) { rc = fail(B3AMD_ERR_DEVICE, "hipEventRecord"); break; }
+            for (uint32_t r = 0; r < repeats && rc == B3AMD_OK; ++r)
+                rc = b3amd_collide_device(ctx, n, d_pairs, d_xfs, nullptr, nullptr, d_out);
+            if (rc) break;
+            float ms = 0.0f;
+            if (hipEventRecord(e1, ctx->stream) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                hipEventElapsedTime(&ms, e0, e1) != hipSuccess) {
+                rc = fail(B3AMD_ERR_DEVICE, "event timing failed");
+                break;
+            }
+            ms /= (float)repeats;
+            if (bestMs < 0.0f || ms < bestMs) {
+                bestMs = ms;
+                best = ctx->tuning;
+            }
+        }
+    }
+    ctx->freeEvents.push_back(e0);
+    ctx->freeEvents.push_back(e1);
+    ctx->profiling = savedProfiling;
+    ctx->tuning = rc == B3AMD_OK ? best : saved;
+    if (rc == B3AMD_OK && best_ms) *best_ms = bestMs;
+    return rc;
 }
 
 int b3amd_upload_mesh(b3amd_context* ctx, uint32_t vertexCount, const float* vertices, uint32_t triangleCount,

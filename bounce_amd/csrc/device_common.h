@@ -13,6 +13,26 @@ namespace b3amd {
 
 __device__ __forceinline__ V3 ld3(const float4& q) { return V3{q.x, q.y, q.z}; }
 
+// (separation, key) arg-max over the wave: larger separation wins, equal separations
+// keep the smaller key.  Sequential strict-'>' scans in index order (sat.cpp:47,158)
+// pick exactly this element.
+__device__ __forceinline__ void wave_argmax(float& s, uint32_t& key) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        float os = __shfl_xor(s, off, 64);
+        uint32_t ok = __shfl_xor(key, off, 64);
+        if (os > s || (os == s && ok < key)) {
+            s = os;
+            key = ok;
+        }
+    }
+}
+
+// Rank of this lane among the set lanes of a wave mask.
+__device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
 __device__ __forceinline__ Xf load_xf(const float* p) {
     Xf x;
     x.R.c0 = V3{p[0], p[1], p[2]};

@@ -10,7 +10,14 @@
 //   then          HE half-edges, one u32 each: origin | face << 8 | next << 16; the twin
 //                 of half-edge h is h ^ 1 (required by sat.cpp:123-128, checked at upload)
 //   then          F u16 first half-edge per face (b3Face::edge)
-// A kernel stages the float4 sections into LDS with one coalesced pass per wave.
+//   then          the negated Gauss map, 3 units per unique edge k padded to a multiple
+//                 of 4 edges with zero records: {-U.xyz, -V.x}, {-V.yz, -E.xy},
+//                 {-E.z, P.xyz} with U, V the normals of faces(2k), face(2k+1), P the
+//                 origin of 2k and E = origin(2k+1) - P, in the hull's own frame.  These
+//                 are exactly the operands hull 2 contributes to b3IsMinkowskiFace and
+//                 b3Project in the edge query (sat.cpp:131-156), so the SAT kernel
+//                 broadcasts them from the scalar cache instead of recomputing them.
+//                 A zero record fails the Gauss-map test (all four dots are 0).
 #ifndef B3AMD_DEVICE_HULL_H
 #define B3AMD_DEVICE_HULL_H
 
@@ -32,7 +39,9 @@ struct HullLayout {
     B3_HD uint32_t uedgeOff() const { return 2 + V + F; }
     B3_HD uint32_t halfOff() const { return uedgeOff() + (E() + 3) / 4; }
     B3_HD uint32_t faceOff() const { return halfOff() + (HE + 3) / 4; }
-    B3_HD uint32_t units() const { return faceOff() + (F + 7) / 8; }
+    B3_HD uint32_t gaussOff() const { return faceOff() + (F + 7) / 8; }
+    B3_HD uint32_t gaussEdges() const { return (E() + 3) & ~3u; }
+    B3_HD uint32_t units() const { return gaussOff() + 3 * gaussEdges(); }
 };
 
 // Read-only view of one hull record in global memory.

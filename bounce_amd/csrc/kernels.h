@@ -80,16 +80,34 @@ struct PairInputs {
 constexpr uint32_t kFlagInvalidPair = 2u;
 constexpr uint32_t kStageConvex = 5u;  // sphere/capsule pair: handled by the convex kernel
 
+// Work split of the persistent SAT kernel: wave w of G takes pairs w, w + G, ... below
+// staticEnd (an interleave gives every wave the same mix of hull sizes), then chunks of
+// `chunk` pairs from a device counter until the batch is done.  Only the tail is handed
+// out dynamically: device-scope atomics on one address serialise across the XCDs.
+struct SatSplit {
+    uint32_t* counter;       // pairs handed out past staticEnd; zeroed before each batch
+    uint32_t staticEnd;
+    uint32_t chunk;
+};
+
 struct CacheArgs : PairInputs {
     const float* sweeps;     // [n][4][4] or null
     ConvexCache* caches;     // [n] or null
     uint32_t* stage;         // [n] out
     Manifold* out;           // [n]
+    SatResult* sat;          // [n] out: stage kStageSkip for pairs that need no SAT
 };
 
 struct SatArgs : PairInputs {
     const uint32_t* stage;   // [n] from the cache kernel
     SatResult* out;          // [n]
+    SatSplit split;
+};
+
+// Launch shape of the persistent SAT kernel (chosen by the autotuner).
+struct SatTuning {
+    uint32_t wavesPerSimd;   // persistent waves per SIMD and register budget, 0 = 5
+    uint32_t maxSlots;       // edge-query register slots per lane: 1, 2 or 3
 };
 
 struct ContactArgs : PairInputs {
@@ -145,9 +163,13 @@ hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
+hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // sizeClass 0: every hull of the batch has V <= 64, F <= 128, HE/2 <= 192; 1: the limits.
 hipError_t launch_sat_hull(const SatArgs& a, int sizeClass, hipStream_t s);
+// Persistent SAT over the pairs the cache kernel marked; any hull size within the record
+// limits.  Fills a.split (the counter must be zero).
+hipError_t launch_sat_persistent(SatArgs a, const SatTuning& t, hipStream_t s);
 hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s);
 
 }  // namespace b3amd

@@ -20,22 +20,6 @@
 
 namespace b3amd {
 
-// ------------------------------------------------------------------ wave reductions
-
-// (separation, key) arg-max: larger separation wins, equal separations keep the smaller
-// key.  Sequential strict-'>' scans in index order (sat.cpp:47,158) pick exactly this.
-__device__ __forceinline__ void wave_argmax(float& s, uint32_t& key) {
-#pragma unroll
-    for (int off = 32; off >= 1; off >>= 1) {
-        float os = __shfl_xor(s, off, 64);
-        uint32_t ok = __shfl_xor(key, off, 64);
-        if (os > s || (os == s && ok < key)) {
-            s = os;
-            key = ok;
-        }
-    }
-}
-
 // ------------------------------------------------------------------ SAT kernel
 
 // Ring buffer of Gauss-map hits per wave: at most 64 pending before an iteration plus
@@ -412,9 +396,8 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
 // ------------------------------------------------------------------ feature-cache kernel
 
 // b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush.
-__global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
+// Returns true when the pair needs the full SAT.
+__device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
     const uint32_t cls = a.classify(p);
     if (cls != 1u) {
         if (cls == 0u) {
@@ -425,7 +408,7 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
             a.out[p].flags = kFlagInvalidPair;
         }
         a.stage[p] = cls == 0u ? kStageSkip : kStageConvex;
-        return;
+        return false;
     }
     FeatureCache* fc = a.caches ? &a.caches[p].feature : nullptr;
     if (!fc || fc->state == kCacheEmpty) {
@@ -433,7 +416,7 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
         // discarded by the flush (sat.cpp:182-185): go straight to the SAT.
         if (fc) fc->state = kCacheEmpty;
         a.stage[p] = kStageOverlap;
-        return;
+        return true;
     }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
@@ -449,7 +432,7 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     if (!inRange || state0 > kCacheEmpty) {
         fc->state = kCacheEmpty;
         a.stage[p] = kStageOverlap;
-        return;
+        return true;
     }
     uint32_t state1;
     switch (type) {
@@ -464,7 +447,7 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     if (state0 == kCacheSeparation && state1 == kCacheSeparation) {
         write_manifold(a.out + p, m);
         a.stage[p] = kStageSkip;
-        return;
+        return false;
     }
     if (state0 == kCacheOverlap && state1 == kCacheOverlap) {
         Quat q10{0, 0, 0, 1}, q1{0, 0, 0, 1}, q20{0, 0, 0, 1}, q2{0, 0, 0, 1};
@@ -484,11 +467,18 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
         if (m.pointCount > 0) {
             write_manifold(a.out + p, m);
             a.stage[p] = kStageSkip;
-            return;
+            return false;
         }
     }
     fc->state = kCacheEmpty;
     a.stage[p] = kStageOverlap;
+    return true;
+}
+
+__global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= a.n) return;
+    if (!cache_pair(a, p)) a.sat[p].stage = kStageSkip;
 }
 
 // ------------------------------------------------------------------ launchers

@@ -1,0 +1,389 @@
+// Persistent SAT kernel for hull-hull pairs on gfx950: the separation queries of
+// b3CollideHulls (face query of hull 1, face query of hull 2, edge query; sat.cpp:33-173,
+// collide_hulls.cpp:206-289 up to the contact build), one 64-lane wave per pair.
+//
+// A grid sized to the resident occupancy takes the pairs the cache kernel marked: each
+// wave a static interleave of the batch, then chunks of the last sixteenth from a device
+// counter (one atomic per chunk; per-pair atomics on one address serialise).  Per pair:
+//   * face queries: the queried hull's planes are transformed in registers (face lane +
+//     64k in slot k); the other hull's vertices are staged in LDS and read in index order
+//     by every lane at once (LDS broadcast), so each lane's support search is the
+//     reference's first maximum;
+//   * edge query: hull 1's unique edges, transformed into hull 2's frame, sit in up to
+//     three register slots per lane (9 floats each); hull 2's negated Gauss map, stored
+//     per hull at upload (device_hull.h), is staged in LDS 128 edges at a time and
+//     broadcast one edge per step.  A lane marks the (i, j) that pass b3IsMinkowskiFace in
+//     a 32-bit mask per slot; after every 32 edges the marks are appended to an LDS ring
+//     (one entry per lane per round) and projected 64 at a time by all lanes together,
+//     so b3Project's square roots and division run converged.
+// Ties resolve to the lowest index in the reference's scan order (sat.cpp:47,158), so the
+// result is bit-identical to the sequential loops whatever the processing order.
+#include <hip/hip_runtime.h>
+
+#include "b3_math.h"
+#include "contact_build.h"
+#include "device_common.h"
+#include "device_hull.h"
+#include "kernels.h"
+
+namespace b3amd {
+
+// Read-only inputs through the constant address space: a load whose address is the same
+// on every lane becomes a scalar load into SGPRs; per-lane addresses stay vector loads.
+typedef const __attribute__((address_space(4))) uint32_t* CU32;
+typedef const __attribute__((address_space(4))) float* CF32;
+
+// float4 units through a constant-space float pointer (vector types have no copy from the
+// constant address space; four adjacent loads merge into one dwordx4 load).
+struct CF4 {
+    CF32 p;
+    __device__ __forceinline__ float4 operator[](uint32_t i) const {
+        const CF32 q = p + 4u * i;
+        return make_float4(q[0], q[1], q[2], q[3]);
+    }
+    __device__ __forceinline__ CF4 operator+(uint32_t i) const { return CF4{p + 4u * i}; }
+};
+
+__device__ __forceinline__ Xf load_xf_u(CF32 p) {
+    Xf x;
+    x.R.c0 = V3{p[0], p[1], p[2]};
+    x.R.c1 = V3{p[3], p[4], p[5]};
+    x.R.c2 = V3{p[6], p[7], p[8]};
+    x.p = V3{p[9], p[10], p[11]};
+    return x;
+}
+
+// A hull record (device_hull.h) addressed through a wave-uniform base.
+struct UHull {
+    CF4 base;
+    uint32_t V, F, HE;
+    __device__ static UHull at(CF4 pool, CU32 offsets, uint32_t h) {
+        UHull r;
+        r.base = pool + offsets[h];
+        const CU32 w = (CU32)r.base.p;
+        const uint32_t x = w[0], y = w[1];
+        r.V = x & 0xFFFFu;
+        r.F = x >> 16;
+        r.HE = y & 0xFFFFu;
+        return r;
+    }
+    __device__ HullLayout layout() const { return HullLayout{V, F, HE}; }
+};
+
+constexpr uint32_t kStageEdges = 128;  // hull-2 edges staged in LDS at a time
+constexpr uint32_t kHitCap = 256;      // ring of pending Gauss-map hits (power of two)
+
+// LDS of one wave (the block is one wave).
+struct SatStage {
+    union {
+        float4 verts[kMaxHullVertices + 1];  // face queries: the other hull's vertices
+        float4 gauss[3 * kStageEdges];       // edge query: a chunk of hull 2's Gauss map
+    } u;
+    uint32_t hits[kHitCap];  // pending (i << 16 | j) pairs that passed the Gauss-map test
+};
+
+// The block is a single wave: __syncthreads() orders its LDS writes and reads.
+__device__ __forceinline__ void stage_vertices(const UHull& h, SatStage& st, uint32_t lane) {
+    __syncthreads();
+    for (uint32_t k = lane; k < h.V; k += 64u) st.u.verts[k] = h.base[2 + k];
+    __syncthreads();
+}
+
+// One face query (sat.cpp:33-58): planes of hull A transformed by xf (T9, geometry.h:77-81)
+// against hull B's vertices in LDS; support = first maximum in index order
+// (hull.inl:31-45); separation = b3Distance (geometry.h:90-93).
+template <int KF>
+__device__ __forceinline__ void face_query_l(const Xf& xf, const UHull& hA, uint32_t VB, const float4* vB,
+                                             uint32_t lane, float& bestOut, uint32_t& keyOut) {
+    Plane pl[KF];
+    V3 d[KF];
+    float mp[KF];
+    uint32_t mi[KF];
+    const V3 w0 = ld3(vB[0]);
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        const uint32_t f = lane + 64u * k;
+        const float4 q = hA.base[2 + hA.V + (f < hA.F ? f : 0u)];
+        pl[k] = mul(xf, Plane{V3{q.x, q.y, q.z}, q.w});
+        d[k] = -pl[k].n;
+        mp[k] = dot(d[k], w0);
+        mi[k] = 0;
+    }
+#pragma unroll 4
+    for (uint32_t v = 1; v < VB; ++v) {
+        const V3 w = ld3(vB[v]);
+#pragma unroll
+        for (int k = 0; k < KF; ++k) {
+            const float t = dot(d[k], w);
+            if (t > mp[k]) {
+                mp[k] = t;
+                mi[k] = v;
+            }
+        }
+    }
+    float best = -kMaxFloat;
+    uint32_t key = 0;
+#pragma unroll
+    for (int k = 0; k < KF; ++k) {
+        const uint32_t f = lane + 64u * k;
+        if (f < hA.F) {
+            const float s = distance(ld3(vB[mi[k]]), pl[k]);
+            if (s > best) {
+                best = s;
+                key = f;
+            }
+        }
+    }
+    wave_argmax(best, key);
+    bestOut = best;
+    keyOut = key;
+}
+
+__device__ __forceinline__ void face_query_any(const Xf& xf, const UHull& hA, const UHull& hB, SatStage& st,
+                                               uint32_t lane, float& best, uint32_t& key) {
+    stage_vertices(hB, st, lane);
+    if (hA.F <= 64u) face_query_l<1>(xf, hA, hB.V, st.u.verts, lane, best, key);
+    else if (hA.F <= 128u) face_query_l<2>(xf, hA, hB.V, st.u.verts, lane, best, key);
+    else face_query_l<4>(xf, hA, hB.V, st.u.verts, lane, best, key);
+}
+
+// Projects `count` queued hits (lane l takes entry head + l) with b3Project (sat.cpp:73-108)
+// and folds them into (best, key); j is a hull-2 edge of the staged chunk starting at c0.
+__device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const SatStage& st,
+                                           uint32_t c0, uint32_t head, uint32_t count, uint32_t lane, float& best,
+                                           uint32_t& key) {
+    if (lane < count) {
+        const uint32_t e = st.hits[(head + lane) & (kHitCap - 1u)];
+        const uint32_t i = e >> 16, j = e & 0xFFFFu;
+        const uint32_t u = ue1[i];
+        const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
+        const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
+        const V3 E1 = Q1 - P1;
+        const float4 r1 = st.u.gauss[3u * (j - c0) + 1u], r2 = st.u.gauss[3u * (j - c0) + 2u];
+        const V3 E2 = -V3{r1.z, r1.w, r2.x};
+        const V3 P2{r2.y, r2.z, r2.w};
+        const float s = project_edges(P1, E1, length(E1), P1 - C1, P2, E2, length(E2));
+        if (s > best || (s == best && e < key)) {
+            best = s;
+            key = e;
+        }
+    }
+}
+
+// Edge query (sat.cpp:110-173) with K register slots of hull-1 edges per lane.
+template <int K>
+__device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h1, const UHull& h2, SatStage& st,
+                                             uint32_t lane, float& best, uint32_t& key) {
+    const uint32_t E1n = h1.HE >> 1;
+    const HullLayout L1 = h1.layout(), L2 = h2.layout();
+    const CU32 ue1 = (CU32)(h1.base + L1.uedgeOff()).p;
+    const CF4 g2 = h2.base + L2.gaussOff();
+    const uint32_t E2p = L2.gaussEdges();  // padded with records that never pass
+    uint32_t head = 0, tail = 0;
+    for (uint32_t i0 = 0; i0 < E1n; i0 += 64u * K) {
+        // Slot k: A = U1, B = V1, nE = -E1 in hull 2's frame (sat.cpp:123-146); lanes past
+        // the last edge hold zeros, which fail the test.
+        V3 A[K], B[K], nE[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const uint32_t i = i0 + lane + 64u * k;
+            const V3 zero{0.0f, 0.0f, 0.0f};
+            A[k] = B[k] = nE[k] = zero;
+            if (i < E1n) {
+                const uint32_t u = ue1[i];
+                const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
+                const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
+                A[k] = mul(xf.R, ld3(h1.base[2 + h1.V + ((u >> 16) & 0xFFu)]));
+                B[k] = mul(xf.R, ld3(h1.base[2 + h1.V + (u >> 24)]));
+                nE[k] = -(Q1 - P1);
+            }
+        }
+        for (uint32_t c0 = 0; c0 < E2p; c0 += kStageEdges) {
+            const uint32_t cn = E2p - c0 < kStageEdges ? E2p - c0 : kStageEdges;  // a multiple of 4
+            __syncthreads();
+            for (uint32_t u = lane; u < 3u * cn; u += 64u) st.u.gauss[u] = g2[3u * c0 + u];
+            __syncthreads();
+            for (uint32_t j0 = 0; j0 < cn; j0 += 32u) {
+                uint32_t bits[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) bits[k] = 0u;
+                const uint32_t jn = cn - j0 < 32u ? cn - j0 : 32u;
+                for (uint32_t jj = 0; jj < jn; jj += 4u) {
+#pragma unroll
+                    for (uint32_t t = 0; t < 4u; ++t) {
+                        const float4* r = st.u.gauss + 3u * (j0 + jj + t);
+                        const float4 r0 = r[0], r1 = r[1];
+                        const float r2x = r[2].x;
+                        const V3 C{r0.x, r0.y, r0.z}, Dn{r0.w, r1.x, r1.y}, DxC{r1.z, r1.w, r2x};
+                        const uint32_t bit = 1u << (jj + t);
+#pragma unroll
+                        for (int k = 0; k < K; ++k) {
+                            // b3IsMinkowskiFace(U1, V1, -E1, -U2, -V2, -E2) (sat.cpp:60-71)
+                            const float CBA = dot(C, nE[k]);
+                            const float DBA = dot(Dn, nE[k]);
+                            const float ADC = dot(A[k], DxC);
+                            const float BDC = dot(B[k], DxC);
+                            if ((CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f)) bits[k] |= bit;
+                        }
+                    }
+                }
+                // Queue the marks: each round appends one hit per lane that has any.
+                for (;;) {
+                    uint32_t sel = (uint32_t)K;
+#pragma unroll
+                    for (int k = K - 1; k >= 0; --k)
+                        if (bits[k]) sel = (uint32_t)k;
+                    const uint64_t m = __ballot(sel < (uint32_t)K);
+                    if (m == 0) break;
+                    if (sel < (uint32_t)K) {
+                        uint32_t w = 0;
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            if ((uint32_t)k == sel) w = bits[k];
+#pragma unroll
+                        for (int k = 0; k < K; ++k)
+                            if ((uint32_t)k == sel) bits[k] = w & (w - 1u);
+                        const uint32_t e = ((i0 + lane + 64u * sel) << 16) | (c0 + j0 + (uint32_t)__builtin_ctz(w));
+                        st.hits[(tail + mbcnt(m)) & (kHitCap - 1u)] = e;
+                    }
+                    tail += (uint32_t)__popcll(m);
+                    if (tail - head >= 64u) {
+                        __syncthreads();
+                        drain_hits(xf, C1, h1, ue1, st, c0, head, 64u, lane, best, key);
+                        head += 64u;
+                    }
+                }
+            }
+            // Project what is queued before the staged chunk is replaced.
+            __syncthreads();
+            while (tail != head) {
+                const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
+                drain_hits(xf, C1, h1, ue1, st, c0, head, cnt, lane, best, key);
+                head += cnt;
+            }
+        }
+    }
+}
+
+// b3CollideHulls' queries for one pair with the early exits on totalRadius
+// (collide_hulls.cpp:206-289).
+__device__ __forceinline__ void sat_pair(const SatArgs& a, uint32_t p, SatStage& st, uint32_t lane,
+                                         uint32_t maxSlots) {
+    const CU32 pairs = (const CU32)a.pairs;
+    const CU32 shapes = (const CU32)a.shapes;  // ShapeRec: 12 words (type, hull, radius, ...)
+    const uint32_t sa = pairs[2 * p], sb = pairs[2 * p + 1];
+    const uint32_t ha = shapes[12 * sa + 1], hb = shapes[12 * sb + 1];
+    const float totalRadius = __uint_as_float(shapes[12 * sa + 2]) + __uint_as_float(shapes[12 * sb + 2]);
+    const CF32 xp = (const CF32)a.xfs + 24 * (size_t)p;
+    const Xf xf1 = load_xf_u(xp), xf2 = load_xf_u(xp + 12);
+    const CF4 pool{(CF32)a.pool};
+    const CU32 offsets = (const CU32)a.offsets;
+    const UHull h1 = UHull::at(pool, offsets, ha);
+    const UHull h2 = UHull::at(pool, offsets, hb);
+
+    SatResult r;
+    r.f2Index = 0;
+    r.f2Sep = -kMaxFloat;
+    r.eIndex1 = 0;
+    r.eIndex2 = 0;
+    r.eSep = -kMaxFloat;
+    const Xf xf = mulT(xf2, xf1);
+    face_query_any(xf, h1, h2, st, lane, r.f1Sep, r.f1Index);
+    if (r.f1Sep > totalRadius) {
+        r.stage = kStageSepFace1;
+        if (lane == 0) a.out[p] = r;
+        return;
+    }
+    face_query_any(mulT(xf1, xf2), h2, h1, st, lane, r.f2Sep, r.f2Index);
+    if (r.f2Sep > totalRadius) {
+        r.stage = kStageSepFace2;
+        if (lane == 0) a.out[p] = r;
+        return;
+    }
+    const V3 C1 = mul(xf, ld3(h1.base[1]));
+    float best = -kMaxFloat;
+    uint32_t key = 0;
+    const uint32_t need = ((h1.HE >> 1) + 63u) / 64u;
+    const uint32_t K = need < maxSlots ? need : maxSlots;
+    if (K <= 1u) edge_query_l<1>(xf, C1, h1, h2, st, lane, best, key);
+    else if (K == 2u) edge_query_l<2>(xf, C1, h1, h2, st, lane, best, key);
+    else edge_query_l<3>(xf, C1, h1, h2, st, lane, best, key);
+    wave_argmax(best, key);
+    r.eIndex1 = 2 * (key >> 16);
+    r.eIndex2 = 2 * (key & 0xFFFFu);
+    r.eSep = best;
+    r.stage = r.eSep > totalRadius ? kStageSepEdge : kStageOverlap;
+    if (lane == 0) a.out[p] = r;
+}
+
+// The pairs the cache kernel marked for a SAT: a static interleave up to staticEnd, then
+// chunks from the tail counter (every wave leaves once the counter passes the batch).
+// W is the register budget: the compiler fits the kernel into 512 / W VGPRs so that W
+// waves share a SIMD.
+template <int W>
+__global__ __launch_bounds__(64, W) void sat_persistent_kernel(SatArgs a, uint32_t maxSlots) {
+    __shared__ SatStage st;
+    const uint32_t lane = threadIdx.x;
+    const CU32 stage = (const CU32)a.stage;
+    for (uint32_t p = blockIdx.x; p < a.split.staticEnd; p += gridDim.x)
+        if (stage[p] == kStageOverlap) sat_pair(a, p, st, lane, maxSlots);
+    for (;;) {
+        uint32_t t = 0;
+        if (lane == 0) t = atomicAdd(a.split.counter, a.split.chunk);
+        const uint32_t p0 = a.split.staticEnd + (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
+        if (p0 >= a.n) break;
+        const uint32_t p1 = a.n - p0 < a.split.chunk ? a.n : p0 + a.split.chunk;
+        for (uint32_t p = p0; p < p1; ++p)
+            if (stage[p] == kStageOverlap) sat_pair(a, p, st, lane, maxSlots);
+    }
+}
+
+__global__ void reset_counter_kernel(uint32_t* counter) {
+    if (threadIdx.x == 0) *counter = 0u;
+}
+
+hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s) {
+    reset_counter_kernel<<<1, 64, 0, s>>>(counter);
+    return hipGetLastError();
+}
+
+template <int W>
+hipError_t launch_sat_w(SatArgs a, uint32_t wavesPerSimd, uint32_t slots, hipStream_t s) {
+    static int cus = 0, perCu = 0;
+    if (cus == 0) {
+        int dev = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&perCu, sat_persistent_kernel<W>, 64, 0);
+        if (e != hipSuccess) {
+            cus = 0;
+            return e;
+        }
+        if (perCu < 1) perCu = 1;
+    }
+    // A block is one wave; a CU has four SIMDs.
+    uint32_t blocksPerCu = (uint32_t)perCu;
+    if (4u * wavesPerSimd < blocksPerCu) blocksPerCu = 4u * wavesPerSimd;
+    uint64_t blocks = (uint64_t)cus * blocksPerCu;
+    if (a.n < blocks) blocks = a.n;
+    // The last sixteenth of the batch (at least two chunks per wave) goes out dynamically.
+    a.split.chunk = 8u;
+    uint64_t tail = a.n / 16u;
+    if (tail < 2u * a.split.chunk * blocks) tail = 2u * a.split.chunk * blocks;
+    if (tail > a.n) tail = a.n;
+    a.split.staticEnd = a.n - (uint32_t)tail;
+    sat_persistent_kernel<W><<<(uint32_t)blocks, 64, 0, s>>>(a, slots);
+    return hipGetLastError();
+}
+
+hipError_t launch_sat_persistent(SatArgs a, const SatTuning& t, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    const uint32_t slots = t.maxSlots == 0 ? 3u : (t.maxSlots > 3u ? 3u : t.maxSlots);
+    const uint32_t w = t.wavesPerSimd == 0 ? 5u : t.wavesPerSimd;
+    // Register budget of the variant: the largest that still holds w waves per SIMD.
+    if (w >= 6u) return launch_sat_w<6>(a, w, slots, s);
+    if (w == 5u) return launch_sat_w<5>(a, w, slots, s);
+    return launch_sat_w<4>(a, w, slots, s);
+}
+
+}  // namespace b3amd

@@ -144,6 +144,22 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
 
 int b3amd_synchronize(b3amd_context* ctx);
 
+/* ---- launch shape of the hull SAT (no reference counterpart: the reference runs the
+ * queries of sat.cpp:33-173 one pair at a time on one thread) ----
+ * The SAT kernel is one persistent grid draining a device work queue.  waves_per_simd
+ * (0 = 5, at most 32) is the number of resident waves per SIMD the grid is sized for; it
+ * also picks the kernel's register budget (the variant built for min(max(w, 4), 6) waves).
+ * max_slots (1..3) is the number of hull-1 edges each lane holds in registers during the
+ * edge query. */
+int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t max_slots);
+int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* max_slots);
+/* Times every launch shape on a device-resident batch (as b3amd_collide_device, without
+ * feature caches or sweeps, so every hull pair runs the full SAT), `repeats` batches per
+ * shape, keeps the fastest in the context and returns its mean batch time in best_ms
+ * (may be NULL).  d_out is overwritten. */
+int b3amd_autotune(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                   b3amd_manifold* d_out, uint32_t repeats, float* best_ms);
+
 /* ---- triangle meshes ----
  * b3Mesh (mesh.h:41-99): vertices [V][3], triangles [T][3] (b3Triangle v1, v2, v3).  The
  * static tree is built on the host with the node numbering of b3Mesh::BuildTree /

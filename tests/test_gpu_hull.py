@@ -136,3 +136,42 @@ def test_gpu_full_size_properties(ctx, lib, oracle):
     assert_manifolds_equal(got[idx], oracle.collide(pool, shapes, pairs[idx], xfs[idx]))
     again = ctx.collide(pairs, xfs)
     assert_manifolds_equal(again, got, "repeat")
+
+
+@pytest.mark.parametrize("waves,slots", [(6, 3), (5, 2), (4, 1), (0, 3), (2, 3)])
+def test_gpu_sat_launch_shapes_agree(ctx, lib, oracle, waves, slots):
+    """Every launch shape of the persistent SAT kernel (resident waves, register slots,
+    hence edge-query chunking) gives the oracle's manifolds bit for bit, for hulls of
+    8-64 points and beyond 192 unique edges (several register chunks)."""
+    say("case: SAT launch shape waves=%d slots=%d" % (waves, slots))
+    try:
+        ctx.set_sat_tuning(waves, slots)
+        for seed, nmin, nmax, n in ((707, 8, 64, 3000), (808, 70, 128, 300)):
+            pool, shapes, pairs, xfs = hull_workload(lib, seed, 60, n, nmin=nmin, nmax=nmax)
+            assert_manifolds_equal(run(ctx, pool, shapes, pairs, xfs), oracle.collide(pool, shapes, pairs, xfs))
+    finally:
+        ctx.set_sat_tuning(0, 3)
+
+
+def test_gpu_autotune(ctx, lib, oracle):
+    """The autotuner times every launch shape on a device batch and keeps a valid one;
+    results after tuning still match the oracle."""
+    say("case: autotune")
+    pool, shapes, pairs, xfs = hull_workload(lib, 909, 400, 20000)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    n = len(pairs)
+    dp, dx, do = ctx.alloc(pairs.nbytes), ctx.alloc(xfs.nbytes), ctx.alloc(n * MANIFOLD_DTYPE.itemsize)
+    try:
+        dp.upload(pairs)
+        dx.upload(xfs)
+        (waves, slots), ms = ctx.autotune(n, dp, dx, do, repeats=1)
+        say("autotune: waves=%d slots=%d %.3f ms" % (waves, slots, ms))
+        assert 1 <= slots <= 3 and 0 <= waves <= 32 and ms > 0
+        idx = np.arange(0, n, 7)
+        got = ctx.collide(pairs[idx], xfs[idx])
+        assert_manifolds_equal(got, oracle.collide(pool, shapes, pairs[idx], xfs[idx]))
+    finally:
+        for b in (dp, dx, do):
+            b.free()
+        ctx.set_sat_tuning(0, 3)
