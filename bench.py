@@ -115,6 +115,21 @@ def algorithmic_work(pool, pairs, xfs, stride):
 
 # ----------------------------------------------------------------- main
 
+TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
+                            "sat_hull_1M_pairs_hbm_traffic.md")
+
+
+def measured_traffic(n_pairs):
+    """HBM bytes per SAT launch from the committed FETCH_SIZE/WRITE_SIZE passes (measured on
+    the 1,048,576-pair workload only; None for other sizes)."""
+    if n_pairs != 1 << 20 or not os.path.exists(TRAFFIC_FILE):
+        return None
+    for line in open(TRAFFIC_FILE):
+        if line.startswith("traffic_bytes_per_launch sat_hull:"):
+            return float(line.split(":")[1])
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -226,7 +241,9 @@ def main():
             "hbm_frac": achieved_gbs / HBM_PEAK_GBS,
             "flops_per_launch": work["sat_flops"],
             "bytes_per_launch": work["sat_bytes"],
-            "traffic": None,
+            "traffic": measured_traffic(n),
+            "traffic_source": "rocprofv3 FETCH_SIZE x2.0 + WRITE_SIZE x1.0 (scaled by a 1 GiB copy), "
+                              "profiles/sat_hull_1M_pairs_hbm_traffic.md",
         },
         "cpu_baseline": cpu,
     }

@@ -5,7 +5,6 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdarg>
-#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <map>
@@ -63,7 +62,6 @@ struct b3amd_context {
     float4* dPool = nullptr;
     uint32_t* dOffsets = nullptr;
     uint32_t hullCount = 0;
-    int sizeClass = 0;
     std::vector<HullLayout> layouts;
 
     // Shape table.
@@ -78,7 +76,6 @@ struct b3amd_context {
     uint32_t* dCounter = nullptr;  // tail counter of the persistent SAT kernel
     uint32_t scratchCap = 0;
     SatTuning tuning{0u, 3u};
-    bool legacySat = false;      // the one-wave-per-pair LDS kernel, kept for comparison
     void* dIn = nullptr;
     uint64_t dInCap = 0;
 
@@ -317,7 +314,6 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     cta.caches = caches;
     cta.out = out;
     hipStream_t s = ctx->stream;
-    int cls = ctx->sizeClass;
     if ((rc = timed(ctx, "reset_counter", [&] { return launch_reset_counter(ctx->dCounter, s); }))) return rc;
     if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     if (ctx->roundShapes) {
@@ -327,12 +323,8 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
         cv.out = out;
         if ((rc = timed(ctx, "convex", [&] { return launch_convex(cv, s); }))) return rc;
     }
-    if (ctx->legacySat) {
-        if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_hull(sa, cls, s); }))) return rc;
-    } else {
-        const SatTuning t = ctx->tuning;
-        if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
-    }
+    const SatTuning t = ctx->tuning;
+    if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
     if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
     return B3AMD_OK;
 }
@@ -353,8 +345,6 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
     b3amd_context* ctx = new b3amd_context();
     ctx->device = device;
     ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
-    const char* legacy = getenv("B3AMD_SAT_LEGACY");
-    ctx->legacySat = legacy && legacy[0] == '1';
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(B3AMD_ERR_DEVICE, "hipStreamCreate failed");
@@ -387,15 +377,12 @@ int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
     std::vector<uint32_t> all, words;
     std::vector<uint32_t> offsets(pool->count);
     std::vector<HullLayout> layouts(pool->count);
-    int cls = 0;
     for (uint32_t h = 0; h < pool->count; ++h) {
         const char* err = pack_hull(pool, h, words, &layouts[h]);
         if (err) return fail(B3AMD_ERR_UNSUPPORTED, "hull %u: %s", h, err);
         if ((all.size() / 4) > 0xFFFFFFFFull) return fail(B3AMD_ERR_UNSUPPORTED, "pool too large");
         offsets[h] = (uint32_t)(all.size() / 4);
         all.insert(all.end(), words.begin(), words.end());
-        const HullLayout& L = layouts[h];
-        if (L.V > 64 || L.F > 128 || L.E() > 192) cls = 1;
     }
     int rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -408,7 +395,6 @@ int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
     HIP_TRY(hipMemcpy(ctx->dPool, all.data(), 4ull * all.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->dOffsets, offsets.data(), 4ull * offsets.size(), hipMemcpyHostToDevice));
     ctx->hullCount = pool->count;
-    ctx->sizeClass = cls;
     ctx->layouts.swap(layouts);
     return B3AMD_OK;
 }

@@ -165,8 +165,6 @@ hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
-// sizeClass 0: every hull of the batch has V <= 64, F <= 128, HE/2 <= 192; 1: the limits.
-hipError_t launch_sat_hull(const SatArgs& a, int sizeClass, hipStream_t s);
 // Persistent SAT over the pairs the cache kernel marked; any hull size within the record
 // limits.  Fills a.split (the counter must be zero).
 hipError_t launch_sat_persistent(SatArgs a, const SatTuning& t, hipStream_t s);
