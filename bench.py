@@ -130,6 +130,61 @@ def measured_traffic(n_pairs):
     return None
 
 
+def serving_run(lib, worlds_n, rate, duration, seed=4096):
+    """Serving workload: `worlds_n` worlds (10-2000 bodies, log-uniform; boxes, spheres and
+    8-32-point hulls in resting stacks), each step one request.  Measures the flood
+    capacity (two warm steps of every world submitted at once) and the latency of Poisson
+    arrivals at `rate` steps/s, and at half the capacity, for `duration` seconds each."""
+    import bounce_amd
+    from bounce_amd.workloads import gen_serving_worlds
+    t = time.perf_counter()
+    pool, shapes, worlds = gen_serving_worlds(lib, seed, worlds_n, steps=4)
+    total_pairs = sum(len(w["pairs"]) for w in worlds)
+    log("serving workload: %d worlds, %d bodies, %d contacts in %.1f s"
+        % (worlds_n, sum(len(w["body_shapes"]) for w in worlds), total_pairs, time.perf_counter() - t))
+    rng = np.random.default_rng(seed)
+    out = {"worlds": worlds_n, "contacts": total_pairs,
+           "workload": "worlds of 10-2000 bodies (log-uniform): boxes, spheres, 8-32-point hulls in resting "
+                       "stacks on a static ground, one contact per body; per-step position jitter 1 mm"}
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(pool)
+        ctx.upload_shapes(shapes)
+        with bounce_amd.Server(ctx, 1 << 16, 1 << 16, total_pairs + 4096) as srv:
+            ids = np.array([srv.create_world(w["body_shapes"], w["pairs"]) for w in worlds], np.uint32)
+            # Cold step (empty caches) of every world, then the flood capacity on warm steps.
+            srv.replay(ids, np.zeros(worlds_n), [w["xfs"][0] for w in worlds])
+            order = np.concatenate([ids, ids])
+            steps = np.concatenate([np.ones(worlds_n, int), np.full(worlds_n, 2)])
+            lat = srv.replay(order, np.zeros(len(order)), [worlds[i]["xfs"][s] for i, s in zip(order, steps)])
+            capacity = len(order) / (lat.max() / 1000.0)
+            out["capacity_steps_per_s"] = capacity
+            out["capacity_contacts_per_s"] = capacity * total_pairs / worlds_n
+            log("serving capacity: %.0f steps/s" % capacity)
+            for name, r in (("at_rate", rate), ("at_half_capacity", 0.5 * capacity)):
+                n = int(r * duration)
+                times = np.cumsum(rng.exponential(1.0 / r, n))
+                wsel = rng.integers(0, worlds_n, n)
+                ssel = rng.integers(1, 4, n)
+                before = srv.stats()
+                lat = srv.replay(ids[wsel], times, [worlds[w]["xfs"][s] for w, s in zip(wsel, ssel)])
+                after = srv.stats()
+                batches = after["batches"] - before["batches"]
+                out[name] = {
+                    "rate_steps_per_s": r,
+                    "requests": n,
+                    "p50_ms": float(np.percentile(lat, 50)),
+                    "p99_ms": float(np.percentile(lat, 99)),
+                    "max_ms": float(lat.max()),
+                    "achieved_steps_per_s": n / float((times + lat / 1000.0).max()),
+                    "batches": batches,
+                    "mean_batch_contacts": (after["pairs"] - before["pairs"]) / max(1, batches),
+                }
+                log("serving %s: %.0f steps/s, p50 %.3f ms, p99 %.3f ms, %d batches"
+                    % (name, r, out[name]["p50_ms"], out[name]["p99_ms"], batches))
+    out["latency"] = "narrowphase only: request submitted (scheduled arrival) -> manifolds in HBM"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -139,6 +194,10 @@ def main():
     ap.add_argument("--pool", type=int, default=65536)
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--serving-worlds", type=int, default=4096)
+    ap.add_argument("--serving-rate", type=float, default=240000.0)
+    ap.add_argument("--serving-seconds", type=float, default=0.5)
     ap.add_argument("--tuning", default="auto", metavar="auto|W,K",
                     help="SAT launch shape: autotune on the batch, or waves-per-SIMD,register-slots")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -247,6 +306,8 @@ def main():
         },
         "cpu_baseline": cpu,
     }
+    if not args.no_serving:
+        result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     print(json.dumps(result))
 
 

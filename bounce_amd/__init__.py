@@ -12,7 +12,7 @@ import numpy as np
 from .types import (CACHE_DTYPE, MANIFOLD_DTYPE, SHAPE_DTYPE, HullPool, empty_caches, ptr,
                     SPHERE, CAPSULE, HULL, MESH)
 
-__all__ = ["Context", "HullPool", "load_library", "empty_caches", "BounceError",
+__all__ = ["Context", "Graph", "Server", "HullPool", "load_library", "empty_caches", "BounceError",
            "MANIFOLD_DTYPE", "CACHE_DTYPE", "SHAPE_DTYPE", "SPHERE", "CAPSULE", "HULL", "MESH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbounce_amd.so")
@@ -61,6 +61,18 @@ def load_library():
         "b3amd_set_sat_tuning": (i32, [vp, u32, u32]),
         "b3amd_get_sat_tuning": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "b3amd_autotune": (i32, [vp, u32, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]),
+        "b3amd_graph_create": (i32, [vp, u32, vp, vp, vp, vp, vp, ctypes.POINTER(vp)]),
+        "b3amd_graph_launch": (i32, [vp]),
+        "b3amd_graph_destroy": (i32, [vp]),
+        "b3amd_server_create": (i32, [vp, u32, u32, u32, ctypes.POINTER(vp)]),
+        "b3amd_server_destroy": (i32, [vp]),
+        "b3amd_world_create": (i32, [vp, u32, vp, u32, vp, ctypes.POINTER(u32)]),
+        "b3amd_world_destroy": (i32, [vp, u32]),
+        "b3amd_world_submit": (i32, [vp, u32, vp, vp, ctypes.POINTER(u64)]),
+        "b3amd_server_wait": (i32, [vp, u64, ctypes.POINTER(ctypes.c_double)]),
+        "b3amd_world_read_manifolds": (i32, [vp, u32, vp]),
+        "b3amd_server_replay": (i32, [vp, u32, vp, vp, vp, vp]),
+        "b3amd_server_get_stats": (i32, [vp, vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -99,6 +111,103 @@ class DeviceBuffer:
         if self.ptr is not None and self.ctx.handle is not None:
             _check(self.ctx.lib.b3amd_device_free(self.ctx.handle, self.ptr))
         self.ptr = None
+
+
+class Graph:
+    """A captured b3amd_collide_device batch (b3amd_graph_create)."""
+
+    def __init__(self, ctx, handle):
+        self.ctx = ctx
+        self.handle = handle
+
+    def launch(self):
+        _check(self.ctx.lib.b3amd_graph_launch(self.handle))
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.ctx.lib.b3amd_graph_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+
+class ServerStats(ctypes.Structure):
+    """b3amd_server_stats."""
+    _fields_ = [("batches", ctypes.c_uint64), ("requests", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
+                ("max_batch_pairs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+
+class Server:
+    """Continuous batching of many worlds' narrowphase steps (b3amd_server_create)."""
+
+    def __init__(self, ctx, max_batch_pairs, max_batch_bodies, slab_slots):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_server_create(ctx.handle, max_batch_pairs, max_batch_bodies, slab_slots,
+                                            ctypes.byref(h)))
+        self.handle = h
+        self.pair_counts = {}
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.lib.b3amd_server_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def create_world(self, body_shapes, body_pairs):
+        bs = np.ascontiguousarray(body_shapes, np.uint32)
+        bp = np.ascontiguousarray(body_pairs, np.uint32).reshape(-1, 2)
+        w = ctypes.c_uint32()
+        _check(self.lib.b3amd_world_create(self.handle, len(bs), ptr(bs), len(bp), ptr(bp), ctypes.byref(w)))
+        self.pair_counts[w.value] = len(bp)
+        return w.value
+
+    def destroy_world(self, world):
+        _check(self.lib.b3amd_world_destroy(self.handle, world))
+        self.pair_counts.pop(world, None)
+
+    def submit(self, world, body_xfs, body_sweeps=None):
+        x = np.ascontiguousarray(body_xfs, np.float32)
+        sw = None if body_sweeps is None else np.ascontiguousarray(body_sweeps, np.float32)
+        t = ctypes.c_uint64()
+        _check(self.lib.b3amd_world_submit(self.handle, world, ptr(x), ptr(sw), ctypes.byref(t)))
+        return t.value
+
+    def wait(self, ticket):
+        """Blocks until the step is done; returns its latency in ms."""
+        ms = ctypes.c_double()
+        _check(self.lib.b3amd_server_wait(self.handle, ticket, ctypes.byref(ms)))
+        return ms.value
+
+    def read_manifolds(self, world):
+        out = np.zeros(self.pair_counts[world], MANIFOLD_DTYPE)
+        _check(self.lib.b3amd_world_read_manifolds(self.handle, world, ptr(out)))
+        return out
+
+    def replay(self, worlds, times_s, body_xfs):
+        """Load generator: request i = (worlds[i], body_xfs[i]) at times_s[i]; returns the
+        latencies (ms) from each scheduled arrival.  body_xfs holds float32 arrays."""
+        worlds = np.ascontiguousarray(worlds, np.uint32)
+        times = np.ascontiguousarray(times_s, np.float64)
+        ptrs = np.array([a.ctypes.data for a in body_xfs], np.uint64)
+        lat = np.zeros(len(worlds), np.float64)
+        _check(self.lib.b3amd_server_replay(self.handle, len(worlds), ptr(worlds), ptr(times), ptr(ptrs), ptr(lat)))
+        return lat
+
+    def stats(self):
+        st = ServerStats()
+        _check(self.lib.b3amd_server_get_stats(self.handle, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
 
 
 class Context:
@@ -193,6 +302,14 @@ class Context:
 
     def synchronize(self):
         _check(self.lib.b3amd_synchronize(self.handle))
+
+    def capture(self, n, d_pairs, d_xfs, d_out, d_sweeps=None, d_caches=None):
+        """Captures one device batch as a replayable HIP graph."""
+        p = lambda b: None if b is None else b.ptr
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_graph_create(self.handle, n, p(d_pairs), p(d_xfs), p(d_sweeps), p(d_caches), p(d_out),
+                                           ctypes.byref(h)))
+        return Graph(self, h)
 
     def set_sat_tuning(self, waves_per_simd, max_slots):
         """Launch shape of the persistent SAT kernel (b3amd_set_sat_tuning)."""

@@ -13,15 +13,18 @@
 #include <vector>
 
 #include "bounce_amd.h"
+#include "context.h"
 #include "device_hull.h"
 #include "kernels.h"
 #include "static_tree.h"
 
 using namespace b3amd;
 
-namespace {
+namespace b3amd {
 
+namespace {
 thread_local std::string g_error;
+}
 
 int fail(int code, const char* fmt, ...) {
     char buf[512];
@@ -33,119 +36,7 @@ int fail(int code, const char* fmt, ...) {
     return code;
 }
 
-#define HIP_TRY(expr)                                                                      \
-    do {                                                                                   \
-        hipError_t _e = (expr);                                                            \
-        if (_e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "%s: %s", #expr, hipGetErrorString(_e)); \
-    } while (0)
-
-struct KernelStat {
-    uint32_t launches = 0;
-    double total_ms = 0.0;
-};
-
-struct PendingTiming {
-    std::string name;
-    hipEvent_t start, stop;
-};
-
-}  // namespace
-
-struct b3amd_context {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    uint64_t budget = 0;
-    uint64_t inUse = 0;
-    std::unordered_map<void*, uint64_t> allocs;
-
-    // Hull pool.
-    float4* dPool = nullptr;
-    uint32_t* dOffsets = nullptr;
-    uint32_t hullCount = 0;
-    std::vector<HullLayout> layouts;
-
-    // Shape table.
-    ShapeRec* dShapes = nullptr;
-    uint32_t shapeCount = 0;
-    bool roundShapes = false;  // the table holds spheres or capsules
-    std::vector<ShapeRec> shapes;
-
-    // Per-batch scratch and host-API staging.
-    uint32_t* dStage = nullptr;
-    SatResult* dSat = nullptr;
-    uint32_t* dCounter = nullptr;  // tail counter of the persistent SAT kernel
-    uint32_t scratchCap = 0;
-    SatTuning tuning{0u, 3u};
-    void* dIn = nullptr;
-    uint64_t dInCap = 0;
-
-    // Meshes (b3Mesh with its static tree) and mesh-contact scratch.
-    struct MeshRecord {
-        float4* verts = nullptr;
-        uint4* tris = nullptr;
-        TreeNode* nodes = nullptr;
-        uint32_t vertexCount = 0, triangleCount = 0, nodeCount = 0;
-    };
-    std::vector<MeshRecord> meshes;
-    void* dMeshScratch = nullptr;
-    uint64_t meshScratchCap = 0;
-
-    // Profiling.
-    bool profiling = false;
-    std::vector<hipEvent_t> freeEvents;
-    std::vector<PendingTiming> pending;
-    std::map<std::string, KernelStat> stats;
-
-    int alloc(uint64_t bytes, void** p) {
-        if (bytes == 0) bytes = 16;
-        if (inUse + bytes > budget)
-            return fail(B3AMD_ERR_BUDGET, "HBM budget exceeded: %llu in use + %llu requested > %llu",
-                        (unsigned long long)inUse, (unsigned long long)bytes, (unsigned long long)budget);
-        HIP_TRY(hipSetDevice(device));
-        void* q = nullptr;
-        hipError_t e = hipMalloc(&q, bytes);
-        if (e != hipSuccess) return fail(B3AMD_ERR_BUDGET, "hipMalloc(%llu): %s", (unsigned long long)bytes, hipGetErrorString(e));
-        allocs[q] = bytes;
-        inUse += bytes;
-        *p = q;
-        return B3AMD_OK;
-    }
-    int release(void* p) {
-        if (!p) return B3AMD_OK;
-        auto it = allocs.find(p);
-        if (it == allocs.end()) return fail(B3AMD_ERR_ARGUMENT, "pointer %p not owned by this context", p);
-        HIP_TRY(hipSetDevice(device));
-        HIP_TRY(hipFree(p));
-        inUse -= it->second;
-        allocs.erase(it);
-        return B3AMD_OK;
-    }
-    hipEvent_t event() {
-        if (!freeEvents.empty()) {
-            hipEvent_t e = freeEvents.back();
-            freeEvents.pop_back();
-            return e;
-        }
-        hipEvent_t e;
-        if (hipEventCreate(&e) != hipSuccess) return nullptr;
-        return e;
-    }
-    int resolve_timings() {
-        if (pending.empty()) return B3AMD_OK;
-        HIP_TRY(hipStreamSynchronize(stream));
-        for (PendingTiming& t : pending) {
-            float ms = 0.0f;
-            HIP_TRY(hipEventElapsedTime(&ms, t.start, t.stop));
-            KernelStat& s = stats[t.name];
-            s.launches += 1;
-            s.total_ms += ms;
-            freeEvents.push_back(t.start);
-            freeEvents.push_back(t.stop);
-        }
-        pending.clear();
-        return B3AMD_OK;
-    }
-};
+}  // namespace b3amd
 
 namespace {
 
@@ -168,7 +59,9 @@ int timed(b3amd_context* ctx, const char* name, F launch) {
     return B3AMD_OK;
 }
 
-int ensure_scratch(b3amd_context* ctx, uint32_t n) {
+}  // namespace
+
+int b3amd::ensure_scratch(b3amd_context* ctx, uint32_t n) {
     if (n <= ctx->scratchCap) return B3AMD_OK;
     uint32_t cap = n < 1024 ? 1024 : n;
     int rc;
@@ -184,6 +77,8 @@ int ensure_scratch(b3amd_context* ctx, uint32_t n) {
     ctx->scratchCap = cap;
     return B3AMD_OK;
 }
+
+namespace {
 
 // Converts one b3Hull view into the device record (device_hull.h); returns an error
 // string or nullptr.
@@ -280,13 +175,12 @@ int validate_pairs(const b3amd_context* ctx, uint32_t n, const uint32_t* pairs) 
     return B3AMD_OK;
 }
 
-int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs, const float* sweeps,
-              ConvexCache* caches, Manifold* out) {
-    if (n == 0) return B3AMD_OK;
-    if (!ctx->dPool) return fail(B3AMD_ERR_ARGUMENT, "no hull pool uploaded");
-    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
-    int rc = ensure_scratch(ctx, n);
-    if (rc) return rc;
+}  // namespace
+
+int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs,
+                         const float* sweeps, ConvexCache* caches, Manifold* out, const BatchScratch& sc,
+                         const uint32_t* slots) {
+    int rc = B3AMD_OK;
     PairInputs in;
     in.n = n;
     in.pairs = pairs;
@@ -296,30 +190,31 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     in.offsets = ctx->dOffsets;
     in.shapeCount = ctx->shapeCount;
     in.hullCount = ctx->hullCount;
+    in.slots = slots;
     CacheArgs ca;
     static_cast<PairInputs&>(ca) = in;
     ca.sweeps = sweeps;
     ca.caches = caches;
-    ca.stage = ctx->dStage;
+    ca.stage = sc.stage;
     ca.out = out;
-    ca.sat = ctx->dSat;
+    ca.sat = sc.sat;
     SatArgs sa;
     static_cast<PairInputs&>(sa) = in;
-    sa.stage = ctx->dStage;
-    sa.out = ctx->dSat;
-    sa.split = SatSplit{ctx->dCounter, 0u, 0u};
+    sa.stage = sc.stage;
+    sa.out = sc.sat;
+    sa.split = SatSplit{sc.counter, 0u, 0u};
     ContactArgs cta;
     static_cast<PairInputs&>(cta) = in;
-    cta.sat = ctx->dSat;
+    cta.sat = sc.sat;
     cta.caches = caches;
     cta.out = out;
     hipStream_t s = ctx->stream;
-    if ((rc = timed(ctx, "reset_counter", [&] { return launch_reset_counter(ctx->dCounter, s); }))) return rc;
+    if ((rc = timed(ctx, "reset_counter", [&] { return launch_reset_counter(sc.counter, s); }))) return rc;
     if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     if (ctx->roundShapes) {
         ConvexArgs cv;
         static_cast<PairInputs&>(cv) = in;
-        cv.stage = ctx->dStage;
+        cv.stage = sc.stage;
         cv.out = out;
         if ((rc = timed(ctx, "convex", [&] { return launch_convex(cv, s); }))) return rc;
     }
@@ -327,6 +222,19 @@ int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float
     if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
     if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
     return B3AMD_OK;
+}
+
+namespace {
+
+int run_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs, const float* sweeps,
+              ConvexCache* caches, Manifold* out) {
+    if (n == 0) return B3AMD_OK;
+    if (!ctx->dPool) return fail(B3AMD_ERR_ARGUMENT, "no hull pool uploaded");
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    int rc = ensure_scratch(ctx, n);
+    if (rc) return rc;
+    return enqueue_batch(ctx, n, pairs, xfs, sweeps, caches, out, BatchScratch{ctx->dStage, ctx->dSat, ctx->dCounter},
+                         nullptr);
 }
 
 }  // namespace
@@ -357,6 +265,7 @@ int b3amd_destroy(b3amd_context* ctx) {
     if (!ctx) return B3AMD_OK;
     hipSetDevice(ctx->device);
     hipStreamSynchronize(ctx->stream);
+    while (!ctx->graphs.empty()) b3amd_graph_destroy(ctx->graphs.back());
     for (auto& kv : ctx->allocs) hipFree(kv.first);
     for (PendingTiming& t : ctx->pending) {
         hipEventDestroy(t.start);
@@ -395,6 +304,7 @@ int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
     HIP_TRY(hipMemcpy(ctx->dPool, all.data(), 4ull * all.size(), hipMemcpyHostToDevice));
     HIP_TRY(hipMemcpy(ctx->dOffsets, offsets.data(), 4ull * offsets.size(), hipMemcpyHostToDevice));
     ctx->hullCount = pool->count;
+    ++ctx->generation;
     ctx->layouts.swap(layouts);
     return B3AMD_OK;
 }
@@ -416,6 +326,7 @@ int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* s
     if (count) HIP_TRY(hipMemcpy(ctx->dShapes, shapes, sizeof(ShapeRec) * (uint64_t)count, hipMemcpyHostToDevice));
     ctx->shapes.assign(reinterpret_cast<const ShapeRec*>(shapes), reinterpret_cast<const ShapeRec*>(shapes) + count);
     ctx->shapeCount = count;
+    ++ctx->generation;
     ctx->roundShapes = false;
     for (uint32_t i = 0; i < count; ++i) ctx->roundShapes |= shapes[i].type != B3AMD_HULL;
     return B3AMD_OK;
@@ -718,6 +629,84 @@ int b3amd_reset_kernel_stats(b3amd_context* ctx) {
     int rc = ctx->resolve_timings();
     ctx->stats.clear();
     return rc;
+}
+
+int b3amd_graph_create(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                       const b3amd_quat* d_sweeps, b3amd_convex_cache* d_caches, b3amd_manifold* d_out,
+                       b3amd_graph** out) {
+    if (!out) return fail(B3AMD_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (!ctx || !n || !d_pairs || !d_xfs || !d_out) return fail(B3AMD_ERR_ARGUMENT, "null or empty batch");
+    if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
+    if (!ctx->dPool || !ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no hull pool or shape table uploaded");
+    b3amd_graph* g = new b3amd_graph();
+    g->ctx = ctx;
+    g->generation = ctx->generation;
+    int rc;
+    if ((rc = ctx->alloc(sizeof(uint32_t) * (uint64_t)n, (void**)&g->scratch.stage)) ||
+        (rc = ctx->alloc(sizeof(SatResult) * (uint64_t)n, (void**)&g->scratch.sat)) ||
+        (rc = ctx->alloc(256, (void**)&g->scratch.counter))) {
+        ctx->release(g->scratch.stage);
+        ctx->release(g->scratch.sat);
+        delete g;
+        return rc;
+    }
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    const bool profiling = ctx->profiling;
+    ctx->profiling = false;
+    hipError_t e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
+    if (e == hipSuccess) {
+        rc = enqueue_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs),
+                           reinterpret_cast<const float*>(d_sweeps), reinterpret_cast<ConvexCache*>(d_caches),
+                           reinterpret_cast<Manifold*>(d_out), g->scratch, nullptr);
+        hipError_t e2 = hipStreamEndCapture(ctx->stream, &g->graph);
+        if (rc == B3AMD_OK && e2 != hipSuccess) rc = fail(B3AMD_ERR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e2));
+        if (rc == B3AMD_OK) {
+            e2 = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+            if (e2 != hipSuccess) rc = fail(B3AMD_ERR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(e2));
+        }
+    } else {
+        rc = fail(B3AMD_ERR_DEVICE, "hipStreamBeginCapture: %s", hipGetErrorString(e));
+    }
+    ctx->profiling = profiling;
+    if (rc != B3AMD_OK) {
+        if (g->exec) hipGraphExecDestroy(g->exec);
+        if (g->graph) hipGraphDestroy(g->graph);
+        ctx->release(g->scratch.stage);
+        ctx->release(g->scratch.sat);
+        ctx->release(g->scratch.counter);
+        delete g;
+        return rc;
+    }
+    ctx->graphs.push_back(g);
+    *out = g;
+    return B3AMD_OK;
+}
+
+int b3amd_graph_launch(b3amd_graph* g) {
+    if (!g) return fail(B3AMD_ERR_ARGUMENT, "null graph");
+    if (g->generation != g->ctx->generation)
+        return fail(B3AMD_ERR_ARGUMENT, "graph captured before the hull pool or shape table was replaced");
+    HIP_TRY(hipGraphLaunch(g->exec, g->ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_graph_destroy(b3amd_graph* g) {
+    if (!g) return B3AMD_OK;
+    b3amd_context* ctx = g->ctx;
+    hipStreamSynchronize(ctx->stream);
+    if (g->exec) hipGraphExecDestroy(g->exec);
+    if (g->graph) hipGraphDestroy(g->graph);
+    ctx->release(g->scratch.stage);
+    ctx->release(g->scratch.sat);
+    ctx->release(g->scratch.counter);
+    for (size_t i = 0; i < ctx->graphs.size(); ++i)
+        if (ctx->graphs[i] == g) {
+            ctx->graphs.erase(ctx->graphs.begin() + (long)i);
+            break;
+        }
+    delete g;
+    return B3AMD_OK;
 }
 
 }  // extern "C"

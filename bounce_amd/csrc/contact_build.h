@@ -93,24 +93,35 @@ __device__ inline void clip_to_plane(Poly& out, const Poly& in, const Plane& pl,
     }
 }
 
-// clip.cpp:243-280; returns the clipped polygon in `a` (b is scratch).
+// clip.cpp:243-280; returns the clipped polygon in `a` (b is scratch).  The two buffers
+// alternate between clip planes; the result is copied into `a` once at the end.
 template <class H>
 __device__ inline void clip_to_face(Poly& a, Poly& b, const Xf& xf, float r, uint32_t index, const H& h) {
     uint32_t begin = h.face_edge(index);
     uint32_t e = begin;
     uint32_t guard = 0;
+    Poly* src = &a;
+    Poly* dst = &b;
+    bool overflow = a.overflow;
     do {
         uint32_t edgeId = e;  // (edge->twin)->twin
         Plane pl = edge_side_plane(h, edgeId);
         pl.d += r;
         Plane wp = mul(xf, pl);
-        clip_to_plane(b, a, wp, edgeId);
-        a.count = b.count;
-        a.overflow = a.overflow || b.overflow;
-        for (uint32_t i = 0; i < b.count; ++i) a.v[i] = b.v[i];
-        if (a.count == 0) return;
+        dst->overflow = false;
+        clip_to_plane(*dst, *src, wp, edgeId);
+        overflow = overflow || dst->overflow;
+        Poly* t = src;
+        src = dst;
+        dst = t;
+        if (src->count == 0) break;
         e = h.next(e);
     } while (e != begin && ++guard < 256);
+    if (src != &a) {
+        a.count = src->count;
+        for (uint32_t i = 0; i < src->count; ++i) a.v[i] = src->v[i];
+    }
+    a.overflow = overflow;
 }
 
 // contact_cluster.cpp:22-26

@@ -62,6 +62,9 @@ struct PairInputs {
     const uint32_t* offsets; // hull -> first 16-byte unit
     uint32_t shapeCount;
     uint32_t hullCount;
+    const uint32_t* slots;   // [n] cache and manifold slot of each pair; null: slot p
+
+    __device__ uint32_t slot_of(uint32_t p) const { return slots ? slots[p] : p; }
 
     // 0: a pair kernels must not read (shape or hull index out of range, a type outside
     // sphere/capsule/hull, or typeA > typeB, which b3ContactManager::Create never makes,

@@ -44,7 +44,7 @@ __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
     } else {
         collide_capsules(m, xf1, s1, xf2, s2);
     }
-    write_manifold(a.out + p, m);
+    write_manifold(a.out + a.slot_of(p), m);
 }
 
 }  // namespace

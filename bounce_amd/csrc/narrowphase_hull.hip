@@ -33,7 +33,7 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
     if (p >= a.n) return;
     const SatResult r = a.sat[p];
     if (r.stage == kStageSkip) return;  // the cache kernel wrote this pair
-    FeatureCache* fc = a.caches ? &a.caches[p].feature : nullptr;
+    FeatureCache* fc = a.caches ? &a.caches[a.slot_of(p)].feature : nullptr;
     Contact m;
     m.pointCount = 0;
     m.overflow = false;
@@ -41,7 +41,7 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         if (r.stage == kStageSepFace1) set_feature(fc, kCacheSeparation, kFeatureFace1, r.f1Index, r.f1Index);
         else if (r.stage == kStageSepFace2) set_feature(fc, kCacheSeparation, kFeatureFace2, r.f2Index, r.f2Index);
         else set_feature(fc, kCacheSeparation, kFeatureEdge1, r.eIndex1, r.eIndex2);
-        write_manifold(a.out + p, m);
+        write_manifold(a.out + a.slot_of(p), m);
         return;
     }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
@@ -57,21 +57,21 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     } else if (r.f1Sep + kTol > r.f2Sep) {
         build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     } else {
         build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     }
@@ -80,28 +80,28 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     } else if (r.f1Sep > r.f2Sep) {
         build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     } else {
         build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             return;
         }
     }
     // Last resort: a contact between the edge lines.
     build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
     set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-    write_manifold(a.out + p, m);
+    write_manifold(a.out + a.slot_of(p), m);
 }
 
 // ------------------------------------------------------------------ feature-cache kernel
@@ -115,13 +115,13 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
             Contact m;
             m.pointCount = 0;
             m.overflow = false;
-            write_manifold(a.out + p, m);
-            a.out[p].flags = kFlagInvalidPair;
+            write_manifold(a.out + a.slot_of(p), m);
+            a.out[a.slot_of(p)].flags = kFlagInvalidPair;
         }
         a.stage[p] = cls == 0u ? kStageSkip : kStageConvex;
         return false;
     }
-    FeatureCache* fc = a.caches ? &a.caches[p].feature : nullptr;
+    FeatureCache* fc = a.caches ? &a.caches[a.slot_of(p)].feature : nullptr;
     if (!fc || fc->state == kCacheEmpty) {
         // ReadState on an empty cache evaluates face 0 of hull 1 and the result is
         // discarded by the flush (sat.cpp:182-185): go straight to the SAT.
@@ -156,7 +156,7 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
     m.pointCount = 0;
     m.overflow = false;
     if (state0 == kCacheSeparation && state1 == kCacheSeparation) {
-        write_manifold(a.out + p, m);
+        write_manifold(a.out + a.slot_of(p), m);
         a.stage[p] = kStageSkip;
         return false;
     }
@@ -176,7 +176,7 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
             default: break;
         }
         if (m.pointCount > 0) {
-            write_manifold(a.out + p, m);
+            write_manifold(a.out + a.slot_of(p), m);
             a.stage[p] = kStageSkip;
             return false;
         }

@@ -1,0 +1,618 @@
+// Serving loop: many worlds' narrowphase steps on one context, continuously batched.
+//
+// The reference runs one b3World per b3ContactManager::UpdateContacts call
+// (contact_manager.cpp:167-231); a server runs the narrowphase of many such worlds on one
+// GPU.  A world keeps its contact list, feature caches and manifolds in HBM; a request is
+// one step of one world (its body transforms, optionally sweeps).  A worker thread packs
+// whole pending requests, oldest first and at most one per world, into one device batch of
+// up to max_batch_pairs contacts; a gather kernel expands each request's body transforms
+// into per-contact inputs, the batch kernels (cache, convex, SAT, contact) run with each
+// contact's cache and manifold slot in the world's slab range, and the request completes
+// when the batch's event has fired.  Two batches are in flight: the next is packed and
+// copied while the previous computes.
+//
+// Results do not depend on the batching: every contact is computed from its own inputs
+// and cache slot only, so a world's manifolds are bit-identical alone or in any mix.
+#include <hip/hip_runtime.h>
+
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <mutex>
+#include <thread>
+#include <unordered_map>
+#include <vector>
+
+#include "bounce_amd.h"
+#include "context.h"
+#include "kernels.h"
+
+namespace b3amd {
+
+// One request inside a batch (device copy).
+struct ReqDesc {
+    uint32_t pairOffset;  // first batch contact of the request
+    uint32_t pairCount;
+    uint32_t bodyOffset;  // first body of the request in the batch body arrays
+    uint32_t slotBase;    // the world's first cache/manifold slot
+    const uint32_t* bodyShapes;
+    const uint32_t* bodyPairs;
+};
+
+struct GatherArgs {
+    uint32_t n;              // contacts in the batch
+    uint32_t requests;
+    const ReqDesc* descs;    // sorted by pairOffset
+    const float* bodyXfs;    // [bodies][12]
+    const float* bodySweeps; // [bodies][8] (orientation0, orientation) or null
+    uint32_t* pairs;         // [n][2] out
+    float* xfs;              // [n][24] out
+    float* sweeps;           // [n][16] out, or null
+    uint32_t* slots;         // [n] out
+};
+
+// Per contact: find its request, then read the world's contact (body indices) and the
+// bodies' shapes and transforms.
+__global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    uint32_t lo = 0, hi = a.requests;  // last desc with pairOffset <= k
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (a.descs[mid].pairOffset <= k) lo = mid;
+        else hi = mid;
+    }
+    const ReqDesc d = a.descs[lo];
+    const uint32_t i = k - d.pairOffset;
+    const uint32_t ba = d.bodyPairs[2 * i], bb = d.bodyPairs[2 * i + 1];
+    a.pairs[2 * k] = d.bodyShapes[ba];
+    a.pairs[2 * k + 1] = d.bodyShapes[bb];
+    const float* xa = a.bodyXfs + 12 * (size_t)(d.bodyOffset + ba);
+    const float* xb = a.bodyXfs + 12 * (size_t)(d.bodyOffset + bb);
+    float* xo = a.xfs + 24 * (size_t)k;
+#pragma unroll
+    for (int t = 0; t < 12; ++t) {
+        xo[t] = xa[t];
+        xo[12 + t] = xb[t];
+    }
+    if (a.sweeps) {
+        float* so = a.sweeps + 16 * (size_t)k;
+        if (a.bodySweeps) {
+            const float* sa = a.bodySweeps + 8 * (size_t)(d.bodyOffset + ba);
+            const float* sb = a.bodySweeps + 8 * (size_t)(d.bodyOffset + bb);
+#pragma unroll
+            for (int t = 0; t < 8; ++t) {
+                so[t] = sa[t];
+                so[8 + t] = sb[t];
+            }
+        }
+    }
+    a.slots[k] = d.slotBase + i;
+}
+
+}  // namespace b3amd
+
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+struct World {
+    bool live = false;
+    uint32_t bodyCount = 0, pairCount = 0;
+    uint32_t* dBodyShapes = nullptr;
+    uint32_t* dBodyPairs = nullptr;
+    uint32_t slotBase = 0;
+    uint64_t lastTicket = 0;  // most recent request (0: none yet)
+};
+
+struct Request {
+    uint64_t ticket = 0;
+    uint32_t world = 0;
+    const float* xfs = nullptr;     // [bodies][12]
+    const float* sweeps = nullptr;  // [bodies][8] or null
+    std::vector<float> ownXfs, ownSweeps;
+    Clock::time_point submit;
+    double latencyMs = 0.0;
+    bool done = false;
+};
+
+// Pinned host staging of one batch.
+struct Staging {
+    float* bodyXfs = nullptr;
+    float* bodySweeps = nullptr;
+    ReqDesc* descs = nullptr;
+};
+
+struct Batch {
+    std::vector<Request*> requests;
+    hipEvent_t done = nullptr;
+    int staging = 0;
+};
+
+}  // namespace
+
+struct b3amd_server {
+    b3amd_context* ctx = nullptr;
+    uint32_t maxPairs = 0, maxBodies = 0, maxRequests = 0, slabSlots = 0;
+
+    // Device state.
+    ConvexCache* dCaches = nullptr;     // [slabSlots]
+    Manifold* dManifolds = nullptr;     // [slabSlots]
+    float* dBodyXfs = nullptr;
+    float* dBodySweeps = nullptr;
+    ReqDesc* dDescs = nullptr;
+    uint32_t* dPairs = nullptr;
+    float* dXfs = nullptr;
+    float* dSweeps = nullptr;
+    uint32_t* dSlots = nullptr;
+    BatchScratch scratch{nullptr, nullptr, nullptr};
+    Staging staging[2];
+
+    // Worlds and their slot ranges (first fit over free ranges).
+    std::vector<World> worlds;
+    std::vector<std::pair<uint32_t, uint32_t>> freeRanges;  // (offset, length), sorted
+
+    // Requests.
+    std::mutex mu;
+    std::condition_variable cvWork, cvDone;
+    std::deque<Request*> pending;
+    std::unordered_map<uint64_t, Request*> requests;
+    std::deque<Batch> inflight;
+    uint64_t nextTicket = 1;
+    bool stop = false;
+    std::thread worker;
+    int workerError = B3AMD_OK;
+    std::string workerMessage;
+
+    // Statistics.
+    b3amd_server_stats stats{};
+};
+
+namespace {
+
+// Empty b3ConvexCache records (b3SimplexCache zeroed, feature state empty) for a slot range.
+__global__ void init_slots_kernel(ConvexCache* caches, Manifold* manifolds, uint32_t base, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= count) return;
+    ConvexCache c;
+    for (int t = 0; t < 5; ++t) c.simplex[t] = 0u;
+    c.feature.state = kCacheEmpty;
+    c.feature.type = kFeatureEdge1;
+    c.feature.index1 = 0u;
+    c.feature.index2 = 0u;
+    caches[base + i] = c;
+    manifolds[base + i].pointCount = 0u;
+    manifolds[base + i].flags = 0u;
+}
+
+bool take_range(b3amd_server* s, uint32_t n, uint32_t* base) {
+    for (size_t i = 0; i < s->freeRanges.size(); ++i) {
+        auto& r = s->freeRanges[i];
+        if (r.second >= n) {
+            *base = r.first;
+            r.first += n;
+            r.second -= n;
+            if (r.second == 0) s->freeRanges.erase(s->freeRanges.begin() + (long)i);
+            return true;
+        }
+    }
+    return false;
+}
+
+void give_range(b3amd_server* s, uint32_t base, uint32_t n) {
+    auto& fr = s->freeRanges;
+    size_t i = 0;
+    while (i < fr.size() && fr[i].first < base) ++i;
+    fr.insert(fr.begin() + (long)i, {base, n});
+    // Merge with the neighbours.
+    if (i + 1 < fr.size() && fr[i].first + fr[i].second == fr[i + 1].first) {
+        fr[i].second += fr[i + 1].second;
+        fr.erase(fr.begin() + (long)i + 1);
+    }
+    if (i > 0 && fr[i - 1].first + fr[i - 1].second == fr[i].first) {
+        fr[i - 1].second += fr[i].second;
+        fr.erase(fr.begin() + (long)i);
+    }
+}
+
+// Packs pending requests into staging buffer `si` and enqueues the batch.  Called by the
+// worker with the lock held; returns false when nothing was taken.
+bool launch_batch(b3amd_server* s, int si, Batch& batch) {
+    Staging& st = s->staging[si];
+    std::vector<uint32_t> seen;  // worlds already taken or passed over in this batch
+    uint32_t pairs = 0, bodies = 0, nreq = 0;
+    bool anySweeps = false;
+    for (auto it = s->pending.begin(); it != s->pending.end();) {
+        Request* r = *it;
+        const World& w = s->worlds[r->world];
+        bool dup = false;
+        for (uint32_t v : seen)
+            if (v == r->world) dup = true;
+        if (dup) {  // keep the world's requests in order: one per batch
+            ++it;
+            continue;
+        }
+        seen.push_back(r->world);
+        if (nreq == s->maxRequests || pairs + w.pairCount > s->maxPairs || bodies + w.bodyCount > s->maxBodies) break;
+        ReqDesc& d = st.descs[nreq];
+        d.pairOffset = pairs;
+        d.pairCount = w.pairCount;
+        d.bodyOffset = bodies;
+        d.slotBase = w.slotBase;
+        d.bodyShapes = w.dBodyShapes;
+        d.bodyPairs = w.dBodyPairs;
+        std::memcpy(st.bodyXfs + 12 * (size_t)bodies, r->xfs, 48 * (size_t)w.bodyCount);
+        if (r->sweeps) {
+            std::memcpy(st.bodySweeps + 8 * (size_t)bodies, r->sweeps, 32 * (size_t)w.bodyCount);
+            anySweeps = true;
+        }
+        pairs += w.pairCount;
+        bodies += w.bodyCount;
+        ++nreq;
+        batch.requests.push_back(r);
+        it = s->pending.erase(it);
+    }
+    if (nreq == 0) return false;
+    if (anySweeps) {
+        // Requests without sweeps get identity orientations (what a null sweep array means).
+        for (uint32_t q = 0; q < nreq; ++q) {
+            if (batch.requests[q]->sweeps) continue;
+            float* o = st.bodySweeps + 8 * (size_t)st.descs[q].bodyOffset;
+            const uint32_t nb = s->worlds[batch.requests[q]->world].bodyCount;
+            for (uint32_t b = 0; b < nb; ++b, o += 8) {
+                o[0] = o[1] = o[2] = 0.0f, o[3] = 1.0f;
+                o[4] = o[5] = o[6] = 0.0f, o[7] = 1.0f;
+            }
+        }
+    }
+    b3amd_context* ctx = s->ctx;
+    hipStream_t stream = ctx->stream;
+    batch.staging = si;
+    bool ok = hipMemcpyAsync(s->dBodyXfs, st.bodyXfs, 48ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess &&
+              (!anySweeps ||
+               hipMemcpyAsync(s->dBodySweeps, st.bodySweeps, 32ull * bodies, hipMemcpyHostToDevice, stream) ==
+                   hipSuccess) &&
+              hipMemcpyAsync(s->dDescs, st.descs, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, stream) == hipSuccess;
+    if (ok) {
+        GatherArgs g;
+        g.n = pairs;
+        g.requests = nreq;
+        g.descs = s->dDescs;
+        g.bodyXfs = s->dBodyXfs;
+        g.bodySweeps = anySweeps ? s->dBodySweeps : nullptr;
+        g.pairs = s->dPairs;
+        g.xfs = s->dXfs;
+        g.sweeps = anySweeps ? s->dSweeps : nullptr;
+        g.slots = s->dSlots;
+        gather_kernel<<<(pairs + 255) / 256, 256, 0, stream>>>(g);
+        ok = hipGetLastError() == hipSuccess;
+    }
+    if (ok)
+        ok = enqueue_batch(ctx, pairs, s->dPairs, s->dXfs, anySweeps ? s->dSweeps : nullptr, s->dCaches,
+                           s->dManifolds, s->scratch, s->dSlots) == B3AMD_OK;
+    if (ok) ok = hipEventRecord(batch.done, stream) == hipSuccess;
+    if (!ok) {
+        s->workerError = B3AMD_ERR_DEVICE;
+        s->workerMessage = b3amd_last_error();
+    }
+    s->stats.batches += 1;
+    s->stats.requests += nreq;
+    s->stats.pairs += pairs;
+    if (pairs > s->stats.max_batch_pairs) s->stats.max_batch_pairs = pairs;
+    return true;
+}
+
+void complete_batch(b3amd_server* s, Batch& b) {
+    const Clock::time_point now = Clock::now();
+    for (Request* r : b.requests) {
+        r->latencyMs = std::chrono::duration<double, std::milli>(now - r->submit).count();
+        r->done = true;
+    }
+    b.requests.clear();
+}
+
+void worker_main(b3amd_server* s) {
+    hipSetDevice(s->ctx->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        s->cvWork.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->inflight.empty(); });
+        if (s->stop && s->pending.empty() && s->inflight.empty()) break;
+        // Retire finished batches without blocking.
+        bool retired = false;
+        while (!s->inflight.empty() && hipEventQuery(s->inflight.front().done) == hipSuccess) {
+            complete_batch(s, s->inflight.front());
+            hipEvent_t e = s->inflight.front().done;
+            s->inflight.pop_front();
+            hipEventDestroy(e);
+            retired = true;
+        }
+        if (retired) s->cvDone.notify_all();
+        if (!s->pending.empty() && s->inflight.size() < 2 && s->workerError == B3AMD_OK) {
+            // The staging buffer not used by the batch still in flight.
+            int si = 0;
+            if (!s->inflight.empty()) si = 1 - s->inflight.front().staging;
+            Batch b;
+            if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) {
+                s->workerError = B3AMD_ERR_DEVICE;
+                s->workerMessage = "hipEventCreate failed";
+                continue;
+            }
+            if (launch_batch(s, si, b)) s->inflight.push_back(std::move(b));
+            else hipEventDestroy(b.done);
+            continue;
+        }
+        if (!s->inflight.empty()) {
+            // Nothing to pack (or both buffers busy): wait for the oldest batch.
+            hipEvent_t e = s->inflight.front().done;
+            lk.unlock();
+            hipEventSynchronize(e);
+            lk.lock();
+            continue;
+        }
+        if (s->workerError != B3AMD_OK) {
+            // Device failure: fail everything queued so waiters return.
+            for (Request* r : s->pending) r->done = true;
+            s->pending.clear();
+            s->cvDone.notify_all();
+        }
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t max_batch_bodies,
+                        uint32_t slab_slots, b3amd_server** out) {
+    if (!out) return fail(B3AMD_ERR_ARGUMENT, "out is null");
+    *out = nullptr;
+    if (!ctx || !max_batch_pairs || !max_batch_bodies || !slab_slots)
+        return fail(B3AMD_ERR_ARGUMENT, "null context or zero capacity");
+    b3amd_server* s = new b3amd_server();
+    s->ctx = ctx;
+    s->maxPairs = max_batch_pairs;
+    s->maxBodies = max_batch_bodies;
+    s->maxRequests = max_batch_pairs < 65536u ? max_batch_pairs : 65536u;
+    s->slabSlots = slab_slots;
+    const uint64_t P = max_batch_pairs, B = max_batch_bodies;
+    int rc = B3AMD_OK;
+    auto A = [&](uint64_t bytes, void** p) {
+        if (rc == B3AMD_OK) rc = ctx->alloc(bytes, p);
+    };
+    A(sizeof(ConvexCache) * (uint64_t)slab_slots, (void**)&s->dCaches);
+    A(sizeof(Manifold) * (uint64_t)slab_slots, (void**)&s->dManifolds);
+    A(48 * B, (void**)&s->dBodyXfs);
+    A(32 * B, (void**)&s->dBodySweeps);
+    A(sizeof(ReqDesc) * (uint64_t)s->maxRequests, (void**)&s->dDescs);
+    A(8 * P, (void**)&s->dPairs);
+    A(96 * P, (void**)&s->dXfs);
+    A(64 * P, (void**)&s->dSweeps);
+    A(4 * P, (void**)&s->dSlots);
+    A(4 * P, (void**)&s->scratch.stage);
+    A(sizeof(SatResult) * P, (void**)&s->scratch.sat);
+    A(256, (void**)&s->scratch.counter);
+    for (int i = 0; i < 2 && rc == B3AMD_OK; ++i) {
+        Staging& st = s->staging[i];
+        if (hipHostMalloc((void**)&st.bodyXfs, 48 * B, 0) != hipSuccess ||
+            hipHostMalloc((void**)&st.bodySweeps, 32 * B, 0) != hipSuccess ||
+            hipHostMalloc((void**)&st.descs, sizeof(ReqDesc) * s->maxRequests, 0) != hipSuccess)
+            rc = fail(B3AMD_ERR_DEVICE, "hipHostMalloc of the staging buffers failed");
+    }
+    if (rc != B3AMD_OK) {
+        std::string msg = b3amd_last_error();
+        b3amd_server_destroy(s);
+        return fail(rc, "%s", msg.c_str());
+    }
+    s->freeRanges.push_back({0u, slab_slots});
+    s->worker = std::thread(worker_main, s);
+    *out = s;
+    return B3AMD_OK;
+}
+
+int b3amd_server_destroy(b3amd_server* s) {
+    if (!s) return B3AMD_OK;
+    if (s->worker.joinable()) {
+        {
+            std::lock_guard<std::mutex> lk(s->mu);
+            s->stop = true;
+        }
+        s->cvWork.notify_all();
+        s->worker.join();
+    }
+    b3amd_context* ctx = s->ctx;
+    hipStreamSynchronize(ctx->stream);
+    for (World& w : s->worlds) {
+        if (!w.live) continue;
+        ctx->release(w.dBodyShapes);
+        ctx->release(w.dBodyPairs);
+    }
+    void* bufs[] = {s->dCaches, s->dManifolds, s->dBodyXfs, s->dBodySweeps, s->dDescs, s->dPairs, s->dXfs,
+                    s->dSweeps, s->dSlots, s->scratch.stage, s->scratch.sat, s->scratch.counter};
+    for (void* p : bufs) ctx->release(p);
+    for (Staging& st : s->staging) {
+        if (st.bodyXfs) hipHostFree(st.bodyXfs);
+        if (st.bodySweeps) hipHostFree(st.bodySweeps);
+        if (st.descs) hipHostFree(st.descs);
+    }
+    for (auto& kv : s->requests) delete kv.second;
+    delete s;
+    return B3AMD_OK;
+}
+
+int b3amd_world_create(b3amd_server* s, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
+                       const uint32_t* body_pairs, uint32_t* world) {
+    if (!s || !world || !body_shapes || !body_pairs || !body_count || !pair_count)
+        return fail(B3AMD_ERR_ARGUMENT, "null argument or empty world");
+    b3amd_context* ctx = s->ctx;
+    if (pair_count > s->maxPairs || body_count > s->maxBodies)
+        return fail(B3AMD_ERR_ARGUMENT, "world of %u contacts / %u bodies exceeds the batch capacity %u / %u",
+                    pair_count, body_count, s->maxPairs, s->maxBodies);
+    for (uint32_t b = 0; b < body_count; ++b)
+        if (body_shapes[b] >= ctx->shapeCount)
+            return fail(B3AMD_ERR_ARGUMENT, "body %u: shape %u of %u", b, body_shapes[b], ctx->shapeCount);
+    for (uint32_t i = 0; i < pair_count; ++i) {
+        const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
+        if (a >= body_count || b >= body_count)
+            return fail(B3AMD_ERR_ARGUMENT, "contact %u refers to body %u/%u of %u", i, a, b, body_count);
+        if (ctx->shapes[body_shapes[a]].type > ctx->shapes[body_shapes[b]].type)
+            return fail(B3AMD_ERR_ARGUMENT, "contact %u: typeA > typeB (contact_manager.cpp:238-242)", i);
+    }
+    std::lock_guard<std::mutex> lk(s->mu);
+    uint32_t base = 0;
+    if (!take_range(s, pair_count, &base))
+        return fail(B3AMD_ERR_BUDGET, "no room for %u contact slots in the server's slab", pair_count);
+    World w;
+    int rc = ctx->alloc(4ull * body_count, (void**)&w.dBodyShapes);
+    if (rc == B3AMD_OK) rc = ctx->alloc(8ull * pair_count, (void**)&w.dBodyPairs);
+    if (rc != B3AMD_OK) {
+        std::string msg = b3amd_last_error();
+        ctx->release(w.dBodyShapes);
+        give_range(s, base, pair_count);
+        return fail(rc, "%s", msg.c_str());
+    }
+    // Uploads on the context stream: ordered before any batch that uses the world.
+    HIP_TRY(hipMemcpyAsync(w.dBodyShapes, body_shapes, 4ull * body_count, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(w.dBodyPairs, body_pairs, 8ull * pair_count, hipMemcpyHostToDevice, ctx->stream));
+    init_slots_kernel<<<(pair_count + 255) / 256, 256, 0, ctx->stream>>>(s->dCaches, s->dManifolds, base, pair_count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    w.live = true;
+    w.bodyCount = body_count;
+    w.pairCount = pair_count;
+    w.slotBase = base;
+    uint32_t id = 0;
+    while (id < s->worlds.size() && s->worlds[id].live) ++id;
+    if (id == s->worlds.size()) s->worlds.push_back(w);
+    else s->worlds[id] = w;
+    *world = id;
+    return B3AMD_OK;
+}
+
+int b3amd_world_destroy(b3amd_server* s, uint32_t world) {
+    if (!s) return fail(B3AMD_ERR_ARGUMENT, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    if (world >= s->worlds.size() || !s->worlds[world].live) return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+    // Wait until no request of the world is queued or in flight.
+    const uint64_t last = s->worlds[world].lastTicket;
+    if (last) {
+        auto it = s->requests.find(last);
+        if (it != s->requests.end()) s->cvDone.wait(lk, [&] { return it->second->done; });
+    }
+    World& w = s->worlds[world];
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    s->ctx->release(w.dBodyShapes);
+    s->ctx->release(w.dBodyPairs);
+    give_range(s, w.slotBase, w.pairCount);
+    w = World();
+    return B3AMD_OK;
+}
+
+// Queues one step of a world.  copy: keep a private copy of the transforms (otherwise the
+// caller keeps them alive until the request completes); at: arrival time (null: now).
+static int submit(b3amd_server* s, uint32_t world, const float* xfs, const float* sweeps, bool copy,
+                  const Clock::time_point* at, uint64_t* ticket) {
+    if (!s || !xfs || !ticket) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    Request* r = new Request();
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (s->workerError != B3AMD_OK) {
+            delete r;
+            return fail(s->workerError, "server failed: %s", s->workerMessage.c_str());
+        }
+        if (world >= s->worlds.size() || !s->worlds[world].live) {
+            delete r;
+            return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+        }
+        World& w = s->worlds[world];
+        if (copy) {
+            r->ownXfs.assign(xfs, xfs + 12 * (size_t)w.bodyCount);
+            r->xfs = r->ownXfs.data();
+            if (sweeps) {
+                r->ownSweeps.assign(sweeps, sweeps + 8 * (size_t)w.bodyCount);
+                r->sweeps = r->ownSweeps.data();
+            }
+        } else {
+            r->xfs = xfs;
+            r->sweeps = sweeps;
+        }
+        r->ticket = s->nextTicket++;
+        r->world = world;
+        r->submit = at ? *at : Clock::now();
+        w.lastTicket = r->ticket;
+        s->requests[r->ticket] = r;
+        s->pending.push_back(r);
+        *ticket = r->ticket;
+    }
+    s->cvWork.notify_one();
+    return B3AMD_OK;
+}
+
+int b3amd_world_submit(b3amd_server* s, uint32_t world, const b3amd_transform* body_xfs,
+                       const b3amd_quat* body_sweeps, uint64_t* ticket) {
+    return submit(s, world, reinterpret_cast<const float*>(body_xfs), reinterpret_cast<const float*>(body_sweeps),
+                  true, nullptr, ticket);
+}
+
+int b3amd_server_wait(b3amd_server* s, uint64_t ticket, double* latency_ms) {
+    if (!s) return fail(B3AMD_ERR_ARGUMENT, "null server");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->requests.find(ticket);
+    if (it == s->requests.end()) return fail(B3AMD_ERR_ARGUMENT, "unknown or already collected ticket %llu",
+                                             (unsigned long long)ticket);
+    Request* r = it->second;
+    s->cvDone.wait(lk, [&] { return r->done; });
+    if (latency_ms) *latency_ms = r->latencyMs;
+    s->requests.erase(it);
+    delete r;
+    if (s->workerError != B3AMD_OK) return fail(s->workerError, "server failed: %s", s->workerMessage.c_str());
+    return B3AMD_OK;
+}
+
+int b3amd_world_read_manifolds(b3amd_server* s, uint32_t world, b3amd_manifold* out) {
+    if (!s || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    uint32_t base, count;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        if (world >= s->worlds.size() || !s->worlds[world].live) return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+        base = s->worlds[world].slotBase;
+        count = s->worlds[world].pairCount;
+    }
+    // On the context stream: ordered after every batch enqueued so far.
+    HIP_TRY(hipMemcpyAsync(out, s->dManifolds + base, sizeof(Manifold) * (uint64_t)count, hipMemcpyDeviceToHost,
+                           s->ctx->stream));
+    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_server_replay(b3amd_server* s, uint32_t count, const uint32_t* worlds, const double* times_s,
+                        const b3amd_transform* const* body_xfs, double* latency_ms) {
+    if (!s || !worlds || !times_s || !body_xfs || !latency_ms) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<uint64_t> tickets(count);
+    std::vector<Clock::time_point> due(count);
+    const Clock::time_point t0 = Clock::now();
+    for (uint32_t i = 0; i < count; ++i) {
+        due[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(times_s[i]));
+        while (Clock::now() < due[i]) {
+        }
+        // Latency counts from the scheduled arrival, so submission lag is included.
+        int rc = submit(s, worlds[i], reinterpret_cast<const float*>(body_xfs[i]), nullptr, false, &due[i],
+                        &tickets[i]);
+        if (rc) return rc;
+    }
+    int rc = B3AMD_OK;
+    for (uint32_t i = 0; i < count; ++i) {
+        int r = b3amd_server_wait(s, tickets[i], &latency_ms[i]);
+        if (r && rc == B3AMD_OK) rc = r;
+    }
+    return rc;
+}
+
+int b3amd_server_get_stats(b3amd_server* s, b3amd_server_stats* out) {
+    if (!s || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->stats;
+    return B3AMD_OK;
+}
+
+}  // extern "C"

@@ -146,7 +146,7 @@ int b3amd_synchronize(b3amd_context* ctx);
 
 /* ---- launch shape of the hull SAT (no reference counterpart: the reference runs the
  * queries of sat.cpp:33-173 one pair at a time on one thread) ----
- * The SAT kernel is one persistent grid draining a device work queue.  waves_per_simd
+ * The SAT kernel is one persistent grid over the batch's hull pairs.  waves_per_simd
  * (0 = 5, at most 32) is the number of resident waves per SIMD the grid is sized for; it
  * also picks the kernel's register budget (the variant built for min(max(w, 4), 6) waves).
  * max_slots (1..3) is the number of hull-1 edges each lane holds in registers during the
@@ -159,6 +159,66 @@ int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uin
  * (may be NULL).  d_out is overwritten. */
 int b3amd_autotune(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
                    b3amd_manifold* d_out, uint32_t repeats, float* best_ms);
+
+/* ---- captured batches (no reference counterpart: the reference calls
+ * b3CollideShapeAndShape per contact, contact.cpp:41-140) ----
+ * A HIP graph of one b3amd_collide_device batch on fixed device buffers, captured with the
+ * current launch shape and its own scratch memory.  Each launch re-runs the batch on
+ * whatever the buffers hold at that time, on the context stream, without synchronizing.
+ * A graph refuses to launch after the hull pool or shape table has been re-uploaded.
+ * Destroy graphs before their context (b3amd_destroy destroys any left). */
+typedef struct b3amd_graph b3amd_graph;
+int b3amd_graph_create(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                       const b3amd_quat* d_sweeps, b3amd_convex_cache* d_caches, b3amd_manifold* d_out,
+                       b3amd_graph** out);
+int b3amd_graph_launch(b3amd_graph* graph);
+int b3amd_graph_destroy(b3amd_graph* graph);
+
+/* ---- serving many worlds (no reference counterpart: the reference runs one b3World's
+ * b3ContactManager::UpdateContacts, contact_manager.cpp:167-231, per call) ----
+ * A server continuously batches the narrowphase steps of many worlds on one context.  A
+ * world is a set of bodies (each an index into the context's shape table) and a fixed
+ * contact list of body-index pairs (typeA <= typeB, in b3ContactManager list order); its
+ * contact list, feature caches (created empty, as b3ConvexContact's constructor does,
+ * convex_contact.cpp:35-36) and manifolds stay in HBM.  A request is one step: the body
+ * transforms [bodyCount] (b3Body::m_xf) and optionally the sweeps [bodyCount][2]
+ * (orientation0, orientation) read by the cached face rebuild.  A worker thread packs the
+ * oldest pending requests, at most one per world, into batches of at most
+ * max_batch_pairs contacts and max_batch_bodies bodies, two batches in flight.
+ * Admission: a world needs pair_count slots of the server's slab of slab_slots contacts
+ * (created with the server and counted in the context's HBM budget); when they do not
+ * fit, b3amd_world_create returns B3AMD_ERR_BUDGET.  While a server exists it owns the
+ * context's stream: call no other entry point on that context concurrently. */
+typedef struct b3amd_server b3amd_server;
+typedef struct b3amd_server_stats {
+    uint64_t batches;         /* device batches launched */
+    uint64_t requests;        /* world steps packed into them */
+    uint64_t pairs;           /* contacts computed */
+    uint32_t max_batch_pairs; /* largest batch */
+    uint32_t reserved;
+} b3amd_server_stats;
+int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t max_batch_bodies,
+                        uint32_t slab_slots, b3amd_server** out);
+int b3amd_server_destroy(b3amd_server* server);
+int b3amd_world_create(b3amd_server* server, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
+                       const uint32_t* body_pairs, uint32_t* world);
+/* Waits for the world's queued steps, then frees its slots. */
+int b3amd_world_destroy(b3amd_server* server, uint32_t world);
+/* Queues one step (inputs are copied before return); ticket identifies it. */
+int b3amd_world_submit(b3amd_server* server, uint32_t world, const b3amd_transform* body_xfs,
+                       const b3amd_quat* body_sweeps, uint64_t* ticket);
+/* Blocks until the step's manifolds are in the world's device slots; latency_ms (may be
+ * NULL) = time from submission to completion.  Each ticket is collected once. */
+int b3amd_server_wait(b3amd_server* server, uint64_t ticket, double* latency_ms);
+/* The world's manifolds, in contact-list order, after every step queued so far (call it
+ * after waiting for the world's last submitted step). */
+int b3amd_world_read_manifolds(b3amd_server* server, uint32_t world, b3amd_manifold* out);
+/* Load generator: submits request i (world worlds[i], transforms body_xfs[i], which must
+ * stay valid until the call returns) at times_s[i] seconds after the call starts, waits
+ * for all, and writes each request's latency from its scheduled arrival. */
+int b3amd_server_replay(b3amd_server* server, uint32_t count, const uint32_t* worlds, const double* times_s,
+                        const b3amd_transform* const* body_xfs, double* latency_ms);
+int b3amd_server_get_stats(b3amd_server* server, b3amd_server_stats* out);
 
 /* ---- triangle meshes ----
  * b3Mesh (mesh.h:41-99): vertices [V][3], triangles [T][3] (b3Triangle v1, v2, v3).  The

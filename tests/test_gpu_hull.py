@@ -175,3 +175,42 @@ def test_gpu_autotune(ctx, lib, oracle):
         for b in (dp, dx, do):
             b.free()
         ctx.set_sat_tuning(0, 3)
+
+
+def test_gpu_graph_replay_matches_eager(ctx, lib):
+    """A captured batch replayed over changing transforms with persistent caches equals
+    the eager batch on every replay (eager work and synchronisations in between); a graph
+    refuses to launch after the shape table is replaced."""
+    say("case: graph replay")
+    pool, shapes, pairs, xfs = hull_workload(lib, 1111, 200, 4000)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    n = len(pairs)
+    dp, dx, do = ctx.alloc(pairs.nbytes), ctx.alloc(xfs.nbytes), ctx.alloc(n * MANIFOLD_DTYPE.itemsize)
+    dc = ctx.alloc(n * empty_caches(1).itemsize)
+    try:
+        dp.upload(pairs)
+        dc.upload(empty_caches(n))
+        cur = np.array(xfs, np.float32).reshape(n, 2, 12)
+        dx.upload(cur)
+        rng = np.random.default_rng(11)
+        eager_caches = empty_caches(n)
+        with ctx.capture(n, dp, dx, do, d_caches=dc) as g:
+            for step in range(4):
+                want = ctx.collide(pairs, cur, caches=eager_caches)
+                dx.upload(cur)
+                g.launch()
+                ctx.synchronize()
+                got = do.download(np.zeros(n, MANIFOLD_DTYPE))
+                got_c = dc.download(empty_caches(n))
+                assert_manifolds_equal(got, want, "replay %d" % step)
+                assert_caches_equal(got_c, eager_caches)
+                cur = cur.copy()
+                cur[:, 1, 9:12] += rng.normal(0.0, 0.01, size=(n, 3)).astype(np.float32)
+            ctx.upload_shapes(shapes)
+            with pytest.raises(bounce_amd.BounceError) as e:
+                g.launch()
+            assert e.value.code == bounce_amd.ERR_ARGUMENT
+    finally:
+        for b in (dp, dx, do, dc):
+            b.free()
