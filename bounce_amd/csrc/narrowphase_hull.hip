@@ -33,7 +33,9 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
     if (p >= a.n) return;
     const SatResult r = a.sat[p];
     if (r.stage == kStageSkip) return;  // the cache kernel wrote this pair
-    FeatureCache* fc = a.caches ? &a.caches[a.slot_of(p)].feature : nullptr;
+    const uint32_t slot = a.slot_of(p);
+    Manifold* const out = a.out + slot;
+    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
     Contact m;
     m.pointCount = 0;
     m.overflow = false;
@@ -41,7 +43,7 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         if (r.stage == kStageSepFace1) set_feature(fc, kCacheSeparation, kFeatureFace1, r.f1Index, r.f1Index);
         else if (r.stage == kStageSepFace2) set_feature(fc, kCacheSeparation, kFeatureFace2, r.f2Index, r.f2Index);
         else set_feature(fc, kCacheSeparation, kFeatureEdge1, r.eIndex1, r.eIndex2);
-        write_manifold(a.out + a.slot_of(p), m);
+        write_manifold(out, m);
         return;
     }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
@@ -57,21 +59,21 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     } else if (r.f1Sep + kTol > r.f2Sep) {
         build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     } else {
         build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     }
@@ -80,54 +82,52 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     } else if (r.f1Sep > r.f2Sep) {
         build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     } else {
         build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
         if (m.pointCount > 0) {
             set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(a.out + a.slot_of(p), m);
+            write_manifold(out, m);
             return;
         }
     }
     // Last resort: a contact between the edge lines.
     build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
     set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-    write_manifold(a.out + a.slot_of(p), m);
+    write_manifold(out, m);
 }
 
 // ------------------------------------------------------------------ feature-cache kernel
 
 // b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush.
-// Returns true when the pair needs the full SAT.
-__device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
+// Writes the manifold through `out` and the cache through `fc` when the pair is resolved
+// here; returns the pair's stage (kStageOverlap when it needs the full SAT).
+__device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, Manifold* out, FeatureCache* fc) {
     const uint32_t cls = a.classify(p);
     if (cls != 1u) {
         if (cls == 0u) {
             Contact m;
             m.pointCount = 0;
             m.overflow = false;
-            write_manifold(a.out + a.slot_of(p), m);
-            a.out[a.slot_of(p)].flags = kFlagInvalidPair;
+            write_manifold(out, m);
+            out->flags = kFlagInvalidPair;
         }
-        a.stage[p] = cls == 0u ? kStageSkip : kStageConvex;
-        return false;
+        return cls == 0u ? kStageSkip : kStageConvex;
     }
-    FeatureCache* fc = a.caches ? &a.caches[a.slot_of(p)].feature : nullptr;
     if (!fc || fc->state == kCacheEmpty) {
         // ReadState on an empty cache evaluates face 0 of hull 1 and the result is
         // discarded by the flush (sat.cpp:182-185): go straight to the SAT.
         if (fc) fc->state = kCacheEmpty;
-        a.stage[p] = kStageOverlap;
-        return true;
+        return kStageOverlap;
     }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
@@ -142,8 +142,7 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
                          (type == kFeatureFace1 && i1 < h1.F) || (type == kFeatureFace2 && i1 < h2.F);
     if (!inRange || state0 > kCacheEmpty) {
         fc->state = kCacheEmpty;
-        a.stage[p] = kStageOverlap;
-        return true;
+        return kStageOverlap;
     }
     uint32_t state1;
     switch (type) {
@@ -156,9 +155,8 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
     m.pointCount = 0;
     m.overflow = false;
     if (state0 == kCacheSeparation && state1 == kCacheSeparation) {
-        write_manifold(a.out + a.slot_of(p), m);
-        a.stage[p] = kStageSkip;
-        return false;
+        write_manifold(out, m);
+        return kStageSkip;
     }
     if (state0 == kCacheOverlap && state1 == kCacheOverlap) {
         Quat q10{0, 0, 0, 1}, q1{0, 0, 0, 1}, q20{0, 0, 0, 1}, q2{0, 0, 0, 1};
@@ -176,20 +174,28 @@ __device__ __forceinline__ bool cache_pair(const CacheArgs& a, uint32_t p) {
             default: break;
         }
         if (m.pointCount > 0) {
-            write_manifold(a.out + a.slot_of(p), m);
-            a.stage[p] = kStageSkip;
-            return false;
+            write_manifold(out, m);
+            return kStageSkip;
         }
     }
     fc->state = kCacheEmpty;
-    a.stage[p] = kStageOverlap;
-    return true;
+    return kStageOverlap;
 }
 
+// Every destination is addressed once, before the divergent branches, and the stage words
+// are stored unconditionally at the end, so no store address is a merge of per-path
+// values.
 __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= a.n) return;
-    if (!cache_pair(a, p)) a.sat[p].stage = kStageSkip;
+    const uint32_t slot = a.slot_of(p);
+    Manifold* const out = a.out + slot;
+    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+    uint32_t* const stage = a.stage + p;
+    SatResult* const sat = a.sat + p;
+    const uint32_t st = cache_pair(a, p, out, fc);
+    *stage = st;
+    sat->stage = st == kStageOverlap ? kStageOverlap : kStageSkip;
 }
 
 // ------------------------------------------------------------------ launchers
