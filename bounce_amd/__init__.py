@@ -73,6 +73,9 @@ def load_library():
         "b3amd_world_read_manifolds": (i32, [vp, u32, vp]),
         "b3amd_server_replay": (i32, [vp, u32, vp, vp, vp, vp]),
         "b3amd_server_get_stats": (i32, [vp, vp]),
+        "b3amd_set_counting": (i32, [vp, i32]),
+        "b3amd_get_counters": (i32, [vp, vp]),
+        "b3amd_reset_counters": (i32, [vp]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -326,6 +329,21 @@ class Context:
         ms = ctypes.c_float()
         _check(self.lib.b3amd_autotune(self.handle, n, d_pairs.ptr, d_xfs.ptr, d_out.ptr, repeats, ctypes.byref(ms)))
         return self.sat_tuning(), ms.value
+
+    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters")
+
+    def set_counting(self, on):
+        """Statistics counters (b3_convexCalls, b3_convexCacheHits, b3_gjk*) on or off."""
+        _check(self.lib.b3amd_set_counting(self.handle, 1 if on else 0))
+
+    def counters(self):
+        raw = np.zeros(5, np.uint64)
+        _check(self.lib.b3amd_get_counters(self.handle, ptr(raw)))
+        vals = [int(raw[0]), int(raw[1]), int(raw[2]), int(raw[3]), int(raw[4]) & 0xFFFFFFFF]
+        return dict(zip(self.COUNTERS, vals))
+
+    def reset_counters(self):
+        _check(self.lib.b3amd_reset_counters(self.handle))
 
     def set_profiling(self, on):
         _check(self.lib.b3amd_set_profiling(self.handle, 1 if on else 0))

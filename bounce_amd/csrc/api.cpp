@@ -191,6 +191,7 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     in.shapeCount = ctx->shapeCount;
     in.hullCount = ctx->hullCount;
     in.slots = slots;
+    in.counters = ctx->counting ? ctx->dCounters : nullptr;
     CacheArgs ca;
     static_cast<PairInputs&>(ca) = in;
     ca.sweeps = sweeps;
@@ -256,6 +257,12 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
     if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
         delete ctx;
         return fail(B3AMD_ERR_DEVICE, "hipStreamCreate failed");
+    }
+    if (ctx->alloc(sizeof(Counters), (void**)&ctx->dCounters) != B3AMD_OK ||
+        hipMemset(ctx->dCounters, 0, sizeof(Counters)) != hipSuccess) {
+        std::string msg = b3amd_last_error();
+        b3amd_destroy(ctx);
+        return fail(B3AMD_ERR_DEVICE, "counter allocation failed: %s", msg.c_str());
     }
     *out = ctx;
     return B3AMD_OK;
@@ -519,6 +526,7 @@ int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const ui
         return q;
     };
     MeshArgs a;
+    a.counters = ctx->counting ? ctx->dCounters : nullptr;
     a.n = n;
     a.shapeA = reinterpret_cast<uint32_t*>(take(bShape));
     a.xfs = reinterpret_cast<float*>(take(bXf));
@@ -706,6 +714,26 @@ int b3amd_graph_destroy(b3amd_graph* g) {
             break;
         }
     delete g;
+    return B3AMD_OK;
+}
+
+int b3amd_set_counting(b3amd_context* ctx, int on) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    ctx->counting = on != 0;
+    return B3AMD_OK;
+}
+
+int b3amd_get_counters(b3amd_context* ctx, b3amd_counters* out) {
+    if (!ctx || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    static_assert(sizeof(b3amd_counters) == sizeof(Counters), "counter layout");
+    HIP_TRY(hipMemcpyAsync(out, ctx->dCounters, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_reset_counters(b3amd_context* ctx) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    HIP_TRY(launch_zero_words(reinterpret_cast<uint32_t*>(ctx->dCounters), sizeof(Counters) / 4, ctx->stream));
     return B3AMD_OK;
 }
 

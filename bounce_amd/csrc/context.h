@@ -73,6 +73,8 @@ struct b3amd_context {
     uint32_t* dCounter = nullptr;  // tail counter of the persistent SAT kernel
     uint32_t scratchCap = 0;
     SatTuning tuning{0u, 3u};
+    Counters* dCounters = nullptr;  // statistics totals (b3amd_get_counters)
+    bool counting = false;
     void* dIn = nullptr;
     uint64_t dInCap = 0;
 

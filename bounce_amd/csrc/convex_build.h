@@ -205,6 +205,7 @@ __device__ inline void solve4(Simplex& s) {
 struct GJKOut {
     V3 point1, point2;
     float distance;
+    uint32_t iterations;  // support calls (gjk.cpp:671-673)
 };
 
 // b3GJK with an empty cache.  A vertex added to the simplex starts with weight 0 (the
@@ -219,7 +220,8 @@ __device__ inline GJKOut gjk(const Xf& xf1, const Proxy& p1, const Xf& xf2, cons
     s.v[0].index2 = 0;
     s.count = 1;
     uint32_t save1[4], save2[4];
-    for (uint32_t iter = 0; iter < 20;) {
+    uint32_t iter = 0;
+    while (iter < 20) {
         const uint32_t saveCount = s.count;
         for (uint32_t i = 0; i < saveCount; ++i) {
             save1[i] = s.v[i].index1;
@@ -260,6 +262,7 @@ __device__ inline GJKOut gjk(const Xf& xf1, const Proxy& p1, const Xf& xf2, cons
     }
     // gjk.cpp:168-198
     GJKOut o;
+    o.iterations = iter;
     const SVert* v = s.v;
     if (s.count == 1) {
         o.point1 = v[0].point1;
@@ -574,6 +577,8 @@ template <class H2>
 __device__ inline void collide_sphere_hull(Contact& m, const Xf& xf1, const ShapeRec& s1, const Xf& xf2, const H2& h2,
                                     float r2) {
     GJKOut g = gjk(xf1, round_proxy(s1), xf2, hull_proxy(h2));
+    m.gjkCalls += 1;
+    m.gjkIters += g.iterations;
     const V3 c{s1.v[0], s1.v[1], s1.v[2]};
     float totalRadius = s1.radius + r2;
     if (g.distance > totalRadius) return;
@@ -677,6 +682,8 @@ template <class H2>
 __device__ inline void collide_capsule_hull(Contact& m, const Xf& xf1, const ShapeRec& s1, const Xf& xf2, const H2& h2,
                                      float r2) {
     GJKOut g = gjk(xf1, round_proxy(s1), xf2, hull_proxy(h2));
+    m.gjkCalls += 1;
+    m.gjkIters += g.iterations;
     float totalRadius = s1.radius + r2;
     if (g.distance > totalRadius) return;
     const V3 c0{s1.v[0], s1.v[1], s1.v[2]}, c1{s1.v[3], s1.v[4], s1.v[5]};

@@ -52,7 +52,25 @@ struct Contact {
     bool overflow;
     V3 n[4], p1[4], p2[4];
     uint32_t key[4][4];  // inEdge1, outEdge1, inEdge2, outEdge2
+    uint32_t gjkCalls = 0, gjkIters = 0;  // b3_gjkCalls / b3_gjkIters contributions (gjk.cpp:583,673)
 };
+
+// Adds a per-lane value to a device total with one atomic per wave.  Every lane of the
+// wave must be active (callers keep finished lanes alive to the end of the kernel).
+__device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) {
+    unsigned long long s = v;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63u) == 0 && s) atomicAdd(dst, s);
+}
+__device__ __forceinline__ void wave_max(unsigned int* dst, uint32_t v) {
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) {
+        const uint32_t o = __shfl_xor(v, off, 64);
+        v = o > v ? o : v;
+    }
+    if ((threadIdx.x & 63u) == 0 && v) atomicMax(dst, v);
+}
 
 // b3MakePair argument order is (inEdge1, inEdge2, outEdge1, outEdge2) (clip.h:36-44).
 __device__ __forceinline__ ClipV clipv(V3 p, uint32_t in1, uint32_t in2, uint32_t out1, uint32_t out2) {

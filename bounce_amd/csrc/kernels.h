@@ -53,6 +53,17 @@ static_assert(sizeof(ConvexCache) == 36, "convex cache layout");
 enum : uint32_t { kCacheSeparation = 0, kCacheOverlap = 1, kCacheEmpty = 2 };
 enum : uint32_t { kFeatureEdge1 = 0, kFeatureFace1 = 1, kFeatureFace2 = 2 };
 
+// Device equivalents of the reference's global statistics (collide_hulls.cpp:291-292,
+// gjk.cpp:27): running totals over batches while counting is on.
+struct Counters {
+    unsigned long long convexCalls;      // b3_convexCalls: hull-hull collides
+    unsigned long long convexCacheHits;  // b3_convexCacheHits: separation hits + rebuilt overlaps
+    unsigned long long gjkCalls;         // b3_gjkCalls
+    unsigned long long gjkIters;         // b3_gjkIters
+    unsigned int gjkMaxIters;            // b3_gjkMaxIters
+    unsigned int pad;
+};
+
 struct PairInputs {
     uint32_t n;
     const uint32_t* pairs;   // [n][2] shape indices
@@ -63,6 +74,7 @@ struct PairInputs {
     uint32_t shapeCount;
     uint32_t hullCount;
     const uint32_t* slots;   // [n] cache and manifold slot of each pair; null: slot p
+    Counters* counters;      // statistics totals, or null when counting is off
 
     __device__ uint32_t slot_of(uint32_t p) const { return slots ? slots[p] : p; }
 
@@ -140,6 +152,7 @@ constexpr uint32_t kFlagTriangleOverflow = 4u;
 constexpr uint32_t kFlagTreeStackOverflow = 8u;
 
 struct MeshArgs {
+    Counters* counters;         // statistics totals, or null when counting is off
     uint32_t n;                 // mesh contacts
     const uint32_t* shapeA;     // [n] shape table index of the convex shape
     const ShapeRec* shapes;
@@ -167,6 +180,7 @@ hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s);
+hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // Persistent SAT over the pairs the cache kernel marked; any hull size within the record
 // limits.  Fills a.split (the counter must be zero).

@@ -22,29 +22,40 @@ namespace b3amd {
 namespace {
 
 // s_CollideMatrix (collide.cpp:172-177) for every pair the cache kernel marked convex.
+// With counting on, GJK calls and support iterations go to the b3_gjk* totals
+// (gjk.cpp:583,673,697).
 __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    if (a.stage[p] != kStageConvex) return;
-    const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
-    const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];
-    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
-    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
-    Contact m;
-    m.pointCount = 0;
-    m.overflow = false;
-    if (s2.type == 2u) {
-        const DevHull h2 = DevHull::at(a.pool, a.offsets, s2.hull);
-        if (s1.type == 0u) collide_sphere_hull(m, xf1, s1, xf2, h2, s2.radius);
-        else collide_capsule_hull(m, xf1, s1, xf2, h2, s2.radius);
-    } else if (s1.type == 0u && s2.type == 0u) {
-        collide_spheres(m, xf1, s1, xf2, s2);
-    } else if (s1.type == 0u) {
-        collide_sphere_capsule(m, xf1, s1, xf2, s2);
-    } else {
-        collide_capsules(m, xf1, s1, xf2, s2);
+    uint32_t calls = 0, iters = 0;
+    if (p < a.n && a.stage[p] == kStageConvex) {
+        Manifold* const out = a.out + a.slot_of(p);
+        const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
+        const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];
+        const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
+        const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+        Contact m;
+        m.pointCount = 0;
+        m.overflow = false;
+        if (s2.type == 2u) {
+            const DevHull h2 = DevHull::at(a.pool, a.offsets, s2.hull);
+            if (s1.type == 0u) collide_sphere_hull(m, xf1, s1, xf2, h2, s2.radius);
+            else collide_capsule_hull(m, xf1, s1, xf2, h2, s2.radius);
+        } else if (s1.type == 0u && s2.type == 0u) {
+            collide_spheres(m, xf1, s1, xf2, s2);
+        } else if (s1.type == 0u) {
+            collide_sphere_capsule(m, xf1, s1, xf2, s2);
+        } else {
+            collide_capsules(m, xf1, s1, xf2, s2);
+        }
+        write_manifold(out, m);
+        calls = m.gjkCalls;
+        iters = m.gjkIters;
     }
-    write_manifold(a.out + a.slot_of(p), m);
+    if (a.counters) {  // all lanes of the block reach here
+        wave_count(&a.counters->gjkCalls, calls);
+        wave_count(&a.counters->gjkIters, iters);
+        wave_max(&a.counters->gjkMaxIters, iters);
+    }
 }
 
 }  // namespace

@@ -111,8 +111,8 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
 // b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush.
 // Writes the manifold through `out` and the cache through `fc` when the pair is resolved
 // here; returns the pair's stage (kStageOverlap when it needs the full SAT).
-__device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, Manifold* out, FeatureCache* fc) {
-    const uint32_t cls = a.classify(p);
+__device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, uint32_t cls, Manifold* out,
+                                               FeatureCache* fc) {
     if (cls != 1u) {
         if (cls == 0u) {
             Contact m;
@@ -185,17 +185,28 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, M
 // Every destination is addressed once, before the divergent branches, and the stage words
 // are stored unconditionally at the end, so no store address is a merge of per-path
 // values.
+// With counting on, b3_convexCalls counts the hull pairs and b3_convexCacheHits the ones
+// the cache resolved (collide_hulls.cpp:304, collide_hulls_cache.cpp:300,334).
 __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    const uint32_t slot = a.slot_of(p);
-    Manifold* const out = a.out + slot;
-    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
-    uint32_t* const stage = a.stage + p;
-    SatResult* const sat = a.sat + p;
-    const uint32_t st = cache_pair(a, p, out, fc);
-    *stage = st;
-    sat->stage = st == kStageOverlap ? kStageOverlap : kStageSkip;
+    uint32_t call = 0, hit = 0;
+    if (p < a.n) {
+        const uint32_t slot = a.slot_of(p);
+        Manifold* const out = a.out + slot;
+        FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+        uint32_t* const stage = a.stage + p;
+        SatResult* const sat = a.sat + p;
+        const uint32_t cls = a.classify(p);
+        const uint32_t st = cache_pair(a, p, cls, out, fc);
+        *stage = st;
+        sat->stage = st == kStageOverlap ? kStageOverlap : kStageSkip;
+        call = cls == 1u ? 1u : 0u;
+        hit = (cls == 1u && st == kStageSkip) ? 1u : 0u;
+    }
+    if (a.counters) {  // all lanes of the block reach here
+        wave_count(&a.counters->convexCalls, call);
+        wave_count(&a.counters->convexCacheHits, hit);
+    }
 }
 
 // ------------------------------------------------------------------ launchers

@@ -117,31 +117,46 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
     for (uint32_t k = 0; k < m; ++k) a.pairContact[off + k] = c;
 }
 
+// With counting on, hull-triangle collides go to b3_convexCalls and the sphere/capsule
+// GJK calls to the b3_gjk* totals, as b3MeshContact::Collide's per-triangle
+// b3CollideShapeAndShape calls count them in the reference (mesh_contact.cpp:217-275).
 __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a, uint32_t total) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= total) return;
-    const uint32_t c = a.pairContact[f];
-    const uint32_t k = f - a.contactOffset[c];
-    const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
-    const uint4 t = a.mesh.tris[ti];
-    const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
-    const TriHull tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
-    const ShapeRec s = a.shapes[a.shapeA[c]];
-    const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
-    const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
-    Contact m;
-    m.pointCount = 0;
-    m.overflow = false;
-    if (s.type == 2u) {
-        const DevHull h = DevHull::at(a.pool, a.offsets, s.hull);
-        collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
-    } else if (s.type == 0u) {
-        collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
-    } else {
-        collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
+    uint32_t hullCalls = 0, calls = 0, iters = 0;
+    if (f < total) {
+        const uint32_t c = a.pairContact[f];
+        const uint32_t k = f - a.contactOffset[c];
+        const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
+        Manifold* const out = a.temp + f;
+        const uint4 t = a.mesh.tris[ti];
+        const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
+        const TriHull tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
+        const ShapeRec s = a.shapes[a.shapeA[c]];
+        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+        Contact m;
+        m.pointCount = 0;
+        m.overflow = false;
+        if (s.type == 2u) {
+            const DevHull h = DevHull::at(a.pool, a.offsets, s.hull);
+            collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
+            hullCalls = 1;
+        } else if (s.type == 0u) {
+            collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
+        } else {
+            collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
+        }
+        write_manifold(out, m);
+        for (uint32_t j = 0; j < m.pointCount; ++j) out->points[j].triangleKey = ti;
+        calls = m.gjkCalls;
+        iters = m.gjkIters;
     }
-    write_manifold(a.temp + f, m);
-    for (uint32_t j = 0; j < m.pointCount; ++j) a.temp[f].points[j].triangleKey = ti;
+    if (a.counters) {  // all lanes of the block reach here
+        wave_count(&a.counters->convexCalls, hullCalls);
+        wave_count(&a.counters->gjkCalls, calls);
+        wave_count(&a.counters->gjkIters, iters);
+        wave_max(&a.counters->gjkMaxIters, iters);
+    }
 }
 
 // ------------------------------------------------------------------ clustering

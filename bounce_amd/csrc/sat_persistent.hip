@@ -347,6 +347,19 @@ hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s) {
     return hipGetLastError();
 }
 
+// Zeroes device words with a kernel (a stream operation that graph capture records as a
+// kernel node, unlike a memset).
+__global__ void zero_words_kernel(uint32_t* words, uint32_t count) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < count) words[i] = 0u;
+}
+
+hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s) {
+    if (count == 0) return hipSuccess;
+    zero_words_kernel<<<(count + 255) / 256, 256, 0, s>>>(words, count);
+    return hipGetLastError();
+}
+
 template <int W>
 hipError_t launch_sat_w(SatArgs a, uint32_t wavesPerSimd, uint32_t slots, hipStream_t s) {
     static int cus = 0, perCu = 0;

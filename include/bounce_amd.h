@@ -220,6 +220,28 @@ int b3amd_server_replay(b3amd_server* server, uint32_t count, const uint32_t* wo
                         const b3amd_transform* const* body_xfs, double* latency_ms);
 int b3amd_server_get_stats(b3amd_server* server, b3amd_server_stats* out);
 
+/* ---- statistics ----
+ * Device equivalents of the reference's global counters: b3_convexCalls and
+ * b3_convexCacheHits (collide_hulls.cpp:291-292,304; collide_hulls_cache.cpp:300,334),
+ * b3_gjkCalls, b3_gjkIters and b3_gjkMaxIters (gjk.cpp:27,583,673,697), accumulated by
+ * every batch (convex pairs and mesh contacts) while counting is on.  Counting is off by
+ * default (it adds one atomic per wave and counter).  b3World::Step resets the
+ * reference's counters every step (world.cpp:122-127); call b3amd_reset_counters for the
+ * same.  The simplex-cache counter b3_gjkCacheHits has no counterpart: the narrowphase
+ * runs GJK without a simplex cache (gjk.cpp:845-851), as the reference's does. */
+typedef struct b3amd_counters {
+    uint64_t convex_calls;
+    uint64_t convex_cache_hits;
+    uint64_t gjk_calls;
+    uint64_t gjk_iters;
+    uint32_t gjk_max_iters;
+    uint32_t reserved;
+} b3amd_counters;
+int b3amd_set_counting(b3amd_context* ctx, int on);
+/* Synchronizes the context stream and reads the totals. */
+int b3amd_get_counters(b3amd_context* ctx, b3amd_counters* out);
+int b3amd_reset_counters(b3amd_context* ctx);
+
 /* ---- triangle meshes ----
  * b3Mesh (mesh.h:41-99): vertices [V][3], triangles [T][3] (b3Triangle v1, v2, v3).  The
  * static tree is built on the host with the node numbering of b3Mesh::BuildTree /

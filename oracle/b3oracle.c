@@ -669,15 +669,25 @@ static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float 
     make_fp(c, B3O_OVERLAP, B3O_EDGE1, eq.index1, eq.index2);
 }
 
+/* Statistics kept by the reference in globals: b3_convexCalls, b3_convexCacheHits
+ * (collide_hulls.cpp:291-292), b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters (gjk.cpp:27). */
+static uint64_t g_stats[5];
+void b3o_get_counters(uint64_t out[5]) { memcpy(out, g_stats, sizeof(g_stats)); }
+void b3o_reset_counters(void) { memset(g_stats, 0, sizeof(g_stats)); }
+
 static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1, /* collide_hulls_cache.cpp:279-344 */
                                  const XF* xf2, const Hull* h2, float r2, b3o_feature_cache* c,
                                  QT q10, QT q1, QT q20, QT q2) {
     PHASE(0);
     FL(1);
+    ++g_stats[0];  /* b3CollideHullAndHull (collide_hulls.cpp:304) */
     float totalRadius = r1 + r2;
     int state0 = (int)c->state;
     int state1 = read_state(c, xf1, h1, xf2, h2, totalRadius);
-    if (state0 == B3O_SEPARATION && state1 == B3O_SEPARATION) return;
+    if (state0 == B3O_SEPARATION && state1 == B3O_SEPARATION) {
+        ++g_stats[1];  /* separation cache hit (collide_hulls_cache.cpp:300) */
+        return;
+    }
     if (state0 == B3O_OVERLAP && state1 == B3O_OVERLAP) {
         switch (c->type) {
         case B3O_EDGE1: build_edge_contact(m, xf1, c->index1, h1, xf2, c->index2, h2, 1); break;
@@ -685,7 +695,10 @@ static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1,
         case B3O_FACE2: rebuild_face_contact(m, xf2, c->index1, h2, r2, xf1, h1, r1, 1, q20, q2, q10, q1); break;
         default: break;
         }
-        if (m->pointCount > 0) return;
+        if (m->pointCount > 0) {
+            ++g_stats[1];  /* overlap cache hit (collide_hulls_cache.cpp:334) */
+            return;
+        }
     }
     c->state = B3O_EMPTY;
     collide_cache(m, xf1, h1, r1, xf2, h2, r2, c);
@@ -907,6 +920,7 @@ static void solve4(Simplex* s, V3 Q) {                                          
 static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2) {
     Simplex s;
     memset(&s, 0, sizeof(s));
+    ++g_stats[2];  /* gjk.cpp:583 */
     s.v[0].point1 = xf_mul(xf1, p1->v[0]);
     s.v[0].point2 = xf_mul(xf2, p2->v[0]);
     s.v[0].point = sub(s.v[0].point2, s.v[0].point1);
@@ -943,6 +957,8 @@ static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2
         if (dup) break;
         ++s.count;
     }
+    g_stats[3] += iter;                          /* gjk.cpp:673 */
+    if (iter > g_stats[4]) g_stats[4] = iter;    /* gjk.cpp:697 */
     GJKOut o;
     closest_points(&s, &o.point1, &o.point2);
     o.distance = length(sub(o.point1, o.point2));

@@ -132,6 +132,11 @@ void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, cons
 void b3ref_mesh_build_tree(const float* vertices, uint32_t vertexCount, uint32_t triangleCount,
                            const uint32_t* triangles, float* nodesOut);
 
+/* Totals of the reference's statistics globals since the last reset: b3_convexCalls,
+ * b3_convexCacheHits, b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters. */
+void b3o_get_counters(uint64_t out[5]);
+void b3o_reset_counters(void);
+
 /* FP32 ops executed by the last b3o_collide_pairs call (liboracle_count.so only). */
 uint64_t b3o_flop_count(void);
 /* Per phase: 0 feature-cache read/rebuild, 1 separation queries, 2 contact building. */

@@ -125,6 +125,23 @@ void to_out(const b3Manifold& m, b3o_manifold* o) {
 
 }  // namespace
 
+// The reference's statistics globals (collide_hulls.cpp:291-292, gjk.cpp:27).
+extern u32 b3_convexCalls, b3_convexCacheHits;
+extern u32 b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters;
+
+extern "C" void b3ref_get_counters(uint64_t out[5]) {
+    out[0] = b3_convexCalls;
+    out[1] = b3_convexCacheHits;
+    out[2] = b3_gjkCalls;
+    out[3] = b3_gjkIters;
+    out[4] = b3_gjkMaxIters;
+}
+
+extern "C" void b3ref_reset_counters(void) {
+    b3_convexCalls = b3_convexCacheHits = 0;
+    b3_gjkCalls = b3_gjkIters = b3_gjkMaxIters = 0;
+}
+
 extern "C" void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
                                     uint32_t n, const uint32_t* pairs, const b3o_xf* xfs,
                                     const b3o_quat* sweeps, b3o_convex_cache* caches,
@@ -136,6 +153,13 @@ extern "C" void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* 
     BodyImage bodyA, bodyB;
     std::memset(bodyA.bytes, 0, sizeof(bodyA.bytes));
     std::memset(bodyB.bytes, 0, sizeof(bodyB.bytes));
+    // No sweeps: the bodies did not rotate during the step (orientation0 == orientation),
+    // here the identity, which is what a null sweep array means at the boundary.
+    const b3Quat identity(0.0f, 0.0f, 0.0f, 1.0f);
+    for (BodyImage* b : {&bodyA, &bodyB}) {
+        b->body()->m_sweep.orientation0 = identity;
+        b->body()->m_sweep.orientation = identity;
+    }
     for (uint32_t i = 0; i < n; ++i) {
         const b3o_shape& sa = shapes[pairs[2 * i]];
         const b3o_shape& sb = shapes[pairs[2 * i + 1]];
@@ -232,6 +256,10 @@ extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* s
     build_hulls(pool, scene.hulls);
     b3Mesh mesh;
     make_mesh(mesh, m->vertices, m->vertexCount, m->triangleCount, m->triangles);
+    // b3World's constructor and destructor reset the statistics globals
+    // (world.cpp:39-45, 67-71): keep the totals across the world's lifetime.
+    uint64_t before[5];
+    b3ref_get_counters(before);
     b3World* world = new b3World();
     b3BodyDef bdA;
     bdA.type = e_dynamicBody;
@@ -261,5 +289,12 @@ extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* s
         }
         delete mc;
     }
+    uint64_t during[5];
+    b3ref_get_counters(during);
     delete world;
+    b3_convexCalls = (u32)(before[0] + during[0]);
+    b3_convexCacheHits = (u32)(before[1] + during[1]);
+    b3_gjkCalls = (u32)(before[2] + during[2]);
+    b3_gjkIters = (u32)(before[3] + during[3]);
+    b3_gjkMaxIters = (u32)(before[4] > during[4] ? before[4] : during[4]);
 }

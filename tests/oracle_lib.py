@@ -38,6 +38,24 @@ class Checker:
     def __init__(self, lib, prefix):
         self.lib = lib
         self._collide, self._queries = _bind(lib, prefix)
+        self._get_counters = getattr(lib, prefix + "get_counters")
+        self._get_counters.restype = None
+        self._get_counters.argtypes = [ctypes.c_void_p]
+        self._reset_counters = getattr(lib, prefix + "reset_counters")
+        self._reset_counters.restype = None
+        self._reset_counters.argtypes = []
+
+    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters")
+
+    def counters(self):
+        """The statistics globals (b3_convexCalls, b3_convexCacheHits, b3_gjkCalls,
+        b3_gjkIters, b3_gjkMaxIters) since the last reset."""
+        out = np.zeros(5, np.uint64)
+        self._get_counters(ptr(out))
+        return dict(zip(self.COUNTERS, (int(x) for x in out)))
+
+    def reset_counters(self):
+        self._reset_counters()
 
     def collide(self, pool, shapes, pairs, xfs, sweeps=None, caches=None):
         pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)

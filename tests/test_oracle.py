@@ -269,3 +269,38 @@ def test_oracle_mesh_golden(oracle):
     np.testing.assert_array_equal(co[0], g["tri_counts"])
     np.testing.assert_array_equal(co[3], g["manifold_counts"])
     assert_manifolds_equal(co[2].reshape(-1), g["out"].reshape(-1), "golden mesh manifolds")
+
+
+def test_oracle_counters_equal_reference(lib, workload, oracle, reference):
+    """The statistics globals (b3_convexCalls, b3_convexCacheHits, b3_gjkCalls,
+    b3_gjkIters, b3_gjkMaxIters) counted by the oracle equal the compiled reference's
+    over warm-started hull steps, mixed convex pairs and mesh contacts."""
+    from util import mixed_workload, mesh_workload
+    import oracle_lib
+    pool, shapes, pairs, xfs = workload
+    rng = np.random.default_rng(8)
+    co, cr = empty_caches(len(pairs)), empty_caches(len(pairs))
+    x = xfs.copy()
+    for impl in (oracle, reference):
+        impl.reset_counters()
+    for step in range(3):
+        oracle.collide(pool, shapes, pairs, x, caches=co)
+        reference.collide(pool, shapes, pairs, x, caches=cr)
+        assert oracle.counters() == reference.counters(), "hull step %d" % step
+        x[:, 1, 9:12] += rng.normal(0, 0.003, size=(len(pairs), 3)).astype(np.float32)
+    got = oracle.counters()
+    assert got["convex_calls"] == 3 * len(pairs) and got["convex_cache_hits"] > len(pairs) // 2
+    mpool, mshapes, mpairs, mxfs = mixed_workload(lib, 22, 3000)
+    for impl in (oracle, reference):
+        impl.reset_counters()
+        impl.collide(mpool, mshapes, mpairs, mxfs)
+    assert oracle.counters() == reference.counters()
+    assert oracle.counters()["gjk_calls"] > 0 and oracle.counters()["gjk_max_iters"] > 1
+    pool, shapes, v, t, mx = mesh_workload(lib, 10, 32, 32, 300)
+    nodes = oracle_lib.build_tree(reference.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    for impl, prefix in ((oracle, "b3o_"), (reference, "b3ref_")):
+        impl.reset_counters()
+        oracle_lib.collide_mesh(impl.lib, prefix, pool, shapes, v, t, nodes, sa, mx)
+    assert oracle.counters() == reference.counters()
+    assert oracle.counters()["convex_calls"] > 0 and oracle.counters()["gjk_calls"] > 0
