@@ -178,6 +178,8 @@ def serving_run(lib, worlds_n, rate, duration, seed=4096):
                     "achieved_steps_per_s": n / float((times + lat / 1000.0).max()),
                     "batches": batches,
                     "mean_batch_contacts": (after["pairs"] - before["pairs"]) / max(1, batches),
+                    "device_ms_per_batch": (after["gpu_ms"] - before["gpu_ms"]) / max(1, batches),
+                    "pack_ms_per_batch": (after["pack_ms"] - before["pack_ms"]) / max(1, batches),
                 }
                 log("serving %s: %.0f steps/s, p50 %.3f ms, p99 %.3f ms, %d batches"
                     % (name, r, out[name]["p50_ms"], out[name]["p99_ms"], batches))
@@ -255,7 +257,7 @@ def main():
         ctx.synchronize()
         t1 = time.perf_counter()
         ctx.set_profiling(False)
-        stats = {k: ctx.kernel_stats(k) for k in ("reset_counter", "cache_hull", "sat_hull", "contact_hull")}
+        stats = {k: ctx.kernel_stats(k) for k in ("cache_hull", "sat_hull", "contact_hull")}
         out = do.download(np.zeros(n, MANIFOLD_DTYPE))
     ms = (t1 - t0) * 1000.0 / args.steps
     value = n / (ms / 1000.0)

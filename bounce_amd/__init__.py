@@ -73,6 +73,7 @@ def load_library():
         "b3amd_world_read_manifolds": (i32, [vp, u32, vp]),
         "b3amd_server_replay": (i32, [vp, u32, vp, vp, vp, vp]),
         "b3amd_server_get_stats": (i32, [vp, vp]),
+        "b3amd_server_configure": (i32, [vp, u32, u32]),
         "b3amd_set_counting": (i32, [vp, i32]),
         "b3amd_get_counters": (i32, [vp, vp]),
         "b3amd_reset_counters": (i32, [vp]),
@@ -141,7 +142,8 @@ class Graph:
 class ServerStats(ctypes.Structure):
     """b3amd_server_stats."""
     _fields_ = [("batches", ctypes.c_uint64), ("requests", ctypes.c_uint64), ("pairs", ctypes.c_uint64),
-                ("max_batch_pairs", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+                ("max_batch_pairs", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("gpu_ms", ctypes.c_double), ("pack_ms", ctypes.c_double)]
 
 
 class Server:
@@ -206,6 +208,10 @@ class Server:
         lat = np.zeros(len(worlds), np.float64)
         _check(self.lib.b3amd_server_replay(self.handle, len(worlds), ptr(worlds), ptr(times), ptr(ptrs), ptr(lat)))
         return lat
+
+    def configure(self, batch_pairs=0, max_inflight=2):
+        """Batch cap (0: the server maximum) and device pipeline depth (1..3)."""
+        _check(self.lib.b3amd_server_configure(self.handle, batch_pairs, max_inflight))
 
     def stats(self):
         st = ServerStats()

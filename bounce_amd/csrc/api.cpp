@@ -42,18 +42,18 @@ namespace {
 
 // Brackets one kernel launch with events while profiling is on.
 template <class F>
-int timed(b3amd_context* ctx, const char* name, F launch) {
+int timed(b3amd_context* ctx, hipStream_t s, const char* name, F launch) {
     hipEvent_t a = nullptr, b = nullptr;
     if (ctx->profiling) {
         a = ctx->event();
         b = ctx->event();
         if (!a || !b) return fail(B3AMD_ERR_DEVICE, "hipEventCreate failed");
-        HIP_TRY(hipEventRecord(a, ctx->stream));
+        HIP_TRY(hipEventRecord(a, s));
     }
     hipError_t e = launch();
     if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
     if (ctx->profiling) {
-        HIP_TRY(hipEventRecord(b, ctx->stream));
+        HIP_TRY(hipEventRecord(b, s));
         ctx->pending.push_back(PendingTiming{name, a, b});
     }
     return B3AMD_OK;
@@ -194,6 +194,7 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     in.counters = ctx->counting ? ctx->dCounters : nullptr;
     CacheArgs ca;
     static_cast<PairInputs&>(ca) = in;
+    ca.satCounter = sc.counter;
     ca.sweeps = sweeps;
     ca.caches = caches;
     ca.stage = sc.stage;
@@ -210,18 +211,21 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     cta.caches = caches;
     cta.out = out;
     hipStream_t s = ctx->stream;
-    if ((rc = timed(ctx, "reset_counter", [&] { return launch_reset_counter(sc.counter, s); }))) return rc;
-    if ((rc = timed(ctx, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     if (ctx->roundShapes) {
+        // Sphere/capsule pairs depend on nothing the hull kernels write.
         ConvexArgs cv;
         static_cast<PairInputs&>(cv) = in;
-        cv.stage = sc.stage;
         cv.out = out;
-        if ((rc = timed(ctx, "convex", [&] { return launch_convex(cv, s); }))) return rc;
+        HIP_TRY(hipEventRecord(ctx->forkEvent, s));
+        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->forkEvent, 0));
+        if ((rc = timed(ctx, ctx->side, "convex", [&] { return launch_convex(cv, ctx->side); }))) return rc;
+        HIP_TRY(hipEventRecord(ctx->joinEvent, ctx->side));
     }
+    if ((rc = timed(ctx, s, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     const SatTuning t = ctx->tuning;
-    if ((rc = timed(ctx, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
-    if ((rc = timed(ctx, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
+    if ((rc = timed(ctx, s, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
+    if ((rc = timed(ctx, s, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
+    if (ctx->roundShapes) HIP_TRY(hipStreamWaitEvent(s, ctx->joinEvent, 0));
     return B3AMD_OK;
 }
 
@@ -254,9 +258,12 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
     b3amd_context* ctx = new b3amd_context();
     ctx->device = device;
     ctx->budget = hbm_budget_bytes ? hbm_budget_bytes : (64ull << 30);
-    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete ctx;
-        return fail(B3AMD_ERR_DEVICE, "hipStreamCreate failed");
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->forkEvent, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&ctx->joinEvent, hipEventDisableTiming) != hipSuccess) {
+        b3amd_destroy(ctx);
+        return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
     }
     if (ctx->alloc(sizeof(Counters), (void**)&ctx->dCounters) != B3AMD_OK ||
         hipMemset(ctx->dCounters, 0, sizeof(Counters)) != hipSuccess) {
@@ -271,7 +278,8 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
 int b3amd_destroy(b3amd_context* ctx) {
     if (!ctx) return B3AMD_OK;
     hipSetDevice(ctx->device);
-    hipStreamSynchronize(ctx->stream);
+    if (ctx->stream) hipStreamSynchronize(ctx->stream);
+    if (ctx->side) hipStreamSynchronize(ctx->side);
     while (!ctx->graphs.empty()) b3amd_graph_destroy(ctx->graphs.back());
     for (auto& kv : ctx->allocs) hipFree(kv.first);
     for (PendingTiming& t : ctx->pending) {
@@ -279,7 +287,10 @@ int b3amd_destroy(b3amd_context* ctx) {
         hipEventDestroy(t.stop);
     }
     for (hipEvent_t e : ctx->freeEvents) hipEventDestroy(e);
-    hipStreamDestroy(ctx->stream);
+    if (ctx->forkEvent) hipEventDestroy(ctx->forkEvent);
+    if (ctx->joinEvent) hipEventDestroy(ctx->joinEvent);
+    if (ctx->side) hipStreamDestroy(ctx->side);
+    if (ctx->stream) hipStreamDestroy(ctx->stream);
     delete ctx;
     return B3AMD_OK;
 }
@@ -551,7 +562,7 @@ int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const ui
     HIP_TRY(hipMemcpyAsync(const_cast<uint32_t*>(a.shapeA), shapeA, bShape, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(const_cast<float*>(a.xfs), xfs, bXf, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemsetAsync(a.total, 0, 4, s));
-    if ((rc = timed(ctx, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
     uint32_t total = 0;
     HIP_TRY(hipMemcpyAsync(&total, a.total, 4, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
@@ -559,8 +570,8 @@ int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const ui
     void* temp = nullptr;
     if (total && (rc = ctx->alloc(240ull * total, &temp))) return rc;
     a.temp = static_cast<Manifold*>(temp);
-    rc = timed(ctx, "mesh_narrow", [&] { return launch_mesh_narrow(a, total, s); });
-    if (!rc) rc = timed(ctx, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
+    rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, total, s); });
+    if (!rc) rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
     if (!rc) {
         hipError_t e = hipMemcpyAsync(out, a.out, bOut, hipMemcpyDeviceToHost, s);
         if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, a.manifoldCounts, bMc, hipMemcpyDeviceToHost, s);

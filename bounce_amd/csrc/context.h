@@ -51,6 +51,10 @@ using namespace b3amd;  // internal header: included by the library's .cpp files
 struct b3amd_context {
     int device = 0;
     hipStream_t stream = nullptr;
+    // Sphere/capsule pairs run here beside the hull kernels: forked after the batch's inputs
+    // are on `stream` and joined back before its end (captured graphs keep the fork).
+    hipStream_t side = nullptr;
+    hipEvent_t forkEvent = nullptr, joinEvent = nullptr;
     uint64_t budget = 0;
     uint64_t inUse = 0;
     std::unordered_map<void*, uint64_t> allocs;

@@ -106,6 +106,7 @@ struct SatSplit {
 };
 
 struct CacheArgs : PairInputs {
+    uint32_t* satCounter;    // the SAT kernel's tail counter, zeroed here
     const float* sweeps;     // [n][4][4] or null
     ConvexCache* caches;     // [n] or null
     uint32_t* stage;         // [n] out
@@ -132,7 +133,6 @@ struct ContactArgs : PairInputs {
 };
 
 struct ConvexArgs : PairInputs {
-    const uint32_t* stage;   // [n] from the cache kernel
     Manifold* out;           // [n]
 };
 
@@ -179,7 +179,6 @@ hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
-hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s);
 hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s);
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // Persistent SAT over the pairs the cache kernel marked; any hull size within the record

@@ -21,13 +21,13 @@
 namespace b3amd {
 namespace {
 
-// s_CollideMatrix (collide.cpp:172-177) for every pair the cache kernel marked convex.
+// s_CollideMatrix (collide.cpp:172-177) for every valid pair that is not hull-hull.
 // With counting on, GJK calls and support iterations go to the b3_gjk* totals
 // (gjk.cpp:583,673,697).
 __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     uint32_t calls = 0, iters = 0;
-    if (p < a.n && a.stage[p] == kStageConvex) {
+    if (p < a.n && a.classify(p) == 2u) {
         Manifold* const out = a.out + a.slot_of(p);
         const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
         const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];

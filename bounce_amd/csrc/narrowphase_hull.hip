@@ -189,6 +189,7 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
 // the cache resolved (collide_hulls.cpp:304, collide_hulls_cache.cpp:300,334).
 __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p == 0) *a.satCounter = 0u;  // the SAT kernel runs after this one on the stream
     uint32_t call = 0, hit = 0;
     if (p < a.n) {
         const uint32_t slot = a.slot_of(p);

@@ -338,15 +338,6 @@ __global__ __launch_bounds__(64, W) void sat_persistent_kernel(SatArgs a, uint32
     }
 }
 
-__global__ void reset_counter_kernel(uint32_t* counter) {
-    if (threadIdx.x == 0) *counter = 0u;
-}
-
-hipError_t launch_reset_counter(uint32_t* counter, hipStream_t s) {
-    reset_counter_kernel<<<1, 64, 0, s>>>(counter);
-    return hipGetLastError();
-}
-
 // Zeroes device words with a kernel (a stream operation that graph capture records as a
 // kernel node, unlike a memset).
 __global__ void zero_words_kernel(uint32_t* words, uint32_t count) {

@@ -126,9 +126,13 @@ struct Staging {
 
 struct Batch {
     std::vector<Request*> requests;
-    hipEvent_t done = nullptr;
+    std::vector<uint32_t> bodyCounts;            // per request
+    uint32_t pairs = 0, bodies = 0;
+    hipEvent_t start = nullptr, done = nullptr;  // around the batch's copies and kernels
     int staging = 0;
 };
+
+constexpr int kMaxInflight = 3;
 
 }  // namespace
 
@@ -147,7 +151,10 @@ struct b3amd_server {
     float* dSweeps = nullptr;
     uint32_t* dSlots = nullptr;
     BatchScratch scratch{nullptr, nullptr, nullptr};
-    Staging staging[2];
+    Staging staging[kMaxInflight];
+    bool stagingBusy[kMaxInflight] = {false, false, false};
+    uint32_t batchPairs = 0;   // current cap on contacts per batch (<= maxPairs)
+    uint32_t maxInflight = 2;
 
     // Worlds and their slot ranges (first fit over free ranges).
     std::vector<World> worlds;
@@ -216,13 +223,13 @@ void give_range(b3amd_server* s, uint32_t base, uint32_t n) {
     }
 }
 
-// Packs pending requests into staging buffer `si` and enqueues the batch.  Called by the
-// worker with the lock held; returns false when nothing was taken.
-bool launch_batch(b3amd_server* s, int si, Batch& batch) {
-    Staging& st = s->staging[si];
+// Takes the oldest pending requests, at most one per world, up to the batch caps, and
+// writes their descriptors into the batch's staging buffer (lock held: world records are
+// only read here).  Returns the number taken.
+uint32_t select_batch(b3amd_server* s, Batch& batch) {
+    Staging& st = s->staging[batch.staging];
     std::vector<uint32_t> seen;  // worlds already taken or passed over in this batch
     uint32_t pairs = 0, bodies = 0, nreq = 0;
-    bool anySweeps = false;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
         Request* r = *it;
         const World& w = s->worlds[r->world];
@@ -234,7 +241,9 @@ bool launch_batch(b3amd_server* s, int si, Batch& batch) {
             continue;
         }
         seen.push_back(r->world);
-        if (nreq == s->maxRequests || pairs + w.pairCount > s->maxPairs || bodies + w.bodyCount > s->maxBodies) break;
+        if (nreq == s->maxRequests || (nreq > 0 && pairs + w.pairCount > s->batchPairs) ||
+            pairs + w.pairCount > s->maxPairs || bodies + w.bodyCount > s->maxBodies)
+            break;
         ReqDesc& d = st.descs[nreq];
         d.pairOffset = pairs;
         d.pairCount = w.pairCount;
@@ -242,25 +251,42 @@ bool launch_batch(b3amd_server* s, int si, Batch& batch) {
         d.slotBase = w.slotBase;
         d.bodyShapes = w.dBodyShapes;
         d.bodyPairs = w.dBodyPairs;
-        std::memcpy(st.bodyXfs + 12 * (size_t)bodies, r->xfs, 48 * (size_t)w.bodyCount);
-        if (r->sweeps) {
-            std::memcpy(st.bodySweeps + 8 * (size_t)bodies, r->sweeps, 32 * (size_t)w.bodyCount);
-            anySweeps = true;
-        }
+        batch.bodyCounts.push_back(w.bodyCount);
         pairs += w.pairCount;
         bodies += w.bodyCount;
         ++nreq;
         batch.requests.push_back(r);
         it = s->pending.erase(it);
     }
-    if (nreq == 0) return false;
+    batch.pairs = pairs;
+    batch.bodies = bodies;
+    return nreq;
+}
+
+// Copies the batch's transforms into its staging buffer and enqueues copies and kernels
+// (lock not held: the requests' inputs and the worlds' device arrays stay unchanged until
+// the batch completes).
+void launch_batch(b3amd_server* s, Batch& batch) {
+    const Clock::time_point t0 = Clock::now();
+    Staging& st = s->staging[batch.staging];
+    const uint32_t pairs = batch.pairs, bodies = batch.bodies;
+    const uint32_t nreq = (uint32_t)batch.requests.size();
+    bool anySweeps = false;
+    for (uint32_t q = 0; q < nreq; ++q) {
+        const Request* r = batch.requests[q];
+        const size_t off = st.descs[q].bodyOffset, nb = batch.bodyCounts[q];
+        std::memcpy(st.bodyXfs + 12 * off, r->xfs, 48 * nb);
+        if (r->sweeps) {
+            std::memcpy(st.bodySweeps + 8 * off, r->sweeps, 32 * nb);
+            anySweeps = true;
+        }
+    }
     if (anySweeps) {
         // Requests without sweeps get identity orientations (what a null sweep array means).
         for (uint32_t q = 0; q < nreq; ++q) {
             if (batch.requests[q]->sweeps) continue;
             float* o = st.bodySweeps + 8 * (size_t)st.descs[q].bodyOffset;
-            const uint32_t nb = s->worlds[batch.requests[q]->world].bodyCount;
-            for (uint32_t b = 0; b < nb; ++b, o += 8) {
+            for (uint32_t b = 0; b < batch.bodyCounts[q]; ++b, o += 8) {
                 o[0] = o[1] = o[2] = 0.0f, o[3] = 1.0f;
                 o[4] = o[5] = o[6] = 0.0f, o[7] = 1.0f;
             }
@@ -268,8 +294,8 @@ bool launch_batch(b3amd_server* s, int si, Batch& batch) {
     }
     b3amd_context* ctx = s->ctx;
     hipStream_t stream = ctx->stream;
-    batch.staging = si;
-    bool ok = hipMemcpyAsync(s->dBodyXfs, st.bodyXfs, 48ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess &&
+    bool ok = hipEventRecord(batch.start, stream) == hipSuccess &&
+              hipMemcpyAsync(s->dBodyXfs, st.bodyXfs, 48ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess &&
               (!anySweeps ||
                hipMemcpyAsync(s->dBodySweeps, st.bodySweeps, 32ull * bodies, hipMemcpyHostToDevice, stream) ==
                    hipSuccess) &&
@@ -292,7 +318,8 @@ bool launch_batch(b3amd_server* s, int si, Batch& batch) {
         ok = enqueue_batch(ctx, pairs, s->dPairs, s->dXfs, anySweeps ? s->dSweeps : nullptr, s->dCaches,
                            s->dManifolds, s->scratch, s->dSlots) == B3AMD_OK;
     if (ok) ok = hipEventRecord(batch.done, stream) == hipSuccess;
-    if (!ok) {
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (!ok && s->workerError == B3AMD_OK) {
         s->workerError = B3AMD_ERR_DEVICE;
         s->workerMessage = b3amd_last_error();
     }
@@ -300,7 +327,7 @@ bool launch_batch(b3amd_server* s, int si, Batch& batch) {
     s->stats.requests += nreq;
     s->stats.pairs += pairs;
     if (pairs > s->stats.max_batch_pairs) s->stats.max_batch_pairs = pairs;
-    return true;
+    s->stats.pack_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
 void complete_batch(b3amd_server* s, Batch& b) {
@@ -312,6 +339,17 @@ void complete_batch(b3amd_server* s, Batch& b) {
     b.requests.clear();
 }
 
+void retire(b3amd_server* s) {
+    Batch& b = s->inflight.front();
+    complete_batch(s, b);
+    float ms = 0.0f;
+    if (hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess) s->stats.gpu_ms += ms;
+    hipEventDestroy(b.start);
+    hipEventDestroy(b.done);
+    s->stagingBusy[b.staging] = false;
+    s->inflight.pop_front();
+}
+
 void worker_main(b3amd_server* s) {
     hipSetDevice(s->ctx->device);
     std::unique_lock<std::mutex> lk(s->mu);
@@ -321,29 +359,33 @@ void worker_main(b3amd_server* s) {
         // Retire finished batches without blocking.
         bool retired = false;
         while (!s->inflight.empty() && hipEventQuery(s->inflight.front().done) == hipSuccess) {
-            complete_batch(s, s->inflight.front());
-            hipEvent_t e = s->inflight.front().done;
-            s->inflight.pop_front();
-            hipEventDestroy(e);
+            retire(s);
             retired = true;
         }
         if (retired) s->cvDone.notify_all();
-        if (!s->pending.empty() && s->inflight.size() < 2 && s->workerError == B3AMD_OK) {
-            // The staging buffer not used by the batch still in flight.
-            int si = 0;
-            if (!s->inflight.empty()) si = 1 - s->inflight.front().staging;
+        if (!s->pending.empty() && s->inflight.size() < s->maxInflight && s->workerError == B3AMD_OK) {
             Batch b;
-            if (hipEventCreateWithFlags(&b.done, hipEventDisableTiming) != hipSuccess) {
+            while (s->stagingBusy[b.staging]) ++b.staging;  // a free buffer exists: inflight < max
+            if (hipEventCreate(&b.start) != hipSuccess || hipEventCreate(&b.done) != hipSuccess) {
                 s->workerError = B3AMD_ERR_DEVICE;
                 s->workerMessage = "hipEventCreate failed";
                 continue;
             }
-            if (launch_batch(s, si, b)) s->inflight.push_back(std::move(b));
-            else hipEventDestroy(b.done);
+            if (select_batch(s, b) == 0) {
+                hipEventDestroy(b.start);
+                hipEventDestroy(b.done);
+                continue;
+            }
+            s->stagingBusy[b.staging] = true;
+            s->inflight.push_back(b);
+            Batch& queued = s->inflight.back();
+            lk.unlock();
+            launch_batch(s, queued);
+            lk.lock();
             continue;
         }
         if (!s->inflight.empty()) {
-            // Nothing to pack (or both buffers busy): wait for the oldest batch.
+            // Nothing to pack (or every buffer busy): wait for the oldest batch.
             hipEvent_t e = s->inflight.front().done;
             lk.unlock();
             hipEventSynchronize(e);
@@ -392,7 +434,7 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
     A(4 * P, (void**)&s->scratch.stage);
     A(sizeof(SatResult) * P, (void**)&s->scratch.sat);
     A(256, (void**)&s->scratch.counter);
-    for (int i = 0; i < 2 && rc == B3AMD_OK; ++i) {
+    for (int i = 0; i < kMaxInflight && rc == B3AMD_OK; ++i) {
         Staging& st = s->staging[i];
         if (hipHostMalloc((void**)&st.bodyXfs, 48 * B, 0) != hipSuccess ||
             hipHostMalloc((void**)&st.bodySweeps, 32 * B, 0) != hipSuccess ||
@@ -405,6 +447,7 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
         return fail(rc, "%s", msg.c_str());
     }
     s->freeRanges.push_back({0u, slab_slots});
+    s->batchPairs = max_batch_pairs;
     s->worker = std::thread(worker_main, s);
     *out = s;
     return B3AMD_OK;
@@ -430,7 +473,7 @@ int b3amd_server_destroy(b3amd_server* s) {
     void* bufs[] = {s->dCaches, s->dManifolds, s->dBodyXfs, s->dBodySweeps, s->dDescs, s->dPairs, s->dXfs,
                     s->dSweeps, s->dSlots, s->scratch.stage, s->scratch.sat, s->scratch.counter};
     for (void* p : bufs) ctx->release(p);
-    for (Staging& st : s->staging) {
+    for (Staging& st : s->staging) {  // also those of the third pipeline stage
         if (st.bodyXfs) hipHostFree(st.bodyXfs);
         if (st.bodySweeps) hipHostFree(st.bodySweeps);
         if (st.descs) hipHostFree(st.descs);
@@ -606,6 +649,16 @@ int b3amd_server_replay(b3amd_server* s, uint32_t count, const uint32_t* worlds,
         if (r && rc == B3AMD_OK) rc = r;
     }
     return rc;
+}
+
+int b3amd_server_configure(b3amd_server* s, uint32_t batch_pairs, uint32_t max_inflight) {
+    if (!s) return fail(B3AMD_ERR_ARGUMENT, "null server");
+    if (max_inflight < 1 || max_inflight > (uint32_t)kMaxInflight)
+        return fail(B3AMD_ERR_ARGUMENT, "max_inflight %u outside 1..%d", max_inflight, kMaxInflight);
+    std::lock_guard<std::mutex> lk(s->mu);
+    s->batchPairs = batch_pairs == 0 || batch_pairs > s->maxPairs ? s->maxPairs : batch_pairs;
+    s->maxInflight = max_inflight;
+    return B3AMD_OK;
 }
 
 int b3amd_server_get_stats(b3amd_server* s, b3amd_server_stats* out) {

@@ -196,6 +196,8 @@ typedef struct b3amd_server_stats {
     uint64_t pairs;           /* contacts computed */
     uint32_t max_batch_pairs; /* largest batch */
     uint32_t reserved;
+    double gpu_ms;            /* device time of all batches (copies, gather, kernels) */
+    double pack_ms;           /* host time spent packing batches */
 } b3amd_server_stats;
 int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t max_batch_bodies,
                         uint32_t slab_slots, b3amd_server** out);
@@ -219,6 +221,11 @@ int b3amd_world_read_manifolds(b3amd_server* server, uint32_t world, b3amd_manif
 int b3amd_server_replay(b3amd_server* server, uint32_t count, const uint32_t* worlds, const double* times_s,
                         const b3amd_transform* const* body_xfs, double* latency_ms);
 int b3amd_server_get_stats(b3amd_server* server, b3amd_server_stats* out);
+/* Batch policy: at most batch_pairs contacts per batch (0 or more than the server's
+ * max_batch_pairs: that maximum; a world larger than the cap still goes alone) and at most
+ * max_inflight (1..3) batches queued on the device while the next one is packed.  Defaults:
+ * max_batch_pairs, 2. */
+int b3amd_server_configure(b3amd_server* server, uint32_t batch_pairs, uint32_t max_inflight);
 
 /* ---- statistics ----
  * Device equivalents of the reference's global counters: b3_convexCalls and
