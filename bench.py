@@ -113,6 +113,72 @@ def algorithmic_work(pool, pairs, xfs, stride):
             "overlap": float(((f1 <= 0) & (f2 <= 0) & (e <= 0)).mean())}
 
 
+# ----------------------------------------------------------------- FP64 separation error
+
+def separation_error_fp64(out, xfs, radius_a=0.0, radius_b=0.0):
+    """Error of every manifold point's separation: b3WorldManifoldPoint::Initialize
+    (manifold.cpp:57-69) evaluated in FP32 in the reference's operation order
+    (mat33.h:129-132, transform.h:74-77, vec3.h:158-161) against the same formula in FP64
+    from the same FP32 local points, normals and transforms."""
+    cnt = out["pointCount"].astype(np.int64)
+    mask = np.arange(4)[None, :] < cnt[:, None]
+    pts = out["points"][mask]
+    pair = np.repeat(np.arange(len(out)), cnt)
+    xa, xb = xfs[pair, 0], xfs[pair, 1]
+    seps = []
+    for dt in (np.float32, np.float64):
+        A, B = xa.astype(dt), xb.astype(dt)
+
+        def rot(X, v):
+            return (v[:, 0:1] * X[:, 0:3] + v[:, 1:2] * X[:, 3:6]) + v[:, 2:3] * X[:, 6:9]
+
+        nA = rot(A, pts["localNormal1"].astype(dt))
+        cA = rot(A, pts["localPoint1"].astype(dt)) + A[:, 9:12]
+        cB = rot(B, pts["localPoint2"].astype(dt)) + B[:, 9:12]
+        d = cB - cA
+        seps.append(((d[:, 0] * nA[:, 0] + d[:, 1] * nA[:, 1]) + d[:, 2] * nA[:, 2]) - dt(radius_a) - dt(radius_b))
+    err = np.abs(seps[0].astype(np.float64) - seps[1])
+    if len(err) == 0:
+        return {"points": 0}
+    return {"points": int(len(err)), "median_abs_m": float(np.median(err)),
+            "p99_abs_m": float(np.percentile(err, 99)), "max_abs_m": float(err.max()),
+            "max_abs_over_abs_separation_ge_1mm": float((err / np.maximum(np.abs(seps[1]), 1e-3)).max())}
+
+
+# ----------------------------------------------------------------- C1: the box pyramid
+
+def pyramid_run(lib):
+    """The box pyramid (210 unit boxes on a static ground) replayed step by step from the
+    trace of the reference's own world: one b3amd_collide call per step on host arrays
+    (the plain drop-in form), feature caches carried by contact id, each step's output
+    digest compared with the reference's."""
+    import bounce_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from util import (PYRAMID_STEPS, CacheChain, load_golden, pyramid_inputs, pyramid_shapes, step_digest)
+    t = load_golden("pyramid_trace.npz")
+    chain = CacheChain()
+    equal = contacts = points = 0
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(t["pool"])
+        ctx.upload_shapes(pyramid_shapes(t["body_xfs"].shape[1]))
+        inputs = [pyramid_inputs(t, s) for s in range(PYRAMID_STEPS)]
+        ctx.collide(inputs[-1][1], inputs[-1][2])  # warm-up (scratch allocation)
+        wall = 0.0
+        for s, (ids, pairs, xfs, sweeps) in enumerate(inputs):
+            caches = chain.gather(ids)
+            t0 = time.perf_counter()
+            out = ctx.collide(pairs, xfs, sweeps=sweeps, caches=caches)
+            wall += time.perf_counter() - t0
+            chain.scatter(ids, caches)
+            equal += int(np.array_equal(step_digest(out, caches), t["digests"][s]))
+            contacts += len(ids)
+            points += int(out["pointCount"].sum())
+    return {"steps": PYRAMID_STEPS, "steps_equal_to_reference": equal, "contacts": contacts, "points": points,
+            "ms_per_step": wall * 1000.0 / PYRAMID_STEPS,
+            "what": "b3amd_collide per step on host arrays (H2D, kernels, D2H, synchronise); inputs from "
+                    "the reference world's trace (tests/golden/pyramid_trace.npz)"}
+
+
 # ----------------------------------------------------------------- main
 
 TRAFFIC_FILE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles",
@@ -197,6 +263,7 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=262144)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
+    ap.add_argument("--no-pyramid", action="store_true")
     ap.add_argument("--serving-worlds", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=240000.0)
     ap.add_argument("--serving-seconds", type=float, default=0.5)
@@ -259,6 +326,13 @@ def main():
         ctx.set_profiling(False)
         stats = {k: ctx.kernel_stats(k) for k in ("cache_hull", "sat_hull", "contact_hull")}
         out = do.download(np.zeros(n, MANIFOLD_DTYPE))
+        # The plain drop-in form: one b3amd_collide per step on host arrays (PCIe both ways).
+        ctx.collide(pairs, xfs)
+        t2 = time.perf_counter()
+        plain_steps = 3
+        for _ in range(plain_steps):
+            ctx.collide(pairs, xfs)
+        plain_ms = (time.perf_counter() - t2) * 1000.0 / plain_steps
     ms = (t1 - t0) * 1000.0 / args.steps
     value = n / (ms / 1000.0)
     launches, sat_total = stats["sat_hull"]
@@ -307,7 +381,16 @@ def main():
                               "profiles/sat_hull_1M_pairs_hbm_traffic.md",
         },
         "cpu_baseline": cpu,
+        "plain_drop_in": {"value": n / (plain_ms / 1000.0), "unit": "pairs/s", "ms_per_step": plain_ms,
+                          "what": "b3amd_collide on host arrays, one call per step: H2D of pairs and "
+                                  "transforms, the same kernels, D2H of manifolds"},
+        "separation_error_fp64": separation_error_fp64(out, xfs),
     }
+    if not args.no_pyramid:
+        result["pyramid"] = pyramid_run(lib)
+        log("pyramid: %d/%d steps equal to the reference, %.3f ms per step"
+            % (result["pyramid"]["steps_equal_to_reference"], result["pyramid"]["steps"],
+               result["pyramid"]["ms_per_step"]))
     if not args.no_serving:
         result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     print(json.dumps(result))

@@ -9,6 +9,7 @@
 // zeroed b3Body image whose m_sweep carries the pair's orientation0/orientation.
 #include <cstring>
 #include <cstdlib>
+#include <unordered_map>
 #include <vector>
 #include <new>
 
@@ -24,6 +25,8 @@
 #include <bounce/collision/shapes/mesh.h>
 #include <bounce/dynamics/shapes/mesh_shape.h>
 #include <bounce/dynamics/contacts/mesh_contact.h>
+#include <bounce/dynamics/contacts/convex_contact.h>
+#include <bounce/collision/shapes/box_hull.h>
 #include <bounce/dynamics/world.h>
 #include <bounce/common/draw.h>
 #undef private
@@ -297,4 +300,162 @@ extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* s
     b3_gjkCalls = (u32)(before[2] + during[2]);
     b3_gjkIters = (u32)(before[3] + during[3]);
     b3_gjkMaxIters = (u32)(before[4] > during[4] ? before[4] : during[4]);
+}
+
+// ---------------------------------------------------------------------------------
+// A whole reference world, stepped by b3World::Step (world.cpp:115-144), recording the
+// narrowphase calls of each step: the box pyramid (20 levels = 210 unit boxes,
+// b3BoxHull_identity, collision.cpp:29) on a static 100 x 1 x 100 m box.  Bounce world
+// defaults (world.cpp:47-50: gravity -9.8 y, warm starting on, sleeping off), density 1,
+// default friction/restitution (shape.h:50-58).
+namespace {
+
+struct Pyramid {
+    b3World* world = nullptr;
+    b3BoxHull ground;
+    std::vector<b3Body*> bodies;
+    std::unordered_map<const b3Shape*, uint32_t> bodyOf;
+    std::unordered_map<const b3Contact*, uint32_t> ids;  // live contacts -> stable id
+    uint32_t nextId = 0;
+    uint64_t saved[5];
+};
+
+uint32_t contact_id(Pyramid* p, const b3Contact* c) {
+    auto it = p->ids.find(c);
+    if (it != p->ids.end()) return it->second;
+    const uint32_t id = p->nextId++;
+    p->ids[c] = id;
+    return id;
+}
+
+}  // namespace
+
+extern "C" void* b3ref_pyramid_create(uint32_t levels) {
+    Pyramid* p = new Pyramid();
+    b3ref_get_counters(p->saved);
+    p->world = new b3World();
+    p->ground.Set(50.0f, 0.5f, 50.0f);
+    {
+        b3BodyDef bd;  // static at the origin
+        b3Body* g = p->world->CreateBody(bd);
+        b3HullShape hs;
+        hs.m_hull = &p->ground;
+        b3ShapeDef sd;
+        sd.shape = &hs;
+        b3Shape* s = g->CreateShape(sd);
+        p->bodyOf[s] = 0;
+        p->bodies.push_back(g);
+    }
+    for (uint32_t l = 0; l < levels; ++l) {
+        for (uint32_t j = 0; j + l < levels; ++j) {
+            b3BodyDef bd;
+            bd.type = e_dynamicBody;
+            bd.position.x = 2.1f * float32(j) + float32(l) * 1.0f - 0.5f * float32(levels) * 2.0f;
+            bd.position.y = 3.0f * float32(l + 1);
+            bd.position.z = 0.0f;
+            b3Body* b = p->world->CreateBody(bd);
+            b3HullShape hs;
+            hs.m_hull = &b3BoxHull_identity;
+            b3ShapeDef sd;
+            sd.shape = &hs;
+            sd.density = 1.0f;
+            b3Shape* s = b->CreateShape(sd);
+            p->bodyOf[s] = (uint32_t)p->bodies.size();
+            p->bodies.push_back(b);
+        }
+    }
+    return p;
+}
+
+extern "C" void b3ref_pyramid_destroy(void* handle) {
+    Pyramid* p = (Pyramid*)handle;
+    // b3World's constructor, Step and destructor reset the statistics globals
+    // (world.cpp:39-45, 122-127, 62-71): leave them as they were before the world.
+    delete p->world;
+    b3_convexCalls = (u32)p->saved[0];
+    b3_convexCacheHits = (u32)p->saved[1];
+    b3_gjkCalls = (u32)p->saved[2];
+    b3_gjkIters = (u32)p->saved[3];
+    b3_gjkMaxIters = (u32)p->saved[4];
+    delete p;
+}
+
+extern "C" uint32_t b3ref_pyramid_body_count(void* handle) { return (uint32_t)((Pyramid*)handle)->bodies.size(); }
+
+// Hull records of the world: which = 0 the ground box, 1 the unit box.  counts = V, HE, F.
+extern "C" void b3ref_pyramid_hull(void* handle, uint32_t which, uint32_t counts[3], float* centroid,
+                                   float* vertices, uint32_t* edges, uint32_t* faces, float* planes) {
+    Pyramid* p = (Pyramid*)handle;
+    const b3Hull* h = which == 0 ? (const b3Hull*)&p->ground : (const b3Hull*)&b3BoxHull_identity;
+    counts[0] = h->vertexCount;
+    counts[1] = h->edgeCount;
+    counts[2] = h->faceCount;
+    if (centroid) std::memcpy(centroid, &h->centroid, 12);
+    if (vertices) std::memcpy(vertices, h->vertices, 12ull * h->vertexCount);
+    if (edges) std::memcpy(edges, h->edges, 16ull * h->edgeCount);
+    if (faces) std::memcpy(faces, h->faces, 4ull * h->faceCount);
+    if (planes) std::memcpy(planes, h->planes, 16ull * h->faceCount);
+}
+
+// Body transforms and sweep orientations (orientation0, orientation) as the next step's
+// narrowphase will read them.
+extern "C" void b3ref_pyramid_bodies(void* handle, b3o_xf* xfs, b3o_quat* sweeps) {
+    Pyramid* p = (Pyramid*)handle;
+    for (size_t i = 0; i < p->bodies.size(); ++i) {
+        const b3Body* b = p->bodies[i];
+        std::memcpy(xfs + i, &b->m_xf, sizeof(b3o_xf));
+        std::memcpy(sweeps + 2 * i, &b->m_sweep.orientation0, 16);
+        std::memcpy(sweeps + 2 * i + 1, &b->m_sweep.orientation, 16);
+    }
+}
+
+// One b3World::Step(1/60, 8, 2).  Records, in m_contactList order, every contact that
+// UpdateContacts collides in this step (contact_manager.cpp:167-231: bodies may collide,
+// one of them active, fat AABBs overlapping): its stable id, body pair, the transforms and
+// sweeps the collider reads, the cache before and after, and the manifold.  Returns the
+// number of such contacts; records beyond cap are not written.
+extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids, uint32_t* bodyPairs,
+                                       b3o_xf* xfs, b3o_quat* sweeps, b3o_convex_cache* cacheIn,
+                                       b3o_convex_cache* cacheOut, b3o_manifold* out) {
+    Pyramid* p = (Pyramid*)handle;
+    b3World* w = p->world;
+    if (w->m_flags & b3World::e_shapeAddedFlag) {  // what Step does first (world.cpp:129-134)
+        w->m_contactMan.FindNewContacts();
+        w->m_flags &= ~b3World::e_shapeAddedFlag;
+    }
+    std::vector<b3Contact*> collided;
+    for (b3Contact* c = w->m_contactMan.m_contactList.m_head; c; c = c->m_next) {
+        b3Shape* sA = c->m_pair.shapeA;
+        b3Shape* sB = c->m_pair.shapeB;
+        b3Body* bA = sA->m_body;
+        b3Body* bB = sB->m_body;
+        const bool activeA = bA->IsAwake() && bA->m_type != e_staticBody;
+        const bool activeB = bB->IsAwake() && bB->m_type != e_staticBody;
+        if (!bA->ShouldCollide(bB) || !w->m_contactMan.m_broadPhase.TestOverlap(sA->m_broadPhaseID, sB->m_broadPhaseID)) {
+            if (activeA || activeB || !bA->ShouldCollide(bB)) p->ids.erase(c);  // destroyed by UpdateContacts
+            continue;
+        }
+        if (!activeA && !activeB) continue;
+        const uint32_t k = (uint32_t)collided.size();
+        collided.push_back(c);
+        if (k >= cap) continue;
+        b3ConvexContact* cc = (b3ConvexContact*)c;
+        ids[k] = contact_id(p, c);
+        bodyPairs[2 * k] = p->bodyOf[sA];
+        bodyPairs[2 * k + 1] = p->bodyOf[sB];
+        std::memcpy(xfs + 2 * k, &bA->m_xf, sizeof(b3o_xf));
+        std::memcpy(xfs + 2 * k + 1, &bB->m_xf, sizeof(b3o_xf));
+        std::memcpy(sweeps + 4 * k, &bA->m_sweep.orientation0, 16);
+        std::memcpy(sweeps + 4 * k + 1, &bA->m_sweep.orientation, 16);
+        std::memcpy(sweeps + 4 * k + 2, &bB->m_sweep.orientation0, 16);
+        std::memcpy(sweeps + 4 * k + 3, &bB->m_sweep.orientation, 16);
+        std::memcpy(cacheIn + k, &cc->m_cache, sizeof(b3o_convex_cache));
+    }
+    w->Step(1.0f / 60.0f, 8, 2);
+    for (uint32_t k = 0; k < collided.size() && k < cap; ++k) {
+        b3ConvexContact* cc = (b3ConvexContact*)collided[k];
+        std::memcpy(cacheOut + k, &cc->m_cache, sizeof(b3o_convex_cache));
+        to_out(cc->m_stackManifold, out + k);
+    }
+    return (uint32_t)collided.size();
 }

@@ -5,7 +5,7 @@ Run in a container where the reference tree is present and oracle/_ref is built
 sweeps, caches) and the outputs of b3CollideShapeAndShape as computed by the reference
 library compiled from its own sources (oracle/_ref/libb3ref.so).
 
-    python tests/golden/make_golden.py
+    python tests/golden/make_golden.py [pairs] [mesh] [pyramid]
 """
 import os
 import sys
@@ -109,6 +109,55 @@ def make_mesh_fixture():
     print("mesh fixture: manifold counts", np.bincount(mcount))
 
 
+def make_pyramid_fixture():
+    """6. The box pyramid (20 levels, 210 unit boxes on a 100 x 1 x 100 m static box)
+    stepped 300 times at 60 Hz by the reference's own b3World (8 velocity / 2 position
+    iterations, warm starting on, sleeping off).  Kept per step: every body's transform
+    and sweep orientations before the step, the ids of the contacts UpdateContacts
+    collides in list order, and a digest of their manifolds and caches; whole manifolds
+    and caches for a few steps."""
+    from util import PYRAMID_FULL_STEPS, PYRAMID_STEPS, step_digest
+    ref = oracle_lib.reference()
+    world = oracle_lib.RefPyramid(ref.lib)
+    pool = world.hulls()
+    B = world.body_count
+    body_xfs = np.zeros((PYRAMID_STEPS, B, 12), np.float32)
+    body_sweeps = np.zeros((PYRAMID_STEPS, B, 2, 4), np.float32)
+    ids, offsets, digests, full_out, full_caches, full_offsets = [], [0], [], [], [], [0]
+    id_pairs = {}
+    for s in range(PYRAMID_STEPS):
+        body_xfs[s], body_sweeps[s] = world.bodies()
+        r = world.step()
+        assert np.array_equal(r["xfs"], body_xfs[s][r["pairs"]])
+        ids.append(r["ids"].astype(np.uint16))
+        offsets.append(offsets[-1] + len(r["ids"]))
+        for i, pr in zip(r["ids"], r["pairs"]):
+            id_pairs[int(i)] = pr
+        digests.append(step_digest(r["out"], r["caches_out"]))
+        if s in PYRAMID_FULL_STEPS:
+            full_out.append(r["out"])
+            c = r["caches_out"].copy()
+            for f in ("metric", "iterations", "sindex1", "sindex2"):  # undefined for hull pairs
+                c[f] = 0
+            full_caches.append(c)
+            full_offsets.append(full_offsets[-1] + len(r["ids"]))
+    world.close()
+    pairs = np.zeros((max(id_pairs) + 1, 2), np.uint32)
+    for i, pr in id_pairs.items():
+        pairs[i] = pr
+    np.savez_compressed(os.path.join(HERE, "pyramid_trace.npz"), desc=pool.desc, centroids=pool.centroids,
+                        vertices=pool.vertices, edges=pool.edges, faces=pool.faces, planes=pool.planes,
+                        body_xfs=body_xfs, body_sweeps=body_sweeps, ids=np.concatenate(ids),
+                        step_offsets=np.array(offsets, np.int64), id_pairs=pairs, digests=np.array(digests),
+                        full_steps=np.array(PYRAMID_FULL_STEPS, np.int64), full_out=np.concatenate(full_out),
+                        full_caches=np.concatenate(full_caches), full_offsets=np.array(full_offsets, np.int64))
+
+
 if __name__ == "__main__":
-    main()
-    make_mesh_fixture()
+    which = sys.argv[1:] or ["pairs", "mesh", "pyramid"]
+    if "pairs" in which:
+        main()
+    if "mesh" in which:
+        make_mesh_fixture()
+    if "pyramid" in which:
+        make_pyramid_fixture()

@@ -154,3 +154,77 @@ def collide_mesh(lib, prefix, pool, shapes, vertices, triangles, nodes, shape_a,
     f(ctypes.byref(ps), ptr(np.ascontiguousarray(shapes)), ctypes.byref(m), n, ptr(shape_a), ptr(xfs), cap,
       ptr(counts), ptr(lists), ptr(out), ptr(mcount))
     return counts, lists, out, mcount
+
+
+class RefPyramid:
+    """The reference's own world (b3World::Step) running the box pyramid, recording every
+    narrowphase call per step (oracle/ref_driver.cpp: b3ref_pyramid_*)."""
+
+    def __init__(self, lib, levels=20, cap=8192):
+        self.lib = lib
+        self.cap = cap
+        lib.b3ref_pyramid_create.restype = ctypes.c_void_p
+        lib.b3ref_pyramid_create.argtypes = [ctypes.c_uint32]
+        lib.b3ref_pyramid_destroy.restype = None
+        lib.b3ref_pyramid_destroy.argtypes = [ctypes.c_void_p]
+        lib.b3ref_pyramid_body_count.restype = ctypes.c_uint32
+        lib.b3ref_pyramid_body_count.argtypes = [ctypes.c_void_p]
+        lib.b3ref_pyramid_hull.restype = None
+        lib.b3ref_pyramid_hull.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 6
+        lib.b3ref_pyramid_bodies.restype = None
+        lib.b3ref_pyramid_bodies.argtypes = [ctypes.c_void_p] * 3
+        lib.b3ref_pyramid_step.restype = ctypes.c_uint32
+        lib.b3ref_pyramid_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 7
+        self.handle = lib.b3ref_pyramid_create(levels)
+        self.body_count = lib.b3ref_pyramid_body_count(self.handle)
+
+    def close(self):
+        if self.handle:
+            self.lib.b3ref_pyramid_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        self.close()
+
+    def hulls(self):
+        """HullPool of the world's two hulls: 0 the ground box, 1 the unit box."""
+        from bounce_amd.types import HullPool
+        out = []
+        for which in (0, 1):
+            counts = np.zeros(3, np.uint32)
+            self.lib.b3ref_pyramid_hull(self.handle, which, ptr(counts), None, None, None, None, None)
+            v, he, f = (int(x) for x in counts)
+            c = np.zeros(3, np.float32)
+            verts = np.zeros((v, 3), np.float32)
+            edges = np.zeros((he, 4), np.uint32)
+            faces = np.zeros(f, np.uint32)
+            planes = np.zeros((f, 4), np.float32)
+            self.lib.b3ref_pyramid_hull(self.handle, which, ptr(counts), ptr(c), ptr(verts), ptr(edges), ptr(faces),
+                                        ptr(planes))
+            out.append((verts, edges, faces, planes, c))
+        return HullPool.concat(out)
+
+    def bodies(self):
+        """(transforms [B][12], sweeps [B][2][4]) as the next step's narrowphase reads them."""
+        xfs = np.zeros((self.body_count, 12), np.float32)
+        sweeps = np.zeros((self.body_count, 2, 4), np.float32)
+        self.lib.b3ref_pyramid_bodies(self.handle, ptr(xfs), ptr(sweeps))
+        return xfs, sweeps
+
+    def step(self):
+        """One b3World::Step: dict of ids, body pairs, xfs [n][2][12], sweeps [n][4][4],
+        caches in/out and manifolds of the contacts UpdateContacts collided, in list order."""
+        cap = self.cap
+        ids = np.zeros(cap, np.uint32)
+        pairs = np.zeros((cap, 2), np.uint32)
+        xfs = np.zeros((cap, 2, 12), np.float32)
+        sweeps = np.zeros((cap, 4, 4), np.float32)
+        cin = np.zeros(cap, CACHE_DTYPE)
+        cout = np.zeros(cap, CACHE_DTYPE)
+        out = np.zeros(cap, MANIFOLD_DTYPE)
+        n = self.lib.b3ref_pyramid_step(self.handle, cap, ptr(ids), ptr(pairs), ptr(xfs), ptr(sweeps), ptr(cin),
+                                        ptr(cout), ptr(out))
+        if n > cap:
+            raise RuntimeError("pyramid step collided %d contacts, more than the capacity %d" % (n, cap))
+        return dict(ids=ids[:n], pairs=pairs[:n], xfs=xfs[:n], sweeps=sweeps[:n], caches_in=cin[:n],
+                    caches_out=cout[:n], out=out[:n])

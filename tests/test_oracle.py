@@ -11,8 +11,9 @@ import pytest
 
 from bounce_amd.shapes import box_hull, identity_xf
 from bounce_amd.types import HullPool, empty_caches
-from util import (assert_caches_equal, assert_manifolds_equal, hull_workload, load_golden, sat_separation_fp64,
-                  world_separations, xf_matrix)
+from util import (HULL_CACHE_FIELDS, PYRAMID_FULL_STEPS, PYRAMID_STEPS, CacheChain, assert_caches_equal,
+                  assert_manifolds_equal, defined_hull_cache, hull_workload, load_golden, pyramid_full_outputs, pyramid_inputs, pyramid_shapes,
+                  sat_separation_fp64, step_digest, world_separations, xf_matrix)
 
 
 @pytest.fixture(scope="module")
@@ -304,3 +305,50 @@ def test_oracle_counters_equal_reference(lib, workload, oracle, reference):
         oracle_lib.collide_mesh(impl.lib, prefix, pool, shapes, v, t, nodes, sa, mx)
     assert oracle.counters() == reference.counters()
     assert oracle.counters()["convex_calls"] > 0 and oracle.counters()["gjk_calls"] > 0
+
+
+@pytest.fixture(scope="module")
+def pyramid_trace():
+    return load_golden("pyramid_trace.npz")
+
+
+def test_oracle_pyramid_golden(oracle, pyramid_trace):
+    """The box pyramid as the reference world stepped it: 300 steps, every contact
+    UpdateContacts collided, caches carried from step to step by the restatement itself;
+    each step's manifolds and caches equal the reference's (digest per step, whole
+    records at the kept steps)."""
+    t = pyramid_trace
+    shapes = pyramid_shapes(t["body_xfs"].shape[1])
+    chain = CacheChain()
+    for s in range(PYRAMID_STEPS):
+        ids, pairs, xfs, sweeps = pyramid_inputs(t, s)
+        caches = chain.gather(ids)
+        out = oracle.collide(t["pool"], shapes, pairs, xfs, sweeps, caches)
+        chain.scatter(ids, caches)
+        if s in PYRAMID_FULL_STEPS:
+            want_out, want_caches = pyramid_full_outputs(t, s)
+            assert_manifolds_equal(out, want_out, "step %d" % s)
+            assert_caches_equal(caches, want_caches, "step %d" % s, HULL_CACHE_FIELDS)
+        assert np.array_equal(step_digest(out, caches), t["digests"][s]), "step %d differs" % s
+
+
+def test_oracle_pyramid_equals_reference_world(oracle, reference):
+    """Live: the reference world steps the pyramid; the restatement, given the same
+    inputs and the reference's caches before each call, returns the same manifolds and
+    caches at every step."""
+    from oracle_lib import RefPyramid
+    world = RefPyramid(reference.lib)
+    pool = world.hulls()
+    shapes = pyramid_shapes(world.body_count)
+    chain = CacheChain()
+    for s in range(PYRAMID_STEPS):
+        r = world.step()
+        ids = r["ids"].astype(np.int64)
+        assert_caches_equal(defined_hull_cache(chain.gather(ids)), defined_hull_cache(r["caches_in"]),
+                            "chained caches before step %d" % s)
+        caches = r["caches_in"].copy()
+        out = oracle.collide(pool, shapes, r["pairs"], r["xfs"], r["sweeps"], caches)
+        assert_manifolds_equal(out, r["out"], "step %d" % s)
+        assert_caches_equal(caches, r["caches_out"], "step %d" % s, HULL_CACHE_FIELDS)
+        chain.scatter(ids, caches)
+    world.close()

@@ -100,10 +100,30 @@ def load_golden(name):
     return g
 
 
-def assert_caches_equal(got, want, what="caches"):
+# The b3ConvexCache fields a hull-hull step defines: b3ConvexContact's constructor sets only
+# the simplex count and the feature state (convex_contact.cpp:35-36) and hull pairs never
+# touch the simplex cache, so its other words keep whatever the contact's memory held.
+HULL_CACHE_FIELDS = ("count", "state", "ftype", "index1", "index2")
+
+
+def defined_hull_cache(c):
+    """Copy of hull-pair cache records with the words the reference leaves undefined set
+    to zero: the simplex cache apart from its count, and the feature type and indices of
+    an empty feature cache (ReadState ignores them, sat.cpp:177-185)."""
+    from bounce_amd.types import CACHE_EMPTY
+    c = c.copy()
+    for f in ("metric", "iterations", "sindex1", "sindex2"):
+        c[f] = 0
+    empty = c["state"] == CACHE_EMPTY
+    for f in ("ftype", "index1", "index2"):
+        c[f][empty] = 0
+    return c
+
+
+def assert_caches_equal(got, want, what="caches", fields=None):
     """Field-wise b3ConvexCache equality (the two padding bytes of b3SimplexCache are
     not part of the record's value)."""
-    for f in got.dtype.names:
+    for f in fields or got.dtype.names:
         if not np.array_equal(got[f], want[f]):
             bad = np.where((got[f] != want[f]).reshape(len(got), -1).any(1))[0]
             raise AssertionError("%s: field %s differs in %d records; first %d: got %r want %r" % (
@@ -326,3 +346,64 @@ def mesh_workload(lib, seed, H, W, n_shapes, gap_sigma=0.05):
         xfs[i, 0] = identity_xf((x, y, z), R)
         xfs[i, 1] = identity_xf((0, 0, 0))
     return pool, shapes, verts, tris, xfs
+
+
+# ---- the box pyramid (210 unit boxes on a static ground) stepped by the reference world
+
+PYRAMID_STEPS = 300
+PYRAMID_FULL_STEPS = (60, 120, 180, 240, 299)  # steps whose manifolds the fixture keeps whole
+
+
+def pyramid_shapes(body_count):
+    """Shape table of the pyramid world: shape i belongs to body i; body 0 is the ground
+    (hull 0), every other body a unit box (hull 1)."""
+    from bounce_amd.types import HULL, SHAPE_DTYPE
+    s = np.zeros(body_count, SHAPE_DTYPE)
+    s["type"] = HULL
+    s["hull"] = 1
+    s["hull"][0] = 0
+    return s
+
+
+def step_digest(out, caches):
+    """SHA-256 over a step's manifolds (all bytes) and the defined fields of its hull-pair
+    cache records."""
+    import hashlib
+    h = hashlib.sha256()
+    h.update(manifold_bytes(out).tobytes())
+    for f in HULL_CACHE_FIELDS:
+        h.update(np.ascontiguousarray(caches[f]).tobytes())
+    return np.frombuffer(h.digest(), np.uint8)
+
+
+class CacheChain:
+    """b3ConvexCache records carried from step to step by contact id, as each
+    b3ConvexContact keeps its m_cache (convex_contact.h); new contacts start empty
+    (convex_contact.cpp:35-36)."""
+
+    def __init__(self, capacity=1 << 14):
+        from bounce_amd.types import empty_caches
+        self.caches = empty_caches(capacity)
+
+    def gather(self, ids):
+        return np.ascontiguousarray(self.caches[ids])
+
+    def scatter(self, ids, caches):
+        self.caches[ids] = caches
+
+
+def pyramid_inputs(trace, step):
+    """(ids, body pairs, xfs [n][2][12], sweeps [n][4][4]) of one step of a pyramid fixture."""
+    a, b = int(trace["step_offsets"][step]), int(trace["step_offsets"][step + 1])
+    ids = trace["ids"][a:b].astype(np.int64)
+    pairs = trace["id_pairs"][ids]
+    xfs = trace["body_xfs"][step][pairs]
+    sweeps = trace["body_sweeps"][step][pairs].reshape(-1, 4, 4)
+    return ids, pairs, xfs, sweeps
+
+
+def pyramid_full_outputs(trace, step):
+    """(manifolds, caches) the fixture keeps whole for one of PYRAMID_FULL_STEPS."""
+    k = list(int(s) for s in trace["full_steps"]).index(step)
+    a, b = int(trace["full_offsets"][k]), int(trace["full_offsets"][k + 1])
+    return trace["full_out"][a:b], trace["full_caches"][a:b]
