@@ -145,6 +145,98 @@ def separation_error_fp64(out, xfs, radius_a=0.0, radius_b=0.0):
             "max_abs_over_abs_separation_ge_1mm": float((err / np.maximum(np.abs(seps[1]), 1e-3)).max())}
 
 
+# ----------------------------------------------------------------- mixed shapes vs terrain
+
+_MESH_JOB = {}
+
+
+def _mesh_cpu_worker(chunk):
+    j = _MESH_JOB
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    ref = oracle_lib.reference()
+    lib, prefix = (ref.lib, "b3ref_") if ref is not None else (oracle_lib.oracle().lib, "b3o_")
+    # A first call on a few contacts builds the mesh's tree (b3Mesh::BuildTree), which the
+    # timed call then reuses.
+    oracle_lib.collide_mesh(lib, prefix, j["pool"], j["shapes"], j["verts"], j["tris"], j["nodes"],
+                            chunk[:8].astype(np.uint32), j["xfs"][chunk[:8]])
+    t0 = time.perf_counter()
+    oracle_lib.collide_mesh(lib, prefix, j["pool"], j["shapes"], j["verts"], j["tris"], j["nodes"],
+                            chunk.astype(np.uint32), j["xfs"][chunk])
+    return time.perf_counter() - t0
+
+
+def mesh_cpu_baseline(pool, shapes, verts, tris, xfs, sample):
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    kind = "reference" if oracle_lib.reference() is not None else "port"
+    nodes = oracle_lib.build_tree(oracle_lib.oracle().lib, "b3o_", verts, tris)
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    idx = np.arange(0, len(shapes), max(1, len(shapes) // sample))[:sample]
+    _MESH_JOB.update(pool=pool, shapes=shapes, verts=verts, tris=tris, nodes=nodes, xfs=xfs)
+    with mp.get_context("fork").Pool(cores) as p:
+        res = p.map(_mesh_cpu_worker, np.array_split(idx, cores), chunksize=1)
+    cpu_s = sum(res)
+    return {"value": len(idx) / (cpu_s / cores), "unit": "contacts/s", "cores": cores, "kind": kind,
+            "sample": "%d of the %d contacts (strided), %d processes, %.1f CPU-s (tree built beforehand)"
+                      % (len(idx), len(shapes), cores, cpu_s)}
+
+
+def mesh_run(world, steps, cpu):
+    """Mixed shapes on a 256 x 256 heightfield (131,072 triangles): 100,000 fresh mesh
+    contacts per batch (static-tree midphase, per-triangle narrowphase, clustering)."""
+    import bounce_amd
+    pool, shapes, verts, tris, xfs = world
+    res = {"workload": "256x256 heightfield (131,072 triangles, value noise 2 m, cell 0.5 m); 100,000 shapes: "
+                       "40% spheres r 0.1-0.5, 30% capsules L 0.2-1 r 0.1-0.3, 30% 8-32-point hulls 0.1-0.5 m; "
+                       "uniform over the grid, resting gap N(0, 0.05 m); fresh contacts",
+           "contacts": len(shapes)}
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    n = len(shapes)
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(pool)
+        ctx.upload_shapes(shapes)
+        mid = ctx.upload_mesh(verts, tris)
+        # Host arrays (PCIe both ways, the b3amd_collide_mesh form).
+        ctx.collide_mesh(mid, sa, xfs)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            counts, lists, out, mcount = ctx.collide_mesh(mid, sa, xfs)
+        host_wall = (time.perf_counter() - t0) / steps
+        # Device-resident inputs and outputs (b3amd_collide_mesh_device).
+        ds, dx = ctx.alloc(sa.nbytes), ctx.alloc(xfs.nbytes)
+        do, dm = ctx.alloc(out.nbytes), ctx.alloc(mcount.nbytes)
+        ds.upload(sa)
+        dx.upload(xfs)
+        ctx.collide_mesh_device(mid, n, ds, dx, do, dm)
+        ctx.synchronize()
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.collide_mesh_device(mid, n, ds, dx, do, dm)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        ctx.set_profiling(False)
+        kern = {k: ctx.kernel_stats(k) for k in ("mesh_midphase", "mesh_narrow", "mesh_cluster")}
+        same = bool(np.array_equal(do.download(np.zeros_like(out)).view(np.uint8), out.view(np.uint8)))
+    kms = {k: v[1] / max(1, v[0]) for k, v in kern.items()}
+    dev_ms = sum(kms.values())
+    res.update({"ms_per_batch": wall * 1000.0, "contacts_per_s": n / wall,
+                "host_arrays_ms_per_batch": host_wall * 1000.0, "host_arrays_contacts_per_s": n / host_wall,
+                "kernels_ms": kms, "kernels_sum_ms": dev_ms, "device_equals_host_path": same,
+                "triangles_per_contact": float(counts.mean()),
+                "manifolds": int(mcount.sum()), "points": int(out["pointCount"].sum()),
+                "touching_fraction": float((mcount > 0).mean()),
+                "what": "b3amd_collide_mesh_device on resident inputs (value); b3amd_collide_mesh on host "
+                        "arrays beside it"})
+    if "cpu_baseline" in res:
+        res["gpu_over_cpu"] = res["contacts_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
 # ----------------------------------------------------------------- C1: the box pyramid
 
 def pyramid_run(lib):
@@ -264,6 +356,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-pyramid", action="store_true")
+    ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--serving-worlds", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=240000.0)
     ap.add_argument("--serving-seconds", type=float, default=0.5)
@@ -289,6 +382,13 @@ def main():
         t = time.perf_counter()
         cpu = cpu_baseline(pool, pairs, xfs, args.cpu_sample)
         log("cpu baseline: %.0f pairs/s (%s) in %.1f s" % (cpu["value"], cpu["kind"], time.perf_counter() - t))
+    mesh_world = mesh_cpu = None
+    if not args.no_mesh:
+        from bounce_amd.workloads import gen_mesh_world
+        mesh_world = gen_mesh_world(lib, 7)
+        if not args.no_cpu:  # also before the GPU is initialised
+            mesh_cpu = mesh_cpu_baseline(*mesh_world, 100000)
+            log("mesh cpu baseline: %.0f contacts/s (%s)" % (mesh_cpu["value"], mesh_cpu["kind"]))
     work = algorithmic_work(pool, pairs, xfs, max(1, len(pairs) // 8192))
     log("algorithmic work: %.3g SAT flops, %.3g SAT bytes per batch" % (work["sat_flops"], work["sat_bytes"]))
 
@@ -386,6 +486,11 @@ def main():
                                   "transforms, the same kernels, D2H of manifolds"},
         "separation_error_fp64": separation_error_fp64(out, xfs),
     }
+    if not args.no_mesh:
+        result["mesh"] = mesh_run(mesh_world, 3, mesh_cpu)
+        log("mesh: %.0f contacts/s resident (%.3f ms per batch), %.0f contacts/s from host arrays"
+            % (result["mesh"]["contacts_per_s"], result["mesh"]["ms_per_batch"],
+               result["mesh"]["host_arrays_contacts_per_s"]))
     if not args.no_pyramid:
         result["pyramid"] = pyramid_run(lib)
         log("pyramid: %d/%d steps equal to the reference, %.3f ms per step"

@@ -58,6 +58,7 @@ def load_library():
         "b3amd_upload_mesh": (i32, [vp, u32, vp, u32, vp, ctypes.POINTER(u32)]),
         "b3amd_mesh_tree": (i32, [vp, u32, ctypes.POINTER(u32), vp]),
         "b3amd_collide_mesh": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
+        "b3amd_collide_mesh_device": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
         "b3amd_set_sat_tuning": (i32, [vp, u32, u32]),
         "b3amd_get_sat_tuning": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "b3amd_autotune": (i32, [vp, u32, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]),
@@ -296,6 +297,13 @@ class Context:
         _check(self.lib.b3amd_collide_mesh(self.handle, mesh_id, n, ptr(shape_a), ptr(xfs), ptr(counts), ptr(lists),
                                            ptr(out), ptr(mcount)))
         return counts, lists, out, mcount
+
+    def collide_mesh_device(self, mesh_id, n, d_shape_a, d_xfs, d_out, d_manifold_counts, d_triangle_counts=None,
+                            d_triangle_lists=None):
+        """b3amd_collide_mesh_device on DeviceBuffers (no synchronisation)."""
+        _check(self.lib.b3amd_collide_mesh_device(
+            self.handle, mesh_id, n, d_shape_a.ptr, d_xfs.ptr, d_triangle_counts.ptr if d_triangle_counts else None,
+            d_triangle_lists.ptr if d_triangle_lists else None, d_out.ptr, d_manifold_counts.ptr))
 
     def alloc(self, nbytes):
         return DeviceBuffer(self, nbytes)

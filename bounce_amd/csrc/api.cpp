@@ -503,52 +503,73 @@ int b3amd_mesh_tree(b3amd_context* ctx, uint32_t meshId, uint32_t* nodeCount, fl
     return B3AMD_OK;
 }
 
-int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
-                       const b3amd_transform* xfs, uint32_t* triangleCounts, uint32_t* triangleLists,
-                       b3amd_manifold* out, uint32_t* manifoldCounts) {
-    if (!ctx || (n && (!shapeA || !xfs || !out || !manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    if (meshId >= ctx->meshes.size()) return fail(B3AMD_ERR_ARGUMENT, "bad mesh id %u", meshId);
-    if (n == 0) return B3AMD_OK;
-    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
-    for (uint32_t i = 0; i < n; ++i) {
-        if (shapeA[i] >= ctx->shapeCount) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape %u of %u", i, shapeA[i], ctx->shapeCount);
-        if (ctx->shapes[shapeA[i]].type > B3AMD_HULL) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape A must be convex", i);
-    }
-    const uint32_t cap = kMeshTriangleCap;
+}  // extern "C"
+
+namespace {
+
+// Grows the persistent mesh scratch to n contacts: staged inputs, per-contact midphase
+// results, the segment work lists and the per-triangle manifolds (n x cap bound, so no
+// step needs the work-item count on the host).
+struct MeshScratch {
+    uint32_t* shapeA;
+    float* xfs;
+    uint32_t* triCounts;
+    uint32_t* triLists;
+    uint32_t* flags;
+    uint32_t* offsets;
+    uint32_t* totals;
+    uint32_t* pairContact;
+    Manifold* out;
+    uint32_t* manifoldCounts;
+    Manifold* temp;
+};
+
+int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
+    const uint64_t cap = kMeshTriangleCap;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    // Fixed part: inputs, per-contact outputs and the flat work list.
-    const uint64_t bShape = 4ull * n, bXf = 96ull * n, bCounts = 4ull * n, bLists = 4ull * n * cap,
-                   bFlags = 4ull * n, bOff = 4ull * n, bTotal = 256, bPairs = 4ull * n * cap, bOut = 720ull * n,
-                   bMc = 4ull * n;
-    const uint64_t fixed = al(bShape) + al(bXf) + al(bCounts) + al(bLists) + al(bFlags) + al(bOff) + al(bTotal) +
-                           al(bPairs) + al(bOut) + al(bMc);
-    int rc;
-    if (fixed > ctx->meshScratchCap) {
+    const uint64_t sizes[11] = {4ull * n, 96ull * n, 4ull * n, 4ull * n * cap, 4ull * n, 4ull * n, 256,
+                                12ull * n * cap, 720ull * n, 4ull * n, sizeof(Manifold) * n * cap};
+    uint64_t total = 0;
+    for (uint64_t b : sizes) total += al(b);
+    if (total > ctx->meshScratchCap) {
+        int rc;
         if ((rc = ctx->release(ctx->dMeshScratch))) return rc;
         ctx->dMeshScratch = nullptr;
         ctx->meshScratchCap = 0;
-        if ((rc = ctx->alloc(fixed, &ctx->dMeshScratch))) return rc;
-        ctx->meshScratchCap = fixed;
+        if ((rc = ctx->alloc(total, &ctx->dMeshScratch))) return rc;
+        ctx->meshScratchCap = total;
     }
     char* p = static_cast<char*>(ctx->dMeshScratch);
-    auto take = [&](uint64_t b) {
-        char* q = p;
-        p += al(b);
-        return q;
-    };
+    void* parts[11];
+    for (int i = 0; i < 11; ++i) {
+        parts[i] = p;
+        p += al(sizes[i]);
+    }
+    *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2], (uint32_t*)parts[3],
+                     (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint32_t*)parts[7],
+                     (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10]};
+    return B3AMD_OK;
+}
+
+// Enqueues midphase, per-triangle narrowphase and clustering for n contacts whose inputs
+// are on the device (no synchronisation).
+int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA, const float* xfs,
+                 uint32_t* triCounts, uint32_t* triLists, Manifold* out, uint32_t* manifoldCounts,
+                 const MeshScratch& sc) {
     MeshArgs a;
     a.counters = ctx->counting ? ctx->dCounters : nullptr;
     a.n = n;
-    a.shapeA = reinterpret_cast<uint32_t*>(take(bShape));
-    a.xfs = reinterpret_cast<float*>(take(bXf));
-    a.triCounts = reinterpret_cast<uint32_t*>(take(bCounts));
-    a.triLists = reinterpret_cast<uint32_t*>(take(bLists));
-    a.contactFlags = reinterpret_cast<uint32_t*>(take(bFlags));
-    a.contactOffset = reinterpret_cast<uint32_t*>(take(bOff));
-    a.total = reinterpret_cast<uint32_t*>(take(bTotal));
-    a.pairContact = reinterpret_cast<uint32_t*>(take(bPairs));
-    a.out = reinterpret_cast<Manifold*>(take(bOut));
-    a.manifoldCounts = reinterpret_cast<uint32_t*>(take(bMc));
+    a.shapeA = shapeA;
+    a.xfs = xfs;
+    a.triCounts = triCounts ? triCounts : sc.triCounts;
+    a.triLists = triLists ? triLists : sc.triLists;
+    a.contactFlags = sc.flags;
+    a.contactOffset = sc.offsets;
+    a.totals = sc.totals;
+    a.pairContact = sc.pairContact;
+    a.out = out;
+    a.manifoldCounts = manifoldCounts;
+    a.temp = sc.temp;
     a.shapes = ctx->dShapes;
     a.shapeCount = ctx->shapeCount;
     a.hullCount = ctx->hullCount;
@@ -556,36 +577,68 @@ int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const ui
     a.offsets = ctx->dOffsets;
     const auto& mr = ctx->meshes[meshId];
     a.mesh = MeshDev{mr.verts, mr.tris, reinterpret_cast<const uint32_t*>(mr.nodes), mr.nodeCount};
-    a.cap = cap;
-    a.temp = nullptr;
+    a.cap = kMeshTriangleCap;
     hipStream_t s = ctx->stream;
-    HIP_TRY(hipMemcpyAsync(const_cast<uint32_t*>(a.shapeA), shapeA, bShape, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemcpyAsync(const_cast<float*>(a.xfs), xfs, bXf, hipMemcpyHostToDevice, s));
-    HIP_TRY(hipMemsetAsync(a.total, 0, 4, s));
+    int rc;
+    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 4, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
-    uint32_t total = 0;
-    HIP_TRY(hipMemcpyAsync(&total, a.total, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    // Per-triangle manifolds, sized by what the midphase found.
-    void* temp = nullptr;
-    if (total && (rc = ctx->alloc(240ull * total, &temp))) return rc;
-    a.temp = static_cast<Manifold*>(temp);
-    rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, total, s); });
-    if (!rc) rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
-    if (!rc) {
-        hipError_t e = hipMemcpyAsync(out, a.out, bOut, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, a.manifoldCounts, bMc, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess && triangleCounts) e = hipMemcpyAsync(triangleCounts, a.triCounts, bCounts, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess && triangleLists) e = hipMemcpyAsync(triangleLists, a.triLists, bLists, hipMemcpyDeviceToHost, s);
-        if (e == hipSuccess) e = hipStreamSynchronize(s);
-        if (e != hipSuccess) rc = fail(B3AMD_ERR_DEVICE, "mesh readback: %s", hipGetErrorString(e));
+    if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
+    return timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
+}
+
+int check_mesh_call(b3amd_context* ctx, uint32_t meshId, uint32_t n) {
+    if (meshId >= ctx->meshes.size()) return fail(B3AMD_ERR_ARGUMENT, "bad mesh id %u", meshId);
+    if (n && !ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    if ((uint64_t)n * kMeshTriangleCap > 0xFFFFFFFFull)
+        return fail(B3AMD_ERR_UNSUPPORTED, "%u mesh contacts exceed the work-item index range", n);
+    return B3AMD_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
+                       const b3amd_transform* xfs, uint32_t* triangleCounts, uint32_t* triangleLists,
+                       b3amd_manifold* out, uint32_t* manifoldCounts) {
+    if (!ctx || (n && (!shapeA || !xfs || !out || !manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    int rc;
+    if ((rc = check_mesh_call(ctx, meshId, n))) return rc;
+    if (n == 0) return B3AMD_OK;
+    for (uint32_t i = 0; i < n; ++i) {
+        if (shapeA[i] >= ctx->shapeCount) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape %u of %u", i, shapeA[i], ctx->shapeCount);
+        if (ctx->shapes[shapeA[i]].type > B3AMD_HULL) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape A must be convex", i);
     }
-    if (temp) {
-        int rc2 = ctx->release(temp);
-        if (!rc) rc = rc2;
-    }
-    if (!rc) rc = ctx->resolve_timings();
-    return rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(sc.shapeA, shapeA, 4ull * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
+    if ((rc = enqueue_mesh(ctx, meshId, n, sc.shapeA, sc.xfs, nullptr, nullptr, sc.out, sc.manifoldCounts, sc)))
+        return rc;
+    hipError_t e = hipMemcpyAsync(out, sc.out, 720ull * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, sc.manifoldCounts, 4ull * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && triangleCounts)
+        e = hipMemcpyAsync(triangleCounts, sc.triCounts, 4ull * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && triangleLists)
+        e = hipMemcpyAsync(triangleLists, sc.triLists, 4ull * n * kMeshTriangleCap, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "mesh readback: %s", hipGetErrorString(e));
+    return ctx->resolve_timings();
+}
+
+int b3amd_collide_mesh_device(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* d_shapeA,
+                              const b3amd_transform* d_xfs, uint32_t* d_triangleCounts, uint32_t* d_triangleLists,
+                              b3amd_manifold* d_out, uint32_t* d_manifoldCounts) {
+    if (!ctx || (n && (!d_shapeA || !d_xfs || !d_out || !d_manifoldCounts)))
+        return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    int rc;
+    if ((rc = check_mesh_call(ctx, meshId, n))) return rc;
+    if (n == 0) return B3AMD_OK;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    return enqueue_mesh(ctx, meshId, n, d_shapeA, reinterpret_cast<const float*>(d_xfs), d_triangleCounts,
+                        d_triangleLists, reinterpret_cast<Manifold*>(d_out), d_manifoldCounts, sc);
 }
 
 int b3amd_synchronize(b3amd_context* ctx) {

@@ -166,16 +166,19 @@ struct MeshArgs {
     uint32_t* triCounts;        // [n] triangles the midphase found (may exceed cap)
     uint32_t* triLists;         // [n][cap] in DFS order
     uint32_t* contactFlags;     // [n]
-    uint32_t* contactOffset;    // [n] first flat work item of the contact
-    uint32_t* total;            // flat work items (atomic counter, zeroed before the midphase)
-    uint32_t* pairContact;      // [n * cap] flat work item -> contact
-    Manifold* temp;             // [total] per-triangle manifolds
+    // Work items are grouped by the convex shape's collider so that a wave runs one of
+    // them: segment 0 hulls, 1 capsules, 2 spheres, in that order in `temp`.
+    uint32_t* contactOffset;    // [n] first work item of the contact within its segment
+    uint32_t* totals;           // [4] work items per segment (zeroed before the midphase)
+    uint32_t* pairContact;      // [3][n * cap] work item of a segment -> contact
+    Manifold* temp;             // [n * cap] per-triangle manifolds, segments back to back
     Manifold* out;              // [n][3] clustered manifolds
     uint32_t* manifoldCounts;   // [n]
 };
 
 hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
-hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s);
+// The grid covers the n * cap bound: the work-item count stays on the device.
+hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);

@@ -3,7 +3,8 @@
 //   mesh_midphase_kernel  one lane per mesh contact: the fat AABB of shape A in the mesh
 //                         frame (mesh_contact.cpp:30-60) and the static-tree query in DFS
 //                         order, child2 first (static_tree.h:100-142); triangle lists are
-//                         compacted into one flat (contact, triangle) work list.
+//                         compacted into (contact, triangle) work items, grouped by the
+//                         convex shape's type so that each wave runs one collider.
 //   mesh_narrow_kernel    one lane per (contact, triangle): b3TriangleHull built in
 //                         registers and b3CollideShapeAndShape with an empty cache, the
 //                         triangle index stamped into every point key (:238-266).
@@ -70,6 +71,9 @@ __device__ __forceinline__ bool overlaps(const float4& lo, const float4& hi, con
 
 constexpr uint32_t kTreeStack = 128;
 
+// Work-item segment of a convex shape type: hulls, capsules, spheres.
+__device__ __forceinline__ uint32_t segment_of(uint32_t type) { return 2u - type; }
+
 __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
@@ -109,23 +113,36 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
         flags |= kFlagInvalidPair;
     }
     if (found > a.cap) flags |= kFlagTriangleOverflow;
-    const uint32_t m = found < a.cap ? found : a.cap;
+    const uint32_t m = (flags & kFlagInvalidPair) ? 0u : (found < a.cap ? found : a.cap);
     a.triCounts[c] = found;
     a.contactFlags[c] = flags;
-    const uint32_t off = atomicAdd(a.total, m);
+    const uint32_t seg = m > 0 ? segment_of(a.shapes[si].type) : 3u;
+    // One atomic per segment with a wave-uniform address, so each folds into a single
+    // wave-level add (a per-lane segment address would serialise every lane's add).
+    uint32_t off = 0;
+    for (uint32_t g = 0; g < 3u; ++g) {
+        const uint32_t o = atomicAdd(a.totals + g, seg == g ? m : 0u);
+        if (seg == g) off = o;
+    }
+    if (m > 0) {
+        uint32_t* const items = a.pairContact + (size_t)seg * a.n * a.cap + off;
+        for (uint32_t k = 0; k < m; ++k) items[k] = c;
+    }
     a.contactOffset[c] = off;
-    for (uint32_t k = 0; k < m; ++k) a.pairContact[off + k] = c;
 }
 
 // With counting on, hull-triangle collides go to b3_convexCalls and the sphere/capsule
 // GJK calls to the b3_gjk* totals, as b3MeshContact::Collide's per-triangle
 // b3CollideShapeAndShape calls count them in the reference (mesh_contact.cpp:217-275).
-__global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a, uint32_t total) {
+__global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t t0 = a.totals[0], t1 = a.totals[1], t2 = a.totals[2];
     uint32_t hullCalls = 0, calls = 0, iters = 0;
-    if (f < total) {
-        const uint32_t c = a.pairContact[f];
-        const uint32_t k = f - a.contactOffset[c];
+    if (f < t0 + t1 + t2) {
+        const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
+        const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
+        const uint32_t c = a.pairContact[(size_t)seg * a.n * a.cap + local];
+        const uint32_t k = local - a.contactOffset[c];
         const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
         Manifold* const out = a.temp + f;
         const uint4 t = a.mesh.tris[ti];
@@ -230,8 +247,10 @@ __global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
     const uint32_t m = found < a.cap ? found : a.cap;
     uint32_t numOut = 0;
     if (!(flags & kFlagInvalidPair) && m > 0) {
-        const Manifold* in = a.temp + a.contactOffset[ci];
         const ShapeRec s = a.shapes[a.shapeA[ci]];
+        const uint32_t seg = segment_of(s.type);
+        const uint32_t base = seg == 0u ? 0u : (seg == 1u ? a.totals[0] : a.totals[0] + a.totals[1]);
+        const Manifold* in = a.temp + base + a.contactOffset[ci];
         const Xf xfA = load_xf(a.xfs + 24 * (size_t)ci);
         const Xf xfB = load_xf(a.xfs + 24 * (size_t)ci + 12);
         const float rA = s.radius, rB = 0.0f;
@@ -410,9 +429,10 @@ hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_mesh_narrow(const MeshArgs& a, uint32_t total, hipStream_t s) {
-    if (total == 0) return hipSuccess;
-    mesh_narrow_kernel<<<(total + 255) / 256, 256, 0, s>>>(a, total);
+hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s) {
+    const uint64_t bound = (uint64_t)a.n * a.cap;
+    if (bound == 0) return hipSuccess;
+    mesh_narrow_kernel<<<(uint32_t)((bound + 255) / 256), 256, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -154,3 +154,92 @@ def world_pair_inputs(world, step):
     pairs = world["body_shapes"][bp].astype(np.uint32)
     xfs = world["xfs"][step][bp]
     return np.ascontiguousarray(pairs), np.ascontiguousarray(xfs)
+
+
+def gen_heightfield(seed, H=256, W=256, cell=0.5, amplitude=2.0, lattice=8):
+    """(H+1) x (W+1) vertex heightfield triangulated as b3GridMesh (grid_mesh.h:57-77):
+    heights = amplitude x value noise (uniform lattice values every `lattice` cells,
+    smoothstep interpolation).  Returns (vertices [V,3] f32, triangles [2HW,3] u32)."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(-1.0, 1.0, size=(H // lattice + 2, W // lattice + 2))
+    i, j = np.meshgrid(np.arange(H + 1), np.arange(W + 1), indexing="ij")
+    u, v = i / lattice, j / lattice
+    i0, j0 = u.astype(np.int64), v.astype(np.int64)
+    fu, fv = u - i0, v - j0
+    su, sv = fu * fu * (3 - 2 * fu), fv * fv * (3 - 2 * fv)
+    h = (g[i0, j0] * (1 - su) * (1 - sv) + g[i0 + 1, j0] * su * (1 - sv) +
+         g[i0, j0 + 1] * (1 - su) * sv + g[i0 + 1, j0 + 1] * su * sv)
+    verts = np.stack([(j - 0.5 * W) * cell, amplitude * h, (i - 0.5 * H) * cell], -1).reshape(-1, 3)
+    ii, jj = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    v1 = (ii * (W + 1) + jj).ravel()
+    v2 = ((ii + 1) * (W + 1) + jj).ravel()
+    v3 = ((ii + 1) * (W + 1) + jj + 1).ravel()
+    v4 = (ii * (W + 1) + jj + 1).ravel()
+    tris = np.empty((2 * H * W, 3), np.uint32)
+    tris[0::2] = np.stack([v3, v2, v1], -1)
+    tris[1::2] = np.stack([v1, v4, v3], -1)
+    return verts.astype(np.float32), tris
+
+
+def _uniform_rotations(rng, n):
+    """n uniformly distributed rotation matrices (normalised 4-D Gaussian quaternions)."""
+    q = rng.normal(size=(n, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    x, y, z, w = q.T
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def gen_mesh_world(lib, seed, H=256, W=256, n_shapes=100000, cell=0.5, amplitude=2.0, gap_sigma=0.05,
+                   hull_pool=2048):
+    """Mixed shapes resting on a heightfield: 40% spheres (r ~ U[0.1, 0.5]), 30% capsules
+    (length ~ U[0.2, 1], r ~ U[0.1, 0.3]), 30% hulls of 8-32 points (semi-axes
+    U[0.1, 0.5] m), uniform over the grid interior, uniform orientations, lowest point at
+    the terrain height under the shape's centre + N(0, gap_sigma).  Returns (pool,
+    shapes [n], vertices, triangles, xfs [n][2][12] = (shape body, identity mesh body))."""
+    from .types import CAPSULE, HULL, SHAPE_DTYPE, SPHERE
+    rng = np.random.default_rng(seed)
+    verts, tris = gen_heightfield(seed, H, W, cell, amplitude)
+    pool = gen_hull_pool(lib, seed, hull_pool, 8, 32, 0.1, 0.5)
+    n = n_shapes
+    kind = rng.random(n)
+    R = _uniform_rotations(rng, n)
+    shapes = np.zeros(n, SHAPE_DTYPE)
+    low = np.zeros(n)
+    sph = kind < 0.4
+    cap = (kind >= 0.4) & (kind < 0.7)
+    hul = kind >= 0.7
+    shapes["type"][sph] = SPHERE
+    shapes["radius"][sph] = rng.uniform(0.1, 0.5, sph.sum())
+    low[sph] = shapes["radius"][sph]
+    L = rng.uniform(0.2, 1.0, n)
+    shapes["type"][cap] = CAPSULE
+    shapes["radius"][cap] = rng.uniform(0.1, 0.3, cap.sum())
+    shapes["v"][cap, 1] = -L[cap] / 2
+    shapes["v"][cap, 4] = L[cap] / 2
+    low[cap] = shapes["radius"][cap] + np.abs(R[cap, 1, 1]) * L[cap] / 2
+    shapes["type"][hul] = HULL
+    hid = rng.integers(0, pool.count, n)
+    shapes["hull"][hul] = hid[hul]
+    for h in np.unique(hid[hul]):  # lowest vertex of each rotated hull
+        sel = np.where(hul & (hid == h))[0]
+        v = pool.hull(int(h))[0].astype(np.float64)
+        low[sel] = -np.einsum("kj,nj->nk", v, R[sel, 1, :]).min(1)
+    half = 0.5 * cell * np.array([W, H])
+    x = rng.uniform(-half[0] + 1.0, half[0] - 1.0, n)
+    z = rng.uniform(-half[1] + 1.0, half[1] - 1.0, n)
+    # Terrain height under (x, z): bilinear over the grid cell (between its two triangles).
+    gx, gz = (x + half[0]) / cell, (z + half[1]) / cell
+    j0, i0 = np.floor(gx).astype(np.int64), np.floor(gz).astype(np.int64)
+    fx, fz = gx - j0, gz - i0
+    hv = verts[:, 1].astype(np.float64).reshape(H + 1, W + 1)
+    ground = (hv[i0, j0] * (1 - fx) * (1 - fz) + hv[i0, j0 + 1] * fx * (1 - fz) +
+              hv[i0 + 1, j0] * (1 - fx) * fz + hv[i0 + 1, j0 + 1] * fx * fz)
+    y = ground + low + rng.normal(0.0, gap_sigma, n)
+    xfs = np.zeros((n, 2, 12), np.float32)
+    xfs[:, 0, 0:3], xfs[:, 0, 3:6], xfs[:, 0, 6:9] = R[:, :, 0], R[:, :, 1], R[:, :, 2]
+    xfs[:, 0, 9:12] = np.stack([x, y, z], -1)
+    xfs[:, 1, 0], xfs[:, 1, 4], xfs[:, 1, 8] = 1.0, 1.0, 1.0
+    return pool, shapes, verts, tris, xfs

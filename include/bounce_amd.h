@@ -276,6 +276,14 @@ int b3amd_mesh_tree(b3amd_context* ctx, uint32_t meshId, uint32_t* nodeCount, fl
 int b3amd_collide_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
                        const b3amd_transform* xfs, uint32_t* triangleCounts, uint32_t* triangleLists,
                        b3amd_manifold* out, uint32_t* manifoldCounts);
+/* Same on device-resident buffers (b3amd_device_alloc), enqueued on the context stream
+ * without synchronizing: the work-item count stays on the device, so a batch needs no
+ * host round trip between midphase and narrowphase.  d_triangleCounts and d_triangleLists
+ * may be NULL.  A shape index out of range or a non-convex shape A gives
+ * B3AMD_FLAG_INVALID_PAIR in that contact's manifold flags and no manifolds. */
+int b3amd_collide_mesh_device(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* d_shapeA,
+                              const b3amd_transform* d_xfs, uint32_t* d_triangleCounts, uint32_t* d_triangleLists,
+                              b3amd_manifold* d_out, uint32_t* d_manifoldCounts);
 
 /* ---- device memory helpers (callers need no other GPU runtime) ---- */
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);

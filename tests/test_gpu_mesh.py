@@ -39,3 +39,66 @@ def test_gpu_mesh_equals_oracle(ctx, lib, oracle, seed, size):
     np.testing.assert_array_equal(counts, co[0])
     np.testing.assert_array_equal(mcount, co[3])
     assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "mesh manifolds")
+
+
+def test_gpu_mesh_heightfield_full_size(ctx, lib, oracle):
+    """The mixed-shapes-on-terrain configuration at full size: a 256 x 256 heightfield
+    (131,072 triangles) and 100,000 spheres, capsules and hulls.  All contacts run on the
+    device; every 50th is checked against the oracle (tree, triangle lists, clustered
+    manifolds, bit-exact); the whole batch keeps the structural invariants."""
+    from bounce_amd.workloads import gen_mesh_world
+    print("case: heightfield 256x256, 100000 shapes", flush=True)
+    pool, shapes, v, t, xfs = gen_mesh_world(lib, 11)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    nodes = ctx.mesh_tree(mid)
+    np.testing.assert_array_equal(nodes.view(np.uint32),
+                                  oracle_lib.build_tree(oracle.lib, "b3o_", v, t).view(np.uint32))
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    counts, lists, out, mcount = ctx.collide_mesh(mid, sa, xfs)
+    assert (counts <= ctx.MESH_TRIANGLE_CAP).all()
+    assert (mcount <= 3).all() and (out["pointCount"] <= 4).all()
+    assert (out["pointCount"][np.arange(3)[None, :] >= mcount[:, None]] == 0).all()
+    assert 0.5 < (mcount > 0).mean() < 0.9
+    idx = np.arange(0, len(sa), 50)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa[idx], xfs[idx])
+    np.testing.assert_array_equal(counts[idx], co[0])
+    for k, i in enumerate(idx):
+        c = int(counts[i])
+        np.testing.assert_array_equal(lists[i, :c], co[1][k, :c])
+    np.testing.assert_array_equal(mcount[idx], co[3])
+    assert_manifolds_equal(out[idx].reshape(-1), co[2].reshape(-1), "heightfield manifolds")
+
+
+def test_gpu_mesh_device_resident_equals_host_api(ctx, lib):
+    """b3amd_collide_mesh_device on resident buffers returns what b3amd_collide_mesh
+    returns, on two batches in a row (the second smaller, reusing the scratch), and flags
+    a contact whose shape index is out of range instead of reading past the table."""
+    from bounce_amd.types import MANIFOLD_DTYPE
+    print("case: mesh device-resident entry point", flush=True)
+    pool, shapes, v, t, xfs = mesh_workload(lib, 77, 64, 64, 600)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    for n in (600, 250):
+        sa = np.arange(n, dtype=np.uint32)[::-1].copy()
+        x = np.ascontiguousarray(xfs[sa])
+        counts, lists, out, mcount = ctx.collide_mesh(mid, sa, x)
+        bad = sa.copy()
+        bad[7] = len(shapes) + 5
+        ds, dx, do, dm = ctx.alloc(sa.nbytes), ctx.alloc(x.nbytes), ctx.alloc(out.nbytes), ctx.alloc(4 * n)
+        dc, dl = ctx.alloc(counts.nbytes), ctx.alloc(lists.nbytes)
+        ds.upload(bad)
+        dx.upload(x)
+        ctx.collide_mesh_device(mid, n, ds, dx, do, dm, dc, dl)
+        ctx.synchronize()
+        got = do.download(np.zeros((n, 3), MANIFOLD_DTYPE))
+        gm = dm.download(np.zeros(n, np.uint32))
+        keep = np.arange(n) != 7
+        assert_manifolds_equal(got[keep].reshape(-1), out[keep].reshape(-1), "device-resident mesh")
+        np.testing.assert_array_equal(gm[keep], mcount[keep])
+        np.testing.assert_array_equal(dc.download(np.zeros(n, np.uint32))[keep], counts[keep])
+        assert gm[7] == 0 and (got[7]["flags"] & 2).all() and (got[7]["pointCount"] == 0).all()
+        for b in (ds, dx, do, dm, dc, dl):
+            b.free()
