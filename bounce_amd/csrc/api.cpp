@@ -4,6 +4,7 @@
 // (contact_manager.cpp:167-231).
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -82,6 +83,33 @@ namespace {
 
 // Converts one b3Hull view into the device record (device_hull.h); returns an error
 // string or nullptr.
+// Bounding cap of the Gauss-map arc between unit normals U and V (the shorter great-circle
+// arc): centre m = (U + V) / |U + V|, radius angle(U, V) / 2 = atan2(|U - V|, |U + V|),
+// widened by 0.01 rad; returned as cos and sin of the widened radius.  A radius of pi or
+// more (or U = -V) covers the sphere: cos = -1, sin = 0, m = 0.
+void arc_cap(const float* U, const float* V, float* m, float* cosA, float* sinA) {
+    double u[3], v[3], nu = 0.0, nv = 0.0;
+    for (int i = 0; i < 3; ++i) u[i] = U[i], v[i] = V[i], nu += u[i] * u[i], nv += v[i] * v[i];
+    nu = std::sqrt(nu), nv = std::sqrt(nv);
+    double s[3], d[3], ls = 0.0, ld = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        u[i] /= nu, v[i] /= nv;
+        s[i] = u[i] + v[i], d[i] = u[i] - v[i];
+        ls += s[i] * s[i], ld += d[i] * d[i];
+    }
+    ls = std::sqrt(ls), ld = std::sqrt(ld);
+    const double a = std::atan2(ld, ls) + 0.01;
+    if (ls < 1e-9 || a >= 3.141592653589793) {
+        m[0] = m[1] = m[2] = 0.0f;
+        *cosA = -1.0f;
+        *sinA = 0.0f;
+        return;
+    }
+    for (int i = 0; i < 3; ++i) m[i] = (float)(s[i] / ls);
+    *cosA = (float)std::cos(a);
+    *sinA = (float)std::sin(a);
+}
+
 const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint32_t>& words,
                       HullLayout* layoutOut) {
     const uint32_t* d = pool->desc + 6 * (size_t)h;
@@ -130,9 +158,10 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     }
     uint16_t* fe = reinterpret_cast<uint16_t*>(words.data() + 4 * L.faceOff());
     for (uint32_t f = 0; f < F; ++f) fe[f] = (uint16_t)faces[f];
-    // Negated Gauss map of the hull in its own frame (sat.cpp:131-156 for hull 2):
-    // -U, -V, -E = -(Q - P) and P per unique edge; the padding records stay zero.
+    // Negated Gauss map of the hull in its own frame (sat.cpp:131-156 for hull 2): -U, -V,
+    // -E = -(Q - P) per unique edge and the arc's bounding cap (centre, widened radius).
     float* gm = fw + 4 * L.gaussOff();
+    for (uint32_t k = L.E(); k < L.gaussEdges(); ++k) gm[16 * k + 15] = 1e30f;
     for (uint32_t k = 0; k < L.E(); ++k) {
         const uint32_t* a = edges + 8 * k;
         const uint32_t* b = a + 4;
@@ -141,10 +170,13 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
         const float* U = planes + 4 * a[2];
         const float* Vn = planes + 4 * b[2];
         const float E[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
-        float* r = gm + 12 * k;
+        float m[3], ca, sa;
+        arc_cap(U, Vn, m, &ca, &sa);
+        float* r = gm + 16 * k;
         r[0] = -U[0], r[1] = -U[1], r[2] = -U[2], r[3] = -Vn[0];
         r[4] = -Vn[1], r[5] = -Vn[2], r[6] = -E[0], r[7] = -E[1];
-        r[8] = -E[2], r[9] = P[0], r[10] = P[1], r[11] = P[2];
+        r[8] = -E[2], r[9] = sa, r[10] = 0.0f, r[11] = 0.0f;
+        r[12] = -m[0], r[13] = -m[1], r[14] = -m[2], r[15] = ca;
     }
     *layoutOut = L;
     return nullptr;

@@ -10,14 +10,19 @@
 //   then          HE half-edges, one u32 each: origin | face << 8 | next << 16; the twin
 //                 of half-edge h is h ^ 1 (required by sat.cpp:123-128, checked at upload)
 //   then          F u16 first half-edge per face (b3Face::edge)
-//   then          the negated Gauss map, 3 units per unique edge k padded to a multiple
-//                 of 4 edges with zero records: {-U.xyz, -V.x}, {-V.yz, -E.xy},
-//                 {-E.z, P.xyz} with U, V the normals of faces(2k), face(2k+1), P the
-//                 origin of 2k and E = origin(2k+1) - P, in the hull's own frame.  These
-//                 are exactly the operands hull 2 contributes to b3IsMinkowskiFace and
-//                 b3Project in the edge query (sat.cpp:131-156), so the SAT kernel
-//                 broadcasts them from the scalar cache instead of recomputing them.
-//                 A zero record fails the Gauss-map test (all four dots are 0).
+//   then          the negated Gauss map, 4 units per unique edge k padded to a multiple
+//                 of 4 edges: {-U.xyz, -V.x}, {-V.yz, -E.xy}, {-E.z, sin a, 0, 0},
+//                 {-m.xyz, cos a} with U, V the normals of faces(2k), face(2k+1) and
+//                 E = origin(2k+1) - origin(2k), in the hull's own frame.  -U, -V, -E are
+//                 exactly the operands hull 2 contributes to b3IsMinkowskiFace in the edge
+//                 query (sat.cpp:131-156).  m and a bound the edge's Gauss-map arc from U
+//                 to V: the cap of centre m = (U + V) / |U + V| and angular radius
+//                 a = angle(U, V) / 2 + 0.01 rad contains the arc with a margin far above
+//                 FP32 rounding, so two arcs whose caps are disjoint cannot pass the
+//                 Gauss-map test; the SAT kernel tests caps first and runs the exact test
+//                 only for pairs whose caps overlap.  (-m, a) is the cap of the negated
+//                 arc, and a hull used as hull 1 reads m back as the negation.  Padding
+//                 records are zero with cos a = 1e30: they fail both tests.
 #ifndef B3AMD_DEVICE_HULL_H
 #define B3AMD_DEVICE_HULL_H
 
@@ -41,7 +46,7 @@ struct HullLayout {
     B3_HD uint32_t faceOff() const { return halfOff() + (HE + 3) / 4; }
     B3_HD uint32_t gaussOff() const { return faceOff() + (F + 7) / 8; }
     B3_HD uint32_t gaussEdges() const { return (E() + 3) & ~3u; }
-    B3_HD uint32_t units() const { return gaussOff() + 3 * gaussEdges(); }
+    B3_HD uint32_t units() const { return gaussOff() + 4 * gaussEdges(); }
 };
 
 // Read-only view of one hull record in global memory.

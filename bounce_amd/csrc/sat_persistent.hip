@@ -9,13 +9,14 @@
 //     64k in slot k); the other hull's vertices are staged in LDS and read in index order
 //     by every lane at once (LDS broadcast), so each lane's support search is the
 //     reference's first maximum;
-//   * edge query: hull 1's unique edges, transformed into hull 2's frame, sit in up to
-//     three register slots per lane (9 floats each); hull 2's negated Gauss map, stored
-//     per hull at upload (device_hull.h), is staged in LDS 128 edges at a time and
-//     broadcast one edge per step.  A lane marks the (i, j) that pass b3IsMinkowskiFace in
-//     a 32-bit mask per slot; after every 32 edges the marks are appended to an LDS ring
-//     (one entry per lane per round) and projected 64 at a time by all lanes together,
-//     so b3Project's square roots and division run converged.
+//   * edge query: hull 1's unique edges, rotated into hull 2's frame, sit in up to three
+//     register slots per lane as the bounding caps of their Gauss-map arcs;
+//     hull 2's negated Gauss map and arc caps, stored per hull at upload (device_hull.h),
+//     are staged in LDS 96 edges at a time and broadcast one edge per step.  A cheap
+//     cap-overlap test selects the (i, j) worth the exact b3IsMinkowskiFace, which runs 64
+//     candidates at a time from an LDS ring; the pairs that pass go to a second ring and
+//     are projected 64 at a time by all lanes together, so b3Project's square roots and
+//     division run converged.
 // Ties resolve to the lowest index in the reference's scan order (sat.cpp:47,158), so the
 // result is bit-identical to the sequential loops whatever the processing order.
 #include <hip/hip_runtime.h>
@@ -70,16 +71,18 @@ struct UHull {
     __device__ HullLayout layout() const { return HullLayout{V, F, HE}; }
 };
 
-constexpr uint32_t kStageEdges = 128;  // hull-2 edges staged in LDS at a time
-constexpr uint32_t kHitCap = 256;      // ring of pending Gauss-map hits (power of two)
+constexpr uint32_t kStageEdges = 96;   // hull-2 edges staged in LDS at a time (a multiple of 32)
+constexpr uint32_t kHitCap = 128;      // rings of pending pairs: fewer than 64 + 64 at any time
+constexpr uint32_t kCandCap = 128;
 
 // LDS of one wave (the block is one wave).
 struct SatStage {
     union {
         float4 verts[kMaxHullVertices + 1];  // face queries: the other hull's vertices
-        float4 gauss[3 * kStageEdges];       // edge query: a chunk of hull 2's Gauss map
+        float4 gauss[4 * kStageEdges];       // edge query: a chunk of hull 2's Gauss map
     } u;
-    uint32_t hits[kHitCap];  // pending (i << 16 | j) pairs that passed the Gauss-map test
+    uint32_t hits[kHitCap];    // pending (i << 16 | j) pairs that passed the Gauss-map test
+    uint32_t cands[kCandCap];  // pending (i << 16 | j) pairs whose arc caps overlap
 };
 
 // The block is a single wave: __syncthreads() orders its LDS writes and reads.
@@ -149,9 +152,9 @@ __device__ __forceinline__ void face_query_any(const Xf& xf, const UHull& hA, co
 
 // Projects `count` queued hits (lane l takes entry head + l) with b3Project (sat.cpp:73-108)
 // and folds them into (best, key); j is a hull-2 edge of the staged chunk starting at c0.
-__device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const SatStage& st,
-                                           uint32_t c0, uint32_t head, uint32_t count, uint32_t lane, float& best,
-                                           uint32_t& key) {
+__device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const UHull& h2,
+                                           CU32 ue2, const SatStage& st, uint32_t c0, uint32_t head, uint32_t count,
+                                           uint32_t lane, float& best, uint32_t& key) {
     if (lane < count) {
         const uint32_t e = st.hits[(head + lane) & (kHitCap - 1u)];
         const uint32_t i = e >> 16, j = e & 0xFFFFu;
@@ -159,9 +162,9 @@ __device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1,
         const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
         const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
         const V3 E1 = Q1 - P1;
-        const float4 r1 = st.u.gauss[3u * (j - c0) + 1u], r2 = st.u.gauss[3u * (j - c0) + 2u];
+        const float4 r1 = st.u.gauss[4u * (j - c0) + 1u], r2 = st.u.gauss[4u * (j - c0) + 2u];
         const V3 E2 = -V3{r1.z, r1.w, r2.x};
-        const V3 P2{r2.y, r2.z, r2.w};
+        const V3 P2 = ld3(h2.base[2 + (ue2[j] & 0xFFu)]);
         const float s = project_edges(P1, E1, length(E1), P1 - C1, P2, E2, length(E2));
         if (s > best || (s == best && e < key)) {
             best = s;
@@ -170,95 +173,155 @@ __device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1,
     }
 }
 
-// Edge query (sat.cpp:110-173) with K register slots of hull-1 edges per lane.
+// Hull 1's operands of b3IsMinkowskiFace for unique edge i in hull 2's frame (sat.cpp:123-146):
+// A = U1, B = V1, nE = -E1.
+__device__ __forceinline__ void edge1_operands(const Xf& xf, const UHull& h1, CU32 ue1, uint32_t i, V3& A, V3& B,
+                                               V3& nE) {
+    const uint32_t u = ue1[i];
+    const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
+    const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
+    A = mul(xf.R, ld3(h1.base[2 + h1.V + ((u >> 16) & 0xFFu)]));
+    B = mul(xf.R, ld3(h1.base[2 + h1.V + (u >> 24)]));
+    nE = -(Q1 - P1);
+}
+
+// Runs b3IsMinkowskiFace(U1, V1, -E1, -U2, -V2, -E2) (sat.cpp:60-71) on `count` queued
+// candidates (lane l takes entry head + l), hull 1's operands recomputed exactly as the
+// reference computes them, and queues the pairs that pass for projection.
+__device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const UHull& h2,
+                                                 CU32 ue2, SatStage& st, uint32_t c0, uint32_t chead,
+                                                 uint32_t count, uint32_t lane, uint32_t& head, uint32_t& tail,
+                                                 float& best, uint32_t& key) {
+    uint32_t e = 0;
+    bool pass = false;
+    if (lane < count) {
+        e = st.cands[(chead + lane) & (kCandCap - 1u)];
+        V3 A, B, nE;
+        edge1_operands(xf, h1, ue1, e >> 16, A, B, nE);
+        const float4* r = st.u.gauss + 4u * ((e & 0xFFFFu) - c0);
+        const float4 r0 = r[0], r1 = r[1];
+        const float r2x = r[2].x;
+        const V3 C{r0.x, r0.y, r0.z}, Dn{r0.w, r1.x, r1.y}, DxC{r1.z, r1.w, r2x};
+        const float CBA = dot(C, nE);
+        const float DBA = dot(Dn, nE);
+        const float ADC = dot(A, DxC);
+        const float BDC = dot(B, DxC);
+        pass = (CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f);
+    }
+    const uint64_t m = __ballot(pass);
+    if (pass) st.hits[(tail + mbcnt(m)) & (kHitCap - 1u)] = e;
+    tail += (uint32_t)__popcll(m);
+    if (tail - head >= 64u) {
+        __syncthreads();
+        drain_hits(xf, C1, h1, ue1, h2, ue2, st, c0, head, 64u, lane, best, key);
+        head += 64u;
+    }
+}
+
+// Edge query (sat.cpp:110-173).  Each lane holds the bounding caps of K hull-1 arcs (edges
+// lane + 64k of the current block, rotated into hull 2's frame) and tests them against
+// every staged hull-2 arc cap: angle(M1, M2) <= a1 + a2, i.e. M1.M2 >= cos a1 cos a2 -
+// sin a1 sin a2 (five fused operations instead of b3IsMinkowskiFace's 26).  The caps contain the
+// arcs with a margin, so a pair they reject cannot pass the exact test; the ~5 % whose
+// caps overlap go through an LDS ring and are tested 64 at a time, converged, with the
+// reference's operands and operation order, so the set of hits is unchanged.
 template <int K>
 __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h1, const UHull& h2, SatStage& st,
                                              uint32_t lane, float& best, uint32_t& key) {
     const uint32_t E1n = h1.HE >> 1;
     const HullLayout L1 = h1.layout(), L2 = h2.layout();
     const CU32 ue1 = (CU32)(h1.base + L1.uedgeOff()).p;
+    const CU32 ue2 = (CU32)(h2.base + L2.uedgeOff()).p;
+    const CF4 g1 = h1.base + L1.gaussOff();
     const CF4 g2 = h2.base + L2.gaussOff();
     const uint32_t E2p = L2.gaussEdges();  // padded with records that never pass
-    uint32_t head = 0, tail = 0;
+    uint32_t head = 0, tail = 0, chead = 0, ctail = 0;
     for (uint32_t i0 = 0; i0 < E1n; i0 += 64u * K) {
-        // Slot k: A = U1, B = V1, nE = -E1 in hull 2's frame (sat.cpp:123-146); lanes past
-        // the last edge hold zeros, which fail the test.
-        V3 A[K], B[K], nE[K];
+        // Slot k: the cap of the arc (U1, V1) of edge i0 + lane + 64k; lanes past the last
+        // edge hold a cap that overlaps nothing.
+        V3 M[K];
+        float cA[K], sA[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t i = i0 + lane + 64u * k;
-            const V3 zero{0.0f, 0.0f, 0.0f};
-            A[k] = B[k] = nE[k] = zero;
+            M[k] = V3{0.0f, 0.0f, 0.0f};
+            cA[k] = 1e30f;
+            sA[k] = 0.0f;
             if (i < E1n) {
-                const uint32_t u = ue1[i];
-                const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
-                const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
-                A[k] = mul(xf.R, ld3(h1.base[2 + h1.V + ((u >> 16) & 0xFFu)]));
-                B[k] = mul(xf.R, ld3(h1.base[2 + h1.V + (u >> 24)]));
-                nE[k] = -(Q1 - P1);
+                const float4 cap = g1[4u * i + 3u];
+                M[k] = mul(xf.R, V3{-cap.x, -cap.y, -cap.z});
+                cA[k] = cap.w;
+                sA[k] = g1[4u * i + 2u].y;
             }
         }
         for (uint32_t c0 = 0; c0 < E2p; c0 += kStageEdges) {
             const uint32_t cn = E2p - c0 < kStageEdges ? E2p - c0 : kStageEdges;  // a multiple of 4
             __syncthreads();
-            for (uint32_t u = lane; u < 3u * cn; u += 64u) st.u.gauss[u] = g2[3u * c0 + u];
+            for (uint32_t u = lane; u < 4u * cn; u += 64u) st.u.gauss[u] = g2[4u * c0 + u];
             __syncthreads();
             for (uint32_t j0 = 0; j0 < cn; j0 += 32u) {
-                uint32_t bits[K];
+                uint32_t cand[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) bits[k] = 0u;
+                for (int k = 0; k < K; ++k) cand[k] = 0u;
                 const uint32_t jn = cn - j0 < 32u ? cn - j0 : 32u;
+                // The filter is not part of the result (only the exact test below decides a
+                // hit), so it may use fused multiply-adds: its rounding is far inside the
+                // caps' margin.  Test jj + t lands in bit jn - 1 - (jj + t).
                 for (uint32_t jj = 0; jj < jn; jj += 4u) {
 #pragma unroll
                     for (uint32_t t = 0; t < 4u; ++t) {
-                        const float4* r = st.u.gauss + 3u * (j0 + jj + t);
-                        const float4 r0 = r[0], r1 = r[1];
-                        const float r2x = r[2].x;
-                        const V3 C{r0.x, r0.y, r0.z}, Dn{r0.w, r1.x, r1.y}, DxC{r1.z, r1.w, r2x};
-                        const uint32_t bit = 1u << (jj + t);
+                        const float4* r = st.u.gauss + 4u * (j0 + jj + t);
+                        const float4 cap = r[3];
+                        const float s2 = r[2].y;
 #pragma unroll
                         for (int k = 0; k < K; ++k) {
-                            // b3IsMinkowskiFace(U1, V1, -E1, -U2, -V2, -E2) (sat.cpp:60-71)
-                            const float CBA = dot(C, nE[k]);
-                            const float DBA = dot(Dn, nE[k]);
-                            const float ADC = dot(A[k], DxC);
-                            const float BDC = dot(B[k], DxC);
-                            if ((CBA * DBA < 0.0f) && (ADC * BDC < 0.0f) && (CBA * BDC > 0.0f)) bits[k] |= bit;
+                            float g = sA[k] * s2;
+                            g = __builtin_fmaf(-cA[k], cap.w, g);
+                            g = __builtin_fmaf(M[k].x, cap.x, g);
+                            g = __builtin_fmaf(M[k].y, cap.y, g);
+                            g = __builtin_fmaf(M[k].z, cap.z, g);
+                            cand[k] = cand[k] + cand[k] + (g >= 0.0f ? 1u : 0u);
                         }
                     }
                 }
-                // Queue the marks: each round appends one hit per lane that has any.
+                // Queue the candidates: each round appends one per lane that has any.
                 for (;;) {
                     uint32_t sel = (uint32_t)K;
 #pragma unroll
                     for (int k = K - 1; k >= 0; --k)
-                        if (bits[k]) sel = (uint32_t)k;
+                        if (cand[k]) sel = (uint32_t)k;
                     const uint64_t m = __ballot(sel < (uint32_t)K);
                     if (m == 0) break;
                     if (sel < (uint32_t)K) {
                         uint32_t w = 0;
 #pragma unroll
                         for (int k = 0; k < K; ++k)
-                            if ((uint32_t)k == sel) w = bits[k];
+                            if ((uint32_t)k == sel) w = cand[k];
 #pragma unroll
                         for (int k = 0; k < K; ++k)
-                            if ((uint32_t)k == sel) bits[k] = w & (w - 1u);
-                        const uint32_t e = ((i0 + lane + 64u * sel) << 16) | (c0 + j0 + (uint32_t)__builtin_ctz(w));
-                        st.hits[(tail + mbcnt(m)) & (kHitCap - 1u)] = e;
+                            if ((uint32_t)k == sel) cand[k] = w & (w - 1u);
+                        const uint32_t e = ((i0 + lane + 64u * sel) << 16) | (c0 + j0 + jn - 1u - (uint32_t)__builtin_ctz(w));
+                        st.cands[(ctail + mbcnt(m)) & (kCandCap - 1u)] = e;
                     }
-                    tail += (uint32_t)__popcll(m);
-                    if (tail - head >= 64u) {
+                    ctail += (uint32_t)__popcll(m);
+                    if (ctail - chead >= 64u) {
                         __syncthreads();
-                        drain_hits(xf, C1, h1, ue1, st, c0, head, 64u, lane, best, key);
-                        head += 64u;
+                        drain_candidates(xf, C1, h1, ue1, h2, ue2, st, c0, chead, 64u, lane, head, tail, best, key);
+                        chead += 64u;
                     }
                 }
             }
-            // Project what is queued before the staged chunk is replaced.
+            // Test and project what is queued before the staged chunk is replaced.
+            __syncthreads();
+            while (ctail != chead) {
+                const uint32_t cnt = ctail - chead < 64u ? ctail - chead : 64u;
+                drain_candidates(xf, C1, h1, ue1, h2, ue2, st, c0, chead, cnt, lane, head, tail, best, key);
+                chead += cnt;
+            }
             __syncthreads();
             while (tail != head) {
                 const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
-                drain_hits(xf, C1, h1, ue1, st, c0, head, cnt, lane, best, key);
+                drain_hits(xf, C1, h1, ue1, h2, ue2, st, c0, head, cnt, lane, best, key);
                 head += cnt;
             }
         }
