@@ -242,6 +242,8 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     cta.sat = sc.sat;
     cta.caches = caches;
     cta.out = out;
+    cta.list = sc.stage;  // the stage words are dead once the SAT kernel has read them
+    cta.listCount = sc.counter + 1;
     hipStream_t s = ctx->stream;
     if (ctx->roundShapes) {
         // Sphere/capsule pairs depend on nothing the hull kernels write.

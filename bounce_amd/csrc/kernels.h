@@ -106,7 +106,7 @@ struct SatSplit {
 };
 
 struct CacheArgs : PairInputs {
-    uint32_t* satCounter;    // the SAT kernel's tail counter, zeroed here
+    uint32_t* satCounter;    // the SAT kernel's tail counter and the overlap-list count, zeroed here
     const float* sweeps;     // [n][4][4] or null
     ConvexCache* caches;     // [n] or null
     uint32_t* stage;         // [n] out
@@ -127,10 +127,16 @@ struct SatTuning {
 };
 
 struct ContactArgs : PairInputs {
-    const SatResult* sat;    // [n]
+    SatResult* sat;          // [n]; the group kernel marks pairs it leaves with kStageFallback
     ConvexCache* caches;     // [n] or null
     Manifold* out;           // [n]
+    uint32_t* list;          // [n] the overlapping pairs (reuses the cache kernel's stage array)
+    uint32_t* listCount;     // zeroed by the cache kernel
 };
+
+// A pair whose clip polygon outgrows the group kernel's capacity: the lane-per-pair
+// contact kernel builds it instead.
+constexpr uint32_t kStageFallback = 6u;
 
 struct ConvexArgs : PairInputs {
     Manifold* out;           // [n]
@@ -187,6 +193,9 @@ hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // Persistent SAT over the pairs the cache kernel marked; any hull size within the record
 // limits.  Fills a.split (the counter must be zero).
 hipError_t launch_sat_persistent(SatArgs a, const SatTuning& t, hipStream_t s);
+// Contact building after the SAT: a lane-per-pair pass writes the separated pairs and lists
+// the overlapping ones; eight lanes per listed pair build the contact with the clip polygon
+// spread across them; the lane-per-pair kernel takes the pairs marked kStageFallback.
 hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s);
 
 }  // namespace b3amd

@@ -4,9 +4,14 @@
 // sat_persistent.hip:
 //   cache_hull_kernel    one lane per pair: b3FeatureCache read and the cached rebuilds
 //                        (collide_hulls_cache.cpp:279-344); marks pairs that need a SAT.
-//   contact_hull_kernel  one lane per pair: the decision tree of b3CollideCache
-//                        (collide_hulls_cache.cpp:155-275), edge/face contact, clipping,
-//                        reduction, keys, manifold and feature-cache write.
+//   contact_split_kernel one lane per pair: the separated exits (empty manifold, feature
+//                        cache) and a list of the overlapping pairs.
+//   contact_group_kernel eight lanes per listed pair: the decision tree of b3CollideCache
+//                        (collide_hulls_cache.cpp:155-275), edge/face contact, clipping
+//                        with the polygon spread across the lanes, reduction, keys,
+//                        manifold and feature-cache write.
+//   contact_hull_kernel  one lane per listed pair, the same for the pairs whose clip
+//                        polygon outgrows the group kernel (kStageFallback).
 // All arithmetic keeps the reference's FP32 operation order (b3_math.h); the build uses
 // -ffp-contract=off and IEEE division/square root.
 #include <hip/hip_runtime.h>
@@ -27,12 +32,15 @@ __device__ __forceinline__ void set_feature(FeatureCache* c, uint32_t state, uin
     c->index2 = i2;
 }
 
-// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275).
+// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275), one lane per pair, for
+// the pairs the group kernel marked kStageFallback (polygons beyond its capacity).
 __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= a.n) return;
-    const SatResult r = a.sat[p];
-    if (r.stage == kStageSkip) return;  // the cache kernel wrote this pair
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *a.listCount) return;
+    const uint32_t p = a.list[k];
+    SatResult r = a.sat[p];
+    if (r.stage != kStageFallback) return;  // written by the group kernel
+    r.stage = kStageOverlap;
     const uint32_t slot = a.slot_of(p);
     Manifold* const out = a.out + slot;
     FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
@@ -104,6 +112,396 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
     build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
     set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
     write_manifold(out, m);
+}
+
+// ------------------------------------------------------------------ group contact kernel
+//
+// Eight lanes per pair.  Every lane of a group walks the same decision tree; in a face
+// contact (collide_hulls.cpp:93-204) lane g holds clip-polygon vertex g.  One clip plane
+// (clip.cpp:99-149): each lane takes the previous vertex from its neighbour, emits 0, 1 or
+// 2 vertices exactly as the sequential loop does for that vertex, and the outputs are
+// compacted in vertex order with two ballots through a small per-group LDS buffer, so the
+// polygon is the sequential one.  Reductions over the group keep the first maximum
+// (strict comparisons in index order).  A polygon longer than kGroupPoly vertices sends
+// the pair to the lane-per-pair kernel (kStageFallback).
+
+constexpr uint32_t kGroup = 8;
+constexpr uint32_t kGroupPoly = 8;   // polygon capacity of the group path
+constexpr uint32_t kGroupBuf = 16;   // a clip step emits at most two vertices per input
+
+struct GroupLds {
+    ClipV poly[kGroupBuf];  // the compacted output of the last clip step
+    CPV kept[kGroupPoly];   // vertices within totalRadius, projected (b3ClusterPolygonVertex)
+    CPV quad[4];            // the reduced polygon
+};
+
+__device__ __forceinline__ void group_sync() {
+    // The group lives in one wave: order its LDS writes before the reads of other lanes.
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// The group's bits of a wave ballot.
+__device__ __forceinline__ uint32_t group_bits(bool pred, uint32_t gbase) {
+    return (uint32_t)((__ballot(pred) >> gbase) & 0xFFull);
+}
+
+__device__ __forceinline__ ClipV shfl_clipv(const ClipV& v, uint32_t src) {
+    ClipV o;
+    o.position = V3{__shfl(v.position.x, (int)src, (int)kGroup), __shfl(v.position.y, (int)src, (int)kGroup),
+                    __shfl(v.position.z, (int)src, (int)kGroup)};
+    o.inEdge1 = __shfl(v.inEdge1, (int)src, (int)kGroup);
+    o.outEdge1 = __shfl(v.outEdge1, (int)src, (int)kGroup);
+    o.inEdge2 = __shfl(v.inEdge2, (int)src, (int)kGroup);
+    o.outEdge2 = __shfl(v.outEdge2, (int)src, (int)kGroup);
+    return o;
+}
+
+// sort_polygon (contact_cluster.cpp:33-56) on LDS, one lane.
+__device__ inline void sort_quad(CPV* q, uint32_t count, V3 N) {
+    V3 A = q[0].position;
+    for (uint32_t i = 1; i < count; ++i) {
+        V3 B = q[i].position;
+        for (uint32_t j = i + 1; j < count; ++j) {
+            V3 C = q[j].position;
+            if (!is_ccw(A, B, C, N)) {
+                CPV t = q[i];
+                q[i] = q[j];
+                q[j] = t;
+            }
+        }
+    }
+}
+
+// reduce_polygon (contact_cluster.cpp:58-200) from LDS to LDS, one lane; returns the count.
+__device__ inline uint32_t reduce_quad(const CPV* in, uint32_t n, V3 N, uint32_t initial, CPV* out) {
+    if (n <= 4) {
+        for (uint32_t i = 0; i < n; ++i) out[i] = in[i];
+        sort_quad(out, n, N);
+        return n;
+    }
+    uint32_t count = 0, chosen = 0;
+    out[count++] = in[initial];
+    chosen |= 1u << initial;
+    {
+        V3 A = out[0].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 d = in[i].position - A;
+            float dd = dot(d, d);
+            if (dd > mx) {
+                mx = dd;
+                index = i;
+            }
+        }
+        if (mx < kEpsilon * kEpsilon) return count;
+        out[count++] = in[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out[0].position, B = out[1].position;
+        uint32_t index = 0;
+        float mx = -kMaxFloat;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 C = in[i].position;
+            V3 nn = cross(B - A, C - A);
+            float a2 = fabsf(dot(nn, N));
+            if (a2 > mx) {
+                mx = a2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (mx < 2.0f * kAreaTol) return count;
+        V3 C = in[index].position;
+        if (!is_ccw(A, B, C, N)) {
+            CPV t = out[0];
+            out[0] = out[1];
+            out[1] = t;
+        }
+        out[count++] = in[index];
+        chosen |= 1u << index;
+    }
+    {
+        V3 A = out[0].position, B = out[1].position;
+        uint32_t index = 0;
+        float mn = kMaxFloat;
+        for (uint32_t i = 0; i < n; ++i) {
+            if (chosen & (1u << i)) continue;
+            V3 D = in[i].position;
+            V3 nn = cross(B - A, D - A);
+            float sa2 = dot(nn, N);
+            if (sa2 < mn) {
+                mn = sa2;
+                index = i;
+            }
+        }
+        const float kAreaTol = 0.01f;
+        if (fabsf(mn) < 2.0f * kAreaTol) return count;
+        out[count++] = in[index];
+        chosen |= 1u << index;
+    }
+    sort_quad(out, count, N);
+    return count;
+}
+
+// b3BuildFaceContact (collide_hulls.cpp:93-204) by the group; returns the point count, or
+// -1 if a polygon outgrew the group's capacity.  With points it writes the manifold
+// (lanes 0-3 one point each, lane 0 the header).
+__device__ int group_face_contact(GroupLds& L, uint32_t g, uint32_t gbase, Manifold* out, const Xf& xf1,
+                                  uint32_t index1, const DevHull& h1, float r1, const Xf& xf2, const DevHull& h2,
+                                  float r2, bool flip) {
+    const float totalRadius = r1 + r2;
+    const Plane plane1 = mul(xf1, h1.plane(index1));
+    const V3 normal1 = mulT(xf2.R, plane1.n);
+    // Support face of hull 2 along -normal1 (hull.inl:47-61): strided first maximum.
+    const V3 dn = -normal1;
+    float bv = -__builtin_inff();
+    uint32_t bi = 0xFFFFFFFFu;
+    for (uint32_t i = g; i < h2.F; i += kGroup) {
+        const float q = dot(dn, h2.plane(i).n);
+        if (q > bv) {
+            bv = q;
+            bi = i;
+        }
+    }
+#pragma unroll
+    for (uint32_t off = 1; off < kGroup; off <<= 1) {
+        const float ov = __shfl_xor(bv, (int)off, (int)kGroup);
+        const uint32_t oi = __shfl_xor(bi, (int)off, (int)kGroup);
+        if (ov > bv || (ov == bv && oi < bi)) {
+            bv = ov;
+            bi = oi;
+        }
+    }
+    const uint32_t index2 = bi;
+    // Incident polygon (clip.cpp:33-57): lane g takes the face's g-th half-edge.
+    const uint32_t begin2 = h2.face_edge(index2);
+    uint32_t e = begin2, deg = 0, mine = begin2;
+    do {
+        if (deg == g) mine = e;
+        ++deg;
+        e = h2.next(e);
+    } while (e != begin2 && deg <= kGroupPoly);
+    if (e != begin2) return -1;
+    uint32_t count = deg;
+    ClipV v = clipv(mul(xf2, h2.vertex(h2.origin(mine))), h2.next(mine ^ 1u), kNullEdge, mine, kNullEdge);
+    // Clip against the side planes of the reference face (clip.cpp:243-280).
+    const uint32_t begin1 = h1.face_edge(index1);
+    uint32_t e1 = begin1, guard = 0;
+    do {
+        Plane pl = edge_side_plane(h1, e1);
+        pl.d += totalRadius;
+        const Plane wp = mul(xf1, pl);
+        const bool valid = g < count;
+        const float d2 = valid ? distance(v.position, wp) : 0.0f;
+        const uint32_t src = g == 0 ? count - 1u : g - 1u;
+        const V3 p1 = V3{__shfl(v.position.x, (int)src, (int)kGroup), __shfl(v.position.y, (int)src, (int)kGroup),
+                         __shfl(v.position.z, (int)src, (int)kGroup)};
+        const float d1 = __shfl(d2, (int)src, (int)kGroup);
+        const uint32_t in1 = __shfl(v.inEdge1, (int)src, (int)kGroup);
+        uint32_t nout = 0;
+        ClipV o0 = v, o1 = v;
+        if (valid) {
+            if (d1 <= 0.0f && d2 <= 0.0f) {
+                nout = 1;
+            } else if (d1 <= 0.0f && d2 > 0.0f) {
+                const float f = d1 / (d1 - d2);
+                o0 = clipv(p1 + f * (v.position - p1), in1, kNullEdge, kNullEdge, e1);
+                nout = 1;
+            } else if (d1 > 0.0f && d2 <= 0.0f) {
+                const float f = d1 / (d1 - d2);
+                o0 = clipv(p1 + f * (v.position - p1), in1, e1, kNullEdge, kNullEdge);
+                nout = 2;
+            }
+        }
+        const uint32_t b1 = group_bits(nout >= 1u, gbase), b2 = group_bits(nout == 2u, gbase);
+        const uint32_t below = (1u << g) - 1u;
+        const uint32_t at = (uint32_t)(__builtin_popcount(b1 & below) + __builtin_popcount(b2 & below));
+        count = (uint32_t)(__builtin_popcount(b1) + __builtin_popcount(b2));
+        if (nout >= 1u) L.poly[at] = o0;
+        if (nout == 2u) L.poly[at + 1u] = o1;
+        group_sync();
+        if (count > kGroupPoly) return -1;
+        if (g < count) v = L.poly[g];
+        group_sync();
+        if (count == 0) break;
+        e1 = h1.next(e1);
+    } while (e1 != begin1 && ++guard < 256);
+    if (count == 0) return 0;
+    // Points within totalRadius, projected onto the reference plane; the deepest first.
+    const bool valid = g < count;
+    const float sep = valid ? distance(v.position, plane1) : 0.0f;
+    const bool keep = valid && sep <= totalRadius;
+    const uint32_t kb = group_bits(keep, gbase);
+    const uint32_t nk = (uint32_t)__builtin_popcount(kb);
+    if (nk == 0) return 0;
+    const uint32_t ki = (uint32_t)__builtin_popcount(kb & ((1u << g) - 1u));
+    if (keep) {
+        L.kept[ki].position = closest_on_plane(v.position, plane1);
+        L.kept[ki].clipIndex = g;
+    }
+    float ms = keep ? sep : __builtin_inff();
+    uint32_t mi = keep ? ki : 0xFFFFFFFFu;
+#pragma unroll
+    for (uint32_t off = 1; off < kGroup; off <<= 1) {
+        const float os = __shfl_xor(ms, (int)off, (int)kGroup);
+        const uint32_t oi = __shfl_xor(mi, (int)off, (int)kGroup);
+        if (os < ms || (os == ms && oi < mi)) {
+            ms = os;
+            mi = oi;
+        }
+    }
+    group_sync();
+    const V3 normal = plane1.n;
+    const V3 s_normal = flip ? -normal : normal;
+    uint32_t nq = 0;
+    if (g == 0) nq = reduce_quad(L.kept, nk, s_normal, mi, L.quad);
+    nq = __shfl(nq, 0, (int)kGroup);
+    group_sync();
+    // Manifold: lane i < 4 writes point i (collide_hulls.cpp:170-199).
+    const uint32_t ci = g < nq ? L.quad[g].clipIndex : 0u;
+    const ClipV v2 = shfl_clipv(v, ci);
+    if (g < 4u) {
+        ManifoldPoint q;
+        if (g < nq) {
+            const V3 v1 = closest_on_plane(v2.position, plane1);
+            if (flip) {
+                q.localNormal1 = mulT(xf2.R, s_normal);
+                q.localPoint1 = mulT(xf2, v2.position);
+                q.localPoint2 = mulT(xf1, v1);
+                q.key[0] = v2.inEdge2;
+                q.key[1] = v2.outEdge2;
+                q.key[2] = v2.inEdge1;
+                q.key[3] = v2.outEdge1;
+            } else {
+                q.localNormal1 = mulT(xf1.R, normal);
+                q.localPoint1 = mulT(xf1, v1);
+                q.localPoint2 = mulT(xf2, v2.position);
+                q.key[0] = v2.inEdge1;
+                q.key[1] = v2.outEdge1;
+                q.key[2] = v2.inEdge2;
+                q.key[3] = v2.outEdge2;
+            }
+            q.triangleKey = kNullTriangle;
+        } else {
+            q.localNormal1 = V3{0.0f, 0.0f, 0.0f};
+            q.localPoint1 = V3{0.0f, 0.0f, 0.0f};
+            q.localPoint2 = V3{0.0f, 0.0f, 0.0f};
+            q.triangleKey = 0;
+            q.key[0] = q.key[1] = q.key[2] = q.key[3] = 0;
+        }
+        if (nq > 0) out->points[g] = q;
+    }
+    if (g == 0 && nq > 0) {
+        out->pointCount = nq;
+        out->flags = 0u;
+        out->reserved[0] = 0u;
+        out->reserved[1] = 0u;
+    }
+    return (int)nq;
+}
+
+// The separated exits of b3CollideCache (collide_hulls_cache.cpp:155-275) for every pair the
+// SAT ran on, one lane per pair; the overlapping pairs are appended to a list (one atomic
+// per block) so that the group kernel's waves hold only pairs that build a contact.
+__global__ __launch_bounds__(256) void contact_split_kernel(ContactArgs a) {
+    __shared__ uint32_t waveCount[4], waveBase[4];
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t wave = threadIdx.x >> 6;
+    bool overlap = false;
+    if (p < a.n) {
+        const uint32_t slot = a.slot_of(p);
+        Manifold* const out = a.out + slot;
+        FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+        const SatResult r = a.sat[p];
+        overlap = r.stage == kStageOverlap;
+        if (r.stage != kStageSkip && !overlap) {
+            if (r.stage == kStageSepFace1) set_feature(fc, kCacheSeparation, kFeatureFace1, r.f1Index, r.f1Index);
+            else if (r.stage == kStageSepFace2) set_feature(fc, kCacheSeparation, kFeatureFace2, r.f2Index, r.f2Index);
+            else set_feature(fc, kCacheSeparation, kFeatureEdge1, r.eIndex1, r.eIndex2);
+            Contact none;
+            none.pointCount = 0;
+            none.overflow = false;
+            write_manifold(out, none);
+        }
+    }
+    const uint64_t m = __ballot(overlap);
+    if ((threadIdx.x & 63u) == 0) waveCount[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = waveCount[0] + waveCount[1] + waveCount[2] + waveCount[3];
+        uint32_t base = total ? atomicAdd(a.listCount, total) : 0u;
+        for (uint32_t w = 0; w < 4u; ++w) {
+            waveBase[w] = base;
+            base += waveCount[w];
+        }
+    }
+    __syncthreads();
+    if (overlap) a.list[waveBase[wave] + mbcnt(m)] = p;
+}
+
+// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275) for the listed pairs,
+// eight lanes per pair.  Edge contacts are built by every lane and written by lane 0.
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void contact_group_kernel(ContactArgs a) {
+    __shared__ GroupLds lds[256 / kGroup];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t k = t / kGroup, g = threadIdx.x & (kGroup - 1u);
+    const uint32_t gbase = threadIdx.x & 63u & ~(kGroup - 1u);
+    GroupLds& L = lds[threadIdx.x / kGroup];
+    if (k >= *a.listCount) return;  // whole groups
+    const uint32_t p = a.list[k];
+    const SatResult r = a.sat[p];
+    const uint32_t slot = a.slot_of(p);
+    Manifold* const out = a.out + slot;
+    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+    SatResult* const mark = a.sat + p;
+    const bool lead = g == 0;
+    const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
+    const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
+    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
+    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
+    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+
+    const float kTol = 0.1f * kLinearSlop;
+    const float fmax12 = r.f1Sep > r.f2Sep ? r.f1Sep : r.f2Sep;  // b3Max
+    // Which builder each step of the tree uses: 0 edge, 1 face of hull 1, 2 face of hull 2.
+    const uint32_t first = r.eSep > fmax12 + kTol ? 0u : (r.f1Sep + kTol > r.f2Sep ? 1u : 2u);
+    const uint32_t second = r.eSep > fmax12 ? 0u : (r.f1Sep > r.f2Sep ? 1u : 2u);
+    for (uint32_t step = 0; step < 3u; ++step) {
+        const uint32_t kind = step == 0u ? first : (step == 1u ? second : 0u);
+        int n = 0;
+        if (kind == 0u) {
+            Contact m;
+            m.pointCount = 0;
+            m.overflow = false;
+            build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
+            n = (int)m.pointCount;
+            if ((n > 0 || step == 2u) && lead) write_manifold(out, m);
+        } else {
+            // One call site for both reference faces: hull 1's face, or hull 2's flipped.
+            const bool flip = kind == 2u;
+            n = group_face_contact(L, g, gbase, out, flip ? xf2 : xf1, flip ? r.f2Index : r.f1Index, flip ? h2 : h1,
+                                   flip ? r2 : r1, flip ? xf1 : xf2, flip ? h1 : h2, flip ? r1 : r2, flip);
+        }
+        if (n < 0) {  // beyond the group's polygon capacity
+            if (lead) mark->stage = kStageFallback;
+            return;
+        }
+        if (n > 0 || step == 2u) {
+            if (lead) {
+                if (kind == 0u) set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
+                else if (kind == 1u) set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
+                else set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
+            }
+            return;
+        }
+    }
 }
 
 // ------------------------------------------------------------------ feature-cache kernel
@@ -189,7 +587,7 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
 // the cache resolved (collide_hulls.cpp:304, collide_hulls_cache.cpp:300,334).
 __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) *a.satCounter = 0u;  // the SAT kernel runs after this one on the stream
+    if (p == 0) a.satCounter[0] = a.satCounter[1] = 0u;  // the SAT and contact kernels run after this one
     uint32_t call = 0, hit = 0;
     if (p < a.n) {
         const uint32_t slot = a.slot_of(p);
@@ -220,6 +618,13 @@ hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
 
 hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
+    contact_split_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // Grids cover the n-pair bound: the list length stays on the device.
+    const uint64_t threads = (uint64_t)a.n * kGroup;
+    contact_group_kernel<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
     contact_hull_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
     return hipGetLastError();
 }
