@@ -216,3 +216,39 @@ def test_gpu_graph_replay_matches_eager(ctx, lib):
     finally:
         for b in (dp, dx, do, dc):
             b.free()
+
+
+def prism(n, r, h):
+    """Right prism over a regular n-gon (radius r, half-height h): two n-gon caps."""
+    from bounce_amd.shapes import polyhedron
+    a = 2 * np.pi * np.arange(n) / n
+    ring = np.stack([r * np.cos(a), np.zeros(n), r * np.sin(a)], 1)
+    v = np.concatenate([ring + (0, -h, 0), ring + (0, h, 0)])
+    faces = [list(range(n)), list(range(2 * n - 1, n - 1, -1))]  # outward: -y, +y
+    faces += [[i, n + i, n + (i + 1) % n, (i + 1) % n] for i in range(n)]
+    return polyhedron(v, faces)
+
+
+def test_gpu_large_faces_fallback(ctx, oracle):
+    """Prisms stacked cap to cap: hexagons rotated against each other clip to polygons of up
+    to 12 vertices and 12-gon caps start above the group kernel's 8, so these contacts take
+    the lane-per-pair path; everything still equals the oracle bit for bit."""
+    say("case: large faces (prism caps)")
+    rng = np.random.default_rng(17)
+    hulls = [prism(n, r, 0.3) for n, r in ((6, 0.6), (8, 0.5), (12, 0.7), (6, 0.45))]
+    pool = HullPool.concat(hulls)
+    n = 400
+    pairs = rng.integers(0, pool.count, (n, 2)).astype(np.uint32)
+    xfs = np.zeros((n, 2, 12), np.float32)
+    for i in range(n):
+        yaw = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(yaw), np.sin(yaw)
+        R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+        tilt = rng.normal(0, 0.002, 3)
+        R = R @ np.array([[1, -tilt[2], tilt[1]], [tilt[2], 1, -tilt[0]], [-tilt[1], tilt[0], 1]])
+        xfs[i, 0] = identity_xf((0, 0, 0))
+        xfs[i, 1] = identity_xf((rng.normal(0, 0.1), 0.6 - rng.uniform(0.0, 0.02), rng.normal(0, 0.1)), R)
+    want = oracle.collide(pool, pool.shapes(), pairs, xfs)
+    assert (want["pointCount"] > 0).mean() > 0.9
+    got = run(ctx, pool, pool.shapes(), pairs, xfs)
+    assert_manifolds_equal(got, want, "prism caps")
