@@ -59,6 +59,11 @@ def load_library():
         "b3amd_mesh_tree": (i32, [vp, u32, ctypes.POINTER(u32), vp]),
         "b3amd_collide_mesh": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
         "b3amd_collide_mesh_device": (i32, [vp, u32, u32, vp, vp, vp, vp, vp, vp]),
+        "b3amd_mesh_contacts_create": (i32, [vp, u32, u32, vp, vp, vp]),
+        "b3amd_mesh_contacts_destroy": (i32, [vp]),
+        "b3amd_mesh_contacts_collide": (i32, [vp, vp, vp, vp, vp]),
+        "b3amd_mesh_contacts_synchronize": (i32, [vp, vp, vp]),
+        "b3amd_mesh_contacts_read": (i32, [vp, vp, vp, vp, vp]),
         "b3amd_set_sat_tuning": (i32, [vp, u32, u32]),
         "b3amd_get_sat_tuning": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "b3amd_autotune": (i32, [vp, u32, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]),
@@ -218,6 +223,57 @@ class Server:
         st = ServerStats()
         _check(self.lib.b3amd_server_get_stats(self.handle, ctypes.byref(st)))
         return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
+
+
+class MeshContacts:
+    """Persistent mesh contacts (b3amd_mesh_contacts_*): b3MeshContact across steps."""
+
+    def __init__(self, ctx, mesh_id, shape_a, xfs):
+        shape_a = np.ascontiguousarray(shape_a, np.uint32)
+        self.n = len(shape_a)
+        self.lib = ctx.lib
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_mesh_contacts_create(ctx.handle, mesh_id, self.n, ptr(shape_a), ptr(xfs), ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle:
+            self.lib.b3amd_mesh_contacts_destroy(self.handle)
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def collide(self, xfs, sweeps=None):
+        """b3MeshContact::Collide: (manifolds [n][3], manifold counts)."""
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
+        if sweeps is not None:
+            sweeps = np.ascontiguousarray(sweeps, np.float32).reshape(self.n, 4, 4)
+        out = np.zeros((self.n, 3), MANIFOLD_DTYPE)
+        counts = np.zeros(self.n, np.uint32)
+        _check(self.lib.b3amd_mesh_contacts_collide(self.handle, ptr(xfs), ptr(sweeps), ptr(out), ptr(counts)))
+        return out, counts
+
+    def synchronize(self, xfs, centers0):
+        """SynchronizeShapes + FindNewPairs for new transforms (centers0: body A's worldCenter0)."""
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
+        c0 = np.ascontiguousarray(centers0, np.float32).reshape(self.n, 3)
+        _check(self.lib.b3amd_mesh_contacts_synchronize(self.handle, ptr(xfs), ptr(c0)))
+
+    def read(self):
+        """(triangle counts, lists [n][64], caches [n][64], fat AABBs [n][6])."""
+        from .types import CACHE_DTYPE
+        cap = Context.MESH_TRIANGLE_CAP
+        counts = np.zeros(self.n, np.uint32)
+        lists = np.zeros((self.n, cap), np.uint32)
+        caches = np.zeros((self.n, cap), CACHE_DTYPE)
+        fat = np.zeros((self.n, 6), np.float32)
+        _check(self.lib.b3amd_mesh_contacts_read(self.handle, ptr(counts), ptr(lists), ptr(caches), ptr(fat)))
+        return counts, lists, caches, fat
 
 
 class Context:

@@ -556,13 +556,16 @@ struct MeshScratch {
     Manifold* out;
     uint32_t* manifoldCounts;
     Manifold* temp;
+    float* sweeps;   // staged per-contact sweeps (persistent contacts)
+    float* centers;  // staged body-A centres (persistent contacts)
 };
 
 int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
     const uint64_t cap = kMeshTriangleCap;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    const uint64_t sizes[11] = {4ull * n, 96ull * n, 4ull * n, 4ull * n * cap, 4ull * n, 4ull * n, 256,
-                                12ull * n * cap, 720ull * n, 4ull * n, sizeof(Manifold) * n * cap};
+    const uint64_t sizes[13] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
+                                4ull * n,  256,       12ull * n * cap, 720ull * n, 4ull * n,
+                                sizeof(Manifold) * n * cap, 64ull * n, 12ull * n};
     uint64_t total = 0;
     for (uint64_t b : sizes) total += al(b);
     if (total > ctx->meshScratchCap) {
@@ -574,14 +577,15 @@ int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
         ctx->meshScratchCap = total;
     }
     char* p = static_cast<char*>(ctx->dMeshScratch);
-    void* parts[11];
-    for (int i = 0; i < 11; ++i) {
+    void* parts[13];
+    for (int i = 0; i < 13; ++i) {
         parts[i] = p;
         p += al(sizes[i]);
     }
-    *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2], (uint32_t*)parts[3],
+    *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2],  (uint32_t*)parts[3],
                      (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint32_t*)parts[7],
-                     (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10]};
+                     (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10], (float*)parts[11],
+                     (float*)parts[12]};
     return B3AMD_OK;
 }
 
@@ -604,6 +608,10 @@ int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t
     a.out = out;
     a.manifoldCounts = manifoldCounts;
     a.temp = sc.temp;
+    a.triCaches = nullptr;
+    a.sweeps = nullptr;
+    a.fat = nullptr;
+    a.centers0 = nullptr;
     a.shapes = ctx->dShapes;
     a.shapeCount = ctx->shapeCount;
     a.hullCount = ctx->hullCount;
@@ -673,6 +681,162 @@ int b3amd_collide_mesh_device(b3amd_context* ctx, uint32_t meshId, uint32_t n, c
     if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
     return enqueue_mesh(ctx, meshId, n, d_shapeA, reinterpret_cast<const float*>(d_xfs), d_triangleCounts,
                         d_triangleLists, reinterpret_cast<Manifold*>(d_out), d_manifoldCounts, sc);
+}
+
+}  // extern "C"
+
+struct b3amd_mesh_contacts {
+    b3amd_context* ctx = nullptr;
+    uint32_t meshId = 0, n = 0;
+    uint32_t* dShapeA = nullptr;
+    float* dFat = nullptr;            // [n][6]
+    uint32_t* dTriCounts = nullptr;   // [n]
+    uint32_t* dTriLists = nullptr;    // [n][cap]
+    uint32_t* dFlags = nullptr;       // [n]
+    ConvexCache* dCaches = nullptr;   // [n][cap]
+};
+
+namespace {
+
+// MeshArgs over a persistent contact set and the per-call scratch.
+MeshArgs mesh_set_args(b3amd_mesh_contacts* set, const MeshScratch& sc) {
+    b3amd_context* ctx = set->ctx;
+    MeshArgs a;
+    a.counters = ctx->counting ? ctx->dCounters : nullptr;
+    a.n = set->n;
+    a.shapeA = set->dShapeA;
+    a.shapes = ctx->dShapes;
+    a.shapeCount = ctx->shapeCount;
+    a.hullCount = ctx->hullCount;
+    a.xfs = sc.xfs;
+    a.pool = ctx->dPool;
+    a.offsets = ctx->dOffsets;
+    const auto& mr = ctx->meshes[set->meshId];
+    a.mesh = MeshDev{mr.verts, mr.tris, reinterpret_cast<const uint32_t*>(mr.nodes), mr.nodeCount};
+    a.cap = kMeshTriangleCap;
+    a.triCounts = set->dTriCounts;
+    a.triLists = set->dTriLists;
+    a.contactFlags = set->dFlags;
+    a.contactOffset = sc.offsets;
+    a.totals = sc.totals;
+    a.pairContact = sc.pairContact;
+    a.temp = sc.temp;
+    a.out = sc.out;
+    a.manifoldCounts = sc.manifoldCounts;
+    a.triCaches = set->dCaches;
+    a.sweeps = nullptr;
+    a.fat = set->dFat;
+    a.centers0 = nullptr;
+    return a;
+}
+
+}  // namespace
+
+extern "C" {
+
+int b3amd_mesh_contacts_create(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
+                               const b3amd_transform* xfs, b3amd_mesh_contacts** out) {
+    if (!ctx || !out || (n && (!shapeA || !xfs))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    *out = nullptr;
+    int rc;
+    if ((rc = check_mesh_call(ctx, meshId, n))) return rc;
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    for (uint32_t i = 0; i < n; ++i) {
+        if (shapeA[i] >= ctx->shapeCount) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape %u of %u", i, shapeA[i], ctx->shapeCount);
+        if (ctx->shapes[shapeA[i]].type > B3AMD_HULL) return fail(B3AMD_ERR_ARGUMENT, "contact %u: shape A must be convex", i);
+    }
+    b3amd_mesh_contacts* set = new b3amd_mesh_contacts();
+    set->ctx = ctx;
+    set->meshId = meshId;
+    set->n = n;
+    const uint64_t cap = kMeshTriangleCap, m = n ? n : 1;
+    if ((rc = ctx->alloc(4 * m, (void**)&set->dShapeA)) || (rc = ctx->alloc(24 * m, (void**)&set->dFat)) ||
+        (rc = ctx->alloc(4 * m, (void**)&set->dTriCounts)) || (rc = ctx->alloc(4 * m * cap, (void**)&set->dTriLists)) ||
+        (rc = ctx->alloc(4 * m, (void**)&set->dFlags)) ||
+        (rc = ctx->alloc(sizeof(ConvexCache) * m * cap, (void**)&set->dCaches))) {
+        const std::string msg = b3amd_last_error();
+        b3amd_mesh_contacts_destroy(set);
+        return fail(rc, "%s", msg.c_str());
+    }
+    *out = set;
+    if (n == 0) return B3AMD_OK;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(set->dShapeA, shapeA, 4ull * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
+    const MeshArgs a = mesh_set_args(set, sc);
+    if ((rc = timed(ctx, s, "mesh_sync", [&] { return launch_mesh_sync(a, kMeshCreate, s); }))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_destroy(b3amd_mesh_contacts* set) {
+    if (!set) return B3AMD_OK;
+    b3amd_context* ctx = set->ctx;
+    hipStreamSynchronize(ctx->stream);
+    for (void* p : {(void*)set->dShapeA, (void*)set->dFat, (void*)set->dTriCounts, (void*)set->dTriLists,
+                    (void*)set->dFlags, (void*)set->dCaches})
+        ctx->release(p);
+    delete set;
+    return B3AMD_OK;
+}
+
+int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* set, const b3amd_transform* xfs, const b3amd_quat* sweeps,
+                                b3amd_manifold* out, uint32_t* manifoldCounts) {
+    if (!set || (set->n && (!xfs || !out || !manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    b3amd_context* ctx = set->ctx;
+    const uint32_t n = set->n;
+    if (n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
+    if (sweeps) HIP_TRY(hipMemcpyAsync(sc.sweeps, sweeps, 64ull * n, hipMemcpyHostToDevice, s));
+    MeshArgs a = mesh_set_args(set, sc);
+    a.sweeps = sweeps ? sc.sweeps : nullptr;
+    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 4, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_list", [&] { return launch_mesh_list(a, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); }))) return rc;
+    hipError_t e = hipMemcpyAsync(out, sc.out, 720ull * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, sc.manifoldCounts, 4ull * n, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "mesh readback: %s", hipGetErrorString(e));
+    return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* set, const b3amd_transform* xfs, const float* centers0) {
+    if (!set || (set->n && (!xfs || !centers0))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    b3amd_context* ctx = set->ctx;
+    const uint32_t n = set->n;
+    if (n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(sc.centers, centers0, 12ull * n, hipMemcpyHostToDevice, s));
+    MeshArgs a = mesh_set_args(set, sc);
+    a.centers0 = sc.centers;
+    if ((rc = timed(ctx, s, "mesh_sync", [&] { return launch_mesh_sync(a, kMeshSync, s); }))) return rc;
+    HIP_TRY(hipStreamSynchronize(s));
+    return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_read(b3amd_mesh_contacts* set, uint32_t* triangleCounts, uint32_t* triangleLists,
+                             b3amd_convex_cache* caches, float* fatAabbs) {
+    if (!set) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    const uint64_t n = set->n, cap = kMeshTriangleCap;
+    if (n == 0) return B3AMD_OK;
+    hipStream_t s = set->ctx->stream;
+    if (triangleCounts) HIP_TRY(hipMemcpyAsync(triangleCounts, set->dTriCounts, 4 * n, hipMemcpyDeviceToHost, s));
+    if (triangleLists) HIP_TRY(hipMemcpyAsync(triangleLists, set->dTriLists, 4 * n * cap, hipMemcpyDeviceToHost, s));
+    if (caches) HIP_TRY(hipMemcpyAsync(caches, set->dCaches, sizeof(ConvexCache) * n * cap, hipMemcpyDeviceToHost, s));
+    if (fatAabbs) HIP_TRY(hipMemcpyAsync(fatAabbs, set->dFat, 24 * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return B3AMD_OK;
 }
 
 int b3amd_synchronize(b3amd_context* ctx) {

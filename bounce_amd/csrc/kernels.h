@@ -180,7 +180,21 @@ struct MeshArgs {
     Manifold* temp;             // [n * cap] per-triangle manifolds, segments back to back
     Manifold* out;              // [n][3] clustered manifolds
     uint32_t* manifoldCounts;   // [n]
+    // Persistent contacts (b3MeshContact across steps); null for freshly created ones.
+    ConvexCache* triCaches;     // [n][cap] b3TriangleCache::cache of each listed triangle
+    const float* sweeps;        // [n][4][4] orientation0/orientation of body A, then the mesh body
+    float* fat;                 // [n][6] m_aabbA in the mesh frame (lower xyz, upper xyz)
+    const float* centers0;      // [n][3] body A's sweep worldCenter0 (the displacement's origin)
 };
+
+// Mesh-contact state update: kMeshCreate is the b3MeshContact constructor followed by
+// FindNewPairs (mesh_contact.cpp:30-55, 141-160); kMeshSync is SynchronizeShapes, MoveAABB
+// and FindNewPairs (:62-160), which query the tree again only when the shape left its fat
+// AABB and then reset the triangle caches.
+enum : uint32_t { kMeshCreate = 0, kMeshSync = 1 };
+hipError_t launch_mesh_sync(const MeshArgs& a, uint32_t mode, hipStream_t s);
+// Work items of the listed triangles (as the fresh midphase builds them).
+hipError_t launch_mesh_list(const MeshArgs& a, hipStream_t s);
 
 hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
 // The grid covers the n * cap bound: the work-item count stays on the device.

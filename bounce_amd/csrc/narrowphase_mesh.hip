@@ -74,14 +74,64 @@ constexpr uint32_t kTreeStack = 128;
 // Work-item segment of a convex shape type: hulls, capsules, spheres.
 __device__ __forceinline__ uint32_t segment_of(uint32_t type) { return 2u - type; }
 
+// Whether contact c's shape A is a convex shape the kernels may read.
+__device__ __forceinline__ bool mesh_pair_valid(const MeshArgs& a, uint32_t si) {
+    return si < a.shapeCount && a.shapes[si].type <= 2u && (a.shapes[si].type != 2u || a.shapes[si].hull < a.hullCount);
+}
+
+// b3StaticTree::QueryAABB (static_tree.h:100-142) reporting into contact c's triangle list
+// (b3MeshContact::Report, mesh_contact.cpp:162-189): DFS, child2 first, every overlapping
+// leaf in order.  Returns the number of triangles found (the list keeps the first cap).
+__device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, uint32_t& flags) {
+    uint32_t found = 0;
+    uint32_t stack[kTreeStack];
+    uint32_t top = 0;
+    if (a.mesh.nodeCount > 0) stack[top++] = 0;
+    uint32_t* list = a.triLists + (size_t)c * a.cap;
+    while (top) {
+        const uint32_t ni = stack[--top];
+        const float4* nd = reinterpret_cast<const float4*>(a.mesh.nodes + 8 * (size_t)ni);
+        const float4 q0 = nd[0], q1 = nd[1];  // lower xyz, upper x | upper yz, child1, child2
+        if (!overlaps(q0, q1, box)) continue;
+        const uint32_t child1 = __float_as_uint(q1.z), child2 = __float_as_uint(q1.w);
+        if (child1 == 0xFFFFFFFFu) {
+            if (found < a.cap) list[found] = child2;
+            ++found;
+        } else if (top + 2 <= kTreeStack) {
+            stack[top++] = child1;
+            stack[top++] = child2;
+        } else {
+            flags |= kFlagTreeStackOverflow;
+        }
+    }
+    if (found > a.cap) flags |= kFlagTriangleOverflow;
+    return found;
+}
+
+// Appends contact c's m triangles as work items of its collider's segment and returns the
+// first item's offset in the segment (0 without items).  Every lane of the wave must call it
+// (one atomic per segment with a wave-uniform address, so each folds into a single
+// wave-level add; a per-lane segment address would serialise every add).
+__device__ __forceinline__ uint32_t list_work(const MeshArgs& a, uint32_t c, uint32_t seg, uint32_t m) {
+    uint32_t off = 0;
+    for (uint32_t g = 0; g < 3u; ++g) {
+        const uint32_t o = atomicAdd(a.totals + g, seg == g ? m : 0u);
+        if (seg == g) off = o;
+    }
+    uint32_t* const items = a.pairContact + (size_t)(seg < 3u ? seg : 0u) * a.n * a.cap + off;
+    for (uint32_t k = 0; k < m; ++k) items[k] = c;
+    return off;
+}
+
+// Freshly created contacts: the fat AABB of shape A in the mesh frame (mesh_contact.cpp:30-60),
+// the tree query and the work items.
 __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
     uint32_t flags = 0;
     const uint32_t si = a.shapeA[c];
     uint32_t found = 0;
-    if (si < a.shapeCount && a.shapes[si].type <= 2u &&
-        (a.shapes[si].type != 2u || a.shapes[si].hull < a.hullCount)) {
+    if (mesh_pair_valid(a, si)) {
         const ShapeRec s = a.shapes[si];
         const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
         const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
@@ -89,46 +139,84 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
         const V3 ext{0.2f, 0.2f, 0.2f};  // B3_AABB_EXTENSION (settings.h:54)
         box.lo = box.lo - ext;
         box.hi = box.hi + ext;
-        uint32_t stack[kTreeStack];
-        uint32_t top = 0;
-        if (a.mesh.nodeCount > 0) stack[top++] = 0;
-        uint32_t* list = a.triLists + (size_t)c * a.cap;
-        while (top) {
-            const uint32_t ni = stack[--top];
-            const float4* nd = reinterpret_cast<const float4*>(a.mesh.nodes + 8 * (size_t)ni);
-            const float4 q0 = nd[0], q1 = nd[1];  // lower xyz, upper x | upper yz, child1, child2
-            if (!overlaps(q0, q1, box)) continue;
-            const uint32_t child1 = __float_as_uint(q1.z), child2 = __float_as_uint(q1.w);
-            if (child1 == 0xFFFFFFFFu) {
-                if (found < a.cap) list[found] = child2;
-                ++found;
-            } else if (top + 2 <= kTreeStack) {
-                stack[top++] = child1;
-                stack[top++] = child2;
-            } else {
-                flags |= kFlagTreeStackOverflow;
-            }
-        }
+        found = tree_query(a, c, box, flags);
     } else {
         flags |= kFlagInvalidPair;
     }
-    if (found > a.cap) flags |= kFlagTriangleOverflow;
     const uint32_t m = (flags & kFlagInvalidPair) ? 0u : (found < a.cap ? found : a.cap);
     a.triCounts[c] = found;
     a.contactFlags[c] = flags;
-    const uint32_t seg = m > 0 ? segment_of(a.shapes[si].type) : 3u;
-    // One atomic per segment with a wave-uniform address, so each folds into a single
-    // wave-level add (a per-lane segment address would serialise every lane's add).
-    uint32_t off = 0;
-    for (uint32_t g = 0; g < 3u; ++g) {
-        const uint32_t o = atomicAdd(a.totals + g, seg == g ? m : 0u);
-        if (seg == g) off = o;
+    a.contactOffset[c] = list_work(a, c, m > 0 ? segment_of(a.shapes[si].type) : 3u, m);
+}
+
+// Persistent contacts: kMeshCreate (constructor + FindNewPairs) or kMeshSync
+// (SynchronizeShapes + MoveAABB + FindNewPairs, mesh_contact.cpp:62-160).  A query resets
+// the caches of the listed triangles to empty (b3MeshContact::Report).
+__global__ __launch_bounds__(256) void mesh_sync_kernel(MeshArgs a, uint32_t mode) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const uint32_t si = a.shapeA[c];
+    float* const fat = a.fat + 6 * (size_t)c;
+    ConvexCache* const caches = a.triCaches + (size_t)c * a.cap;
+    uint32_t* const countOut = a.triCounts + c;
+    uint32_t* const flagsOut = a.contactFlags + c;
+    const bool valid = mesh_pair_valid(a, si);
+    bool query = valid;
+    Aabb box{V3{0.0f, 0.0f, 0.0f}, V3{0.0f, 0.0f, 0.0f}};
+    if (valid) {
+        const ShapeRec s = a.shapes[si];
+        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+        const Aabb aabb = shape_aabb(s, mulT(xfB, xfA), a.pool, a.offsets);
+        const float ext = 0.2f;  // B3_AABB_EXTENSION (settings.h:54)
+        box = Aabb{aabb.lo - V3{ext, ext, ext}, aabb.hi + V3{ext, ext, ext}};
+        if (mode == kMeshSync) {
+            const Aabb old{V3{fat[0], fat[1], fat[2]}, V3{fat[3], fat[4], fat[5]}};
+            // b3AABB3::Contains (aabb3.h:156-167): both corners of the new box in the fat one.
+            const bool inside = old.lo.x <= aabb.lo.x && aabb.lo.x <= old.hi.x && old.lo.y <= aabb.lo.y &&
+                                aabb.lo.y <= old.hi.y && old.lo.z <= aabb.lo.z && aabb.lo.z <= old.hi.z &&
+                                old.lo.x <= aabb.hi.x && aabb.hi.x <= old.hi.x && old.lo.y <= aabb.hi.y &&
+                                aabb.hi.y <= old.hi.y && old.lo.z <= aabb.hi.z && aabb.hi.z <= old.hi.z;
+            query = !inside;  // m_aabbMoved: a contained box keeps the list and its caches
+            // Motion-predicted fat box (B3_AABB_MULTIPLIER, settings.h:59).
+            const float* c0 = a.centers0 + 3 * (size_t)c;
+            const V3 d = xfA.p - V3{c0[0], c0[1], c0[2]};
+            const float kMul = 2.0f;
+            if (d.x < 0.0f) box.lo.x += kMul * d.x;
+            else box.hi.x += kMul * d.x;
+            if (d.y < 0.0f) box.lo.y += kMul * d.y;
+            else box.hi.y += kMul * d.y;
+            if (d.z < 0.0f) box.lo.z += kMul * d.z;
+            else box.hi.z += kMul * d.z;
+        }
     }
-    if (m > 0) {
-        uint32_t* const items = a.pairContact + (size_t)seg * a.n * a.cap + off;
-        for (uint32_t k = 0; k < m; ++k) items[k] = c;
+    uint32_t flags = valid ? 0u : kFlagInvalidPair, found = 0;
+    if (query) {
+        fat[0] = box.lo.x, fat[1] = box.lo.y, fat[2] = box.lo.z;
+        fat[3] = box.hi.x, fat[4] = box.hi.y, fat[5] = box.hi.z;
+        found = tree_query(a, c, box, flags);
+        const uint32_t m = found < a.cap ? found : a.cap;
+        for (uint32_t k = 0; k < m; ++k) {
+            ConvexCache e;
+            for (int w = 0; w < 5; ++w) e.simplex[w] = 0u;  // b3SimplexCache with count 0
+            e.feature.state = kCacheEmpty;
+            e.feature.type = e.feature.index1 = e.feature.index2 = 0u;
+            caches[k] = e;
+        }
     }
-    a.contactOffset[c] = off;
+    if (query || !valid) {
+        *countOut = found;
+        *flagsOut = flags;
+    }
+}
+
+// Work items of persistent contacts from their current triangle lists.
+__global__ __launch_bounds__(256) void mesh_list_kernel(MeshArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= a.n) return;
+    const uint32_t flags = a.contactFlags[c], found = a.triCounts[c];
+    const uint32_t m = (flags & kFlagInvalidPair) ? 0u : (found < a.cap ? found : a.cap);
+    a.contactOffset[c] = list_work(a, c, m > 0 ? segment_of(a.shapes[a.shapeA[c]].type) : 3u, m);
 }
 
 // With counting on, hull-triangle collides go to b3_convexCalls and the sphere/capsule
@@ -137,7 +225,7 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
 __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t t0 = a.totals[0], t1 = a.totals[1], t2 = a.totals[2];
-    uint32_t hullCalls = 0, calls = 0, iters = 0;
+    uint32_t hullCalls = 0, hits = 0, calls = 0, iters = 0;
     if (f < t0 + t1 + t2) {
         const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
         const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
@@ -156,7 +244,23 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
         m.overflow = false;
         if (s.type == 2u) {
             const DevHull h = DevHull::at(a.pool, a.offsets, s.hull);
-            collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
+            if (a.triCaches) {
+                // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
+                // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.
+                FeatureCache* const fc = &a.triCaches[(size_t)c * a.cap + k].feature;
+                Quat q[4];
+                for (int w = 0; w < 4; ++w) {
+                    if (a.sweeps) {
+                        const float* qs = a.sweeps + 16 * (size_t)c + 4 * w;
+                        q[w] = Quat{qs[0], qs[1], qs[2], qs[3]};
+                    } else {
+                        q[w] = Quat{0.0f, 0.0f, 0.0f, 1.0f};
+                    }
+                }
+                collide_hulls_cached_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, fc, q[0], q[1], q[2], q[3], hits);
+            } else {
+                collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
+            }
             hullCalls = 1;
         } else if (s.type == 0u) {
             collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
@@ -170,6 +274,7 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
     }
     if (a.counters) {  // all lanes of the block reach here
         wave_count(&a.counters->convexCalls, hullCalls);
+        wave_count(&a.counters->convexCacheHits, hits);
         wave_count(&a.counters->gjkCalls, calls);
         wave_count(&a.counters->gjkIters, iters);
         wave_max(&a.counters->gjkMaxIters, iters);
@@ -426,6 +531,18 @@ __global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
 hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     mesh_midphase_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_sync(const MeshArgs& a, uint32_t mode, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    mesh_sync_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a, mode);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_list(const MeshArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    mesh_list_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -285,6 +285,34 @@ int b3amd_collide_mesh_device(b3amd_context* ctx, uint32_t meshId, uint32_t n, c
                               const b3amd_transform* d_xfs, uint32_t* d_triangleCounts, uint32_t* d_triangleLists,
                               b3amd_manifold* d_out, uint32_t* d_manifoldCounts);
 
+/* ---- persistent mesh contacts: b3MeshContact across steps (mesh_contact.cpp) ----
+ * A set of n contacts between convex shapes[shapeA[i]] and mesh meshId that keeps, per
+ * contact, the fat AABB of shape A in the mesh frame (m_aabbA), the triangles the tree
+ * query found (at most B3AMD_MESH_TRIANGLE_CAP, as in b3amd_collide_mesh) and one
+ * b3ConvexCache per triangle (b3TriangleCache, mesh_contact.h:30-34), on the device.
+ *   create       the constructor and FindNewPairs (mesh_contact.cpp:30-55, 141-160),
+ *                xfs[2i] = body A, xfs[2i+1] = mesh body;
+ *   collide      b3MeshContact::Collide (:217-275) with every triangle's cache: per
+ *                triangle b3CollideShapeAndShape, triangle keys, clustering into at most 3
+ *                manifolds per contact in out[3i..]; sweeps (may be NULL: identity
+ *                orientations) gives per contact body A's orientation0/orientation then
+ *                the mesh body's (4 quats), read by the cached face rebuild;
+ *   synchronize  SynchronizeShapes, MoveAABB and FindNewPairs (:62-160) for new transforms;
+ *                centers0[3i..] is body A's sweep worldCenter0 (the displacement's origin).
+ *                A contact whose shape still fits its fat AABB keeps its triangles and
+ *                caches; otherwise the tree is queried again and the caches restart empty.
+ *   read         triangle counts, lists [n][B3AMD_MESH_TRIANGLE_CAP], caches (same
+ *                layout) and fat AABBs [n][6] (lower xyz, upper xyz); any may be NULL. */
+typedef struct b3amd_mesh_contacts b3amd_mesh_contacts;
+int b3amd_mesh_contacts_create(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t* shapeA,
+                               const b3amd_transform* xfs, b3amd_mesh_contacts** out);
+int b3amd_mesh_contacts_destroy(b3amd_mesh_contacts* contacts);
+int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, const b3amd_quat* sweeps,
+                                b3amd_manifold* out, uint32_t* manifoldCounts);
+int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, const float* centers0);
+int b3amd_mesh_contacts_read(b3amd_mesh_contacts* contacts, uint32_t* triangleCounts, uint32_t* triangleLists,
+                             b3amd_convex_cache* caches, float* fatAabbs);
+
 /* ---- device memory helpers (callers need no other GPU runtime) ---- */
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);
 int b3amd_device_free(b3amd_context* ctx, void* ptr);

@@ -482,3 +482,131 @@ extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids
     }
     return (uint32_t)collided.size();
 }
+
+// ---------------------------------------------------------------------------------
+// Persistent mesh contacts: real b3MeshContact objects kept across calls and driven as
+// b3World does each step: Collide in UpdateContacts (contact.cpp:41-140), then after the
+// solver SynchronizeShapes and FindNewPairs (contact_manager.cpp:144-162).  One body image
+// per side is re-posed before each contact's call (a contact keeps only m_aabbA, its
+// triangle caches and manifolds).
+namespace {
+
+struct RefMeshContacts {
+    Scene scene;
+    b3Mesh mesh;
+    b3World* world = nullptr;
+    b3Body* bodyA = nullptr;
+    b3Body* bodyB = nullptr;
+    b3MeshShape meshShape;
+    std::vector<b3MeshContact*> contacts;
+    uint64_t saved[5];
+};
+
+void pose(b3Body* b, const b3o_xf* xf) { std::memcpy(&b->m_xf, xf, sizeof(b3Transform)); }
+
+}  // namespace
+
+extern "C" void* b3ref_mesh_contacts_create(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* m,
+                                            uint32_t n, const uint32_t* shapeA, const b3o_xf* xfs) {
+    RefMeshContacts* r = new RefMeshContacts();
+    build_hulls(pool, r->scene.hulls);
+    make_mesh(r->mesh, m->vertices, m->vertexCount, m->triangleCount, m->triangles);
+    b3ref_get_counters(r->saved);
+    r->world = new b3World();
+    b3BodyDef bdA;
+    bdA.type = e_dynamicBody;
+    r->bodyA = r->world->CreateBody(bdA);
+    b3BodyDef bdB;
+    r->bodyB = r->world->CreateBody(bdB);
+    r->meshShape.m_mesh = &r->mesh;
+    r->meshShape.m_body = r->bodyB;
+    for (uint32_t i = 0; i < n; ++i) {
+        b3Shape* A = make_shape(shapes[shapeA[i]], r->scene.hulls);
+        r->scene.shapes.push_back(A);
+        A->m_body = r->bodyA;
+        pose(r->bodyA, xfs + 2 * (size_t)i);
+        pose(r->bodyB, xfs + 2 * (size_t)i + 1);
+        b3MeshContact* mc = new b3MeshContact(A, &r->meshShape);
+        mc->m_pair.shapeA = A;
+        mc->m_pair.shapeB = &r->meshShape;
+        mc->FindNewPairs();
+        r->contacts.push_back(mc);
+    }
+    // The world's constructor reset the statistics globals: keep the totals running.
+    b3_convexCalls = (u32)r->saved[0];
+    b3_convexCacheHits = (u32)r->saved[1];
+    b3_gjkCalls = (u32)r->saved[2];
+    b3_gjkIters = (u32)r->saved[3];
+    b3_gjkMaxIters = (u32)r->saved[4];
+    return r;
+}
+
+extern "C" void b3ref_mesh_contacts_collide(void* handle, const b3o_xf* xfs, const b3o_quat* sweeps, b3o_manifold* out,
+                                            uint32_t* manifoldCounts) {
+    RefMeshContacts* r = (RefMeshContacts*)handle;
+    const b3Quat identity(0.0f, 0.0f, 0.0f, 1.0f);
+    for (size_t i = 0; i < r->contacts.size(); ++i) {
+        b3MeshContact* mc = r->contacts[i];
+        pose(r->bodyA, xfs + 2 * i);
+        pose(r->bodyB, xfs + 2 * i + 1);
+        b3Sweep& sa = r->bodyA->m_sweep;
+        b3Sweep& sb = r->bodyB->m_sweep;
+        if (sweeps) {
+            std::memcpy(&sa.orientation0, sweeps + 4 * i, 16);
+            std::memcpy(&sa.orientation, sweeps + 4 * i + 1, 16);
+            std::memcpy(&sb.orientation0, sweeps + 4 * i + 2, 16);
+            std::memcpy(&sb.orientation, sweeps + 4 * i + 3, 16);
+        } else {
+            sa.orientation0 = sa.orientation = sb.orientation0 = sb.orientation = identity;
+        }
+        mc->m_manifoldCount = 0;
+        for (u32 k = 0; k < mc->m_manifoldCapacity; ++k) mc->m_manifolds[k].Initialize();
+        mc->Collide();
+        manifoldCounts[i] = mc->m_manifoldCount;
+        for (uint32_t k = 0; k < 3; ++k) {
+            if (k < mc->m_manifoldCount) to_out(mc->m_manifolds[k], out + 3 * i + k);
+            else std::memset(out + 3 * i + k, 0, sizeof(b3o_manifold));
+        }
+    }
+}
+
+extern "C" void b3ref_mesh_contacts_synchronize(void* handle, const b3o_xf* xfs, const float* centers0) {
+    RefMeshContacts* r = (RefMeshContacts*)handle;
+    for (size_t i = 0; i < r->contacts.size(); ++i) {
+        b3MeshContact* mc = r->contacts[i];
+        pose(r->bodyA, xfs + 2 * i);
+        pose(r->bodyB, xfs + 2 * i + 1);
+        std::memcpy(&r->bodyA->m_sweep.worldCenter0, centers0 + 3 * i, 12);
+        mc->SynchronizeShapes();
+        mc->FindNewPairs();
+    }
+}
+
+extern "C" void b3ref_mesh_contacts_read(void* handle, uint32_t cap, uint32_t* triCounts, uint32_t* triLists,
+                                         b3o_convex_cache* caches, float* fat) {
+    RefMeshContacts* r = (RefMeshContacts*)handle;
+    for (size_t i = 0; i < r->contacts.size(); ++i) {
+        const b3MeshContact* mc = r->contacts[i];
+        triCounts[i] = mc->m_triangleCount;
+        for (uint32_t k = 0; k < mc->m_triangleCount && k < cap; ++k) {
+            triLists[i * cap + k] = mc->m_triangles[k].index;
+            std::memcpy(caches + i * cap + k, &mc->m_triangles[k].cache, sizeof(b3o_convex_cache));
+        }
+        std::memcpy(fat + 6 * i, &mc->m_aabbA.m_lower, 12);
+        std::memcpy(fat + 6 * i + 3, &mc->m_aabbA.m_upper, 12);
+    }
+}
+
+extern "C" void b3ref_mesh_contacts_destroy(void* handle) {
+    RefMeshContacts* r = (RefMeshContacts*)handle;
+    uint64_t now[5];
+    b3ref_get_counters(now);
+    for (b3MeshContact* mc : r->contacts) delete mc;
+    delete r->world;  // resets the statistics globals: keep them
+    b3_convexCalls = (u32)now[0];
+    b3_convexCacheHits = (u32)now[1];
+    b3_gjkCalls = (u32)now[2];
+    b3_gjkIters = (u32)now[3];
+    b3_gjkMaxIters = (u32)now[4];
+    delete r;
+}

@@ -102,3 +102,59 @@ def test_gpu_mesh_device_resident_equals_host_api(ctx, lib):
         assert gm[7] == 0 and (got[7]["flags"] & 2).all() and (got[7]["pointCount"] == 0).all()
         for b in (ds, dx, do, dm, dc, dl):
             b.free()
+
+
+def test_gpu_mesh_contacts_persistent_equal_reference(ctx, lib, reference):
+    """Mesh contacts kept across eight steps of moving shapes (b3MeshContact in the
+    reference, b3amd_mesh_contacts_* here): after creation and after every
+    SynchronizeShapes + FindNewPairs the fat AABBs, triangle lists and triangle caches equal
+    the reference's; every Collide's clustered manifolds are bit-exact; the statistics
+    counters (hull-triangle calls and cache hits, GJK calls and iterations) agree."""
+    import bounce_amd
+    from util import HULL_CACHE_FIELDS, assert_caches_equal, defined_hull_cache, moving_mesh_steps
+    print("case: persistent mesh contacts, 8 steps", flush=True)
+    pool, shapes, v, t, xfs = mesh_workload(lib, 91, 48, 48, 300)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    steps = moving_mesh_steps(xfs, 9, 5)
+    ref = oracle_lib.RefMeshContacts(reference.lib, pool, shapes, v, t, sa, steps[0][0])
+
+    def same_state(dev, what):
+        dc, dl, dca, df = dev.read()
+        rc, rl, rca, rf = ref.read()
+        np.testing.assert_array_equal(dc, rc, what)
+        np.testing.assert_array_equal(df.view(np.uint32), rf.view(np.uint32), what)
+        for i in range(len(dc)):
+            k = min(int(dc[i]), 64)
+            np.testing.assert_array_equal(dl[i, :k], rl[i, :k], what)
+            assert_caches_equal(defined_hull_cache(dca[i, :k]), defined_hull_cache(rca[i, :k]),
+                                "%s contact %d" % (what, i))
+        return int((dc > 0).sum())
+
+    ctx.set_counting(True)
+    ctx.reset_counters()
+    reference.reset_counters()
+    with bounce_amd.MeshContacts(ctx, mid, sa, steps[0][0]) as dev:
+        same_state(dev, "after creation")
+        requeried = 0
+        for s in range(8):
+            x, sw, p = steps[s]
+            got, gc = dev.collide(x, sw)
+            want, wc = ref.collide(x, sw)
+            np.testing.assert_array_equal(gc, wc, "step %d manifold counts" % s)
+            assert_manifolds_equal(got.reshape(-1), want.reshape(-1), "step %d manifolds" % s)
+            before = dev.read()[1].copy()
+            dev.synchronize(steps[s + 1][0], p)
+            ref.synchronize(steps[s + 1][0], p)
+            same_state(dev, "after synchronize %d" % s)
+            requeried += int((dev.read()[1] != before).any(1).sum())
+        assert requeried > 10  # the fast shapes left their fat boxes
+    got_c = ctx.counters()
+    want_c = reference.counters()
+    ctx.set_counting(False)
+    for k in ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters"):
+        assert got_c[k] == want_c[k], (k, got_c[k], want_c[k])
+    assert want_c["convex_cache_hits"] > 0
+    ref.close()

@@ -407,3 +407,67 @@ def pyramid_full_outputs(trace, step):
     k = list(int(s) for s in trace["full_steps"]).index(step)
     a, b = int(trace["full_offsets"][k]), int(trace["full_offsets"][k + 1])
     return trace["full_out"][a:b], trace["full_caches"][a:b]
+
+
+def quat_matrix(q):
+    """Rotation matrices [n][3][3] of unit quaternions [n][4] (x, y, z, w) in float64."""
+    x, y, z, w = (q[:, i].astype(np.float64) for i in range(4))
+    return np.stack([
+        np.stack([1 - 2 * (y * y + z * z), 2 * (x * y - w * z), 2 * (x * z + w * y)], -1),
+        np.stack([2 * (x * y + w * z), 1 - 2 * (x * x + z * z), 2 * (y * z - w * x)], -1),
+        np.stack([2 * (x * z - w * y), 2 * (y * z + w * x), 1 - 2 * (x * x + y * y)], -1)], 1)
+
+
+def moving_mesh_steps(xfs0, steps, seed, fast_fraction=0.15):
+    """Synthetic motion of the convex shapes of mesh contacts (body A; the mesh body stays
+    at identity): per step a translation (mostly ~5 mm, a fast fraction ~8 cm, which leaves
+    the 0.2 m fat AABB within a few steps) and a small rotation.  Returns, per step s,
+    (xfs [n][2][12], sweeps [n][4][4] = (q_{s-1}, q_s, identity, identity), positions [n][3]);
+    the orientations are quaternions and the transforms their float32 matrices."""
+    rng = np.random.default_rng(seed)
+    n = len(xfs0)
+    R0 = np.stack([xfs0[:, 0, 0:3], xfs0[:, 0, 3:6], xfs0[:, 0, 6:9]], 2).astype(np.float64)
+    # Quaternion of each initial rotation (Shepperd's method on the trace).
+    q = np.zeros((n, 4))
+    for i in range(n):
+        m = R0[i]
+        t = np.trace(m)
+        if t > 0:
+            s = np.sqrt(t + 1.0) * 2
+            q[i] = [(m[2, 1] - m[1, 2]) / s, (m[0, 2] - m[2, 0]) / s, (m[1, 0] - m[0, 1]) / s, 0.25 * s]
+        else:
+            k = int(np.argmax(np.diag(m)))
+            a, b, c = k, (k + 1) % 3, (k + 2) % 3
+            s = np.sqrt(1.0 + m[a, a] - m[b, b] - m[c, c]) * 2
+            v = np.zeros(4)
+            v[a] = 0.25 * s
+            v[b] = (m[b, a] + m[a, b]) / s
+            v[c] = (m[c, a] + m[a, c]) / s
+            v[3] = (m[c, b] - m[b, c]) / s
+            q[i] = v
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    pos = xfs0[:, 0, 9:12].astype(np.float64)
+    speed = np.where(rng.random(n) < fast_fraction, 0.08, 0.005)
+    vel = rng.normal(size=(n, 3))
+    vel *= (speed / np.linalg.norm(vel, axis=1))[:, None]
+    omega = rng.normal(0.0, 0.01, (n, 3))
+    out = []
+    for s in range(steps):
+        q_prev = q.copy()
+        if s > 0:
+            pos = pos + vel
+            dq = np.concatenate([0.5 * omega, np.ones((n, 1))], 1)
+            x1, y1, z1, w1 = dq.T
+            x2, y2, z2, w2 = q.T
+            q = np.stack([w1 * x2 + x1 * w2 + y1 * z2 - z1 * y2, w1 * y2 - x1 * z2 + y1 * w2 + z1 * x2,
+                          w1 * z2 + x1 * y2 - y1 * x2 + z1 * w2, w1 * w2 - x1 * x2 - y1 * y2 - z1 * z2], 1)
+            q /= np.linalg.norm(q, axis=1, keepdims=True)
+        R = quat_matrix(q)
+        xfs = xfs0.copy()
+        xfs[:, 0, 0:3], xfs[:, 0, 3:6], xfs[:, 0, 6:9] = R[:, :, 0], R[:, :, 1], R[:, :, 2]
+        xfs[:, 0, 9:12] = pos
+        sweeps = np.zeros((n, 4, 4), np.float32)
+        sweeps[:, 0], sweeps[:, 1] = q_prev, q
+        sweeps[:, 2, 3] = sweeps[:, 3, 3] = 1.0
+        out.append((xfs.astype(np.float32), sweeps, pos.astype(np.float32)))
+    return out
