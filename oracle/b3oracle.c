@@ -1863,6 +1863,48 @@ static uint32_t cluster_run(b3o_manifold out[3], const b3o_manifold* in, uint32_
 }
 
 /* ---------------------------------------------------------------- mesh_contact.cpp:30-275 */
+/* b3MeshContact::Collide (mesh_contact.cpp:217-275) over `count` triangles: per triangle a
+ * b3TriangleHull, b3CollideShapeAndShape with the triangle's cache (caches may be NULL: fresh
+ * empty caches), the triangle index stamped into the keys; then b3ClusterSolver::Run.  The
+ * triangle hull belongs to the mesh body: the cached face rebuild reads body A's sweep
+ * (q10, q1) and the mesh body's (q20, q2). */
+static uint32_t mesh_collide_contact(const b3o_hull_pool* pool, const b3o_shape* sa, const b3o_mesh* mesh,
+                                     const XF* xfA, const XF* xfB, const uint32_t* tris, uint32_t count,
+                                     b3o_convex_cache* caches, QT q10, QT q1, QT q20, QT q2, b3o_manifold out[3]) {
+    static b3o_manifold temp[65536];
+    for (uint32_t k = 0; k < count; ++k) {
+        uint32_t ti = tris[k];
+        const uint32_t* t = mesh->triangles + 3 * (size_t)ti;
+        V3 a = v3(mesh->vertices[3 * t[0]], mesh->vertices[3 * t[0] + 1], mesh->vertices[3 * t[0] + 2]);
+        V3 b = v3(mesh->vertices[3 * t[1]], mesh->vertices[3 * t[1] + 1], mesh->vertices[3 * t[1] + 2]);
+        V3 c = v3(mesh->vertices[3 * t[2]], mesh->vertices[3 * t[2] + 1], mesh->vertices[3 * t[2] + 2]);
+        TriRec rec;
+        V3 centroid;
+        Hull th = triangle_hull(&rec, a, b, c, &centroid);
+        b3o_manifold* m = temp + k;
+        memset(m, 0, sizeof(*m));
+        if (sa->type == B3O_HULL) {
+            Hull h1 = hull_at(pool, sa->hull);
+            b3o_feature_cache fresh;
+            b3o_feature_cache* fc = caches ? &caches[k].feature : &fresh;
+            if (!caches) {
+                fresh.state = B3O_EMPTY;
+                fresh.type = 0; fresh.index1 = 0; fresh.index2 = 0;
+            }
+            collide_hulls_cached(m, xfA, &h1, sa->radius, xfB, &th, 0.0f, fc, q10, q1, q20, q2);
+        } else {
+            Round r1 = round_of(sa);
+            PHASE(1);
+            if (sa->type == B3O_SPHERE) collide_sphere_hull(m, xfA, &r1, xfB, &th, 0.0f);
+            else collide_capsule_hull(m, xfA, &r1, xfB, &th, 0.0f);
+        }
+        for (uint32_t j = 0; j < m->pointCount; ++j) m->points[j].triangleKey = ti;
+    }
+    PHASE(2);
+    memset(out, 0, 3 * sizeof(b3o_manifold));
+    return cluster_run(out, temp, count, xfA, sa->radius, xfB, 0.0f);
+}
+
 /* A freshly created b3MeshContact between shape A and the mesh: fat AABB of A in the mesh
  * frame extended by B3_AABB_EXTENSION (mesh_contact.cpp:30-60), FindNewPairs through the
  * static tree (:140-189, all caches empty), then Collide (:217-275). */
@@ -1871,7 +1913,7 @@ void b3o_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const 
                       uint32_t* triLists, b3o_manifold* out, uint32_t* manifoldCounts) {
     const TNode* nodes = (const TNode*)mesh->nodes;
     static uint32_t tris[65536];
-    static b3o_manifold temp[65536];
+    static const QT qid = {0.0f, 0.0f, 0.0f, 1.0f};
     for (uint32_t i = 0; i < n; ++i) {
         const b3o_shape* sa = shapes + shapeA[i];
         const XF* xfA = xfs + 2 * (size_t)i;
@@ -1883,37 +1925,134 @@ void b3o_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, const 
         if (found > 65536) found = 65536;
         triCounts[i] = found;
         for (uint32_t k = 0; k < found && k < cap; ++k) triLists[(size_t)i * cap + k] = tris[k];
-        for (uint32_t k = 0; k < found; ++k) {
-            uint32_t ti = tris[k];
-            const uint32_t* t = mesh->triangles + 3 * (size_t)ti;
-            V3 a = v3(mesh->vertices[3 * t[0]], mesh->vertices[3 * t[0] + 1], mesh->vertices[3 * t[0] + 2]);
-            V3 b = v3(mesh->vertices[3 * t[1]], mesh->vertices[3 * t[1] + 1], mesh->vertices[3 * t[1] + 2]);
-            V3 c = v3(mesh->vertices[3 * t[2]], mesh->vertices[3 * t[2] + 1], mesh->vertices[3 * t[2] + 2]);
-            TriRec rec;
-            V3 centroid;
-            Hull th = triangle_hull(&rec, a, b, c, &centroid);
-            b3o_manifold* m = temp + k;
-            memset(m, 0, sizeof(*m));
-            static const QT qid = {0.0f, 0.0f, 0.0f, 1.0f};
-            if (sa->type == B3O_HULL) {
-                Hull h1 = hull_at(pool, sa->hull);
-                b3o_feature_cache fc;
-                fc.state = B3O_EMPTY;
-                fc.type = 0; fc.index1 = 0; fc.index2 = 0;
-                collide_hulls_cached(m, xfA, &h1, sa->radius, xfB, &th, 0.0f, &fc, qid, qid, qid, qid);
-            } else {
-                Round r1 = round_of(sa);
-                PHASE(1);
-                if (sa->type == B3O_SPHERE) collide_sphere_hull(m, xfA, &r1, xfB, &th, 0.0f);
-                else collide_capsule_hull(m, xfA, &r1, xfB, &th, 0.0f);
-            }
-            for (uint32_t j = 0; j < m->pointCount; ++j) m->points[j].triangleKey = ti;
-        }
-        PHASE(2);
-        b3o_manifold* o = out + 3 * (size_t)i;
-        memset(o, 0, 3 * sizeof(b3o_manifold));
-        manifoldCounts[i] = cluster_run(o, temp, found, xfA, sa->radius, xfB, 0.0f);
+        manifoldCounts[i] = mesh_collide_contact(pool, sa, mesh, xfA, xfB, tris, found, NULL, qid, qid, qid, qid,
+                                                 out + 3 * (size_t)i);
     }
+}
+
+/* Persistent b3MeshContacts (mesh_contact.cpp:30-275) kept across steps: per contact the fat
+ * AABB m_aabbA, the triangle list with one b3ConvexCache each (b3TriangleCache), grown
+ * without bound as the reference's list is. */
+typedef struct {
+    AABB fat;
+    uint32_t count, capacity;
+    uint32_t* tris;
+    b3o_convex_cache* caches;
+} MeshContact;
+
+struct b3o_mesh_contacts {
+    const b3o_hull_pool* pool;
+    const b3o_shape* shapes;
+    const b3o_mesh* mesh;
+    uint32_t n;
+    uint32_t* shapeA;
+    MeshContact* c;
+};
+
+/* FindNewPairs (:141-160) with Report (:162-189): the list restarts and every triangle's cache
+ * starts empty (simplex count 0, feature state empty). */
+static void mesh_find_new_pairs(b3o_mesh_contacts* h, MeshContact* mc) {
+    static uint32_t tris[65536];
+    uint32_t found = tree_query((const TNode*)h->mesh->nodes, h->mesh->nodeCount, &mc->fat, tris, 65536);
+    if (found > 65536) found = 65536;
+    if (found > mc->capacity) {
+        mc->capacity = found;
+        mc->tris = (uint32_t*)realloc(mc->tris, sizeof(uint32_t) * found);
+        mc->caches = (b3o_convex_cache*)realloc(mc->caches, sizeof(b3o_convex_cache) * found);
+    }
+    mc->count = found;
+    for (uint32_t k = 0; k < found; ++k) {
+        mc->tris[k] = tris[k];
+        memset(mc->caches + k, 0, sizeof(b3o_convex_cache));
+        mc->caches[k].feature.state = B3O_EMPTY;
+    }
+}
+
+b3o_mesh_contacts* b3o_mesh_contacts_create(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh,
+                                            uint32_t n, const uint32_t* shapeA, const b3o_xf* xfs) {
+    b3o_mesh_contacts* h = (b3o_mesh_contacts*)calloc(1, sizeof(b3o_mesh_contacts));
+    h->pool = pool;
+    h->shapes = shapes;
+    h->mesh = mesh;
+    h->n = n;
+    h->shapeA = (uint32_t*)malloc(sizeof(uint32_t) * (n ? n : 1));
+    memcpy(h->shapeA, shapeA, sizeof(uint32_t) * n);
+    h->c = (MeshContact*)calloc(n ? n : 1, sizeof(MeshContact));
+    for (uint32_t i = 0; i < n; ++i) {
+        /* Constructor (:30-55): fat AABB of A in the mesh frame; then FindNewPairs. */
+        const XF* xfA = xfs + 2 * (size_t)i;
+        XF xf = xf_mulT_xf(xfA + 1, xfA);
+        h->c[i].fat = shape_aabb(pool, shapes + shapeA[i], &xf);
+        aabb_extend(&h->c[i].fat, 0.2f);  /* B3_AABB_EXTENSION (settings.h:54) */
+        mesh_find_new_pairs(h, h->c + i);
+    }
+    return h;
+}
+
+void b3o_mesh_contacts_collide(b3o_mesh_contacts* h, const b3o_xf* xfs, const b3o_quat* sweeps, b3o_manifold* out,
+                               uint32_t* manifoldCounts) {
+    static const QT qid = {0.0f, 0.0f, 0.0f, 1.0f};
+    for (uint32_t i = 0; i < h->n; ++i) {
+        const XF* xfA = xfs + 2 * (size_t)i;
+        QT q10 = qid, q1 = qid, q20 = qid, q2 = qid;
+        if (sweeps) {
+            q10 = sweeps[4 * (size_t)i]; q1 = sweeps[4 * (size_t)i + 1];
+            q20 = sweeps[4 * (size_t)i + 2]; q2 = sweeps[4 * (size_t)i + 3];
+        }
+        MeshContact* mc = h->c + i;
+        manifoldCounts[i] = mesh_collide_contact(h->pool, h->shapes + h->shapeA[i], h->mesh, xfA, xfA + 1, mc->tris,
+                                                 mc->count, mc->caches, q10, q1, q20, q2, out + 3 * (size_t)i);
+    }
+}
+
+/* SynchronizeShapes + MoveAABB (:62-139) and FindNewPairs. */
+void b3o_mesh_contacts_synchronize(b3o_mesh_contacts* h, const b3o_xf* xfs, const float* centers0) {
+    for (uint32_t i = 0; i < h->n; ++i) {
+        const XF* xfA = xfs + 2 * (size_t)i;
+        MeshContact* mc = h->c + i;
+        V3 d = sub(xfA->p, v3(centers0[3 * i], centers0[3 * i + 1], centers0[3 * i + 2]));
+        XF xf = xf_mulT_xf(xfA + 1, xfA);
+        AABB aabb = shape_aabb(h->pool, h->shapes + h->shapeA[i], &xf);
+        /* b3AABB3::Contains (aabb3.h:156-167) */
+        int inside = mc->fat.lo.x <= aabb.lo.x && aabb.lo.x <= mc->fat.hi.x && mc->fat.lo.y <= aabb.lo.y &&
+                     aabb.lo.y <= mc->fat.hi.y && mc->fat.lo.z <= aabb.lo.z && aabb.lo.z <= mc->fat.hi.z &&
+                     mc->fat.lo.x <= aabb.hi.x && aabb.hi.x <= mc->fat.hi.x && mc->fat.lo.y <= aabb.hi.y &&
+                     aabb.hi.y <= mc->fat.hi.y && mc->fat.lo.z <= aabb.hi.z && aabb.hi.z <= mc->fat.hi.z;
+        if (inside) continue;
+        AABB fat = aabb;
+        aabb_extend(&fat, 0.2f);  /* B3_AABB_EXTENSION */
+        const float kMul = 2.0f;  /* B3_AABB_MULTIPLIER (settings.h:59) */
+        if (d.x < 0.0f) fat.lo.x += kMul * d.x; else fat.hi.x += kMul * d.x;
+        if (d.y < 0.0f) fat.lo.y += kMul * d.y; else fat.hi.y += kMul * d.y;
+        if (d.z < 0.0f) fat.lo.z += kMul * d.z; else fat.hi.z += kMul * d.z;
+        mc->fat = fat;
+        mesh_find_new_pairs(h, mc);
+    }
+}
+
+void b3o_mesh_contacts_read(b3o_mesh_contacts* h, uint32_t cap, uint32_t* triCounts, uint32_t* triLists,
+                            b3o_convex_cache* caches, float* fat) {
+    for (uint32_t i = 0; i < h->n; ++i) {
+        const MeshContact* mc = h->c + i;
+        triCounts[i] = mc->count;
+        for (uint32_t k = 0; k < mc->count && k < cap; ++k) {
+            triLists[(size_t)i * cap + k] = mc->tris[k];
+            caches[(size_t)i * cap + k] = mc->caches[k];
+        }
+        memcpy(fat + 6 * (size_t)i, &mc->fat.lo, 12);
+        memcpy(fat + 6 * (size_t)i + 3, &mc->fat.hi, 12);
+    }
+}
+
+void b3o_mesh_contacts_destroy(b3o_mesh_contacts* h) {
+    if (!h) return;
+    for (uint32_t i = 0; i < h->n; ++i) {
+        free(h->c[i].tris);
+        free(h->c[i].caches);
+    }
+    free(h->c);
+    free(h->shapeA);
+    free(h);
 }
 
 /* Triangle AABBs (b3Mesh::GetTriangleAABB, mesh.h:80-86) and the static tree over them. */

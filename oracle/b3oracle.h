@@ -132,6 +132,32 @@ void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* shapes, cons
 void b3ref_mesh_build_tree(const float* vertices, uint32_t vertexCount, uint32_t triangleCount,
                            const uint32_t* triangles, float* nodesOut);
 
+/* Persistent mesh contacts (b3MeshContact across steps, mesh_contact.cpp:30-275): create is
+ * the constructor and FindNewPairs; collide is Collide with every triangle's cache (sweeps may
+ * be NULL: identity orientations; per contact body A's orientation0/orientation then the mesh
+ * body's); synchronize is SynchronizeShapes, MoveAABB and FindNewPairs with body A's sweep
+ * worldCenter0 in centers0[3i..]; read copies the first cap triangles and caches of each list
+ * and the fat AABBs [n][6].  The oracle version reads the tree nodes from mesh->nodes; the
+ * compiled reference's builds its own b3Mesh tree.  The arrays passed to create must outlive
+ * the set. */
+typedef struct b3o_mesh_contacts b3o_mesh_contacts;
+b3o_mesh_contacts* b3o_mesh_contacts_create(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh,
+                                            uint32_t n, const uint32_t* shapeA, const b3o_xf* xfs);
+void b3o_mesh_contacts_collide(b3o_mesh_contacts* h, const b3o_xf* xfs, const b3o_quat* sweeps, b3o_manifold* out,
+                               uint32_t* manifoldCounts);
+void b3o_mesh_contacts_synchronize(b3o_mesh_contacts* h, const b3o_xf* xfs, const float* centers0);
+void b3o_mesh_contacts_read(b3o_mesh_contacts* h, uint32_t cap, uint32_t* triCounts, uint32_t* triLists,
+                            b3o_convex_cache* caches, float* fat);
+void b3o_mesh_contacts_destroy(b3o_mesh_contacts* h);
+void* b3ref_mesh_contacts_create(const b3o_hull_pool* pool, const b3o_shape* shapes, const b3o_mesh* mesh, uint32_t n,
+                                 const uint32_t* shapeA, const b3o_xf* xfs);
+void b3ref_mesh_contacts_collide(void* h, const b3o_xf* xfs, const b3o_quat* sweeps, b3o_manifold* out,
+                                 uint32_t* manifoldCounts);
+void b3ref_mesh_contacts_synchronize(void* h, const b3o_xf* xfs, const float* centers0);
+void b3ref_mesh_contacts_read(void* h, uint32_t cap, uint32_t* triCounts, uint32_t* triLists, b3o_convex_cache* caches,
+                              float* fat);
+void b3ref_mesh_contacts_destroy(void* h);
+
 /* Totals of the reference's statistics globals since the last reset: b3_convexCalls,
  * b3_convexCacheHits, b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters. */
 void b3o_get_counters(uint64_t out[5]);

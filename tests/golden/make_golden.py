@@ -5,7 +5,7 @@ Run in a container where the reference tree is present and oracle/_ref is built
 sweeps, caches) and the outputs of b3CollideShapeAndShape as computed by the reference
 library compiled from its own sources (oracle/_ref/libb3ref.so).
 
-    python tests/golden/make_golden.py [pairs] [mesh] [pyramid]
+    python tests/golden/make_golden.py [pairs] [mesh] [pyramid] [mesh_persistent]
 """
 import os
 import sys
@@ -153,11 +153,55 @@ def make_pyramid_fixture():
                         full_caches=np.concatenate(full_caches), full_offsets=np.array(full_offsets, np.int64))
 
 
+def make_persistent_mesh_fixture():
+    """7. Mesh contacts kept across six steps of moving shapes (120 spheres, capsules and
+    hulls over a 32 x 32 heightfield), driven through the reference's own b3MeshContact:
+    per step the inputs, the clustered manifolds, and after creation and every
+    SynchronizeShapes + FindNewPairs the triangle counts, lists, caches (the fields the
+    reference defines) and fat AABBs."""
+    from util import defined_hull_cache, mesh_workload, moving_mesh_steps
+    ref = oracle_lib.reference()
+    lib = load_library()
+    pool, shapes, v, t, xfs = mesh_workload(lib, 93, 32, 32, 120)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    S = 6
+    steps = moving_mesh_steps(xfs, S + 1, 7)
+    r = oracle_lib.RefMeshContacts(ref.lib, pool, shapes, v, t, sa, steps[0][0])
+
+    def state():
+        c, l, ca, f = r.read()
+        keep = np.arange(l.shape[1])[None, :] < np.minimum(c, l.shape[1])[:, None]
+        l = np.where(keep, l, 0).astype(np.uint32)
+        ca = defined_hull_cache(ca.reshape(-1)).reshape(ca.shape)
+        ca[~keep] = np.zeros(1, ca.dtype)
+        return c, l, ca, f
+
+    states = [state()]
+    outs, counts = [], []
+    for s in range(S):
+        x, sw, p = steps[s]
+        o, mc = r.collide(x, sw)
+        outs.append(o)
+        counts.append(mc)
+        r.synchronize(steps[s + 1][0], p)
+        states.append(state())
+    r.close()
+    np.savez_compressed(os.path.join(HERE, "mesh_contacts_persistent.npz"), desc=pool.desc,
+                        centroids=pool.centroids, vertices=pool.vertices, edges=pool.edges, faces=pool.faces,
+                        planes=pool.planes, shapes=shapes, mesh_vertices=v, mesh_triangles=t, shape_a=sa,
+                        xfs=np.stack([st[0] for st in steps]), sweeps=np.stack([st[1] for st in steps]),
+                        centers=np.stack([st[2] for st in steps]), out=np.stack(outs), manifold_counts=np.stack(counts),
+                        tri_counts=np.stack([x[0] for x in states]), tri_lists=np.stack([x[1] for x in states]),
+                        tri_caches=np.stack([x[2] for x in states]), fat=np.stack([x[3] for x in states]))
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["pairs", "mesh", "pyramid"]
+    which = sys.argv[1:] or ["pairs", "mesh", "pyramid", "mesh_persistent"]
     if "pairs" in which:
         main()
     if "mesh" in which:
         make_mesh_fixture()
     if "pyramid" in which:
         make_pyramid_fixture()
+    if "mesh_persistent" in which:
+        make_persistent_mesh_fixture()

@@ -231,37 +231,44 @@ class RefPyramid:
 
 
 class RefMeshContacts:
-    """The reference's b3MeshContact objects kept across steps (oracle/ref_driver.cpp:
-    b3ref_mesh_contacts_*): create, Collide, SynchronizeShapes + FindNewPairs, read."""
+    """b3MeshContact objects kept across steps: create, Collide, SynchronizeShapes +
+    FindNewPairs, read.  prefix "b3ref_" drives the compiled reference's own objects
+    (oracle/ref_driver.cpp, which builds its own tree), "b3o_" the restatement (oracle/
+    b3oracle.c, which reads the tree nodes it is given)."""
 
-    def __init__(self, lib, pool, shapes, vertices, triangles, shape_a, xfs, cap=64):
+    def __init__(self, lib, pool, shapes, vertices, triangles, shape_a, xfs, cap=64, prefix="b3ref_"):
         self.lib = lib
         self.cap = cap
-        # The reference mesh keeps pointers into these arrays for the set's lifetime.
-        self._keep = [np.ascontiguousarray(vertices, np.float32), np.ascontiguousarray(triangles, np.uint32),
-                      np.ascontiguousarray(shapes), pool]
+        # The mesh keeps pointers into these arrays for the set's lifetime.
+        v = np.ascontiguousarray(vertices, np.float32)
+        t = np.ascontiguousarray(triangles, np.uint32)
+        nodes = build_tree(lib, prefix, v, t) if prefix == "b3o_" else None
+        self._keep = [v, t, np.ascontiguousarray(shapes), pool, nodes]
         self.shape_a = np.ascontiguousarray(shape_a, np.uint32)
         self.n = len(self.shape_a)
-        v, t = self._keep[0], self._keep[1]
-        self._mesh = MeshStruct(len(v), v.ctypes.data, len(t), t.ctypes.data, 0, None)
-        lib.b3ref_mesh_contacts_create.restype = ctypes.c_void_p
-        lib.b3ref_mesh_contacts_create.argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32] + [ctypes.c_void_p] * 2
-        lib.b3ref_mesh_contacts_collide.restype = None
-        lib.b3ref_mesh_contacts_collide.argtypes = [ctypes.c_void_p] * 5
-        lib.b3ref_mesh_contacts_synchronize.restype = None
-        lib.b3ref_mesh_contacts_synchronize.argtypes = [ctypes.c_void_p] * 3
-        lib.b3ref_mesh_contacts_read.restype = None
-        lib.b3ref_mesh_contacts_read.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4
-        lib.b3ref_mesh_contacts_destroy.restype = None
-        lib.b3ref_mesh_contacts_destroy.argtypes = [ctypes.c_void_p]
+        self._mesh = MeshStruct(len(v), v.ctypes.data, len(t), t.ctypes.data, 0 if nodes is None else len(nodes),
+                                None if nodes is None else nodes.ctypes.data)
+        f = {name: getattr(lib, prefix + "mesh_contacts_" + name)
+             for name in ("create", "collide", "synchronize", "read", "destroy")}
+        f["create"].restype = ctypes.c_void_p
+        f["create"].argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32] + [ctypes.c_void_p] * 2
+        f["collide"].restype = None
+        f["collide"].argtypes = [ctypes.c_void_p] * 5
+        f["synchronize"].restype = None
+        f["synchronize"].argtypes = [ctypes.c_void_p] * 3
+        f["read"].restype = None
+        f["read"].argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4
+        f["destroy"].restype = None
+        f["destroy"].argtypes = [ctypes.c_void_p]
+        self.f = f
         self._ps = pool.struct()
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
-        self.handle = lib.b3ref_mesh_contacts_create(ctypes.byref(self._ps), ptr(self._keep[2]), ctypes.byref(self._mesh),
-                                                     self.n, ptr(self.shape_a), ptr(xfs))
+        self.handle = f["create"](ctypes.byref(self._ps), ptr(self._keep[2]), ctypes.byref(self._mesh), self.n,
+                                  ptr(self.shape_a), ptr(xfs))
 
     def close(self):
         if self.handle:
-            self.lib.b3ref_mesh_contacts_destroy(self.handle)
+            self.f["destroy"](self.handle)
             self.handle = None
 
     def __del__(self):
@@ -273,18 +280,18 @@ class RefMeshContacts:
             sweeps = np.ascontiguousarray(sweeps, np.float32).reshape(self.n, 4, 4)
         out = np.zeros((self.n, 3), MANIFOLD_DTYPE)
         counts = np.zeros(self.n, np.uint32)
-        self.lib.b3ref_mesh_contacts_collide(self.handle, ptr(xfs), ptr(sweeps), ptr(out), ptr(counts))
+        self.f["collide"](self.handle, ptr(xfs), ptr(sweeps), ptr(out), ptr(counts))
         return out, counts
 
     def synchronize(self, xfs, centers0):
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
         c0 = np.ascontiguousarray(centers0, np.float32).reshape(self.n, 3)
-        self.lib.b3ref_mesh_contacts_synchronize(self.handle, ptr(xfs), ptr(c0))
+        self.f["synchronize"](self.handle, ptr(xfs), ptr(c0))
 
     def read(self):
         counts = np.zeros(self.n, np.uint32)
         lists = np.zeros((self.n, self.cap), np.uint32)
         caches = np.zeros((self.n, self.cap), CACHE_DTYPE)
         fat = np.zeros((self.n, 6), np.float32)
-        self.lib.b3ref_mesh_contacts_read(self.handle, self.cap, ptr(counts), ptr(lists), ptr(caches), ptr(fat))
+        self.f["read"](self.handle, self.cap, ptr(counts), ptr(lists), ptr(caches), ptr(fat))
         return counts, lists, caches, fat

@@ -158,3 +158,17 @@ def test_gpu_mesh_contacts_persistent_equal_reference(ctx, lib, reference):
         assert got_c[k] == want_c[k], (k, got_c[k], want_c[k])
     assert want_c["convex_cache_hits"] > 0
     ref.close()
+
+
+def test_gpu_mesh_contacts_persistent_golden(ctx):
+    """The persistent mesh-contact fixture written by the reference's own b3MeshContact,
+    replayed through b3amd_mesh_contacts_*: six steps, bit-exact."""
+    import bounce_amd
+    from util import check_persistent_mesh_fixture
+    print("case: persistent mesh contacts fixture", flush=True)
+    g = load_golden("mesh_contacts_persistent.npz")
+    ctx.upload_hulls(g["pool"])
+    ctx.upload_shapes(g["shapes"])
+    mid = ctx.upload_mesh(g["mesh_vertices"], g["mesh_triangles"])
+    with bounce_amd.MeshContacts(ctx, mid, g["shape_a"], g["xfs"][0]) as dev:
+        check_persistent_mesh_fixture(g, dev)

@@ -352,3 +352,49 @@ def test_oracle_pyramid_equals_reference_world(oracle, reference):
         assert_caches_equal(caches, r["caches_out"], "step %d" % s, HULL_CACHE_FIELDS)
         chain.scatter(ids, caches)
     world.close()
+
+
+def test_oracle_persistent_mesh_golden(oracle):
+    """Mesh contacts kept across six steps (fixture from the reference's own b3MeshContact):
+    the restatement reproduces every step's manifolds, triangle lists, caches and fat AABBs."""
+    from oracle_lib import RefMeshContacts
+    from util import check_persistent_mesh_fixture
+    g = load_golden("mesh_contacts_persistent.npz")
+    o = RefMeshContacts(oracle.lib, g["pool"], g["shapes"], g["mesh_vertices"], g["mesh_triangles"], g["shape_a"],
+                        g["xfs"][0], prefix="b3o_")
+    check_persistent_mesh_fixture(g, o)
+    o.close()
+
+
+def test_oracle_persistent_mesh_equals_reference(lib, oracle, reference):
+    """Live, eight steps of moving shapes over a 48 x 48 heightfield: restatement and compiled
+    reference agree on every manifold, triangle list, cache and fat AABB."""
+    from oracle_lib import RefMeshContacts
+    from util import mesh_workload, moving_mesh_steps
+    pool, shapes, v, t, xfs = mesh_workload(lib, 91, 48, 48, 300)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    steps = moving_mesh_steps(xfs, 9, 5)
+    r = RefMeshContacts(reference.lib, pool, shapes, v, t, sa, steps[0][0])
+    o = RefMeshContacts(oracle.lib, pool, shapes, v, t, sa, steps[0][0], prefix="b3o_")
+
+    def same(what):
+        a, b = r.read(), o.read()
+        np.testing.assert_array_equal(a[0], b[0], what)
+        np.testing.assert_array_equal(a[3].view(np.uint32), b[3].view(np.uint32), what)
+        for i in range(len(a[0])):
+            k = min(int(a[0][i]), 64)
+            np.testing.assert_array_equal(a[1][i, :k], b[1][i, :k], what)
+            assert_caches_equal(defined_hull_cache(a[2][i, :k]), defined_hull_cache(b[2][i, :k]), what)
+
+    same("after creation")
+    for s in range(8):
+        x, sw, p = steps[s]
+        a, ac = r.collide(x, sw)
+        b, bc = o.collide(x, sw)
+        np.testing.assert_array_equal(ac, bc)
+        assert_manifolds_equal(a.reshape(-1), b.reshape(-1), "step %d" % s)
+        r.synchronize(steps[s + 1][0], p)
+        o.synchronize(steps[s + 1][0], p)
+        same("after synchronize %d" % s)
+    r.close()
+    o.close()

@@ -471,3 +471,26 @@ def moving_mesh_steps(xfs0, steps, seed, fast_fraction=0.15):
         sweeps[:, 2, 3] = sweeps[:, 3, 3] = 1.0
         out.append((xfs.astype(np.float32), sweeps, pos.astype(np.float32)))
     return out
+
+
+def check_persistent_mesh_fixture(g, contacts):
+    """Replays tests/golden/mesh_contacts_persistent.npz through `contacts` (an object with
+    collide / synchronize / read, created at the fixture's first transforms) and compares
+    every step's manifolds and the triangle state after creation and every synchronize."""
+
+    def state(k, what):
+        c, l, ca, f = contacts.read()
+        np.testing.assert_array_equal(c, g["tri_counts"][k], what)
+        np.testing.assert_array_equal(f.view(np.uint32), g["fat"][k].view(np.uint32), what)
+        for i in range(len(c)):
+            m = min(int(c[i]), l.shape[1])
+            np.testing.assert_array_equal(l[i, :m], g["tri_lists"][k][i, :m], what)
+            assert_caches_equal(defined_hull_cache(ca[i, :m]), g["tri_caches"][k][i, :m], "%s contact %d" % (what, i))
+
+    state(0, "after creation")
+    for s in range(len(g["out"])):
+        out, mc = contacts.collide(g["xfs"][s], g["sweeps"][s])
+        np.testing.assert_array_equal(mc, g["manifold_counts"][s], "step %d" % s)
+        assert_manifolds_equal(out.reshape(-1), g["out"][s].reshape(-1), "step %d manifolds" % s)
+        contacts.synchronize(g["xfs"][s + 1], g["centers"][s])
+        state(s + 1, "after synchronize %d" % s)
