@@ -6,9 +6,9 @@
 // wave a static interleave of the batch, then chunks of the last sixteenth from a device
 // counter (one atomic per chunk; per-pair atomics on one address serialise).  Per pair:
 //   * face queries: the queried hull's planes are transformed in registers (face lane +
-//     64k in slot k); the other hull's vertices are staged in LDS and read in index order
-//     by every lane at once (LDS broadcast), so each lane's support search is the
-//     reference's first maximum;
+//     64k in slot k); the other hull's vertices are staged in LDS and read by every lane
+//     at once (LDS broadcast); each lane keeps the maximum support value, from which the
+//     reference's separation follows exactly;
 //   * edge query: hull 1's unique edges, rotated into hull 2's frame, sit in up to three
 //     register slots per lane as the bounding caps of their Gauss-map arcs;
 //     hull 2's negated Gauss map and arc caps, stored per hull at upload (device_hull.h),
@@ -93,15 +93,17 @@ __device__ __forceinline__ void stage_vertices(const UHull& h, SatStage& st, uin
 }
 
 // One face query (sat.cpp:33-58): planes of hull A transformed by xf (T9, geometry.h:77-81)
-// against hull B's vertices in LDS; support = first maximum in index order
-// (hull.inl:31-45); separation = b3Distance (geometry.h:90-93).
+// against hull B's vertices in LDS.  The reference takes the first vertex of maximum
+// dot(-n, v) (hull.inl:31-45) and returns b3Distance(v, plane) = dot(n, v) - d
+// (geometry.h:90-93).  Negating n negates every FP32 product and sum exactly, so
+// dot(n, v) = -max and the separation is -max - d whichever tied vertex is first: the
+// search keeps only the running maximum.
 template <int KF>
 __device__ __forceinline__ void face_query_l(const Xf& xf, const UHull& hA, uint32_t VB, const float4* vB,
                                              uint32_t lane, float& bestOut, uint32_t& keyOut) {
     Plane pl[KF];
     V3 d[KF];
     float mp[KF];
-    uint32_t mi[KF];
     const V3 w0 = ld3(vB[0]);
 #pragma unroll
     for (int k = 0; k < KF; ++k) {
@@ -110,19 +112,12 @@ __device__ __forceinline__ void face_query_l(const Xf& xf, const UHull& hA, uint
         pl[k] = mul(xf, Plane{V3{q.x, q.y, q.z}, q.w});
         d[k] = -pl[k].n;
         mp[k] = dot(d[k], w0);
-        mi[k] = 0;
     }
 #pragma unroll 4
     for (uint32_t v = 1; v < VB; ++v) {
         const V3 w = ld3(vB[v]);
 #pragma unroll
-        for (int k = 0; k < KF; ++k) {
-            const float t = dot(d[k], w);
-            if (t > mp[k]) {
-                mp[k] = t;
-                mi[k] = v;
-            }
-        }
+        for (int k = 0; k < KF; ++k) mp[k] = fmaxf(mp[k], dot(d[k], w));
     }
     float best = -kMaxFloat;
     uint32_t key = 0;
@@ -130,7 +125,7 @@ __device__ __forceinline__ void face_query_l(const Xf& xf, const UHull& hA, uint
     for (int k = 0; k < KF; ++k) {
         const uint32_t f = lane + 64u * k;
         if (f < hA.F) {
-            const float s = distance(ld3(vB[mi[k]]), pl[k]);
+            const float s = -mp[k] - pl[k].d;
             if (s > best) {
                 best = s;
                 key = f;
