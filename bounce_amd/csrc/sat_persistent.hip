@@ -12,7 +12,7 @@
 //   * edge query: hull 1's unique edges, rotated into hull 2's frame, sit in up to three
 //     register slots per lane as the bounding caps of their Gauss-map arcs;
 //     hull 2's negated Gauss map and arc caps, stored per hull at upload (device_hull.h),
-//     are staged in LDS 96 edges at a time and broadcast one edge per step.  A cheap
+//     are staged in LDS 64 edges at a time and broadcast one edge per step.  A cheap
 //     cap-overlap test selects the (i, j) worth the exact b3IsMinkowskiFace, which runs 64
 //     candidates at a time from an LDS ring; the pairs that pass go to a second ring and
 //     are projected 64 at a time by all lanes together, so b3Project's square roots and
@@ -72,6 +72,8 @@ struct UHull {
 };
 
 constexpr uint32_t kStageEdges = 96;   // hull-2 edges staged in LDS at a time (a multiple of 32)
+constexpr uint32_t kLdsVerts = 64;     // hull 1 transformed into hull 2's frame in LDS when it has
+constexpr uint32_t kLdsFaces = 128;    // at most this many vertices and faces
 constexpr uint32_t kHitCap = 128;      // rings of pending pairs: fewer than 64 + 64 at any time
 constexpr uint32_t kCandCap = 128;
 
@@ -81,6 +83,8 @@ struct SatStage {
         float4 verts[kMaxHullVertices + 1];  // face queries: the other hull's vertices
         float4 gauss[4 * kStageEdges];       // edge query: a chunk of hull 2's Gauss map
     } u;
+    float4 tv1[kLdsVerts];  // edge query: hull 1's vertices, b3Mul(xf, v)
+    float4 tn1[kLdsFaces];  // edge query: hull 1's face normals, b3Mul(xf.rotation, n)
     uint32_t hits[kHitCap];    // pending (i << 16 | j) pairs that passed the Gauss-map test
     uint32_t cands[kCandCap];  // pending (i << 16 | j) pairs whose arc caps overlap
 };
@@ -148,14 +152,14 @@ __device__ __forceinline__ void face_query_any(const Xf& xf, const UHull& hA, co
 // Projects `count` queued hits (lane l takes entry head + l) with b3Project (sat.cpp:73-108)
 // and folds them into (best, key); j is a hull-2 edge of the staged chunk starting at c0.
 __device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const UHull& h2,
-                                           CU32 ue2, const SatStage& st, uint32_t c0, uint32_t head, uint32_t count,
-                                           uint32_t lane, float& best, uint32_t& key) {
+                                           CU32 ue2, const SatStage& st, bool lds1, uint32_t c0, uint32_t head,
+                                           uint32_t count, uint32_t lane, float& best, uint32_t& key) {
     if (lane < count) {
         const uint32_t e = st.hits[(head + lane) & (kHitCap - 1u)];
         const uint32_t i = e >> 16, j = e & 0xFFFFu;
         const uint32_t u = ue1[i];
-        const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
-        const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
+        const V3 P1 = lds1 ? ld3(st.tv1[u & 0xFFu]) : mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
+        const V3 Q1 = lds1 ? ld3(st.tv1[(u >> 8) & 0xFFu]) : mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
         const V3 E1 = Q1 - P1;
         const float4 r1 = st.u.gauss[4u * (j - c0) + 1u], r2 = st.u.gauss[4u * (j - c0) + 2u];
         const V3 E2 = -V3{r1.z, r1.w, r2.x};
@@ -169,22 +173,31 @@ __device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1,
 }
 
 // Hull 1's operands of b3IsMinkowskiFace for unique edge i in hull 2's frame (sat.cpp:123-146):
-// A = U1, B = V1, nE = -E1.
-__device__ __forceinline__ void edge1_operands(const Xf& xf, const UHull& h1, CU32 ue1, uint32_t i, V3& A, V3& B,
-                                               V3& nE) {
+// A = U1, B = V1, nE = -E1; from the transformed hull in LDS when it is there (the same
+// FP32 operations, done once per pair).
+__device__ __forceinline__ void edge1_operands(const Xf& xf, const UHull& h1, CU32 ue1, const SatStage& st, bool lds1,
+                                               uint32_t i, V3& A, V3& B, V3& nE) {
     const uint32_t u = ue1[i];
-    const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
-    const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
-    A = mul(xf.R, ld3(h1.base[2 + h1.V + ((u >> 16) & 0xFFu)]));
-    B = mul(xf.R, ld3(h1.base[2 + h1.V + (u >> 24)]));
-    nE = -(Q1 - P1);
+    if (lds1) {
+        const V3 P1 = ld3(st.tv1[u & 0xFFu]);
+        const V3 Q1 = ld3(st.tv1[(u >> 8) & 0xFFu]);
+        A = ld3(st.tn1[(u >> 16) & 0xFFu]);
+        B = ld3(st.tn1[u >> 24]);
+        nE = -(Q1 - P1);
+    } else {
+        const V3 P1 = mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
+        const V3 Q1 = mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
+        A = mul(xf.R, ld3(h1.base[2 + h1.V + ((u >> 16) & 0xFFu)]));
+        B = mul(xf.R, ld3(h1.base[2 + h1.V + (u >> 24)]));
+        nE = -(Q1 - P1);
+    }
 }
 
 // Runs b3IsMinkowskiFace(U1, V1, -E1, -U2, -V2, -E2) (sat.cpp:60-71) on `count` queued
 // candidates (lane l takes entry head + l), hull 1's operands recomputed exactly as the
 // reference computes them, and queues the pairs that pass for projection.
 __device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHull& h1, CU32 ue1, const UHull& h2,
-                                                 CU32 ue2, SatStage& st, uint32_t c0, uint32_t chead,
+                                                 CU32 ue2, SatStage& st, bool lds1, uint32_t c0, uint32_t chead,
                                                  uint32_t count, uint32_t lane, uint32_t& head, uint32_t& tail,
                                                  float& best, uint32_t& key) {
     uint32_t e = 0;
@@ -192,7 +205,7 @@ __device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHul
     if (lane < count) {
         e = st.cands[(chead + lane) & (kCandCap - 1u)];
         V3 A, B, nE;
-        edge1_operands(xf, h1, ue1, e >> 16, A, B, nE);
+        edge1_operands(xf, h1, ue1, st, lds1, e >> 16, A, B, nE);
         const float4* r = st.u.gauss + 4u * ((e & 0xFFFFu) - c0);
         const float4 r0 = r[0], r1 = r[1];
         const float r2x = r[2].x;
@@ -208,7 +221,7 @@ __device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHul
     tail += (uint32_t)__popcll(m);
     if (tail - head >= 64u) {
         __syncthreads();
-        drain_hits(xf, C1, h1, ue1, h2, ue2, st, c0, head, 64u, lane, best, key);
+        drain_hits(xf, C1, h1, ue1, h2, ue2, st, lds1, c0, head, 64u, lane, best, key);
         head += 64u;
     }
 }
@@ -230,6 +243,20 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
     const CF4 g1 = h1.base + L1.gaussOff();
     const CF4 g2 = h2.base + L2.gaussOff();
     const uint32_t E2p = L2.gaussEdges();  // padded with records that never pass
+    // Hull 1 in hull 2's frame, once per pair, for the exact tests and projections.
+    const bool lds1 = h1.V <= kLdsVerts && h1.F <= kLdsFaces;
+    if (lds1) {
+        __syncthreads();
+        if (lane < h1.V) {
+            const V3 p = mul(xf, ld3(h1.base[2 + lane]));
+            st.tv1[lane] = make_float4(p.x, p.y, p.z, 0.0f);
+        }
+        for (uint32_t f = lane; f < h1.F; f += 64u) {
+            const V3 n = mul(xf.R, ld3(h1.base[2 + h1.V + f]));
+            st.tn1[f] = make_float4(n.x, n.y, n.z, 0.0f);
+        }
+        __syncthreads();
+    }
     uint32_t head = 0, tail = 0, chead = 0, ctail = 0;
     for (uint32_t i0 = 0; i0 < E1n; i0 += 64u * K) {
         // Slot k: the cap of the arc (U1, V1) of edge i0 + lane + 64k; lanes past the last
@@ -301,7 +328,7 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
                     ctail += (uint32_t)__popcll(m);
                     if (ctail - chead >= 64u) {
                         __syncthreads();
-                        drain_candidates(xf, C1, h1, ue1, h2, ue2, st, c0, chead, 64u, lane, head, tail, best, key);
+                        drain_candidates(xf, C1, h1, ue1, h2, ue2, st, lds1, c0, chead, 64u, lane, head, tail, best, key);
                         chead += 64u;
                     }
                 }
@@ -310,13 +337,13 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
             __syncthreads();
             while (ctail != chead) {
                 const uint32_t cnt = ctail - chead < 64u ? ctail - chead : 64u;
-                drain_candidates(xf, C1, h1, ue1, h2, ue2, st, c0, chead, cnt, lane, head, tail, best, key);
+                drain_candidates(xf, C1, h1, ue1, h2, ue2, st, lds1, c0, chead, cnt, lane, head, tail, best, key);
                 chead += cnt;
             }
             __syncthreads();
             while (tail != head) {
                 const uint32_t cnt = tail - head < 64u ? tail - head : 64u;
-                drain_hits(xf, C1, h1, ue1, h2, ue2, st, c0, head, cnt, lane, best, key);
+                drain_hits(xf, C1, h1, ue1, h2, ue2, st, lds1, c0, head, cnt, lane, best, key);
                 head += cnt;
             }
         }
