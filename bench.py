@@ -237,6 +237,172 @@ def mesh_run(world, steps, cpu):
     return res
 
 
+# ----------------------------------------------------------------- large warm world
+
+_WARM_JOB = {}
+WARM_CPU_STEPS = 8
+
+
+def _warm_cpu_worker(chunk):
+    """The reference's chain over steps 0..WARM_CPU_STEPS-1 for the contacts in `chunk`;
+    returns (cold-step seconds, warm-step seconds, last step's manifolds and caches)."""
+    j = _WARM_JOB
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from bounce_amd.types import empty_caches
+    from bounce_amd.workloads import warm_world_pair_inputs
+    chk = oracle_lib.reference() if j["kind"] == "reference" else oracle_lib.oracle()
+    w = j["world"]
+    contacts = w["contacts"][chunk]
+    caches = empty_caches(len(contacts))
+    inputs = [warm_world_pair_inputs(w, xfs, sw, contacts) for xfs, sw in j["states"]]
+    chk.collide(w["pool"], w["shapes"], inputs[0][0][:8], inputs[0][1][:8])  # load the library
+    cold = warm = 0.0
+    for s, (pairs, pxf, psw) in enumerate(inputs):
+        t0 = time.perf_counter()
+        out = chk.collide(w["pool"], w["shapes"], pairs, pxf, sweeps=psw, caches=caches)
+        dt = time.perf_counter() - t0
+        if s == 0:
+            cold += dt
+        else:
+            warm += dt
+    return cold, warm, out, caches
+
+
+def warm_cpu_baseline(world, stride):
+    """The reference (oracle/_ref; the oracle restatement where that is absent) stepping
+    every `stride`-th contact of the warm world through WARM_CPU_STEPS steps with its
+    caches carried, one process per core.  Runs before the GPU is initialised (forks)."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from bounce_amd.workloads import warm_world_state
+    kind = "reference" if oracle_lib.reference() is not None else "port"
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    idx = np.arange(0, len(world["contacts"]), stride)
+    _WARM_JOB.update(kind=kind, world=world, states=[warm_world_state(world, s) for s in range(WARM_CPU_STEPS)])
+    with mp.get_context("fork").Pool(cores) as p:
+        res = p.map(_warm_cpu_worker, np.array_split(idx, cores), chunksize=1)
+    cold = sum(r[0] for r in res)
+    warm = sum(r[1] for r in res)
+    out = np.concatenate([r[2] for r in res])
+    caches = np.concatenate([r[3] for r in res])
+    return {"value": len(idx) * (WARM_CPU_STEPS - 1) / (warm / cores), "unit": "contacts/s (warm steps)",
+            "cores": cores, "kind": kind,
+            "cold_step_contacts_per_s": len(idx) / (cold / cores),
+            "sample": "every %d-th contact (%d of %d), steps 0-%d with caches carried, %d processes, "
+                      "%.1f CPU-s" % (stride, len(idx), len(world["contacts"]), WARM_CPU_STEPS - 1, cores,
+                                      cold + warm)}, idx, out, caches
+
+
+def warm_world_run(world, steps, cpu):
+    """The large warm world: `steps` world steps (60 Hz) of b3amd_update_contacts on
+    device-resident body state, each contact's feature cache kept on the device from step
+    to step.  The per-step body states (the motion repeats every `period` steps, so
+    period + 1 distinct states) are uploaded before the timed run."""
+    import bounce_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from util import defined_hull_cache, HULL_CACHE_FIELDS
+    from bounce_amd.types import CACHE_DTYPE, MANIFOLD_DTYPE, empty_caches
+    from bounce_amd.workloads import warm_world_pair_inputs, warm_world_state, warm_world_state_index
+    w = world
+    B, n = len(w["body_shapes"]), len(w["contacts"])
+    nstates = w["period"] + 1
+    res = {"workload": "%d hulls of 8-64 points (semi-axes U[0.25,0.5] m, uniform orientations) in %d x %d "
+                       "stacks of up to %d in a 50 x 50 m container (floor and 4 walls); contact list = fat-AABB "
+                       "overlaps (0.2 m); synthetic motion: oscillation of 2 mm / 0.005 rad (10%%: 2 cm / "
+                       "0.05 rad) at 0.5-2 Hz, period %d steps" % (B - w["statics"], w["columns"], w["columns"],
+                                                                    w["layers"], w["period"]),
+           "bodies": B, "contacts": n, "steps": steps}
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(w["pool"])
+        ctx.upload_shapes(w["shapes"])
+        d_shapes, d_contacts = ctx.alloc(4 * B), ctx.alloc(8 * n)
+        d_caches, d_out = ctx.alloc(CACHE_DTYPE.itemsize * n), ctx.alloc(MANIFOLD_DTYPE.itemsize * n)
+        d_shapes.upload(w["body_shapes"])
+        d_contacts.upload(w["contacts"])
+        d_x, d_s = ctx.alloc(48 * B * nstates), ctx.alloc(32 * B * nstates)
+        for i in range(nstates):
+            xfs, sw = warm_world_state(w, i)
+            d_x.view(48 * B * i, 48 * B).upload(xfs)
+            d_s.view(32 * B * i, 32 * B).upload(sw)
+
+        def step(s):
+            i = warm_world_state_index(w, s)
+            ctx.update_contacts(B, d_shapes, d_x.view(48 * B * i, 48 * B), n, d_contacts, d_out,
+                                d_body_sweeps=d_s.view(32 * B * i, 32 * B), d_caches=d_caches)
+
+        errs = []
+
+        def fp64_error(s):
+            out = d_out.download(np.zeros(n, MANIFOLD_DTYPE))
+            xfs, sw = warm_world_state(w, s)
+            errs.append((out, warm_world_pair_inputs(w, xfs, sw)[1]))
+            return out
+
+        # Parity pass: the first WARM_CPU_STEPS steps, compared with the reference's chain; the
+        # statistics counters (b3_convexCalls, b3_convexCacheHits) count its warm steps.
+        d_caches.upload(empty_caches(n))
+        step(0)
+        ctx.synchronize()
+        fp64_error(0)
+        ctx.set_counting(True)
+        ctx.reset_counters()
+        for s in range(1, WARM_CPU_STEPS):
+            step(s)
+        ctx.synchronize()
+        warm = ctx.counters()
+        ctx.set_counting(False)
+        if cpu is not None:
+            stats, idx, want_out, want_caches = cpu
+            out = d_out.download(np.zeros(n, MANIFOLD_DTYPE))[idx]
+            caches = d_caches.download(np.zeros(n, CACHE_DTYPE))[idx]
+            man_eq = (out.view(np.uint8).reshape(len(idx), -1) == want_out.view(np.uint8).reshape(len(idx), -1)).all(1)
+            dc, dw = defined_hull_cache(caches), defined_hull_cache(want_caches)
+            cache_eq = np.all([dc[f] == dw[f] for f in HULL_CACHE_FIELDS], axis=0)
+            res["sampled_parity"] = {"contacts": int(len(idx)), "step": WARM_CPU_STEPS - 1,
+                                     "manifolds_equal": int(man_eq.sum()), "caches_equal": int(cache_eq.sum()),
+                                     "against": stats["kind"]}
+            res["cpu_baseline"] = stats
+        # Timed run from empty caches: the cold step, then the warm steps.
+        d_caches.upload(empty_caches(n))
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        step(0)
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        t2 = time.perf_counter()
+        for s in range(1, steps):
+            step(s)
+        ctx.synchronize()
+        t3 = time.perf_counter()
+        ctx.set_profiling(False)
+        kern = {k: ctx.kernel_stats(k) for k in ("gather", "cache_hull", "sat_hull", "contact_hull")}
+        last = fp64_error(steps - 1)
+    warm_s = t3 - t2
+    calls, hits = warm["convex_calls"], warm["convex_cache_hits"]
+    res.update({
+        "cold_step_ms": (t1 - t0) * 1000.0,
+        "warm_ms_per_step": warm_s * 1000.0 / (steps - 1),
+        "warm_contacts_per_s": n * (steps - 1) / warm_s,
+        "warm_steps_per_s": (steps - 1) / warm_s,
+        "cache_hit_fraction": hits / max(1, calls),
+        "cache_hit_fraction_steps": [1, WARM_CPU_STEPS - 1],
+        "kernels_ms": {k: v[1] / max(1, v[0]) for k, v in kern.items()},
+        "touching_fraction_last_step": float((last["pointCount"] > 0).mean()),
+        "points_last_step": int(last["pointCount"].sum()),
+        "what": "b3amd_update_contacts per step on device-resident body transforms and sweeps; caches and "
+                "manifolds stay in HBM; statistics counters off in the timed steps"})
+    outs = np.concatenate([o for o, _ in errs])
+    xfs = np.concatenate([x for _, x in errs])
+    res["separation_error_fp64"] = separation_error_fp64(outs, xfs)
+    res["separation_error_fp64"]["steps"] = [0, steps - 1]
+    if "cpu_baseline" in res:
+        res["gpu_over_cpu"] = res["warm_contacts_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
 # ----------------------------------------------------------------- C1: the box pyramid
 
 def pyramid_run(lib):
@@ -358,6 +524,8 @@ def main():
     ap.add_argument("--no-pyramid", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
     ap.add_argument("--no-plain", action="store_true", help="skip the host-array (PCIe) drop-in timing")
+    ap.add_argument("--no-warm", action="store_true", help="skip the large warm world")
+    ap.add_argument("--warm-steps", type=int, default=600)
     ap.add_argument("--serving-worlds", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=240000.0)
     ap.add_argument("--serving-seconds", type=float, default=0.5)
@@ -390,6 +558,16 @@ def main():
         if not args.no_cpu:  # also before the GPU is initialised
             mesh_cpu = mesh_cpu_baseline(*mesh_world, 100000)
             log("mesh cpu baseline: %.0f contacts/s (%s)" % (mesh_cpu["value"], mesh_cpu["kind"]))
+    warm_world = warm_cpu = None
+    if not args.no_warm:
+        from bounce_amd.workloads import gen_warm_world
+        t = time.perf_counter()
+        warm_world = gen_warm_world(lib, 11)
+        log("warm world: %d bodies, %d contacts in %.1f s"
+            % (len(warm_world["body_shapes"]), len(warm_world["contacts"]), time.perf_counter() - t))
+        if not args.no_cpu:  # also before the GPU is initialised
+            warm_cpu = warm_cpu_baseline(warm_world, 4)
+            log("warm world cpu baseline: %.0f contacts/s warm (%s)" % (warm_cpu[0]["value"], warm_cpu[0]["kind"]))
     work = algorithmic_work(pool, pairs, xfs, max(1, len(pairs) // 8192))
     log("algorithmic work: %.3g SAT flops, %.3g SAT bytes per batch" % (work["sat_flops"], work["sat_bytes"]))
 
@@ -500,6 +678,12 @@ def main():
         log("pyramid: %d/%d steps equal to the reference, %.3f ms per step"
             % (result["pyramid"]["steps_equal_to_reference"], result["pyramid"]["steps"],
                result["pyramid"]["ms_per_step"]))
+    if not args.no_warm:
+        result["warm_world"] = warm_world_run(warm_world, args.warm_steps, warm_cpu)
+        log("warm world: %.0f contacts/s over %d warm steps (%.3f ms per step), cold step %.3f ms, "
+            "cache hits %.3f" % (result["warm_world"]["warm_contacts_per_s"], args.warm_steps - 1,
+                                 result["warm_world"]["warm_ms_per_step"], result["warm_world"]["cold_step_ms"],
+                                 result["warm_world"]["cache_hit_fraction"]))
     if not args.no_serving:
         result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     print(json.dumps(result))

@@ -46,6 +46,7 @@ def load_library():
         "b3amd_upload_shapes": (i32, [vp, u32, vp]),
         "b3amd_collide": (i32, [vp, u32, vp, vp, vp, vp, vp]),
         "b3amd_collide_device": (i32, [vp, u32, vp, vp, vp, vp, vp]),
+        "b3amd_update_contacts": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, vp]),
         "b3amd_synchronize": (i32, [vp]),
         "b3amd_device_alloc": (i32, [vp, u64, ctypes.POINTER(vp)]),
         "b3amd_device_free": (i32, [vp, vp]),
@@ -121,6 +122,25 @@ class DeviceBuffer:
         if self.ptr is not None and self.ctx.handle is not None:
             _check(self.ctx.lib.b3amd_device_free(self.ctx.handle, self.ptr))
         self.ptr = None
+
+    def view(self, offset, nbytes):
+        """A byte range of this buffer, usable wherever a DeviceBuffer is (not freeable)."""
+        if offset < 0 or nbytes < 0 or offset + nbytes > self.nbytes:
+            raise ValueError("view [%d, %d) outside a buffer of %d bytes" % (offset, offset + nbytes, self.nbytes))
+        return DeviceView(self, offset, nbytes)
+
+
+class DeviceView(DeviceBuffer):
+    """A byte range inside a DeviceBuffer; the parent owns the memory."""
+
+    def __init__(self, parent, offset, nbytes):
+        self.ctx = parent.ctx
+        self.parent = parent
+        self.nbytes = int(nbytes)
+        self.ptr = ctypes.c_void_p(parent.ptr.value + int(offset))
+
+    def free(self):
+        raise TypeError("a DeviceView does not own its memory")
 
 
 class Graph:
@@ -372,6 +392,15 @@ class Context:
         p = lambda b: None if b is None else b.ptr
         _check(self.lib.b3amd_collide_device(self.handle, n, p(d_pairs), p(d_xfs), p(d_sweeps), p(d_caches),
                                              p(d_out)))
+
+    def update_contacts(self, body_count, d_body_shapes, d_body_xfs, contact_count, d_contacts, d_out,
+                        d_body_sweeps=None, d_caches=None):
+        """Enqueue one world step from device-resident body state (b3amd_update_contacts):
+        contact i joins bodies d_contacts[2i], d_contacts[2i+1]."""
+        p = lambda b: None if b is None else b.ptr
+        _check(self.lib.b3amd_update_contacts(self.handle, body_count, p(d_body_shapes), p(d_body_xfs),
+                                              p(d_body_sweeps), contact_count, p(d_contacts), p(d_caches),
+                                              p(d_out)))
 
     def synchronize(self):
         _check(self.lib.b3amd_synchronize(self.handle))

@@ -72,9 +72,9 @@ int b3amd::ensure_scratch(b3amd_context* ctx, uint32_t n) {
     ctx->dSat = nullptr;
     ctx->dCounter = nullptr;
     ctx->scratchCap = 0;
-    if ((rc = ctx->alloc(sizeof(uint32_t) * (uint64_t)cap, (void**)&ctx->dStage))) return rc;
+    if ((rc = ctx->alloc(sizeof(uint32_t) * scratch_words(cap), (void**)&ctx->dStage))) return rc;
     if ((rc = ctx->alloc(sizeof(SatResult) * (uint64_t)cap, (void**)&ctx->dSat))) return rc;
-    if ((rc = ctx->alloc(256, (void**)&ctx->dCounter))) return rc;
+    if ((rc = ctx->alloc(4 * kCountWords, (void**)&ctx->dCounter))) return rc;
     ctx->scratchCap = cap;
     return B3AMD_OK;
 }
@@ -211,7 +211,7 @@ int validate_pairs(const b3amd_context* ctx, uint32_t n, const uint32_t* pairs) 
 
 int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs,
                          const float* sweeps, ConvexCache* caches, Manifold* out, const BatchScratch& sc,
-                         const uint32_t* slots) {
+                         const uint32_t* slots, const BodyRefs* bodies) {
     int rc = B3AMD_OK;
     PairInputs in;
     in.n = n;
@@ -224,26 +224,33 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     in.hullCount = ctx->hullCount;
     in.slots = slots;
     in.counters = ctx->counting ? ctx->dCounters : nullptr;
+    in.bodies = bodies ? bodies->bodies : nullptr;
+    in.bodyXfs = bodies ? bodies->xfs : nullptr;
+    in.bodySweeps = bodies ? bodies->sweeps : nullptr;
+    // sc.stage holds scratch_words(capacity) words for a capacity >= n.
     CacheArgs ca;
     static_cast<PairInputs&>(ca) = in;
-    ca.satCounter = sc.counter;
+    ca.counts = sc.counter;
     ca.sweeps = sweeps;
     ca.caches = caches;
     ca.stage = sc.stage;
+    ca.satList = sc.stage + n;
+    ca.rebuildList = sc.stage + 2 * (size_t)n;
     ca.out = out;
     ca.sat = sc.sat;
     SatArgs sa;
     static_cast<PairInputs&>(sa) = in;
-    sa.stage = sc.stage;
+    sa.list = ca.satList;
+    sa.listCount = sc.counter + kCountSat;
     sa.out = sc.sat;
-    sa.split = SatSplit{sc.counter, 0u, 0u};
+    sa.split = SatSplit{sc.counter + kCountSatTail, 0u};
     ContactArgs cta;
     static_cast<PairInputs&>(cta) = in;
     cta.sat = sc.sat;
     cta.caches = caches;
     cta.out = out;
-    cta.list = sc.stage;  // the stage words are dead once the SAT kernel has read them
-    cta.listCount = sc.counter + 1;
+    cta.list = sc.stage;  // the stage words are dead once the cache kernels are done
+    cta.listCount = sc.counter + kCountContacts;
     hipStream_t s = ctx->stream;
     if (ctx->roundShapes) {
         // Sphere/capsule pairs depend on nothing the hull kernels write.
@@ -299,8 +306,8 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
         b3amd_destroy(ctx);
         return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
     }
-    if (ctx->alloc(sizeof(Counters), (void**)&ctx->dCounters) != B3AMD_OK ||
-        hipMemset(ctx->dCounters, 0, sizeof(Counters)) != hipSuccess) {
+    if (ctx->alloc(sizeof(Counters) * kCounterSlots, (void**)&ctx->dCounters) != B3AMD_OK ||
+        hipMemset(ctx->dCounters, 0, sizeof(Counters) * kCounterSlots) != hipSuccess) {
         std::string msg = b3amd_last_error();
         b3amd_destroy(ctx);
         return fail(B3AMD_ERR_DEVICE, "counter allocation failed: %s", msg.c_str());
@@ -428,6 +435,44 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
     if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
     return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
                      reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
+}
+
+int b3amd_update_contacts(b3amd_context* ctx, uint32_t body_count, const uint32_t* d_body_shapes,
+                          const b3amd_transform* d_body_xfs, const b3amd_quat* d_body_sweeps,
+                          uint32_t contact_count, const uint32_t* d_contacts, b3amd_convex_cache* d_caches,
+                          b3amd_manifold* d_out) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    if (contact_count == 0) return B3AMD_OK;
+    if (!d_body_shapes || !d_body_xfs || !d_contacts || !d_out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (d_body_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
+    if ((reinterpret_cast<uintptr_t>(d_body_xfs) | reinterpret_cast<uintptr_t>(d_body_sweeps)) & 15u)
+        return fail(B3AMD_ERR_ARGUMENT, "body transforms and sweeps must be 16-byte aligned");
+    if (!ctx->dPool) return fail(B3AMD_ERR_ARGUMENT, "no hull pool uploaded");
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    int rc;
+    // The contacts' shape pairs (8 B per contact); transforms and sweeps stay per body.
+    if (contact_count > ctx->gatherCap) {
+        const uint64_t cap = contact_count < 1024u ? 1024u : contact_count;
+        if ((rc = ctx->release(ctx->dGather))) return rc;
+        ctx->dGather = nullptr;
+        ctx->gatherCap = 0;
+        if ((rc = ctx->alloc(8 * cap, &ctx->dGather))) return rc;
+        ctx->gatherCap = (uint32_t)cap;
+    }
+    if ((rc = ensure_scratch(ctx, contact_count))) return rc;
+    BodyGatherArgs g;
+    g.n = contact_count;
+    g.bodyCount = body_count;
+    g.bodyShapes = d_body_shapes;
+    g.contacts = d_contacts;
+    g.pairs = static_cast<uint32_t*>(ctx->dGather);
+    hipStream_t s = ctx->stream;
+    if ((rc = timed(ctx, s, "gather", [&] { return launch_body_gather(g, s); }))) return rc;
+    const BodyRefs bodies{d_contacts, reinterpret_cast<const float*>(d_body_xfs),
+                          reinterpret_cast<const float*>(d_body_sweeps)};
+    return enqueue_batch(ctx, contact_count, g.pairs, nullptr, nullptr, reinterpret_cast<ConvexCache*>(d_caches),
+                         reinterpret_cast<Manifold*>(d_out), BatchScratch{ctx->dStage, ctx->dSat, ctx->dCounter},
+                         nullptr, &bodies);
 }
 
 int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t max_slots) {
@@ -913,9 +958,9 @@ int b3amd_graph_create(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, 
     g->ctx = ctx;
     g->generation = ctx->generation;
     int rc;
-    if ((rc = ctx->alloc(sizeof(uint32_t) * (uint64_t)n, (void**)&g->scratch.stage)) ||
+    if ((rc = ctx->alloc(sizeof(uint32_t) * scratch_words(n), (void**)&g->scratch.stage)) ||
         (rc = ctx->alloc(sizeof(SatResult) * (uint64_t)n, (void**)&g->scratch.sat)) ||
-        (rc = ctx->alloc(256, (void**)&g->scratch.counter))) {
+        (rc = ctx->alloc(4 * kCountWords, (void**)&g->scratch.counter))) {
         ctx->release(g->scratch.stage);
         ctx->release(g->scratch.sat);
         delete g;
@@ -988,14 +1033,25 @@ int b3amd_set_counting(b3amd_context* ctx, int on) {
 int b3amd_get_counters(b3amd_context* ctx, b3amd_counters* out) {
     if (!ctx || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     static_assert(sizeof(b3amd_counters) == sizeof(Counters), "counter layout");
-    HIP_TRY(hipMemcpyAsync(out, ctx->dCounters, sizeof(Counters), hipMemcpyDeviceToHost, ctx->stream));
+    Counters slots[kCounterSlots];
+    HIP_TRY(hipMemcpyAsync(slots, ctx->dCounters, sizeof(slots), hipMemcpyDeviceToHost, ctx->stream));
     HIP_TRY(hipStreamSynchronize(ctx->stream));
+    Counters t{};
+    for (const Counters& c : slots) {
+        t.convexCalls += c.convexCalls;
+        t.convexCacheHits += c.convexCacheHits;
+        t.gjkCalls += c.gjkCalls;
+        t.gjkIters += c.gjkIters;
+        t.gjkMaxIters = c.gjkMaxIters > t.gjkMaxIters ? c.gjkMaxIters : t.gjkMaxIters;
+    }
+    std::memcpy(out, &t, sizeof(t));
     return B3AMD_OK;
 }
 
 int b3amd_reset_counters(b3amd_context* ctx) {
     if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
-    HIP_TRY(launch_zero_words(reinterpret_cast<uint32_t*>(ctx->dCounters), sizeof(Counters) / 4, ctx->stream));
+    HIP_TRY(launch_zero_words(reinterpret_cast<uint32_t*>(ctx->dCounters), sizeof(Counters) * kCounterSlots / 4,
+                              ctx->stream));
     return B3AMD_OK;
 }
 

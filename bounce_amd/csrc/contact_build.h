@@ -339,16 +339,22 @@ __device__ inline void build_face_contact(Contact& m, const Xf& xf1, uint32_t in
     m.pointCount = red.count;
 }
 
+// The rotation test of the cached face rebuild (collide_hulls_cache.cpp:117-126): the
+// relative rotation of the bodies since the last step is small.
+__device__ inline bool rebuild_face_allowed(Quat q10, Quat q1, Quat q20, Quat q2) {
+    Quat dq0 = qmul(conj(q10), q20);
+    Quat dq = qmul(conj(q1), q2);
+    Quat q = qmul(conj(dq0), dq);
+    const float kTol = 0.995f;
+    return fabsf(q.w) > kTol;
+}
+
 // collide_hulls_cache.cpp:113-153
 template <class H1, class H2>
 __device__ inline void rebuild_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1, float r1,
                                      const Xf& xf2, const H2& h2, float r2, bool flip,
                                      Quat q10, Quat q1, Quat q20, Quat q2) {
-    Quat dq0 = qmul(conj(q10), q20);
-    Quat dq = qmul(conj(q1), q2);
-    Quat q = qmul(conj(dq0), dq);
-    const float kTol = 0.995f;
-    if (fabsf(q.w) > kTol) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
+    if (rebuild_face_allowed(q10, q1, q20, q2)) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
 }
 
 // b3FeatureCache::ReadFace (sat.cpp:209-222)

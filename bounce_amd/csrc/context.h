@@ -37,7 +37,16 @@ struct PendingTiming {
     hipEvent_t start, stop;
 };
 
-// Per-batch device scratch: stage and SAT result per pair, the SAT kernel's tail counter.
+// Body-level inputs of a world step (b3amd_update_contacts): contact i joins bodies
+// bodies[2i], bodies[2i + 1]; the kernels read the body transforms and sweeps directly.
+struct BodyRefs {
+    const uint32_t* bodies;  // [n][2]
+    const float* xfs;        // [B][12]
+    const float* sweeps;     // [B][8] or null
+};
+
+// Per-batch device scratch: scratch_words(capacity) words (stage, SAT list, rebuild lists),
+// the SAT result per pair, and the counter words (kCountWords).
 struct BatchScratch {
     uint32_t* stage;
     SatResult* sat;
@@ -81,6 +90,9 @@ struct b3amd_context {
     bool counting = false;
     void* dIn = nullptr;
     uint64_t dInCap = 0;
+    // Shape pairs of the contacts of a world step (b3amd_update_contacts).
+    void* dGather = nullptr;
+    uint32_t gatherCap = 0;
 
     // Meshes (b3Mesh with its static tree) and mesh-contact scratch.
     struct MeshRecord {
@@ -171,7 +183,8 @@ int ensure_scratch(b3amd_context* ctx, uint32_t n);
 // Enqueues the kernels of one batch on the context stream (no synchronisation); slots maps
 // each pair to its cache/manifold slot (null: slot p).
 int enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs, const float* sweeps,
-                  ConvexCache* caches, Manifold* out, const BatchScratch& sc, const uint32_t* slots);
+                  ConvexCache* caches, Manifold* out, const BatchScratch& sc, const uint32_t* slots,
+                  const BodyRefs* bodies = nullptr);
 
 }  // namespace b3amd
 

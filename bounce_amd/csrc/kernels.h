@@ -64,6 +64,14 @@ struct Counters {
     unsigned int pad;
 };
 
+// The totals are kept in kCounterSlots copies, block b adding into copy b % kCounterSlots, so
+// that the per-wave atomics of a large batch do not all meet on one address; the host sums
+// the copies (and takes the largest gjkMaxIters).
+constexpr uint32_t kCounterSlots = 64;
+__device__ __forceinline__ Counters* counter_slot(Counters* base) {
+    return base + (blockIdx.x & (kCounterSlots - 1u));
+}
+
 struct PairInputs {
     uint32_t n;
     const uint32_t* pairs;   // [n][2] shape indices
@@ -75,8 +83,18 @@ struct PairInputs {
     uint32_t hullCount;
     const uint32_t* slots;   // [n] cache and manifold slot of each pair; null: slot p
     Counters* counters;      // statistics totals, or null when counting is off
+    // Body-level inputs (a world step): when bodies is set, pair p joins bodies
+    // bodies[2p], bodies[2p + 1] and its transforms (and sweeps) are read from the body
+    // arrays instead of xfs (and the per-pair sweeps).
+    const uint32_t* bodies = nullptr;    // [n][2] or null
+    const float* bodyXfs = nullptr;      // [B][12]
+    const float* bodySweeps = nullptr;   // [B][8] (orientation0, orientation) or null
 
     __device__ uint32_t slot_of(uint32_t p) const { return slots ? slots[p] : p; }
+    // The 12 floats of pair p's transform of side 0 (shape A) or 1 (shape B).
+    __device__ const float* xf_ptr(uint32_t p, uint32_t side) const {
+        return bodies ? bodyXfs + 12 * (size_t)bodies[2 * p + side] : xfs + 24 * (size_t)p + 12 * side;
+    }
 
     // 0: a pair kernels must not read (shape or hull index out of range, a type outside
     // sphere/capsule/hull, or typeA > typeB, which b3ContactManager::Create never makes,
@@ -95,27 +113,76 @@ struct PairInputs {
 constexpr uint32_t kFlagInvalidPair = 2u;
 constexpr uint32_t kStageConvex = 5u;  // sphere/capsule pair: handled by the convex kernel
 
-// Work split of the persistent SAT kernel: wave w of G takes pairs w, w + G, ... below
-// staticEnd (an interleave gives every wave the same mix of hull sizes), then chunks of
-// `chunk` pairs from a device counter until the batch is done.  Only the tail is handed
-// out dynamically: device-scope atomics on one address serialise across the XCDs.
+// Work split of the persistent SAT kernel over its list of L pairs: wave w of G takes list
+// entries w, w + G, ... below staticEnd = L - max(L / 16, 2 chunk G) (an interleave gives
+// every wave the same mix of hull sizes), then chunks of `chunk` entries (1 when L < chunk G)
+// from a device counter until the list is done.  Only the tail is handed out dynamically:
+// device-scope atomics on one address serialise across the XCDs.
 struct SatSplit {
-    uint32_t* counter;       // pairs handed out past staticEnd; zeroed before each batch
-    uint32_t staticEnd;
+    uint32_t* counter;       // list entries handed out past the static part; zeroed before each batch
     uint32_t chunk;
 };
 
+// Per-batch scratch words: the stage word of each pair, the SAT list (pair indices) and the
+// rebuild lists.  The rebuild lists are kRebuildLists sub-lists, one counter each, so that
+// the per-wave appends of a large batch spread over many addresses: wave w of the cache-read
+// kernel (16 pairs) appends to sub-list w % kRebuildLists, whose capacity is
+// rebuild_capacity(n).  The counter words (zeroed at the start of each batch): the SAT
+// kernel's tail counter, the contact list's length, the SAT list's length, and from
+// kCountRebuild the sub-lists' lengths.
+constexpr uint32_t kRebuildLists = 64u;
+constexpr uint32_t kReadPairsPerWave = 16u;
+__host__ __device__ inline uint32_t rebuild_capacity(uint32_t n) {
+    const uint32_t waves = (n + kReadPairsPerWave - 1u) / kReadPairsPerWave;
+    return (waves + kRebuildLists - 1u) / kRebuildLists * kReadPairsPerWave;
+}
+enum : uint32_t {
+    kCountSatTail = 0, kCountContacts = 1, kCountSat = 2, kCountRebuild = 64,
+    kCountWords = kCountRebuild + kRebuildLists
+};
+// Scratch words for a capacity of n pairs: stage, SAT list, rebuild lists.
+inline uint64_t scratch_words(uint64_t n) {
+    return 2u * n + (uint64_t)kRebuildLists * rebuild_capacity((uint32_t)n);
+}
+
 struct CacheArgs : PairInputs {
-    uint32_t* satCounter;    // the SAT kernel's tail counter and the overlap-list count, zeroed here
-    const float* sweeps;     // [n][4][4] or null
+    uint32_t* counts;        // [kCountWords], zeroed before the cache kernels
+    const float* sweeps;     // [n][4][4] or null (per-pair sweeps; body sweeps come with bodies)
     ConvexCache* caches;     // [n] or null
     uint32_t* stage;         // [n] out
+    uint32_t* satList;       // [n] out: pairs that need the SAT
+    uint32_t* rebuildList;   // [kRebuildLists][rebuild_capacity(n)] out: cached contacts to rebuild
     Manifold* out;           // [n]
     SatResult* sat;          // [n] out: stage kStageSkip for pairs that need no SAT
+
+    // Pair p's sweep orientations (orientation0, orientation of body A, then of body B);
+    // identities when no sweeps were given.
+    __device__ void sweep(uint32_t p, Quat& q10, Quat& q1, Quat& q20, Quat& q2) const {
+        const float4 one = make_float4(0.f, 0.f, 0.f, 1.f);
+        float4 s0 = one, s1 = one, s2 = one, s3 = one;
+        if (sweeps) {
+            const float4* w = reinterpret_cast<const float4*>(sweeps + 16 * (size_t)p);
+            s0 = w[0], s1 = w[1], s2 = w[2], s3 = w[3];
+        } else if (bodies && bodySweeps) {
+            const float4* wa = reinterpret_cast<const float4*>(bodySweeps + 8 * (size_t)bodies[2 * p]);
+            const float4* wb = reinterpret_cast<const float4*>(bodySweeps + 8 * (size_t)bodies[2 * p + 1]);
+            s0 = wa[0], s1 = wa[1], s2 = wb[0], s3 = wb[1];
+        }
+        q10 = Quat{s0.x, s0.y, s0.z, s0.w};
+        q1 = Quat{s1.x, s1.y, s1.z, s1.w};
+        q20 = Quat{s2.x, s2.y, s2.z, s2.w};
+        q2 = Quat{s3.x, s3.y, s3.z, s3.w};
+    }
 };
 
+// Stage words of the cache kernels: a hull pair with a cached feature to read, and one whose
+// rebuilt clip polygon outgrows the group kernel (the lane-per-pair cache kernel takes it).
+constexpr uint32_t kStageCached = 7u;
+constexpr uint32_t kStageCacheFallback = 8u;
+
 struct SatArgs : PairInputs {
-    const uint32_t* stage;   // [n] from the cache kernel
+    const uint32_t* list;    // pairs to run, *listCount of them
+    const uint32_t* listCount;
     SatResult* out;          // [n]
     SatSplit split;
 };
@@ -201,8 +268,24 @@ hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
 hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
 
+// The shape pairs of one world step's contacts: contact i joins bodies contacts[2i],
+// contacts[2i+1] (the kernels read their transforms and sweeps through PairInputs::bodies).
+// A body index outside the world gives an invalid shape pair, which no kernel reads further.
+struct BodyGatherArgs {
+    uint32_t n;
+    uint32_t bodyCount;
+    const uint32_t* bodyShapes;  // [bodyCount]
+    const uint32_t* contacts;    // [n][2]
+    uint32_t* pairs;             // [n][2] out: shape indices
+};
+hipError_t launch_body_gather(const BodyGatherArgs& a, hipStream_t s);
+
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s);
+// Zeroes the counter words, marks the cached pairs and lists the SAT-bound ones, reads the
+// feature caches four lanes per cached pair (listing the contacts to rebuild), rebuilds them
+// eight lanes per pair, and resolves the pairs whose rebuild outgrows the group on one lane
+// each.
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s);
 // Persistent SAT over the pairs the cache kernel marked; any hull size within the record
 // limits.  Fills a.split (the counter must be zero).

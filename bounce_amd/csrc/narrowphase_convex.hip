@@ -31,8 +31,8 @@ __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
         Manifold* const out = a.out + a.slot_of(p);
         const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
         const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];
-        const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
-        const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+        const Xf xf1 = load_xf(a.xf_ptr(p, 0));
+        const Xf xf2 = load_xf(a.xf_ptr(p, 1));
         Contact m;
         m.pointCount = 0;
         m.overflow = false;
@@ -52,9 +52,10 @@ __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
         iters = m.gjkIters;
     }
     if (a.counters) {  // all lanes of the block reach here
-        wave_count(&a.counters->gjkCalls, calls);
-        wave_count(&a.counters->gjkIters, iters);
-        wave_max(&a.counters->gjkMaxIters, iters);
+        Counters* const c = counter_slot(a.counters);
+        wave_count(&c->gjkCalls, calls);
+        wave_count(&c->gjkIters, iters);
+        wave_max(&c->gjkMaxIters, iters);
     }
 }
 

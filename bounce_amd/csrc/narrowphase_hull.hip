@@ -56,8 +56,8 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
     }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
-    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
-    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const Xf xf1 = load_xf(a.xf_ptr(p, 0));
+    const Xf xf2 = load_xf(a.xf_ptr(p, 1));
     const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
     const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
 
@@ -252,7 +252,7 @@ __device__ inline uint32_t reduce_quad(const CPV* in, uint32_t n, V3 N, uint32_t
 // b3BuildFaceContact (collide_hulls.cpp:93-204) by the group; returns the point count, or
 // -1 if a polygon outgrew the group's capacity.  With points it writes the manifold
 // (lanes 0-3 one point each, lane 0 the header).
-__device__ int group_face_contact(GroupLds& L, uint32_t g, uint32_t gbase, Manifold* out, const Xf& xf1,
+__device__ __forceinline__ int group_face_contact(GroupLds& L, uint32_t g, uint32_t gbase, Manifold* out, const Xf& xf1,
                                   uint32_t index1, const DevHull& h1, float r1, const Xf& xf2, const DevHull& h2,
                                   float r2, bool flip) {
     const float totalRadius = r1 + r2;
@@ -406,6 +406,26 @@ __device__ int group_face_contact(GroupLds& L, uint32_t g, uint32_t gbase, Manif
     return (int)nq;
 }
 
+// Field-wise selections: a reference chosen at run time between two locals would keep both
+// in private memory; selecting the values keeps them in registers.
+__device__ __forceinline__ V3 pick(bool c, V3 a, V3 b) { return V3{c ? b.x : a.x, c ? b.y : a.y, c ? b.z : a.z}; }
+__device__ __forceinline__ Xf pick(bool c, const Xf& a, const Xf& b) {
+    Xf r;
+    r.R.c0 = pick(c, a.R.c0, b.R.c0);
+    r.R.c1 = pick(c, a.R.c1, b.R.c1);
+    r.R.c2 = pick(c, a.R.c2, b.R.c2);
+    r.p = pick(c, a.p, b.p);
+    return r;
+}
+__device__ __forceinline__ DevHull pick(bool c, const DevHull& a, const DevHull& b) {
+    DevHull r;
+    r.base = c ? b.base : a.base;
+    r.V = c ? b.V : a.V;
+    r.F = c ? b.F : a.F;
+    r.HE = c ? b.HE : a.HE;
+    return r;
+}
+
 // The separated exits of b3CollideCache (collide_hulls_cache.cpp:155-275) for every pair the
 // SAT ran on, one lane per pair; the overlapping pairs are appended to a list (one atomic
 // per block) so that the group kernel's waves hold only pairs that build a contact.
@@ -463,8 +483,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
     const bool lead = g == 0;
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
-    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
-    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const Xf xf1 = load_xf(a.xf_ptr(p, 0));
+    const Xf xf2 = load_xf(a.xf_ptr(p, 1));
     const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
     const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
 
@@ -530,8 +550,8 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
     const float totalRadius = r1 + r2;
-    const Xf xf1 = load_xf(a.xfs + 24 * (size_t)p);
-    const Xf xf2 = load_xf(a.xfs + 24 * (size_t)p + 12);
+    const Xf xf1 = load_xf(a.xf_ptr(p, 0));
+    const Xf xf2 = load_xf(a.xf_ptr(p, 1));
     const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
     const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
     const uint32_t state0 = fc->state, type = fc->type, i1 = fc->index1, i2 = fc->index2;
@@ -557,14 +577,8 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
         return kStageSkip;
     }
     if (state0 == kCacheOverlap && state1 == kCacheOverlap) {
-        Quat q10{0, 0, 0, 1}, q1{0, 0, 0, 1}, q20{0, 0, 0, 1}, q2{0, 0, 0, 1};
-        if (a.sweeps) {
-            const float* s = a.sweeps + 16 * (size_t)p;
-            q10 = Quat{s[0], s[1], s[2], s[3]};
-            q1 = Quat{s[4], s[5], s[6], s[7]};
-            q20 = Quat{s[8], s[9], s[10], s[11]};
-            q2 = Quat{s[12], s[13], s[14], s[15]};
-        }
+        Quat q10, q1, q20, q2;
+        a.sweep(p, q10, q1, q20, q2);
         switch (type) {
             case kFeatureEdge1: build_edge_contact(m, xf1, i1, h1, xf2, i2, h2, true); break;
             case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2); break;
@@ -580,15 +594,49 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
     return kStageOverlap;
 }
 
-// Every destination is addressed once, before the divergent branches, and the stage words
-// are stored unconditionally at the end, so no store address is a merge of per-path
-// values.
-// With counting on, b3_convexCalls counts the hull pairs and b3_convexCacheHits the ones
-// the cache resolved (collide_hulls.cpp:304, collide_hulls_cache.cpp:300,334).
-__global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
+// Appends the flagged pairs of a 256-lane block to a list with one atomic per block (waves
+// in block order, lanes in lane order).  Every lane of the block calls it; sh holds 8 words.
+__device__ __forceinline__ void block_append(bool flag, uint32_t p, uint32_t* list, uint32_t* count, uint32_t* sh) {
+    const uint32_t wave = threadIdx.x >> 6;
+    const uint64_t m = __ballot(flag);
+    if ((threadIdx.x & 63u) == 0) sh[wave] = (uint32_t)__popcll(m);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const uint32_t total = sh[0] + sh[1] + sh[2] + sh[3];
+        uint32_t base = total ? atomicAdd(count, total) : 0u;
+        for (uint32_t w = 0; w < 4u; ++w) {
+            sh[4 + w] = base;
+            base += sh[w];
+        }
+    }
+    __syncthreads();
+    if (flag) list[sh[4 + wave] + mbcnt(m)] = p;
+    __syncthreads();
+}
+
+// Appends the flagged pairs of one wave to a list with one atomic (lane 0 of the wave must
+// be active; every lane of the wave calls it).
+__device__ __forceinline__ void wave_append(bool flag, uint32_t p, uint32_t* list, uint32_t* count) {
+    const uint64_t m = __ballot(flag);
+    if (!m) return;
+    uint32_t base = 0;
+    if ((threadIdx.x & 63u) == 0) base = atomicAdd(count, (uint32_t)__popcll(m));
+    base = (uint32_t)__shfl((int)base, 0, 64);
+    if (flag) list[base + mbcnt(m)] = p;
+}
+
+// The start of b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-297), one
+// lane per pair: invalid pairs get an empty manifold and the invalid flag, sphere/capsule
+// pairs go to the convex kernel, hull pairs with a cached feature are marked kStageCached
+// and the others (no cache, an empty one, or a state outside the enum) go to the SAT list
+// with their cache flushed.  ReadState on an empty cache evaluates face 0 of hull 1 and the
+// result is discarded by the flush (sat.cpp:182-185), so those go straight to the SAT.
+// With counting on, b3_convexCalls counts the hull pairs (collide_hulls.cpp:304).
+__global__ __launch_bounds__(256) void cache_list_kernel(CacheArgs a) {
+    __shared__ uint32_t sh[8];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p == 0) a.satCounter[0] = a.satCounter[1] = 0u;  // the SAT and contact kernels run after this one
-    uint32_t call = 0, hit = 0;
+    bool toSat = false;
+    uint32_t call = 0;
     if (p < a.n) {
         const uint32_t slot = a.slot_of(p);
         Manifold* const out = a.out + slot;
@@ -596,23 +644,247 @@ __global__ __launch_bounds__(256) void cache_hull_kernel(CacheArgs a) {
         uint32_t* const stage = a.stage + p;
         SatResult* const sat = a.sat + p;
         const uint32_t cls = a.classify(p);
-        const uint32_t st = cache_pair(a, p, cls, out, fc);
-        *stage = st;
-        sat->stage = st == kStageOverlap ? kStageOverlap : kStageSkip;
-        call = cls == 1u ? 1u : 0u;
-        hit = (cls == 1u && st == kStageSkip) ? 1u : 0u;
+        bool cached = false;
+        if (cls == 0u) {
+            Contact m;
+            m.pointCount = 0;
+            m.overflow = false;
+            write_manifold(out, m);
+            out->flags = kFlagInvalidPair;
+        } else if (cls == 1u) {
+            call = 1u;
+            cached = fc && fc->state < kCacheEmpty;
+            toSat = !cached;
+            if (fc && toSat) fc->state = kCacheEmpty;
+        }
+        *stage = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
+        sat->stage = toSat ? kStageOverlap : kStageSkip;
     }
-    if (a.counters) {  // all lanes of the block reach here
-        wave_count(&a.counters->convexCalls, call);
-        wave_count(&a.counters->convexCacheHits, hit);
+    block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCalls, call);
+}
+
+// ReadFace (sat.cpp:209-222) by a group of G lanes: the support search over hull 2's
+// vertices strided across the lanes.  Only the maximum of dot(-n, v) is needed: the
+// reference's separation dot(n, v*) - d equals -max - d exactly, since negating n negates
+// every product and sum exactly, whichever of tied maxima the reference picks.
+template <uint32_t G>
+__device__ __forceinline__ uint32_t group_read_face(uint32_t g, uint32_t index1, const Xf& xf1, const DevHull& h1,
+                                                    const Xf& xf2, const DevHull& h2, float totalRadius) {
+    const Xf xf = mulT(xf2, xf1);
+    const Plane pl = mul(xf, h1.plane(index1));
+    const V3 d = -pl.n;
+    float mp = -__builtin_inff();
+    for (uint32_t k = g; k < h2.V; k += G) mp = fmaxf(mp, dot(d, h2.vertex(k)));
+#pragma unroll
+    for (uint32_t off = 1; off < G; off <<= 1) mp = fmaxf(mp, __shfl_xor(mp, (int)off, (int)G));
+    const float sep = -mp - pl.d;
+    return sep > totalRadius ? kCacheSeparation : kCacheOverlap;
+}
+
+// An empty manifold (all 240 bytes zero, as write_manifold writes pointCount 0) stored by a
+// group of G lanes, one 16-byte word per lane and store.
+template <uint32_t G>
+__device__ __forceinline__ void group_write_empty(Manifold* out, uint32_t g) {
+    static_assert(sizeof(Manifold) == 15 * 16, "manifold of 15 16-byte words");
+    float4* const w = reinterpret_cast<float4*>(out);
+    const float4 z = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+    for (uint32_t k = 0; k < 15u; k += G)
+        if (k + g < 15u) w[k + g] = z;
+}
+
+enum : uint32_t { kCachedHit = 0, kCachedMiss = 1, kCachedFallback = 2, kCachedRebuild = 3 };
+
+// The inputs of one cached pair.
+struct CachedPair {
+    Xf xf1, xf2;
+    DevHull h1, h2;
+    float r1, r2;
+    uint32_t state0, type, i1, i2;
+    __device__ static CachedPair load(const CacheArgs& a, uint32_t p, const FeatureCache* fc) {
+        CachedPair c;
+        const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
+        c.r1 = a.shapes[sa].radius;
+        c.r2 = a.shapes[sb].radius;
+        c.xf1 = load_xf(a.xf_ptr(p, 0));
+        c.xf2 = load_xf(a.xf_ptr(p, 1));
+        c.h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
+        c.h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+        c.state0 = fc->state;
+        c.type = fc->type;
+        c.i1 = fc->index1;
+        c.i2 = fc->index2;
+        return c;
     }
+    // A cached feature outside its hull cannot come from this pair.
+    __device__ bool in_range() const {
+        return (type == kFeatureEdge1 && i1 + 1 < h1.HE && i2 + 1 < h2.HE && !(i1 & 1u) && !(i2 & 1u)) ||
+               (type == kFeatureFace1 && i1 < h1.F) || (type == kFeatureFace2 && i1 < h2.F);
+    }
+};
+
+// b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the rebuilds,
+// for one cached pair, by a group of G lanes: ReadState (sat.cpp:177-278), the separation
+// hit (empty manifold written here), and the decision to rebuild: an overlap hit needs the
+// edge rebuild, or the face rebuild when the relative rotation allows it
+// (collide_hulls_cache.cpp:117-126; hull 2's face swaps the bodies, :325-329).
+template <uint32_t G>
+__device__ __forceinline__ uint32_t cached_read(const CacheArgs& a, uint32_t g, uint32_t p, Manifold* out,
+                                                const FeatureCache* fc) {
+    const CachedPair c = CachedPair::load(a, p, fc);
+    if (!c.in_range()) return kCachedMiss;
+    const float totalRadius = c.r1 + c.r2;
+    uint32_t state1;
+    if (c.type == kFeatureEdge1) {
+        state1 = read_edge(c.i1, c.i2, c.xf1, c.h1, c.xf2, c.h2, totalRadius);
+    } else {
+        const bool two = c.type == kFeatureFace2;
+        state1 = group_read_face<G>(g, c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2), pick(two, c.xf2, c.xf1),
+                                    pick(two, c.h2, c.h1), totalRadius);
+    }
+    if (c.state0 == kCacheSeparation && state1 == kCacheSeparation) {
+        group_write_empty<G>(out, g);
+        return kCachedHit;
+    }
+    if (c.state0 != kCacheOverlap || state1 != kCacheOverlap) return kCachedMiss;
+    if (c.type == kFeatureEdge1) return kCachedRebuild;
+    Quat q10, q1, q20, q2;
+    a.sweep(p, q10, q1, q20, q2);
+    const bool flip = c.type == kFeatureFace2;
+    const bool allowed = flip ? rebuild_face_allowed(q20, q2, q10, q1) : rebuild_face_allowed(q10, q1, q20, q2);
+    return allowed ? kCachedRebuild : kCachedMiss;
+}
+
+// The cached rebuilds (collide_hulls_cache.cpp:35-153) of one pair by its group of eight:
+// the edge contact on every lane, or the face contact with the clip polygon across the group.
+// Writes the manifold on a hit.
+__device__ __forceinline__ uint32_t cached_rebuild(const CacheArgs& a, GroupLds& L, uint32_t g, uint32_t gbase,
+                                                   uint32_t p, Manifold* out, const FeatureCache* fc) {
+    const CachedPair c = CachedPair::load(a, p, fc);
+    if (c.type == kFeatureEdge1) {
+        Contact m;
+        m.pointCount = 0;
+        m.overflow = false;
+        build_edge_contact(m, c.xf1, c.i1, c.h1, c.xf2, c.i2, c.h2, true);
+        if (m.pointCount == 0) return kCachedMiss;
+        if (g == 0) write_manifold(out, m);
+        return kCachedHit;
+    }
+    const bool flip = c.type == kFeatureFace2;
+    const Xf xa = pick(flip, c.xf1, c.xf2), xb = pick(flip, c.xf2, c.xf1);
+    const DevHull ha = pick(flip, c.h1, c.h2), hb = pick(flip, c.h2, c.h1);
+    const int n = group_face_contact(L, g, gbase, out, xa, c.i1, ha, flip ? c.r2 : c.r1, xb, hb, flip ? c.r1 : c.r2,
+                                     flip);
+    if (n < 0) return kCachedFallback;
+    return n > 0 ? kCachedHit : kCachedMiss;
+}
+
+constexpr uint32_t kReadGroup = 4;
+static_assert(64u / kReadGroup == kReadPairsPerWave, "cache-read wave width");
+
+// The pairs marked kStageCached, four lanes per pair (the grid covers the batch): the
+// feature-cache read.  Separation hits are finished here; overlap hits go to the rebuild
+// lists; a miss flushes the cache and appends the pair to the SAT list.  With counting on,
+// b3_convexCacheHits counts the separation hits (collide_hulls_cache.cpp:300).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void cache_read_kernel(CacheArgs a) {
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t p = t / kReadGroup, g = threadIdx.x & (kReadGroup - 1u);
+    bool miss = false, hit = false, rebuild = false;
+    if (p < a.n && a.stage[p] == kStageCached) {
+        const uint32_t slot = a.slot_of(p);
+        Manifold* const out = a.out + slot;
+        FeatureCache* const fc = &a.caches[slot].feature;
+        const uint32_t r = cached_read<kReadGroup>(a, g, p, out, fc);
+        const bool lead = g == 0;
+        hit = lead && r == kCachedHit;
+        miss = lead && r == kCachedMiss;
+        rebuild = lead && r == kCachedRebuild;
+        if (miss) fc->state = kCacheEmpty;
+    }
+    wave_append(miss, p, a.satList, a.counts + kCountSat);
+    const uint32_t k = (t / 64u) % kRebuildLists;
+    wave_append(rebuild, p, a.rebuildList + (size_t)k * rebuild_capacity(a.n), a.counts + kCountRebuild + k);
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);
+}
+
+// The rebuild lists, eight lanes per pair.  Group q of the grid's G groups takes entry
+// i = q / kRebuildLists of sub-list q % kRebuildLists, then q + G, ..., until i passes the
+// longest sub-list.  A miss flushes the cache and appends the pair to the SAT list; a face
+// rebuild whose polygon outgrows the group marks the pair for the lane-per-pair kernel.
+// With counting on, b3_convexCacheHits counts the overlap hits
+// (collide_hulls_cache.cpp:334).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void cache_group_kernel(CacheArgs a) {
+    __shared__ GroupLds lds[256 / kGroup];
+    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t g = threadIdx.x & (kGroup - 1u);
+    const uint32_t gbase = threadIdx.x & 63u & ~(kGroup - 1u);
+    GroupLds& L = lds[threadIdx.x / kGroup];
+    const uint32_t groups = gridDim.x * (blockDim.x / kGroup);
+    const uint32_t cap = rebuild_capacity(a.n);
+    uint32_t longest = 0;
+    for (uint32_t k = g; k < kRebuildLists; k += kGroup) longest = max(longest, a.counts[kCountRebuild + k]);
+#pragma unroll
+    for (uint32_t off = 1; off < kGroup; off <<= 1) longest = max(longest, (uint32_t)__shfl_xor((int)longest, (int)off, (int)kGroup));
+    uint32_t hits = 0;
+    for (uint32_t q = t / kGroup;; q += groups) {
+        const uint32_t i = q / kRebuildLists, k = q % kRebuildLists;
+        if (i >= longest) break;
+        if (i >= a.counts[kCountRebuild + k]) continue;
+        const uint32_t p = a.rebuildList[(size_t)k * cap + i];
+        const uint32_t slot = a.slot_of(p);
+        Manifold* const out = a.out + slot;
+        FeatureCache* const fc = &a.caches[slot].feature;
+        uint32_t* const stage = a.stage + p;
+        const uint32_t r = cached_rebuild(a, L, g, gbase, p, out, fc);
+        if (g == 0) {
+            hits += r == kCachedHit ? 1u : 0u;
+            if (r == kCachedMiss) {
+                fc->state = kCacheEmpty;
+                a.satList[atomicAdd(a.counts + kCountSat, 1u)] = p;
+            }
+            *stage = r == kCachedFallback ? kStageCacheFallback : kStageSkip;
+        }
+    }
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hits);
+}
+
+// The pairs the group kernel marked kStageCacheFallback, one lane each, with the
+// reference's 32-vertex clip polygon.
+__global__ __launch_bounds__(256) void cache_fallback_kernel(CacheArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    bool miss = false, hit = false;
+    if (p < a.n && a.stage[p] == kStageCacheFallback) {
+        const uint32_t slot = a.slot_of(p);
+        Manifold* const out = a.out + slot;
+        FeatureCache* const fc = &a.caches[slot].feature;
+        const uint32_t st = cache_pair(a, p, 1u, out, fc);
+        hit = st == kStageSkip;
+        miss = !hit;
+    }
+    wave_append(miss, p, a.satList, a.counts + kCountSat);
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);
 }
 
 // ------------------------------------------------------------------ launchers
 
 hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    cache_hull_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    hipError_t e = launch_zero_words(a.counts, kCountWords, s);
+    if (e != hipSuccess) return e;
+    cache_list_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if (!a.caches) return hipSuccess;  // nothing is cached
+    // The grids cover the batch; the stage words select the pairs.
+    const uint64_t readThreads = (uint64_t)a.n * kReadGroup;
+    cache_read_kernel<<<(uint32_t)((readThreads + 255) / 256), 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // A grid of at most 2048 blocks (65,536 groups) walks the rebuild lists.
+    uint64_t blocks = ((uint64_t)a.n * kGroup + 255) / 256;
+    if (blocks > 2048u) blocks = 2048u;
+    cache_group_kernel<<<(uint32_t)blocks, 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    cache_fallback_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -273,11 +273,12 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
         iters = m.gjkIters;
     }
     if (a.counters) {  // all lanes of the block reach here
-        wave_count(&a.counters->convexCalls, hullCalls);
-        wave_count(&a.counters->convexCacheHits, hits);
-        wave_count(&a.counters->gjkCalls, calls);
-        wave_count(&a.counters->gjkIters, iters);
-        wave_max(&a.counters->gjkMaxIters, iters);
+        Counters* const c = counter_slot(a.counters);
+        wave_count(&c->convexCalls, hullCalls);
+        wave_count(&c->convexCacheHits, hits);
+        wave_count(&c->gjkCalls, calls);
+        wave_count(&c->gjkIters, iters);
+        wave_max(&c->gjkMaxIters, iters);
     }
 }
 

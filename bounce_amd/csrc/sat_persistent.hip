@@ -359,8 +359,7 @@ __device__ __forceinline__ void sat_pair(const SatArgs& a, uint32_t p, SatStage&
     const uint32_t sa = pairs[2 * p], sb = pairs[2 * p + 1];
     const uint32_t ha = shapes[12 * sa + 1], hb = shapes[12 * sb + 1];
     const float totalRadius = __uint_as_float(shapes[12 * sa + 2]) + __uint_as_float(shapes[12 * sb + 2]);
-    const CF32 xp = (const CF32)a.xfs + 24 * (size_t)p;
-    const Xf xf1 = load_xf_u(xp), xf2 = load_xf_u(xp + 12);
+    const Xf xf1 = load_xf_u((const CF32)a.xf_ptr(p, 0)), xf2 = load_xf_u((const CF32)a.xf_ptr(p, 1));
     const CF4 pool{(CF32)a.pool};
     const CU32 offsets = (const CU32)a.offsets;
     const UHull h1 = UHull::at(pool, offsets, ha);
@@ -401,25 +400,30 @@ __device__ __forceinline__ void sat_pair(const SatArgs& a, uint32_t p, SatStage&
     if (lane == 0) a.out[p] = r;
 }
 
-// The pairs the cache kernel marked for a SAT: a static interleave up to staticEnd, then
-// chunks from the tail counter (every wave leaves once the counter passes the batch).
+// The pairs the cache kernels listed for a SAT: a static interleave up to staticEnd, then
+// chunks from the tail counter (every wave leaves once the counter passes the list).
 // W is the register budget: the compiler fits the kernel into 512 / W VGPRs so that W
 // waves share a SIMD.
 template <int W>
 __global__ __launch_bounds__(64, W) void sat_persistent_kernel(SatArgs a, uint32_t maxSlots) {
     __shared__ SatStage st;
     const uint32_t lane = threadIdx.x;
-    const CU32 stage = (const CU32)a.stage;
-    for (uint32_t p = blockIdx.x; p < a.split.staticEnd; p += gridDim.x)
-        if (stage[p] == kStageOverlap) sat_pair(a, p, st, lane, maxSlots);
+    const CU32 list = (const CU32)a.list;
+    const uint32_t count = *a.listCount < a.n ? *a.listCount : a.n;
+    // A short list (fewer than `chunk` pairs per wave) goes out one pair at a time.
+    const uint32_t chunk = count >= a.split.chunk * gridDim.x ? a.split.chunk : 1u;
+    uint32_t tail = count / 16u;
+    if (tail < 2u * chunk * gridDim.x) tail = 2u * chunk * gridDim.x;
+    if (tail > count) tail = count;
+    const uint32_t staticEnd = count - tail;
+    for (uint32_t i = blockIdx.x; i < staticEnd; i += gridDim.x) sat_pair(a, list[i], st, lane, maxSlots);
     for (;;) {
         uint32_t t = 0;
-        if (lane == 0) t = atomicAdd(a.split.counter, a.split.chunk);
-        const uint32_t p0 = a.split.staticEnd + (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
-        if (p0 >= a.n) break;
-        const uint32_t p1 = a.n - p0 < a.split.chunk ? a.n : p0 + a.split.chunk;
-        for (uint32_t p = p0; p < p1; ++p)
-            if (stage[p] == kStageOverlap) sat_pair(a, p, st, lane, maxSlots);
+        if (lane == 0) t = atomicAdd(a.split.counter, chunk);
+        const uint32_t i0 = staticEnd + (uint32_t)__builtin_amdgcn_readlane((int)t, 0);
+        if (i0 >= count) break;
+        const uint32_t i1 = count - i0 < chunk ? count : i0 + chunk;
+        for (uint32_t i = i0; i < i1; ++i) sat_pair(a, list[i], st, lane, maxSlots);
     }
 }
 
@@ -455,12 +459,8 @@ hipError_t launch_sat_w(SatArgs a, uint32_t wavesPerSimd, uint32_t slots, hipStr
     if (4u * wavesPerSimd < blocksPerCu) blocksPerCu = 4u * wavesPerSimd;
     uint64_t blocks = (uint64_t)cus * blocksPerCu;
     if (a.n < blocks) blocks = a.n;
-    // The last sixteenth of the batch (at least two chunks per wave) goes out dynamically.
+    // The last sixteenth of the list (at least two chunks per wave) goes out dynamically.
     a.split.chunk = 8u;
-    uint64_t tail = a.n / 16u;
-    if (tail < 2u * a.split.chunk * blocks) tail = 2u * a.split.chunk * blocks;
-    if (tail > a.n) tail = a.n;
-    a.split.staticEnd = a.n - (uint32_t)tail;
     sat_persistent_kernel<W><<<(uint32_t)blocks, 64, 0, s>>>(a, slots);
     return hipGetLastError();
 }

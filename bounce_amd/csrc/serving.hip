@@ -91,6 +91,22 @@ __global__ __launch_bounds__(256) void gather_kernel(GatherArgs a) {
     a.slots[k] = d.slotBase + i;
 }
 
+// One lane per contact.
+__global__ __launch_bounds__(256) void body_gather_kernel(BodyGatherArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= a.n) return;
+    const uint32_t ba = a.contacts[2 * k], bb = a.contacts[2 * k + 1];
+    const bool ok = ba < a.bodyCount && bb < a.bodyCount;
+    const uint2 sh = ok ? make_uint2(a.bodyShapes[ba], a.bodyShapes[bb]) : make_uint2(0xFFFFFFFFu, 0xFFFFFFFFu);
+    reinterpret_cast<uint2*>(a.pairs)[k] = sh;
+}
+
+hipError_t launch_body_gather(const BodyGatherArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    body_gather_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
 }  // namespace b3amd
 
 namespace {
@@ -431,9 +447,9 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
     A(96 * P, (void**)&s->dXfs);
     A(64 * P, (void**)&s->dSweeps);
     A(4 * P, (void**)&s->dSlots);
-    A(4 * P, (void**)&s->scratch.stage);
+    A(4 * scratch_words(P), (void**)&s->scratch.stage);
     A(sizeof(SatResult) * P, (void**)&s->scratch.sat);
-    A(256, (void**)&s->scratch.counter);
+    A(4 * kCountWords, (void**)&s->scratch.counter);
     for (int i = 0; i < kMaxInflight && rc == B3AMD_OK; ++i) {
         Staging& st = s->staging[i];
         if (hipHostMalloc((void**)&st.bodyXfs, 48 * B, 0) != hipSuccess ||

@@ -291,3 +291,22 @@ extern "C" int b3amd_gen_hull_pairs(uint64_t seed, const b3amd_hull_pool* pool, 
     for (auto& th : threads) th.join();
     return B3AMD_OK;
 }
+
+extern "C" int b3amd_gen_place_along(const b3amd_hull_pool* pool, uint32_t n, const uint32_t* ha, const float* ra,
+                                     const uint32_t* hb, const float* rb, const double* dirs, const double* gaps,
+                                     double* t) {
+    if (!pool || (n && (!ha || !ra || !hb || !rb || !dirs || !gaps || !t))) return B3AMD_ERR_ARGUMENT;
+    for (uint32_t i = 0; i < n; ++i)
+        if (ha[i] >= pool->count || hb[i] >= pool->count) return B3AMD_ERR_ARGUMENT;
+    unsigned nth = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    std::vector<std::thread> threads;
+    for (unsigned k = 0; k < nth; ++k) {
+        threads.emplace_back([=]() {
+            for (uint32_t i = k; i < n; i += nth)
+                t[i] = place_along(pool, ha[i], ra + 9 * (size_t)i, hb[i], rb + 9 * (size_t)i, dirs + 3 * (size_t)i,
+                                   gaps[i]);
+        });
+    }
+    for (auto& th : threads) th.join();
+    return B3AMD_OK;
+}

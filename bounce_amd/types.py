@@ -39,6 +39,9 @@ CACHE_DTYPE = np.dtype({
 })
 
 SPHERE, CAPSULE, HULL, MESH = 0, 1, 2, 3
+# Manifold status bits (B3AMD_FLAG_* of bounce_amd.h).
+FLAG_POLY_OVERFLOW, FLAG_INVALID_PAIR, FLAG_TRIANGLE_OVERFLOW = 1, 2, 4
+
 CACHE_SEPARATION, CACHE_OVERLAP, CACHE_EMPTY = 0, 1, 2
 FEATURE_EDGE1, FEATURE_FACE1, FEATURE_FACE2 = 0, 1, 2
 NULL_KEY = 0xFFFFFFFF
@@ -114,6 +117,14 @@ class HullPool:
             fo += len(f)
         return HullPool(np.array(desc, np.uint32), np.array(cs, np.float32), np.concatenate(vs),
                         np.concatenate(es), np.concatenate(fs), np.concatenate(ps))
+
+    @staticmethod
+    def join(a, b):
+        """Pool holding a's hulls, then b's (indices of b shift by a.count)."""
+        shift = np.array([len(a.vertices), 0, len(a.edges), 0, len(a.faces), 0], np.uint32)
+        return HullPool(np.concatenate([a.desc, b.desc + shift]), np.concatenate([a.centroids, b.centroids]),
+                        np.concatenate([a.vertices, b.vertices]), np.concatenate([a.edges, b.edges]),
+                        np.concatenate([a.faces, b.faces]), np.concatenate([a.planes, b.planes]))
 
     def shapes(self, radius=0.0):
         """One b3HullShape record per pool hull (shape index == hull index)."""

@@ -43,6 +43,26 @@ def gen_hull_pairs(lib, seed, pool, n, gap_mean=0.0, gap_sigma=0.05):
     return pairs, xfs
 
 
+def place_along(lib, pool, ha, ra, hb, rb, dirs, gaps):
+    """b3amd_gen_place_along: per i, the distance along dirs[i] at which hull hb[i]
+    (rotation columns rb[i]) has SAT signed separation gaps[i] from hull ha[i] (rotation
+    ra[i]) at the origin."""
+    n = len(ha)
+    ha, hb = np.ascontiguousarray(ha, np.uint32), np.ascontiguousarray(hb, np.uint32)
+    ra, rb = np.ascontiguousarray(ra, np.float32).reshape(n, 9), np.ascontiguousarray(rb, np.float32).reshape(n, 9)
+    dirs = np.ascontiguousarray(dirs, np.float64).reshape(n, 3)
+    gaps = np.ascontiguousarray(gaps, np.float64).reshape(n)
+    t = np.zeros(n, np.float64)
+    f = lib.b3amd_gen_place_along
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 7
+    ps = pool.struct()
+    rc = f(ctypes.byref(ps), n, ptr(ha), ptr(ra), ptr(hb), ptr(rb), ptr(dirs), ptr(gaps), ptr(t))
+    if rc != 0:
+        raise RuntimeError("b3amd_gen_place_along failed: %d" % rc)
+    return t
+
+
 def _rotations(rng, n, tilt_sigma=0.05):
     """n rotation matrices: uniform yaw about y with small tilts about x and z."""
     yaw = rng.uniform(0.0, 2.0 * np.pi, n)
@@ -243,3 +263,203 @@ def gen_mesh_world(lib, seed, H=256, W=256, n_shapes=100000, cell=0.5, amplitude
     xfs[:, 0, 9:12] = np.stack([x, y, z], -1)
     xfs[:, 1, 0], xfs[:, 1, 4], xfs[:, 1, 8] = 1.0, 1.0, 1.0
     return pool, shapes, verts, tris, xfs
+
+
+def quat_xf_rotation(q):
+    """b3QuatMat33 (quat.h:306-318) in float32 and its operation order: the rotation columns
+    [n][9] of unit quaternions [n][4] (x, y, z, w), as b3Body::SynchronizeTransform sets
+    m_xf.rotation from the sweep orientation."""
+    q = np.asarray(q, np.float32)
+    x, y, z, w = q[:, 0], q[:, 1], q[:, 2], q[:, 3]
+    x2, y2, z2 = x + x, y + y, z + z
+    xx, xy, xz = x * x2, x * y2, x * z2
+    yy, yz, zz = y * y2, y * z2, z * z2
+    wx, wy, wz = w * x2, w * y2, w * z2
+    one = np.float32(1.0)
+    return np.stack([one - (yy + zz), xy + wz, xz - wy,
+                     xy - wz, one - (xx + zz), yz + wx,
+                     xz + wy, yz - wx, one - (xx + yy)], -1)
+
+
+def _quat_mul(a, b):
+    ax, ay, az, aw = a[..., 0], a[..., 1], a[..., 2], a[..., 3]
+    bx, by, bz, bw = b[..., 0], b[..., 1], b[..., 2], b[..., 3]
+    return np.stack([aw * bx + ax * bw + ay * bz - az * by,
+                     aw * by - ax * bz + ay * bw + az * bx,
+                     aw * bz + ax * by - ay * bx + az * bw,
+                     aw * bw - ax * bx - ay * by - az * bz], -1)
+
+
+def gen_warm_world(lib, seed, n_hulls=200000, columns=62, half_width=25.0, amin=0.25, amax=0.5,
+                   gap_mean=-0.005, gap_sigma=0.005, pitch_gap=0.05, extension=0.2, settling=0.1, hz=60.0,
+                   period=120):
+    """One large world for warm-started steps: n_hulls distinct hulls of 8-64 points (on
+    ellipsoids with semi-axes U[amin, amax] m, uniform orientations) packed in layers of
+    columns x columns into a container of 2 half_width x 2 half_width m (static boxes:
+    floor, body 0, top at y = 0; walls at x = -/+half_width, bodies 1-2, and z =
+    -/+half_width, bodies 3-4).  Lattice cell (i, j, k) holds body 5 + k columns^2 +
+    j columns + i.  The bodies of a column (i, j) form a stack: each rests on the one
+    below at SAT signed separation g ~ N(gap_mean, gap_sigma), placed along +y by
+    b3amd_gen_place_along (on the floor: its lowest vertex at height g).  Columns stand on a
+    square grid whose pitch is the mean lateral support width plus pitch_gap, from the -x
+    and -z walls.
+
+    The contact list is every pair whose AABBs, extended by `extension` on each side (the
+    broadphase's fat AABBs, settings.h:54), overlap, dynamic-dynamic and dynamic-static,
+    in (lower body, higher body) order sorted by that pair, the order b3BroadPhase's sorted
+    pair buffer reports them in (broad_phase.h:160-208).
+
+    Motion (warm_world_state): each body oscillates from its place, a fraction `settling`
+    by up to ~2 cm and 0.05 rad, the rest by ~2 mm and 0.005 rad, at 1-4 times
+    hz / period Hz (0.5-2 Hz at 60 Hz, period 120), so the motion repeats every `period`
+    steps and warm_world_state needs only period + 1 distinct states.  The pile is
+    synthetic: no host solver runs here.
+
+    Returns a dict: pool, shapes (one per body), body_shapes [B] u32, contacts [P][2] u32,
+    pos0 [B][3] f64, q0 [B][4] f64, stack_gaps [n_hulls] (each hull's drawn separation from
+    what it rests on) and the motion parameters."""
+    from .shapes import box_hull
+    from .types import SHAPE_DTYPE, HULL
+    rng = np.random.default_rng(seed)
+    hulls = gen_hull_pool(lib, seed ^ 0xC5C5, n_hulls, 8, 64, amin, amax)
+    layers = -(-n_hulls // (columns * columns))
+    wall_half = 0.5 * (layers * 2.0 * amax + 2.0)
+    H = half_width
+    statics = [((H + 1.0, 0.5, H + 1.0), (0.0, -0.5, 0.0)),
+               ((0.5, wall_half, H + 1.0), (-H - 0.5, wall_half, 0.0)),
+               ((0.5, wall_half, H + 1.0), (H + 0.5, wall_half, 0.0)),
+               ((H + 1.0, wall_half, 0.5), (0.0, wall_half, -H - 0.5)),
+               ((H + 1.0, wall_half, 0.5), (0.0, wall_half, H + 0.5))]
+    ns = len(statics)
+    pool = HullPool.join(HullPool.concat([box_hull(*e) for e, _ in statics]), hulls)
+    B = ns + n_hulls
+    shapes = np.zeros(B, SHAPE_DTYPE)
+    shapes["type"] = HULL
+    shapes["hull"] = np.arange(B, dtype=np.uint32)
+    body_shapes = np.arange(B, dtype=np.uint32)
+
+    # Uniform orientations as quaternions (x, y, z, w); statics at identity.
+    q0 = np.zeros((B, 4))
+    q0[:, 3] = 1.0
+    qd = rng.normal(size=(n_hulls, 4))
+    q0[ns:] = qd / np.linalg.norm(qd, axis=1, keepdims=True)
+    # Support extents of each rotated hull along +-x, +-y, +-z.
+    R = quat_xf_rotation(q0[ns:].astype(np.float32)).astype(np.float64).reshape(-1, 3, 3)  # [n][col][row]
+    d = hulls.desc.astype(np.int64)
+    hi = np.zeros((n_hulls, 3))
+    lo = np.zeros((n_hulls, 3))
+    for c0 in range(0, n_hulls, 20000):
+        c1 = min(n_hulls, c0 + 20000)
+        idx = d[c0:c1, 0:1] + np.minimum(np.arange(64)[None, :], d[c0:c1, 1:2] - 1)
+        v = hulls.vertices[idx].astype(np.float64)                     # [m][64][3]
+        w = np.einsum("mvj,mja->mva", v, R[c0:c1])                    # world-axis components
+        hi[c0:c1], lo[c0:c1] = w.max(1), -w.min(1)
+    # Lattice: chains of support-to-support placements along x (i), z (j) and y (k).
+    cells = layers * columns * columns
+    ext_hi = np.zeros((cells, 3))
+    ext_lo = np.zeros((cells, 3))
+    ext_hi[:n_hulls], ext_lo[:n_hulls] = hi, lo
+    ext_hi = ext_hi.reshape(layers, columns, columns, 3)
+    ext_lo = ext_lo.reshape(layers, columns, columns, 3)
+    gap = rng.normal(gap_mean, gap_sigma, (layers, columns, columns, 3))
+
+    # Stacks: the body of layer k rests on the one of layer k - 1 in its column (the floor
+    # for the first), support to support along y plus its gap; columns stand on a square
+    # grid of the mean lateral width plus `pitch_gap`, against the -x and -z walls.
+    pitch = float((hi[:, [0, 2]] + lo[:, [0, 2]]).mean()) + pitch_gap
+    # Rise of each body over the one below it: the distance along +y at which their SAT
+    # separation is the body's gap (exact, not a support-extent estimate).
+    rise = np.zeros(cells)
+    rise[:columns * columns] = ext_lo[0, ..., 1].ravel() + gap[0, ..., 1].ravel()
+    up = np.arange(columns * columns, n_hulls)
+    rot = quat_xf_rotation(q0[ns:].astype(np.float32))
+    rise[up] = place_along(lib, hulls, up - columns * columns, rot[up - columns * columns], up, rot[up],
+                           np.tile([0.0, 1.0, 0.0], (len(up), 1)), gap.reshape(-1, 3)[up, 1])
+    y = np.cumsum(rise.reshape(layers, columns, columns), axis=0)
+    gi = -H + (np.arange(columns) + 0.5) * pitch
+    x = np.broadcast_to(gi[None, None, :], y.shape)
+    z = np.broadcast_to(gi[None, :, None], y.shape)
+    pos0 = np.zeros((B, 3))
+    for a, (e, c) in enumerate(statics):
+        pos0[a] = c
+    pos0[ns:] = np.stack([x, y, z], -1).reshape(-1, 3)[:n_hulls]
+
+    # Contacts: fat AABB overlaps.
+    from scipy.spatial import cKDTree
+    lo_b = np.zeros((B, 3))
+    hi_b = np.zeros((B, 3))
+    for a, (e, c) in enumerate(statics):
+        lo_b[a], hi_b[a] = np.subtract(c, e), np.add(c, e)
+    lo_b[ns:], hi_b[ns:] = pos0[ns:] - lo, pos0[ns:] + hi
+    lo_b -= extension
+    hi_b += extension
+    tree = cKDTree(pos0[ns:])
+    reach = 2.0 * (float(max(hi.max(), lo.max())) + extension)
+    pr = tree.query_pairs(reach, p=np.inf, output_type="ndarray").astype(np.int64) + ns
+    keep = np.all((lo_b[pr[:, 0]] <= hi_b[pr[:, 1]]) & (lo_b[pr[:, 1]] <= hi_b[pr[:, 0]]), axis=1)
+    dyn = pr[keep]
+    st = []
+    for a in range(ns):
+        m = np.all((lo_b[ns:] <= hi_b[a]) & (lo_b[a] <= hi_b[ns:]), axis=1)
+        idx = np.nonzero(m)[0] + ns
+        st.append(np.stack([np.full(len(idx), a), idx], -1))
+    contacts = np.concatenate([dyn] + st)
+    contacts = np.sort(contacts, axis=1)
+    contacts = contacts[np.lexsort((contacts[:, 1], contacts[:, 0]))]
+
+    # Motion parameters.
+    n = n_hulls
+    fast = rng.random(n) < settling
+    amp = np.where(fast, 0.02, 0.002)[:, None] * rng.normal(size=(n, 3)) / np.sqrt(3.0)
+    rot_amp = np.where(fast, 0.05, 0.005) * rng.uniform(0.5, 1.0, n)
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    return {"pool": pool, "shapes": shapes, "body_shapes": body_shapes, "contacts": contacts.astype(np.uint32),
+            "pos0": pos0, "q0": q0, "statics": ns, "hz": hz, "amp": amp, "period": period,
+            "freq": rng.integers(1, 5, n) * (hz / period), "phase": rng.uniform(0.0, 2.0 * np.pi, n),
+            "rot_amp": rot_amp, "rot_axis": axis, "rot_phase": rng.uniform(0.0, 2.0 * np.pi, n),
+            "layers": layers, "columns": columns, "stack_gaps": gap.reshape(-1, 3)[:n_hulls, 1].copy()}
+
+
+def _warm_world_quat(world, s):
+    ns = world["statics"]
+    q = world["q0"].copy()
+    t = 2.0 * np.pi * world["freq"] * ((s % world["period"]) / world["hz"])
+    theta = world["rot_amp"] * (np.sin(t + world["rot_phase"]) - np.sin(world["rot_phase"]))
+    half = 0.5 * theta[:, None]
+    dq = np.concatenate([world["rot_axis"] * np.sin(half), np.cos(half)], 1)
+    qs = _quat_mul(world["q0"][ns:], dq)
+    q[ns:] = qs / np.linalg.norm(qs, axis=1, keepdims=True)
+    return q.astype(np.float32)
+
+
+def warm_world_state_index(world, s):
+    """Index of step s's body state among the period + 1 distinct ones: 0 for the first
+    step, then 1..period repeating (step s > 0 uses state (s - 1) % period + 1)."""
+    return 0 if s == 0 else (s - 1) % world["period"] + 1
+
+
+def warm_world_state(world, s):
+    """Body state at step s: (xfs [B][12] f32: rotation columns from the orientation by
+    b3QuatMat33, then the position; sweeps [B][2][4] f32: (orientation0, orientation) =
+    the orientations at steps s - 1 and s, both the step-0 one at s = 0)."""
+    ns = world["statics"]
+    q = _warm_world_quat(world, s)
+    q_prev = _warm_world_quat(world, s - 1) if s > 0 else q
+    t = 2.0 * np.pi * world["freq"] * ((s % world["period"]) / world["hz"])
+    pos = world["pos0"].copy()
+    pos[ns:] += world["amp"] * (np.sin(t + world["phase"]) - np.sin(world["phase"]))[:, None]
+    xfs = np.empty((len(q), 12), np.float32)
+    xfs[:, 0:9] = quat_xf_rotation(q)
+    xfs[:, 9:12] = pos
+    sweeps = np.stack([q_prev, q], 1)
+    return xfs, sweeps
+
+
+def warm_world_pair_inputs(world, xfs, sweeps, contacts=None):
+    """Batch-API inputs of one step for the listed contacts (all by default): (pairs [n][2]
+    shape indices, xfs [n][2][12], sweeps [n][4][4])."""
+    c = (world["contacts"] if contacts is None else contacts).astype(np.int64)
+    pairs = world["body_shapes"][c].astype(np.uint32)
+    return (np.ascontiguousarray(pairs), np.ascontiguousarray(xfs[c]),
+            np.ascontiguousarray(sweeps[c].reshape(-1, 4, 4)))

@@ -142,6 +142,23 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
                          const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
                          b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
 
+/* One world step of b3ContactManager::UpdateContacts (contact_manager.cpp:167-231) on
+ * device-resident body state: for each contact i, b3ConvexContact::Collide
+ * (convex_contact.cpp:50-62) with bodies A = d_contacts[2i], B = d_contacts[2i+1], i.e.
+ * b3CollideShapeAndShape(d_out[i], xf[A], shape[A], xf[B], shape[B], &d_caches[i]) with
+ * shape[b] = d_body_shapes[b] (an index into the shape table) and xf[b] = d_body_xfs[b]
+ * (b3Body::m_xf).  d_body_sweeps (may be NULL) holds per body (orientation0,
+ * orientation), the sweep the cached face rebuild reads.  The caller passes the contacts
+ * UpdateContacts would collide: those it destroys or skips (filtered, no AABB overlap, both
+ * bodies inactive, :186-222) are left out of the list.  Shape types must satisfy
+ * typeA <= typeB; a body index >= body_count marks the contact B3AMD_FLAG_INVALID_PAIR.
+ * Only the body transforms change per step, so only they need to cross PCIe.  Enqueued on
+ * the context stream without synchronizing; transforms and sweeps 16-byte aligned. */
+int b3amd_update_contacts(b3amd_context* ctx, uint32_t body_count, const uint32_t* d_body_shapes,
+                          const b3amd_transform* d_body_xfs, const b3amd_quat* d_body_sweeps,
+                          uint32_t contact_count, const uint32_t* d_contacts, b3amd_convex_cache* d_caches,
+                          b3amd_manifold* d_out);
+
 int b3amd_synchronize(b3amd_context* ctx);
 
 /* ---- launch shape of the hull SAT (no reference counterpart: the reference runs the
@@ -347,6 +364,13 @@ int b3amd_gen_hull_pool(uint64_t seed, uint32_t count, uint32_t nmin, uint32_t n
 int b3amd_gen_hull_pairs(uint64_t seed, const b3amd_hull_pool* pool, uint32_t n,
                          float gap_mean, float gap_sigma, uint32_t* pairs,
                          b3amd_transform* xfs);
+
+/* For i < n: the distance t[i] along the unit direction dirs[3i..] at which hull
+ * hb[i] (rotation columns rb[9i..]) has SAT signed separation gaps[i] from hull ha[i]
+ * (rotation ra[9i..]) placed at the origin, both hull origins on the line. */
+int b3amd_gen_place_along(const b3amd_hull_pool* pool, uint32_t n, const uint32_t* ha, const float* ra,
+                          const uint32_t* hb, const float* rb, const double* dirs, const double* gaps,
+                          double* t);
 
 #ifdef __cplusplus
 }
