@@ -161,7 +161,9 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     // Negated Gauss map of the hull in its own frame (sat.cpp:131-156 for hull 2): -U, -V,
     // -E = -(Q - P) per unique edge and the arc's bounding cap (centre, widened radius).
     float* gm = fw + 4 * L.gaussOff();
-    for (uint32_t k = L.E(); k < L.gaussEdges(); ++k) gm[16 * k + 15] = 1e30f;
+    float* gc = fw + 4 * L.capOff();
+    _Float16* gf = reinterpret_cast<_Float16*>(fw + 4 * L.filterOff());
+    for (uint32_t k = L.E(); k < L.gaussEdges(); ++k) gc[4 * k + 3] = 1e30f;
     for (uint32_t k = 0; k < L.E(); ++k) {
         const uint32_t* a = edges + 8 * k;
         const uint32_t* b = a + 4;
@@ -172,11 +174,16 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
         const float E[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
         float m[3], ca, sa;
         arc_cap(U, Vn, m, &ca, &sa);
-        float* r = gm + 16 * k;
+        float* r = gm + 12 * k;
         r[0] = -U[0], r[1] = -U[1], r[2] = -U[2], r[3] = -Vn[0];
         r[4] = -Vn[1], r[5] = -Vn[2], r[6] = -E[0], r[7] = -E[1];
         r[8] = -E[2], r[9] = sa, r[10] = 0.0f, r[11] = 0.0f;
-        r[12] = -m[0], r[13] = -m[1], r[14] = -m[2], r[15] = ca;
+        float* c = gc + 4 * k;
+        c[0] = -m[0], c[1] = -m[1], c[2] = -m[2], c[3] = ca;
+        // Packed halves: block k / 32, word c * 16 + (k % 16), half (k % 32) / 16.
+        const float comp[5] = {-m[0], -m[1], -m[2], ca, sa};
+        _Float16* blk = gf + 160 * (k / 32);
+        for (uint32_t q = 0; q < 5; ++q) blk[2 * (16 * q + k % 16) + (k % 32) / 16] = (_Float16)comp[q];
     }
     *layoutOut = L;
     return nullptr;

@@ -10,19 +10,24 @@
 //   then          HE half-edges, one u32 each: origin | face << 8 | next << 16; the twin
 //                 of half-edge h is h ^ 1 (required by sat.cpp:123-128, checked at upload)
 //   then          F u16 first half-edge per face (b3Face::edge)
-//   then          the negated Gauss map, 4 units per unique edge k padded to a multiple
-//                 of 4 edges: {-U.xyz, -V.x}, {-V.yz, -E.xy}, {-E.z, sin a, 0, 0},
-//                 {-m.xyz, cos a} with U, V the normals of faces(2k), face(2k+1) and
-//                 E = origin(2k+1) - origin(2k), in the hull's own frame.  -U, -V, -E are
-//                 exactly the operands hull 2 contributes to b3IsMinkowskiFace in the edge
-//                 query (sat.cpp:131-156).  m and a bound the edge's Gauss-map arc from U
-//                 to V: the cap of centre m = (U + V) / |U + V| and angular radius
-//                 a = angle(U, V) / 2 + 0.01 rad contains the arc with a margin far above
-//                 FP32 rounding, so two arcs whose caps are disjoint cannot pass the
-//                 Gauss-map test; the SAT kernel tests caps first and runs the exact test
-//                 only for pairs whose caps overlap.  (-m, a) is the cap of the negated
-//                 arc, and a hull used as hull 1 reads m back as the negation.  Padding
-//                 records are zero with cos a = 1e30: they fail both tests.
+//   then          the negated Gauss map, 3 units per unique edge k padded to a multiple
+//                 of 4 edges: {-U.xyz, -V.x}, {-V.yz, -E.xy}, {-E.z, sin a, 0, 0} with
+//                 U, V the normals of faces(2k), face(2k+1) and E = origin(2k+1) -
+//                 origin(2k), in the hull's own frame.  -U, -V, -E are exactly the
+//                 operands hull 2 contributes to b3IsMinkowskiFace in the edge query
+//                 (sat.cpp:131-156).
+//   then          one unit per (padded) edge, {-m.xyz, cos a}: the bounding cap of the
+//                 edge's Gauss-map arc from U to V, centre m = (U + V) / |U + V| and angular
+//                 radius a = angle(U, V) / 2 + 0.01 rad, which contains the arc with a
+//                 margin, so two arcs whose caps are disjoint cannot pass the Gauss-map
+//                 test; the SAT kernel tests caps first and runs the exact test only for
+//                 pairs whose caps overlap.  (-m, a) is the cap of the negated arc, and a
+//                 hull used as hull 1 reads m back as the negation.  Padding: cos a = 1e30.
+//   then          the same caps as packed halves for the cap test of a hull used as hull
+//                 2: per block of 32 (padded) edges, 20 units = 80 words, word c * 16 + t
+//                 holding component c (-m.x, -m.y, -m.z, cos a, sin a) of edges 32b + t
+//                 (low half) and 32b + 16 + t (high half), rounded to nearest; padding
+//                 edges are zero (the kernel masks them).
 #ifndef B3AMD_DEVICE_HULL_H
 #define B3AMD_DEVICE_HULL_H
 
@@ -46,7 +51,10 @@ struct HullLayout {
     B3_HD uint32_t faceOff() const { return halfOff() + (HE + 3) / 4; }
     B3_HD uint32_t gaussOff() const { return faceOff() + (F + 7) / 8; }
     B3_HD uint32_t gaussEdges() const { return (E() + 3) & ~3u; }
-    B3_HD uint32_t units() const { return gaussOff() + 4 * gaussEdges(); }
+    B3_HD uint32_t capOff() const { return gaussOff() + 3 * gaussEdges(); }
+    B3_HD uint32_t filterBlocks() const { return (gaussEdges() + 31) / 32; }
+    B3_HD uint32_t filterOff() const { return capOff() + gaussEdges(); }
+    B3_HD uint32_t units() const { return filterOff() + 20 * filterBlocks(); }
 };
 
 // Read-only view of one hull record in global memory.

@@ -81,7 +81,10 @@ constexpr uint32_t kCandCap = 128;
 struct SatStage {
     union {
         float4 verts[kMaxHullVertices + 1];  // face queries: the other hull's vertices
-        float4 gauss[4 * kStageEdges];       // edge query: a chunk of hull 2's Gauss map
+        struct {
+            float4 exact[3 * kStageEdges];        // edge query: a chunk of hull 2's Gauss map
+            float4 filter[20 * (kStageEdges / 32)];  // and its caps as packed halves
+        } e;
     } u;
     float4 tv1[kLdsVerts];  // edge query: hull 1's vertices, b3Mul(xf, v)
     float4 tn1[kLdsFaces];  // edge query: hull 1's face normals, b3Mul(xf.rotation, n)
@@ -161,7 +164,7 @@ __device__ __forceinline__ void drain_hits(const Xf& xf, V3 C1, const UHull& h1,
         const V3 P1 = lds1 ? ld3(st.tv1[u & 0xFFu]) : mul(xf, ld3(h1.base[2 + (u & 0xFFu)]));
         const V3 Q1 = lds1 ? ld3(st.tv1[(u >> 8) & 0xFFu]) : mul(xf, ld3(h1.base[2 + ((u >> 8) & 0xFFu)]));
         const V3 E1 = Q1 - P1;
-        const float4 r1 = st.u.gauss[4u * (j - c0) + 1u], r2 = st.u.gauss[4u * (j - c0) + 2u];
+        const float4 r1 = st.u.e.exact[3u * (j - c0) + 1u], r2 = st.u.e.exact[3u * (j - c0) + 2u];
         const V3 E2 = -V3{r1.z, r1.w, r2.x};
         const V3 P2 = ld3(h2.base[2 + (ue2[j] & 0xFFu)]);
         const float s = project_edges(P1, E1, length(E1), P1 - C1, P2, E2, length(E2));
@@ -206,7 +209,7 @@ __device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHul
         e = st.cands[(chead + lane) & (kCandCap - 1u)];
         V3 A, B, nE;
         edge1_operands(xf, h1, ue1, st, lds1, e >> 16, A, B, nE);
-        const float4* r = st.u.gauss + 4u * ((e & 0xFFFFu) - c0);
+        const float4* r = st.u.e.exact + 3u * ((e & 0xFFFFu) - c0);
         const float4 r0 = r[0], r1 = r[1];
         const float r2x = r[2].x;
         const V3 C{r0.x, r0.y, r0.z}, Dn{r0.w, r1.x, r1.y}, DxC{r1.z, r1.w, r2x};
@@ -228,11 +231,22 @@ __device__ __forceinline__ void drain_candidates(const Xf& xf, V3 C1, const UHul
 
 // Edge query (sat.cpp:110-173).  Each lane holds the bounding caps of K hull-1 arcs (edges
 // lane + 64k of the current block, rotated into hull 2's frame) and tests them against
-// every staged hull-2 arc cap: angle(M1, M2) <= a1 + a2, i.e. M1.M2 >= cos a1 cos a2 -
-// sin a1 sin a2 (five fused operations instead of b3IsMinkowskiFace's 26).  The caps contain the
-// arcs with a margin, so a pair they reject cannot pass the exact test; the ~5 % whose
-// caps overlap go through an LDS ring and are tested 64 at a time, converged, with the
-// reference's operands and operation order, so the set of hits is unchanged.
+// every staged hull-2 arc cap: angle(M1, M2) <= a1 + a2, i.e. sin a1 sin a2 - cos a1 cos a2
+// + M1.M2 >= 0, on packed halves, two hull-2 edges per instruction: five fused
+// multiply-adds on half2 plus one bit insert per two tests.  Rounding the inputs to halves
+// (relative 2^-11) and five half roundings of partial sums bounded by 2 change the
+// left-hand side by at most 14 * 2^-11 < 2^-7, so the test is run with 2^-7 added: it never
+// rejects a pair whose caps meet.  The caps contain the arcs with a margin, so a pair they
+// reject cannot pass the exact test; the ~6 % that pass go through an LDS ring and are
+// tested 64 at a time, converged, with the reference's operands and operation order, so
+// the set of hits is unchanged.
+typedef _Float16 half2v __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ half2v splat_half(float x) {
+    const _Float16 h = (_Float16)x;
+    return half2v{h, h};
+}
+__device__ __forceinline__ half2v as_half2(uint32_t w) { return __builtin_bit_cast(half2v, w); }
+
 template <int K>
 __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h1, const UHull& h2, SatStage& st,
                                              uint32_t lane, float& best, uint32_t& key) {
@@ -241,8 +255,11 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
     const CU32 ue1 = (CU32)(h1.base + L1.uedgeOff()).p;
     const CU32 ue2 = (CU32)(h2.base + L2.uedgeOff()).p;
     const CF4 g1 = h1.base + L1.gaussOff();
+    const CF4 cap1 = h1.base + L1.capOff();
     const CF4 g2 = h2.base + L2.gaussOff();
-    const uint32_t E2p = L2.gaussEdges();  // padded with records that never pass
+    const CF4 f2 = h2.base + L2.filterOff();
+    const uint32_t E2 = h2.HE >> 1;
+    const uint32_t E2p = L2.gaussEdges();  // staged in chunks; edges past E2 are masked
     // Hull 1 in hull 2's frame, once per pair, for the exact tests and projections.
     const bool lds1 = h1.V <= kLdsVerts && h1.F <= kLdsFaces;
     if (lds1) {
@@ -259,53 +276,66 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
     }
     uint32_t head = 0, tail = 0, chead = 0, ctail = 0;
     for (uint32_t i0 = 0; i0 < E1n; i0 += 64u * K) {
-        // Slot k: the cap of the arc (U1, V1) of edge i0 + lane + 64k; lanes past the last
-        // edge hold a cap that overlaps nothing.
-        V3 M[K];
-        float cA[K], sA[K];
+        // Slot k: the cap of the arc (U1, V1) of edge i0 + lane + 64k as splatted halves;
+        // lanes past the last edge are masked.
+        half2v Mx[K], My[K], Mz[K], nC[K], sS[K];
+        bool live[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             const uint32_t i = i0 + lane + 64u * k;
-            M[k] = V3{0.0f, 0.0f, 0.0f};
-            cA[k] = 1e30f;
-            sA[k] = 0.0f;
-            if (i < E1n) {
-                const float4 cap = g1[4u * i + 3u];
-                M[k] = mul(xf.R, V3{-cap.x, -cap.y, -cap.z});
-                cA[k] = cap.w;
-                sA[k] = g1[4u * i + 2u].y;
+            live[k] = i < E1n;
+            V3 M{0.0f, 0.0f, 0.0f};
+            float cA = 0.0f, sA = 0.0f;
+            if (live[k]) {
+                const float4 cap = cap1[i];
+                M = mul(xf.R, V3{-cap.x, -cap.y, -cap.z});
+                cA = cap.w;
+                sA = g1[3u * i + 2u].y;
             }
+            Mx[k] = splat_half(M.x);
+            My[k] = splat_half(M.y);
+            Mz[k] = splat_half(M.z);
+            nC[k] = splat_half(-cA);
+            sS[k] = splat_half(sA);
         }
+        const half2v slack = splat_half(0.0078125f);  // 2^-7
         for (uint32_t c0 = 0; c0 < E2p; c0 += kStageEdges) {
             const uint32_t cn = E2p - c0 < kStageEdges ? E2p - c0 : kStageEdges;  // a multiple of 4
+            const uint32_t nb = (cn + 31u) / 32u;
             __syncthreads();
-            for (uint32_t u = lane; u < 4u * cn; u += 64u) st.u.gauss[u] = g2[4u * c0 + u];
+            for (uint32_t u = lane; u < 3u * cn; u += 64u) st.u.e.exact[u] = g2[3u * c0 + u];
+            for (uint32_t u = lane; u < 20u * nb; u += 64u) st.u.e.filter[u] = f2[20u * (c0 / 32u) + u];
             __syncthreads();
             for (uint32_t j0 = 0; j0 < cn; j0 += 32u) {
-                uint32_t cand[K];
+                // Bit t of acc: the sign of the test of edge j0 + t (a set sign fails).
+                uint32_t acc[K];
 #pragma unroll
-                for (int k = 0; k < K; ++k) cand[k] = 0u;
-                const uint32_t jn = cn - j0 < 32u ? cn - j0 : 32u;
-                // The filter is not part of the result (only the exact test below decides a
-                // hit), so it may use fused multiply-adds: its rounding is far inside the
-                // caps' margin.  Test jj + t lands in bit jn - 1 - (jj + t).
-                for (uint32_t jj = 0; jj < jn; jj += 4u) {
+                for (int k = 0; k < K; ++k) acc[k] = 0u;
+                const uint4* w = reinterpret_cast<const uint4*>(st.u.e.filter + 20u * (j0 / 32u));
+                for (uint32_t t4 = 0; t4 < 4u; ++t4) {
+                    const uint4 wx = w[t4], wy = w[4u + t4], wz = w[8u + t4], wc = w[12u + t4], ws = w[16u + t4];
+                    const uint32_t ax[4] = {wx.x, wx.y, wx.z, wx.w}, ay[4] = {wy.x, wy.y, wy.z, wy.w};
+                    const uint32_t az[4] = {wz.x, wz.y, wz.z, wz.w}, ac[4] = {wc.x, wc.y, wc.z, wc.w};
+                    const uint32_t as[4] = {ws.x, ws.y, ws.z, ws.w};
 #pragma unroll
-                    for (uint32_t t = 0; t < 4u; ++t) {
-                        const float4* r = st.u.gauss + 4u * (j0 + jj + t);
-                        const float4 cap = r[3];
-                        const float s2 = r[2].y;
+                    for (uint32_t tt = 0; tt < 4u; ++tt) {
 #pragma unroll
                         for (int k = 0; k < K; ++k) {
-                            float g = sA[k] * s2;
-                            g = __builtin_fmaf(-cA[k], cap.w, g);
-                            g = __builtin_fmaf(M[k].x, cap.x, g);
-                            g = __builtin_fmaf(M[k].y, cap.y, g);
-                            g = __builtin_fmaf(M[k].z, cap.z, g);
-                            cand[k] = cand[k] + cand[k] + (g >= 0.0f ? 1u : 0u);
+                            half2v g = __builtin_elementwise_fma(sS[k], as_half2(as[tt]), slack);
+                            g = __builtin_elementwise_fma(nC[k], as_half2(ac[tt]), g);
+                            g = __builtin_elementwise_fma(Mx[k], as_half2(ax[tt]), g);
+                            g = __builtin_elementwise_fma(My[k], as_half2(ay[tt]), g);
+                            g = __builtin_elementwise_fma(Mz[k], as_half2(az[tt]), g);
+                            const uint32_t gb = __builtin_bit_cast(uint32_t, g);
+                            acc[k] = (gb & 0x80008000u) | ((acc[k] >> 1) & 0x7FFF7FFFu);
                         }
                     }
                 }
+                const uint32_t real = E2 > c0 + j0 ? E2 - (c0 + j0) : 0u;
+                const uint32_t valid = real >= 32u ? 0xFFFFFFFFu : ((1u << real) - 1u);
+                uint32_t cand[K];
+#pragma unroll
+                for (int k = 0; k < K; ++k) cand[k] = live[k] ? (~acc[k] & valid) : 0u;
                 // Queue the candidates: each round appends one per lane that has any.
                 for (;;) {
                     uint32_t sel = (uint32_t)K;
@@ -322,7 +352,7 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
 #pragma unroll
                         for (int k = 0; k < K; ++k)
                             if ((uint32_t)k == sel) cand[k] = w & (w - 1u);
-                        const uint32_t e = ((i0 + lane + 64u * sel) << 16) | (c0 + j0 + jn - 1u - (uint32_t)__builtin_ctz(w));
+                        const uint32_t e = ((i0 + lane + 64u * sel) << 16) | (c0 + j0 + (uint32_t)__builtin_ctz(w));
                         st.cands[(ctail + mbcnt(m)) & (kCandCap - 1u)] = e;
                     }
                     ctail += (uint32_t)__popcll(m);
