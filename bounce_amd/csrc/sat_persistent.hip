@@ -247,6 +247,32 @@ __device__ __forceinline__ half2v splat_half(float x) {
 }
 __device__ __forceinline__ half2v as_half2(uint32_t w) { return __builtin_bit_cast(half2v, w); }
 
+// Slot k of each lane: the cap of the arc (U1, V1) of hull-1 edge i0 + lane + 64k rotated
+// into hull 2's frame, as splatted halves; lanes past the last edge are masked.
+template <int K>
+__device__ __forceinline__ void load_slots(const Xf& xf, CF4 g1, CF4 cap1, uint32_t E1n, uint32_t i0,
+                                           uint32_t lane, half2v* Mx, half2v* My, half2v* Mz, half2v* nC,
+                                           half2v* sS, bool* live) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        const uint32_t i = i0 + lane + 64u * k;
+        live[k] = i < E1n;
+        V3 M{0.0f, 0.0f, 0.0f};
+        float cA = 0.0f, sA = 0.0f;
+        if (live[k]) {
+            const float4 cap = cap1[i];
+            M = mul(xf.R, V3{-cap.x, -cap.y, -cap.z});
+            cA = cap.w;
+            sA = g1[3u * i + 2u].y;
+        }
+        Mx[k] = splat_half(M.x);
+        My[k] = splat_half(M.y);
+        Mz[k] = splat_half(M.z);
+        nC[k] = splat_half(-cA);
+        sS[k] = splat_half(sA);
+    }
+}
+
 template <int K>
 __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h1, const UHull& h2, SatStage& st,
                                              uint32_t lane, float& best, uint32_t& key) {
@@ -260,52 +286,57 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
     const CF4 f2 = h2.base + L2.filterOff();
     const uint32_t E2 = h2.HE >> 1;
     const uint32_t E2p = L2.gaussEdges();  // staged in chunks; edges past E2 are masked
-    // Hull 1 in hull 2's frame, once per pair, for the exact tests and projections.
+    // Hull 1 in hull 2's frame, once per pair, for the exact tests and projections, and the
+    // first chunk of hull 2: their loads are issued together, before one barrier, so that
+    // the wave waits for memory once.
     const bool lds1 = h1.V <= kLdsVerts && h1.F <= kLdsFaces;
+    half2v Mx[K], My[K], Mz[K], nC[K], sS[K];
+    bool live[K];
+    load_slots<K>(xf, g1, cap1, E1n, 0u, lane, Mx, My, Mz, nC, sS, live);
+    const uint32_t cn0 = E2p < kStageEdges ? E2p : kStageEdges;
+    constexpr uint32_t kPreExact = (3u * kStageEdges + 63u) / 64u;
+    float4 preExact[kPreExact], preFilter = make_float4(0.f, 0.f, 0.f, 0.f);
+    float4 preV = make_float4(0.f, 0.f, 0.f, 0.f), preN0 = preV, preN1 = preV;
+    __syncthreads();
+#pragma unroll
+    for (uint32_t q = 0; q < kPreExact; ++q)
+        if (lane + 64u * q < 3u * cn0) preExact[q] = g2[lane + 64u * q];
+    if (lane < 20u * ((cn0 + 31u) / 32u)) preFilter = f2[lane];
     if (lds1) {
-        __syncthreads();
+        if (lane < h1.V) preV = h1.base[2 + lane];
+        if (lane < h1.F) preN0 = h1.base[2 + h1.V + lane];
+        if (lane + 64u < h1.F) preN1 = h1.base[2 + h1.V + lane + 64u];
         if (lane < h1.V) {
-            const V3 p = mul(xf, ld3(h1.base[2 + lane]));
+            const V3 p = mul(xf, ld3(preV));
             st.tv1[lane] = make_float4(p.x, p.y, p.z, 0.0f);
         }
-        for (uint32_t f = lane; f < h1.F; f += 64u) {
-            const V3 n = mul(xf.R, ld3(h1.base[2 + h1.V + f]));
-            st.tn1[f] = make_float4(n.x, n.y, n.z, 0.0f);
+        if (lane < h1.F) {
+            const V3 n = mul(xf.R, ld3(preN0));
+            st.tn1[lane] = make_float4(n.x, n.y, n.z, 0.0f);
         }
-        __syncthreads();
+        if (lane + 64u < h1.F) {
+            const V3 n = mul(xf.R, ld3(preN1));
+            st.tn1[lane + 64u] = make_float4(n.x, n.y, n.z, 0.0f);
+        }
     }
+#pragma unroll
+    for (uint32_t q = 0; q < kPreExact; ++q)
+        if (lane + 64u * q < 3u * cn0) st.u.e.exact[lane + 64u * q] = preExact[q];
+    if (lane < 20u * ((cn0 + 31u) / 32u)) st.u.e.filter[lane] = preFilter;
+    __syncthreads();
     uint32_t head = 0, tail = 0, chead = 0, ctail = 0;
     for (uint32_t i0 = 0; i0 < E1n; i0 += 64u * K) {
-        // Slot k: the cap of the arc (U1, V1) of edge i0 + lane + 64k as splatted halves;
-        // lanes past the last edge are masked.
-        half2v Mx[K], My[K], Mz[K], nC[K], sS[K];
-        bool live[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const uint32_t i = i0 + lane + 64u * k;
-            live[k] = i < E1n;
-            V3 M{0.0f, 0.0f, 0.0f};
-            float cA = 0.0f, sA = 0.0f;
-            if (live[k]) {
-                const float4 cap = cap1[i];
-                M = mul(xf.R, V3{-cap.x, -cap.y, -cap.z});
-                cA = cap.w;
-                sA = g1[3u * i + 2u].y;
-            }
-            Mx[k] = splat_half(M.x);
-            My[k] = splat_half(M.y);
-            Mz[k] = splat_half(M.z);
-            nC[k] = splat_half(-cA);
-            sS[k] = splat_half(sA);
-        }
+        if (i0 != 0u) load_slots<K>(xf, g1, cap1, E1n, i0, lane, Mx, My, Mz, nC, sS, live);
         const half2v slack = splat_half(0.0078125f);  // 2^-7
         for (uint32_t c0 = 0; c0 < E2p; c0 += kStageEdges) {
             const uint32_t cn = E2p - c0 < kStageEdges ? E2p - c0 : kStageEdges;  // a multiple of 4
             const uint32_t nb = (cn + 31u) / 32u;
-            __syncthreads();
-            for (uint32_t u = lane; u < 3u * cn; u += 64u) st.u.e.exact[u] = g2[3u * c0 + u];
-            for (uint32_t u = lane; u < 20u * nb; u += 64u) st.u.e.filter[u] = f2[20u * (c0 / 32u) + u];
-            __syncthreads();
+            if (i0 != 0u || c0 != 0u) {  // the first chunk was staged above
+                __syncthreads();
+                for (uint32_t u = lane; u < 3u * cn; u += 64u) st.u.e.exact[u] = g2[3u * c0 + u];
+                for (uint32_t u = lane; u < 20u * nb; u += 64u) st.u.e.filter[u] = f2[20u * (c0 / 32u) + u];
+                __syncthreads();
+            }
             for (uint32_t j0 = 0; j0 < cn; j0 += 32u) {
                 // Bit t of acc: the sign of the test of edge j0 + t (a set sign fails).
                 uint32_t acc[K];
