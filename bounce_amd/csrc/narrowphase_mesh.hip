@@ -443,22 +443,31 @@ __global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
                 choose(index);
             }
         }
-        // Ten Lloyd iterations with sequential sums (:404-444).
+        // Ten Lloyd iterations (:404-444).  The reference assigns every observation, then sums
+        // each cluster's observations in observation order; one pass that assigns observation
+        // j and adds it to its cluster's sum adds the same terms in the same order to each
+        // sum, so the centroids are the same, and the observations are read once per
+        // iteration instead of once per cluster.
         for (uint32_t iter = 0; iter < 10; ++iter) {
-            for (uint32_t i = 0; i < nObs; ++i) cl[i] = C.find(obs[i]);
-            for (uint32_t i = 0; i < C.n; ++i) {
-                V3 centroid{0.0f, 0.0f, 0.0f};
-                uint32_t count = 0;
-                for (uint32_t j = 0; j < nObs; ++j)
-                    if (cl[j] == i) {
-                        centroid = centroid + obs[j];
-                        ++count;
+            V3 sum[4] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
+            uint32_t count[4] = {0u, 0u, 0u, 0u};
+            for (uint32_t j = 0; j < nObs; ++j) {
+                const V3 o = obs[j];
+                const uint32_t c = C.find(o);
+                if (iter == 9u) cl[j] = c;
+#pragma unroll
+                for (uint32_t k = 0; k < 4u; ++k)
+                    if (c == k) {
+                        sum[k] = sum[k] + o;
+                        ++count[k];
                     }
-                if (count > 0) {
-                    float sInv = 1.0f / (float)count;  // b3Vec3::operator/= (vec3.h:70-76)
-                    C.c[i] = V3{centroid.x * sInv, centroid.y * sInv, centroid.z * sInv};
-                }
             }
+#pragma unroll
+            for (uint32_t i = 0; i < 4u; ++i)
+                if (i < C.n && count[i] > 0) {
+                    float sInv = 1.0f / (float)count[i];  // b3Vec3::operator/= (vec3.h:70-76)
+                    C.c[i] = V3{sum[i].x * sInv, sum[i].y * sInv, sum[i].z * sInv};
+                }
         }
         // Drop empty clusters, renumbering the rest in order (:449-472).
         uint32_t used = 0;
