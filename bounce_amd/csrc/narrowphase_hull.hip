@@ -290,13 +290,26 @@ __device__ __forceinline__ int group_face_contact(GroupLds& L, uint32_t g, uint3
     if (e != begin2) return -1;
     uint32_t count = deg;
     ClipV v = clipv(mul(xf2, h2.vertex(h2.origin(mine))), h2.next(mine ^ 1u), kNullEdge, mine, kNullEdge);
-    // Clip against the side planes of the reference face (clip.cpp:243-280).
+    // Clip against the side planes of the reference face (clip.cpp:243-280), in the face's
+    // edge order.  Lane j builds the side plane of the face's j-th edge up front (the same
+    // operations, all planes at once instead of one dependent chain of record loads per
+    // clip step); step k takes plane k from lane k.
     const uint32_t begin1 = h1.face_edge(index1);
-    uint32_t e1 = begin1, guard = 0;
+    uint32_t w1 = begin1, deg1 = 0, myEdge = begin1;
     do {
-        Plane pl = edge_side_plane(h1, e1);
-        pl.d += totalRadius;
-        const Plane wp = mul(xf1, pl);
+        if (deg1 == g) myEdge = w1;
+        ++deg1;
+        w1 = h1.next(w1);
+    } while (w1 != begin1 && deg1 <= kGroup);
+    if (w1 != begin1) return -1;  // a reference face of more than kGroup edges
+    Plane side = edge_side_plane(h1, myEdge);
+    side.d += totalRadius;
+    const Plane wmine = mul(xf1, side);
+    for (uint32_t step = 0; step < deg1; ++step) {
+        const uint32_t e1 = (uint32_t)__shfl((int)myEdge, (int)step, (int)kGroup);
+        const Plane wp{V3{__shfl(wmine.n.x, (int)step, (int)kGroup), __shfl(wmine.n.y, (int)step, (int)kGroup),
+                          __shfl(wmine.n.z, (int)step, (int)kGroup)},
+                       __shfl(wmine.d, (int)step, (int)kGroup)};
         const bool valid = g < count;
         const float d2 = valid ? distance(v.position, wp) : 0.0f;
         const uint32_t src = g == 0 ? count - 1u : g - 1u;
@@ -330,8 +343,7 @@ __device__ __forceinline__ int group_face_contact(GroupLds& L, uint32_t g, uint3
         if (g < count) v = L.poly[g];
         group_sync();
         if (count == 0) break;
-        e1 = h1.next(e1);
-    } while (e1 != begin1 && ++guard < 256);
+    }
     if (count == 0) return 0;
     // Points within totalRadius, projected onto the reference plane; the deepest first.
     const bool valid = g < count;
