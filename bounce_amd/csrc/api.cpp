@@ -189,6 +189,13 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     return nullptr;
 }
 
+// The record unit of each hull shape's hull in the current pool (0 for other shapes and for
+// hull indices outside the pool, which the kernels reject before reading a record).
+void set_hull_units(const b3amd_context* ctx, std::vector<ShapeRec>& shapes) {
+    for (ShapeRec& r : shapes)
+        r.hullUnit = (r.type == B3AMD_HULL && r.hull < ctx->hullOffsets.size()) ? ctx->hullOffsets[r.hull] : 0u;
+}
+
 int upload(b3amd_context* ctx, void** dptr, uint64_t* cap, const void* src, uint64_t bytes) {
     if (bytes > *cap) {
         int rc = ctx->release(*dptr);
@@ -372,6 +379,13 @@ int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
     ctx->hullCount = pool->count;
     ++ctx->generation;
     ctx->layouts.swap(layouts);
+    ctx->hullOffsets.swap(offsets);
+    // Shapes uploaded before this pool point into the old one: refresh their record units.
+    if (ctx->shapeCount && ctx->dShapes) {
+        set_hull_units(ctx, ctx->shapes);
+        HIP_TRY(hipMemcpy(ctx->dShapes, ctx->shapes.data(), sizeof(ShapeRec) * (uint64_t)ctx->shapeCount,
+                          hipMemcpyHostToDevice));
+    }
     return B3AMD_OK;
 }
 
@@ -388,9 +402,11 @@ int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* s
     if ((rc = ctx->release(ctx->dShapes))) return rc;
     ctx->dShapes = nullptr;
     ctx->shapeCount = 0;
-    if ((rc = ctx->alloc(sizeof(ShapeRec) * (uint64_t)count, (void**)&ctx->dShapes))) return rc;
-    if (count) HIP_TRY(hipMemcpy(ctx->dShapes, shapes, sizeof(ShapeRec) * (uint64_t)count, hipMemcpyHostToDevice));
     ctx->shapes.assign(reinterpret_cast<const ShapeRec*>(shapes), reinterpret_cast<const ShapeRec*>(shapes) + count);
+    set_hull_units(ctx, ctx->shapes);
+    if ((rc = ctx->alloc(sizeof(ShapeRec) * (uint64_t)count, (void**)&ctx->dShapes))) return rc;
+    if (count)
+        HIP_TRY(hipMemcpy(ctx->dShapes, ctx->shapes.data(), sizeof(ShapeRec) * (uint64_t)count, hipMemcpyHostToDevice));
     ctx->shapeCount = count;
     ++ctx->generation;
     ctx->roundShapes = false;

@@ -73,6 +73,7 @@ struct b3amd_context {
     uint32_t* dOffsets = nullptr;
     uint32_t hullCount = 0;
     std::vector<HullLayout> layouts;
+    std::vector<uint32_t> hullOffsets;  // first 16-byte unit of each hull's record
 
     // Shape table.
     ShapeRec* dShapes = nullptr;

@@ -63,8 +63,12 @@ struct DevHull {
     uint32_t V, F, HE;
 
     __device__ static DevHull at(const float4* pool, const uint32_t* offsets, uint32_t h) {
+        return at_unit(pool, offsets[h]);
+    }
+    // The record starting at 16-byte unit `unit` of the pool (ShapeRec::hullUnit).
+    __device__ static DevHull at_unit(const float4* pool, uint32_t unit) {
         DevHull r;
-        r.base = pool + offsets[h];
+        r.base = pool + unit;
         uint4 hd = reinterpret_cast<const uint4*>(r.base)[0];
         r.V = hd.x & 0xFFFFu;
         r.F = hd.x >> 16;

@@ -17,7 +17,8 @@ struct ShapeRec {
     uint32_t hull;
     float radius;
     float v[6];
-    uint32_t pad[3];
+    uint32_t hullUnit;  // first 16-byte unit of the hull's record in the pool (set by the library)
+    uint32_t pad[2];
 };
 static_assert(sizeof(ShapeRec) == 48, "shape record layout");
 

@@ -37,7 +37,7 @@ __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
         m.pointCount = 0;
         m.overflow = false;
         if (s2.type == 2u) {
-            const DevHull h2 = DevHull::at(a.pool, a.offsets, s2.hull);
+            const DevHull h2 = DevHull::at_unit(a.pool, s2.hullUnit);
             if (s1.type == 0u) collide_sphere_hull(m, xf1, s1, xf2, h2, s2.radius);
             else collide_capsule_hull(m, xf1, s1, xf2, h2, s2.radius);
         } else if (s1.type == 0u && s2.type == 0u) {

@@ -58,8 +58,8 @@ __global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
     const Xf xf1 = load_xf(a.xf_ptr(p, 0));
     const Xf xf2 = load_xf(a.xf_ptr(p, 1));
-    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
-    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+    const DevHull h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
+    const DevHull h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
 
     const float kTol = 0.1f * kLinearSlop;
     const float fmax12 = r.f1Sep > r.f2Sep ? r.f1Sep : r.f2Sep;  // b3Max
@@ -497,8 +497,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
     const Xf xf1 = load_xf(a.xf_ptr(p, 0));
     const Xf xf2 = load_xf(a.xf_ptr(p, 1));
-    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
-    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+    const DevHull h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
+    const DevHull h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
 
     const float kTol = 0.1f * kLinearSlop;
     const float fmax12 = r.f1Sep > r.f2Sep ? r.f1Sep : r.f2Sep;  // b3Max
@@ -564,8 +564,8 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
     const float totalRadius = r1 + r2;
     const Xf xf1 = load_xf(a.xf_ptr(p, 0));
     const Xf xf2 = load_xf(a.xf_ptr(p, 1));
-    const DevHull h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
-    const DevHull h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+    const DevHull h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
+    const DevHull h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
     const uint32_t state0 = fc->state, type = fc->type, i1 = fc->index1, i2 = fc->index2;
     // A cached feature outside its hull cannot come from this pair: treat it as a miss.
     const bool inRange = (type == kFeatureEdge1 && i1 + 1 < h1.HE && i2 + 1 < h2.HE && !(i1 & 1u) && !(i2 & 1u)) ||
@@ -721,8 +721,8 @@ struct CachedPair {
         c.r2 = a.shapes[sb].radius;
         c.xf1 = load_xf(a.xf_ptr(p, 0));
         c.xf2 = load_xf(a.xf_ptr(p, 1));
-        c.h1 = DevHull::at(a.pool, a.offsets, a.shapes[sa].hull);
-        c.h2 = DevHull::at(a.pool, a.offsets, a.shapes[sb].hull);
+        c.h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
+        c.h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
         c.state0 = fc->state;
         c.type = fc->type;
         c.i1 = fc->index1;

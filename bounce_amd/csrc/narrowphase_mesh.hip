@@ -49,7 +49,7 @@ __device__ Aabb shape_aabb(const ShapeRec& s, const Xf& xf, const float4* pool, 
         a.lo = vmin(c1, c2) - r;
         a.hi = vmax(c1, c2) + r;
     } else {
-        const DevHull h = DevHull::at(pool, offsets, s.hull);
+        const DevHull h = DevHull::at_unit(pool, s.hullUnit);
         a.lo = a.hi = mul(xf, h.vertex(0));
         for (uint32_t i = 1; i < h.V; ++i) {
             V3 v = mul(xf, h.vertex(i));
@@ -243,7 +243,7 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
         m.pointCount = 0;
         m.overflow = false;
         if (s.type == 2u) {
-            const DevHull h = DevHull::at(a.pool, a.offsets, s.hull);
+            const DevHull h = DevHull::at_unit(a.pool, s.hullUnit);
             if (a.triCaches) {
                 // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
                 // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.

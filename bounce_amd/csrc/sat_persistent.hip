@@ -58,9 +58,9 @@ __device__ __forceinline__ Xf load_xf_u(CF32 p) {
 struct UHull {
     CF4 base;
     uint32_t V, F, HE;
-    __device__ static UHull at(CF4 pool, CU32 offsets, uint32_t h) {
+    __device__ static UHull at_unit(CF4 pool, uint32_t unit) {
         UHull r;
-        r.base = pool + offsets[h];
+        r.base = pool + unit;
         const CU32 w = (CU32)r.base.p;
         const uint32_t x = w[0], y = w[1];
         r.V = x & 0xFFFFu;
@@ -416,15 +416,14 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
 __device__ __forceinline__ void sat_pair(const SatArgs& a, uint32_t p, SatStage& st, uint32_t lane,
                                          uint32_t maxSlots) {
     const CU32 pairs = (const CU32)a.pairs;
-    const CU32 shapes = (const CU32)a.shapes;  // ShapeRec: 12 words (type, hull, radius, ...)
+    const CU32 shapes = (const CU32)a.shapes;  // ShapeRec: 12 words (type, hull, radius, v[6], hullUnit, ...)
     const uint32_t sa = pairs[2 * p], sb = pairs[2 * p + 1];
-    const uint32_t ha = shapes[12 * sa + 1], hb = shapes[12 * sb + 1];
+    const uint32_t ua = shapes[12 * sa + 9], ub = shapes[12 * sb + 9];  // ShapeRec::hullUnit
     const float totalRadius = __uint_as_float(shapes[12 * sa + 2]) + __uint_as_float(shapes[12 * sb + 2]);
     const Xf xf1 = load_xf_u((const CF32)a.xf_ptr(p, 0)), xf2 = load_xf_u((const CF32)a.xf_ptr(p, 1));
     const CF4 pool{(CF32)a.pool};
-    const CU32 offsets = (const CU32)a.offsets;
-    const UHull h1 = UHull::at(pool, offsets, ha);
-    const UHull h2 = UHull::at(pool, offsets, hb);
+    const UHull h1 = UHull::at_unit(pool, ua);
+    const UHull h2 = UHull::at_unit(pool, ub);
 
     SatResult r;
     r.f2Index = 0;

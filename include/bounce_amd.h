@@ -65,7 +65,7 @@ typedef struct {
     uint32_t hull;
     float radius;
     float v[6];
-    uint32_t pad[3];
+    uint32_t pad[3];          /* ignored: the library fills in its own copy */
 } b3amd_shape;
 
 /* b3ManifoldPoint fields written by the narrowphase (manifold.h:66-76); key[] holds the
