@@ -670,7 +670,8 @@ __global__ __launch_bounds__(256) void cache_list_kernel(CacheArgs a) {
             if (fc && toSat) fc->state = kCacheEmpty;
         }
         *stage = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
-        sat->stage = toSat ? kStageOverlap : kStageSkip;
+        if (toSat) sat_prefill(a, p, sat);
+        else sat->stage = kStageSkip;
     }
     block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCalls, call);
@@ -812,7 +813,10 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void c
         hit = lead && r == kCachedHit;
         miss = lead && r == kCachedMiss;
         rebuild = lead && r == kCachedRebuild;
-        if (miss) fc->state = kCacheEmpty;
+        if (miss) {
+            fc->state = kCacheEmpty;
+            sat_prefill(a, p, a.sat + p);
+        }
     }
     wave_append(miss, p, a.satList, a.counts + kCountSat);
     const uint32_t k = (t / 64u) % kRebuildLists;
@@ -853,6 +857,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
             hits += r == kCachedHit ? 1u : 0u;
             if (r == kCachedMiss) {
                 fc->state = kCacheEmpty;
+                sat_prefill(a, p, a.sat + p);
                 a.satList[atomicAdd(a.counts + kCountSat, 1u)] = p;
             }
             *stage = r == kCachedFallback ? kStageCacheFallback : kStageSkip;
@@ -870,9 +875,11 @@ __global__ __launch_bounds__(256) void cache_fallback_kernel(CacheArgs a) {
         const uint32_t slot = a.slot_of(p);
         Manifold* const out = a.out + slot;
         FeatureCache* const fc = &a.caches[slot].feature;
+        SatResult* const sat = a.sat + p;
         const uint32_t st = cache_pair(a, p, 1u, out, fc);
         hit = st == kStageSkip;
         miss = !hit;
+        if (miss) sat_prefill(a, p, sat);
     }
     wave_append(miss, p, a.satList, a.counts + kCountSat);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);
