@@ -181,17 +181,17 @@ struct CacheArgs : PairInputs {
 constexpr uint32_t kStageCached = 7u;
 constexpr uint32_t kStageCacheFallback = 8u;
 
-// What the SAT kernel needs of a listed pair before its queries, written into the pair's
-// SatResult by the kernel that lists it: the two hull record units (f1Index, f2Index) and
-// the total radius (f1Sep), so the SAT kernel reads one record instead of the pair's shape
-// indices and then their shape records.
-__device__ __forceinline__ void sat_prefill(const PairInputs& a, uint32_t p, SatResult* r) {
+// What the cache and SAT kernels need of a hull pair, written into the pair's SatResult by
+// the listing kernel before any query: the hull record units (f1Index, f2Index) and the
+// radii (f1Sep, f2Sep) of shapes A and B, so later kernels read one record instead of the
+// pair's shape indices and then their shape records; `stage` as given.
+__device__ __forceinline__ void sat_prefill(const PairInputs& a, uint32_t p, SatResult* r, uint32_t stage) {
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const ShapeRec& A = a.shapes[sa];
     const ShapeRec& B = a.shapes[sb];
     *reinterpret_cast<float4*>(r) =
-        make_float4(__uint_as_float(A.hullUnit), A.radius + B.radius, __uint_as_float(B.hullUnit), 0.0f);
-    r->stage = kStageOverlap;
+        make_float4(__uint_as_float(A.hullUnit), A.radius, __uint_as_float(B.hullUnit), B.radius);
+    r->stage = stage;
 }
 
 struct SatArgs : PairInputs {

@@ -670,7 +670,7 @@ __global__ __launch_bounds__(256) void cache_list_kernel(CacheArgs a) {
             if (fc && toSat) fc->state = kCacheEmpty;
         }
         *stage = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
-        if (toSat) sat_prefill(a, p, sat);
+        if (cls == 1u) sat_prefill(a, p, sat, toSat ? kStageOverlap : kStageSkip);
         else sat->stage = kStageSkip;
     }
     block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
@@ -717,13 +717,14 @@ struct CachedPair {
     uint32_t state0, type, i1, i2;
     __device__ static CachedPair load(const CacheArgs& a, uint32_t p, const FeatureCache* fc) {
         CachedPair c;
-        const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
-        c.r1 = a.shapes[sa].radius;
-        c.r2 = a.shapes[sb].radius;
+        // Record units and radii, as the list kernel stored them (sat_prefill).
+        const float4 pre = *reinterpret_cast<const float4*>(a.sat + p);
+        c.r1 = pre.y;
+        c.r2 = pre.w;
         c.xf1 = load_xf(a.xf_ptr(p, 0));
         c.xf2 = load_xf(a.xf_ptr(p, 1));
-        c.h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
-        c.h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
+        c.h1 = DevHull::at_unit(a.pool, __float_as_uint(pre.x));
+        c.h2 = DevHull::at_unit(a.pool, __float_as_uint(pre.z));
         c.state0 = fc->state;
         c.type = fc->type;
         c.i1 = fc->index1;
@@ -815,7 +816,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void c
         rebuild = lead && r == kCachedRebuild;
         if (miss) {
             fc->state = kCacheEmpty;
-            sat_prefill(a, p, a.sat + p);
+            a.sat[p].stage = kStageOverlap;
         }
     }
     wave_append(miss, p, a.satList, a.counts + kCountSat);
@@ -857,7 +858,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
             hits += r == kCachedHit ? 1u : 0u;
             if (r == kCachedMiss) {
                 fc->state = kCacheEmpty;
-                sat_prefill(a, p, a.sat + p);
+                a.sat[p].stage = kStageOverlap;
                 a.satList[atomicAdd(a.counts + kCountSat, 1u)] = p;
             }
             *stage = r == kCachedFallback ? kStageCacheFallback : kStageSkip;
@@ -879,7 +880,7 @@ __global__ __launch_bounds__(256) void cache_fallback_kernel(CacheArgs a) {
         const uint32_t st = cache_pair(a, p, 1u, out, fc);
         hit = st == kStageSkip;
         miss = !hit;
-        if (miss) sat_prefill(a, p, sat);
+        if (miss) sat->stage = kStageOverlap;
     }
     wave_append(miss, p, a.satList, a.counts + kCountSat);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);

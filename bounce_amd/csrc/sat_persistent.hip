@@ -415,10 +415,10 @@ __device__ __forceinline__ void edge_query_l(const Xf& xf, V3 C1, const UHull& h
 // (collide_hulls.cpp:206-289).
 __device__ __forceinline__ void sat_pair(const SatArgs& a, uint32_t p, SatStage& st, uint32_t lane,
                                          uint32_t maxSlots) {
-    // The listing kernel stored the record units and the total radius in the result slot.
+    // The list kernel stored the record units and the radii in the result slot.
     const CU32 pre = (const CU32)(a.out + p);
     const uint32_t ua = pre[0], ub = pre[2];
-    const float totalRadius = __uint_as_float(pre[1]);
+    const float totalRadius = __uint_as_float(pre[1]) + __uint_as_float(pre[3]);
     const Xf xf1 = load_xf_u((const CF32)a.xf_ptr(p, 0)), xf2 = load_xf_u((const CF32)a.xf_ptr(p, 1));
     const CF4 pool{(CF32)a.pool};
     const UHull h1 = UHull::at_unit(pool, ua);
