@@ -82,14 +82,26 @@ __device__ __forceinline__ bool mesh_pair_valid(const MeshArgs& a, uint32_t si) 
 // b3StaticTree::QueryAABB (static_tree.h:100-142) reporting into contact c's triangle list
 // (b3MeshContact::Report, mesh_contact.cpp:162-189): DFS, child2 first, every overlapping
 // leaf in order.  Returns the number of triangles found (the list keeps the first cap).
-__device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, uint32_t& flags) {
+// The DFS stack: its first kLdsStack entries in LDS (entry k of the lane at lds[256 k],
+// consecutive lanes in consecutive banks), deeper ones (only an unbalanced tree reaches
+// them) in private memory.
+constexpr uint32_t kLdsStack = 32;
+constexpr uint32_t kMidphaseBlock = 256;
+
+__device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, uint32_t& flags, uint32_t* lds) {
     uint32_t found = 0;
-    uint32_t stack[kTreeStack];
+    uint32_t deep[kTreeStack - kLdsStack];
     uint32_t top = 0;
-    if (a.mesh.nodeCount > 0) stack[top++] = 0;
+    auto push = [&](uint32_t v) {
+        if (top < kLdsStack) lds[kMidphaseBlock * top] = v;
+        else deep[top - kLdsStack] = v;
+        ++top;
+    };
+    if (a.mesh.nodeCount > 0) push(0u);
     uint32_t* list = a.triLists + (size_t)c * a.cap;
     while (top) {
-        const uint32_t ni = stack[--top];
+        --top;
+        const uint32_t ni = top < kLdsStack ? lds[kMidphaseBlock * top] : deep[top - kLdsStack];
         const float4* nd = reinterpret_cast<const float4*>(a.mesh.nodes + 8 * (size_t)ni);
         const float4 q0 = nd[0], q1 = nd[1];  // lower xyz, upper x | upper yz, child1, child2
         if (!overlaps(q0, q1, box)) continue;
@@ -98,8 +110,8 @@ __device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, u
             if (found < a.cap) list[found] = child2;
             ++found;
         } else if (top + 2 <= kTreeStack) {
-            stack[top++] = child1;
-            stack[top++] = child2;
+            push(child1);
+            push(child2);
         } else {
             flags |= kFlagTreeStackOverflow;
         }
@@ -125,7 +137,8 @@ __device__ __forceinline__ uint32_t list_work(const MeshArgs& a, uint32_t c, uin
 
 // Freshly created contacts: the fat AABB of shape A in the mesh frame (mesh_contact.cpp:30-60),
 // the tree query and the work items.
-__global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
+__global__ __launch_bounds__(kMidphaseBlock) void mesh_midphase_kernel(MeshArgs a) {
+    __shared__ uint32_t stackLds[kLdsStack * kMidphaseBlock];
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
     uint32_t flags = 0;
@@ -139,7 +152,7 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
         const V3 ext{0.2f, 0.2f, 0.2f};  // B3_AABB_EXTENSION (settings.h:54)
         box.lo = box.lo - ext;
         box.hi = box.hi + ext;
-        found = tree_query(a, c, box, flags);
+        found = tree_query(a, c, box, flags, stackLds + threadIdx.x);
     } else {
         flags |= kFlagInvalidPair;
     }
@@ -152,7 +165,8 @@ __global__ __launch_bounds__(256) void mesh_midphase_kernel(MeshArgs a) {
 // Persistent contacts: kMeshCreate (constructor + FindNewPairs) or kMeshSync
 // (SynchronizeShapes + MoveAABB + FindNewPairs, mesh_contact.cpp:62-160).  A query resets
 // the caches of the listed triangles to empty (b3MeshContact::Report).
-__global__ __launch_bounds__(256) void mesh_sync_kernel(MeshArgs a, uint32_t mode) {
+__global__ __launch_bounds__(kMidphaseBlock) void mesh_sync_kernel(MeshArgs a, uint32_t mode) {
+    __shared__ uint32_t stackLds[kLdsStack * kMidphaseBlock];
     const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= a.n) return;
     const uint32_t si = a.shapeA[c];
@@ -194,7 +208,7 @@ __global__ __launch_bounds__(256) void mesh_sync_kernel(MeshArgs a, uint32_t mod
     if (query) {
         fat[0] = box.lo.x, fat[1] = box.lo.y, fat[2] = box.lo.z;
         fat[3] = box.hi.x, fat[4] = box.hi.y, fat[5] = box.hi.z;
-        found = tree_query(a, c, box, flags);
+        found = tree_query(a, c, box, flags, stackLds + threadIdx.x);
         const uint32_t m = found < a.cap ? found : a.cap;
         for (uint32_t k = 0; k < m; ++k) {
             ConvexCache e;
@@ -540,13 +554,13 @@ __global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
 
 hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    mesh_midphase_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    mesh_midphase_kernel<<<(a.n + kMidphaseBlock - 1) / kMidphaseBlock, kMidphaseBlock, 0, s>>>(a);
     return hipGetLastError();
 }
 
 hipError_t launch_mesh_sync(const MeshArgs& a, uint32_t mode, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    mesh_sync_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a, mode);
+    mesh_sync_kernel<<<(a.n + kMidphaseBlock - 1) / kMidphaseBlock, kMidphaseBlock, 0, s>>>(a, mode);
     return hipGetLastError();
 }
 
