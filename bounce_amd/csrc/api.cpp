@@ -8,6 +8,7 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <limits>
 #include <map>
 #include <string>
 #include <unordered_map>
@@ -144,6 +145,19 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     words[2] = HE >> 1;
     std::memcpy(fw + 4, pool->centroids + 3 * (size_t)h, 12);
     for (uint32_t i = 0; i < V; ++i) std::memcpy(fw + 4 * (L.vertOff() + i), verts + 3 * i, 12);
+    {
+        // Bounding radius about the centroid, rounded up: no vertex lies farther than fw[7]
+        // (non-finite coordinates give a non-finite radius, which no bound test passes).
+        const float* c = pool->centroids + 3 * (size_t)h;
+        double r2 = 0.0;
+        for (uint32_t i = 0; i < V; ++i) {
+            const double dx = (double)verts[3 * i] - c[0], dy = (double)verts[3 * i + 1] - c[1],
+                         dz = (double)verts[3 * i + 2] - c[2];
+            const double q = dx * dx + dy * dy + dz * dz;
+            r2 = (q > r2 || q != q) ? q : r2;
+        }
+        fw[7] = std::nextafter((float)(std::sqrt(r2) * (1.0 + 1e-6)), std::numeric_limits<float>::infinity());
+    }
     for (uint32_t i = 0; i < F; ++i) std::memcpy(fw + 4 * (L.planeOff() + i), planes + 4 * i, 16);
     uint32_t* ue = words.data() + 4 * L.uedgeOff();
     for (uint32_t k = 0; k < L.E(); ++k) {

@@ -2,7 +2,7 @@
 //
 // One b3Hull (hull.h:37-64) becomes one contiguous block of 16-byte units in HBM:
 //   unit 0        header  x = V | F << 16, y = HE | maxFaceDegree << 16, z = HE / 2
-//   unit 1        centroid (x, y, z, 0)
+//   unit 1        centroid (x, y, z) and w = bounding radius (no vertex farther from it)
 //   units 2..     V vertices (x, y, z, 0)
 //   then          F planes (nx, ny, nz, offset)
 //   then          HE/2 unique edges, one u32 each: origin(2k) | origin(2k+1) << 8 |

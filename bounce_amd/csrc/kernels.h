@@ -126,16 +126,16 @@ struct SatSplit {
 
 // Per-batch scratch words: the stage word of each pair, the SAT list (pair indices) and the
 // rebuild lists.  The rebuild lists are kRebuildLists sub-lists, one counter each, so that
-// the per-wave appends of a large batch spread over many addresses: wave w of the cache-read
-// kernel (16 pairs) appends to sub-list w % kRebuildLists, whose capacity is
+// the per-wave appends of a large batch spread over many addresses: block b of the cache-read
+// kernel (256 pairs) appends to sub-list b % kRebuildLists, whose capacity is
 // rebuild_capacity(n).  The counter words (zeroed at the start of each batch): the SAT
 // kernel's tail counter, the contact list's length, the SAT list's length, and from
 // kCountRebuild the sub-lists' lengths.
 constexpr uint32_t kRebuildLists = 64u;
-constexpr uint32_t kReadPairsPerWave = 16u;
+constexpr uint32_t kReadPairsPerBlock = 256u;
 __host__ __device__ inline uint32_t rebuild_capacity(uint32_t n) {
-    const uint32_t waves = (n + kReadPairsPerWave - 1u) / kReadPairsPerWave;
-    return (waves + kRebuildLists - 1u) / kRebuildLists * kReadPairsPerWave;
+    const uint32_t blocks = (n + kReadPairsPerBlock - 1u) / kReadPairsPerBlock;
+    return (blocks + kRebuildLists - 1u) / kRebuildLists * kReadPairsPerBlock;
 }
 enum : uint32_t {
     kCountSatTail = 0, kCountContacts = 1, kCountSat = 2, kCountRebuild = 64,

@@ -695,6 +695,26 @@ __device__ __forceinline__ uint32_t group_read_face(uint32_t g, uint32_t index1,
     return sep > totalRadius ? kCacheSeparation : kCacheOverlap;
 }
 
+// Whether ReadFace's state is kCacheSeparation by the bounding sphere of hull 2 alone
+// (centroid c, radius R >= |v - c| for every vertex): max dot(-n, v) <= dot(-n, c) + |n| R
+// for the same plane (n, offset) as group_read_face computes.  The FP32 search result lies
+// within (9 u) |n| (|c| + R) of the exact maximum and the FP32 bound within as much of its
+// exact value, so when the bound clears totalRadius by more than 1e-5 (|n| (|c| + R) +
+// |offset|), far above those rounding terms, the search's separation is above totalRadius
+// too.  Non-finite inputs fail the comparison (and take the search).
+__device__ __forceinline__ bool face_bound_separated(uint32_t index1, const Xf& xf1, const DevHull& h1, const Xf& xf2,
+                                                     const DevHull& h2, float totalRadius) {
+    const Xf xf = mulT(xf2, xf1);
+    const Plane pl = mul(xf, h1.plane(index1));
+    const V3 d = -pl.n;
+    const float4 cs = h2.base[1];  // centroid, bounding radius
+    const V3 c{cs.x, cs.y, cs.z};
+    const float nd = sqrtf(dot(d, d)), cn = sqrtf(dot(c, c));
+    const float bound = dot(d, c) + nd * cs.w;
+    const float slack = 1e-5f * (nd * (cn + cs.w) + fabsf(pl.d));
+    return (-bound - pl.d) - slack > totalRadius;
+}
+
 // An empty manifold (all 240 bytes zero, as write_manifold writes pointCount 0) stored by a
 // group of G lanes, one 16-byte word per lane and store.
 template <uint32_t G>
@@ -739,28 +759,12 @@ struct CachedPair {
 };
 
 // b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the rebuilds,
-// for one cached pair, by a group of G lanes: ReadState (sat.cpp:177-278), the separation
-// hit (empty manifold written here), and the decision to rebuild: an overlap hit needs the
-// edge rebuild, or the face rebuild when the relative rotation allows it
-// (collide_hulls_cache.cpp:117-126; hull 2's face swaps the bodies, :325-329).
-template <uint32_t G>
-__device__ __forceinline__ uint32_t cached_read(const CacheArgs& a, uint32_t g, uint32_t p, Manifold* out,
-                                                const FeatureCache* fc) {
-    const CachedPair c = CachedPair::load(a, p, fc);
-    if (!c.in_range()) return kCachedMiss;
-    const float totalRadius = c.r1 + c.r2;
-    uint32_t state1;
-    if (c.type == kFeatureEdge1) {
-        state1 = read_edge(c.i1, c.i2, c.xf1, c.h1, c.xf2, c.h2, totalRadius);
-    } else {
-        const bool two = c.type == kFeatureFace2;
-        state1 = group_read_face<G>(g, c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2), pick(two, c.xf2, c.xf1),
-                                    pick(two, c.h2, c.h1), totalRadius);
-    }
-    if (c.state0 == kCacheSeparation && state1 == kCacheSeparation) {
-        group_write_empty<G>(out, g);
-        return kCachedHit;
-    }
+// given ReadState's new state (sat.cpp:177-278): the separation hit, or the decision to
+// rebuild: an overlap hit needs the edge rebuild, or the face rebuild when the relative
+// rotation allows it (collide_hulls_cache.cpp:117-126; hull 2's face swaps the bodies,
+// :325-329); anything else is a miss.
+__device__ __forceinline__ uint32_t cached_decide(const CacheArgs& a, uint32_t p, const CachedPair& c, uint32_t state1) {
+    if (c.state0 == kCacheSeparation && state1 == kCacheSeparation) return kCachedHit;
     if (c.state0 != kCacheOverlap || state1 != kCacheOverlap) return kCachedMiss;
     if (c.type == kFeatureEdge1) return kCachedRebuild;
     Quat q10, q1, q20, q2;
@@ -795,34 +799,84 @@ __device__ __forceinline__ uint32_t cached_rebuild(const CacheArgs& a, GroupLds&
 }
 
 constexpr uint32_t kReadGroup = 4;
-static_assert(64u / kReadGroup == kReadPairsPerWave, "cache-read wave width");
+constexpr uint32_t kCachedNone = 4;
+static_assert(kReadPairsPerBlock == 256u, "cache-read block: one lane per pair");
 
-// The pairs marked kStageCached, four lanes per pair (the grid covers the batch): the
-// feature-cache read.  Separation hits are finished here; overlap hits go to the rebuild
-// lists; a miss flushes the cache and appends the pair to the SAT list.  With counting on,
-// b3_convexCacheHits counts the separation hits (collide_hulls_cache.cpp:300).
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(8))) void cache_read_kernel(CacheArgs a) {
-    const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    const uint32_t p = t / kReadGroup, g = threadIdx.x & (kReadGroup - 1u);
-    bool miss = false, hit = false, rebuild = false;
+// The outcome of one cached pair's read (r: kCached*, or kCachedNone for no pair): a miss
+// flushes the cache and appends the pair to the SAT list, a rebuild appends it to the block's
+// rebuild sub-list.  Every lane of the block calls it together.
+__device__ __forceinline__ void cached_finish(const CacheArgs& a, uint32_t r, uint32_t p, FeatureCache* fc) {
+    if (r == kCachedMiss) {
+        fc->state = kCacheEmpty;
+        a.sat[p].stage = kStageOverlap;
+    }
+    wave_append(r == kCachedMiss, p, a.satList, a.counts + kCountSat);
+    const uint32_t k = blockIdx.x % kRebuildLists;
+    wave_append(r == kCachedRebuild, p, a.rebuildList + (size_t)k * rebuild_capacity(a.n), a.counts + kCountRebuild + k);
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, r == kCachedHit ? 1u : 0u);
+}
+
+// The pairs marked kStageCached: the feature-cache read (ReadState, sat.cpp:177-278), one
+// lane per pair over the block's 256 pairs.  An edge feature is read by its lane; a face
+// feature whose state the bounding sphere of the other hull settles (face_bound_separated)
+// needs no vertex; the remaining face features are queued in LDS and read by groups of four
+// lanes (group_read_face).  Separation hits get their empty manifold here, written by the
+// whole block; overlap hits go to the rebuild lists; a miss flushes the cache and appends
+// the pair to the SAT list.  With counting on, b3_convexCacheHits counts the separation hits
+// (collide_hulls_cache.cpp:300).
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void cache_read_kernel(CacheArgs a) {
+    __shared__ uint32_t readQ[256], hitQ[256], qn[2];
+    if (threadIdx.x < 2u) qn[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t p = blockIdx.x * 256u + threadIdx.x;
+    uint32_t r = kCachedNone;
+    FeatureCache* fc = nullptr;
     if (p < a.n && a.stage[p] == kStageCached) {
         const uint32_t slot = a.slot_of(p);
-        Manifold* const out = a.out + slot;
-        FeatureCache* const fc = &a.caches[slot].feature;
-        const uint32_t r = cached_read<kReadGroup>(a, g, p, out, fc);
-        const bool lead = g == 0;
-        hit = lead && r == kCachedHit;
-        miss = lead && r == kCachedMiss;
-        rebuild = lead && r == kCachedRebuild;
-        if (miss) {
-            fc->state = kCacheEmpty;
-            a.sat[p].stage = kStageOverlap;
+        fc = &a.caches[slot].feature;
+        const CachedPair c = CachedPair::load(a, p, fc);
+        const float totalRadius = c.r1 + c.r2;
+        if (!c.in_range()) {
+            r = kCachedMiss;
+        } else if (c.type == kFeatureEdge1) {
+            r = cached_decide(a, p, c, read_edge(c.i1, c.i2, c.xf1, c.h1, c.xf2, c.h2, totalRadius));
+        } else {
+            const bool two = c.type == kFeatureFace2;
+            if (face_bound_separated(c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2), pick(two, c.xf2, c.xf1),
+                                     pick(two, c.h2, c.h1), totalRadius))
+                r = cached_decide(a, p, c, kCacheSeparation);
+            else
+                readQ[atomicAdd(&qn[0], 1u)] = p;
         }
+        if (r == kCachedHit) hitQ[atomicAdd(&qn[1], 1u)] = slot;
     }
-    wave_append(miss, p, a.satList, a.counts + kCountSat);
-    const uint32_t k = (t / 64u) % kRebuildLists;
-    wave_append(rebuild, p, a.rebuildList + (size_t)k * rebuild_capacity(a.n), a.counts + kCountRebuild + k);
-    if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);
+    cached_finish(a, r, p, fc);
+    __syncthreads();
+    const uint32_t m = qn[0], nh = qn[1];
+    // The empty manifolds of the hits, 15 16-byte words each, across the block.
+    static_assert(sizeof(Manifold) == 15 * 16, "manifold of 15 16-byte words");
+    for (uint32_t w = threadIdx.x; w < 15u * nh; w += 256u)
+        reinterpret_cast<float4*>(a.out + hitQ[w / 15u])[w % 15u] = make_float4(0.f, 0.f, 0.f, 0.f);
+    // The queued face reads, 64 pairs per round.
+    for (uint32_t base = 0; base < m; base += 256u / kReadGroup) {
+        const uint32_t q = base + threadIdx.x / kReadGroup, g = threadIdx.x & (kReadGroup - 1u);
+        uint32_t r2 = kCachedNone, p2 = 0;
+        FeatureCache* fc2 = nullptr;
+        if (q < m) {
+            p2 = readQ[q];
+            const uint32_t slot = a.slot_of(p2);
+            fc2 = &a.caches[slot].feature;
+            const CachedPair c = CachedPair::load(a, p2, fc2);
+            const bool two = c.type == kFeatureFace2;
+            const uint32_t state1 = group_read_face<kReadGroup>(g, c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2),
+                                                                pick(two, c.xf2, c.xf1), pick(two, c.h2, c.h1),
+                                                                c.r1 + c.r2);
+            r2 = cached_decide(a, p2, c, state1);
+            if (r2 == kCachedHit) group_write_empty<kReadGroup>(a.out + slot, g);
+            if (g != 0) r2 = kCachedNone;
+        }
+        cached_finish(a, r2, p2, fc2);
+    }
 }
 
 // The rebuild lists, eight lanes per pair.  Group q of the grid's G groups takes entry
@@ -896,8 +950,7 @@ hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if (!a.caches) return hipSuccess;  // nothing is cached
     // The grids cover the batch; the stage words select the pairs.
-    const uint64_t readThreads = (uint64_t)a.n * kReadGroup;
-    cache_read_kernel<<<(uint32_t)((readThreads + 255) / 256), 256, 0, s>>>(a);
+    cache_read_kernel<<<(a.n + kReadPairsPerBlock - 1) / kReadPairsPerBlock, kReadPairsPerBlock, 0, s>>>(a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     // A grid of at most 2048 blocks (65,536 groups) walks the rebuild lists.
     uint64_t blocks = ((uint64_t)a.n * kGroup + 255) / 256;
