@@ -185,13 +185,17 @@ constexpr uint32_t kStageCacheFallback = 8u;
 // the listing kernel before any query: the hull record units (f1Index, f2Index) and the
 // radii (f1Sep, f2Sep) of shapes A and B, so later kernels read one record instead of the
 // pair's shape indices and then their shape records; `stage` as given.
-__device__ __forceinline__ void sat_prefill(const PairInputs& a, uint32_t p, SatResult* r, uint32_t stage) {
+__device__ __forceinline__ float4 prefill_units(const PairInputs& a, uint32_t p) {
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const ShapeRec& A = a.shapes[sa];
     const ShapeRec& B = a.shapes[sb];
-    *reinterpret_cast<float4*>(r) =
-        make_float4(__uint_as_float(A.hullUnit), A.radius, __uint_as_float(B.hullUnit), B.radius);
+    return make_float4(__uint_as_float(A.hullUnit), A.radius, __uint_as_float(B.hullUnit), B.radius);
+}
+__device__ __forceinline__ float4 sat_prefill(const PairInputs& a, uint32_t p, SatResult* r, uint32_t stage) {
+    const float4 pre = prefill_units(a, p);
+    *reinterpret_cast<float4*>(r) = pre;
     r->stage = stage;
+    return pre;
 }
 
 struct SatArgs : PairInputs {

@@ -637,42 +637,49 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t p, uint32_t* lis
     if (flag) list[base + mbcnt(m)] = p;
 }
 
-// The start of b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-297), one
-// lane per pair: invalid pairs get an empty manifold and the invalid flag, sphere/capsule
-// pairs go to the convex kernel, hull pairs with a cached feature are marked kStageCached
-// and the others (no cache, an empty one, or a state outside the enum) go to the SAT list
-// with their cache flushed.  ReadState on an empty cache evaluates face 0 of hull 1 and the
-// result is discarded by the flush (sat.cpp:182-185), so those go straight to the SAT.
-// With counting on, b3_convexCalls counts the hull pairs (collide_hulls.cpp:304).
+// The start of b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-297) for
+// pair p, by its lane: invalid pairs get an empty manifold and the invalid flag,
+// sphere/capsule pairs go to the convex kernel, hull pairs with a cached feature are marked
+// kStageCached (the return value) and the others (no cache, an empty one, or a state outside
+// the enum) go to the SAT list (toSat) with their cache flushed.  ReadState on an empty
+// cache evaluates face 0 of hull 1 and the result is discarded by the flush
+// (sat.cpp:182-185), so those go straight to the SAT.  `call` is 1 for a hull pair
+// (b3_convexCalls, collide_hulls.cpp:304); `pre` receives a hull pair's record units and
+// radii, stored in its SAT result slot.
+__device__ __forceinline__ bool list_pair(const CacheArgs& a, uint32_t p, bool& toSat, uint32_t& call, float4& pre) {
+    const uint32_t slot = a.slot_of(p);
+    Manifold* const out = a.out + slot;
+    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+    SatResult* const sat = a.sat + p;
+    const uint32_t cls = a.classify(p);
+    bool cached = false;
+    if (cls == 0u) {
+        Contact m;
+        m.pointCount = 0;
+        m.overflow = false;
+        write_manifold(out, m);
+        out->flags = kFlagInvalidPair;
+    } else if (cls == 1u) {
+        call = 1u;
+        cached = fc && fc->state < kCacheEmpty;
+        toSat = !cached;
+        if (fc && toSat) fc->state = kCacheEmpty;
+    }
+    a.stage[p] = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
+    if (cls == 1u) pre = sat_prefill(a, p, sat, toSat ? kStageOverlap : kStageSkip);
+    else sat->stage = kStageSkip;
+    return cached;
+}
+
+// The batch without feature caches, one lane per pair (list_pair); every hull pair goes to
+// the SAT list.
 __global__ __launch_bounds__(256) void cache_list_kernel(CacheArgs a) {
     __shared__ uint32_t sh[8];
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool toSat = false;
     uint32_t call = 0;
-    if (p < a.n) {
-        const uint32_t slot = a.slot_of(p);
-        Manifold* const out = a.out + slot;
-        FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
-        uint32_t* const stage = a.stage + p;
-        SatResult* const sat = a.sat + p;
-        const uint32_t cls = a.classify(p);
-        bool cached = false;
-        if (cls == 0u) {
-            Contact m;
-            m.pointCount = 0;
-            m.overflow = false;
-            write_manifold(out, m);
-            out->flags = kFlagInvalidPair;
-        } else if (cls == 1u) {
-            call = 1u;
-            cached = fc && fc->state < kCacheEmpty;
-            toSat = !cached;
-            if (fc && toSat) fc->state = kCacheEmpty;
-        }
-        *stage = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
-        if (cls == 1u) sat_prefill(a, p, sat, toSat ? kStageOverlap : kStageSkip);
-        else sat->stage = kStageSkip;
-    }
+    float4 pre;
+    if (p < a.n) list_pair(a, p, toSat, call, pre);
     block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCalls, call);
 }
@@ -736,9 +743,11 @@ struct CachedPair {
     float r1, r2;
     uint32_t state0, type, i1, i2;
     __device__ static CachedPair load(const CacheArgs& a, uint32_t p, const FeatureCache* fc) {
+        // Record units and radii, as list_pair stored them (sat_prefill).
+        return make(a, p, fc, *reinterpret_cast<const float4*>(a.sat + p));
+    }
+    __device__ static CachedPair make(const CacheArgs& a, uint32_t p, const FeatureCache* fc, float4 pre) {
         CachedPair c;
-        // Record units and radii, as the list kernel stored them (sat_prefill).
-        const float4 pre = *reinterpret_cast<const float4*>(a.sat + p);
         c.r1 = pre.y;
         c.r2 = pre.w;
         c.xf1 = load_xf(a.xf_ptr(p, 0));
@@ -816,25 +825,29 @@ __device__ __forceinline__ void cached_finish(const CacheArgs& a, uint32_t r, ui
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, r == kCachedHit ? 1u : 0u);
 }
 
-// The pairs marked kStageCached: the feature-cache read (ReadState, sat.cpp:177-278), one
-// lane per pair over the block's 256 pairs.  An edge feature is read by its lane; a face
-// feature whose state the bounding sphere of the other hull settles (face_bound_separated)
-// needs no vertex; the remaining face features are queued in LDS and read by groups of four
-// lanes (group_read_face).  Separation hits get their empty manifold here, written by the
-// whole block; overlap hits go to the rebuild lists; a miss flushes the cache and appends
-// the pair to the SAT list.  With counting on, b3_convexCacheHits counts the separation hits
+// The batch with feature caches: list_pair, then for the pairs marked kStageCached the
+// feature-cache read (ReadState, sat.cpp:177-278), one lane per pair over the block's 256
+// pairs.  An edge feature is read by its lane; a face feature whose state the bounding
+// sphere of the other hull settles (face_bound_separated) needs no vertex; the remaining
+// face features are queued in LDS and read by groups of four lanes (group_read_face).
+// Separation hits get their empty manifold here, written by the whole block; overlap hits
+// go to the rebuild lists; a miss flushes the cache and appends the pair to the SAT list.
+// With counting on, b3_convexCacheHits counts the separation hits
 // (collide_hulls_cache.cpp:300).
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void cache_read_kernel(CacheArgs a) {
-    __shared__ uint32_t readQ[256], hitQ[256], qn[2];
+    __shared__ uint32_t sh[8], readQ[256], hitQ[256], qn[2];
+    __shared__ float4 preQ[256];  // the queued pairs' record units and radii
     if (threadIdx.x < 2u) qn[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t p = blockIdx.x * 256u + threadIdx.x;
-    uint32_t r = kCachedNone;
+    bool toSat = false;
+    uint32_t call = 0, r = kCachedNone;
     FeatureCache* fc = nullptr;
-    if (p < a.n && a.stage[p] == kStageCached) {
+    float4 pre;
+    if (p < a.n && list_pair(a, p, toSat, call, pre)) {
         const uint32_t slot = a.slot_of(p);
         fc = &a.caches[slot].feature;
-        const CachedPair c = CachedPair::load(a, p, fc);
+        const CachedPair c = CachedPair::make(a, p, fc, pre);
         const float totalRadius = c.r1 + c.r2;
         if (!c.in_range()) {
             r = kCachedMiss;
@@ -845,11 +858,16 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
             if (face_bound_separated(c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2), pick(two, c.xf2, c.xf1),
                                      pick(two, c.h2, c.h1), totalRadius))
                 r = cached_decide(a, p, c, kCacheSeparation);
-            else
-                readQ[atomicAdd(&qn[0], 1u)] = p;
+            else {
+                const uint32_t q = atomicAdd(&qn[0], 1u);
+                readQ[q] = p;
+                preQ[q] = pre;
+            }
         }
         if (r == kCachedHit) hitQ[atomicAdd(&qn[1], 1u)] = slot;
     }
+    block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
+    if (a.counters) wave_count(&counter_slot(a.counters)->convexCalls, call);
     cached_finish(a, r, p, fc);
     __syncthreads();
     const uint32_t m = qn[0], nh = qn[1];
@@ -866,7 +884,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
             p2 = readQ[q];
             const uint32_t slot = a.slot_of(p2);
             fc2 = &a.caches[slot].feature;
-            const CachedPair c = CachedPair::load(a, p2, fc2);
+            const CachedPair c = CachedPair::make(a, p2, fc2, preQ[q]);
             const bool two = c.type == kFeatureFace2;
             const uint32_t state1 = group_read_face<kReadGroup>(g, c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2),
                                                                 pick(two, c.xf2, c.xf1), pick(two, c.h2, c.h1),
@@ -946,9 +964,10 @@ hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     hipError_t e = launch_zero_words(a.counts, kCountWords, s);
     if (e != hipSuccess) return e;
-    cache_list_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if (!a.caches) return hipSuccess;  // nothing is cached
+    if (!a.caches) {  // nothing is cached
+        cache_list_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+        return hipGetLastError();
+    }
     // The grids cover the batch; the stage words select the pairs.
     cache_read_kernel<<<(a.n + kReadPairsPerBlock - 1) / kReadPairsPerBlock, kReadPairsPerBlock, 0, s>>>(a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
