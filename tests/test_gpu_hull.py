@@ -92,6 +92,45 @@ def test_gpu_warm_start_sequence(ctx, lib, oracle):
         x[:, 1, 9:12] += rng.normal(0, 0.003, size=(len(pairs), 3)).astype(np.float32)
 
 
+def _corner_down_rotation():
+    """Rotation matrix taking the box corner direction (1, 1, 1) / sqrt(3) to (0, -1, 0)."""
+    a = np.ones(3) / np.sqrt(3.0)
+    b = np.array([0.0, -1.0, 0.0])
+    v = np.cross(a, b)
+    K = np.array([[0.0, -v[2], v[1]], [v[2], 0.0, -v[0]], [-v[1], v[0], 0.0]])
+    return np.eye(3) + K + K @ K / (1.0 + a @ b)
+
+
+def test_gpu_cached_face_bound_at_threshold(ctx, oracle):
+    """Face-cached box pairs whose separation crosses the radius sum in half-micrometre
+    steps, the upper box standing on a corner over the lower box's top face so that its
+    bounding sphere is tight along the face normal: the cached reads that the bounding
+    sphere settles and those that search the vertices both equal the reference's."""
+    say("case: cached face read at the bounding-sphere threshold")
+    pool = HullPool.concat([box_hull(1, 1, 1), box_hull(1, 1, 1)])
+    shapes = pool.shapes()
+    r = float(shapes["radius"][0]) + float(shapes["radius"][1])
+    R = _corner_down_rotation()
+    gaps = r + np.linspace(-1e-4, 3e-4, 801)
+    rng = np.random.default_rng(17)
+    n = len(gaps)
+    xfs = np.zeros((n, 2, 12), np.float32)
+    for i, g in enumerate(gaps):
+        xfs[i, 0] = identity_xf((0.0, 0.0, 0.0))
+        xfs[i, 1] = identity_xf((rng.uniform(-0.3, 0.3), 1.0 + np.sqrt(3.0) + g, rng.uniform(-0.3, 0.3)), R)
+    pairs = np.tile(np.array([[0, 1]], np.uint32), (n, 1))
+    cg, co = empty_caches(n), empty_caches(n)
+    for step in range(3):
+        got = run(ctx, pool, shapes, pairs, xfs, caches=cg)
+        want = oracle.collide(pool, shapes, pairs, xfs, caches=co)
+        assert_manifolds_equal(got, want, "step %d" % step)
+        assert_caches_equal(cg, co, "step %d" % step)
+        xfs[:, 1, 10] += np.float32(2e-6)
+    # Both cached states occur, on hull 1's top face.
+    assert (co["state"] == 0).any() and (co["state"] == 1).any()
+    assert (co["ftype"] == 1).all() and (co["index1"] == 3).all()
+
+
 def test_gpu_device_api_equals_host_api(ctx, lib):
     say("case: device-resident API")
     pool, shapes, pairs, xfs = hull_workload(lib, 606, 100, 3000)
