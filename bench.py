@@ -1,8 +1,9 @@
 """Narrowphase throughput on one MI355X: 1,048,576 batched hull-hull pairs.
 
 One step = one b3amd_collide_device batch over all pairs (feature-cache kernel, SAT
-kernel, contact kernel) with inputs resident in HBM and empty caches (every pair takes
-b3CollideCache, collide_hulls_cache.cpp:155-275).  Prints one JSON line.
+kernel, contact kernel) with inputs resident in HBM and no feature caches: every pair takes
+b3CollideHulls without a cache (collide_hulls.cpp:206-289), as b3CollideHullAndHull does
+with b3_convexCache off (collide_hulls.cpp:300-316).  Prints one JSON line.
 
     python bench.py [--steps K] [--warmup W] [--pairs N] [--pool P] [--no-cpu]
 """
@@ -636,7 +637,7 @@ def main():
         "config": {
             "workload": "batched hull-hull narrowphase: %d pairs over %d hulls of 8-64 points on ellipsoids "
                         "(semi-axes U[0.25,2] m), uniform orientations, SAT signed gap ~ N(-0.0127, 0.05) m, "
-                        "empty feature caches" % (n, pool.count),
+                        "no feature caches (every pair through the SAT)" % (n, pool.count),
             "pairs": n,
             "hull_pool": pool.count,
             "overlap_fraction": float((out["pointCount"] > 0).mean()),
