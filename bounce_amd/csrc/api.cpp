@@ -279,6 +279,8 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     cta.out = out;
     cta.list = sc.stage;  // the stage words are dead once the cache kernels are done
     cta.listCount = sc.counter + kCountContacts;
+    cta.satList = ca.satList;
+    cta.satCount = sc.counter + kCountSat;
     hipStream_t s = ctx->stream;
     if (ctx->roundShapes) {
         // Sphere/capsule pairs depend on nothing the hull kernels write.

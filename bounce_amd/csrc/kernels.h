@@ -217,6 +217,8 @@ struct ContactArgs : PairInputs {
     Manifold* out;           // [n]
     uint32_t* list;          // [n] the overlapping pairs (reuses the cache kernel's stage array)
     uint32_t* listCount;     // zeroed by the cache kernel
+    const uint32_t* satList;   // the pairs the SAT kernel ran, *satCount of them
+    const uint32_t* satCount;
 };
 
 // A pair whose clip polygon outgrows the group kernel's capacity: the lane-per-pair

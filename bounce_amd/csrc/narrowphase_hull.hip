@@ -443,10 +443,14 @@ __device__ __forceinline__ DevHull pick(bool c, const DevHull& a, const DevHull&
 // per block) so that the group kernel's waves hold only pairs that build a contact.
 __global__ __launch_bounds__(256) void contact_split_kernel(ContactArgs a) {
     __shared__ uint32_t waveCount[4], waveBase[4];
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const uint32_t count = *a.satCount;
+    if (blockIdx.x * blockDim.x >= count) return;  // the whole block is past the list
     const uint32_t wave = threadIdx.x >> 6;
     bool overlap = false;
-    if (p < a.n) {
+    uint32_t p = 0;
+    if (i < count) {
+        p = a.satList[i];
         const uint32_t slot = a.slot_of(p);
         Manifold* const out = a.out + slot;
         FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
@@ -645,7 +649,7 @@ __device__ __forceinline__ void wave_append(bool flag, uint32_t p, uint32_t* lis
 // cache evaluates face 0 of hull 1 and the result is discarded by the flush
 // (sat.cpp:182-185), so those go straight to the SAT.  `call` is 1 for a hull pair
 // (b3_convexCalls, collide_hulls.cpp:304); `pre` receives a hull pair's record units and
-// radii, stored in its SAT result slot.
+// radii, stored in its SAT result slot for a pair bound for the SAT.
 __device__ __forceinline__ bool list_pair(const CacheArgs& a, uint32_t p, bool& toSat, uint32_t& call, float4& pre) {
     const uint32_t slot = a.slot_of(p);
     Manifold* const out = a.out + slot;
@@ -666,8 +670,8 @@ __device__ __forceinline__ bool list_pair(const CacheArgs& a, uint32_t p, bool& 
         if (fc && toSat) fc->state = kCacheEmpty;
     }
     a.stage[p] = cls == 2u ? kStageConvex : (toSat ? kStageOverlap : (cached ? kStageCached : kStageSkip));
-    if (cls == 1u) pre = sat_prefill(a, p, sat, toSat ? kStageOverlap : kStageSkip);
-    else sat->stage = kStageSkip;
+    if (toSat) pre = sat_prefill(a, p, sat, kStageOverlap);
+    else if (cached) pre = prefill_units(a, p);  // stored only if the read sends it on
     return cached;
 }
 
@@ -813,11 +817,17 @@ static_assert(kReadPairsPerBlock == 256u, "cache-read block: one lane per pair")
 
 // The outcome of one cached pair's read (r: kCached*, or kCachedNone for no pair): a miss
 // flushes the cache and appends the pair to the SAT list, a rebuild appends it to the block's
-// rebuild sub-list.  Every lane of the block calls it together.
-__device__ __forceinline__ void cached_finish(const CacheArgs& a, uint32_t r, uint32_t p, FeatureCache* fc) {
+// rebuild sub-list; both store the record units and radii `pre` in the pair's SAT result
+// slot, where the SAT and rebuild kernels read them.  Every lane of the block calls it
+// together.
+__device__ __forceinline__ void cached_finish(const CacheArgs& a, uint32_t r, uint32_t p, FeatureCache* fc,
+                                              float4 pre) {
     if (r == kCachedMiss) {
         fc->state = kCacheEmpty;
+        *reinterpret_cast<float4*>(a.sat + p) = pre;
         a.sat[p].stage = kStageOverlap;
+    } else if (r == kCachedRebuild) {
+        *reinterpret_cast<float4*>(a.sat + p) = pre;
     }
     wave_append(r == kCachedMiss, p, a.satList, a.counts + kCountSat);
     const uint32_t k = blockIdx.x % kRebuildLists;
@@ -868,7 +878,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
     }
     block_append(toSat, p, a.satList, a.counts + kCountSat, sh);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCalls, call);
-    cached_finish(a, r, p, fc);
+    cached_finish(a, r, p, fc, pre);
     __syncthreads();
     const uint32_t m = qn[0], nh = qn[1];
     // The empty manifolds of the hits, 15 16-byte words each, across the block.
@@ -880,11 +890,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
         const uint32_t q = base + threadIdx.x / kReadGroup, g = threadIdx.x & (kReadGroup - 1u);
         uint32_t r2 = kCachedNone, p2 = 0;
         FeatureCache* fc2 = nullptr;
+        float4 pre2 = make_float4(0.f, 0.f, 0.f, 0.f);
         if (q < m) {
             p2 = readQ[q];
+            pre2 = preQ[q];
             const uint32_t slot = a.slot_of(p2);
             fc2 = &a.caches[slot].feature;
-            const CachedPair c = CachedPair::make(a, p2, fc2, preQ[q]);
+            const CachedPair c = CachedPair::make(a, p2, fc2, pre2);
             const bool two = c.type == kFeatureFace2;
             const uint32_t state1 = group_read_face<kReadGroup>(g, c.i1, pick(two, c.xf1, c.xf2), pick(two, c.h1, c.h2),
                                                                 pick(two, c.xf2, c.xf1), pick(two, c.h2, c.h1),
@@ -893,7 +905,7 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(6))) void c
             if (r2 == kCachedHit) group_write_empty<kReadGroup>(a.out + slot, g);
             if (g != 0) r2 = kCachedNone;
         }
-        cached_finish(a, r2, p2, fc2);
+        cached_finish(a, r2, p2, fc2, pre2);
     }
 }
 
