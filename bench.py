@@ -29,6 +29,33 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
+# The CPU share of a one-GPU box is 16 host cores: worker pools stay within it whatever
+# nproc reports, and every cpu_baseline object states nproc, the affinity set and the CPU
+# model beside the number of processes used.
+CPU_SHARE = 16
+
+
+def host_cpu():
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    aff = len(os.sched_getaffinity(0))
+    return {"nproc": os.cpu_count(), "affinity_cores": aff, "cpu_model": model,
+            "processes": max(1, min(CPU_SHARE, aff))}
+
+
+def wall_span(spans):
+    """Wall time of work timed in several processes: from the first start to the last end
+    (time.perf_counter is the system-wide monotonic clock on Linux)."""
+    return max(e for s, e in spans) - min(s for s, e in spans)
+
+
 def workload(lib, n_pairs, n_pool):
     from bounce_amd.workloads import gen_hull_pool, gen_hull_pairs
     pool = gen_hull_pool(lib, 2024, n_pool, 8, 64, 0.25, 2.0)
@@ -49,7 +76,7 @@ def _cpu_worker(chunk):
     pairs, xfs = j["pairs"][chunk], j["xfs"][chunk]
     t = time.perf_counter()
     m = chk.collide(j["pool"], j["shapes"], pairs, xfs)
-    return time.perf_counter() - t, int((m["pointCount"] > 0).sum())
+    return t, time.perf_counter(), int((m["pointCount"] > 0).sum())
 
 
 def cpu_baseline(pool, pairs, xfs, sample):
@@ -59,7 +86,8 @@ def cpu_baseline(pool, pairs, xfs, sample):
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     kind = "reference" if oracle_lib.reference() is not None else "port"
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    host = host_cpu()
+    cores = host["processes"]
     stride = max(1, len(pairs) // sample)
     idx = np.arange(0, len(pairs), stride)[:sample]
     chunks = np.array_split(idx, cores)
@@ -67,19 +95,18 @@ def cpu_baseline(pool, pairs, xfs, sample):
     ctx = mp.get_context("fork")
     with ctx.Pool(cores) as p:
         p.map(_cpu_worker, [c[:64] for c in chunks])  # load the library in every worker
-        t0 = time.perf_counter()
         res = p.map(_cpu_worker, chunks, chunksize=1)
-        wall = time.perf_counter() - t0
-    cpu_s = sum(r[0] for r in res)
-    return {
+    wall = wall_span([(r[0], r[1]) for r in res])
+    cpu_s = sum(r[1] - r[0] for r in res)
+    return dict({
         "value": len(idx) / wall,
         "unit": "pairs/s",
         "cores": cores,
         "kind": kind,
-        "sample": "%d of the %d pairs (every %d-th), %d processes, %.1f CPU-s, %s" % (
-            len(idx), len(pairs), stride, cores, cpu_s, os.uname().machine),
+        "sample": "%d of the %d pairs (every %d-th), %d processes, %.1f CPU-s, wall %.2f s, %s" % (
+            len(idx), len(pairs), stride, cores, cpu_s, wall, os.uname().machine),
         "per_core_pairs_per_s": len(idx) / cpu_s,
-    }
+    }, **host)
 
 
 # ----------------------------------------------------------------- algorithmic work
@@ -164,7 +191,7 @@ def _mesh_cpu_worker(chunk):
     t0 = time.perf_counter()
     oracle_lib.collide_mesh(lib, prefix, j["pool"], j["shapes"], j["verts"], j["tris"], j["nodes"],
                             chunk.astype(np.uint32), j["xfs"][chunk])
-    return time.perf_counter() - t0
+    return t0, time.perf_counter()
 
 
 def mesh_cpu_baseline(pool, shapes, verts, tris, xfs, sample):
@@ -172,15 +199,17 @@ def mesh_cpu_baseline(pool, shapes, verts, tris, xfs, sample):
     import oracle_lib
     kind = "reference" if oracle_lib.reference() is not None else "port"
     nodes = oracle_lib.build_tree(oracle_lib.oracle().lib, "b3o_", verts, tris)
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    host = host_cpu()
+    cores = host["processes"]
     idx = np.arange(0, len(shapes), max(1, len(shapes) // sample))[:sample]
     _MESH_JOB.update(pool=pool, shapes=shapes, verts=verts, tris=tris, nodes=nodes, xfs=xfs)
     with mp.get_context("fork").Pool(cores) as p:
         res = p.map(_mesh_cpu_worker, np.array_split(idx, cores), chunksize=1)
-    cpu_s = sum(res)
-    return {"value": len(idx) / (cpu_s / cores), "unit": "contacts/s", "cores": cores, "kind": kind,
-            "sample": "%d of the %d contacts (strided), %d processes, %.1f CPU-s (tree built beforehand)"
-                      % (len(idx), len(shapes), cores, cpu_s)}
+    wall = wall_span(res)
+    cpu_s = sum(e - s for s, e in res)
+    return dict({"value": len(idx) / wall, "unit": "contacts/s", "cores": cores, "kind": kind,
+                 "sample": "%d of the %d contacts (strided), %d processes, %.1f CPU-s, wall %.2f s (tree built "
+                           "beforehand in each process)" % (len(idx), len(shapes), cores, cpu_s, wall)}, **host)
 
 
 def mesh_run(world, steps, cpu):
@@ -258,16 +287,12 @@ def _warm_cpu_worker(chunk):
     caches = empty_caches(len(contacts))
     inputs = [warm_world_pair_inputs(w, xfs, sw, contacts) for xfs, sw in j["states"]]
     chk.collide(w["pool"], w["shapes"], inputs[0][0][:8], inputs[0][1][:8])  # load the library
-    cold = warm = 0.0
-    for s, (pairs, pxf, psw) in enumerate(inputs):
+    spans = []
+    for pairs, pxf, psw in inputs:
         t0 = time.perf_counter()
         out = chk.collide(w["pool"], w["shapes"], pairs, pxf, sweeps=psw, caches=caches)
-        dt = time.perf_counter() - t0
-        if s == 0:
-            cold += dt
-        else:
-            warm += dt
-    return cold, warm, out, caches
+        spans.append((t0, time.perf_counter()))
+    return spans[0], (spans[1][0], spans[-1][1]), out, caches
 
 
 def warm_cpu_baseline(world, stride):
@@ -278,21 +303,24 @@ def warm_cpu_baseline(world, stride):
     import oracle_lib
     from bounce_amd.workloads import warm_world_state
     kind = "reference" if oracle_lib.reference() is not None else "port"
-    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    host = host_cpu()
+    cores = host["processes"]
     idx = np.arange(0, len(world["contacts"]), stride)
     _WARM_JOB.update(kind=kind, world=world, states=[warm_world_state(world, s) for s in range(WARM_CPU_STEPS)])
     with mp.get_context("fork").Pool(cores) as p:
         res = p.map(_warm_cpu_worker, np.array_split(idx, cores), chunksize=1)
-    cold = sum(r[0] for r in res)
-    warm = sum(r[1] for r in res)
+    cold = wall_span([r[0] for r in res])
+    warm = wall_span([r[1] for r in res])
+    cpu_s = sum(r[0][1] - r[0][0] + r[1][1] - r[1][0] for r in res)
     out = np.concatenate([r[2] for r in res])
     caches = np.concatenate([r[3] for r in res])
-    return {"value": len(idx) * (WARM_CPU_STEPS - 1) / (warm / cores), "unit": "contacts/s (warm steps)",
-            "cores": cores, "kind": kind,
-            "cold_step_contacts_per_s": len(idx) / (cold / cores),
-            "sample": "every %d-th contact (%d of %d), steps 0-%d with caches carried, %d processes, "
-                      "%.1f CPU-s" % (stride, len(idx), len(world["contacts"]), WARM_CPU_STEPS - 1, cores,
-                                      cold + warm)}, idx, out, caches
+    return dict({"value": len(idx) * (WARM_CPU_STEPS - 1) / warm, "unit": "contacts/s (warm steps)",
+                 "cores": cores, "kind": kind,
+                 "cold_step_contacts_per_s": len(idx) / cold,
+                 "sample": "every %d-th contact (%d of %d), steps 0-%d with caches carried, %d processes, "
+                           "%.1f CPU-s, wall %.2f s cold + %.2f s warm" % (
+                               stride, len(idx), len(world["contacts"]), WARM_CPU_STEPS - 1, cores, cpu_s,
+                               cold, warm)}, **host), idx, out, caches
 
 
 def warm_world_run(world, steps, cpu):

@@ -82,34 +82,67 @@ int b3amd::ensure_scratch(b3amd_context* ctx, uint32_t n) {
 
 namespace {
 
-// Converts one b3Hull view into the device record (device_hull.h); returns an error
-// string or nullptr.
-// Bounding cap of the Gauss-map arc between unit normals U and V (the shorter great-circle
-// arc): centre m = (U + V) / |U + V|, radius angle(U, V) / 2 = atan2(|U - V|, |U + V|),
-// widened by 0.01 rad; returned as cos and sin of the widened radius.  A radius of pi or
-// more (or U = -V) covers the sphere: cos = -1, sin = 0, m = 0.
-void arc_cap(const float* U, const float* V, float* m, float* cosA, float* sinA) {
-    double u[3], v[3], nu = 0.0, nv = 0.0;
-    for (int i = 0; i < 3; ++i) u[i] = U[i], v[i] = V[i], nu += u[i] * u[i], nv += v[i] * v[i];
-    nu = std::sqrt(nu), nv = std::sqrt(nv);
+// Bounding cap of the Gauss-map arc of an edge with face normals U, V and direction E
+// (origin to twin origin): a region containing every crossing that b3IsMinkowskiFace
+// (sat.cpp:60-71), called with (U, V, -E) in place of (A, B, B x A), can accept for an edge
+// pair that b3Project scores.  It is the shorter great-circle arc from U to V, centre
+// m = (U + V) / |U + V|, radius angle(U, V) / 2 = atan2(|U - V|, |U + V|), widened by a
+// margin; returned as m and cos, sin of the widened radius.
+//
+// Why the margin: the test follows the great circle normal to -E, not the arc's own circle
+// (normal V x U).  With delta the angle between the two circles, a crossing it accepts lies
+// within about 2 delta + delta / sin(phi) of the arc, phi being the angle between the two
+// edges; b3Project returns -FLT_MAX below sin(phi) = 0.005 (sat.cpp:88-95), so those hits
+// never count and delta / sin(phi) <= 200 delta.  delta is bounded by the angle between
+// V x U and -E here plus what the SAT's FP32 arithmetic adds when the edge is hull 1's,
+// transformed into hull 2's frame: the rotated normals (about 4e-7 / |V x U|) and the
+// transformed end points (about 2.4e-6 posScale / |E|, posScale the distance of the
+// hull's farthest vertex from its origin).  The margin is 0.001 rad (FP32 rounding of the
+// test itself, below 1e-4 rad at sin(phi) >= 0.005) + 210 delta.
+//
+// The whole sphere is returned where no such cap exists or the shorter arc is not the
+// tested one: normals nearly opposite (|U + V| < 0.02, as in flat two-face hulls), -E
+// against or far from V x U (a reflex or degenerate edge, where the test follows the longer
+// arc: delta > 1e-3 or |V x U| < 8 delta), a widened radius beyond pi/2 + 0.01, or zero or
+// non-finite input.  It is encoded as m = 0, cos 0, sin 1, for which the kernel's test
+// sin a1 sin a2 - cos a1 cos a2 + M1.M2 >= 0 reduces to sin a2 >= 0 and passes every pair,
+// every term still at most 1 in magnitude (so the half-rounding bound of that test holds).
+void arc_cap(const float* U, const float* V, const float* E, double posScale, float* m, float* cosA,
+             float* sinA) {
+    double u[3], v[3], e[3], nu = 0.0, nv = 0.0, ne = 0.0;
+    for (int i = 0; i < 3; ++i) {
+        u[i] = U[i], v[i] = V[i], e[i] = -(double)E[i];
+        nu += u[i] * u[i], nv += v[i] * v[i], ne += e[i] * e[i];
+    }
+    nu = std::sqrt(nu), nv = std::sqrt(nv), ne = std::sqrt(ne);
     double s[3], d[3], ls = 0.0, ld = 0.0;
     for (int i = 0; i < 3; ++i) {
-        u[i] /= nu, v[i] /= nv;
+        u[i] /= nu, v[i] /= nv, e[i] /= ne;
         s[i] = u[i] + v[i], d[i] = u[i] - v[i];
         ls += s[i] * s[i], ld += d[i] * d[i];
     }
     ls = std::sqrt(ls), ld = std::sqrt(ld);
-    const double a = std::atan2(ld, ls) + 0.01;
-    if (ls < 1e-9 || a >= 3.141592653589793) {
+    const double c[3] = {v[1] * u[2] - v[2] * u[1], v[2] * u[0] - v[0] * u[2], v[0] * u[1] - v[1] * u[0]};
+    const double lc = std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+    const double x[3] = {c[1] * e[2] - c[2] * e[1], c[2] * e[0] - c[0] * e[2], c[0] * e[1] - c[1] * e[0]};
+    const double delta = std::atan2(std::sqrt(x[0] * x[0] + x[1] * x[1] + x[2] * x[2]),
+                                    c[0] * e[0] + c[1] * e[1] + c[2] * e[2]) +
+                         4e-7 / lc + 2.4e-6 * posScale / ne;
+    const double a = std::atan2(ld, ls) + 0.001 + 210.0 * delta;
+    const bool shortArc = ls >= 0.02 && delta <= 1e-3 && lc >= 8.0 * delta && a <= 1.5807963267948966;
+    if (!shortArc) {  // false for NaN too
         m[0] = m[1] = m[2] = 0.0f;
-        *cosA = -1.0f;
-        *sinA = 0.0f;
+        *cosA = 0.0f;
+        *sinA = 1.0f;
         return;
     }
     for (int i = 0; i < 3; ++i) m[i] = (float)(s[i] / ls);
     *cosA = (float)std::cos(a);
     *sinA = (float)std::sin(a);
 }
+
+// Converts one b3Hull view into the device record (device_hull.h); returns an error
+// string or nullptr.
 
 const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint32_t>& words,
                       HullLayout* layoutOut) {
@@ -178,6 +211,12 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
     float* gc = fw + 4 * L.capOff();
     _Float16* gf = reinterpret_cast<_Float16*>(fw + 4 * L.filterOff());
     for (uint32_t k = L.E(); k < L.gaussEdges(); ++k) gc[4 * k + 3] = 1e30f;
+    double posScale = 0.0;  // farthest vertex from the hull's origin
+    for (uint32_t i = 0; i < V; ++i) {
+        const double q = std::sqrt((double)verts[3 * i] * verts[3 * i] + (double)verts[3 * i + 1] * verts[3 * i + 1] +
+                                   (double)verts[3 * i + 2] * verts[3 * i + 2]);
+        posScale = q > posScale ? q : posScale;
+    }
     for (uint32_t k = 0; k < L.E(); ++k) {
         const uint32_t* a = edges + 8 * k;
         const uint32_t* b = a + 4;
@@ -187,7 +226,7 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
         const float* Vn = planes + 4 * b[2];
         const float E[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
         float m[3], ca, sa;
-        arc_cap(U, Vn, m, &ca, &sa);
+        arc_cap(U, Vn, E, posScale, m, &ca, &sa);
         float* r = gm + 12 * k;
         r[0] = -U[0], r[1] = -U[1], r[2] = -U[2], r[3] = -Vn[0];
         r[4] = -Vn[1], r[5] = -Vn[2], r[6] = -E[0], r[7] = -E[1];
@@ -239,7 +278,7 @@ int validate_pairs(const b3amd_context* ctx, uint32_t n, const uint32_t* pairs) 
 
 int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs,
                          const float* sweeps, ConvexCache* caches, Manifold* out, const BatchScratch& sc,
-                         const uint32_t* slots, const BodyRefs* bodies) {
+                         const uint32_t* slots, const BodyRefs* bodies, const BatchStreams* streams) {
     int rc = B3AMD_OK;
     PairInputs in;
     in.n = n;
@@ -281,22 +320,23 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     cta.listCount = sc.counter + kCountContacts;
     cta.satList = ca.satList;
     cta.satCount = sc.counter + kCountSat;
-    hipStream_t s = ctx->stream;
+    const BatchStreams ss = streams ? *streams : BatchStreams{ctx->stream, ctx->side, ctx->forkEvent, ctx->joinEvent};
+    hipStream_t s = ss.main;
     if (ctx->roundShapes) {
         // Sphere/capsule pairs depend on nothing the hull kernels write.
         ConvexArgs cv;
         static_cast<PairInputs&>(cv) = in;
         cv.out = out;
-        HIP_TRY(hipEventRecord(ctx->forkEvent, s));
-        HIP_TRY(hipStreamWaitEvent(ctx->side, ctx->forkEvent, 0));
-        if ((rc = timed(ctx, ctx->side, "convex", [&] { return launch_convex(cv, ctx->side); }))) return rc;
-        HIP_TRY(hipEventRecord(ctx->joinEvent, ctx->side));
+        HIP_TRY(hipEventRecord(ss.fork, s));
+        HIP_TRY(hipStreamWaitEvent(ss.side, ss.fork, 0));
+        if ((rc = timed(ctx, ss.side, "convex", [&] { return launch_convex(cv, ss.side); }))) return rc;
+        HIP_TRY(hipEventRecord(ss.join, ss.side));
     }
     if ((rc = timed(ctx, s, "cache_hull", [&] { return launch_cache_hull(ca, s); }))) return rc;
     const SatTuning t = ctx->tuning;
     if ((rc = timed(ctx, s, "sat_hull", [&] { return launch_sat_persistent(sa, t, s); }))) return rc;
     if ((rc = timed(ctx, s, "contact_hull", [&] { return launch_contact_hull(cta, s); }))) return rc;
-    if (ctx->roundShapes) HIP_TRY(hipStreamWaitEvent(s, ctx->joinEvent, 0));
+    if (ctx->roundShapes) HIP_TRY(hipStreamWaitEvent(s, ss.join, 0));
     return B3AMD_OK;
 }
 
@@ -610,6 +650,25 @@ int b3amd_build_static_tree(uint32_t vertexCount, const float* vertices, uint32_
     if (!build_static_tree(vertices, vertexCount, triangles, triangleCount, nodes))
         return fail(B3AMD_ERR_ARGUMENT, "triangle refers to a vertex out of range");
     std::memcpy(nodesOut, nodes.data(), sizeof(TreeNode) * nodes.size());
+    return B3AMD_OK;
+}
+
+int b3amd_hull_edge_caps(const b3amd_hull_pool* pool, uint32_t hull, float* capsOut) {
+    if (!pool || !capsOut || hull >= pool->count) return fail(B3AMD_ERR_ARGUMENT, "null pool or hull out of range");
+    std::vector<uint32_t> words;
+    HullLayout L{};
+    if (const char* err = pack_hull(pool, hull, words, &L)) return fail(B3AMD_ERR_ARGUMENT, err);
+    const float* fw = reinterpret_cast<const float*>(words.data());
+    const float* gm = fw + 4 * L.gaussOff();
+    const float* gc = fw + 4 * L.capOff();
+    for (uint32_t k = 0; k < L.E(); ++k) {
+        // The record holds -m and cos (cap unit) and sin (Gauss-map word 9).
+        capsOut[5 * k + 0] = -gc[4 * k + 0];
+        capsOut[5 * k + 1] = -gc[4 * k + 1];
+        capsOut[5 * k + 2] = -gc[4 * k + 2];
+        capsOut[5 * k + 3] = gc[4 * k + 3];
+        capsOut[5 * k + 4] = gm[12 * k + 9];
+    }
     return B3AMD_OK;
 }
 

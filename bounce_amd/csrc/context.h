@@ -45,6 +45,14 @@ struct BodyRefs {
     const float* sweeps;     // [B][8] or null
 };
 
+// The streams a batch runs on: the hull kernels on `main`; sphere/capsule pairs forked to
+// `side` after `fork` is recorded on main, joined back through `join`.  All at the default
+// priority.
+struct BatchStreams {
+    hipStream_t main, side;
+    hipEvent_t fork, join;
+};
+
 // Per-batch device scratch: scratch_words(capacity) words (stage, SAT list, rebuild lists),
 // the SAT result per pair, and the counter words (kCountWords).
 struct BatchScratch {
@@ -181,11 +189,11 @@ namespace b3amd {
 
 // Grows the context's per-batch scratch to n pairs.
 int ensure_scratch(b3amd_context* ctx, uint32_t n);
-// Enqueues the kernels of one batch on the context stream (no synchronisation); slots maps
-// each pair to its cache/manifold slot (null: slot p).
+// Enqueues the kernels of one batch on the context's streams, or on `streams` when given
+// (no synchronisation); slots maps each pair to its cache/manifold slot (null: slot p).
 int enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const float* xfs, const float* sweeps,
                   ConvexCache* caches, Manifold* out, const BatchScratch& sc, const uint32_t* slots,
-                  const BodyRefs* bodies = nullptr);
+                  const BodyRefs* bodies = nullptr, const BatchStreams* streams = nullptr);
 
 }  // namespace b3amd
 

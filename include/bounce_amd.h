@@ -346,6 +346,13 @@ int b3amd_set_profiling(b3amd_context* ctx, int enabled);
 int b3amd_kernel_stats(b3amd_context* ctx, const char* name, uint32_t* launches, double* total_ms);
 int b3amd_reset_kernel_stats(b3amd_context* ctx);
 
+/* ---- SAT edge-query filter data (host only; no reference counterpart) ----
+ * For each unique edge k of hull `hull` (half-edges 2k, 2k+1), the bounding cap the SAT
+ * kernel tests before the exact b3IsMinkowskiFace (sat.cpp:60-71): capsOut[5k..5k+4] =
+ * centre m (3 floats), cos and sin of the widened radius, as the hull record stores them
+ * at upload.  A whole-sphere cap is m = 0, cos 0, sin 1. */
+int b3amd_hull_edge_caps(const b3amd_hull_pool* pool, uint32_t hull, float* capsOut);
+
 /* ---- workload generators (seeded, deterministic on every machine) ---- */
 
 /* `count` hulls from N ~ U{nmin..nmax} points on ellipsoids with semi-axes ~ U[amin,amax].

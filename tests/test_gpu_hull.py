@@ -291,3 +291,51 @@ def test_gpu_large_faces_fallback(ctx, oracle):
     assert (want["pointCount"] > 0).mean() > 0.9
     got = run(ctx, pool, pool.shapes(), pairs, xfs)
     assert_manifolds_equal(got, want, "prism caps")
+
+    # Warm steps with feature caches and sweeps: cached face features on 12-gon caps are
+    # rebuilt (collide_hulls_cache.cpp:113-153) with an incident polygon of 12 vertices, beyond
+    # the group kernel's 8, so they take the lane-per-pair cache fallback.
+    say("case: large faces, warm steps with caches")
+    cg, co = empty_caches(n), empty_caches(n)
+    x = xfs.copy()
+    rebuilt_12gon = 0
+    for step in range(3):
+        sweeps = np.zeros((n, 4, 4), np.float32)
+        sweeps[:, :, 3] = 1.0
+        if step > 0:
+            yaw = rng.uniform(0.0, 0.04, n).astype(np.float32)
+            sweeps[:, 3, 1] = yaw
+            sweeps[:, 3, 3] = np.sqrt(1.0 - yaw * yaw)
+            both12 = (pairs[:, 0] == 2) & (pairs[:, 1] == 2)
+            face_overlap = (co["state"] == 1) & (co["ftype"] != 0)
+            rebuilt_12gon += int((both12 & face_overlap).sum())
+        got = run(ctx, pool, pool.shapes(), pairs, x, sweeps=sweeps, caches=cg)
+        want = oracle.collide(pool, pool.shapes(), pairs, x, sweeps=sweeps, caches=co)
+        assert_manifolds_equal(got, want, "prism caps, warm step %d" % step)
+        assert_caches_equal(cg, co, "prism caps, warm step %d" % step)
+        x[:, 1, 9:12] += rng.normal(0.0, 0.0005, size=(n, 3)).astype(np.float32)
+    assert rebuilt_12gon >= 10, rebuilt_12gon
+
+
+def test_gpu_degenerate_hulls(ctx, oracle):
+    """Flat two-face plates (edge arcs are half circles), knife edges, boxes at identity and
+    at 45 degrees, identical placements and 12-gon prisms, against each other in every
+    combination: the SAT kernel's arc-cap filter keeps every edge hit the reference scores,
+    so manifolds and caches equal the oracle's bit for bit."""
+    from test_cap_filter import degenerate_pairs, degenerate_pool
+    say("case: degenerate hulls (plates, knife edges, axis-aligned boxes)")
+    pool = degenerate_pool()
+    pairs, xfs = degenerate_pairs(seed=9, per_kind=24)
+    shapes = pool.shapes()
+    want = oracle.collide(pool, shapes, pairs, xfs)
+    got = run(ctx, pool, shapes, pairs, xfs)
+    assert_manifolds_equal(got, want, "degenerate hulls")
+    cg, co = empty_caches(len(pairs)), empty_caches(len(pairs))
+    got = run(ctx, pool, shapes, pairs, xfs, caches=cg)
+    oracle.collide(pool, shapes, pairs, xfs, caches=co)
+    assert_manifolds_equal(got, want, "degenerate hulls, caches")
+    assert_caches_equal(cg, co, "degenerate hulls")
+    plates = (pairs[:, 0] < 2) & (pairs[:, 1] < 2)
+    assert (want["pointCount"][plates] > 0).sum() > 20
+    # Edge-feature results occur among the plate pairs (the case whole-sphere caps fix).
+    assert ((co["ftype"] == 0) & (co["state"] != 2) & plates).sum() > 0
