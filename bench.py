@@ -485,34 +485,44 @@ def measured_traffic(n_pairs):
 
 def serving_run(lib, worlds_n, rate, duration, seed=4096):
     """Serving workload: `worlds_n` worlds (10-2000 bodies, log-uniform; boxes, spheres and
-    8-32-point hulls in resting stacks), each step one request.  Measures the flood
-    capacity (two warm steps of every world submitted at once) and the latency of Poisson
-    arrivals at `rate` steps/s, and at half the capacity, for `duration` seconds each."""
+    8-32-point hulls in stacks and piles, contacts from fat-AABB overlaps), each step one
+    request.  Measures the flood capacity (two warm steps of every world submitted at once)
+    with one lane and with three overlapping lanes, the latency of Poisson arrivals at
+    `rate` steps/s and at half the capacity for `duration` seconds each, and beside them the
+    plain drop-in: one b3amd_collide call on host arrays per world step."""
     import bounce_amd
-    from bounce_amd.workloads import gen_serving_worlds
+    from bounce_amd.types import empty_caches
+    from bounce_amd.workloads import gen_serving_worlds, world_pair_inputs
     t = time.perf_counter()
     pool, shapes, worlds = gen_serving_worlds(lib, seed, worlds_n, steps=4)
     total_pairs = sum(len(w["pairs"]) for w in worlds)
+    total_bodies = sum(len(w["body_shapes"]) for w in worlds)
     log("serving workload: %d worlds, %d bodies, %d contacts in %.1f s"
-        % (worlds_n, sum(len(w["body_shapes"]) for w in worlds), total_pairs, time.perf_counter() - t))
+        % (worlds_n, total_bodies, total_pairs, time.perf_counter() - t))
     rng = np.random.default_rng(seed)
-    out = {"worlds": worlds_n, "contacts": total_pairs,
-           "workload": "worlds of 10-2000 bodies (log-uniform): boxes, spheres, 8-32-point hulls in resting "
-                       "stacks on a static ground, one contact per body; per-step position jitter 1 mm"}
+    out = {"worlds": worlds_n, "bodies": total_bodies, "contacts": total_pairs,
+           "workload": "worlds of 10-2000 bodies (log-uniform): boxes, spheres, 8-32-point hulls in stacks "
+                       "(1-6 high) and piles (2x2 or 3x3 columns of 1-3) on a static ground; contacts = fat-AABB "
+                       "overlaps (0.2 m), %.2f per body; per-step position jitter 1 mm; caches and manifolds "
+                       "resident, body transforms sent per step" % (total_pairs / total_bodies)}
     with bounce_amd.Context(0) as ctx:
         ctx.upload_hulls(pool)
         ctx.upload_shapes(shapes)
         with bounce_amd.Server(ctx, 1 << 16, 1 << 16, total_pairs + 4096) as srv:
-            ids = np.array([srv.create_world(w["body_shapes"], w["pairs"]) for w in worlds], np.uint32)
+            ids = np.array([srv.admit_world(w["body_shapes"], w["pairs"], 1000.0) for w in worlds], np.uint32)
             # Cold step (empty caches) of every world, then the flood capacity on warm steps.
             srv.replay(ids, np.zeros(worlds_n), [w["xfs"][0] for w in worlds])
             order = np.concatenate([ids, ids])
             steps = np.concatenate([np.ones(worlds_n, int), np.full(worlds_n, 2)])
-            lat = srv.replay(order, np.zeros(len(order)), [worlds[i]["xfs"][s] for i, s in zip(order, steps)])
-            capacity = len(order) / (lat.max() / 1000.0)
+            flood = [worlds[i]["xfs"][s] for i, s in zip(order, steps)]
+            for lanes in (1, 3):
+                srv.configure(batch_pairs=1 << 16, max_inflight=lanes)
+                lat = srv.replay(order, np.zeros(len(order)), flood)
+                capacity = len(order) / (lat.max() / 1000.0)
+                out["capacity_steps_per_s_%d_lane%s" % (lanes, "s" if lanes > 1 else "")] = capacity
+                log("serving capacity, %d lane(s): %.0f steps/s" % (lanes, capacity))
             out["capacity_steps_per_s"] = capacity
             out["capacity_contacts_per_s"] = capacity * total_pairs / worlds_n
-            log("serving capacity: %.0f steps/s" % capacity)
             for name, r in (("at_rate", rate), ("at_half_capacity", 0.5 * capacity)):
                 n = int(r * duration)
                 times = np.cumsum(rng.exponential(1.0 / r, n))
@@ -536,6 +546,27 @@ def serving_run(lib, worlds_n, rate, duration, seed=4096):
                 }
                 log("serving %s: %.0f steps/s, p50 %.3f ms, p99 %.3f ms, %d batches"
                     % (name, r, out[name]["p50_ms"], out[name]["p99_ms"], batches))
+        # Plain drop-in: one synchronous b3amd_collide per world step on host arrays, caches
+        # carried per world, for 256 sampled worlds (warm steps 1-3).
+        pick = rng.choice(worlds_n, 256, replace=False)
+        caches = {int(k): empty_caches(len(worlds[k]["pairs"])) for k in pick}
+        for k in pick:
+            p, x = world_pair_inputs(worlds[k], 0)
+            ctx.collide(p, x, caches=caches[int(k)])
+        plain = []
+        for step in (1, 2, 3):
+            for k in pick:
+                p, x = world_pair_inputs(worlds[k], step)
+                t0 = time.perf_counter()
+                ctx.collide(p, x, caches=caches[int(k)])
+                plain.append(time.perf_counter() - t0)
+        plain = np.array(plain) * 1000.0
+        out["plain_drop_in"] = {"p50_ms": float(np.percentile(plain, 50)), "p99_ms": float(np.percentile(plain, 99)),
+                                "steps_per_s": 1000.0 / float(plain.mean()),
+                                "what": "one b3amd_collide per world step (host arrays in and out, synchronous), "
+                                        "256 sampled worlds x 3 warm steps, one caller thread"}
+        log("serving plain drop-in: %.0f steps/s, p50 %.3f ms, p99 %.3f ms"
+            % (out["plain_drop_in"]["steps_per_s"], out["plain_drop_in"]["p50_ms"], out["plain_drop_in"]["p99_ms"]))
     out["latency"] = "narrowphase only: request submitted (scheduled arrival) -> manifolds in HBM"
     return out
 

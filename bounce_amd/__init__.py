@@ -203,6 +203,18 @@ class Server:
         self.pair_counts[w.value] = len(bp)
         return w.value
 
+    def admit_world(self, body_shapes, body_pairs, timeout_ms):
+        """create_world that waits up to timeout_ms for slots (FIFO among admissions)."""
+        bs = np.ascontiguousarray(body_shapes, np.uint32)
+        bp = np.ascontiguousarray(body_pairs, np.uint32).reshape(-1, 2)
+        w = ctypes.c_uint32()
+        f = self.lib.b3amd_world_admit
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p,
+                      ctypes.c_double, ctypes.c_void_p]
+        _check(f(self.handle, len(bs), ptr(bs), len(bp), ptr(bp), float(timeout_ms), ctypes.byref(w)))
+        self.pair_counts[w.value] = len(bp)
+        return w.value
+
     def destroy_world(self, world):
         _check(self.lib.b3amd_world_destroy(self.handle, world))
         self.pair_counts.pop(world, None)
@@ -223,6 +235,11 @@ class Server:
     def read_manifolds(self, world):
         out = np.zeros(self.pair_counts[world], MANIFOLD_DTYPE)
         _check(self.lib.b3amd_world_read_manifolds(self.handle, world, ptr(out)))
+        return out
+
+    def read_caches(self, world):
+        out = empty_caches(self.pair_counts[world])
+        _check(self.lib.b3amd_world_read_caches(self.handle, world, ptr(out)))
         return out
 
     def replay(self, worlds, times_s, body_xfs):

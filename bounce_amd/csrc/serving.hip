@@ -8,8 +8,11 @@
 // up to max_batch_pairs contacts; a gather kernel expands each request's body transforms
 // into per-contact inputs, the batch kernels (cache, convex, SAT, contact) run with each
 // contact's cache and manifold slot in the world's slab range, and the request completes
-// when the batch's event has fired.  Two batches are in flight: the next is packed and
-// copied while the previous computes.
+// when the batch's event has fired.  Up to three batches are in flight, each on its own
+// lane (streams, device inputs and scratch), so they overlap on the device while the next
+// is packed; a batch holding a world's step first waits for the batch holding that
+// world's previous step (an event wait on the device), which keeps each world's cache
+// chain in order.
 //
 // Results do not depend on the batching: every contact is computed from its own inputs
 // and cache slot only, so a world's manifolds are bit-identical alone or in any mix.
@@ -115,11 +118,13 @@ using Clock = std::chrono::steady_clock;
 
 struct World {
     bool live = false;
+    bool closing = false;  // b3amd_world_destroy is waiting: no further submissions
     uint32_t bodyCount = 0, pairCount = 0;
     uint32_t* dBodyShapes = nullptr;
     uint32_t* dBodyPairs = nullptr;
     uint32_t slotBase = 0;
-    uint64_t lastTicket = 0;  // most recent request (0: none yet)
+    uint32_t active = 0;    // requests queued or in flight
+    uint64_t lastBatch = 0; // sequence number of the in-flight batch holding its latest request (0: none)
 };
 
 struct Request {
@@ -133,19 +138,33 @@ struct Request {
     bool done = false;
 };
 
-// Pinned host staging of one batch.
-struct Staging {
-    float* bodyXfs = nullptr;
-    float* bodySweeps = nullptr;
-    ReqDesc* descs = nullptr;
+// One pipeline lane.  A batch in flight owns a lane: its two streams (default priority),
+// its device inputs and scratch and its pinned host staging, so batches on different lanes
+// run concurrently on the device.
+struct Lane {
+    BatchStreams streams{nullptr, nullptr, nullptr, nullptr};
+    float* dBodyXfs = nullptr;
+    float* dBodySweeps = nullptr;
+    ReqDesc* dDescs = nullptr;
+    uint32_t* dPairs = nullptr;
+    float* dXfs = nullptr;
+    float* dSweeps = nullptr;
+    uint32_t* dSlots = nullptr;
+    BatchScratch scratch{nullptr, nullptr, nullptr};
+    float* hBodyXfs = nullptr;
+    float* hBodySweeps = nullptr;
+    ReqDesc* hDescs = nullptr;
+    bool busy = false;
 };
 
 struct Batch {
     std::vector<Request*> requests;
-    std::vector<uint32_t> bodyCounts;            // per request
+    std::vector<uint32_t> bodyCounts;  // per request
+    std::vector<hipEvent_t> waits;     // done events of in-flight batches holding an earlier step of one of its worlds
     uint32_t pairs = 0, bodies = 0;
+    uint64_t seq = 0;
     hipEvent_t start = nullptr, done = nullptr;  // around the batch's copies and kernels
-    int staging = 0;
+    int lane = 0;
 };
 
 constexpr int kMaxInflight = 3;
@@ -156,33 +175,26 @@ struct b3amd_server {
     b3amd_context* ctx = nullptr;
     uint32_t maxPairs = 0, maxBodies = 0, maxRequests = 0, slabSlots = 0;
 
-    // Device state.
-    ConvexCache* dCaches = nullptr;     // [slabSlots]
-    Manifold* dManifolds = nullptr;     // [slabSlots]
-    float* dBodyXfs = nullptr;
-    float* dBodySweeps = nullptr;
-    ReqDesc* dDescs = nullptr;
-    uint32_t* dPairs = nullptr;
-    float* dXfs = nullptr;
-    float* dSweeps = nullptr;
-    uint32_t* dSlots = nullptr;
-    BatchScratch scratch{nullptr, nullptr, nullptr};
-    Staging staging[kMaxInflight];
-    bool stagingBusy[kMaxInflight] = {false, false, false};
-    uint32_t batchPairs = 0;   // current cap on contacts per batch (<= maxPairs)
+    // Device state shared by all lanes: the worlds' cache and manifold slots.
+    ConvexCache* dCaches = nullptr;  // [slabSlots]
+    Manifold* dManifolds = nullptr;  // [slabSlots]
+    Lane lanes[kMaxInflight];
+    uint32_t batchPairs = 0;  // current cap on contacts per batch (<= maxPairs)
     uint32_t maxInflight = 2;
 
     // Worlds and their slot ranges (first fit over free ranges).
     std::vector<World> worlds;
     std::vector<std::pair<uint32_t, uint32_t>> freeRanges;  // (offset, length), sorted
+    std::deque<uint64_t> admitQueue;                         // worlds waiting for slots, in arrival order
+    uint64_t nextAdmit = 1;
 
     // Requests.
     std::mutex mu;
-    std::condition_variable cvWork, cvDone;
+    std::condition_variable cvWork, cvDone, cvAdmit;
     std::deque<Request*> pending;
     std::unordered_map<uint64_t, Request*> requests;
     std::deque<Batch> inflight;
-    uint64_t nextTicket = 1;
+    uint64_t nextTicket = 1, nextBatch = 1;
     bool stop = false;
     std::thread worker;
     int workerError = B3AMD_OK;
@@ -207,6 +219,12 @@ __global__ void init_slots_kernel(ConvexCache* caches, Manifold* manifolds, uint
     caches[base + i] = c;
     manifolds[base + i].pointCount = 0u;
     manifolds[base + i].flags = 0u;
+}
+
+bool fits(const b3amd_server* s, uint32_t n) {
+    for (const auto& r : s->freeRanges)
+        if (r.second >= n) return true;
+    return false;
 }
 
 bool take_range(b3amd_server* s, uint32_t n, uint32_t* base) {
@@ -240,15 +258,17 @@ void give_range(b3amd_server* s, uint32_t base, uint32_t n) {
 }
 
 // Takes the oldest pending requests, at most one per world, up to the batch caps, and
-// writes their descriptors into the batch's staging buffer (lock held: world records are
-// only read here).  Returns the number taken.
+// writes their descriptors into the lane's staging buffer (lock held: world records are
+// only read and updated here).  A world whose previous step is still in flight makes the
+// batch wait for that batch's done event, so each world's steps run in order on its slots.
+// Returns the number taken.
 uint32_t select_batch(b3amd_server* s, Batch& batch) {
-    Staging& st = s->staging[batch.staging];
+    Lane& lane = s->lanes[batch.lane];
     std::vector<uint32_t> seen;  // worlds already taken or passed over in this batch
     uint32_t pairs = 0, bodies = 0, nreq = 0;
     for (auto it = s->pending.begin(); it != s->pending.end();) {
         Request* r = *it;
-        const World& w = s->worlds[r->world];
+        World& w = s->worlds[r->world];
         bool dup = false;
         for (uint32_t v : seen)
             if (v == r->world) dup = true;
@@ -260,13 +280,22 @@ uint32_t select_batch(b3amd_server* s, Batch& batch) {
         if (nreq == s->maxRequests || (nreq > 0 && pairs + w.pairCount > s->batchPairs) ||
             pairs + w.pairCount > s->maxPairs || bodies + w.bodyCount > s->maxBodies)
             break;
-        ReqDesc& d = st.descs[nreq];
+        ReqDesc& d = lane.hDescs[nreq];
         d.pairOffset = pairs;
         d.pairCount = w.pairCount;
         d.bodyOffset = bodies;
         d.slotBase = w.slotBase;
         d.bodyShapes = w.dBodyShapes;
         d.bodyPairs = w.dBodyPairs;
+        if (w.lastBatch != 0) {
+            for (const Batch& b : s->inflight) {
+                if (b.seq != w.lastBatch) continue;
+                bool have = false;
+                for (hipEvent_t e : batch.waits) have = have || e == b.done;
+                if (!have) batch.waits.push_back(b.done);
+            }
+        }
+        w.lastBatch = batch.seq;
         batch.bodyCounts.push_back(w.bodyCount);
         pairs += w.pairCount;
         bodies += w.bodyCount;
@@ -279,21 +308,21 @@ uint32_t select_batch(b3amd_server* s, Batch& batch) {
     return nreq;
 }
 
-// Copies the batch's transforms into its staging buffer and enqueues copies and kernels
-// (lock not held: the requests' inputs and the worlds' device arrays stay unchanged until
-// the batch completes).
+// Copies the batch's transforms into its lane's staging buffer and enqueues the lane's
+// copies and kernels (lock not held: the requests' inputs and the worlds' device arrays
+// stay unchanged until the batch completes, and only the worker touches the lane).
 void launch_batch(b3amd_server* s, Batch& batch) {
     const Clock::time_point t0 = Clock::now();
-    Staging& st = s->staging[batch.staging];
+    Lane& L = s->lanes[batch.lane];
     const uint32_t pairs = batch.pairs, bodies = batch.bodies;
     const uint32_t nreq = (uint32_t)batch.requests.size();
     bool anySweeps = false;
     for (uint32_t q = 0; q < nreq; ++q) {
         const Request* r = batch.requests[q];
-        const size_t off = st.descs[q].bodyOffset, nb = batch.bodyCounts[q];
-        std::memcpy(st.bodyXfs + 12 * off, r->xfs, 48 * nb);
+        const size_t off = L.hDescs[q].bodyOffset, nb = batch.bodyCounts[q];
+        std::memcpy(L.hBodyXfs + 12 * off, r->xfs, 48 * nb);
         if (r->sweeps) {
-            std::memcpy(st.bodySweeps + 8 * off, r->sweeps, 32 * nb);
+            std::memcpy(L.hBodySweeps + 8 * off, r->sweeps, 32 * nb);
             anySweeps = true;
         }
     }
@@ -301,7 +330,7 @@ void launch_batch(b3amd_server* s, Batch& batch) {
         // Requests without sweeps get identity orientations (what a null sweep array means).
         for (uint32_t q = 0; q < nreq; ++q) {
             if (batch.requests[q]->sweeps) continue;
-            float* o = st.bodySweeps + 8 * (size_t)st.descs[q].bodyOffset;
+            float* o = L.hBodySweeps + 8 * (size_t)L.hDescs[q].bodyOffset;
             for (uint32_t b = 0; b < batch.bodyCounts[q]; ++b, o += 8) {
                 o[0] = o[1] = o[2] = 0.0f, o[3] = 1.0f;
                 o[4] = o[5] = o[6] = 0.0f, o[7] = 1.0f;
@@ -309,30 +338,31 @@ void launch_batch(b3amd_server* s, Batch& batch) {
         }
     }
     b3amd_context* ctx = s->ctx;
-    hipStream_t stream = ctx->stream;
-    bool ok = hipEventRecord(batch.start, stream) == hipSuccess &&
-              hipMemcpyAsync(s->dBodyXfs, st.bodyXfs, 48ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess &&
-              (!anySweeps ||
-               hipMemcpyAsync(s->dBodySweeps, st.bodySweeps, 32ull * bodies, hipMemcpyHostToDevice, stream) ==
-                   hipSuccess) &&
-              hipMemcpyAsync(s->dDescs, st.descs, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, stream) == hipSuccess;
+    hipStream_t stream = L.streams.main;
+    bool ok = true;
+    for (hipEvent_t e : batch.waits) ok = ok && hipStreamWaitEvent(stream, e, 0) == hipSuccess;
+    ok = ok && hipEventRecord(batch.start, stream) == hipSuccess &&
+         hipMemcpyAsync(L.dBodyXfs, L.hBodyXfs, 48ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess &&
+         (!anySweeps ||
+          hipMemcpyAsync(L.dBodySweeps, L.hBodySweeps, 32ull * bodies, hipMemcpyHostToDevice, stream) == hipSuccess) &&
+         hipMemcpyAsync(L.dDescs, L.hDescs, sizeof(ReqDesc) * nreq, hipMemcpyHostToDevice, stream) == hipSuccess;
     if (ok) {
         GatherArgs g;
         g.n = pairs;
         g.requests = nreq;
-        g.descs = s->dDescs;
-        g.bodyXfs = s->dBodyXfs;
-        g.bodySweeps = anySweeps ? s->dBodySweeps : nullptr;
-        g.pairs = s->dPairs;
-        g.xfs = s->dXfs;
-        g.sweeps = anySweeps ? s->dSweeps : nullptr;
-        g.slots = s->dSlots;
+        g.descs = L.dDescs;
+        g.bodyXfs = L.dBodyXfs;
+        g.bodySweeps = anySweeps ? L.dBodySweeps : nullptr;
+        g.pairs = L.dPairs;
+        g.xfs = L.dXfs;
+        g.sweeps = anySweeps ? L.dSweeps : nullptr;
+        g.slots = L.dSlots;
         gather_kernel<<<(pairs + 255) / 256, 256, 0, stream>>>(g);
         ok = hipGetLastError() == hipSuccess;
     }
     if (ok)
-        ok = enqueue_batch(ctx, pairs, s->dPairs, s->dXfs, anySweeps ? s->dSweeps : nullptr, s->dCaches,
-                           s->dManifolds, s->scratch, s->dSlots) == B3AMD_OK;
+        ok = enqueue_batch(ctx, pairs, L.dPairs, L.dXfs, anySweeps ? L.dSweeps : nullptr, s->dCaches, s->dManifolds,
+                           L.scratch, L.dSlots, nullptr, &L.streams) == B3AMD_OK;
     if (ok) ok = hipEventRecord(batch.done, stream) == hipSuccess;
     std::lock_guard<std::mutex> lk(s->mu);
     if (!ok && s->workerError == B3AMD_OK) {
@@ -346,24 +376,23 @@ void launch_batch(b3amd_server* s, Batch& batch) {
     s->stats.pack_ms += std::chrono::duration<double, std::milli>(Clock::now() - t0).count();
 }
 
-void complete_batch(b3amd_server* s, Batch& b) {
+// Lock held.
+void retire(b3amd_server* s, std::deque<Batch>::iterator it) {
+    Batch& b = *it;
     const Clock::time_point now = Clock::now();
     for (Request* r : b.requests) {
         r->latencyMs = std::chrono::duration<double, std::milli>(now - r->submit).count();
         r->done = true;
+        World& w = s->worlds[r->world];
+        w.active -= 1;
+        if (w.lastBatch == b.seq) w.lastBatch = 0;
     }
-    b.requests.clear();
-}
-
-void retire(b3amd_server* s) {
-    Batch& b = s->inflight.front();
-    complete_batch(s, b);
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess) s->stats.gpu_ms += ms;
     hipEventDestroy(b.start);
     hipEventDestroy(b.done);
-    s->stagingBusy[b.staging] = false;
-    s->inflight.pop_front();
+    s->lanes[b.lane].busy = false;
+    s->inflight.erase(it);
 }
 
 void worker_main(b3amd_server* s) {
@@ -372,17 +401,24 @@ void worker_main(b3amd_server* s) {
     for (;;) {
         s->cvWork.wait(lk, [&] { return s->stop || !s->pending.empty() || !s->inflight.empty(); });
         if (s->stop && s->pending.empty() && s->inflight.empty()) break;
-        // Retire finished batches without blocking.
+        // Retire every finished batch, in any order (lanes complete independently).
         bool retired = false;
-        while (!s->inflight.empty() && hipEventQuery(s->inflight.front().done) == hipSuccess) {
-            retire(s);
-            retired = true;
+        for (auto it = s->inflight.begin(); it != s->inflight.end();) {
+            if (hipEventQuery(it->done) == hipSuccess) {
+                retire(s, it);
+                it = s->inflight.begin();
+                retired = true;
+            } else {
+                ++it;
+            }
         }
         if (retired) s->cvDone.notify_all();
         if (!s->pending.empty() && s->inflight.size() < s->maxInflight && s->workerError == B3AMD_OK) {
             Batch b;
-            while (s->stagingBusy[b.staging]) ++b.staging;  // a free buffer exists: inflight < max
-            if (hipEventCreate(&b.start) != hipSuccess || hipEventCreate(&b.done) != hipSuccess) {
+            while (s->lanes[b.lane].busy) ++b.lane;  // a free lane exists: inflight < maxInflight <= lanes
+            b.seq = s->nextBatch++;
+            if (hipEventCreateWithFlags(&b.start, 0) != hipSuccess ||
+                hipEventCreateWithFlags(&b.done, 0) != hipSuccess) {
                 s->workerError = B3AMD_ERR_DEVICE;
                 s->workerMessage = "hipEventCreate failed";
                 continue;
@@ -392,29 +428,150 @@ void worker_main(b3amd_server* s) {
                 hipEventDestroy(b.done);
                 continue;
             }
-            s->stagingBusy[b.staging] = true;
+            s->lanes[b.lane].busy = true;
             s->inflight.push_back(b);
-            Batch& queued = s->inflight.back();
+            Batch& queued = s->inflight.back();  // deque: stays valid while others are pushed or erased
             lk.unlock();
             launch_batch(s, queued);
             lk.lock();
             continue;
         }
         if (!s->inflight.empty()) {
-            // Nothing to pack (or every buffer busy): wait for the oldest batch.
-            hipEvent_t e = s->inflight.front().done;
-            lk.unlock();
-            hipEventSynchronize(e);
-            lk.lock();
+            // Nothing to pack (or every lane busy): poll the lanes, waking early for new work.
+            s->cvWork.wait_for(lk, std::chrono::microseconds(20));
             continue;
         }
         if (s->workerError != B3AMD_OK) {
             // Device failure: fail everything queued so waiters return.
-            for (Request* r : s->pending) r->done = true;
+            for (Request* r : s->pending) {
+                r->done = true;
+                s->worlds[r->world].active -= 1;
+            }
             s->pending.clear();
             s->cvDone.notify_all();
         }
     }
+}
+
+int alloc_lane(b3amd_server* s, Lane& L) {
+    b3amd_context* ctx = s->ctx;
+    const uint64_t P = s->maxPairs, B = s->maxBodies;
+    if (hipStreamCreateWithFlags(&L.streams.main, hipStreamNonBlocking) != hipSuccess ||
+        hipStreamCreateWithFlags(&L.streams.side, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&L.streams.fork, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&L.streams.join, hipEventDisableTiming) != hipSuccess)
+        return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
+    int rc = B3AMD_OK;
+    auto A = [&](uint64_t bytes, void** p) {
+        if (rc == B3AMD_OK) rc = ctx->alloc(bytes, p);
+    };
+    A(48 * B, (void**)&L.dBodyXfs);
+    A(32 * B, (void**)&L.dBodySweeps);
+    A(sizeof(ReqDesc) * (uint64_t)s->maxRequests, (void**)&L.dDescs);
+    A(8 * P, (void**)&L.dPairs);
+    A(96 * P, (void**)&L.dXfs);
+    A(64 * P, (void**)&L.dSweeps);
+    A(4 * P, (void**)&L.dSlots);
+    A(4 * scratch_words(P), (void**)&L.scratch.stage);
+    A(sizeof(SatResult) * P, (void**)&L.scratch.sat);
+    A(4 * kCountWords, (void**)&L.scratch.counter);
+    if (rc != B3AMD_OK) return rc;
+    if (hipHostMalloc((void**)&L.hBodyXfs, 48 * B, 0) != hipSuccess ||
+        hipHostMalloc((void**)&L.hBodySweeps, 32 * B, 0) != hipSuccess ||
+        hipHostMalloc((void**)&L.hDescs, sizeof(ReqDesc) * s->maxRequests, 0) != hipSuccess)
+        return fail(B3AMD_ERR_DEVICE, "hipHostMalloc of the staging buffers failed");
+    return B3AMD_OK;
+}
+
+void free_lane(b3amd_server* s, Lane& L) {
+    b3amd_context* ctx = s->ctx;
+    if (L.streams.main) hipStreamSynchronize(L.streams.main);
+    if (L.streams.side) hipStreamSynchronize(L.streams.side);
+    void* bufs[] = {L.dBodyXfs, L.dBodySweeps, L.dDescs, L.dPairs, L.dXfs, L.dSweeps, L.dSlots,
+                    L.scratch.stage, L.scratch.sat, L.scratch.counter};
+    for (void* p : bufs) ctx->release(p);
+    if (L.hBodyXfs) hipHostFree(L.hBodyXfs);
+    if (L.hBodySweeps) hipHostFree(L.hBodySweeps);
+    if (L.hDescs) hipHostFree(L.hDescs);
+    if (L.streams.fork) hipEventDestroy(L.streams.fork);
+    if (L.streams.join) hipEventDestroy(L.streams.join);
+    if (L.streams.main) hipStreamDestroy(L.streams.main);
+    if (L.streams.side) hipStreamDestroy(L.streams.side);
+    L = Lane();
+}
+
+// Admits a world: waits (FIFO among admissions) until the slab has pair_count free slots
+// or until `timeout` passes, then uploads the world's arrays and empty caches.
+int admit_world(b3amd_server* s, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
+                const uint32_t* body_pairs, double timeout_ms, uint32_t* world) {
+    if (!s || !world || !body_shapes || !body_pairs || !body_count || !pair_count)
+        return fail(B3AMD_ERR_ARGUMENT, "null argument or empty world");
+    b3amd_context* ctx = s->ctx;
+    if (pair_count > s->maxPairs || body_count > s->maxBodies)
+        return fail(B3AMD_ERR_ARGUMENT, "world of %u contacts / %u bodies exceeds the batch capacity %u / %u",
+                    pair_count, body_count, s->maxPairs, s->maxBodies);
+    if (pair_count > s->slabSlots)
+        return fail(B3AMD_ERR_BUDGET, "world of %u contacts exceeds the slab of %u slots", pair_count, s->slabSlots);
+    for (uint32_t b = 0; b < body_count; ++b)
+        if (body_shapes[b] >= ctx->shapeCount)
+            return fail(B3AMD_ERR_ARGUMENT, "body %u: shape %u of %u", b, body_shapes[b], ctx->shapeCount);
+    for (uint32_t i = 0; i < pair_count; ++i) {
+        const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
+        if (a >= body_count || b >= body_count)
+            return fail(B3AMD_ERR_ARGUMENT, "contact %u refers to body %u/%u of %u", i, a, b, body_count);
+        if (ctx->shapes[body_shapes[a]].type > ctx->shapes[body_shapes[b]].type)
+            return fail(B3AMD_ERR_ARGUMENT, "contact %u: typeA > typeB (contact_manager.cpp:238-242)", i);
+    }
+    std::unique_lock<std::mutex> lk(s->mu);
+    const uint64_t id = s->nextAdmit++;
+    s->admitQueue.push_back(id);
+    const auto deadline =
+        Clock::now() + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double, std::milli>(timeout_ms));
+    const bool room = s->cvAdmit.wait_until(lk, deadline, [&] {
+        return s->stop || (s->admitQueue.front() == id && fits(s, pair_count));
+    }) && !s->stop;
+    for (auto it = s->admitQueue.begin(); it != s->admitQueue.end(); ++it)
+        if (*it == id) {
+            s->admitQueue.erase(it);
+            break;
+        }
+    s->cvAdmit.notify_all();  // the next waiter may be at the front now
+    uint32_t base = 0;
+    if (!room || !take_range(s, pair_count, &base))
+        return fail(B3AMD_ERR_BUDGET, "no room for %u contact slots in the server's slab", pair_count);
+    World w;
+    int rc = ctx->alloc(4ull * body_count, (void**)&w.dBodyShapes);
+    if (rc == B3AMD_OK) rc = ctx->alloc(8ull * pair_count, (void**)&w.dBodyPairs);
+    if (rc != B3AMD_OK) {
+        std::string msg = b3amd_last_error();
+        ctx->release(w.dBodyShapes);
+        give_range(s, base, pair_count);
+        s->cvAdmit.notify_all();
+        return fail(rc, "%s", msg.c_str());
+    }
+    // Uploads on the context stream, complete before any batch can use the world.
+    HIP_TRY(hipMemcpyAsync(w.dBodyShapes, body_shapes, 4ull * body_count, hipMemcpyHostToDevice, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(w.dBodyPairs, body_pairs, 8ull * pair_count, hipMemcpyHostToDevice, ctx->stream));
+    init_slots_kernel<<<(pair_count + 255) / 256, 256, 0, ctx->stream>>>(s->dCaches, s->dManifolds, base, pair_count);
+    HIP_TRY(hipGetLastError());
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    w.live = true;
+    w.bodyCount = body_count;
+    w.pairCount = pair_count;
+    w.slotBase = base;
+    uint32_t wid = 0;
+    while (wid < s->worlds.size() && s->worlds[wid].live) ++wid;
+    if (wid == s->worlds.size()) s->worlds.push_back(w);
+    else s->worlds[wid] = w;
+    *world = wid;
+    return B3AMD_OK;
+}
+
+// Waits (lock held) until none of the world's requests is queued or in flight.
+int quiesce_world(b3amd_server* s, std::unique_lock<std::mutex>& lk, uint32_t world) {
+    if (world >= s->worlds.size() || !s->worlds[world].live) return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+    s->cvDone.wait(lk, [&] { return s->worlds[world].active == 0; });
+    return B3AMD_OK;
 }
 
 }  // namespace
@@ -433,30 +590,9 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
     s->maxBodies = max_batch_bodies;
     s->maxRequests = max_batch_pairs < 65536u ? max_batch_pairs : 65536u;
     s->slabSlots = slab_slots;
-    const uint64_t P = max_batch_pairs, B = max_batch_bodies;
-    int rc = B3AMD_OK;
-    auto A = [&](uint64_t bytes, void** p) {
-        if (rc == B3AMD_OK) rc = ctx->alloc(bytes, p);
-    };
-    A(sizeof(ConvexCache) * (uint64_t)slab_slots, (void**)&s->dCaches);
-    A(sizeof(Manifold) * (uint64_t)slab_slots, (void**)&s->dManifolds);
-    A(48 * B, (void**)&s->dBodyXfs);
-    A(32 * B, (void**)&s->dBodySweeps);
-    A(sizeof(ReqDesc) * (uint64_t)s->maxRequests, (void**)&s->dDescs);
-    A(8 * P, (void**)&s->dPairs);
-    A(96 * P, (void**)&s->dXfs);
-    A(64 * P, (void**)&s->dSweeps);
-    A(4 * P, (void**)&s->dSlots);
-    A(4 * scratch_words(P), (void**)&s->scratch.stage);
-    A(sizeof(SatResult) * P, (void**)&s->scratch.sat);
-    A(4 * kCountWords, (void**)&s->scratch.counter);
-    for (int i = 0; i < kMaxInflight && rc == B3AMD_OK; ++i) {
-        Staging& st = s->staging[i];
-        if (hipHostMalloc((void**)&st.bodyXfs, 48 * B, 0) != hipSuccess ||
-            hipHostMalloc((void**)&st.bodySweeps, 32 * B, 0) != hipSuccess ||
-            hipHostMalloc((void**)&st.descs, sizeof(ReqDesc) * s->maxRequests, 0) != hipSuccess)
-            rc = fail(B3AMD_ERR_DEVICE, "hipHostMalloc of the staging buffers failed");
-    }
+    int rc = ctx->alloc(sizeof(ConvexCache) * (uint64_t)slab_slots, (void**)&s->dCaches);
+    if (rc == B3AMD_OK) rc = ctx->alloc(sizeof(Manifold) * (uint64_t)slab_slots, (void**)&s->dManifolds);
+    for (int i = 0; i < kMaxInflight && rc == B3AMD_OK; ++i) rc = alloc_lane(s, s->lanes[i]);
     if (rc != B3AMD_OK) {
         std::string msg = b3amd_last_error();
         b3amd_server_destroy(s);
@@ -471,29 +607,25 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
 
 int b3amd_server_destroy(b3amd_server* s) {
     if (!s) return B3AMD_OK;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+    }
+    s->cvAdmit.notify_all();
     if (s->worker.joinable()) {
-        {
-            std::lock_guard<std::mutex> lk(s->mu);
-            s->stop = true;
-        }
         s->cvWork.notify_all();
         s->worker.join();
     }
     b3amd_context* ctx = s->ctx;
+    for (Lane& L : s->lanes) free_lane(s, L);
     hipStreamSynchronize(ctx->stream);
     for (World& w : s->worlds) {
         if (!w.live) continue;
         ctx->release(w.dBodyShapes);
         ctx->release(w.dBodyPairs);
     }
-    void* bufs[] = {s->dCaches, s->dManifolds, s->dBodyXfs, s->dBodySweeps, s->dDescs, s->dPairs, s->dXfs,
-                    s->dSweeps, s->dSlots, s->scratch.stage, s->scratch.sat, s->scratch.counter};
-    for (void* p : bufs) ctx->release(p);
-    for (Staging& st : s->staging) {  // also those of the third pipeline stage
-        if (st.bodyXfs) hipHostFree(st.bodyXfs);
-        if (st.bodySweeps) hipHostFree(st.bodySweeps);
-        if (st.descs) hipHostFree(st.descs);
-    }
+    ctx->release(s->dCaches);
+    ctx->release(s->dManifolds);
     for (auto& kv : s->requests) delete kv.second;
     delete s;
     return B3AMD_OK;
@@ -501,69 +633,29 @@ int b3amd_server_destroy(b3amd_server* s) {
 
 int b3amd_world_create(b3amd_server* s, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
                        const uint32_t* body_pairs, uint32_t* world) {
-    if (!s || !world || !body_shapes || !body_pairs || !body_count || !pair_count)
-        return fail(B3AMD_ERR_ARGUMENT, "null argument or empty world");
-    b3amd_context* ctx = s->ctx;
-    if (pair_count > s->maxPairs || body_count > s->maxBodies)
-        return fail(B3AMD_ERR_ARGUMENT, "world of %u contacts / %u bodies exceeds the batch capacity %u / %u",
-                    pair_count, body_count, s->maxPairs, s->maxBodies);
-    for (uint32_t b = 0; b < body_count; ++b)
-        if (body_shapes[b] >= ctx->shapeCount)
-            return fail(B3AMD_ERR_ARGUMENT, "body %u: shape %u of %u", b, body_shapes[b], ctx->shapeCount);
-    for (uint32_t i = 0; i < pair_count; ++i) {
-        const uint32_t a = body_pairs[2 * i], b = body_pairs[2 * i + 1];
-        if (a >= body_count || b >= body_count)
-            return fail(B3AMD_ERR_ARGUMENT, "contact %u refers to body %u/%u of %u", i, a, b, body_count);
-        if (ctx->shapes[body_shapes[a]].type > ctx->shapes[body_shapes[b]].type)
-            return fail(B3AMD_ERR_ARGUMENT, "contact %u: typeA > typeB (contact_manager.cpp:238-242)", i);
-    }
-    std::lock_guard<std::mutex> lk(s->mu);
-    uint32_t base = 0;
-    if (!take_range(s, pair_count, &base))
-        return fail(B3AMD_ERR_BUDGET, "no room for %u contact slots in the server's slab", pair_count);
-    World w;
-    int rc = ctx->alloc(4ull * body_count, (void**)&w.dBodyShapes);
-    if (rc == B3AMD_OK) rc = ctx->alloc(8ull * pair_count, (void**)&w.dBodyPairs);
-    if (rc != B3AMD_OK) {
-        std::string msg = b3amd_last_error();
-        ctx->release(w.dBodyShapes);
-        give_range(s, base, pair_count);
-        return fail(rc, "%s", msg.c_str());
-    }
-    // Uploads on the context stream: ordered before any batch that uses the world.
-    HIP_TRY(hipMemcpyAsync(w.dBodyShapes, body_shapes, 4ull * body_count, hipMemcpyHostToDevice, ctx->stream));
-    HIP_TRY(hipMemcpyAsync(w.dBodyPairs, body_pairs, 8ull * pair_count, hipMemcpyHostToDevice, ctx->stream));
-    init_slots_kernel<<<(pair_count + 255) / 256, 256, 0, ctx->stream>>>(s->dCaches, s->dManifolds, base, pair_count);
-    HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(ctx->stream));
-    w.live = true;
-    w.bodyCount = body_count;
-    w.pairCount = pair_count;
-    w.slotBase = base;
-    uint32_t id = 0;
-    while (id < s->worlds.size() && s->worlds[id].live) ++id;
-    if (id == s->worlds.size()) s->worlds.push_back(w);
-    else s->worlds[id] = w;
-    *world = id;
-    return B3AMD_OK;
+    return admit_world(s, body_count, body_shapes, pair_count, body_pairs, 0.0, world);
+}
+
+int b3amd_world_admit(b3amd_server* s, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
+                      const uint32_t* body_pairs, double timeout_ms, uint32_t* world) {
+    if (!(timeout_ms >= 0.0)) return fail(B3AMD_ERR_ARGUMENT, "negative or NaN timeout");
+    return admit_world(s, body_count, body_shapes, pair_count, body_pairs, timeout_ms, world);
 }
 
 int b3amd_world_destroy(b3amd_server* s, uint32_t world) {
     if (!s) return fail(B3AMD_ERR_ARGUMENT, "null server");
     std::unique_lock<std::mutex> lk(s->mu);
-    if (world >= s->worlds.size() || !s->worlds[world].live) return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
-    // Wait until no request of the world is queued or in flight.
-    const uint64_t last = s->worlds[world].lastTicket;
-    if (last) {
-        auto it = s->requests.find(last);
-        if (it != s->requests.end()) s->cvDone.wait(lk, [&] { return it->second->done; });
-    }
+    if (world >= s->worlds.size() || !s->worlds[world].live || s->worlds[world].closing)
+        return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+    s->worlds[world].closing = true;
+    int rc = quiesce_world(s, lk, world);
+    if (rc) return rc;
     World& w = s->worlds[world];
-    HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     s->ctx->release(w.dBodyShapes);
     s->ctx->release(w.dBodyPairs);
     give_range(s, w.slotBase, w.pairCount);
     w = World();
+    s->cvAdmit.notify_all();
     return B3AMD_OK;
 }
 
@@ -579,7 +671,7 @@ static int submit(b3amd_server* s, uint32_t world, const float* xfs, const float
             delete r;
             return fail(s->workerError, "server failed: %s", s->workerMessage.c_str());
         }
-        if (world >= s->worlds.size() || !s->worlds[world].live) {
+        if (world >= s->worlds.size() || !s->worlds[world].live || s->worlds[world].closing) {
             delete r;
             return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
         }
@@ -598,7 +690,7 @@ static int submit(b3amd_server* s, uint32_t world, const float* xfs, const float
         r->ticket = s->nextTicket++;
         r->world = world;
         r->submit = at ? *at : Clock::now();
-        w.lastTicket = r->ticket;
+        w.active += 1;
         s->requests[r->ticket] = r;
         s->pending.push_back(r);
         *ticket = r->ticket;
@@ -622,26 +714,39 @@ int b3amd_server_wait(b3amd_server* s, uint64_t ticket, double* latency_ms) {
     Request* r = it->second;
     s->cvDone.wait(lk, [&] { return r->done; });
     if (latency_ms) *latency_ms = r->latencyMs;
-    s->requests.erase(it);
+    s->requests.erase(ticket);
     delete r;
     if (s->workerError != B3AMD_OK) return fail(s->workerError, "server failed: %s", s->workerMessage.c_str());
     return B3AMD_OK;
 }
 
-int b3amd_world_read_manifolds(b3amd_server* s, uint32_t world, b3amd_manifold* out) {
+// Copies `count` records of the world's slots after every step queued so far has run.
+static int read_slots(b3amd_server* s, uint32_t world, void* out, bool caches) {
     if (!s || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     uint32_t base, count;
     {
-        std::lock_guard<std::mutex> lk(s->mu);
-        if (world >= s->worlds.size() || !s->worlds[world].live) return fail(B3AMD_ERR_ARGUMENT, "world %u", world);
+        std::unique_lock<std::mutex> lk(s->mu);
+        int rc = quiesce_world(s, lk, world);
+        if (rc) return rc;
         base = s->worlds[world].slotBase;
         count = s->worlds[world].pairCount;
     }
-    // On the context stream: ordered after every batch enqueued so far.
-    HIP_TRY(hipMemcpyAsync(out, s->dManifolds + base, sizeof(Manifold) * (uint64_t)count, hipMemcpyDeviceToHost,
-                           s->ctx->stream));
+    if (caches)
+        HIP_TRY(hipMemcpyAsync(out, s->dCaches + base, sizeof(ConvexCache) * (uint64_t)count, hipMemcpyDeviceToHost,
+                               s->ctx->stream));
+    else
+        HIP_TRY(hipMemcpyAsync(out, s->dManifolds + base, sizeof(Manifold) * (uint64_t)count, hipMemcpyDeviceToHost,
+                               s->ctx->stream));
     HIP_TRY(hipStreamSynchronize(s->ctx->stream));
     return B3AMD_OK;
+}
+
+int b3amd_world_read_manifolds(b3amd_server* s, uint32_t world, b3amd_manifold* out) {
+    return read_slots(s, world, out, false);
+}
+
+int b3amd_world_read_caches(b3amd_server* s, uint32_t world, b3amd_convex_cache* out) {
+    return read_slots(s, world, out, true);
 }
 
 int b3amd_server_replay(b3amd_server* s, uint32_t count, const uint32_t* worlds, const double* times_s,
@@ -657,7 +762,10 @@ int b3amd_server_replay(b3amd_server* s, uint32_t count, const uint32_t* worlds,
         // Latency counts from the scheduled arrival, so submission lag is included.
         int rc = submit(s, worlds[i], reinterpret_cast<const float*>(body_xfs[i]), nullptr, false, &due[i],
                         &tickets[i]);
-        if (rc) return rc;
+        if (rc) {
+            for (uint32_t k = 0; k < i; ++k) b3amd_server_wait(s, tickets[k], &latency_ms[k]);
+            return rc;
+        }
     }
     int rc = B3AMD_OK;
     for (uint32_t i = 0; i < count; ++i) {

@@ -78,20 +78,27 @@ def _rotations(rng, n, tilt_sigma=0.05):
     return Rx @ Rz @ Ry
 
 
+AABB_EXTENSION = 0.2  # b3_aabbExtension (settings.h:54): fat AABBs of the broadphase
+
+
 def gen_serving_worlds(lib, seed, n_worlds, bodies_min=10, bodies_max=2000, steps=4, hull_shapes=2048,
-                       box_shapes=64, sphere_shapes=64):
-    """Worlds of boxes, spheres and hulls (8-32 points) stacked in piles on a static ground.
+                       box_shapes=64, sphere_shapes=64, pile_fraction=0.5):
+    """Worlds of boxes, spheres and hulls (8-32 points) in stacks and piles on a static ground.
 
     Body counts are log-uniform in [bodies_min, bodies_max]; a third of the bodies are boxes
     (half-extents U[0.2, 0.5] m), a third spheres (radius U[0.2, 0.5] m) and a third hulls
-    (8-32 points on ellipsoids with semi-axes U[0.2, 0.5] m).  Bodies stand in stacks of
-    1-6 on a 1.4 m grid, each resting on the one below (or the ground) with a vertical gap
-    ~ N(-5 mm, 5 mm) between their supports, uniform yaw and tilts of N(0, 0.01 rad).  Each body has one
-    contact, with its support; body 0 is the ground (a 100 x 1 x 100 m box, top at y = 0).
-    Each step jitters body positions by N(0, 1 mm).
+    (8-32 points on ellipsoids with semi-axes U[0.2, 0.5] m), with uniform yaw and tilts of
+    N(0, 0.01 rad).  Bodies are grouped in clusters on a 4.5 m grid: a stack (one column of
+    1-6 bodies) or, with probability pile_fraction, a pile (2 x 2 or 3 x 3 columns of 1-3
+    bodies, columns spaced by the pile's widest body + 0.25 m).  In a column each body rests
+    on the one below (or the ground) with a vertical gap ~ N(-5 mm, 5 mm) between their
+    supports.  The contact list is what the broadphase would hand b3ContactManager: every
+    pair of bodies whose fat AABBs (extended by 0.2 m) overlap, the static ground included
+    (body 0: a 100 x 1 x 100 m box, top at y = 0), ordered by (lower body, higher body) with
+    typeA <= typeB.  Each step jitters body positions by N(0, 1 mm).
 
     Returns (pool, shapes, worlds); a world is a dict with body_shapes [B] u32,
-    pairs [P, 2] u32 body indices (typeA <= typeB) and xfs [steps, B, 12] f32.
+    pairs [P, 2] u32 body indices and xfs [steps, B, 12] f32.
     """
     from .shapes import box_hull
     from .types import SHAPE_DTYPE, SPHERE, HULL
@@ -108,16 +115,14 @@ def gen_serving_worlds(lib, seed, n_worlds, bodies_min=10, bodies_max=2000, step
     shapes["hull"][sph] = 0
     shapes["radius"][sph] = rng.uniform(0.2, 0.5, sphere_shapes)
     shapes["hull"][1 + box_shapes + sphere_shapes:] = 1 + box_shapes + np.arange(hull_shapes)
-    # Padded vertex arrays per shape for support heights (spheres use their radius).
+    # Padded vertex arrays per shape for support extents (spheres use their radius).
     vmax = 32
     verts = np.zeros((n_shapes, vmax, 3))
-    nverts = np.zeros(n_shapes, np.int64)
     for s in range(n_shapes):
         if shapes["type"][s] == HULL:
             v = pool.hull(int(shapes["hull"][s]))[0]
             verts[s, : len(v)] = v
             verts[s, len(v):] = v[0]
-            nverts[s] = len(v)
     worlds = []
     for _ in range(n_worlds):
         nb = int(np.exp(rng.uniform(np.log(bodies_min), np.log(bodies_max))))
@@ -126,34 +131,62 @@ def gen_serving_worlds(lib, seed, n_worlds, bodies_min=10, bodies_max=2000, step
                                np.where(kind == 1, sph.start + rng.integers(0, sphere_shapes, nb),
                                         sph.stop + rng.integers(0, hull_shapes, nb))).astype(np.uint32)
         R = _rotations(rng, nb, 0.01)
-        # Support extents along y of each rotated body about its origin.
+        # Support extents of each rotated body about its origin, per axis.
         is_sphere = shapes["type"][body_shapes] == SPHERE
-        vy = np.einsum("bj,bvj->bv", R[:, 1, :], verts[body_shapes])
-        lo = np.where(is_sphere, -shapes["radius"][body_shapes], vy.min(axis=1))
-        hi = np.where(is_sphere, shapes["radius"][body_shapes], vy.max(axis=1))
-        # Stacks on a grid.
-        heights = []
-        left = nb
-        while left > 0:
-            h = min(left, int(rng.integers(1, 7)))
-            heights.append(h)
-            left -= h
-        side = int(np.ceil(np.sqrt(len(heights))))
+        rv = np.matmul(verts[body_shapes], R.transpose(0, 2, 1))
+        rad = shapes["radius"][body_shapes].astype(np.float64)[:, None]
+        lo = np.where(is_sphere[:, None], -rad, rv.min(axis=1))
+        hi = np.where(is_sphere[:, None], rad, rv.max(axis=1))
+        # Clusters: (columns per side k, bodies per column), bodies taken in order.
+        ks = np.where(rng.uniform(size=nb) < pile_fraction, rng.integers(2, 4, nb), 1)
+        ncols = ks * ks
+        col_c = np.repeat(np.arange(nb), ncols)
+        col_q = np.arange(len(col_c)) - np.repeat(np.cumsum(ncols) - ncols, ncols)
+        col_h = np.where(ks[col_c] > 1, rng.integers(1, 4, len(col_c)), rng.integers(1, 7, len(col_c)))
+        cum = np.cumsum(col_h)
+        last = int(np.searchsorted(cum, nb))                     # the column holding body nb - 1
+        col_h, col_q, col_c = col_h[: last + 1].copy(), col_q[: last + 1], col_c[: last + 1]
+        col_h[last] -= cum[last] - nb
+        ks = ks[: col_c[-1] + 1]
+        col = np.repeat(np.arange(len(col_h)), col_h)           # body -> column
+        group = col_c[col]                                       # body -> cluster
+        side = int(np.ceil(np.sqrt(len(ks))))
+        wbody = np.max(hi[:, [0, 2]] - lo[:, [0, 2]], axis=1)
+        cstart = np.searchsorted(group, np.arange(len(ks)))
+        sp = np.maximum.reduceat(wbody, cstart) + 0.05          # the cluster's widest body + 5 cm
+        c, q = col_c[col], col_q[col]
+        k = ks[c]
         pos = np.zeros((nb, 3))
-        pairs = []
-        b = 0
-        for c, h in enumerate(heights):
-            x, z = 1.4 * (c % side - side / 2.0), 1.4 * (c // side - side / 2.0)
-            top, below = 0.0, -1  # ground top, ground body
-            for _k in range(h):
-                g = rng.normal(-0.005, 0.005)
-                pos[b] = (x, top + g - lo[b], z)
-                top = pos[b, 1] + hi[b]
-                pairs.append((below, b))
-                below = b
-                b += 1
+        pos[:, 0] = 4.5 * (c % side - side / 2.0) + sp[c] * (q % k - (k - 1) / 2.0)
+        pos[:, 2] = 4.5 * (c // side - side / 2.0) + sp[c] * (q // k - (k - 1) / 2.0)
+        # Each body rests on the one below with a gap g: y_b = top_below + g_b - lo_b,
+        # top_b = y_b + hi_b, so y is a running sum within the column.
+        g = rng.normal(-0.005, 0.005, nb)
+        step_h = g - lo[:, 1] + hi[:, 1]
+        run = np.cumsum(step_h)
+        first = np.searchsorted(col, np.arange(len(col_h)))
+        base = run[first] - step_h[first]                        # running sum before each column
+        pos[:, 1] = run - base[col] - hi[:, 1]
+        # Fat AABBs and their overlaps within each cluster (clusters 4.5 m apart never meet).
+        flo, fhi = pos + lo - AABB_EXTENSION, pos + hi + AABB_EXTENSION
+        order = np.argsort(group, kind="stable")
+        starts = np.searchsorted(group[order], np.arange(len(ks) + 1))
+        G = int(np.max(np.diff(starts)))
+        idx = np.full((len(ks), G), -1, np.int64)
+        for g in range(G):
+            sel = starts[:-1] + g < starts[1:]
+            idx[sel, g] = order[starts[:-1][sel] + g]
+        ii, jj = np.triu_indices(G, 1)
+        A, B = idx[:, ii].ravel(), idx[:, jj].ravel()
+        ok = (A >= 0) & (B >= 0)
+        A, B = A[ok], B[ok]
+        hit = np.all((flo[A] <= fhi[B]) & (flo[B] <= fhi[A]), axis=1)
+        A, B = np.minimum(A[hit], B[hit]), np.maximum(A[hit], B[hit])
+        ground = np.nonzero(flo[:, 1] <= AABB_EXTENSION)[0]  # the ground's fat box reaches y = 0.2
+        pairs = np.concatenate([np.stack([np.zeros(len(ground), np.int64), ground + 1], 1),
+                                np.stack([A + 1, B + 1], 1)])
+        pairs = pairs[np.lexsort((pairs[:, 1], pairs[:, 0]))]
         bodies = nb + 1  # body 0 = ground
-        pairs = np.asarray(pairs, np.int64) + 1
         all_shapes = np.concatenate([[0], body_shapes]).astype(np.uint32)
         t = shapes["type"][all_shapes]
         swap = t[pairs[:, 0]] > t[pairs[:, 1]]

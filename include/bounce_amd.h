@@ -201,11 +201,17 @@ int b3amd_graph_destroy(b3amd_graph* graph);
  * transforms [bodyCount] (b3Body::m_xf) and optionally the sweeps [bodyCount][2]
  * (orientation0, orientation) read by the cached face rebuild.  A worker thread packs the
  * oldest pending requests, at most one per world, into batches of at most
- * max_batch_pairs contacts and max_batch_bodies bodies, two batches in flight.
+ * max_batch_pairs contacts and max_batch_bodies bodies.  Up to three batches are in flight
+ * at once, each on its own pipeline lane (two streams, device inputs and scratch), so
+ * batches overlap on the device; a batch holding a world's step waits on the device for
+ * the batch holding that world's previous step, so each world's steps run in order.
  * Admission: a world needs pair_count slots of the server's slab of slab_slots contacts
- * (created with the server and counted in the context's HBM budget); when they do not
- * fit, b3amd_world_create returns B3AMD_ERR_BUDGET.  While a server exists it owns the
- * context's stream: call no other entry point on that context concurrently. */
+ * (created with the server and counted in the context's HBM budget).  b3amd_world_create
+ * returns B3AMD_ERR_BUDGET at once when they do not fit; b3amd_world_admit waits, in
+ * arrival order among admissions, until destroyed worlds free enough slots.  All entry
+ * points are thread-safe with respect to each other; while a server exists it uses the
+ * context's stream for world uploads and reads: call no other entry point on that context
+ * concurrently. */
 typedef struct b3amd_server b3amd_server;
 typedef struct b3amd_server_stats {
     uint64_t batches;         /* device batches launched */
@@ -213,7 +219,8 @@ typedef struct b3amd_server_stats {
     uint64_t pairs;           /* contacts computed */
     uint32_t max_batch_pairs; /* largest batch */
     uint32_t reserved;
-    double gpu_ms;            /* device time of all batches (copies, gather, kernels) */
+    double gpu_ms;            /* device time summed over batches (copies, gather, kernels; batches
+                               * on different lanes overlap, so this can exceed wall time) */
     double pack_ms;           /* host time spent packing batches */
 } b3amd_server_stats;
 int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t max_batch_bodies,
@@ -221,7 +228,12 @@ int b3amd_server_create(b3amd_context* ctx, uint32_t max_batch_pairs, uint32_t m
 int b3amd_server_destroy(b3amd_server* server);
 int b3amd_world_create(b3amd_server* server, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
                        const uint32_t* body_pairs, uint32_t* world);
-/* Waits for the world's queued steps, then frees its slots. */
+/* b3amd_world_create that waits up to timeout_ms for slots (FIFO among waiting admissions);
+ * B3AMD_ERR_BUDGET when the time passes first. */
+int b3amd_world_admit(b3amd_server* server, uint32_t body_count, const uint32_t* body_shapes, uint32_t pair_count,
+                      const uint32_t* body_pairs, double timeout_ms, uint32_t* world);
+/* Refuses further steps of the world, waits until its queued and in-flight steps are done,
+ * then frees its slots and wakes waiting admissions. */
 int b3amd_world_destroy(b3amd_server* server, uint32_t world);
 /* Queues one step (inputs are copied before return); ticket identifies it. */
 int b3amd_world_submit(b3amd_server* server, uint32_t world, const b3amd_transform* body_xfs,
@@ -229,9 +241,11 @@ int b3amd_world_submit(b3amd_server* server, uint32_t world, const b3amd_transfo
 /* Blocks until the step's manifolds are in the world's device slots; latency_ms (may be
  * NULL) = time from submission to completion.  Each ticket is collected once. */
 int b3amd_server_wait(b3amd_server* server, uint64_t ticket, double* latency_ms);
-/* The world's manifolds, in contact-list order, after every step queued so far (call it
- * after waiting for the world's last submitted step). */
+/* The world's manifolds / feature caches (b3ConvexCache per contact, the state each
+ * b3ConvexContact::m_cache holds), in contact-list order, after every step submitted so far
+ * has run (waits for them). */
 int b3amd_world_read_manifolds(b3amd_server* server, uint32_t world, b3amd_manifold* out);
+int b3amd_world_read_caches(b3amd_server* server, uint32_t world, b3amd_convex_cache* out);
 /* Load generator: submits request i (world worlds[i], transforms body_xfs[i], which must
  * stay valid until the call returns) at times_s[i] seconds after the call starts, waits
  * for all, and writes each request's latency from its scheduled arrival. */
@@ -240,7 +254,7 @@ int b3amd_server_replay(b3amd_server* server, uint32_t count, const uint32_t* wo
 int b3amd_server_get_stats(b3amd_server* server, b3amd_server_stats* out);
 /* Batch policy: at most batch_pairs contacts per batch (0 or more than the server's
  * max_batch_pairs: that maximum; a world larger than the cap still goes alone) and at most
- * max_inflight (1..3) batches queued on the device while the next one is packed.  Defaults:
+ * max_inflight (1..3) batches in flight on the device, one per lane.  Defaults:
  * max_batch_pairs, 2. */
 int b3amd_server_configure(b3amd_server* server, uint32_t batch_pairs, uint32_t max_inflight);
 
