@@ -12,6 +12,7 @@
 #include <map>
 #include <string>
 #include <unordered_map>
+#include <mutex>
 #include <vector>
 
 #include "bounce_amd.h"
@@ -61,6 +62,43 @@ int timed(b3amd_context* ctx, hipStream_t s, const char* name, F launch) {
     return B3AMD_OK;
 }
 
+}  // namespace
+
+// Scratch of b3amd_collide_device_noalloc: a file-scope device array, one copy per GPU,
+// holding what ensure_scratch allocates for a batch of up to kNoAllocMaxPairs pairs: the
+// stage / SAT-list / rebuild-list words, the SAT results and the counter words (zeroed by a
+// kernel at the start of every batch).  Host code never names the array: its address on
+// each device comes from hipGetSymbolAddress, taken once per device when the first context
+// on that device is created.
+constexpr uint32_t kNoAllocMaxPairs = 262144u;
+constexpr uint64_t kNoAllocStageWords =
+    2ull * kNoAllocMaxPairs +
+    (uint64_t)kRebuildLists * (((kNoAllocMaxPairs + kReadPairsPerBlock - 1u) / kReadPairsPerBlock + kRebuildLists - 1u) /
+                               kRebuildLists * kReadPairsPerBlock);
+static_assert(sizeof(SatResult) % 16 == 0 && kNoAllocStageWords % 64 == 0, "sections stay 256-byte aligned");
+constexpr uint64_t kNoAllocSatWords = sizeof(SatResult) / 4 * (uint64_t)kNoAllocMaxPairs;
+constexpr uint64_t kNoAllocWords = kNoAllocStageWords + kNoAllocSatWords + kCountWords;
+static __device__ __align__(256) uint32_t scratch[kNoAllocWords];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratchMu;
+void* g_scratchPtr[kMaxDevices] = {};
+
+// The device address of `scratch` on `device` (which must be current).
+int scratch_address(int device, void** out) {
+    if (device < 0 || device >= kMaxDevices) return fail(B3AMD_ERR_ARGUMENT, "device %d beyond %d", device, kMaxDevices);
+    std::lock_guard<std::mutex> lk(g_scratchMu);
+    if (!g_scratchPtr[device]) {
+        void* p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p)
+            return fail(B3AMD_ERR_DEVICE, "hipGetSymbolAddress(scratch) on device %d: %s", device, hipGetErrorString(e));
+        g_scratchPtr[device] = p;
+    }
+    *out = g_scratchPtr[device];
+    return B3AMD_OK;
+}
 }  // namespace
 
 int b3amd::ensure_scratch(b3amd_context* ctx, uint32_t n) {
@@ -376,6 +414,11 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
         b3amd_destroy(ctx);
         return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
     }
+    if (scratch_address(device, &ctx->noAllocScratch) != B3AMD_OK) {
+        std::string msg = b3amd_last_error();
+        b3amd_destroy(ctx);
+        return fail(B3AMD_ERR_DEVICE, "%s", msg.c_str());
+    }
     if (ctx->alloc(sizeof(Counters) * kCounterSlots, (void**)&ctx->dCounters) != B3AMD_OK ||
         hipMemset(ctx->dCounters, 0, sizeof(Counters) * kCounterSlots) != hipSuccess) {
         std::string msg = b3amd_last_error();
@@ -514,6 +557,30 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
     if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
     return run_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs), reinterpret_cast<const float*>(d_sweeps),
                      reinterpret_cast<ConvexCache*>(d_caches), reinterpret_cast<Manifold*>(d_out));
+}
+
+int b3amd_collide_device_noalloc(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs,
+                                 const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
+                                 b3amd_convex_cache* d_caches, b3amd_manifold* d_out) {
+    if (!ctx || (n && (!d_pairs || !d_xfs || !d_out))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (d_sweeps && !d_caches) return fail(B3AMD_ERR_ARGUMENT, "sweeps without caches");
+    if (n > kNoAllocMaxPairs)
+        return fail(B3AMD_ERR_ARGUMENT, "%u pairs exceed the no-allocation maximum of %u per call", n, kNoAllocMaxPairs);
+    if (n == 0) return B3AMD_OK;
+    if (!ctx->dPool) return fail(B3AMD_ERR_ARGUMENT, "no hull pool uploaded");
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    if (!ctx->noAllocScratch) return fail(B3AMD_ERR_DEVICE, "no scratch address for device %d", ctx->device);
+    uint32_t* base = static_cast<uint32_t*>(ctx->noAllocScratch);
+    const BatchScratch sc{base, reinterpret_cast<SatResult*>(base + kNoAllocStageWords),
+                          base + kNoAllocStageWords + kNoAllocSatWords};
+    // Kernel timing would create events on demand: off for this call.
+    const bool profiling = ctx->profiling;
+    ctx->profiling = false;
+    const int rc = enqueue_batch(ctx, n, d_pairs, reinterpret_cast<const float*>(d_xfs),
+                                 reinterpret_cast<const float*>(d_sweeps), reinterpret_cast<ConvexCache*>(d_caches),
+                                 reinterpret_cast<Manifold*>(d_out), sc, nullptr);
+    ctx->profiling = profiling;
+    return rc;
 }
 
 int b3amd_update_contacts(b3amd_context* ctx, uint32_t body_count, const uint32_t* d_body_shapes,

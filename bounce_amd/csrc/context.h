@@ -96,6 +96,7 @@ struct b3amd_context {
     uint32_t scratchCap = 0;
     SatTuning tuning{0u, 3u};
     Counters* dCounters = nullptr;  // statistics totals (b3amd_get_counters)
+    void* noAllocScratch = nullptr; // this device's copy of the no-allocation scratch array
     bool counting = false;
     void* dIn = nullptr;
     uint64_t dInCap = 0;

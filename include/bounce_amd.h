@@ -142,6 +142,18 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
                          const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
                          b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
 
+/* b3amd_collide_device without any allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, serving loops
+ * and harnesses that forbid hipMalloc).  The batch's scratch lives in a file-scope device
+ * array of the library, one copy per GPU, about 11.5 MB, held until the process ends; n
+ * above B3AMD_NOALLOC_MAX_PAIRS returns B3AMD_ERR_ARGUMENT.  Every call on one GPU shares
+ * that array: calls that overlap on one GPU (other contexts or streams) race.  The hull
+ * pool and shape table must already be uploaded (those uploads allocate). */
+#define B3AMD_NOALLOC_MAX_PAIRS 262144u
+int b3amd_collide_device_noalloc(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs,
+                                 const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
+                                 b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
+
 /* One world step of b3ContactManager::UpdateContacts (contact_manager.cpp:167-231) on
  * device-resident body state: for each contact i, b3ConvexContact::Collide
  * (convex_contact.cpp:50-62) with bodies A = d_contacts[2i], B = d_contacts[2i+1], i.e.
