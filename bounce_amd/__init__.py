@@ -295,6 +295,15 @@ class MeshContacts:
         _check(self.lib.b3amd_mesh_contacts_collide(self.handle, ptr(xfs), ptr(sweeps), ptr(out), ptr(counts)))
         return out, counts
 
+    def test_overlap(self, xfs):
+        """b3MeshContact::TestOverlap per contact (sensor contacts); uint32 flags."""
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
+        out = np.zeros(self.n, np.uint32)
+        f = self.lib.b3amd_mesh_contacts_test_overlap
+        f.argtypes = [ctypes.c_void_p] * 3
+        _check(f(self.handle, ptr(xfs), ptr(out)))
+        return out
+
     def synchronize(self, xfs, centers0):
         """SynchronizeShapes + FindNewPairs for new transforms (centers0: body A's worldCenter0)."""
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
@@ -404,6 +413,20 @@ class Context:
     def copy_on_device(self, dst, src, nbytes):
         _check(self.lib.b3amd_copy_on_device(self.handle, dst.ptr, src.ptr, int(nbytes)))
 
+    def test_overlap(self, pairs, xfs, caches=None):
+        """b3TestOverlap per pair (sensors); caches (CACHE_DTYPE, optional) are read and
+        updated in place.  Returns the overlap flags (uint32: 1 overlapping, 0 apart)."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = len(pairs)
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        out = np.zeros(n, np.uint32)
+        if caches is not None:
+            assert caches.dtype == CACHE_DTYPE and len(caches) == n and caches.flags["C_CONTIGUOUS"]
+        f = self.lib.b3amd_test_overlap
+        f.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4
+        _check(f(self.handle, n, ptr(pairs), ptr(xfs), None if caches is None else ptr(caches), ptr(out)))
+        return out
+
     def collide_device(self, n, d_pairs, d_xfs, d_out, d_sweeps=None, d_caches=None):
         """Enqueue a batch on device-resident buffers (no synchronisation)."""
         p = lambda b: None if b is None else b.ptr
@@ -446,7 +469,7 @@ class Context:
         _check(self.lib.b3amd_autotune(self.handle, n, d_pairs.ptr, d_xfs.ptr, d_out.ptr, repeats, ctypes.byref(ms)))
         return self.sat_tuning(), ms.value
 
-    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters")
+    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters", "gjk_cache_hits")
 
     def set_counting(self, on):
         """Statistics counters (b3_convexCalls, b3_convexCacheHits, b3_gjk*) on or off."""
@@ -455,7 +478,7 @@ class Context:
     def counters(self):
         raw = np.zeros(5, np.uint64)
         _check(self.lib.b3amd_get_counters(self.handle, ptr(raw)))
-        vals = [int(raw[0]), int(raw[1]), int(raw[2]), int(raw[3]), int(raw[4]) & 0xFFFFFFFF]
+        vals = [int(raw[0]), int(raw[1]), int(raw[2]), int(raw[3]), int(raw[4]) & 0xFFFFFFFF, int(raw[4]) >> 32]
         return dict(zip(self.COUNTERS, vals))
 
     def reset_counters(self):

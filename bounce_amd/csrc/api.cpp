@@ -551,6 +551,62 @@ int b3amd_collide(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const b
     return ctx->resolve_timings();
 }
 
+int b3amd_test_overlap_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                              b3amd_convex_cache* d_caches, uint32_t* d_overlap) {
+    if (!ctx || (n && (!d_pairs || !d_xfs || !d_overlap))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (n == 0) return B3AMD_OK;
+    if (!ctx->dShapes) return fail(B3AMD_ERR_ARGUMENT, "no shape table uploaded");
+    OverlapArgs a;
+    a.n = n;
+    a.pairs = d_pairs;
+    a.shapes = ctx->dShapes;
+    a.xfs = reinterpret_cast<const float*>(d_xfs);
+    a.pool = ctx->dPool;
+    a.offsets = ctx->dOffsets;
+    a.shapeCount = ctx->shapeCount;
+    a.hullCount = ctx->hullCount;
+    a.slots = nullptr;
+    a.counters = ctx->counting ? ctx->dCounters : nullptr;
+    a.caches = reinterpret_cast<ConvexCache*>(d_caches);
+    a.overlap = d_overlap;
+    return timed(ctx, ctx->stream, "overlap", [&] { return launch_overlap(a, ctx->stream); });
+}
+
+int b3amd_test_overlap(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const b3amd_transform* xfs,
+                       b3amd_convex_cache* caches, uint32_t* overlap) {
+    if (!ctx || (n && (!pairs || !xfs || !overlap))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (n == 0) return B3AMD_OK;
+    int rc = validate_pairs(ctx, n, pairs);
+    if (rc) return rc;
+    const uint64_t bPairs = 8ull * n, bXf = 96ull * n, bC = caches ? 36ull * n : 0, bOut = 4ull * n;
+    auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
+    const uint64_t total = al(bPairs) + al(bXf) + al(bC) + al(bOut);
+    if (total > ctx->dInCap) {
+        if ((rc = ctx->release(ctx->dIn))) return rc;
+        ctx->dIn = nullptr;
+        ctx->dInCap = 0;
+        if ((rc = ctx->alloc(total, &ctx->dIn))) return rc;
+        ctx->dInCap = total;
+    }
+    char* dPairs = static_cast<char*>(ctx->dIn);
+    char* dXf = dPairs + al(bPairs);
+    char* dC = dXf + al(bXf);
+    char* dOut = dC + al(bC);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dPairs, pairs, bPairs, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dXf, xfs, bXf, hipMemcpyHostToDevice, s));
+    if (caches) HIP_TRY(hipMemcpyAsync(dC, caches, bC, hipMemcpyHostToDevice, s));
+    rc = b3amd_test_overlap_device(ctx, n, reinterpret_cast<const uint32_t*>(dPairs),
+                                   reinterpret_cast<const b3amd_transform*>(dXf),
+                                   caches ? reinterpret_cast<b3amd_convex_cache*>(dC) : nullptr,
+                                   reinterpret_cast<uint32_t*>(dOut));
+    if (rc) return rc;
+    HIP_TRY(hipMemcpyAsync(overlap, dOut, bOut, hipMemcpyDeviceToHost, s));
+    if (caches) HIP_TRY(hipMemcpyAsync(caches, dC, bC, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ctx->resolve_timings();
+}
+
 int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
                          const b3amd_quat* d_sweeps, b3amd_convex_cache* d_caches, b3amd_manifold* d_out) {
     if (!ctx || (n && (!d_pairs || !d_xfs || !d_out))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
@@ -1017,6 +1073,24 @@ int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* set, const b3amd_transform*
     return ctx->resolve_timings();
 }
 
+int b3amd_mesh_contacts_test_overlap(b3amd_mesh_contacts* set, const b3amd_transform* xfs, uint32_t* overlap) {
+    if (!set || (set->n && (!xfs || !overlap))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    b3amd_context* ctx = set->ctx;
+    const uint32_t n = set->n;
+    if (n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
+    MeshArgs a = mesh_set_args(set, sc);
+    a.overlap = sc.manifoldCounts;  // [n] words of the scratch, free outside Collide
+    if ((rc = timed(ctx, s, "mesh_overlap", [&] { return launch_mesh_overlap(a, s); }))) return rc;
+    HIP_TRY(hipMemcpyAsync(overlap, a.overlap, 4ull * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ctx->resolve_timings();
+}
+
 int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* set, const b3amd_transform* xfs, const float* centers0) {
     if (!set || (set->n && (!xfs || !centers0))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     b3amd_context* ctx = set->ctx;
@@ -1208,6 +1282,7 @@ int b3amd_get_counters(b3amd_context* ctx, b3amd_counters* out) {
         t.gjkCalls += c.gjkCalls;
         t.gjkIters += c.gjkIters;
         t.gjkMaxIters = c.gjkMaxIters > t.gjkMaxIters ? c.gjkMaxIters : t.gjkMaxIters;
+        t.gjkCacheHits += c.gjkCacheHits;
     }
     std::memcpy(out, &t, sizeof(t));
     return B3AMD_OK;

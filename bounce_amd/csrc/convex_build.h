@@ -208,17 +208,72 @@ struct GJKOut {
     uint32_t iterations;  // support calls (gjk.cpp:671-673)
 };
 
-// b3GJK with an empty cache.  A vertex added to the simplex starts with weight 0 (the
-// reference leaves it unset; only the degenerate give-up returns could observe it).
-__device__ inline GJKOut gjk(const Xf& xf1, const Proxy& p1, const Xf& xf2, const Proxy& p2) {
+// b3Simplex::GetMetric (gjk.cpp:806-841): 0 for one vertex, the segment length, the
+// triangle's doubled area, the tetrahedron's |det| (b3Det = dot(a, cross(b, c)), b3Sign).
+__device__ inline float simplex_metric(const Simplex& s) {
+    if (s.count == 2) return length(s.v[0].point - s.v[1].point);
+    if (s.count == 3) return length(cross(s.v[1].point - s.v[0].point, s.v[2].point - s.v[0].point));
+    if (s.count == 4) {
+        const V3 E1 = s.v[1].point - s.v[0].point, E2 = s.v[2].point - s.v[0].point;
+        const V3 E3 = s.v[3].point - s.v[0].point;
+        const float det = dot(E1, cross(E2, E3));
+        const float sign = (float)(0.0f < det) - (float)(det < 0.0f);
+        return sign * det;
+    }
+    return 0.0f;
+}
+
+// Byte k (0..19) of a b3SimplexCache record held as five words: metric (0-3), iterations
+// (4-7), count (8-9, u16), index1[4] (10-13), index2[4] (14-17), padding.
+__device__ __forceinline__ uint32_t cache_byte(const uint32_t* w, uint32_t k) { return (w[k >> 2] >> (8u * (k & 3u))) & 0xFFu; }
+__device__ __forceinline__ void set_cache_byte(uint32_t* w, uint32_t k, uint32_t b) {
+    const uint32_t sh = 8u * (k & 3u);
+    w[k >> 2] = (w[k >> 2] & ~(0xFFu << sh)) | ((b & 0xFFu) << sh);
+}
+
+// b3GJK (gjk.cpp:579-737) with b3Simplex::ReadCache / WriteCache (gjk.cpp:743-804) on a
+// b3SimplexCache held as five words (null: the no-cache wrapper, gjk.cpp:845-851: an empty
+// cache, not written back).  A vertex added by a support step starts with weight 0 (the
+// reference leaves it unset; only the degenerate give-up returns could observe it;
+// ReadCache zeroes the cached vertices' weights itself).  A cache count above 4 or a cached
+// index beyond a proxy's vertices (not a cache the reference could have written for these
+// shapes) is read as empty.  cacheHit: ReadCache kept a simplex of two or more vertices
+// (b3_gjkCacheHits).
+__device__ inline GJKOut gjk_cached(const Xf& xf1, const Proxy& p1, const Xf& xf2, const Proxy& p2, uint32_t* cache,
+                                    bool& cacheHit) {
     Simplex s;
-    s.v[0].point1 = mul(xf1, p1.vertex(0));
-    s.v[0].point2 = mul(xf2, p2.vertex(0));
-    s.v[0].point = s.v[0].point2 - s.v[0].point1;
-    s.v[0].weight = 1.0f;
-    s.v[0].index1 = 0;
-    s.v[0].index2 = 0;
-    s.count = 1;
+    cacheHit = false;
+    s.count = 0;
+    if (cache) {
+        s.count = cache[2] & 0xFFu;  // (u8)count
+        if (s.count > 4u) s.count = 0;
+        for (uint32_t i = 0; i < s.count; ++i)
+            if (cache_byte(cache, 10u + i) >= p1.count || cache_byte(cache, 14u + i) >= p2.count) s.count = 0;
+        for (uint32_t i = 0; i < s.count; ++i) {
+            SVert& v = s.v[i];
+            v.index1 = cache_byte(cache, 10u + i);
+            v.index2 = cache_byte(cache, 14u + i);
+            v.point1 = mul(xf1, p1.vertex(v.index1));
+            v.point2 = mul(xf2, p2.vertex(v.index2));
+            v.point = v.point2 - v.point1;
+            v.weight = 0.0f;
+        }
+        if (s.count > 1u) {
+            const float metric1 = __uint_as_float(cache[0]);
+            const float metric2 = simplex_metric(s);
+            if (metric2 < 0.5f * metric1 || 2.0f * metric1 < metric2 || metric2 < kEpsilon) s.count = 0;  // flush
+            else cacheHit = true;
+        }
+    }
+    if (s.count == 0) {
+        s.v[0].point1 = mul(xf1, p1.vertex(0));
+        s.v[0].point2 = mul(xf2, p2.vertex(0));
+        s.v[0].point = s.v[0].point2 - s.v[0].point1;
+        s.v[0].weight = 1.0f;
+        s.v[0].index1 = 0;
+        s.v[0].index2 = 0;
+        s.count = 1;
+    }
     uint32_t save1[4], save2[4];
     uint32_t iter = 0;
     while (iter < 20) {
@@ -279,7 +334,30 @@ __device__ inline GJKOut gjk(const Xf& xf1, const Proxy& p1, const Xf& xf2, cons
         o.point2 = o.point1;
     }
     o.distance = length(o.point1 - o.point2);
+    if (cache) {  // WriteCache: metric, count and the used indices; the rest of the record kept
+        cache[0] = __float_as_uint(simplex_metric(s));
+        cache[2] = (cache[2] & 0xFFFF0000u) | s.count;
+        for (uint32_t i = 0; i < s.count; ++i) {
+            set_cache_byte(cache, 10u + i, s.v[i].index1);
+            set_cache_byte(cache, 14u + i, s.v[i].index2);
+        }
+    }
     return o;
+}
+
+__device__ inline GJKOut gjk(const Xf& xf1, const Proxy& p1, const Xf& xf2, const Proxy& p2) {
+    bool hit;
+    return gjk_cached(xf1, p1, xf2, p2, nullptr, hit);
+}
+
+// b3TestOverlap (collide.cpp:84-95): b3GJK's applyRadius branch (gjk.cpp:709-733) reduced to
+// the distance (the witness points it moves do not enter the result), then the 10 FLT_EPSILON
+// tolerance.
+__device__ __forceinline__ bool overlap_of(const GJKOut& g, float r1, float r2) {
+    float distance = g.distance;
+    if (g.distance > r1 + r2 && g.distance > kEpsilon) distance -= r1 + r2;
+    else distance = 0.0f;
+    return distance <= 10.0f * kEpsilon;
 }
 
 // ------------------------------------------------------------------ closed forms

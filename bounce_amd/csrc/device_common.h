@@ -63,6 +63,12 @@ __device__ __forceinline__ void wave_count(unsigned long long* dst, uint32_t v) 
     for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
     if ((threadIdx.x & 63u) == 0 && s) atomicAdd(dst, s);
 }
+__device__ __forceinline__ void wave_count(unsigned int* dst, uint32_t v) {
+    uint32_t s = v;
+#pragma unroll
+    for (int off = 32; off >= 1; off >>= 1) s += __shfl_xor(s, off, 64);
+    if ((threadIdx.x & 63u) == 0 && s) atomicAdd(dst, s);
+}
 __device__ __forceinline__ void wave_max(unsigned int* dst, uint32_t v) {
 #pragma unroll
     for (int off = 32; off >= 1; off >>= 1) {

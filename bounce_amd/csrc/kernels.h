@@ -62,7 +62,7 @@ struct Counters {
     unsigned long long gjkCalls;         // b3_gjkCalls
     unsigned long long gjkIters;         // b3_gjkIters
     unsigned int gjkMaxIters;            // b3_gjkMaxIters
-    unsigned int pad;
+    unsigned int gjkCacheHits;           // b3_gjkCacheHits: simplex caches kept (sensor path)
 };
 
 // The totals are kept in kCounterSlots copies, block b adding into copy b % kCounterSlots, so
@@ -229,6 +229,15 @@ struct ConvexArgs : PairInputs {
     Manifold* out;           // [n]
 };
 
+// Sensor overlap tests (b3TestOverlap): overlap[p] = 1 overlapping, 0 apart, kOverlapInvalid for
+// a pair the shape table cannot resolve; the simplex cache of caches[slot_of(p)] is read and
+// written (null: fresh caches, not kept).
+constexpr uint32_t kOverlapInvalid = 2u;
+struct OverlapArgs : PairInputs {
+    ConvexCache* caches;     // [slots] or null
+    uint32_t* overlap;       // [n]
+};
+
 // A device-resident b3Mesh: vertices, triangles (v1, v2, v3, 0) and the static tree
 // (8 words per node: lower xyz, upper xyz, child1, child2|triangle; static_tree.h:62-77).
 struct MeshDev {
@@ -272,6 +281,7 @@ struct MeshArgs {
     const float* sweeps;        // [n][4][4] orientation0/orientation of body A, then the mesh body
     float* fat;                 // [n][6] m_aabbA in the mesh frame (lower xyz, upper xyz)
     const float* centers0;      // [n][3] body A's sweep worldCenter0 (the displacement's origin)
+    uint32_t* overlap = nullptr;  // [n] sensor test result (b3MeshContact::TestOverlap)
 };
 
 // Mesh-contact state update: kMeshCreate is the b3MeshContact constructor followed by
@@ -287,6 +297,8 @@ hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
 // The grid covers the n * cap bound: the work-item count stays on the device.
 hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
+// b3MeshContact::TestOverlap of every persistent contact (one lane per contact).
+hipError_t launch_mesh_overlap(const MeshArgs& a, hipStream_t s);
 
 // The shape pairs of one world step's contacts: contact i joins bodies contacts[2i],
 // contacts[2i+1] (the kernels read their transforms and sweeps through PairInputs::bodies).
@@ -301,6 +313,7 @@ struct BodyGatherArgs {
 hipError_t launch_body_gather(const BodyGatherArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
+hipError_t launch_overlap(const OverlapArgs& a, hipStream_t s);
 hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s);
 // Zeroes the counter words, marks the cached pairs and lists the SAT-bound ones, reads the
 // feature caches four lanes per cached pair (listing the contacts to rebuild), rebuilds them

@@ -59,7 +59,56 @@ __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
     }
 }
 
+// b3TestOverlap (collide.cpp:84-95) for every pair of convex shapes, one lane per pair: the
+// sensor path of b3ConvexContact::TestOverlap (convex_contact.cpp:39-48).  The shape proxies
+// are b3ShapeGJKProxy::Set's (collide.cpp:30-82): a sphere's centre, a capsule's two centres
+// or a hull's vertices, with the shape radius; GJK reads and writes the pair's simplex cache.
+// The destinations are computed before any divergent branch and stored once at the end.
+__global__ __launch_bounds__(256) void overlap_kernel(OverlapArgs a) {
+    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t calls = 0, iters = 0, hits = 0;
+    if (p < a.n) {
+        uint32_t* const flag = a.overlap + p;
+        uint32_t* const cw = a.caches ? a.caches[a.slot_of(p)].simplex : nullptr;
+        uint32_t result = kOverlapInvalid;
+        uint32_t w[5] = {0u, 0u, 0u, 0u, 0u};
+        const uint32_t cls = a.classify(p);
+        if (cls != 0u) {
+            const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
+            const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];
+            const Xf xf1 = load_xf(a.xf_ptr(p, 0));
+            const Xf xf2 = load_xf(a.xf_ptr(p, 1));
+            const Proxy p1 = s1.type == 2u ? hull_proxy(DevHull::at_unit(a.pool, s1.hullUnit)) : round_proxy(s1);
+            const Proxy p2 = s2.type == 2u ? hull_proxy(DevHull::at_unit(a.pool, s2.hullUnit)) : round_proxy(s2);
+            if (cw)
+                for (int t = 0; t < 5; ++t) w[t] = cw[t];
+            bool hit = false;
+            const GJKOut g = gjk_cached(xf1, p1, xf2, p2, cw ? w : nullptr, hit);
+            result = overlap_of(g, s1.radius, s2.radius) ? 1u : 0u;
+            calls = 1;
+            iters = g.iterations;
+            hits = hit ? 1u : 0u;
+        }
+        *flag = result;
+        if (cw && cls != 0u)
+            for (int t = 0; t < 5; ++t) cw[t] = w[t];
+    }
+    if (a.counters) {  // all lanes of the block reach here
+        Counters* const c = counter_slot(a.counters);
+        wave_count(&c->gjkCalls, calls);
+        wave_count(&c->gjkIters, iters);
+        wave_max(&c->gjkMaxIters, iters);
+        wave_count(&c->gjkCacheHits, hits);
+    }
+}
+
 }  // namespace
+
+hipError_t launch_overlap(const OverlapArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    overlap_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;

@@ -577,6 +577,62 @@ hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s) {
     return hipGetLastError();
 }
 
+// b3MeshContact::TestOverlap (mesh_contact.cpp:191-215), the sensor path of a mesh contact: one
+// lane per contact walks its triangle list in order and runs b3TestOverlap (collide.cpp:84-95)
+// of shape A against each triangle (proxy: the three vertices, radius B3_HULL_RADIUS = 0),
+// reading and writing that triangle's simplex cache, and stops at the first overlap, so the
+// caches of later triangles keep their contents.  Lists cut at the capacity (flagged
+// B3AMD_FLAG_TRIANGLE_OVERFLOW by the midphase) are tested up to the capacity.
+__global__ __launch_bounds__(256) void mesh_overlap_kernel(MeshArgs a) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t calls = 0, iters = 0, maxIters = 0, hits = 0;
+    if (c < a.n) {
+        uint32_t* const flag = a.overlap + c;
+        uint32_t result = 0;
+        const ShapeRec s = a.shapes[a.shapeA[c]];
+        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+        const Proxy pA = s.type == 2u ? hull_proxy(DevHull::at_unit(a.pool, s.hullUnit)) : round_proxy(s);
+        const uint32_t count = a.triCounts[c] < a.cap ? a.triCounts[c] : a.cap;
+        for (uint32_t k = 0; k < count && result == 0u; ++k) {
+            const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
+            const uint4 t = a.mesh.tris[ti];
+            const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
+            Proxy pB;
+            pB.pts[0] = V3{A.x, A.y, A.z};
+            pB.pts[1] = V3{B.x, B.y, B.z};
+            pB.pts[2] = V3{C.x, C.y, C.z};
+            pB.count = 3;
+            pB.hullVerts = nullptr;
+            uint32_t* const cw = a.triCaches[(size_t)c * a.cap + k].simplex;
+            uint32_t w[5];
+            for (int q = 0; q < 5; ++q) w[q] = cw[q];
+            bool hit = false;
+            const GJKOut g = gjk_cached(xfA, pA, xfB, pB, w, hit);
+            for (int q = 0; q < 5; ++q) cw[q] = w[q];
+            calls += 1;
+            iters += g.iterations;
+            maxIters = g.iterations > maxIters ? g.iterations : maxIters;
+            hits += hit ? 1u : 0u;
+            if (overlap_of(g, s.radius, 0.0f)) result = 1u;
+        }
+        *flag = result;
+    }
+    if (a.counters) {  // all lanes of the block reach here
+        Counters* const ct = counter_slot(a.counters);
+        wave_count(&ct->gjkCalls, calls);
+        wave_count(&ct->gjkIters, iters);
+        wave_max(&ct->gjkMaxIters, maxIters);
+        wave_count(&ct->gjkCacheHits, hits);
+    }
+}
+
+hipError_t launch_mesh_overlap(const MeshArgs& a, hipStream_t s) {
+    if (a.n == 0) return hipSuccess;
+    mesh_overlap_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     mesh_cluster_kernel<<<(a.n + 63) / 64, 64, 0, s>>>(a);

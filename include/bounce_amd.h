@@ -142,6 +142,24 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
                          const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
                          b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
 
+/* ---- sensors ----
+ * For each pair i: b3TestOverlap(xfs[2i], 0, shapes[pairs[2i]], xfs[2i+1], 0,
+ * shapes[pairs[2i+1]], &caches[i]) (collide.cpp:84-95), the test b3Contact::Update runs
+ * instead of the narrowphase when either shape is a sensor (contact.cpp:55-61, through
+ * b3ConvexContact::TestOverlap, convex_contact.cpp:39-48): GJK distance between the shapes
+ * (b3ShapeGJKProxy, collide.cpp:30-82) with the radii applied (gjk.cpp:709-733), warm-started
+ * from and written back to each pair's b3SimplexCache (gjk.cpp:743-841; u8 vertex indices),
+ * overlapping when within 10 FLT_EPSILON.  Any convex pair with typeA <= typeB, hull-hull
+ * included.  overlap[i] = 1 overlapping, 0 apart, B3AMD_OVERLAP_INVALID for a pair the
+ * shape table cannot resolve.  caches may be NULL (fresh simplex caches, not kept); only the
+ * simplex part of each record is read and written. */
+#define B3AMD_OVERLAP_INVALID 2u
+int b3amd_test_overlap(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const b3amd_transform* xfs,
+                       b3amd_convex_cache* caches, uint32_t* overlap);
+/* Same on device-resident buffers, enqueued on the context stream without synchronizing. */
+int b3amd_test_overlap_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
+                              b3amd_convex_cache* d_caches, uint32_t* d_overlap);
+
 /* b3amd_collide_device without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops
  * and harnesses that forbid hipMalloc).  The batch's scratch lives in a file-scope device
@@ -273,19 +291,20 @@ int b3amd_server_configure(b3amd_server* server, uint32_t batch_pairs, uint32_t 
 /* ---- statistics ----
  * Device equivalents of the reference's global counters: b3_convexCalls and
  * b3_convexCacheHits (collide_hulls.cpp:291-292,304; collide_hulls_cache.cpp:300,334),
- * b3_gjkCalls, b3_gjkIters and b3_gjkMaxIters (gjk.cpp:27,583,673,697), accumulated by
- * every batch (convex pairs and mesh contacts) while counting is on.  Counting is off by
- * default (it adds one atomic per wave and counter).  b3World::Step resets the
- * reference's counters every step (world.cpp:122-127); call b3amd_reset_counters for the
- * same.  The simplex-cache counter b3_gjkCacheHits has no counterpart: the narrowphase
- * runs GJK without a simplex cache (gjk.cpp:845-851), as the reference's does. */
+ * b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters and b3_gjkCacheHits (gjk.cpp:27,583,673,697,
+ * 740,774), accumulated by every batch (convex pairs, mesh contacts and sensor overlap
+ * tests) while counting is on.  Counting is off by default (it adds one atomic per wave and
+ * counter).  b3World::Step resets the reference's counters every step
+ * (world.cpp:122-127); call b3amd_reset_counters for the same.  Only the sensor path reads
+ * simplex caches (b3TestOverlap); the narrowphase runs GJK without one (gjk.cpp:845-851),
+ * as the reference's does. */
 typedef struct b3amd_counters {
     uint64_t convex_calls;
     uint64_t convex_cache_hits;
     uint64_t gjk_calls;
     uint64_t gjk_iters;
     uint32_t gjk_max_iters;
-    uint32_t reserved;
+    uint32_t gjk_cache_hits;
 } b3amd_counters;
 int b3amd_set_counting(b3amd_context* ctx, int on);
 /* Synchronizes the context stream and reads the totals. */
@@ -352,6 +371,11 @@ int b3amd_mesh_contacts_create(b3amd_context* ctx, uint32_t meshId, uint32_t n, 
 int b3amd_mesh_contacts_destroy(b3amd_mesh_contacts* contacts);
 int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, const b3amd_quat* sweeps,
                                 b3amd_manifold* out, uint32_t* manifoldCounts);
+/* b3MeshContact::TestOverlap (mesh_contact.cpp:191-215) for each contact whose shape pair
+ * holds a sensor: b3TestOverlap of shape A against the listed triangles in list order, each
+ * with its own simplex cache, stopping at the first overlap (the caches of later triangles
+ * are left as they were).  overlap[i] = 1 / 0. */
+int b3amd_mesh_contacts_test_overlap(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, uint32_t* overlap);
 int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, const float* centers0);
 int b3amd_mesh_contacts_read(b3amd_mesh_contacts* contacts, uint32_t* triangleCounts, uint32_t* triangleLists,
                              b3amd_convex_cache* caches, float* fatAabbs);

@@ -671,8 +671,8 @@ static void collide_cache(b3o_manifold* m, const XF* xf1, const Hull* h1, float 
 
 /* Statistics kept by the reference in globals: b3_convexCalls, b3_convexCacheHits
  * (collide_hulls.cpp:291-292), b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters (gjk.cpp:27). */
-static uint64_t g_stats[5];
-void b3o_get_counters(uint64_t out[5]) { memcpy(out, g_stats, sizeof(g_stats)); }
+static uint64_t g_stats[6];  /* ... and b3_gjkCacheHits (gjk.cpp:740) */
+void b3o_get_counters(uint64_t out[6]) { memcpy(out, g_stats, sizeof(g_stats)); }
 void b3o_reset_counters(void) { memset(g_stats, 0, sizeof(g_stats)); }
 
 static void collide_hulls_cached(b3o_manifold* m, const XF* xf1, const Hull* h1, float r1, /* collide_hulls_cache.cpp:279-344 */
@@ -914,18 +914,70 @@ static void solve4(Simplex* s, V3 Q) {                                          
     s->v[3].weight = d * wABCD[3];
 }
 
-/* b3GJK with an empty simplex cache (gjk.cpp:579-737, 743-804, 845-851).  A vertex added
- * to the simplex starts with weight 0; the reference leaves it unset, which is only
- * observable after the degenerate give-up returns of Solve3/Solve4. */
-static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2) {
+/* b3Simplex::GetMetric (gjk.cpp:806-841): 0 for one vertex, the segment length, the
+ * triangle's doubled area, or the tetrahedron's |det| (b3Det = dot(a, cross(b, c)),
+ * vec3.h:171-174; b3Sign, math.h:80-83). */
+static float simplex_metric(const Simplex* s) {
+    switch (s->count) {
+    case 2:
+        return length(sub(s->v[0].point, s->v[1].point));  /* b3Distance (vec3.h:201-204) */
+    case 3: {
+        V3 E1 = sub(s->v[1].point, s->v[0].point), E2 = sub(s->v[2].point, s->v[0].point);
+        return length(cross(E1, E2));
+    }
+    case 4: {
+        V3 E1 = sub(s->v[1].point, s->v[0].point), E2 = sub(s->v[2].point, s->v[0].point);
+        V3 E3 = sub(s->v[3].point, s->v[0].point);
+        float det = dot(E1, cross(E2, E3));
+        float sign = (float)(0.0f < det) - (float)(det < 0.0f);
+        return sign * det;
+    }
+    default:
+        return 0.0f;
+    }
+}
+
+/* b3GJK (gjk.cpp:579-737) with b3Simplex::ReadCache / WriteCache (gjk.cpp:743-804).  cache
+ * NULL: the no-cache wrapper (gjk.cpp:845-851), an empty cache that is not written back.
+ * A vertex added to the simplex by a support step starts with weight 0; the reference
+ * leaves it unset, which is only observable after the degenerate give-up returns of
+ * Solve3/Solve4 (ReadCache sets the cached vertices' weights to 0 itself).  A cache count
+ * above 4 or a cached index beyond a proxy's vertices (a cache the reference could not have
+ * written for these shapes; it would read out of bounds) is read as an empty cache. */
+static GJKOut gjk_cached(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2, b3o_simplex_cache* cache) {
     Simplex s;
     memset(&s, 0, sizeof(s));
     ++g_stats[2];  /* gjk.cpp:583 */
-    s.v[0].point1 = xf_mul(xf1, p1->v[0]);
-    s.v[0].point2 = xf_mul(xf2, p2->v[0]);
-    s.v[0].point = sub(s.v[0].point2, s.v[0].point1);
-    s.v[0].weight = 1.0f;
-    s.count = 1;
+    s.count = cache ? (uint8_t)cache->count : 0u;
+    if (s.count > 4) s.count = 0;
+    for (uint32_t i = 0; i < s.count; ++i)
+        if (cache->index1[i] >= p1->count || cache->index2[i] >= p2->count) s.count = 0;
+    for (uint32_t i = 0; i < s.count; ++i) {
+        SVert* v = s.v + i;
+        v->index1 = cache->index1[i];
+        v->index2 = cache->index2[i];
+        v->point1 = xf_mul(xf1, p1->v[v->index1]);
+        v->point2 = xf_mul(xf2, p2->v[v->index2]);
+        v->point = sub(v->point2, v->point1);
+        v->weight = 0.0f;
+    }
+    if (s.count > 1) {
+        float metric1 = cache->metric;
+        float metric2 = simplex_metric(&s);
+        if (metric2 < 0.5f * metric1 || 2.0f * metric1 < metric2 || metric2 < EPSILON)
+            s.count = 0;  /* flush */
+        else
+            ++g_stats[5]; /* b3_gjkCacheHits (gjk.cpp:774) */
+    }
+    if (s.count == 0) {
+        s.v[0].point1 = xf_mul(xf1, p1->v[0]);
+        s.v[0].point2 = xf_mul(xf2, p2->v[0]);
+        s.v[0].point = sub(s.v[0].point2, s.v[0].point1);
+        s.v[0].weight = 1.0f;
+        s.v[0].index1 = 0;
+        s.v[0].index2 = 0;
+        s.count = 1;
+    }
     uint32_t save1[4], save2[4], saveCount;
     const V3 origin = v3(0.0f, 0.0f, 0.0f);
     uint32_t iter = 0;
@@ -963,7 +1015,19 @@ static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2
     closest_points(&s, &o.point1, &o.point2);
     o.distance = length(sub(o.point1, o.point2));
     o.iterations = iter;
+    if (cache) {  /* b3Simplex::WriteCache (gjk.cpp:796-804): metric, count and the used indices */
+        cache->metric = simplex_metric(&s);
+        cache->count = (uint16_t)s.count;
+        for (uint32_t i = 0; i < s.count; ++i) {
+            cache->index1[i] = (uint8_t)s.v[i].index1;
+            cache->index2[i] = (uint8_t)s.v[i].index2;
+        }
+    }
     return o;
+}
+
+static GJKOut gjk(const XF* xf1, const Proxy* p1, const XF* xf2, const Proxy* p2) {
+    return gjk_cached(xf1, p1, xf2, p2, NULL);
 }
 
 /* ---------------------------------------------------------------- convex shape records */
@@ -976,6 +1040,31 @@ static Round round_of(const b3o_shape* s) {
     r.count = s->type == B3O_SPHERE ? 1u : 2u;
     r.radius = s->radius;
     return r;
+}
+
+/* b3ShapeGJKProxy::Set (collide.cpp:30-82) for a convex shape: its vertices and radius. */
+static Proxy shape_proxy(const b3o_hull_pool* pool, const b3o_shape* s, Round* round, Hull* hull, float* radius) {
+    *radius = s->radius;
+    if (s->type == B3O_HULL) {
+        *hull = hull_at(pool, s->hull);
+        Proxy p = {hull->vertices, hull->vertexCount};
+        return p;
+    }
+    *round = round_of(s);
+    Proxy p = {round->c, round->count};
+    return p;
+}
+
+/* b3GJK's applyRadius branch (gjk.cpp:709-733) reduced to the distance, and b3TestOverlap's
+ * tolerance (collide.cpp:84-95): the witness points it moves do not enter the result. */
+static int overlap_of(GJKOut g, float r1, float r2) {
+    float distance = g.distance;
+    if (g.distance > r1 + r2 && g.distance > EPSILON)
+        distance -= r1 + r2;
+    else
+        distance = 0.0f;
+    const float kTol = 10.0f * EPSILON;
+    return distance <= kTol;
 }
 
 static void set_zero_key(b3o_mpoint* mp) {
@@ -2005,6 +2094,34 @@ void b3o_mesh_contacts_collide(b3o_mesh_contacts* h, const b3o_xf* xfs, const b3
     }
 }
 
+/* b3MeshContact::TestOverlap (mesh_contact.cpp:191-215): b3TestOverlap (collide.cpp:84-95)
+ * of shape A against each listed triangle in list order with that triangle's cache (the
+ * triangle proxy: its three vertices, radius B3_HULL_RADIUS = 0, collide.cpp:61-76), stopping
+ * at the first overlap, so the caches of later triangles are left as they were. */
+void b3o_mesh_contacts_test_overlap(b3o_mesh_contacts* h, const b3o_xf* xfs, uint32_t* overlap) {
+    for (uint32_t i = 0; i < h->n; ++i) {
+        const XF* xfA = xfs + 2 * (size_t)i;
+        MeshContact* mc = h->c + i;
+        Round ra;
+        Hull ha;
+        float rA;
+        Proxy pA = shape_proxy(h->pool, h->shapes + h->shapeA[i], &ra, &ha, &rA);
+        overlap[i] = 0;
+        for (uint32_t k = 0; k < mc->count; ++k) {
+            const uint32_t* t = h->mesh->triangles + 3 * (size_t)mc->tris[k];
+            V3 tv[3];
+            for (int j = 0; j < 3; ++j)
+                tv[j] = v3(h->mesh->vertices[3 * t[j]], h->mesh->vertices[3 * t[j] + 1], h->mesh->vertices[3 * t[j] + 2]);
+            Proxy pB = {tv, 3};
+            GJKOut g = gjk_cached(xfA, &pA, xfA + 1, &pB, &mc->caches[k].simplex);
+            if (overlap_of(g, rA, 0.0f)) {
+                overlap[i] = 1;
+                break;
+            }
+        }
+    }
+}
+
 /* SynchronizeShapes + MoveAABB (:62-139) and FindNewPairs. */
 void b3o_mesh_contacts_synchronize(b3o_mesh_contacts* h, const b3o_xf* xfs, const float* centers0) {
     for (uint32_t i = 0; i < h->n; ++i) {
@@ -2111,6 +2228,23 @@ void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes, uint3
             else if (sa->type == B3O_SPHERE) collide_sphere_capsule(m, xa, &r1, xb, &r2);
             else collide_capsules(m, xa, &r1, xb, &r2);
         }
+    }
+}
+
+void b3o_test_overlap(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n, const uint32_t* pairs,
+                      const b3o_xf* xfs, b3o_convex_cache* caches, uint32_t* overlap) {
+    for (uint32_t i = 0; i < n; ++i) {
+        Round ra, rb;
+        Hull ha, hb;
+        float r1, r2;
+        Proxy p1 = shape_proxy(pool, shapes + pairs[2 * i], &ra, &ha, &r1);
+        Proxy p2 = shape_proxy(pool, shapes + pairs[2 * i + 1], &rb, &hb, &r2);
+        const XF* xa = xfs + 2 * (size_t)i;
+        b3o_convex_cache local;
+        b3o_convex_cache* c = caches ? caches + i : &local;
+        if (!caches) fresh_cache(&local);
+        GJKOut g = gjk_cached(xa, &p1, xa + 1, &p2, &c->simplex);
+        overlap[i] = (uint32_t)overlap_of(g, r1, r2);
     }
 }
 

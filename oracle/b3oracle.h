@@ -97,6 +97,13 @@ void b3o_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
                        const b3o_quat* sweeps, b3o_convex_cache* caches,
                        b3o_manifold* out);
 
+/* Sensor overlap of n convex shape pairs as b3TestOverlap (collide.cpp:84-95), the test
+ * b3ConvexContact::TestOverlap runs for contacts with a sensor shape (convex_contact.cpp:39-48):
+ * GJK with the radii applied and each pair's simplex cache read and written (gjk.cpp:579-841).
+ * overlap[i] = 1 when the distance is within 10 FLT_EPSILON. */
+void b3o_test_overlap(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n, const uint32_t* pairs,
+                      const b3o_xf* xfs, b3o_convex_cache* caches, uint32_t* overlap);
+
 /* Separation queries only (sat.cpp:33-58, 110-173) for hull pairs, for kernel-level checks.
  * out[8i..8i+7] = face1.index, face1.sep(bits), face2.index, face2.sep(bits),
  *                 edge.index1, edge.index2, edge.sep(bits), 0 */
@@ -146,6 +153,11 @@ b3o_mesh_contacts* b3o_mesh_contacts_create(const b3o_hull_pool* pool, const b3o
 void b3o_mesh_contacts_collide(b3o_mesh_contacts* h, const b3o_xf* xfs, const b3o_quat* sweeps, b3o_manifold* out,
                                uint32_t* manifoldCounts);
 void b3o_mesh_contacts_synchronize(b3o_mesh_contacts* h, const b3o_xf* xfs, const float* centers0);
+/* b3MeshContact::TestOverlap (mesh_contact.cpp:191-215) for each contact: the sensor test
+ * against the listed triangles in order, each with its cache, stopping at the first
+ * overlap.  overlap[i] = 1 / 0. */
+void b3o_mesh_contacts_test_overlap(b3o_mesh_contacts* h, const b3o_xf* xfs, uint32_t* overlap);
+void b3ref_mesh_contacts_test_overlap(void* handle, const b3o_xf* xfs, uint32_t* overlap);
 void b3o_mesh_contacts_read(b3o_mesh_contacts* h, uint32_t cap, uint32_t* triCounts, uint32_t* triLists,
                             b3o_convex_cache* caches, float* fat);
 void b3o_mesh_contacts_destroy(b3o_mesh_contacts* h);
@@ -160,7 +172,7 @@ void b3ref_mesh_contacts_destroy(void* h);
 
 /* Totals of the reference's statistics globals since the last reset: b3_convexCalls,
  * b3_convexCacheHits, b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters. */
-void b3o_get_counters(uint64_t out[5]);
+void b3o_get_counters(uint64_t out[6]);
 void b3o_reset_counters(void);
 
 /* FP32 ops executed by the last b3o_collide_pairs call (liboracle_count.so only). */

@@ -130,19 +130,51 @@ void to_out(const b3Manifold& m, b3o_manifold* o) {
 
 // The reference's statistics globals (collide_hulls.cpp:291-292, gjk.cpp:27).
 extern u32 b3_convexCalls, b3_convexCacheHits;
-extern u32 b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters;
+extern u32 b3_gjkCalls, b3_gjkIters, b3_gjkMaxIters, b3_gjkCacheHits;
 
-extern "C" void b3ref_get_counters(uint64_t out[5]) {
+extern "C" void b3ref_get_counters(uint64_t out[6]) {
     out[0] = b3_convexCalls;
     out[1] = b3_convexCacheHits;
     out[2] = b3_gjkCalls;
     out[3] = b3_gjkIters;
     out[4] = b3_gjkMaxIters;
+    out[5] = b3_gjkCacheHits;
 }
 
 extern "C" void b3ref_reset_counters(void) {
     b3_convexCalls = b3_convexCacheHits = 0;
-    b3_gjkCalls = b3_gjkIters = b3_gjkMaxIters = 0;
+    b3_gjkCalls = b3_gjkIters = b3_gjkMaxIters = b3_gjkCacheHits = 0;
+}
+
+// b3TestOverlap (collide.cpp:84-95) per pair with the pair's cache, as
+// b3ConvexContact::TestOverlap calls it for a sensor contact (convex_contact.cpp:39-48).
+extern "C" void b3ref_test_overlap(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n,
+                                   const uint32_t* pairs, const b3o_xf* xfs, b3o_convex_cache* caches,
+                                   uint32_t* overlap) {
+    Scene scene;
+    build_hulls(pool, scene.hulls);
+    for (uint32_t i = 0; i < n; ++i) {
+        b3Shape* A = make_shape(shapes[pairs[2 * i]], scene.hulls);
+        b3Shape* B = make_shape(shapes[pairs[2 * i + 1]], scene.hulls);
+        scene.shapes.push_back(A);
+        scene.shapes.push_back(B);
+        b3ConvexCache cache;
+        if (caches) {
+            std::memcpy(&cache, caches + i, sizeof(cache));
+        } else {
+            std::memset(&cache, 0, sizeof(cache));
+            cache.featureCache.m_featurePair.state = e_empty;
+        }
+        b3Transform xfA, xfB;
+        std::memcpy(&xfA, xfs + 2 * (size_t)i, sizeof(xfA));
+        std::memcpy(&xfB, xfs + 2 * (size_t)i + 1, sizeof(xfB));
+        overlap[i] = b3TestOverlap(xfA, 0, A, xfB, 0, B, &cache) ? 1u : 0u;
+        if (caches) std::memcpy(caches + i, &cache, sizeof(cache));
+        if (scene.shapes.size() > 4096) {
+            Scene tmp;
+            tmp.shapes.swap(scene.shapes);
+        }
+    }
 }
 
 extern "C" void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
@@ -284,7 +316,7 @@ extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* s
     }
     // b3World's constructor and destructor reset the statistics globals
     // (world.cpp:39-45, 67-71): keep the totals across the world's lifetime.
-    uint64_t before[5];
+    uint64_t before[6];
     b3ref_get_counters(before);
     b3World* world = new b3World();
     b3BodyDef bdA;
@@ -315,7 +347,7 @@ extern "C" void b3ref_collide_mesh(const b3o_hull_pool* pool, const b3o_shape* s
         }
         delete mc;
     }
-    uint64_t during[5];
+    uint64_t during[6];
     b3ref_get_counters(during);
     delete world;
     b3_convexCalls = (u32)(before[0] + during[0]);
@@ -340,7 +372,7 @@ struct Pyramid {
     std::unordered_map<const b3Shape*, uint32_t> bodyOf;
     std::unordered_map<const b3Contact*, uint32_t> ids;  // live contacts -> stable id
     uint32_t nextId = 0;
-    uint64_t saved[5];
+    uint64_t saved[6];
 };
 
 uint32_t contact_id(Pyramid* p, const b3Contact* c) {
@@ -400,6 +432,7 @@ extern "C" void b3ref_pyramid_destroy(void* handle) {
     b3_gjkCalls = (u32)p->saved[2];
     b3_gjkIters = (u32)p->saved[3];
     b3_gjkMaxIters = (u32)p->saved[4];
+    b3_gjkCacheHits = (u32)p->saved[5];
     delete p;
 }
 
@@ -499,7 +532,7 @@ struct RefMeshContacts {
     b3Body* bodyB = nullptr;
     b3MeshShape meshShape;
     std::vector<b3MeshContact*> contacts;
-    uint64_t saved[5];
+    uint64_t saved[6];
 };
 
 void pose(b3Body* b, const b3o_xf* xf) { std::memcpy(&b->m_xf, xf, sizeof(b3Transform)); }
@@ -538,6 +571,7 @@ extern "C" void* b3ref_mesh_contacts_create(const b3o_hull_pool* pool, const b3o
     b3_gjkCalls = (u32)r->saved[2];
     b3_gjkIters = (u32)r->saved[3];
     b3_gjkMaxIters = (u32)r->saved[4];
+    b3_gjkCacheHits = (u32)r->saved[5];
     return r;
 }
 
@@ -570,6 +604,15 @@ extern "C" void b3ref_mesh_contacts_collide(void* handle, const b3o_xf* xfs, con
     }
 }
 
+extern "C" void b3ref_mesh_contacts_test_overlap(void* handle, const b3o_xf* xfs, uint32_t* overlap) {
+    RefMeshContacts* r = (RefMeshContacts*)handle;
+    for (size_t i = 0; i < r->contacts.size(); ++i) {
+        pose(r->bodyA, xfs + 2 * i);
+        pose(r->bodyB, xfs + 2 * i + 1);
+        overlap[i] = r->contacts[i]->TestOverlap() ? 1u : 0u;
+    }
+}
+
 extern "C" void b3ref_mesh_contacts_synchronize(void* handle, const b3o_xf* xfs, const float* centers0) {
     RefMeshContacts* r = (RefMeshContacts*)handle;
     for (size_t i = 0; i < r->contacts.size(); ++i) {
@@ -599,7 +642,7 @@ extern "C" void b3ref_mesh_contacts_read(void* handle, uint32_t cap, uint32_t* t
 
 extern "C" void b3ref_mesh_contacts_destroy(void* handle) {
     RefMeshContacts* r = (RefMeshContacts*)handle;
-    uint64_t now[5];
+    uint64_t now[6];
     b3ref_get_counters(now);
     for (b3MeshContact* mc : r->contacts) delete mc;
     delete r->world;  // resets the statistics globals: keep them
@@ -608,5 +651,6 @@ extern "C" void b3ref_mesh_contacts_destroy(void* handle) {
     b3_gjkCalls = (u32)now[2];
     b3_gjkIters = (u32)now[3];
     b3_gjkMaxIters = (u32)now[4];
+    b3_gjkCacheHits = (u32)now[5];
     delete r;
 }

@@ -37,6 +37,7 @@ class Checker:
 
     def __init__(self, lib, prefix):
         self.lib = lib
+        self.prefix = prefix
         self._collide, self._queries = _bind(lib, prefix)
         self._get_counters = getattr(lib, prefix + "get_counters")
         self._get_counters.restype = None
@@ -45,17 +46,35 @@ class Checker:
         self._reset_counters.restype = None
         self._reset_counters.argtypes = []
 
-    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters")
+    COUNTERS = ("convex_calls", "convex_cache_hits", "gjk_calls", "gjk_iters", "gjk_max_iters", "gjk_cache_hits")
 
     def counters(self):
         """The statistics globals (b3_convexCalls, b3_convexCacheHits, b3_gjkCalls,
-        b3_gjkIters, b3_gjkMaxIters) since the last reset."""
-        out = np.zeros(5, np.uint64)
+        b3_gjkIters, b3_gjkMaxIters, b3_gjkCacheHits) since the last reset."""
+        out = np.zeros(6, np.uint64)
         self._get_counters(ptr(out))
         return dict(zip(self.COUNTERS, (int(x) for x in out)))
 
     def reset_counters(self):
         self._reset_counters()
+
+    def test_overlap(self, pool, shapes, pairs, xfs, caches=None):
+        """b3TestOverlap per pair (collide.cpp:84-95) with its simplex cache read and written;
+        returns the overlap flags (uint32)."""
+        pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
+        n = pairs.shape[0]
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 2, 12)
+        out = np.zeros(n, np.uint32)
+        f = getattr(self.lib, self.prefix + "test_overlap")
+        f.restype = None
+        f.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p,
+                      ctypes.c_void_p, ctypes.c_void_p]
+        if caches is not None:
+            assert caches.dtype == CACHE_DTYPE and len(caches) == n and caches.flags["C_CONTIGUOUS"]
+        ps = pool.struct()
+        f(ctypes.byref(ps), np.ascontiguousarray(shapes).ctypes.data, n, ptr(pairs), ptr(xfs),
+          None if caches is None else ptr(caches), ptr(out))
+        return out
 
     def collide(self, pool, shapes, pairs, xfs, sweeps=None, caches=None):
         pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
@@ -249,7 +268,7 @@ class RefMeshContacts:
         self._mesh = MeshStruct(len(v), v.ctypes.data, len(t), t.ctypes.data, 0 if nodes is None else len(nodes),
                                 None if nodes is None else nodes.ctypes.data)
         f = {name: getattr(lib, prefix + "mesh_contacts_" + name)
-             for name in ("create", "collide", "synchronize", "read", "destroy")}
+             for name in ("create", "collide", "synchronize", "read", "destroy", "test_overlap")}
         f["create"].restype = ctypes.c_void_p
         f["create"].argtypes = [ctypes.c_void_p] * 3 + [ctypes.c_uint32] + [ctypes.c_void_p] * 2
         f["collide"].restype = None
@@ -260,6 +279,8 @@ class RefMeshContacts:
         f["read"].argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4
         f["destroy"].restype = None
         f["destroy"].argtypes = [ctypes.c_void_p]
+        f["test_overlap"].restype = None
+        f["test_overlap"].argtypes = [ctypes.c_void_p] * 3
         self.f = f
         self._ps = pool.struct()
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
@@ -287,6 +308,13 @@ class RefMeshContacts:
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
         c0 = np.ascontiguousarray(centers0, np.float32).reshape(self.n, 3)
         self.f["synchronize"](self.handle, ptr(xfs), ptr(c0))
+
+    def test_overlap(self, xfs):
+        """b3MeshContact::TestOverlap per contact (sensor); returns uint32 flags."""
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
+        out = np.zeros(self.n, np.uint32)
+        self.f["test_overlap"](self.handle, ptr(xfs), ptr(out))
+        return out
 
     def read(self):
         counts = np.zeros(self.n, np.uint32)
