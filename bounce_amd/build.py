@@ -21,7 +21,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 SOURCES = ["narrowphase_hull.hip", "sat_persistent.hip", "serving.hip", "narrowphase_convex.hip", "narrowphase_mesh.hip", "api.cpp", "workload.cpp",
-           "hullgen.cpp", "static_tree.cpp"]
+           "static_tree.cpp", "quickhull.cpp", "hullfile.cpp"]
 
 COMMON = [
     "-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-fno-fast-math",

@@ -1,4 +1,5 @@
-// Convex hull records in the reference's b3Hull layout (hull.h:37-64), built on the host.
+// Convex hull records in the reference's b3Hull layout (hull.h:37-64), built on the host by
+// the quickhull restatement (quickhull.cpp: b3QHull::Set, qhull.cpp:104-354).
 #ifndef B3AMD_HULLGEN_H
 #define B3AMD_HULLGEN_H
 
@@ -15,9 +16,10 @@ struct HullRecord {
     std::vector<float> planes;     // [F][4] normal, offset
 };
 
-// Triangulated convex hull of n points; returns the vertex count (>= 4) or a
-// negative code for degenerate input.
-int build_hull(const float* points, int n, HullRecord* out);
+// b3QHull::Set of `count` points (simplify as its parameter); returns the vertex count, 0 when
+// the reference would leave the hull empty.  orphans (may be null): a vertex of qhHull's list
+// that no face uses, whose record the reference leaves unwritten.
+int quickhull_record(const float* points, uint32_t count, bool simplify, HullRecord* out, bool* orphans);
 
 }  // namespace b3amd
 

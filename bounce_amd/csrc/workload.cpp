@@ -91,8 +91,10 @@ int ellipsoid_hull(SplitMix& rng, int n, double a, double b, double c, HullRecor
             pts[3 * i + 1] = (float)(b * d[1]);
             pts[3 * i + 2] = (float)(c * d[2]);
         }
-        int v = build_hull(pts.data(), n, out);
-        if (v >= 4) return v;
+        // b3QHull::Set without simplification: the hull keeps every point in convex position.
+        bool orphans = false;
+        int v = quickhull_record(pts.data(), (uint32_t)n, false, out, &orphans);
+        if (v >= 4 && !orphans) return v;
     }
     return -1;
 }

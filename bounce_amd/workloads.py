@@ -1,5 +1,6 @@
 """Seeded synthetic workloads (wrappers over the generators in libbounce_amd.so)."""
 import ctypes
+import os
 
 import numpy as np
 
@@ -7,8 +8,9 @@ from .types import HullPool, ptr
 
 
 def gen_hull_pool(lib, seed, count, nmin=8, nmax=64, amin=0.25, amax=2.0):
-    """`count` quickhull-style hulls of N ~ U{nmin..nmax} points on ellipsoids with
-    semi-axes ~ U[amin, amax] (triangulated, twins adjacent)."""
+    """`count` hulls, each b3QHull::Set (no simplification, b3amd_quickhull) of N ~ U{nmin..nmax}
+    points on an ellipsoid with semi-axes ~ U[amin, amax]; twins adjacent, the reference's
+    numbering."""
     desc = np.zeros((count, 6), np.uint32)
     cen = np.zeros((count, 3), np.float32)
     verts = np.zeros((count * nmax, 3), np.float32)
@@ -26,6 +28,57 @@ def gen_hull_pool(lib, seed, count, nmin=8, nmax=64, amin=0.25, amax=2.0):
         raise RuntimeError("b3amd_gen_hull_pool failed: %d" % rc)
     nv, ne, nf = (int(x) for x in sizes)
     return HullPool(desc, cen, verts[:nv].copy(), edges[:ne].copy(), faces[:nf].copy(), planes[:nf].copy())
+
+
+def quickhull(lib, points, simplify=False):
+    """b3QHull::Set of `points` [n, 3] (b3amd_quickhull): the hull record
+    (vertices, edges, faces, planes, centroid) as HullPool.concat takes it, or None when the
+    reference would leave the hull empty."""
+    pts = np.ascontiguousarray(points, np.float32).reshape(-1, 3)
+    m = 4 * max(len(pts), 4) + 16
+    counts = np.zeros(3, np.uint32)
+    V, E = np.zeros((m, 3), np.float32), np.zeros((6 * m, 4), np.uint32)
+    F, P, C = np.zeros(2 * m, np.uint32), np.zeros((2 * m, 4), np.float32), np.zeros(3, np.float32)
+    f = lib.b3amd_quickhull
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int] + [ctypes.c_void_p] * 6
+    if f(ptr(pts), len(pts), int(bool(simplify)), ptr(counts), ptr(V), ptr(E), ptr(F), ptr(P), ptr(C)) != 0:
+        raise RuntimeError("b3amd_quickhull failed")
+    nv, ne, nf = (int(x) for x in counts)
+    if nv == 0:
+        return None
+    return V[:nv].copy(), E[:ne].copy(), F[:nf].copy(), P[:nf].copy(), C.copy()
+
+
+def save_hull_pool(lib, pool, path):
+    """Writes `pool` as a binary hull file (b3amd_hull_file_write)."""
+    f = lib.b3amd_hull_file_write
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char_p, ctypes.c_void_p]
+    ps = pool.struct()
+    if f(os.fsencode(path), ctypes.byref(ps)) != 0:
+        raise RuntimeError("b3amd_hull_file_write(%s) failed" % path)
+
+
+def load_hull_pool(lib, path):
+    """Reads a binary hull file (b3amd_hull_file_info / b3amd_hull_file_read) into a HullPool."""
+    count = ctypes.c_uint32()
+    sizes = np.zeros(3, np.uint64)
+    f = lib.b3amd_hull_file_info
+    f.restype = ctypes.c_int
+    f.argtypes = [ctypes.c_char_p, ctypes.c_void_p, ctypes.c_void_p]
+    if f(os.fsencode(path), ctypes.byref(count), ptr(sizes)) != 0:
+        raise RuntimeError("%s is not a hull file" % path)
+    n, (nv, ne, nf) = count.value, (int(x) for x in sizes)
+    desc, cen = np.zeros((n, 6), np.uint32), np.zeros((n, 3), np.float32)
+    verts, edges = np.zeros((nv, 3), np.float32), np.zeros((ne, 4), np.uint32)
+    faces, planes = np.zeros(nf, np.uint32), np.zeros((nf, 4), np.float32)
+    g = lib.b3amd_hull_file_read
+    g.restype = ctypes.c_int
+    g.argtypes = [ctypes.c_char_p] + [ctypes.c_void_p] * 6
+    if g(os.fsencode(path), ptr(desc), ptr(cen), ptr(verts), ptr(edges), ptr(faces), ptr(planes)) != 0:
+        raise RuntimeError("%s: corrupt hull file" % path)
+    return HullPool(desc, cen, verts, edges, faces, planes)
 
 
 def gen_hull_pairs(lib, seed, pool, n, gap_mean=0.0, gap_sigma=0.05):

@@ -403,9 +403,29 @@ int b3amd_reset_kernel_stats(b3amd_context* ctx);
  * at upload.  A whole-sphere cap is m = 0, cos 0, sin 1. */
 int b3amd_hull_edge_caps(const b3amd_hull_pool* pool, uint32_t hull, float* capsOut);
 
+/* ---- hull building and hull files (host only) ----
+ * b3QHull::Set (qhull.cpp:104-354) on `count` points (b3Vec3, 12-byte stride) with the
+ * reference's simplify flag: vertices, half-edges (origin, twin, face, next) and faces are
+ * numbered exactly as the reference numbers them (qhull.cpp:282-347), planes and the
+ * centroid (qhull.cpp:41-102) computed in its operation order.  counts = (V, HE, F), all 0
+ * when the reference leaves the hull empty (fewer than 4 points after welding within
+ * LINEAR_SLOP, or a degenerate point set).  Capacities: V <= 4 count + 16 (simplification
+ * rebuilds the hull from its dual), HE <= 6 V, F <= 2 V.  A vertex quickhull keeps but no
+ * face uses is written as the origin (the reference leaves that slot unwritten). */
+int b3amd_quickhull(const float* points, uint32_t count, int simplify, uint32_t counts[3], float* vertices,
+                    uint32_t* edges, uint32_t* faces, float* planes, float* centroid);
+/* A hull pool as one binary structure-of-arrays file (header, the six pool arrays, FNV-1a
+ * hash).  info reads the hull count and the vertex / half-edge / face totals, so the caller
+ * can size the arrays read fills; read checks the header, the hash and every hull's ranges. */
+int b3amd_hull_file_write(const char* path, const b3amd_hull_pool* pool);
+int b3amd_hull_file_info(const char* path, uint32_t* count, uint64_t sizes[3]);
+int b3amd_hull_file_read(const char* path, uint32_t* desc, float* centroids, float* vertices, uint32_t* edges,
+                         uint32_t* faces, float* planes);
+
 /* ---- workload generators (seeded, deterministic on every machine) ---- */
 
-/* `count` hulls from N ~ U{nmin..nmax} points on ellipsoids with semi-axes ~ U[amin,amax].
+/* `count` hulls, each b3QHull::Set (no simplification) of N ~ U{nmin..nmax} points on an
+ * ellipsoid with semi-axes ~ U[amin,amax] (redrawn if the hull would keep an unused vertex).
  * Output capacities: vertices >= count*nmax*3 floats, edges >= count*(6nmax-12)*4,
  * faces/planes >= count*(2nmax-4) (planes x4).  Writes totals to sizes[0..2]
  * (vertices, half-edges, faces). */

@@ -15,6 +15,7 @@
 
 #define private public
 #define protected public
+#include <bounce/collision/shapes/qhull.h>
 #include <bounce/dynamics/body.h>
 #include <bounce/dynamics/shapes/sphere_shape.h>
 #include <bounce/dynamics/shapes/capsule_shape.h>
@@ -144,6 +145,26 @@ extern "C" void b3ref_get_counters(uint64_t out[6]) {
 extern "C" void b3ref_reset_counters(void) {
     b3_convexCalls = b3_convexCacheHits = 0;
     b3_gjkCalls = b3_gjkIters = b3_gjkMaxIters = b3_gjkCacheHits = 0;
+}
+
+// b3QHull::Set (qhull.cpp:104-354) on `count` points; writes the hull's records and counts
+// (vertices, half-edges, faces), all zero when the reference leaves the hull empty.
+extern "C" void b3ref_qhull(const float* points, uint32_t count, int simplify, uint32_t counts[3], float* vertices,
+                            uint32_t* edges, uint32_t* faces, float* planes, float* centroid) {
+    b3QHull hull;
+    hull.Set(sizeof(b3Vec3), points, count, simplify != 0);
+    counts[0] = hull.vertexCount;
+    counts[1] = hull.edgeCount;
+    counts[2] = hull.faceCount;
+    if (hull.vertexCount == 0) {
+        counts[1] = counts[2] = 0;
+        return;
+    }
+    std::memcpy(vertices, hull.vertices, 12 * (size_t)hull.vertexCount);
+    std::memcpy(edges, hull.edges, 16 * (size_t)hull.edgeCount);
+    std::memcpy(faces, hull.faces, 4 * (size_t)hull.faceCount);
+    std::memcpy(planes, hull.planes, 16 * (size_t)hull.faceCount);
+    std::memcpy(centroid, &hull.centroid, 12);
 }
 
 // b3TestOverlap (collide.cpp:84-95) per pair with the pair's cache, as

@@ -5,7 +5,7 @@ Run in a container where the reference tree is present and oracle/_ref is built
 sweeps, caches) and the outputs of b3CollideShapeAndShape as computed by the reference
 library compiled from its own sources (oracle/_ref/libb3ref.so).
 
-    python tests/golden/make_golden.py [pairs] [mesh] [pyramid] [mesh_persistent]
+    python tests/golden/make_golden.py [pairs] [mesh] [pyramid] [mesh_persistent] [qhull]
 """
 import os
 import sys
@@ -195,8 +195,22 @@ def make_persistent_mesh_fixture():
                         tri_caches=np.stack([x[2] for x in states]), fat=np.stack([x[3] for x in states]))
 
 
+def make_qhull_fixture():
+    """b3QHull::Set of the reference on the clouds of util.qhull_clouds(2027, 150, 30, 20), with and without
+    simplification: per cloud and mode the SHA-256 of the defined records (qhull_defined)."""
+    from util import qhull_clouds, qhull_digest, run_qhull
+    ref = oracle_lib.reference()
+    clouds = qhull_clouds(2027, n_ellipsoid=150, n_box=30, n_slab=20)
+    digests = np.array([[np.frombuffer(bytes.fromhex(qhull_digest(run_qhull(ref.lib.b3ref_qhull, p, s,
+                                                                             returns_status=False))), np.uint8)
+                         for s in (0, 1)] for p in clouds])
+    sizes = np.array([len(p) for p in clouds], np.int64)
+    np.savez_compressed(os.path.join(HERE, "qhull_digests.npz"), points=np.concatenate(clouds), sizes=sizes,
+                        digests=digests)
+
+
 if __name__ == "__main__":
-    which = sys.argv[1:] or ["pairs", "mesh", "pyramid", "mesh_persistent"]
+    which = sys.argv[1:] or ["pairs", "mesh", "pyramid", "mesh_persistent", "qhull"]
     if "pairs" in which:
         main()
     if "mesh" in which:
@@ -205,3 +219,5 @@ if __name__ == "__main__":
         make_pyramid_fixture()
     if "mesh_persistent" in which:
         make_persistent_mesh_fixture()
+    if "qhull" in which:
+        make_qhull_fixture()

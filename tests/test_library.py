@@ -52,14 +52,22 @@ def test_generated_hulls_are_valid_b3hulls(lib):
     for h in range(pool.count):
         v, e, f, p, c = pool.hull(h)
         V, HE, F = len(v), len(e), len(f)
-        assert 4 <= V <= 64 and F == 2 * V - 4 and HE == 6 * V - 12  # triangulated, Euler
+        # quickhull keeps every point in convex position; nearly coplanar triangles merge
+        assert 4 <= V <= 64 and V - HE // 2 + F == 2 and F <= 2 * V - 4  # Euler
         assert np.array_equal(e[:, 1], np.arange(HE) ^ 1)             # twins adjacent (sat.cpp:123-128)
         assert np.array_equal(e[e[:, 1], 1], np.arange(HE))
-        # every face loop closes in 3 steps and the face ids agree
+        # every face loop closes, its half-edges name the face, every half-edge is on one loop
+        seen = np.zeros(HE, bool)
         for fi in range(F):
-            h0 = f[fi]
-            loop = [h0, e[h0, 3], e[e[h0, 3], 3]]
-            assert e[loop[2], 3] == h0 and all(e[x, 2] == fi for x in loop)
+            h0, h, k = f[fi], f[fi], 0
+            while True:
+                assert e[h, 2] == fi and not seen[h]
+                seen[h] = True
+                h, k = e[h, 3], k + 1
+                if h == h0:
+                    break
+            assert k >= 3
+        assert seen.all()
         # unit planes, all vertices behind every plane, centroid strictly inside
         n = p[:, :3].astype(np.float64)
         np.testing.assert_allclose(np.linalg.norm(n, axis=1), 1.0, atol=1e-6)

@@ -567,3 +567,67 @@ class WarmWorldDevice:
         out = self.d_out.download(np.zeros(self.n, MANIFOLD_DTYPE))
         caches = self.d_caches.download(np.zeros(self.n, CACHE_DTYPE))
         return out, caches
+
+
+def qhull_clouds(seed, n_ellipsoid=300, n_box=60, n_slab=30):
+    """Point clouds for the hull builder: points on ellipsoids (4-64 points, semi-axes
+    U[0.25, 2], the headline workload's), uniform in boxes (interior points), thin slabs
+    (nearly coplanar), and fixed special sets: cube corners, a coplanar quad, points closer
+    than LINEAR_SLOP (welded), a duplicated cloud."""
+    rng = np.random.default_rng(seed)
+    out = []
+    for _ in range(n_ellipsoid):
+        n = int(rng.integers(4, 65))
+        d = rng.normal(size=(n, 3))
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        out.append(d * rng.uniform(0.25, 2.0, 3))
+    for _ in range(n_box):
+        out.append(rng.uniform(-1, 1, (int(rng.integers(4, 80)), 3)))
+    for _ in range(n_slab):
+        p = rng.uniform(-1, 1, (30, 3))
+        p[:, 2] *= 1e-3
+        out.append(p)
+    out.append(np.array([[x, y, z] for x in (-1, 1) for y in (-1, 1) for z in (-1, 1)], float))
+    out.append(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [1, 1, 0]], float))
+    out.append(np.array([[0, 0, 0], [0, 0, 0.001], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float))
+    out.append(np.concatenate([out[0], out[0]]))
+    return [np.asarray(p, np.float32) for p in out]
+
+
+def run_qhull(fn, points, simplify, returns_status=True):
+    """(counts [3], vertices, edges, faces, planes, centroid) of b3QHull::Set on `points`
+    through `fn` (the library's b3amd_quickhull or the reference's b3ref_qhull)."""
+    import ctypes
+    pts = np.ascontiguousarray(points, np.float32)
+    m = 4 * max(len(pts), 4) + 16  # simplification can rebuild more vertices than it got
+    counts = np.zeros(3, np.uint32)
+    V, E = np.zeros((m, 3), np.float32), np.zeros((6 * m, 4), np.uint32)
+    F, P, C = np.zeros(2 * m, np.uint32), np.zeros((2 * m, 4), np.float32), np.zeros(3, np.float32)
+    fn.restype = ctypes.c_int if returns_status else None
+    fn.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_int] + [ctypes.c_void_p] * 6
+    rc = fn(pts.ctypes.data, len(pts), int(simplify), counts.ctypes.data, V.ctypes.data, E.ctypes.data,
+            F.ctypes.data, P.ctypes.data, C.ctypes.data)
+    assert not returns_status or rc == 0
+    v, he, f = (int(x) for x in counts)
+    return counts, V[:v].copy(), E[:he].copy(), F[:f].copy(), P[:f].copy(), C
+
+
+def qhull_defined(rec):
+    """The records of a b3QHull::Set result that the reference defines: a vertex that no face
+    uses keeps an unwritten slot (qhull.cpp:317-320), zeroed here, and the centroid, which
+    averages it in (qhull.cpp:54-60), is dropped when one exists.  Returns (records, orphans)."""
+    counts, V, E, F, P, C = rec
+    used = np.zeros(len(V), bool)
+    used[E[:, 0]] = True
+    V = V.copy()
+    V[~used] = 0.0
+    orphans = int((~used).sum())
+    return (counts, V, E, F, P, C if orphans == 0 else np.zeros(3, np.float32)), orphans
+
+
+def qhull_digest(rec):
+    import hashlib
+    h = hashlib.sha256()
+    for a in qhull_defined(rec)[0]:
+        h.update(np.ascontiguousarray(a).tobytes())
+    return h.hexdigest()
