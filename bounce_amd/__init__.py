@@ -413,6 +413,35 @@ class Context:
     def copy_on_device(self, dst, src, nbytes):
         _check(self.lib.b3amd_copy_on_device(self.handle, dst.ptr, src.ptr, int(nbytes)))
 
+    def contact_init(self, M, bodies, state, material, counts, manifolds, old_counts, old_manifolds, old_impulses,
+                     warm=True):
+        """b3amd_contact_init on host arrays (uploaded, run, read back): warm-start matching
+        and b3ContactSolver::InitializeConstraints for n contacts of M manifold slots.
+        Returns (impulses [n, M], velocity constraints [n, M])."""
+        from .types import IMPULSES_DTYPE, VELOCITY_MANIFOLD_DTYPE
+        n = len(counts)
+        arrays = [np.ascontiguousarray(bodies, np.uint32), np.ascontiguousarray(state), np.ascontiguousarray(material),
+                  np.ascontiguousarray(counts, np.uint32), np.ascontiguousarray(manifolds, MANIFOLD_DTYPE),
+                  np.ascontiguousarray(old_counts, np.uint32), np.ascontiguousarray(old_manifolds, MANIFOLD_DTYPE),
+                  np.ascontiguousarray(old_impulses, IMPULSES_DTYPE)]
+        imp = np.zeros((n, M), IMPULSES_DTYPE)
+        out = np.zeros((n, M), VELOCITY_MANIFOLD_DTYPE)
+        bufs = [self.alloc(max(16, a.nbytes)) for a in arrays] + [self.alloc(imp.nbytes), self.alloc(out.nbytes)]
+        try:
+            for b, a in zip(bufs, arrays):
+                b.upload(a)
+            bufs[-2].upload(imp)
+            bufs[-1].upload(out)
+            f = self.lib.b3amd_contact_init
+            f.argtypes = [ctypes.c_void_p, ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 8 + \
+                [ctypes.c_int] + [ctypes.c_void_p] * 2
+            _check(f(self.handle, n, M, *[b.ptr for b in bufs[:8]], int(bool(warm)), bufs[8].ptr, bufs[9].ptr))
+            self.synchronize()
+            return bufs[8].download(imp), bufs[9].download(out)
+        finally:
+            for b in bufs:
+                b.free()
+
     def test_overlap(self, pairs, xfs, caches=None):
         """b3TestOverlap per pair (sensors); caches (CACHE_DTYPE, optional) are read and
         updated in place.  Returns the overlap flags (uint32: 1 overlapping, 0 apart)."""

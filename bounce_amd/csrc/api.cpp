@@ -551,6 +551,37 @@ int b3amd_collide(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, const b
     return ctx->resolve_timings();
 }
 
+int b3amd_contact_init(b3amd_context* ctx, uint32_t n, uint32_t M, const uint32_t* d_bodies,
+                       const b3amd_solver_body* d_state, const b3amd_contact_material* d_material,
+                       const uint32_t* d_counts, const b3amd_manifold* d_manifolds, const uint32_t* d_old_counts,
+                       const b3amd_manifold* d_old_manifolds, const b3amd_manifold_impulses* d_old_impulses,
+                       int warm_starting, b3amd_manifold_impulses* d_impulses, b3amd_velocity_manifold* d_constraints) {
+    static_assert(sizeof(b3amd_solver_body) == 4 * kSolverBodyFloats, "solver body layout");
+    static_assert(sizeof(b3amd_velocity_manifold) == sizeof(VelocityManifold), "velocity constraint layout");
+    static_assert(sizeof(b3amd_manifold_impulses) == sizeof(ManifoldImpulses), "impulse layout");
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    if (n == 0) return B3AMD_OK;
+    if (M == 0 || M > 3) return fail(B3AMD_ERR_ARGUMENT, "manifold slots per contact %u outside 1..3", M);
+    if (!d_bodies || !d_state || !d_material || !d_counts || !d_manifolds || !d_impulses || !d_constraints ||
+        (warm_starting && (!d_old_counts || !d_old_manifolds || !d_old_impulses)))
+        return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    ContactInitArgs a;
+    a.n = n;
+    a.M = M;
+    a.bodies = d_bodies;
+    a.state = reinterpret_cast<const float*>(d_state);
+    a.material = reinterpret_cast<const float*>(d_material);
+    a.counts = d_counts;
+    a.manifolds = reinterpret_cast<const Manifold*>(d_manifolds);
+    a.oldCounts = d_old_counts;
+    a.oldManifolds = reinterpret_cast<const Manifold*>(d_old_manifolds);
+    a.oldImpulses = reinterpret_cast<const ManifoldImpulses*>(d_old_impulses);
+    a.warmStarting = warm_starting ? 1u : 0u;
+    a.impulses = reinterpret_cast<ManifoldImpulses*>(d_impulses);
+    a.constraints = reinterpret_cast<VelocityManifold*>(d_constraints);
+    return timed(ctx, ctx->stream, "contact_init", [&] { return launch_contact_init(a, ctx->stream); });
+}
+
 int b3amd_test_overlap_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
                               b3amd_convex_cache* d_caches, uint32_t* d_overlap) {
     if (!ctx || (n && (!d_pairs || !d_xfs || !d_overlap))) return fail(B3AMD_ERR_ARGUMENT, "null argument");

@@ -41,6 +41,50 @@ static_assert(sizeof(Manifold) == 240, "manifold layout");
 
 constexpr uint32_t kFlagPolyOverflow = 1u;
 
+// The step after the narrowphase (b3amd_contact_init): the solver-owned fields of b3Manifold
+// (manifold.h:66-93) and b3VelocityConstraintPoint / b3VelocityConstraintManifold
+// (contact_solver.h:58-84) with the contact's mixed friction and restitution.
+struct ManifoldImpulses {
+    float normalImpulse[4];
+    uint32_t persisting[4];
+    float tangentImpulse[2];
+    float motorImpulse;
+    uint32_t reserved;
+};
+static_assert(sizeof(ManifoldImpulses) == 48, "impulse record layout");
+struct VelocityPoint {
+    float rA[3], rB[3], normal[3];
+    float normalMass, normalImpulse, velocityBias;
+};
+struct VelocityManifold {
+    VelocityPoint points[4];
+    float rA[3], rB[3], normal[3], tangent1[3], tangent2[3];
+    float tangentMass[4];
+    float tangentImpulse[2];
+    float motorImpulse, motorMass;
+    uint32_t pointCount;
+    float friction, restitution;
+    uint32_t reserved[2];
+};
+static_assert(sizeof(VelocityManifold) == 304, "velocity constraint layout");
+// A solver body: x (3), q (4), v (3), w (3), invMass, invI (9, columns), localCenter (3).
+constexpr uint32_t kSolverBodyFloats = 26;
+struct ContactInitArgs {
+    uint32_t n, M;                     // contacts, manifold slots per contact
+    const uint32_t* bodies;            // [n][2]
+    const float* state;                // [B][kSolverBodyFloats]
+    const float* material;             // [n][8] radiusA, radiusB, frictionA/B, restitutionA/B, pad
+    const uint32_t* counts;            // [n] new manifolds
+    const Manifold* manifolds;         // [n][M]
+    const uint32_t* oldCounts;         // [n]
+    const Manifold* oldManifolds;      // [n][M]
+    const ManifoldImpulses* oldImpulses;  // [n][M]
+    uint32_t warmStarting;
+    ManifoldImpulses* impulses;        // [n][M] out
+    VelocityManifold* constraints;     // [n][M] out
+};
+hipError_t launch_contact_init(const ContactInitArgs& a, hipStream_t s);
+
 // b3SATFeaturePair (sat.h:74-80) inside b3ConvexCache (collide.h:38-42).
 struct FeatureCache {
     uint32_t state, type, index1, index2;

@@ -38,6 +38,24 @@ CACHE_DTYPE = np.dtype({
     "itemsize": 36,
 })
 
+# The step after the narrowphase (b3ContactSolver inputs and outputs, contact_solver.h).
+SOLVER_BODY_DTYPE = np.dtype([("x", "<f4", (3,)), ("q", "<f4", (4,)), ("v", "<f4", (3,)), ("w", "<f4", (3,)),
+                              ("invMass", "<f4"), ("invI", "<f4", (9,)), ("localCenter", "<f4", (3,))])
+assert SOLVER_BODY_DTYPE.itemsize == 104
+MATERIAL_DTYPE = np.dtype([("radiusA", "<f4"), ("radiusB", "<f4"), ("frictionA", "<f4"), ("frictionB", "<f4"),
+                           ("restitutionA", "<f4"), ("restitutionB", "<f4"), ("reserved", "<u4", (2,))])
+IMPULSES_DTYPE = np.dtype([("normalImpulse", "<f4", (4,)), ("persisting", "<u4", (4,)),
+                           ("tangentImpulse", "<f4", (2,)), ("motorImpulse", "<f4"), ("reserved", "<u4")])
+assert IMPULSES_DTYPE.itemsize == 48
+VELOCITY_POINT_DTYPE = np.dtype([("rA", "<f4", (3,)), ("rB", "<f4", (3,)), ("normal", "<f4", (3,)),
+                                 ("normalMass", "<f4"), ("normalImpulse", "<f4"), ("velocityBias", "<f4")])
+VELOCITY_MANIFOLD_DTYPE = np.dtype([
+    ("points", VELOCITY_POINT_DTYPE, (4,)), ("rA", "<f4", (3,)), ("rB", "<f4", (3,)), ("normal", "<f4", (3,)),
+    ("tangent1", "<f4", (3,)), ("tangent2", "<f4", (3,)), ("tangentMass", "<f4", (4,)), ("tangentImpulse", "<f4", (2,)),
+    ("motorImpulse", "<f4"), ("motorMass", "<f4"), ("pointCount", "<u4"), ("friction", "<f4"), ("restitution", "<f4"),
+    ("reserved", "<u4", (2,))])
+assert VELOCITY_MANIFOLD_DTYPE.itemsize == 304
+
 SPHERE, CAPSULE, HULL, MESH = 0, 1, 2, 3
 # Manifold status bits (B3AMD_FLAG_* of bounce_amd.h).
 FLAG_POLY_OVERFLOW, FLAG_INVALID_PAIR, FLAG_TRIANGLE_OVERFLOW = 1, 2, 4

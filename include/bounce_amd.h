@@ -142,6 +142,57 @@ int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs
                          const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
                          b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
 
+/* ---- the step after the narrowphase ----
+ * Warm-start matching and contact-constraint setup on device-resident manifolds, so that a
+ * world's contacts go from the narrowphase to the solver without a host round trip.  For each
+ * contact i of n and manifold slot j < counts[i] (of M slots, M = 1 for convex contacts,
+ * up to 3 for mesh contacts): the new manifold's impulses are b3Manifold::Initialize() and,
+ * when warm_starting, Initialize(old) of each of the contact's oldCounts[i] old manifolds in
+ * turn (contact.cpp:63-91, manifold.cpp:21-55: a point whose key an old manifold holds takes
+ * that point's normal impulse and is marked persisting; tangent and motor impulses come from
+ * the last old manifold); then b3ContactSolver::InitializeConstraints for that manifold
+ * (contact_solver.cpp:70-276): the world manifold (manifold.cpp:57-95), per point the
+ * anchors, normal effective mass and restitution bias, and the friction anchors, 2x2 tangent
+ * mass and twist mass.  Slots j >= counts[i] are not written.  Contact i joins bodies
+ * bodies[2i], bodies[2i+1], indices into the solver-body array (the island's b3Position and
+ * b3Velocity with the body's mass data). */
+typedef struct {
+    b3amd_vec3 x;             /* b3Position::x (sweep world centre) */
+    b3amd_quat q;             /* b3Position::q */
+    b3amd_vec3 v, w;          /* b3Velocity */
+    float invMass;            /* b3Body::m_invMass */
+    float invI[9];            /* b3Body::m_worldInvI, columns */
+    b3amd_vec3 localCenter;   /* b3Body::m_sweep.localCenter */
+} b3amd_solver_body;
+typedef struct { float radiusA, radiusB, frictionA, frictionB, restitutionA, restitutionB, reserved[2]; } b3amd_contact_material;
+/* The solver-owned fields of b3Manifold (manifold.h:66-93). */
+typedef struct {
+    float normalImpulse[4];
+    uint32_t persisting[4];
+    float tangentImpulse[2];
+    float motorImpulse;
+    uint32_t reserved;
+} b3amd_manifold_impulses;
+/* b3VelocityConstraintPoint / b3VelocityConstraintManifold (contact_solver.h:58-84) with the
+ * contact's b3MixFriction / b3MixRestitution. */
+typedef struct { b3amd_vec3 rA, rB, normal; float normalMass, normalImpulse, velocityBias; } b3amd_velocity_point;
+typedef struct {
+    b3amd_velocity_point points[4];
+    b3amd_vec3 rA, rB, normal, tangent1, tangent2;
+    float tangentMass[4];     /* b3Mat22 columns */
+    float tangentImpulse[2];
+    float motorImpulse, motorMass;
+    uint32_t pointCount;
+    float friction, restitution;
+    uint32_t reserved[2];
+} b3amd_velocity_manifold;
+int b3amd_contact_init(b3amd_context* ctx, uint32_t n, uint32_t M, const uint32_t* d_bodies,
+                       const b3amd_solver_body* d_state, const b3amd_contact_material* d_material,
+                       const uint32_t* d_counts, const b3amd_manifold* d_manifolds, const uint32_t* d_old_counts,
+                       const b3amd_manifold* d_old_manifolds, const b3amd_manifold_impulses* d_old_impulses,
+                       int warm_starting, b3amd_manifold_impulses* d_impulses,
+                       b3amd_velocity_manifold* d_constraints);
+
 /* ---- sensors ----
  * For each pair i: b3TestOverlap(xfs[2i], 0, shapes[pairs[2i]], xfs[2i+1], 0,
  * shapes[pairs[2i+1]], &caches[i]) (collide.cpp:84-95), the test b3Contact::Update runs

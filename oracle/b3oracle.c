@@ -2264,3 +2264,159 @@ void b3o_hull_queries(const b3o_hull_pool* pool, uint32_t n, const uint32_t* hul
         o[7] = 0;
     }
 }
+
+/* ---------------------------------------------------------------- the step after the narrowphase */
+static int same_key(const b3o_mpoint* a, const b3o_mpoint* b) {  /* b3ManifoldPointKey::operator== (manifold.h:30-34) */
+    return a->triangleKey == b->triangleKey && a->key[0] == b->key[0] && a->key[1] == b->key[1] &&
+           a->key[2] == b->key[2] && a->key[3] == b->key[3];
+}
+
+static M33 quat_mat33(b3o_quat q) {  /* b3QuatMat33 (quat.h:306-318) */
+    float x = q.x, y = q.y, z = q.z, w = q.w;
+    float x2 = x + x, y2 = y + y, z2 = z + z;
+    float xx = x * x2, xy = x * y2, xz = x * z2;
+    float yy = y * y2, yz = y * z2, zz = z * z2;
+    float wx = w * x2, wy = w * y2, wz = w * z2;
+    M33 m;
+    m.c[0] = v3(1.0f - (yy + zz), xy + wz, xz - wy);
+    m.c[1] = v3(xy - wz, 1.0f - (xx + zz), yz + wx);
+    m.c[2] = v3(xz + wy, yz - wx, 1.0f - (xx + yy));
+    return m;
+}
+
+static M33 m33_of(const float a[9]) {
+    M33 m;
+    m.c[0] = v3(a[0], a[1], a[2]);
+    m.c[1] = v3(a[3], a[4], a[5]);
+    m.c[2] = v3(a[6], a[7], a[8]);
+    return m;
+}
+
+static V3 v3o(b3o_vec3 a) { return v3(a.x, a.y, a.z); }
+static b3o_vec3 o3(V3 a) { b3o_vec3 r = {a.x, a.y, a.z}; return r; }
+
+void b3o_contact_init(uint32_t n, uint32_t M, const uint32_t* bodies, const b3o_solver_body* state,
+                      const b3o_contact_material* material, const uint32_t* counts, const b3o_manifold* manifolds,
+                      const uint32_t* oldCounts, const b3o_manifold* oldManifolds,
+                      const b3o_manifold_impulses* oldImpulses, int warmStarting, b3o_manifold_impulses* impulses,
+                      b3o_velocity_manifold* out) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t count = counts[i];
+        /* b3Manifold::Initialize() of the new manifolds, then Initialize(old) of each old
+         * manifold in turn: tangent and motor impulses end up from the last one (T21). */
+        for (uint32_t j = 0; j < count; ++j) {
+            b3o_manifold_impulses* imp = impulses + (size_t)i * M + j;
+            memset(imp, 0, sizeof(*imp));
+            if (!warmStarting) continue;
+            const b3o_manifold* m2 = manifolds + (size_t)i * M + j;
+            for (uint32_t k = 0; k < oldCounts[i]; ++k) {
+                const b3o_manifold* m1 = oldManifolds + (size_t)i * M + k;
+                const b3o_manifold_impulses* i1 = oldImpulses + (size_t)i * M + k;
+                imp->tangentImpulse[0] = i1->tangentImpulse[0];
+                imp->tangentImpulse[1] = i1->tangentImpulse[1];
+                imp->motorImpulse = i1->motorImpulse;
+                for (uint32_t a = 0; a < m2->pointCount; ++a)
+                    for (uint32_t b = 0; b < m1->pointCount; ++b)
+                        if (same_key(m2->points + a, m1->points + b)) {
+                            imp->normalImpulse[a] = i1->normalImpulse[b];
+                            imp->persisting[a] = 1;
+                            break;
+                        }
+            }
+        }
+        const b3o_solver_body* A = state + bodies[2 * i];
+        const b3o_solver_body* B = state + bodies[2 * i + 1];
+        const b3o_contact_material* mt = material + i;
+        const float mA = A->invMass, mB = B->invMass;
+        const M33 iA = m33_of(A->invI), iB = m33_of(B->invI);
+        const V3 xA = v3o(A->x), xB = v3o(B->x);
+        const V3 vA = v3o(A->v), wA = v3o(A->w), vB = v3o(B->v), wB = v3o(B->w);
+        XF xfA, xfB;
+        M33 rAm = quat_mat33(A->q), rBm = quat_mat33(B->q);
+        xfA.c[0] = rAm.c[0], xfA.c[1] = rAm.c[1], xfA.c[2] = rAm.c[2];
+        xfA.p = sub(xA, mul_mv(&rAm, v3o(A->localCenter)));
+        xfB.c[0] = rBm.c[0], xfB.c[1] = rBm.c[1], xfB.c[2] = rBm.c[2];
+        xfB.p = sub(xB, mul_mv(&rBm, v3o(B->localCenter)));
+        const float friction = sqrtf(mt->frictionA * mt->frictionB);                 /* b3MixFriction (contact_solver.h:93-96) */
+        const float restitution = mt->restitutionA > mt->restitutionB ? mt->restitutionA : mt->restitutionB;  /* b3MixRestitution */
+        for (uint32_t j = 0; j < count; ++j) {
+            const b3o_manifold* m = manifolds + (size_t)i * M + j;
+            const b3o_manifold_impulses* imp = impulses + (size_t)i * M + j;
+            b3o_velocity_manifold* vcm = out + (size_t)i * M + j;
+            memset(vcm, 0, sizeof(*vcm));
+            vcm->pointCount = m->pointCount;
+            vcm->tangentImpulse[0] = imp->tangentImpulse[0];
+            vcm->tangentImpulse[1] = imp->tangentImpulse[1];
+            vcm->motorImpulse = imp->motorImpulse;
+            vcm->friction = friction;
+            vcm->restitution = restitution;
+            /* b3WorldManifold::Initialize (manifold.cpp:57-95) */
+            V3 center = v3(0.0f, 0.0f, 0.0f), normal = v3(0.0f, 0.0f, 0.0f);
+            V3 wpoint[4], wnormal[4];
+            for (uint32_t k = 0; k < m->pointCount; ++k) {
+                const b3o_mpoint* p = m->points + k;
+                V3 nA = mul_mv(rot(&xfA), v3o(p->localNormal1));
+                V3 cA = xf_mul(&xfA, v3o(p->localPoint1));
+                V3 cB = xf_mul(&xfB, v3o(p->localPoint2));
+                V3 pA = add(cA, scale(mt->radiusA, nA));
+                V3 pB = sub(cB, scale(mt->radiusB, nA));
+                wpoint[k] = scale(0.5f, add(pA, pB));
+                wnormal[k] = nA;
+                center = add(center, wpoint[k]);
+                normal = add(normal, wnormal[k]);
+            }
+            if (m->pointCount == 0) continue;
+            {
+                float s = 1.0f / (float)m->pointCount;
+                center = v3(center.x * s, center.y * s, center.z * s);
+            }
+            {
+                float len = sqrtf(normal.x * normal.x + normal.y * normal.y + normal.z * normal.z);
+                if (len > EPSILON) {
+                    float s = 1.0f / len;
+                    normal = v3(normal.x * s, normal.y * s, normal.z * s);
+                }
+            }
+            V3 t1 = fabsf(normal.x) >= 0.57735027f ? v3(normal.y, -normal.x, 0.0f)
+                                                  : v3(0.0f, normal.z, -normal.y);  /* b3Perp (vec3.h:255-267) */
+            V3 t2 = cross(t1, normal);
+            vcm->normal = o3(normal);
+            vcm->tangent1 = o3(t1);
+            vcm->tangent2 = o3(t2);
+            for (uint32_t k = 0; k < m->pointCount; ++k) {
+                b3o_velocity_point* vcp = vcm->points + k;
+                V3 nrm = wnormal[k];
+                V3 rA = sub(wpoint[k], xA), rB = sub(wpoint[k], xB);
+                vcp->rA = o3(rA);
+                vcp->rB = o3(rB);
+                vcp->normal = o3(nrm);
+                vcp->normalImpulse = imp->normalImpulse[k];
+                V3 rnA = cross(rA, nrm), rnB = cross(rB, nrm);
+                float K = mA + mB + dot(mul_mv(&iA, rnA), rnA) + dot(mul_mv(&iB, rnB), rnB);
+                vcp->normalMass = K > 0.0f ? 1.0f / K : 0.0f;
+                V3 dv = sub(sub(add(vB, cross(wB, rB)), vA), cross(wA, rA));
+                float vn = dot(nrm, dv);
+                vcp->velocityBias = 0.0f;
+                if (vn < -1.0f) vcp->velocityBias = -restitution * vn;  /* B3_VELOCITY_THRESHOLD (settings.h:105) */
+            }
+            V3 rA = sub(center, xA), rB = sub(center, xB);
+            vcm->rA = o3(rA);
+            vcm->rB = o3(rB);
+            V3 rn1A = cross(rA, t1), rn1B = cross(rB, t1), rn2A = cross(rA, t2), rn2B = cross(rB, t2);
+            float k11 = mA + mB + dot(mul_mv(&iA, rn1A), rn1A) + dot(mul_mv(&iB, rn1B), rn1B);
+            float k12 = dot(mul_mv(&iA, rn1A), rn2A) + dot(mul_mv(&iB, rn1B), rn2B);
+            float k22 = mA + mB + dot(mul_mv(&iA, rn2A), rn2A) + dot(mul_mv(&iB, rn2B), rn2B);
+            /* b3Inverse(b3Mat22) (mat22.cpp:77-93) of K = [x: (k11, k12), y: (k12, k22)] */
+            float det = k11 * k22 - k12 * k12;
+            if (det != 0.0f) det = 1.0f / det;
+            vcm->tangentMass[0] = det * k22;
+            vcm->tangentMass[1] = -det * k12;
+            vcm->tangentMass[2] = -det * k12;
+            vcm->tangentMass[3] = det * k11;
+            M33 iAB;
+            iAB.c[0] = add(iA.c[0], iB.c[0]), iAB.c[1] = add(iA.c[1], iB.c[1]), iAB.c[2] = add(iA.c[2], iB.c[2]);
+            float mass = dot(normal, mul_mv(&iAB, normal));
+            vcm->motorMass = mass > 0.0f ? 1.0f / mass : 0.0f;
+        }
+    }
+}

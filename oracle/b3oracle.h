@@ -85,6 +85,59 @@ typedef struct { b3o_simplex_cache simplex; b3o_feature_cache feature; } b3o_con
 enum { B3O_SEPARATION = 0, B3O_OVERLAP = 1, B3O_EMPTY = 2 };
 enum { B3O_EDGE1 = 0, B3O_FACE1 = 1, B3O_FACE2 = 2 };
 
+/* ---- the step after the narrowphase: warm-start matching and contact constraints ---- */
+/* A body as b3ContactSolver sees it: island position (sweep world centre, orientation) and
+ * velocity (b3Position / b3Velocity, time_step.h), b3Body::m_invMass, the world inverse
+ * inertia b3Body::m_worldInvI and m_sweep.localCenter. */
+typedef struct {
+    b3o_vec3 x;
+    b3o_quat q;
+    b3o_vec3 v, w;
+    float invMass;
+    float invI[9];            /* b3Mat33 columns */
+    b3o_vec3 localCenter;
+} b3o_solver_body;
+/* The two shapes' m_radius, m_friction and m_restitution (mixed by b3MixFriction /
+ * b3MixRestitution, contact_solver.h:88-96). */
+typedef struct { float radiusA, radiusB, frictionA, frictionB, restitutionA, restitutionB, reserved[2]; } b3o_contact_material;
+/* The solver-owned fields of b3Manifold (manifold.h:66-93). */
+typedef struct {
+    float normalImpulse[4];
+    uint32_t persisting[4];
+    float tangentImpulse[2];
+    float motorImpulse;
+    uint32_t reserved;
+} b3o_manifold_impulses;
+/* b3VelocityConstraintPoint / b3VelocityConstraintManifold (contact_solver.h:58-84) with the
+ * contact's mixed friction and restitution. */
+typedef struct { b3o_vec3 rA, rB, normal; float normalMass, normalImpulse, velocityBias; } b3o_velocity_point;
+typedef struct {
+    b3o_velocity_point points[4];
+    b3o_vec3 rA, rB, normal, tangent1, tangent2;
+    float tangentMass[4];     /* b3Mat22 columns x, y */
+    float tangentImpulse[2];
+    float motorImpulse, motorMass;
+    uint32_t pointCount;
+    float friction, restitution;
+    uint32_t reserved[2];
+} b3o_velocity_manifold;
+
+/* For n contacts of M manifold slots each: the new manifolds' impulses from
+ * b3Manifold::Initialize() and, with warm starting, Initialize(old) of every old manifold in
+ * turn (contact.cpp:63-91, manifold.cpp:21-55), then b3ContactSolver::InitializeConstraints
+ * (contact_solver.cpp:70-276) for each new manifold.  Contact i joins bodies bodies[2i],
+ * bodies[2i+1] (indices into state); counts / oldCounts give its manifold counts. */
+void b3o_contact_init(uint32_t n, uint32_t M, const uint32_t* bodies, const b3o_solver_body* state,
+                      const b3o_contact_material* material, const uint32_t* counts, const b3o_manifold* manifolds,
+                      const uint32_t* oldCounts, const b3o_manifold* oldManifolds,
+                      const b3o_manifold_impulses* oldImpulses, int warmStarting, b3o_manifold_impulses* impulses,
+                      b3o_velocity_manifold* out);
+void b3ref_contact_init(uint32_t n, uint32_t M, const uint32_t* bodies, const b3o_solver_body* state,
+                        const b3o_contact_material* material, const uint32_t* counts, const b3o_manifold* manifolds,
+                        const uint32_t* oldCounts, const b3o_manifold* oldManifolds,
+                        const b3o_manifold_impulses* oldImpulses, int warmStarting, b3o_manifold_impulses* impulses,
+                        b3o_velocity_manifold* out);
+
 /* ---- plain-C restatement (liboracle.so) ---- */
 
 /* Collide n shape pairs as b3CollideShapeAndShape (collide.cpp:162-188) with the

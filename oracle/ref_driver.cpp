@@ -30,6 +30,8 @@
 #include <bounce/collision/shapes/box_hull.h>
 #include <bounce/dynamics/world.h>
 #include <bounce/common/draw.h>
+#include <bounce/common/memory/stack_allocator.h>
+#include <bounce/dynamics/contacts/contact_solver.h>
 #undef private
 #undef protected
 
@@ -145,6 +147,154 @@ extern "C" void b3ref_get_counters(uint64_t out[6]) {
 extern "C" void b3ref_reset_counters(void) {
     b3_convexCalls = b3_convexCacheHits = 0;
     b3_gjkCalls = b3_gjkIters = b3_gjkMaxIters = b3_gjkCacheHits = 0;
+}
+
+// The step after the narrowphase for n contacts of M manifold slots: b3Manifold::Initialize()
+// and, with warm starting, Initialize(old) of each old manifold in turn, in
+// b3Contact::Update's loop (contact.cpp:63-91); then a b3ContactSolver over those contacts runs
+// InitializeConstraints (contact_solver.cpp:70-276) and its velocity constraints are copied out.
+// Each contact is a b3ConvexContact between two hull shapes whose bodies carry the island
+// index, mass data and local centre the solver reads; its m_manifolds points at its M slots.
+extern "C" void b3ref_contact_init(uint32_t n, uint32_t M, const uint32_t* bodies, const b3o_solver_body* state,
+                                   const b3o_contact_material* material, const uint32_t* counts,
+                                   const b3o_manifold* manifolds, const uint32_t* oldCounts,
+                                   const b3o_manifold* oldManifolds, const b3o_manifold_impulses* oldImpulses,
+                                   int warmStarting, b3o_manifold_impulses* impulses, b3o_velocity_manifold* out) {
+    uint32_t nb = 0;
+    for (uint32_t i = 0; i < 2 * n; ++i) nb = bodies[i] + 1 > nb ? bodies[i] + 1 : nb;
+    std::vector<BodyImage> images(nb);
+    std::vector<b3Position> positions(nb);
+    std::vector<b3Velocity> velocities(nb);
+    std::vector<b3Mat33> invInertias(nb);
+    for (uint32_t b = 0; b < nb; ++b) {
+        std::memset(images[b].bytes, 0, sizeof(images[b].bytes));
+        b3Body* body = images[b].body();
+        const b3o_solver_body& s = state[b];
+        body->m_islandID = b;
+        body->m_invMass = s.invMass;
+        std::memcpy(&body->m_sweep.localCenter, &s.localCenter, 12);
+        std::memcpy(&positions[b].x, &s.x, 12);
+        std::memcpy(&positions[b].q, &s.q, 16);
+        std::memcpy(&velocities[b].v, &s.v, 12);
+        std::memcpy(&velocities[b].w, &s.w, 12);
+        std::memcpy(&invInertias[b], s.invI, 36);
+    }
+    std::vector<b3HullShape> shapes(2 * (size_t)n);
+    std::vector<b3Manifold> slots((size_t)n * M);
+    std::vector<b3ConvexContact*> contacts(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3o_contact_material& mt = material[i];
+        b3HullShape& A = shapes[2 * i];
+        b3HullShape& B = shapes[2 * i + 1];
+        A.m_radius = mt.radiusA, A.m_friction = mt.frictionA, A.m_restitution = mt.restitutionA;
+        B.m_radius = mt.radiusB, B.m_friction = mt.frictionB, B.m_restitution = mt.restitutionB;
+        A.m_body = images[bodies[2 * i]].body();
+        B.m_body = images[bodies[2 * i + 1]].body();
+        b3ConvexContact* c = new b3ConvexContact(&A, &B);
+        c->m_pair.shapeA = &A;
+        c->m_pair.shapeB = &B;
+        c->m_manifolds = slots.data() + (size_t)i * M;
+        c->m_manifoldCapacity = M;
+        // The old manifolds with their impulses, then the new ones over them (contact.cpp:65-78).
+        b3Manifold old[3];
+        for (uint32_t k = 0; k < oldCounts[i]; ++k) {
+            b3Manifold& m = old[k];
+            const b3o_manifold& src = oldManifolds[(size_t)i * M + k];
+            const b3o_manifold_impulses& imp = oldImpulses[(size_t)i * M + k];
+            m.Initialize();
+            m.pointCount = src.pointCount;
+            for (uint32_t q = 0; q < src.pointCount; ++q) {
+                std::memcpy(&m.points[q].key.triangleKey, &src.points[q].triangleKey, 4);
+                std::memcpy(&m.points[q].key.key1, src.points[q].key, 8);
+                std::memcpy(&m.points[q].key.key2, src.points[q].key + 2, 8);
+                m.points[q].normalImpulse = imp.normalImpulse[q];
+            }
+            m.tangentImpulse.Set(imp.tangentImpulse[0], imp.tangentImpulse[1]);
+            m.motorImpulse = imp.motorImpulse;
+        }
+        for (uint32_t k = 0; k < M; ++k) c->m_manifolds[k].Initialize();
+        c->m_manifoldCount = counts[i];
+        for (uint32_t j = 0; j < counts[i]; ++j) {
+            b3Manifold& m = c->m_manifolds[j];
+            const b3o_manifold& src = manifolds[(size_t)i * M + j];
+            m.pointCount = src.pointCount;
+            for (uint32_t q = 0; q < src.pointCount; ++q) {
+                std::memcpy(&m.points[q].localNormal1, &src.points[q].localNormal1, 12);
+                std::memcpy(&m.points[q].localPoint1, &src.points[q].localPoint1, 12);
+                std::memcpy(&m.points[q].localPoint2, &src.points[q].localPoint2, 12);
+                std::memcpy(&m.points[q].key.triangleKey, &src.points[q].triangleKey, 4);
+                std::memcpy(&m.points[q].key.key1, src.points[q].key, 8);
+                std::memcpy(&m.points[q].key.key2, src.points[q].key + 2, 8);
+            }
+        }
+        if (warmStarting)
+            for (uint32_t j = 0; j < counts[i]; ++j)
+                for (uint32_t k = 0; k < oldCounts[i]; ++k) c->m_manifolds[j].Initialize(old[k]);
+        for (uint32_t j = 0; j < counts[i]; ++j) {
+            const b3Manifold& m = c->m_manifolds[j];
+            b3o_manifold_impulses& imp = impulses[(size_t)i * M + j];
+            std::memset(&imp, 0, sizeof(imp));
+            for (uint32_t q = 0; q < m.pointCount; ++q) {
+                imp.normalImpulse[q] = m.points[q].normalImpulse;
+                imp.persisting[q] = m.points[q].persisting;
+            }
+            imp.tangentImpulse[0] = m.tangentImpulse.x;
+            imp.tangentImpulse[1] = m.tangentImpulse.y;
+            imp.motorImpulse = m.motorImpulse;
+        }
+        contacts[i] = c;
+    }
+    b3StackAllocator* allocator = new b3StackAllocator();
+    {
+        b3ContactSolverDef def;
+        def.positions = positions.data();
+        def.velocities = velocities.data();
+        def.invInertias = invInertias.data();
+        def.contacts = reinterpret_cast<b3Contact**>(contacts.data());
+        def.count = n;
+        def.allocator = allocator;
+        def.dt = 1.0f / 60.0f;
+        b3ContactSolver solver(&def);
+        solver.InitializeConstraints();
+        for (uint32_t i = 0; i < n; ++i) {
+            const b3ContactVelocityConstraint& vc = solver.m_velocityConstraints[i];
+            for (uint32_t j = 0; j < counts[i]; ++j) {
+                const b3VelocityConstraintManifold& vcm = vc.manifolds[j];
+                b3o_velocity_manifold& o = out[(size_t)i * M + j];
+                std::memset(&o, 0, sizeof(o));
+                o.pointCount = vcm.pointCount;
+                for (uint32_t q = 0; q < vcm.pointCount; ++q) {
+                    const b3VelocityConstraintPoint& p = vcm.points[q];
+                    std::memcpy(&o.points[q].rA, &p.rA, 12);
+                    std::memcpy(&o.points[q].rB, &p.rB, 12);
+                    std::memcpy(&o.points[q].normal, &p.normal, 12);
+                    o.points[q].normalMass = p.normalMass;
+                    o.points[q].normalImpulse = p.normalImpulse;
+                    o.points[q].velocityBias = p.velocityBias;
+                }
+                if (vcm.pointCount > 0) {  // the reference leaves the rest unset for empty manifolds
+                    std::memcpy(&o.rA, &vcm.rA, 12);
+                    std::memcpy(&o.rB, &vcm.rB, 12);
+                    std::memcpy(&o.normal, &vcm.normal, 12);
+                    std::memcpy(&o.tangent1, &vcm.tangent1, 12);
+                    std::memcpy(&o.tangent2, &vcm.tangent2, 12);
+                    o.tangentMass[0] = vcm.tangentMass.x.x, o.tangentMass[1] = vcm.tangentMass.x.y;
+                    o.tangentMass[2] = vcm.tangentMass.y.x, o.tangentMass[3] = vcm.tangentMass.y.y;
+                    o.motorMass = vcm.motorMass;
+                }
+                o.tangentImpulse[0] = vcm.tangentImpulse.x;
+                o.tangentImpulse[1] = vcm.tangentImpulse.y;
+                o.motorImpulse = vcm.motorImpulse;
+                o.friction = vc.friction;
+                o.restitution = vc.restitution;
+            }
+        }
+    }
+    delete allocator;
+    for (b3ConvexContact* c : contacts) {
+        c->m_manifolds = &c->m_stackManifold;
+        delete c;
+    }
 }
 
 // b3QHull::Set (qhull.cpp:104-354) on `count` points; writes the hull's records and counts

@@ -76,6 +76,24 @@ class Checker:
           None if caches is None else ptr(caches), ptr(out))
         return out
 
+    def contact_init(self, M, bodies, state, material, counts, manifolds, old_counts, old_manifolds, old_impulses,
+                     warm=True):
+        """Warm-start matching and b3ContactSolver::InitializeConstraints for n contacts of M
+        manifold slots; returns (impulses [n, M], velocity constraints [n, M])."""
+        from bounce_amd.types import IMPULSES_DTYPE, VELOCITY_MANIFOLD_DTYPE
+        n = len(counts)
+        impulses = np.zeros((n, M), IMPULSES_DTYPE)
+        out = np.zeros((n, M), VELOCITY_MANIFOLD_DTYPE)
+        f = getattr(self.lib, self.prefix + "contact_init")
+        f.restype = None
+        f.argtypes = [ctypes.c_uint32, ctypes.c_uint32] + [ctypes.c_void_p] * 8 + [ctypes.c_int] + [ctypes.c_void_p] * 2
+        c = lambda a, dt: np.ascontiguousarray(a, dt)
+        args = [c(bodies, np.uint32), c(state, state.dtype), c(material, material.dtype), c(counts, np.uint32),
+                c(manifolds, MANIFOLD_DTYPE), c(old_counts, np.uint32), c(old_manifolds, MANIFOLD_DTYPE),
+                c(old_impulses, IMPULSES_DTYPE)]
+        f(n, M, *[ptr(a) for a in args], int(bool(warm)), ptr(impulses), ptr(out))
+        return impulses, out
+
     def collide(self, pool, shapes, pairs, xfs, sweeps=None, caches=None):
         pairs = np.ascontiguousarray(pairs, np.uint32).reshape(-1, 2)
         n = pairs.shape[0]

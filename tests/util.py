@@ -631,3 +631,53 @@ def qhull_digest(rec):
     for a in qhull_defined(rec)[0]:
         h.update(np.ascontiguousarray(a).tobytes())
     return h.hexdigest()
+
+
+def contact_init_workload(lib, oracle, seed, n, M=1):
+    """Inputs of the step after the narrowphase for n contacts of M manifold slots: the new
+    manifolds of a moving step and the old ones of the step before (from the oracle's
+    narrowphase with caches carried, so most keys persist), random old impulses, bodies with
+    random positions, orientations, velocities, mass data (some static) and materials, some
+    contacts fast enough to trip the restitution threshold.
+    Returns dict(bodies, state, material, counts, manifolds, old_counts, old_manifolds,
+    old_impulses)."""
+    from bounce_amd.types import (IMPULSES_DTYPE, MANIFOLD_DTYPE, MATERIAL_DTYPE, SOLVER_BODY_DTYPE,
+                                  empty_caches)
+    rng = np.random.default_rng(seed)
+    npairs = n * M
+    pool, shapes, pairs, xfs = mixed_workload(lib, seed, npairs, gap_mean=-0.02, gap_sigma=0.03)
+    caches = empty_caches(npairs)
+    old = oracle.collide(pool, shapes, pairs, xfs, caches=caches)
+    x2 = xfs.copy()
+    x2[:, 1, 9:12] += rng.normal(0.0, 0.002, (npairs, 3)).astype(np.float32)
+    new = oracle.collide(pool, shapes, pairs, x2, caches=caches)
+    nb = max(2, n)
+    state = np.zeros(nb, SOLVER_BODY_DTYPE)
+    state["x"] = rng.uniform(-5, 5, (nb, 3))
+    q = rng.normal(size=(nb, 4))
+    state["q"] = q / np.linalg.norm(q, axis=1, keepdims=True)
+    state["v"] = rng.normal(0, 1.5, (nb, 3))
+    state["w"] = rng.normal(0, 1.0, (nb, 3))
+    static = rng.random(nb) < 0.1
+    state["invMass"] = np.where(static, 0.0, rng.uniform(0.05, 2.0, nb))
+    A = rng.normal(size=(nb, 3, 3))
+    I = np.einsum("bij,bkj->bik", A, A) * 0.2 + np.eye(3) * 0.05
+    I[static] = 0.0
+    state["invI"] = I.transpose(0, 2, 1).reshape(nb, 9)
+    state["localCenter"] = rng.normal(0, 0.05, (nb, 3))
+    bodies = rng.integers(0, nb, (n, 2)).astype(np.uint32)
+    material = np.zeros(n, MATERIAL_DTYPE)
+    sa, sb = shapes[pairs[::M, 0]], shapes[pairs[::M, 1]]
+    material["radiusA"], material["radiusB"] = sa["radius"], sb["radius"]
+    material["frictionA"], material["frictionB"] = rng.uniform(0, 1, n), rng.uniform(0, 1, n)
+    material["restitutionA"], material["restitutionB"] = rng.uniform(0, 0.6, n), rng.uniform(0, 0.6, n)
+    manifolds = new.reshape(n, M)
+    old_manifolds = old.reshape(n, M)
+    counts = np.full(n, M, np.uint32) if M == 1 else rng.integers(1, M + 1, n).astype(np.uint32)
+    old_counts = np.full(n, M, np.uint32) if M == 1 else rng.integers(0, M + 1, n).astype(np.uint32)
+    old_impulses = np.zeros((n, M), IMPULSES_DTYPE)
+    old_impulses["normalImpulse"] = rng.uniform(0, 3, (n, M, 4))
+    old_impulses["tangentImpulse"] = rng.normal(0, 1, (n, M, 2))
+    old_impulses["motorImpulse"] = rng.normal(0, 1, (n, M))
+    return dict(bodies=bodies, state=state, material=material, counts=counts, manifolds=manifolds,
+                old_counts=old_counts, old_manifolds=old_manifolds, old_impulses=old_impulses)
