@@ -44,6 +44,14 @@ def load_library():
         "b3amd_memory_in_use": (u64, [vp]),
         "b3amd_upload_hulls": (i32, [vp, vp]),
         "b3amd_upload_shapes": (i32, [vp, u32, vp]),
+        "b3amd_hull_add": (i32, [vp, vp, vp]),
+        "b3amd_hull_free": (i32, [vp, u32]),
+        "b3amd_hull_registry_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32), ctypes.POINTER(u64),
+                                           ctypes.POINTER(u64)]),
+        "b3amd_collide_shape_and_shape": (i32, [vp, vp, vp, u32, vp, u32, vp, vp]),
+        "b3amd_cache_alloc": (i32, [vp, u32, ctypes.POINTER(vp)]),
+        "b3amd_cache_reset": (i32, [vp, vp, u32, u32]),
+        "b3amd_cache_free": (i32, [vp, vp]),
         "b3amd_collide": (i32, [vp, u32, vp, vp, vp, vp, vp]),
         "b3amd_collide_device": (i32, [vp, u32, vp, vp, vp, vp, vp]),
         "b3amd_update_contacts": (i32, [vp, u32, vp, vp, vp, u32, vp, vp, vp]),
@@ -141,6 +149,27 @@ class DeviceView(DeviceBuffer):
 
     def free(self):
         raise TypeError("a DeviceView does not own its memory")
+
+
+class CacheArray(DeviceBuffer):
+    """Device b3ConvexCache array from Context.cache_alloc."""
+
+    def __init__(self, ctx, p, count):
+        self.ctx = ctx
+        self.ptr = p
+        self.count = count
+        self.nbytes = 36 * count
+
+    def reset(self, first=0, count=None):
+        count = self.count - first if count is None else count
+        if first < 0 or count < 0 or first + count > self.count:
+            raise ValueError("reset range outside the array")
+        _check(self.ctx.lib.b3amd_cache_reset(self.ctx.handle, self.ptr, first, count))
+
+    def free(self):
+        if self.ptr is not None and self.ctx.handle is not None:
+            _check(self.ctx.lib.b3amd_cache_free(self.ctx.handle, self.ptr))
+        self.ptr = None
 
 
 class Graph:
@@ -349,6 +378,45 @@ class Context:
     def upload_hulls(self, pool):
         s = pool.struct()
         _check(self.lib.b3amd_upload_hulls(self.handle, ctypes.byref(s)))
+
+    def add_hulls(self, pool):
+        """Adds a HullPool's hulls to the registry; returns their handles (uint32)."""
+        s = pool.struct()
+        handles = np.zeros(max(1, pool.count), np.uint32)
+        _check(self.lib.b3amd_hull_add(self.handle, ctypes.byref(s), ptr(handles)))
+        return handles[:pool.count]
+
+    def free_hull(self, handle):
+        _check(self.lib.b3amd_hull_free(self.handle, int(handle)))
+
+    def hull_registry(self):
+        """dict: slots, live, used_bytes, capacity_bytes."""
+        slots, live = ctypes.c_uint32(), ctypes.c_uint32()
+        used, cap = ctypes.c_uint64(), ctypes.c_uint64()
+        _check(self.lib.b3amd_hull_registry_info(self.handle, ctypes.byref(slots), ctypes.byref(live),
+                                                 ctypes.byref(used), ctypes.byref(cap)))
+        return dict(slots=slots.value, live=live.value, used_bytes=used.value, capacity_bytes=cap.value)
+
+    def collide_one(self, xf_a, shape_a, xf_b, shape_b, sweeps=None, cache=None):
+        """One pair through b3amd_collide_shape_and_shape; returns the manifold record.
+        `cache` (a one-element CACHE_DTYPE array) is updated in place when given."""
+        m = np.zeros(1, MANIFOLD_DTYPE)
+        a = np.ascontiguousarray(xf_a, np.float32).reshape(12)
+        b = np.ascontiguousarray(xf_b, np.float32).reshape(12)
+        if sweeps is not None:
+            sweeps = np.ascontiguousarray(sweeps, np.float32).reshape(4, 4)
+        if cache is not None:
+            assert cache.dtype == CACHE_DTYPE and cache.flags["C_CONTIGUOUS"] and len(cache) == 1
+        _check(self.lib.b3amd_collide_shape_and_shape(self.handle, ptr(m), ptr(a), int(shape_a), ptr(b), int(shape_b),
+                                                      ptr(sweeps), ptr(cache)))
+        return m[0]
+
+    def cache_alloc(self, count):
+        """Device caches in the new-contact state; a DeviceBuffer-like handle (free with
+        cache_free)."""
+        p = ctypes.c_void_p()
+        _check(self.lib.b3amd_cache_alloc(self.handle, int(count), ctypes.byref(p)))
+        return CacheArray(self, p, int(count))
 
     def upload_shapes(self, shapes):
         shapes = np.ascontiguousarray(shapes, dtype=SHAPE_DTYPE)

@@ -4,10 +4,12 @@
 // (contact_manager.cpp:167-231).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <functional>
 #include <limits>
 #include <map>
 #include <string>
@@ -284,7 +286,9 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
 // hull indices outside the pool, which the kernels reject before reading a record).
 void set_hull_units(const b3amd_context* ctx, std::vector<ShapeRec>& shapes) {
     for (ShapeRec& r : shapes)
-        r.hullUnit = (r.type == B3AMD_HULL && r.hull < ctx->hullOffsets.size()) ? ctx->hullOffsets[r.hull] : 0u;
+        r.hullUnit = (r.type == B3AMD_HULL && r.hull < ctx->hullOffsets.size() && ctx->hullOffsets[r.hull] != kFreeHull)
+                         ? ctx->hullOffsets[r.hull]
+                         : 0u;
 }
 
 int upload(b3amd_context* ctx, void** dptr, uint64_t* cap, const void* src, uint64_t bytes) {
@@ -297,6 +301,119 @@ int upload(b3amd_context* ctx, void** dptr, uint64_t* cap, const void* src, uint
         *cap = bytes;
     }
     HIP_TRY(hipMemcpyAsync(*dptr, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    return B3AMD_OK;
+}
+
+// Device records of every hull of a pool, validated before anything changes.
+int pack_pool(const b3amd_hull_pool* pool, std::vector<std::vector<uint32_t>>& recs, std::vector<HullLayout>& layouts) {
+    if (pool->count && (!pool->desc || !pool->centroids || !pool->vertices || !pool->edges || !pool->faces ||
+                        !pool->planes))
+        return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    recs.resize(pool->count);
+    layouts.resize(pool->count);
+    for (uint32_t h = 0; h < pool->count; ++h)
+        if (const char* err = pack_hull(pool, h, recs[h], &layouts[h]))
+            return fail(B3AMD_ERR_UNSUPPORTED, "hull %u: %s", h, err);
+    return B3AMD_OK;
+}
+
+// Gives a record's units back: the range joins its neighbours, and a range at the arena's
+// tail shortens the used part instead.
+void release_range(b3amd_context* ctx, uint64_t first, uint64_t units) {
+    auto& R = ctx->freeRanges;
+    auto it = std::lower_bound(R.begin(), R.end(), std::make_pair(first, (uint64_t)0));
+    it = R.insert(it, {first, units});
+    if (it + 1 != R.end() && it->first + it->second == (it + 1)->first) {
+        it->second += (it + 1)->second;
+        R.erase(it + 1);
+    }
+    if (it != R.begin() && (it - 1)->first + (it - 1)->second == it->first) {
+        (it - 1)->second += it->second;
+        it = R.erase(it) - 1;
+    }
+    if (it->first + it->second == ctx->poolUsed) {
+        ctx->poolUsed = it->first;
+        R.erase(it);
+    }
+}
+
+// Places packed records in the arena (first fit in freed ranges, else the tail, growing the
+// arena once by at least double when the tail is short), uploads them and the slot offsets,
+// and writes each record's slot to handles (when not null).  The plan is made before any
+// device memory changes, so a failed allocation leaves the registry as it was.
+int place_hulls(b3amd_context* ctx, const std::vector<std::vector<uint32_t>>& recs,
+                const std::vector<HullLayout>& layouts, uint32_t* handles) {
+    const size_t n = recs.size();
+    std::vector<std::pair<uint64_t, uint64_t>> ranges = ctx->freeRanges;
+    uint64_t used = ctx->poolUsed;
+    std::vector<uint64_t> at(n);
+    for (size_t i = 0; i < n; ++i) {
+        const uint64_t units = recs[i].size() / 4;
+        auto it = std::find_if(ranges.begin(), ranges.end(), [&](const std::pair<uint64_t, uint64_t>& r) {
+            return r.second >= units;
+        });
+        if (it != ranges.end()) {
+            at[i] = it->first;
+            it->first += units;
+            it->second -= units;
+            if (it->second == 0) ranges.erase(it);
+        } else {
+            at[i] = used;
+            used += units;
+        }
+    }
+    if (used > 0xFFFFFFFFull) return fail(B3AMD_ERR_UNSUPPORTED, "hull pool larger than 2^32 units");
+    size_t reuse = ctx->freeSlots.size() < n ? ctx->freeSlots.size() : n;
+    const uint64_t slots = (uint64_t)ctx->hullCount + (n - reuse);
+    if (slots > 0xFFFFFFFEull) return fail(B3AMD_ERR_UNSUPPORTED, "too many hulls");
+    int rc;
+    if (used > ctx->poolCap) {
+        uint64_t cap = 2 * ctx->poolCap > used ? 2 * ctx->poolCap : used;
+        if (cap > 0xFFFFFFFFull) cap = used;
+        float4* p = nullptr;
+        if ((rc = ctx->alloc(16 * cap, (void**)&p))) return rc;
+        if (ctx->poolUsed && hipMemcpy(p, ctx->dPool, 16 * ctx->poolUsed, hipMemcpyDeviceToDevice) != hipSuccess) {
+            ctx->release(p);
+            return fail(B3AMD_ERR_DEVICE, "hull arena copy failed");
+        }
+        if ((rc = ctx->release(ctx->dPool))) return rc;
+        ctx->dPool = p;
+        ctx->poolCap = cap;
+        ++ctx->generation;  // captured graphs hold the old arena
+    }
+    if (slots > ctx->offsetsCap) {
+        const uint64_t cap = slots < 2ull * ctx->offsetsCap ? 2ull * ctx->offsetsCap : (slots < 64 ? 64 : slots);
+        uint32_t* p = nullptr;
+        if ((rc = ctx->alloc(4 * cap, (void**)&p))) return rc;
+        if ((rc = ctx->release(ctx->dOffsets))) return rc;
+        ctx->dOffsets = p;
+        ctx->offsetsCap = (uint32_t)cap;
+        ++ctx->generation;
+    }
+    for (size_t i = 0; i < n; ++i)
+        HIP_TRY(hipMemcpy(ctx->dPool + at[i], recs[i].data(), 4 * recs[i].size(), hipMemcpyHostToDevice));
+    ctx->freeRanges.swap(ranges);
+    ctx->poolUsed = used;
+    // Freed slots are reused lowest first.
+    std::sort(ctx->freeSlots.begin(), ctx->freeSlots.end(), std::greater<uint32_t>());
+    for (size_t i = 0; i < n; ++i) {
+        uint32_t slot;
+        if (!ctx->freeSlots.empty()) {
+            slot = ctx->freeSlots.back();
+            ctx->freeSlots.pop_back();
+        } else {
+            slot = ctx->hullCount++;
+            ctx->hullOffsets.push_back(kFreeHull);
+            ctx->hullUnits.push_back(0);
+            ctx->layouts.push_back(HullLayout{});
+        }
+        ctx->hullOffsets[slot] = (uint32_t)at[i];
+        ctx->hullUnits[slot] = (uint32_t)(recs[i].size() / 4);
+        ctx->layouts[slot] = layouts[i];
+        if (handles) handles[i] = slot;
+    }
+    if (ctx->hullCount)
+        HIP_TRY(hipMemcpy(ctx->dOffsets, ctx->hullOffsets.data(), 4ull * ctx->hullCount, hipMemcpyHostToDevice));
     return B3AMD_OK;
 }
 
@@ -452,39 +569,70 @@ int b3amd_destroy(b3amd_context* ctx) {
 uint64_t b3amd_memory_in_use(const b3amd_context* ctx) { return ctx ? ctx->inUse : 0; }
 
 int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool) {
-    if (!ctx || !pool || !pool->desc || !pool->centroids || !pool->vertices || !pool->edges || !pool->faces ||
-        !pool->planes)
-        return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    std::vector<uint32_t> all, words;
-    std::vector<uint32_t> offsets(pool->count);
-    std::vector<HullLayout> layouts(pool->count);
-    for (uint32_t h = 0; h < pool->count; ++h) {
-        const char* err = pack_hull(pool, h, words, &layouts[h]);
-        if (err) return fail(B3AMD_ERR_UNSUPPORTED, "hull %u: %s", h, err);
-        if ((all.size() / 4) > 0xFFFFFFFFull) return fail(B3AMD_ERR_UNSUPPORTED, "pool too large");
-        offsets[h] = (uint32_t)(all.size() / 4);
-        all.insert(all.end(), words.begin(), words.end());
-    }
-    int rc;
+    if (!ctx || !pool) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<std::vector<uint32_t>> recs;
+    std::vector<HullLayout> layouts;
+    int rc = pack_pool(pool, recs, layouts);
+    if (rc) return rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
     if ((rc = ctx->release(ctx->dPool)) || (rc = ctx->release(ctx->dOffsets))) return rc;
     ctx->dPool = nullptr;
     ctx->dOffsets = nullptr;
+    ctx->poolCap = ctx->poolUsed = 0;
+    ctx->offsetsCap = 0;
     ctx->hullCount = 0;
-    if ((rc = ctx->alloc(4ull * all.size(), (void**)&ctx->dPool))) return rc;
-    if ((rc = ctx->alloc(4ull * offsets.size(), (void**)&ctx->dOffsets))) return rc;
-    HIP_TRY(hipMemcpy(ctx->dPool, all.data(), 4ull * all.size(), hipMemcpyHostToDevice));
-    HIP_TRY(hipMemcpy(ctx->dOffsets, offsets.data(), 4ull * offsets.size(), hipMemcpyHostToDevice));
-    ctx->hullCount = pool->count;
+    ctx->layouts.clear();
+    ctx->hullOffsets.clear();
+    ctx->hullUnits.clear();
+    ctx->freeSlots.clear();
+    ctx->freeRanges.clear();
     ++ctx->generation;
-    ctx->layouts.swap(layouts);
-    ctx->hullOffsets.swap(offsets);
+    if ((rc = place_hulls(ctx, recs, layouts, nullptr))) return rc;
     // Shapes uploaded before this pool point into the old one: refresh their record units.
     if (ctx->shapeCount && ctx->dShapes) {
         set_hull_units(ctx, ctx->shapes);
         HIP_TRY(hipMemcpy(ctx->dShapes, ctx->shapes.data(), sizeof(ShapeRec) * (uint64_t)ctx->shapeCount,
                           hipMemcpyHostToDevice));
     }
+    return B3AMD_OK;
+}
+
+int b3amd_hull_add(b3amd_context* ctx, const b3amd_hull_pool* hulls, uint32_t* handles) {
+    if (!ctx || !hulls || (hulls->count && !handles)) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<std::vector<uint32_t>> recs;
+    std::vector<HullLayout> layouts;
+    int rc = pack_pool(hulls, recs, layouts);
+    if (rc) return rc;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return place_hulls(ctx, recs, layouts, handles);
+}
+
+int b3amd_hull_free(b3amd_context* ctx, uint32_t handle) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    if (handle >= ctx->hullCount || ctx->hullOffsets[handle] == kFreeHull)
+        return fail(B3AMD_ERR_ARGUMENT, "hull %u is not a live handle", handle);
+    for (uint32_t i = 0; i < ctx->shapeCount; ++i)
+        if (ctx->shapes[i].type == B3AMD_HULL && ctx->shapes[i].hull == handle)
+            return fail(B3AMD_ERR_ARGUMENT, "hull %u is in use by shape %u", handle, i);
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    release_range(ctx, ctx->hullOffsets[handle], ctx->hullUnits[handle]);
+    ctx->hullOffsets[handle] = kFreeHull;
+    ctx->hullUnits[handle] = 0;
+    ctx->freeSlots.push_back(handle);
+    return B3AMD_OK;
+}
+
+int b3amd_hull_registry_info(const b3amd_context* ctx, uint32_t* slots, uint32_t* live, uint64_t* used_bytes,
+                             uint64_t* capacity_bytes) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    uint64_t used = 0;
+    uint32_t n = 0;
+    for (uint32_t h = 0; h < ctx->hullCount; ++h)
+        if (ctx->hullOffsets[h] != kFreeHull) ++n, used += ctx->hullUnits[h];
+    if (slots) *slots = ctx->hullCount;
+    if (live) *live = n;
+    if (used_bytes) *used_bytes = 16 * used;
+    if (capacity_bytes) *capacity_bytes = 16 * ctx->poolCap;
     return B3AMD_OK;
 }
 
@@ -495,6 +643,8 @@ int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* s
         if (shapes[i].type > B3AMD_HULL) return fail(B3AMD_ERR_UNSUPPORTED, "shape %u: type %u", i, shapes[i].type);
         if (shapes[i].type == B3AMD_HULL && shapes[i].hull >= ctx->hullCount)
             return fail(B3AMD_ERR_ARGUMENT, "shape %u: hull %u of %u", i, shapes[i].hull, ctx->hullCount);
+        if (shapes[i].type == B3AMD_HULL && ctx->hullOffsets[shapes[i].hull] == kFreeHull)
+            return fail(B3AMD_ERR_ARGUMENT, "shape %u: hull %u was freed", i, shapes[i].hull);
     }
     int rc;
     HIP_TRY(hipStreamSynchronize(ctx->stream));
@@ -1163,6 +1313,46 @@ int b3amd_synchronize(b3amd_context* ctx) {
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr) {
     if (!ctx || !ptr) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     return ctx->alloc(bytes, ptr);
+}
+
+int b3amd_cache_alloc(b3amd_context* ctx, uint32_t count, b3amd_convex_cache** d_caches) {
+    if (!ctx || !d_caches) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    void* p = nullptr;
+    int rc = ctx->alloc(36ull * (count ? count : 1), &p);
+    if (rc) return rc;
+    if (launch_cache_reset(static_cast<ConvexCache*>(p), count, ctx->stream) != hipSuccess ||
+        hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        ctx->release(p);
+        return fail(B3AMD_ERR_DEVICE, "cache reset failed");
+    }
+    *d_caches = static_cast<b3amd_convex_cache*>(p);
+    return B3AMD_OK;
+}
+
+int b3amd_cache_reset(b3amd_context* ctx, b3amd_convex_cache* d_caches, uint32_t first, uint32_t count) {
+    if (!ctx || (count && !d_caches)) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if ((uint64_t)first + count > 0xFFFFFFFFull) return fail(B3AMD_ERR_ARGUMENT, "range beyond 2^32 caches");
+    HIP_TRY(launch_cache_reset(reinterpret_cast<ConvexCache*>(d_caches) + first, count, ctx->stream));
+    return B3AMD_OK;
+}
+
+int b3amd_cache_free(b3amd_context* ctx, b3amd_convex_cache* d_caches) {
+    if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
+    if (!d_caches) return B3AMD_OK;
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ctx->release(d_caches);
+}
+
+int b3amd_collide_shape_and_shape(b3amd_context* ctx, b3amd_manifold* manifold, const b3amd_transform* xfA,
+                                  uint32_t shapeA, const b3amd_transform* xfB, uint32_t shapeB,
+                                  const b3amd_quat* sweeps, b3amd_convex_cache* cache) {
+    if (!ctx || !manifold || !xfA || !xfB) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (sweeps && !cache) return fail(B3AMD_ERR_ARGUMENT, "sweeps without a cache");
+    const uint32_t pair[2] = {shapeA, shapeB};
+    b3amd_transform xf[2];
+    std::memcpy(&xf[0], xfA, sizeof(b3amd_transform));
+    std::memcpy(&xf[1], xfB, sizeof(b3amd_transform));
+    return b3amd_collide(ctx, 1, pair, xf, sweeps, cache, manifold);
 }
 
 int b3amd_device_free(b3amd_context* ctx, void* ptr) {

@@ -9,6 +9,7 @@
 #include <map>
 #include <string>
 #include <unordered_map>
+#include <utility>
 #include <vector>
 
 #include "bounce_amd.h"
@@ -45,6 +46,9 @@ struct BodyRefs {
     const float* sweeps;     // [B][8] or null
 };
 
+// hullOffsets entry of a freed registry slot.
+constexpr uint32_t kFreeHull = 0xFFFFFFFFu;
+
 // The streams a batch runs on: the hull kernels on `main`; sphere/capsule pairs forked to
 // `side` after `fork` is recorded on main, joined back through `join`.  All at the default
 // priority.
@@ -76,12 +80,19 @@ struct b3amd_context {
     uint64_t inUse = 0;
     std::unordered_map<void*, uint64_t> allocs;
 
-    // Hull pool.
+    // Hull registry: one device arena of 16-byte units.  A record goes first-fit into a
+    // freed range, else at the tail (the arena doubles when full).  Slot h's record starts at
+    // hullOffsets[h]; a freed slot holds kFreeHull and is reused by the next add.
     float4* dPool = nullptr;
+    uint64_t poolCap = 0, poolUsed = 0;  // units
     uint32_t* dOffsets = nullptr;
-    uint32_t hullCount = 0;
+    uint32_t offsetsCap = 0;
+    uint32_t hullCount = 0;              // slots, live or free
     std::vector<HullLayout> layouts;
-    std::vector<uint32_t> hullOffsets;  // first 16-byte unit of each hull's record
+    std::vector<uint32_t> hullOffsets;   // first 16-byte unit of each slot's record
+    std::vector<uint32_t> hullUnits;     // record length of each slot, in units
+    std::vector<uint32_t> freeSlots;
+    std::vector<std::pair<uint64_t, uint64_t>> freeRanges;  // (first unit, units), sorted
 
     // Shape table.
     ShapeRec* dShapes = nullptr;

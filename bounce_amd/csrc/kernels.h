@@ -358,6 +358,8 @@ hipError_t launch_body_gather(const BodyGatherArgs& a, hipStream_t s);
 
 hipError_t launch_convex(const ConvexArgs& a, hipStream_t s);
 hipError_t launch_overlap(const OverlapArgs& a, hipStream_t s);
+// Caches [0, n) to the empty state of a new contact (a fill kernel, safe under capture).
+hipError_t launch_cache_reset(ConvexCache* caches, uint32_t n, hipStream_t s);
 hipError_t launch_zero_words(uint32_t* words, uint32_t count, hipStream_t s);
 // Zeroes the counter words, marks the cached pairs and lists the SAT-bound ones, reads the
 // feature caches four lanes per cached pair (listing the contacts to rebuild), rebuilds them

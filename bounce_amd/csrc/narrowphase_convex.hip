@@ -102,7 +102,28 @@ __global__ __launch_bounds__(256) void overlap_kernel(OverlapArgs a) {
     }
 }
 
+// The state a new contact's cache starts in (b3ConvexContact's constructor,
+// convex_contact.cpp:35-36: simplex count 0, feature cache empty); the fields the reference
+// leaves unset are zero.
+__global__ __launch_bounds__(256) void cache_reset_kernel(ConvexCache* c, uint32_t n) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    ConvexCache e;
+    for (int t = 0; t < 5; ++t) e.simplex[t] = 0u;
+    e.feature.state = kCacheEmpty;
+    e.feature.type = 0u;
+    e.feature.index1 = 0u;
+    e.feature.index2 = 0u;
+    c[i] = e;
+}
+
 }  // namespace
+
+hipError_t launch_cache_reset(ConvexCache* caches, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    cache_reset_kernel<<<(n + 255) / 256, 256, 0, s>>>(caches, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_overlap(const OverlapArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;

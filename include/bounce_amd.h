@@ -117,10 +117,22 @@ const char* b3amd_last_error(void);
 uint64_t b3amd_memory_in_use(const b3amd_context* ctx);
 
 /* ---- geometry registry ----
- * Replaces the borrowed `const b3Hull*` of b3HullShape (hull_shape.h:44): the pool is
- * validated, converted to the device record layout and uploaded once; pairs refer to
- * hulls by pool index.  A new upload replaces the previous pool. */
+ * Replaces the borrowed `const b3Hull*` of b3HullShape (hull_shape.h:44): hulls are
+ * validated, converted to the device record layout and kept in one device arena; shapes
+ * refer to hulls by handle (b3amd_shape.hull).  b3amd_upload_hulls replaces every hull with
+ * the pool's, handle i = pool hull i. */
 int b3amd_upload_hulls(b3amd_context* ctx, const b3amd_hull_pool* pool);
+/* Adds the pool's hulls to the registry without touching the others and writes their handles
+ * (freed handles are reused, lowest first).  Only the new records are uploaded; the arena
+ * grows by doubling when full, and only then do graphs captured before refuse to launch.
+ * The b3Hull a b3HullShape borrows (hull_shape.h:44) becomes this handle. */
+int b3amd_hull_add(b3amd_context* ctx, const b3amd_hull_pool* hulls, uint32_t* handles);
+/* Frees a hull's record and handle.  Fails while a shape of the current shape table refers
+ * to it (the reference frees a b3Hull only when no shape borrows it). */
+int b3amd_hull_free(b3amd_context* ctx, uint32_t handle);
+/* Registry state: handle slots, live hulls, bytes of live records, arena capacity. */
+int b3amd_hull_registry_info(const b3amd_context* ctx, uint32_t* slots, uint32_t* live, uint64_t* used_bytes,
+                             uint64_t* capacity_bytes);
 /* The shape table that pair indices refer to (b3Shape records of the world). */
 int b3amd_upload_shapes(b3amd_context* ctx, uint32_t count, const b3amd_shape* shapes);
 
@@ -141,6 +153,16 @@ int b3amd_collide(b3amd_context* ctx, uint32_t n, const uint32_t* pairs,
 int b3amd_collide_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs,
                          const b3amd_transform* d_xfs, const b3amd_quat* d_sweeps,
                          b3amd_convex_cache* d_caches, b3amd_manifold* d_out);
+
+/* One pair, synchronously, with the signature of b3CollideShapeAndShape (collide.h:63-66):
+ * manifold, transform and shape of A, transform and shape of B, cache.  Shapes are indices
+ * into the uploaded shape table; sweeps (may be NULL) are the four orientations the cached
+ * face rebuild reads from the bodies (orientation0, orientation of A, then of B,
+ * collide_hulls_cache.cpp:117-126).  For call sites that collide one contact at a time;
+ * batches belong to b3amd_collide. */
+int b3amd_collide_shape_and_shape(b3amd_context* ctx, b3amd_manifold* manifold, const b3amd_transform* xfA,
+                                  uint32_t shapeA, const b3amd_transform* xfB, uint32_t shapeB,
+                                  const b3amd_quat* sweeps, b3amd_convex_cache* cache);
 
 /* ---- the step after the narrowphase ----
  * Warm-start matching and contact-constraint setup on device-resident manifolds, so that a
@@ -434,6 +456,16 @@ int b3amd_mesh_contacts_read(b3amd_mesh_contacts* contacts, uint32_t* triangleCo
 /* ---- device memory helpers (callers need no other GPU runtime) ---- */
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);
 int b3amd_device_free(b3amd_context* ctx, void* ptr);
+
+/* ---- contact caches ----
+ * Device arrays of b3ConvexCache, one per contact, in the state a new contact's cache starts
+ * in (b3ConvexContact's constructor, convex_contact.cpp:35-36, and each triangle cache of
+ * b3MeshContact::FindNewPairs, mesh_contact.cpp:182-183: simplex count 0, feature cache
+ * empty).  Reset re-empties caches [first, first + count) on the context stream with a fill
+ * kernel (no memset node, so it may be captured), for contacts created in reused slots. */
+int b3amd_cache_alloc(b3amd_context* ctx, uint32_t count, b3amd_convex_cache** d_caches);
+int b3amd_cache_reset(b3amd_context* ctx, b3amd_convex_cache* d_caches, uint32_t first, uint32_t count);
+int b3amd_cache_free(b3amd_context* ctx, b3amd_convex_cache* d_caches);
 int b3amd_copy_to_device(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
 int b3amd_copy_to_host(b3amd_context* ctx, void* dst, const void* src, uint64_t bytes);
 /* Device-to-device copy on the context stream (synchronous).  Also the plain copy of

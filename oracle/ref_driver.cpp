@@ -542,6 +542,7 @@ struct Pyramid {
     std::vector<b3Body*> bodies;
     std::unordered_map<const b3Shape*, uint32_t> bodyOf;
     std::unordered_map<const b3Contact*, uint32_t> ids;  // live contacts -> stable id
+    std::vector<b3Contact*> gathered;                     // the last gather's contacts
     uint32_t nextId = 0;
     uint64_t saved[6];
 };
@@ -636,21 +637,21 @@ extern "C" void b3ref_pyramid_bodies(void* handle, b3o_xf* xfs, b3o_quat* sweeps
     }
 }
 
-// One b3World::Step(1/60, 8, 2).  Records, in m_contactList order, every contact that
-// UpdateContacts collides in this step (contact_manager.cpp:167-231: bodies may collide,
-// one of them active, fat AABBs overlapping): its stable id, body pair, the transforms and
-// sweeps the collider reads, the cache before and after, and the manifold.  Returns the
-// number of such contacts; records beyond cap are not written.
-extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids, uint32_t* bodyPairs,
-                                       b3o_xf* xfs, b3o_quat* sweeps, b3o_convex_cache* cacheIn,
-                                       b3o_convex_cache* cacheOut, b3o_manifold* out) {
+// The contacts the next b3World::Step will collide, in m_contactList order, as
+// UpdateContacts selects them (contact_manager.cpp:167-231: bodies may collide, one of them
+// active, fat AABBs overlapping), after what Step does first when shapes were added
+// (world.cpp:129-134).  Writes, for the first cap of them, a stable id, the body pair, the
+// transforms and sweeps the collider reads and the cache it starts from; keeps the list in
+// p->gathered and returns its length.
+extern "C" uint32_t b3ref_pyramid_gather(void* handle, uint32_t cap, uint32_t* ids, uint32_t* bodyPairs,
+                                         b3o_xf* xfs, b3o_quat* sweeps, b3o_convex_cache* cacheIn) {
     Pyramid* p = (Pyramid*)handle;
     b3World* w = p->world;
-    if (w->m_flags & b3World::e_shapeAddedFlag) {  // what Step does first (world.cpp:129-134)
+    if (w->m_flags & b3World::e_shapeAddedFlag) {
         w->m_contactMan.FindNewContacts();
         w->m_flags &= ~b3World::e_shapeAddedFlag;
     }
-    std::vector<b3Contact*> collided;
+    p->gathered.clear();
     for (b3Contact* c = w->m_contactMan.m_contactList.m_head; c; c = c->m_next) {
         b3Shape* sA = c->m_pair.shapeA;
         b3Shape* sB = c->m_pair.shapeB;
@@ -663,8 +664,8 @@ extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids
             continue;
         }
         if (!activeA && !activeB) continue;
-        const uint32_t k = (uint32_t)collided.size();
-        collided.push_back(c);
+        const uint32_t k = (uint32_t)p->gathered.size();
+        p->gathered.push_back(c);
         if (k >= cap) continue;
         b3ConvexContact* cc = (b3ConvexContact*)c;
         ids[k] = contact_id(p, c);
@@ -678,13 +679,122 @@ extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids
         std::memcpy(sweeps + 4 * k + 3, &bB->m_sweep.orientation, 16);
         std::memcpy(cacheIn + k, &cc->m_cache, sizeof(b3o_convex_cache));
     }
-    w->Step(1.0f / 60.0f, 8, 2);
-    for (uint32_t k = 0; k < collided.size() && k < cap; ++k) {
-        b3ConvexContact* cc = (b3ConvexContact*)collided[k];
+    return (uint32_t)p->gathered.size();
+}
+
+// One b3World::Step(1/60, 8, 2) with the reference's own narrowphase.  Records the gathered
+// contacts (b3ref_pyramid_gather) and, after the step, each one's cache and manifold.
+extern "C" uint32_t b3ref_pyramid_step(void* handle, uint32_t cap, uint32_t* ids, uint32_t* bodyPairs,
+                                       b3o_xf* xfs, b3o_quat* sweeps, b3o_convex_cache* cacheIn,
+                                       b3o_convex_cache* cacheOut, b3o_manifold* out) {
+    Pyramid* p = (Pyramid*)handle;
+    const uint32_t n = b3ref_pyramid_gather(handle, cap, ids, bodyPairs, xfs, sweeps, cacheIn);
+    p->world->Step(1.0f / 60.0f, 8, 2);
+    for (uint32_t k = 0; k < n && k < cap; ++k) {
+        b3ConvexContact* cc = (b3ConvexContact*)p->gathered[k];
         std::memcpy(cacheOut + k, &cc->m_cache, sizeof(b3o_convex_cache));
         to_out(cc->m_stackManifold, out + k);
     }
-    return (uint32_t)collided.size();
+    return n;
+}
+
+// ---------------------------------------------------------------------------------
+// The closed loop: the same world stepped by b3World::Step, but every b3ConvexContact::
+// Collide (convex_contact.cpp:52-63) takes its manifold and cache from a batch the caller
+// computed, before the step, for the contacts b3ref_pyramid_gather listed.  The library is
+// linked with -Wl,--wrap on b3CollideShapeAndShape (collide.cpp:162-186), so the reference's
+// own call sites reach wrap_collide; outside a loop step it forwards to the reference's
+// function.  Everything else in the step (b3Contact::Update's tail, contact.cpp:63-140,
+// the islands and the solver) is the reference's unchanged code.
+namespace {
+
+struct LoopStep {
+    std::unordered_map<const b3ConvexCache*, uint32_t> slot;
+    const b3o_xf* xfs;
+    const b3o_manifold* manifolds;
+    const b3o_convex_cache* caches;
+    uint32_t served = 0, missed = 0, mismatched = 0;
+};
+LoopStep* g_loop = nullptr;
+
+}  // namespace
+
+void real_collide(b3Manifold& manifold, const b3Transform& xfA, const b3Shape* shapeA, const b3Transform& xfB,
+                  const b3Shape* shapeB, b3ConvexCache* cache)
+    __asm__("__real__Z22b3CollideShapeAndShapeR10b3ManifoldRK11b3TransformPK7b3ShapeS3_S6_P13b3ConvexCache");
+void wrap_collide(b3Manifold& manifold, const b3Transform& xfA, const b3Shape* shapeA, const b3Transform& xfB,
+                  const b3Shape* shapeB, b3ConvexCache* cache)
+    __asm__("__wrap__Z22b3CollideShapeAndShapeR10b3ManifoldRK11b3TransformPK7b3ShapeS3_S6_P13b3ConvexCache");
+
+void wrap_collide(b3Manifold& manifold, const b3Transform& xfA, const b3Shape* shapeA, const b3Transform& xfB,
+                  const b3Shape* shapeB, b3ConvexCache* cache) {
+    LoopStep* L = g_loop;
+    if (!L) {
+        real_collide(manifold, xfA, shapeA, xfB, shapeB, cache);
+        return;
+    }
+    auto it = L->slot.find(cache);
+    if (it == L->slot.end()) {  // a contact the gather did not list: the step stays defined
+        ++L->missed;
+        real_collide(manifold, xfA, shapeA, xfB, shapeB, cache);
+        return;
+    }
+    const uint32_t k = it->second;
+    if (std::memcmp(&xfA, L->xfs + 2 * k, sizeof(b3Transform)) != 0 ||
+        std::memcmp(&xfB, L->xfs + 2 * k + 1, sizeof(b3Transform)) != 0)
+        ++L->mismatched;
+    // What the reference's collider writes: the point count and, per point, the local
+    // normal and points and the key; the impulses stay as b3Manifold::Initialize set them.
+    const b3o_manifold& m = L->manifolds[k];
+    manifold.pointCount = m.pointCount;
+    for (u32 i = 0; i < m.pointCount; ++i) {
+        b3ManifoldPoint& q = manifold.points[i];
+        const b3o_mpoint& s = m.points[i];
+        std::memcpy(&q.localNormal1, &s.localNormal1, 12);
+        std::memcpy(&q.localPoint1, &s.localPoint1, 12);
+        std::memcpy(&q.localPoint2, &s.localPoint2, 12);
+        q.key.triangleKey = s.triangleKey;
+        std::memcpy(&q.key.key1, s.key, 8);
+        std::memcpy(&q.key.key2, s.key + 2, 8);
+    }
+    std::memcpy(cache, L->caches + k, sizeof(b3ConvexCache));
+    ++L->served;
+}
+
+// One b3World::Step(1/60, 8, 2) in the closed loop.  n, xfs [n][2], manifolds [n] and
+// caches [n] describe the contacts of the last b3ref_pyramid_gather in its order (xfs as it
+// wrote them; a contact whose transforms differ at Collide time is counted as mismatched).
+// stats = served, missed (collided but not gathered), mismatched.  Returns -1 when n is not
+// the gathered count.
+extern "C" int b3ref_pyramid_loop_step(void* handle, uint32_t n, const b3o_xf* xfs, const b3o_manifold* manifolds,
+                                       const b3o_convex_cache* caches, uint32_t stats[3]) {
+    Pyramid* p = (Pyramid*)handle;
+    if (n != p->gathered.size()) return -1;
+    LoopStep L;
+    L.xfs = xfs, L.manifolds = manifolds, L.caches = caches;
+    for (uint32_t k = 0; k < n; ++k) L.slot[&((b3ConvexContact*)p->gathered[k])->m_cache] = k;
+    g_loop = &L;
+    p->world->Step(1.0f / 60.0f, 8, 2);
+    g_loop = nullptr;
+    stats[0] = L.served, stats[1] = L.missed, stats[2] = L.mismatched;
+    return 0;
+}
+
+// Per body, 32 floats: transform (12), sweep world centre and orientation (3 + 4), their
+// values at the step's start (3 + 4), linear and angular velocity (3 + 3).
+extern "C" void b3ref_pyramid_state(void* handle, float* out) {
+    Pyramid* p = (Pyramid*)handle;
+    for (size_t i = 0; i < p->bodies.size(); ++i) {
+        const b3Body* b = p->bodies[i];
+        float* o = out + 32 * i;
+        std::memcpy(o, &b->m_xf, 48);
+        std::memcpy(o + 12, &b->m_sweep.worldCenter, 12);
+        std::memcpy(o + 15, &b->m_sweep.orientation, 16);
+        std::memcpy(o + 19, &b->m_sweep.worldCenter0, 12);
+        std::memcpy(o + 22, &b->m_sweep.orientation0, 16);
+        std::memcpy(o + 26, &b->m_linearVelocity, 12);
+        std::memcpy(o + 29, &b->m_angularVelocity, 12);
+    }
 }
 
 // ---------------------------------------------------------------------------------

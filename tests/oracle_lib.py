@@ -212,6 +212,12 @@ class RefPyramid:
         lib.b3ref_pyramid_bodies.argtypes = [ctypes.c_void_p] * 3
         lib.b3ref_pyramid_step.restype = ctypes.c_uint32
         lib.b3ref_pyramid_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 7
+        lib.b3ref_pyramid_gather.restype = ctypes.c_uint32
+        lib.b3ref_pyramid_gather.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 5
+        lib.b3ref_pyramid_loop_step.restype = ctypes.c_int
+        lib.b3ref_pyramid_loop_step.argtypes = [ctypes.c_void_p, ctypes.c_uint32] + [ctypes.c_void_p] * 4
+        lib.b3ref_pyramid_state.restype = None
+        lib.b3ref_pyramid_state.argtypes = [ctypes.c_void_p, ctypes.c_void_p]
         self.handle = lib.b3ref_pyramid_create(levels)
         self.body_count = lib.b3ref_pyramid_body_count(self.handle)
 
@@ -265,6 +271,42 @@ class RefPyramid:
             raise RuntimeError("pyramid step collided %d contacts, more than the capacity %d" % (n, cap))
         return dict(ids=ids[:n], pairs=pairs[:n], xfs=xfs[:n], sweeps=sweeps[:n], caches_in=cin[:n],
                     caches_out=cout[:n], out=out[:n])
+
+    def gather(self):
+        """The contacts the next step will collide, in list order, with the inputs their
+        collider reads (b3ref_pyramid_gather): dict of ids, pairs, xfs, sweeps, caches."""
+        cap = self.cap
+        ids = np.zeros(cap, np.uint32)
+        pairs = np.zeros((cap, 2), np.uint32)
+        xfs = np.zeros((cap, 2, 12), np.float32)
+        sweeps = np.zeros((cap, 4, 4), np.float32)
+        caches = np.zeros(cap, CACHE_DTYPE)
+        n = self.lib.b3ref_pyramid_gather(self.handle, cap, ptr(ids), ptr(pairs), ptr(xfs), ptr(sweeps),
+                                          ptr(caches))
+        if n > cap:
+            raise RuntimeError("pyramid gather listed %d contacts, more than the capacity %d" % (n, cap))
+        return dict(ids=ids[:n], pairs=pairs[:n], xfs=np.ascontiguousarray(xfs[:n]),
+                    sweeps=np.ascontiguousarray(sweeps[:n]), caches=caches[:n].copy())
+
+    def loop_step(self, gathered, manifolds, caches):
+        """One b3World::Step whose convex contacts take these manifolds and caches (one per
+        gathered contact, in gather order).  Returns (served, missed, mismatched)."""
+        n = len(gathered["ids"])
+        m = np.ascontiguousarray(manifolds, MANIFOLD_DTYPE)
+        c = np.ascontiguousarray(caches, CACHE_DTYPE)
+        if len(m) != n or len(c) != n:
+            raise ValueError("one manifold and one cache per gathered contact")
+        stats = np.zeros(3, np.uint32)
+        if self.lib.b3ref_pyramid_loop_step(self.handle, n, ptr(gathered["xfs"]), ptr(m), ptr(c), ptr(stats)):
+            raise RuntimeError("loop step: the batch does not match the last gather")
+        return tuple(int(x) for x in stats)
+
+    def state(self):
+        """Per body [B][32] float32: transform, sweep centre and orientation now and at the
+        step's start, linear and angular velocity."""
+        out = np.zeros((self.body_count, 32), np.float32)
+        self.lib.b3ref_pyramid_state(self.handle, ptr(out))
+        return out
 
 
 class RefMeshContacts:
