@@ -571,6 +571,46 @@ def serving_run(lib, worlds_n, rate, duration, seed=4096):
     return out
 
 
+def other_forms(ctx, n, dp, dx, do, steps, eager_out):
+    """The same batch in the two other forms a caller runs it: with feature caches that start
+    empty (b3ConvexContact's new-contact state, convex_contact.cpp:35-36; the batch reads and
+    writes 36 B of cache per pair; the caches are re-emptied by a fill kernel before every
+    batch, inside the timed region) and as a captured HIP graph replayed `steps` times.  Both
+    must give the eager batch's manifolds."""
+    from bounce_amd.types import MANIFOLD_DTYPE
+    res = {}
+    caches = ctx.cache_alloc(n)
+    try:
+        caches.reset()
+        ctx.collide_device(n, dp, dx, do, d_caches=caches)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            caches.reset()
+            ctx.collide_device(n, dp, dx, do, d_caches=caches)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+        same = do.download(np.zeros(n, MANIFOLD_DTYPE)).tobytes() == eager_out.tobytes()
+        res["empty_caches"] = {"value": n / (ms / 1000.0), "unit": "pairs/s", "ms_per_step": ms,
+                               "manifolds_equal_eager": same}
+    finally:
+        caches.free()
+    with ctx.capture(n, dp, dx, do) as g:
+        g.launch()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.launch()
+        ctx.synchronize()
+        ms = (time.perf_counter() - t0) * 1000.0 / steps
+    same = do.download(np.zeros(n, MANIFOLD_DTYPE)).tobytes() == eager_out.tobytes()
+    res["graph_replay"] = {"value": n / (ms / 1000.0), "unit": "pairs/s", "ms_per_step": ms,
+                           "manifolds_equal_eager": same}
+    log("empty-cache form %.3f ms per batch, graph replay %.3f ms per batch"
+        % (res["empty_caches"]["ms_per_step"], ms))
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -590,7 +630,7 @@ def main():
     ap.add_argument("--serving-rate", type=float, default=240000.0)
     ap.add_argument("--serving-seconds", type=float, default=0.5)
     ap.add_argument("--tuning", default="auto", metavar="auto|W,K",
-                    help="SAT launch shape: autotune on the batch, or waves-per-SIMD,register-slots")
+                    help="SAT launch shape: autotune on the batch, or waves-per-SIMD,tail-chunk")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="also run one device-to-device copy of BYTES (reference for HBM counters)")
     args = ap.parse_args()
@@ -645,8 +685,8 @@ def main():
             b.free()
         if args.tuning == "auto":
             t = time.perf_counter()
-            shape, tms = ctx.autotune(n, dp, dx, do, repeats=1)
-            log("autotune: waves/SIMD %d, register slots %d (%.3f ms per batch) in %.1f s"
+            shape, tms = ctx.autotune(n, dp, dx, do, repeats=3)
+            log("autotune: waves/SIMD %d, tail chunk %d (%.3f ms per batch) in %.1f s"
                 % (shape[0], shape[1], tms, time.perf_counter() - t))
         else:
             w, k = (int(x) for x in args.tuning.split(","))
@@ -665,6 +705,7 @@ def main():
         ctx.set_profiling(False)
         stats = {k: ctx.kernel_stats(k) for k in ("cache_hull", "sat_hull", "contact_hull")}
         out = do.download(np.zeros(n, MANIFOLD_DTYPE))
+        forms = other_forms(ctx, n, dp, dx, do, args.steps, out)
         # The plain drop-in form: one b3amd_collide per step on host arrays (PCIe both ways).
         plain_ms = None
         if not args.no_plain:
@@ -701,7 +742,7 @@ def main():
             "hull_pool": pool.count,
             "overlap_fraction": float((out["pointCount"] > 0).mean()),
             "parallelism": "single GPU",
-            "sat_launch_shape": {"waves_per_simd": tuning[0], "register_slots": tuning[1],
+            "sat_launch_shape": {"waves_per_simd": tuning[0], "tail_chunk": tuning[1],
                                  "chosen_by": "autotune" if args.tuning == "auto" else "flag"},
         },
         "kernels_ms": {k: (v[1] / max(1, v[0])) for k, v in stats.items()},
@@ -727,6 +768,7 @@ def main():
             "what": "b3amd_collide on host arrays, one call per step: H2D of pairs and transforms, the "
                     "same kernels, D2H of manifolds"},
         "separation_error_fp64": separation_error_fp64(out, xfs),
+        "other_forms": forms,
     }
     if not args.no_mesh:
         result["mesh"] = mesh_run(mesh_world, 3, mesh_cpu)

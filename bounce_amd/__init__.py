@@ -550,9 +550,9 @@ class Context:
                                            ctypes.byref(h)))
         return Graph(self, h)
 
-    def set_sat_tuning(self, waves_per_simd, max_slots):
+    def set_sat_tuning(self, waves_per_simd, tail_chunk):
         """Launch shape of the persistent SAT kernel (b3amd_set_sat_tuning)."""
-        _check(self.lib.b3amd_set_sat_tuning(self.handle, int(waves_per_simd), int(max_slots)))
+        _check(self.lib.b3amd_set_sat_tuning(self.handle, int(waves_per_simd), int(tail_chunk)))
 
     def sat_tuning(self):
         w, k = ctypes.c_uint32(), ctypes.c_uint32()
@@ -561,7 +561,7 @@ class Context:
 
     def autotune(self, n, d_pairs, d_xfs, d_out, repeats=2):
         """Times every SAT launch shape on a device batch, keeps the fastest; returns
-        ((waves_per_simd, max_slots), mean ms per batch)."""
+        ((waves_per_simd, tail_chunk), mean ms per batch)."""
         ms = ctypes.c_float()
         _check(self.lib.b3amd_autotune(self.handle, n, d_pairs.ptr, d_xfs.ptr, d_out.ptr, repeats, ctypes.byref(ms)))
         return self.sat_tuning(), ms.value

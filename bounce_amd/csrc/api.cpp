@@ -270,13 +270,15 @@ const char* pack_hull(const b3amd_hull_pool* pool, uint32_t h, std::vector<uint3
         float* r = gm + 12 * k;
         r[0] = -U[0], r[1] = -U[1], r[2] = -U[2], r[3] = -Vn[0];
         r[4] = -Vn[1], r[5] = -Vn[2], r[6] = -E[0], r[7] = -E[1];
-        r[8] = -E[2], r[9] = sa, r[10] = 0.0f, r[11] = 0.0f;
+        r[8] = -E[2], r[9] = P[0], r[10] = P[1], r[11] = P[2];
         float* c = gc + 4 * k;
         c[0] = -m[0], c[1] = -m[1], c[2] = -m[2], c[3] = ca;
-        // Packed halves: block k / 32, word c * 16 + (k % 16), half (k % 32) / 16.
-        const float comp[5] = {-m[0], -m[1], -m[2], ca, sa};
+        // Halves: block k / 32 holds (-m, cos a) of edge k at half 4 (k % 32) and sin a at
+        // half 128 + k % 32.
         _Float16* blk = gf + 160 * (k / 32);
-        for (uint32_t q = 0; q < 5; ++q) blk[2 * (16 * q + k % 16) + (k % 32) / 16] = (_Float16)comp[q];
+        const float comp[4] = {-m[0], -m[1], -m[2], ca};
+        for (uint32_t q = 0; q < 4; ++q) blk[4 * (k % 32) + q] = (_Float16)comp[q];
+        blk[128 + k % 32] = (_Float16)sa;
     }
     *layoutOut = L;
     return nullptr;
@@ -858,19 +860,19 @@ int b3amd_update_contacts(b3amd_context* ctx, uint32_t body_count, const uint32_
                          nullptr, &bodies);
 }
 
-int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t max_slots) {
+int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t tail_chunk) {
     if (!ctx) return fail(B3AMD_ERR_ARGUMENT, "null context");
     if (waves_per_simd > 32) return fail(B3AMD_ERR_ARGUMENT, "waves_per_simd %u > 32", waves_per_simd);
-    if (max_slots < 1 || max_slots > 3) return fail(B3AMD_ERR_ARGUMENT, "max_slots %u outside 1..3", max_slots);
+    if (tail_chunk > 64) return fail(B3AMD_ERR_ARGUMENT, "tail_chunk %u outside 0..64", tail_chunk);
     ctx->tuning.wavesPerSimd = waves_per_simd;
-    ctx->tuning.maxSlots = max_slots;
+    ctx->tuning.tailChunk = tail_chunk;
     return B3AMD_OK;
 }
 
-int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* max_slots) {
-    if (!ctx || !waves_per_simd || !max_slots) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* tail_chunk) {
+    if (!ctx || !waves_per_simd || !tail_chunk) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     *waves_per_simd = ctx->tuning.wavesPerSimd;
-    *max_slots = ctx->tuning.maxSlots;
+    *tail_chunk = ctx->tuning.tailChunk;
     return B3AMD_OK;
 }
 
@@ -887,9 +889,11 @@ int b3amd_autotune(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, cons
     SatTuning best = saved;
     float bestMs = -1.0f;
     int rc = B3AMD_OK;
-    for (uint32_t slots = 1; slots <= 3 && rc == B3AMD_OK; ++slots) {
+    static const uint32_t kChunks[] = {4, 8, 16, 32};
+    for (uint32_t chunk : kChunks) {
+        if (rc != B3AMD_OK) break;
         for (uint32_t w : kWaves) {
-            ctx->tuning = SatTuning{w, slots};
+            ctx->tuning = SatTuning{w, chunk};
             // One untimed batch, then the timed ones; no feature caches, so every hull
             // pair runs the full SAT each time.
             if ((rc = b3amd_collide_device(ctx, n, d_pairs, d_xfs, nullptr, nullptr, d_out))) break;
@@ -963,15 +967,32 @@ int b3amd_hull_edge_caps(const b3amd_hull_pool* pool, uint32_t hull, float* caps
     HullLayout L{};
     if (const char* err = pack_hull(pool, hull, words, &L)) return fail(B3AMD_ERR_ARGUMENT, err);
     const float* fw = reinterpret_cast<const float*>(words.data());
-    const float* gm = fw + 4 * L.gaussOff();
     const float* gc = fw + 4 * L.capOff();
+    const uint32_t* d = pool->desc + 6 * (size_t)hull;
+    const float* verts = pool->vertices + 3 * (size_t)d[0];
+    const uint32_t* edges = pool->edges + 4 * (size_t)d[2];
+    const float* planes = pool->planes + 4 * (size_t)d[4];
+    double posScale = 0.0;
+    for (uint32_t i = 0; i < L.V; ++i) {
+        const double q = std::sqrt((double)verts[3 * i] * verts[3 * i] + (double)verts[3 * i + 1] * verts[3 * i + 1] +
+                                   (double)verts[3 * i + 2] * verts[3 * i + 2]);
+        posScale = q > posScale ? q : posScale;
+    }
     for (uint32_t k = 0; k < L.E(); ++k) {
-        // The record holds -m and cos (cap unit) and sin (Gauss-map word 9).
+        // The record holds -m and cos (cap unit); sin is stored only as a half, so it is
+        // recomputed here as the packing computed it.
+        const uint32_t* a = edges + 8 * k;
+        const uint32_t* b = a + 4;
+        const float* P = verts + 3 * a[0];
+        const float* Q = verts + 3 * b[0];
+        const float E[3] = {Q[0] - P[0], Q[1] - P[1], Q[2] - P[2]};
+        float m[3], ca, sa;
+        arc_cap(planes + 4 * a[2], planes + 4 * b[2], E, posScale, m, &ca, &sa);
         capsOut[5 * k + 0] = -gc[4 * k + 0];
         capsOut[5 * k + 1] = -gc[4 * k + 1];
         capsOut[5 * k + 2] = -gc[4 * k + 2];
         capsOut[5 * k + 3] = gc[4 * k + 3];
-        capsOut[5 * k + 4] = gm[12 * k + 9];
+        capsOut[5 * k + 4] = sa;
     }
     return B3AMD_OK;
 }

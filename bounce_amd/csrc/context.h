@@ -105,7 +105,7 @@ struct b3amd_context {
     SatResult* dSat = nullptr;
     uint32_t* dCounter = nullptr;  // tail counter of the persistent SAT kernel
     uint32_t scratchCap = 0;
-    SatTuning tuning{0u, 3u};
+    SatTuning tuning{0u, 0u};
     Counters* dCounters = nullptr;  // statistics totals (b3amd_get_counters)
     void* noAllocScratch = nullptr; // this device's copy of the no-allocation scratch array
     bool counting = false;

@@ -11,11 +11,11 @@
 //                 of half-edge h is h ^ 1 (required by sat.cpp:123-128, checked at upload)
 //   then          F u16 first half-edge per face (b3Face::edge)
 //   then          the negated Gauss map, 3 units per unique edge k padded to a multiple
-//                 of 4 edges: {-U.xyz, -V.x}, {-V.yz, -E.xy}, {-E.z, sin a, 0, 0} with
-//                 U, V the normals of faces(2k), face(2k+1) and E = origin(2k+1) -
-//                 origin(2k), in the hull's own frame.  -U, -V, -E are exactly the
-//                 operands hull 2 contributes to b3IsMinkowskiFace in the edge query
-//                 (sat.cpp:131-156).
+//                 of 4 edges: {-U.xyz, -V.x}, {-V.yz, -E.xy}, {-E.z, P.xyz} with U, V the
+//                 normals of faces(2k), face(2k+1), P = origin(2k) and E = origin(2k+1) -
+//                 P, in the hull's own frame.  -U, -V, -E are exactly the operands hull 2
+//                 contributes to b3IsMinkowskiFace in the edge query (sat.cpp:131-156),
+//                 and P, -E its edge for b3Project (sat.cpp:158).
 //   then          one unit per (padded) edge, {-m.xyz, cos a}: the bounding cap of the
 //                 edge's Gauss-map arc from U to V, centre m = (U + V) / |U + V| and angular
 //                 radius a = angle(U, V) / 2 + 0.01 rad, which contains the arc with a
@@ -23,11 +23,11 @@
 //                 test; the SAT kernel tests caps first and runs the exact test only for
 //                 pairs whose caps overlap.  (-m, a) is the cap of the negated arc, and a
 //                 hull used as hull 1 reads m back as the negation.  Padding: cos a = 1e30.
-//   then          the same caps as packed halves for the cap test of a hull used as hull
-//                 2: per block of 32 (padded) edges, 20 units = 80 words, word c * 16 + t
-//                 holding component c (-m.x, -m.y, -m.z, cos a, sin a) of edges 32b + t
-//                 (low half) and 32b + 16 + t (high half), rounded to nearest; padding
-//                 edges are zero (the kernel masks them).
+//   then          the same caps as halves (rounded to nearest), the matrix-core operands of
+//                 the cap test: per block b of 32 (padded) edges, 20 units = 320 bytes:
+//                 bytes 8t .. 8t + 7 hold (-m.x, -m.y, -m.z, cos a) of edge 32b + t and
+//                 bytes 256 + 2t .. 256 + 2t + 1 its sin a; padding edges are zero (the
+//                 kernel masks them).
 #ifndef B3AMD_DEVICE_HULL_H
 #define B3AMD_DEVICE_HULL_H
 

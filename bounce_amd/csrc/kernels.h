@@ -251,8 +251,8 @@ struct SatArgs : PairInputs {
 
 // Launch shape of the persistent SAT kernel (chosen by the autotuner).
 struct SatTuning {
-    uint32_t wavesPerSimd;   // persistent waves per SIMD and register budget, 0 = 5
-    uint32_t maxSlots;       // edge-query register slots per lane: 1, 2 or 3
+    uint32_t wavesPerSimd;   // persistent waves per SIMD and register budget, 0 = 4
+    uint32_t tailChunk;      // pairs per grab from the list's dynamic tail, 0 = 8
 };
 
 struct ContactArgs : PairInputs {

@@ -267,12 +267,12 @@ int b3amd_synchronize(b3amd_context* ctx);
 /* ---- launch shape of the hull SAT (no reference counterpart: the reference runs the
  * queries of sat.cpp:33-173 one pair at a time on one thread) ----
  * The SAT kernel is one persistent grid over the batch's hull pairs.  waves_per_simd
- * (0 = 5, at most 32) is the number of resident waves per SIMD the grid is sized for; it
+ * (0 = 4, at most 32) is the number of resident waves per SIMD the grid is sized for; it
  * also picks the kernel's register budget (the variant built for min(max(w, 4), 6) waves).
- * max_slots (1..3) is the number of hull-1 edges each lane holds in registers during the
- * edge query. */
-int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t max_slots);
-int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* max_slots);
+ * Each wave takes a static share of the pair list, then the last sixteenth goes out from a
+ * device counter tail_chunk pairs at a time (1..64, 0 = 8). */
+int b3amd_set_sat_tuning(b3amd_context* ctx, uint32_t waves_per_simd, uint32_t tail_chunk);
+int b3amd_get_sat_tuning(const b3amd_context* ctx, uint32_t* waves_per_simd, uint32_t* tail_chunk);
 /* Times every launch shape on a device-resident batch (as b3amd_collide_device, without
  * feature caches or sweeps, so every hull pair runs the full SAT), `repeats` batches per
  * shape, keeps the fastest in the context and returns its mean batch time in best_ms

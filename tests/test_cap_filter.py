@@ -1,13 +1,17 @@
-"""The SAT kernel's edge query tests the bounding caps of two Gauss-map arcs on packed
-halves before the exact b3IsMinkowskiFace test (sat.cpp:60-71): five half fused
-multiply-adds of s1 s2 - c1 c2 + M1.M2 + 2^-7 and the sign of the result.  The kernel
-relies on two facts, checked here on the CPU by emulating that arithmetic (each half
-fused multiply-add is exact in float64 and rounded once to float16) on the caps the
-library itself stores at upload (b3amd_hull_edge_caps):
+"""The SAT kernel's edge query tests the bounding caps of two Gauss-map arcs on the
+matrix cores before the exact b3IsMinkowskiFace test (sat.cpp:60-71): a half-precision
+v_mfma_f32_32x32x8_f16 of the rows (M1, -cos a1, sin a1, 2^-8) against the columns
+(-m2, cos a2, sin a2, 1), i.e. s1 s2 - c1 c2 + M1.M2 + 2^-8, and the sign of the result.
+The kernel relies on two facts, checked here on the CPU by emulating that arithmetic
+(inputs rounded to float16, with and without flushing half denormals, products exact,
+the sum rounded to float32) on the caps the library itself stores at upload
+(b3amd_hull_edge_caps):
 
-* rounding the inputs to halves and the five roundings change the left-hand side by at
-  most 14 * 2^-11 < 2^-7, so the test with 2^-7 added accepts every pair of caps that
-  meet;
+* rounding the inputs to halves changes each product by at most 2^-10 of its magnitude
+  and the products' magnitudes sum to at most 3, flushed half denormals (below 2^-14)
+  move each of the five terms by less than 2^-14, and the float32 sum adds under 2^-19:
+  together under 3.3e-3 < 2^-8, so the test with 2^-8 added accepts every pair of caps
+  that meet;
 * the caps contain the region the reference's test can accept, so every edge pair that
   passes the exact test in FP32 and that b3Project scores (both edges at least
   LINEAR_SLOP long, |E1 x E2| >= 0.005 |E1| |E2|, sat.cpp:75-95; the others return
@@ -32,8 +36,8 @@ from bounce_amd.shapes import box_hull, identity_xf, polyhedron
 from bounce_amd.types import HullPool, ptr
 from bounce_amd.workloads import gen_hull_pairs, gen_hull_pool
 
-SLACK = 2.0 ** -7
-U16 = 2.0 ** -11
+SLACK = 2.0 ** -8
+BOUND = 3 * 2.0 ** -10 * (1 + 2.0 ** -12) + 5 * 2.0 ** -14 + 2.0 ** -19
 LINEAR_SLOP = np.float32(0.005)
 
 
@@ -41,8 +45,15 @@ def _h16(x):
     return np.asarray(x, np.float64).astype(np.float16).astype(np.float64)
 
 
-def _fma16(a, b, c):
-    return _h16(a * b + c)
+def _h16_ftz(x):
+    h = _h16(x)
+    return np.where(np.abs(h) < 2.0 ** -14, 0.0, h)
+
+
+def _mfma16(rows, cols, rnd):
+    """sum_k rnd(rows[:, k]) * rnd(cols[:, k]) over the eight terms, float64 products and
+    sum, rounded to float32."""
+    return (rnd(rows) @ rnd(cols).T).astype(np.float32).astype(np.float64)
 
 
 def lib_caps(lib, pool, h):
@@ -78,10 +89,10 @@ def _cross(a, b):
                      a[..., 0] * b[..., 1] - a[..., 1] * b[..., 0]], -1)
 
 
-def check_pair(pool, caps, a, b, xa, xb):
+def check_pair(pool, caps, a, b, xa, xb, mutate=None):
     """Returns (scored exact hits, all exact hits, cap passes, tests, worst |g - g64|) for the
     edge query of hull a (transform xa) against hull b (xb); asserts that no scored exact
-    hit is rejected by the half cap test."""
+    hit is rejected by the matrix-core cap test."""
     R1, R2 = _rot(xa), _rot(xb)
     R = (R2.T @ R1).astype(np.float32)  # hull 1's rotation in hull 2's frame
     t = (R2.T @ (xa[9:12].astype(np.float64) - xb[9:12])).astype(np.float32)
@@ -110,29 +121,37 @@ def check_pair(pool, caps, a, b, xa, xb):
     M2 = -m2
     g64 = SLACK + s1[:, None].astype(np.float64) * s2[None] - c1[:, None].astype(np.float64) * c2[None] + \
         M1.astype(np.float64) @ M2.T.astype(np.float64)
-    g = _fma16(_h16(s1)[:, None], _h16(s2)[None], SLACK)
-    g = _fma16(_h16(-c1)[:, None], _h16(c2)[None], g)
-    for k in range(3):
-        g = _fma16(_h16(M1[:, k])[:, None], _h16(M2[:, k])[None], g)
-    ok = g >= 0.0
+    na, nb = len(c1), len(c2)
+    rows = np.zeros((na, 8))
+    cols = np.zeros((nb, 8))
+    rows[:, 0:3], rows[:, 3], rows[:, 4], rows[:, 5], rows[:, 7] = M1, -c1, s1, SLACK, 1.0
+    cols[:, 0:3], cols[:, 3], cols[:, 4], cols[:, 5], cols[:, 6] = M2, c2, s2, 1.0, 1.0
+    if mutate == "drop_sin":  # a planted mistake: the s1 s2 term dropped
+        rows[:, 4] = 0.0
+    elif mutate == "swap_arc":  # a planted mistake: hull 1's centre not negated back
+        rows[:, 0:3] = -M1
+    g_keep = _mfma16(rows, cols, _h16)
+    g_ftz = _mfma16(rows, cols, _h16_ftz)
+    ok = (g_keep >= 0.0) & (g_ftz >= 0.0)
+    g = g_keep if np.abs(g_keep - g64).max() >= np.abs(g_ftz - g64).max() else g_ftz
     bad = scored & ~ok
-    assert not bad.any(), "hulls %d/%d: %d scored exact hits fail the half cap test" % (a, b, int(bad.sum()))
+    assert not bad.any(), "hulls %d/%d: %d scored exact hits fail the matrix-core cap test" % (a, b, int(bad.sum()))
     return int(scored.sum()), int(exact.sum()), int(ok.sum()), exact.size, float(np.abs(g - g64).max())
 
 
-def _sweep(lib, pool, pairs, xfs):
+def _sweep(lib, pool, pairs, xfs, mutate=None):
     caps = {h: lib_caps(lib, pool, h) for h in np.unique(pairs)}
     tot = np.zeros(4, np.int64)
     worst = 0.0
     for (a, b), x in zip(pairs, xfs):
-        s, e, p, n, w = check_pair(pool, caps, int(a), int(b), x[0], x[1])
+        s, e, p, n, w = check_pair(pool, caps, int(a), int(b), x[0], x[1], mutate)
         tot += (s, e, p, n)
         worst = max(worst, w)
-    assert worst <= 14 * U16, worst
+    assert worst <= BOUND < SLACK, worst
     return tot
 
 
-def test_half_cap_filter_is_conservative_on_headline_pool(lib):
+def test_cap_filter_is_conservative_on_headline_pool(lib):
     """1,200 random pairs of the headline mix (~12 M edge pairs)."""
     pool = gen_hull_pool(lib, 77, 1024, 8, 64, 0.25, 2.0)
     pairs, xfs = gen_hull_pairs(lib, 78, pool, 1200, -0.0127, 0.05)
@@ -223,11 +242,23 @@ def degenerate_pairs(seed=5, per_kind=24):
     return np.array(pairs, np.uint32), np.array(xfs, np.float32)
 
 
-def test_half_cap_filter_is_conservative_on_degenerate_hulls(lib):
+def test_cap_filter_is_conservative_on_degenerate_hulls(lib):
     pool = degenerate_pool()
     pairs, xfs = degenerate_pairs()
     scored, hits, passed, tests = _sweep(lib, pool, pairs, xfs)
     assert scored > 2000  # the sweep does reach scored hits on these hulls
+
+
+@pytest.mark.parametrize("mutate", ["drop_sin", "swap_arc"])
+def test_cap_filter_check_catches_planted_mistakes(lib, mutate):
+    """The containment check is not vacuous: each planted mistake in the emulated filter
+    rejects scored exact hits on the degenerate sweep.  (The slack itself is pinned by the
+    rounding bound that every sweep asserts, not by a hit: the caps' own angular margin
+    also covers the rounding on these inputs.)"""
+    pool = degenerate_pool()
+    pairs, xfs = degenerate_pairs()
+    with pytest.raises(AssertionError, match="scored exact hits fail"):
+        _sweep(lib, pool, pairs, xfs, mutate)
 
 
 @pytest.mark.parametrize("sides", [4, 6])

@@ -179,19 +179,19 @@ def test_gpu_full_size_properties(ctx, lib, oracle):
     assert_manifolds_equal(again, got, "repeat")
 
 
-@pytest.mark.parametrize("waves,slots", [(6, 3), (5, 2), (4, 1), (0, 3), (2, 3)])
-def test_gpu_sat_launch_shapes_agree(ctx, lib, oracle, waves, slots):
-    """Every launch shape of the persistent SAT kernel (resident waves, register slots,
-    hence edge-query chunking) gives the oracle's manifolds bit for bit, for hulls of
-    8-64 points and beyond 192 unique edges (several register chunks)."""
-    say("case: SAT launch shape waves=%d slots=%d" % (waves, slots))
+@pytest.mark.parametrize("waves,chunk", [(6, 8), (5, 4), (4, 1), (0, 0), (2, 64)])
+def test_gpu_sat_launch_shapes_agree(ctx, lib, oracle, waves, chunk):
+    """Every launch shape of the persistent SAT kernel (resident waves and register budget,
+    tail chunk) gives the oracle's manifolds bit for bit, for hulls of 8-64 points and
+    beyond 192 unique edges (several row-block passes and hull-1 records read from HBM)."""
+    say("case: SAT launch shape waves=%d chunk=%d" % (waves, chunk))
     try:
-        ctx.set_sat_tuning(waves, slots)
+        ctx.set_sat_tuning(waves, chunk)
         for seed, nmin, nmax, n in ((707, 8, 64, 3000), (808, 70, 128, 300)):
             pool, shapes, pairs, xfs = hull_workload(lib, seed, 60, n, nmin=nmin, nmax=nmax)
             assert_manifolds_equal(run(ctx, pool, shapes, pairs, xfs), oracle.collide(pool, shapes, pairs, xfs))
     finally:
-        ctx.set_sat_tuning(0, 3)
+        ctx.set_sat_tuning(0, 0)
 
 
 def test_gpu_autotune(ctx, lib, oracle):
@@ -206,16 +206,16 @@ def test_gpu_autotune(ctx, lib, oracle):
     try:
         dp.upload(pairs)
         dx.upload(xfs)
-        (waves, slots), ms = ctx.autotune(n, dp, dx, do, repeats=1)
-        say("autotune: waves=%d slots=%d %.3f ms" % (waves, slots, ms))
-        assert 1 <= slots <= 3 and 0 <= waves <= 32 and ms > 0
+        (waves, chunk), ms = ctx.autotune(n, dp, dx, do, repeats=1)
+        say("autotune: waves=%d chunk=%d %.3f ms" % (waves, chunk, ms))
+        assert 1 <= chunk <= 64 and 0 <= waves <= 32 and ms > 0
         idx = np.arange(0, n, 7)
         got = ctx.collide(pairs[idx], xfs[idx])
         assert_manifolds_equal(got, oracle.collide(pool, shapes, pairs[idx], xfs[idx]))
     finally:
         for b in (dp, dx, do):
             b.free()
-        ctx.set_sat_tuning(0, 3)
+        ctx.set_sat_tuning(0, 0)
 
 
 def test_gpu_graph_replay_matches_eager(ctx, lib):
