@@ -89,6 +89,17 @@ def load_library():
         "b3amd_server_replay": (i32, [vp, u32, vp, vp, vp, vp]),
         "b3amd_server_get_stats": (i32, [vp, vp]),
         "b3amd_server_configure": (i32, [vp, u32, u32]),
+        "b3amd_broadphase_create": (i32, [vp, ctypes.POINTER(vp)]),
+        "b3amd_broadphase_destroy": (i32, [vp]),
+        "b3amd_broadphase_create_proxies": (i32, [vp, u32, vp, vp, vp]),
+        "b3amd_broadphase_destroy_proxies": (i32, [vp, u32, vp]),
+        "b3amd_broadphase_touch_proxies": (i32, [vp, u32, vp]),
+        "b3amd_broadphase_move_proxies": (i32, [vp, u32, vp, vp, vp, vp]),
+        "b3amd_broadphase_find_pairs": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(vp)]),
+        "b3amd_broadphase_test_overlap": (i32, [vp, u32, vp, vp]),
+        "b3amd_broadphase_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
+        "b3amd_broadphase_read": (i32, [vp, u32, u32, vp, vp, vp]),
+        "b3amd_shape_aabbs": (i32, [vp, u32, vp, vp, u32, vp, vp, vp, vp]),
         "b3amd_set_counting": (i32, [vp, i32]),
         "b3amd_get_counters": (i32, [vp, vp]),
         "b3amd_reset_counters": (i32, [vp]),
@@ -291,6 +302,116 @@ class Server:
         return {k: getattr(st, k) for k, _ in ServerStats._fields_ if k != "reserved"}
 
 
+class Broadphase:
+    """b3BroadPhase on the device (b3amd_broadphase_*): the reference's proxy ids and
+    FindPairs order, fat AABBs in HBM.  Close it before its Context."""
+
+    def __init__(self, ctx):
+        self.ctx = ctx
+        self.lib = ctx.lib
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_broadphase_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+        self._bufs = {}
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.lib.b3amd_broadphase_destroy(self.handle))
+            self.handle = None
+        for b in self._bufs.values():
+            b.free()
+        self._bufs = {}
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def _buf(self, name, nbytes):
+        b = self._bufs.get(name)
+        if b is None or b.nbytes < nbytes:
+            if b is not None:
+                b.free()
+            b = DeviceBuffer(self.ctx, max(256, int(nbytes)))
+            self._bufs[name] = b
+        return b
+
+    def create(self, aabbs, users=None):
+        """CreateProxy for each AABB [n, 6] in order; returns the proxy ids."""
+        a = np.ascontiguousarray(aabbs, np.float32).reshape(-1, 6)
+        u = None if users is None else np.ascontiguousarray(users, np.uint32)
+        ids = np.zeros(max(1, len(a)), np.uint32)
+        _check(self.lib.b3amd_broadphase_create_proxies(self.handle, len(a), ptr(a), ptr(u), ptr(ids)))
+        return ids[:len(a)]
+
+    def destroy(self, ids):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        _check(self.lib.b3amd_broadphase_destroy_proxies(self.handle, len(ids), ptr(ids)))
+
+    def touch(self, ids):
+        ids = np.ascontiguousarray(ids, np.uint32)
+        _check(self.lib.b3amd_broadphase_touch_proxies(self.handle, len(ids), ptr(ids)))
+
+    def move(self, ids, aabbs, displacements):
+        """MoveProxy for host arrays (uploaded, run, moved flags downloaded)."""
+        ids = np.ascontiguousarray(ids, np.uint32)
+        n = len(ids)
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        a = np.ascontiguousarray(aabbs, np.float32).reshape(n, 6)
+        d = np.ascontiguousarray(displacements, np.float32).reshape(n, 3)
+        bi, ba, bd, bm = (self._buf(k, s) for k, s in (("ids", 4 * n), ("aabbs", 24 * n), ("disp", 12 * n),
+                                                       ("moved", 4 * n)))
+        bi.upload(ids), ba.upload(a), bd.upload(d)
+        self.move_device(n, bi, ba, bd, bm)
+        return bm.download(np.zeros(n, np.uint32))
+
+    def move_device(self, n, d_ids, d_aabbs, d_disps, d_moved=None):
+        _check(self.lib.b3amd_broadphase_move_proxies(self.handle, n, d_ids.ptr, d_aabbs.ptr, d_disps.ptr,
+                                                      None if d_moved is None else d_moved.ptr))
+
+    def find_pairs_device(self):
+        """FindPairs: (count, device address of the [count, 4] pair words)."""
+        n = ctypes.c_uint32()
+        p = ctypes.c_void_p()
+        _check(self.lib.b3amd_broadphase_find_pairs(self.handle, ctypes.byref(n), ctypes.byref(p)))
+        return n.value, p
+
+    def find_pairs(self):
+        """FindPairs: [k, 4] uint32 (proxy1, proxy2, user1, user2) in AddPair order."""
+        n, p = self.find_pairs_device()
+        out = np.zeros((n, 4), np.uint32)
+        if n:
+            _check(self.lib.b3amd_copy_to_host(self.ctx.handle, ptr(out), p, out.nbytes))
+        return out
+
+    def test_overlap(self, proxy_pairs):
+        pp = np.ascontiguousarray(proxy_pairs, np.uint32).reshape(-1, 2)
+        n = len(pp)
+        if n == 0:
+            return np.zeros(0, np.uint32)
+        bp, bo = self._buf("tp", 8 * n), self._buf("to", 4 * n)
+        bp.upload(pp)
+        _check(self.lib.b3amd_broadphase_test_overlap(self.handle, n, bp.ptr, bo.ptr))
+        return bo.download(np.zeros(n, np.uint32))
+
+    def info(self):
+        c, cap = ctypes.c_uint32(), ctypes.c_uint32()
+        _check(self.lib.b3amd_broadphase_info(self.handle, ctypes.byref(c), ctypes.byref(cap)))
+        return c.value, cap.value
+
+    def read(self, first=0, count=None):
+        """(fat AABBs [count, 6], user data, flags) of ids first .. first + count - 1."""
+        if count is None:
+            count = self.info()[1] - first
+        a = np.zeros((count, 6), np.float32)
+        u = np.zeros(count, np.uint32)
+        f = np.zeros(count, np.uint32)
+        _check(self.lib.b3amd_broadphase_read(self.handle, first, count, ptr(a), ptr(u), ptr(f)))
+        return a, u, f
+
+
 class MeshContacts:
     """Persistent mesh contacts (b3amd_mesh_contacts_*): b3MeshContact across steps."""
 
@@ -410,6 +531,35 @@ class Context:
         _check(self.lib.b3amd_collide_shape_and_shape(self.handle, ptr(m), ptr(a), int(shape_a), ptr(b), int(shape_b),
                                                       ptr(sweeps), ptr(cache)))
         return m[0]
+
+    def broadphase(self):
+        """A device b3BroadPhase on this context."""
+        return Broadphase(self)
+
+    def shape_aabbs(self, shapes, bodies, body_xfs, body_sweeps=None):
+        """b3Body::SynchronizeShapes' swept AABB and displacement of shape shapes[i] on body
+        bodies[i] (host arrays in and out); with body_sweeps None, the AABB at m_xf."""
+        from .types import SWEEP_DTYPE
+        sh = np.ascontiguousarray(shapes, np.uint32)
+        bo = np.ascontiguousarray(bodies, np.uint32)
+        n = len(sh)
+        x = np.ascontiguousarray(body_xfs, np.float32).reshape(-1, 12)
+        bufs = [DeviceBuffer(self, max(256, a.nbytes)) for a in (sh, bo, x)]
+        for b, a in zip(bufs, (sh, bo, x)):
+            b.upload(a)
+        sw = None
+        if body_sweeps is not None:
+            s_arr = np.ascontiguousarray(body_sweeps, SWEEP_DTYPE)
+            sw = DeviceBuffer(self, max(256, s_arr.nbytes))
+            sw.upload(s_arr)
+        oa, od = DeviceBuffer(self, max(256, 24 * n)), DeviceBuffer(self, max(256, 12 * n))
+        try:
+            _check(self.lib.b3amd_shape_aabbs(self.handle, n, bufs[0].ptr, bufs[1].ptr, len(x), bufs[2].ptr,
+                                              None if sw is None else sw.ptr, oa.ptr, od.ptr))
+            return oa.download(np.zeros((n, 6), np.float32)), od.download(np.zeros((n, 3), np.float32))
+        finally:
+            for b in bufs + [oa, od] + ([sw] if sw is not None else []):
+                b.free()
 
     def cache_alloc(self, count):
         """Device caches in the new-contact state; a DeviceBuffer-like handle (free with

@@ -43,29 +43,6 @@ int fail(int code, const char* fmt, ...) {
 
 }  // namespace b3amd
 
-namespace {
-
-// Brackets one kernel launch with events while profiling is on.
-template <class F>
-int timed(b3amd_context* ctx, hipStream_t s, const char* name, F launch) {
-    hipEvent_t a = nullptr, b = nullptr;
-    if (ctx->profiling) {
-        a = ctx->event();
-        b = ctx->event();
-        if (!a || !b) return fail(B3AMD_ERR_DEVICE, "hipEventCreate failed");
-        HIP_TRY(hipEventRecord(a, s));
-    }
-    hipError_t e = launch();
-    if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
-    if (ctx->profiling) {
-        HIP_TRY(hipEventRecord(b, s));
-        ctx->pending.push_back(PendingTiming{name, a, b});
-    }
-    return B3AMD_OK;
-}
-
-}  // namespace
-
 // Scratch of b3amd_collide_device_noalloc: a file-scope device array, one copy per GPU,
 // holding what ensure_scratch allocates for a batch of up to kNoAllocMaxPairs pairs: the
 // stage / SAT-list / rebuild-list words, the SAT results and the counter words (zeroed by a

@@ -109,6 +109,19 @@ B3_HD Quat qmul(Quat a, Quat b) {  // quat.h:172-179
 }
 B3_HD Quat conj(Quat q) { return Quat{-q.x, -q.y, -q.z, q.w}; }  // quat.h:195-198
 
+// b3QuatMat33 (quat.h:306-318).
+B3_HD M33 quat_mat33(float x, float y, float z, float w) {
+    const float x2 = x + x, y2 = y + y, z2 = z + z;
+    const float xx = x * x2, xy = x * y2, xz = x * z2;
+    const float yy = y * y2, yz = y * z2, zz = z * z2;
+    const float wx = w * x2, wy = w * y2, wz = w * z2;
+    M33 m;
+    m.c0 = V3{1.0f - (yy + zz), xy + wz, xz - wy};
+    m.c1 = V3{xy - wz, 1.0f - (xx + zz), yz + wx};
+    m.c2 = V3{xz + wy, yz - wx, 1.0f - (xx + yy)};
+    return m;
+}
+
 }  // namespace b3amd
 
 #endif

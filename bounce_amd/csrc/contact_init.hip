@@ -23,19 +23,6 @@ namespace {
 __device__ __forceinline__ V3 ld3(const float* p) { return V3{p[0], p[1], p[2]}; }
 __device__ __forceinline__ void st3(float* p, V3 v) { p[0] = v.x, p[1] = v.y, p[2] = v.z; }
 
-// b3QuatMat33 (quat.h:306-318).
-__device__ __forceinline__ M33 quat_mat33(float x, float y, float z, float w) {
-    const float x2 = x + x, y2 = y + y, z2 = z + z;
-    const float xx = x * x2, xy = x * y2, xz = x * z2;
-    const float yy = y * y2, yz = y * z2, zz = z * z2;
-    const float wx = w * x2, wy = w * y2, wz = w * z2;
-    M33 m;
-    m.c0 = V3{1.0f - (yy + zz), xy + wz, xz - wy};
-    m.c1 = V3{xy - wz, 1.0f - (xx + zz), yz + wx};
-    m.c2 = V3{xz + wy, yz - wx, 1.0f - (xx + yy)};
-    return m;
-}
-
 __device__ __forceinline__ M33 ld33(const float* a) {
     M33 m;
     m.c0 = ld3(a);

@@ -111,6 +111,8 @@ struct b3amd_context {
     bool counting = false;
     void* dIn = nullptr;
     uint64_t dInCap = 0;
+    // Status words of entry points that check device-side inputs (16 words, on demand).
+    uint32_t* dStatus = nullptr;
     // Shape pairs of the contacts of a world step (b3amd_update_contacts).
     void* dGather = nullptr;
     uint32_t gatherCap = 0;
@@ -198,6 +200,26 @@ struct b3amd_graph {
 
 
 namespace b3amd {
+
+// Brackets one kernel launch (launch() returns its hipError_t) with events while profiling
+// is on; the timings go to b3amd_kernel_stats.
+template <class F>
+int timed(b3amd_context* ctx, hipStream_t s, const char* name, F launch) {
+    hipEvent_t a = nullptr, b = nullptr;
+    if (ctx->profiling) {
+        a = ctx->event();
+        b = ctx->event();
+        if (!a || !b) return fail(B3AMD_ERR_DEVICE, "hipEventCreate failed");
+        HIP_TRY(hipEventRecord(a, s));
+    }
+    hipError_t e = launch();
+    if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "launch of %s: %s", name, hipGetErrorString(e));
+    if (ctx->profiling) {
+        HIP_TRY(hipEventRecord(b, s));
+        ctx->pending.push_back(PendingTiming{name, a, b});
+    }
+    return B3AMD_OK;
+}
 
 // Grows the context's per-batch scratch to n pairs.
 int ensure_scratch(b3amd_context* ctx, uint32_t n);

@@ -56,6 +56,13 @@ VELOCITY_MANIFOLD_DTYPE = np.dtype([
     ("reserved", "<u4", (2,))])
 assert VELOCITY_MANIFOLD_DTYPE.itemsize == 304
 
+# The step before the narrowphase: b3AABB3 (aabb3.h:26-28) and b3Sweep (transform.h:202-222).
+AABB_DTYPE = np.dtype([("lower", "<f4", (3,)), ("upper", "<f4", (3,))])
+assert AABB_DTYPE.itemsize == 24
+SWEEP_DTYPE = np.dtype([("localCenter", "<f4", (3,)), ("orientation0", "<f4", (4,)), ("worldCenter0", "<f4", (3,)),
+                        ("t0", "<f4"), ("orientation", "<f4", (4,)), ("worldCenter", "<f4", (3,))])
+assert SWEEP_DTYPE.itemsize == 72
+
 SPHERE, CAPSULE, HULL, MESH = 0, 1, 2, 3
 # Manifold status bits (B3AMD_FLAG_* of bounce_amd.h).
 FLAG_POLY_OVERFLOW, FLAG_INVALID_PAIR, FLAG_TRIANGLE_OVERFLOW = 1, 2, 4

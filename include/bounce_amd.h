@@ -233,6 +233,65 @@ int b3amd_test_overlap(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, co
 int b3amd_test_overlap_device(b3amd_context* ctx, uint32_t n, const uint32_t* d_pairs, const b3amd_transform* d_xfs,
                               b3amd_convex_cache* d_caches, uint32_t* d_overlap);
 
+/* ---- the step before the narrowphase: broadphase ----
+ * b3BroadPhase (broad_phase.h:36-119, broad_phase.cpp) with the fat AABBs in HBM.  Proxy ids
+ * are the ones b3BroadPhase::CreateProxy returns for the same sequence of creates and
+ * destroys (b3DynamicTree node ids, dynamic_tree.cpp:41-114; moves do not change them), and
+ * find_pairs reports exactly the pairs FindPairs hands to b3ContactManager::AddPair, in the
+ * same order (broad_phase.h:160-208: sorted by (proxy1, proxy2), unique).  Destroy a
+ * broadphase before its context. */
+typedef struct { b3amd_vec3 lower, upper; } b3amd_aabb;                    /* b3AABB3 aabb3.h:26-28 */
+/* b3Sweep (transform.h:202-222), field for field. */
+typedef struct {
+    b3amd_vec3 localCenter;
+    b3amd_quat orientation0;
+    b3amd_vec3 worldCenter0;
+    float t0;
+    b3amd_quat orientation;
+    b3amd_vec3 worldCenter;
+} b3amd_sweep;
+typedef struct b3amd_broadphase b3amd_broadphase;
+int b3amd_broadphase_create(b3amd_context* ctx, b3amd_broadphase** out);
+int b3amd_broadphase_destroy(b3amd_broadphase* bp);
+/* CreateProxy (broad_phase.cpp:78-90) for each AABB in order: the fat AABB is the AABB
+ * extended by b3_aabbExtension; the proxy is buffered as moved.  user_data[i] (null: i) is
+ * what find_pairs reports for the proxy (the reference's userData, a b3Shape*). */
+int b3amd_broadphase_create_proxies(b3amd_broadphase* bp, uint32_t n, const b3amd_aabb* aabbs,
+                                    const uint32_t* user_data, uint32_t* proxy_ids);
+/* DestroyProxy (broad_phase.cpp:92-97) for each id in order. */
+int b3amd_broadphase_destroy_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids);
+/* TouchProxy (broad_phase.cpp:151-154). */
+int b3amd_broadphase_touch_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids);
+/* MoveProxy (broad_phase.cpp:99-149) on device arrays, enqueued on the context stream:
+ * d_moved[i] (may be null) = whether proxy d_proxy_ids[i] moved.  An id that is not live
+ * moves nothing and makes the next find_pairs return B3AMD_ERR_ARGUMENT. */
+int b3amd_broadphase_move_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* d_proxy_ids,
+                                  const b3amd_aabb* d_aabbs, const b3amd_vec3* d_displacements, uint32_t* d_moved);
+/* FindPairs (broad_phase.h:160-208): *pair_count pairs at *d_pairs (device memory of the
+ * broadphase, valid until the next call), 4 words each: proxy1 < proxy2 and their user
+ * data, in the order AddPair receives them; the move buffer is cleared.  Synchronizes. */
+int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, const uint32_t** d_pairs);
+/* TestOverlap (broad_phase.cpp:73-76) of proxy pairs d_proxy_pairs[2i], [2i+1]: 1, 0, or
+ * B3AMD_OVERLAP_INVALID for an id that is not live. */
+int b3amd_broadphase_test_overlap(b3amd_broadphase* bp, uint32_t n, const uint32_t* d_proxy_pairs,
+                                  uint32_t* d_overlap);
+/* Live proxies and the current id capacity (ids are < id_capacity). */
+int b3amd_broadphase_info(const b3amd_broadphase* bp, uint32_t* proxy_count, uint32_t* id_capacity);
+/* GetAABB / GetUserData for ids first .. first+count-1 (host copies; flags bit 0 live, bit 1
+ * moved; any pointer may be null). */
+int b3amd_broadphase_read(b3amd_broadphase* bp, uint32_t first, uint32_t count, b3amd_aabb* aabbs,
+                          uint32_t* user_data, uint32_t* flags);
+/* The AABB b3Body::SynchronizeShapes moves each shape's proxy to (body.cpp:181-202): shape
+ * d_shapes[i] of the context's table on body d_bodies[i] (< body_count); the shape's ComputeAABB
+ * (sphere_shape.cpp:53-59, capsule_shape.cpp:130-137, hull_shape.cpp:162-166) at the body's
+ * m_xf combined with the one at m_sweep.GetTransform(0) (transform.h:224-234), and the
+ * displacement m_xf.position - that transform's position.  With d_body_sweeps null: the
+ * AABB at m_xf alone (b3Body::CreateShape, body.cpp:110-115) and zero displacement.
+ * d_displacements may be null.  Synchronizes (validates the shape entries). */
+int b3amd_shape_aabbs(b3amd_context* ctx, uint32_t n, const uint32_t* d_shapes, const uint32_t* d_bodies,
+                      uint32_t body_count, const b3amd_transform* d_body_xfs, const b3amd_sweep* d_body_sweeps,
+                      b3amd_aabb* d_aabbs, b3amd_vec3* d_displacements);
+
 /* b3amd_collide_device without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops
  * and harnesses that forbid hipMalloc).  The batch's scratch lives in a file-scope device

@@ -2420,3 +2420,567 @@ void b3o_contact_init(uint32_t n, uint32_t M, const uint32_t* bodies, const b3o_
         }
     }
 }
+
+/* ---------------------------------------------------------------- the step before: broadphase */
+/* b3DynamicTree (dynamic_tree.cpp) as the reference builds it: node pool with its free list
+ * (AllocateNode / FreeNode / AddToFreeList :41-100), surface-area insertion (FindBest
+ * :128-197, InsertLeaf :199-249), removal (RemoveLeaf :251-292), no rebalancing; the AABB
+ * query with an explicit stack (dynamic_tree.h:150-182).  b3BroadPhase (broad_phase.cpp,
+ * broad_phase.h:160-208) on top of it. */
+#define B3O_NULL_NODE 0xFFFFFFFFu
+typedef struct {
+    AABB aabb;
+    uint32_t user;
+    uint32_t parent; /* parent, or next free node while free (the reference's union) */
+    uint32_t child1, child2;
+    int32_t height;
+} DNode;
+
+struct b3o_broadphase {
+    DNode* nodes;
+    uint32_t root, nodeCount, nodeCapacity, freeList;
+    uint32_t proxyCount;
+    uint32_t* move;
+    uint32_t moveCount, moveCapacity;
+    uint32_t* pairs; /* [2 * pairCapacity] */
+    uint32_t pairCount, pairCapacity;
+    uint32_t queryId;
+    uint32_t* unique; /* the last FindPairs' reported pairs, 4 words each */
+    uint32_t uniqueCount, uniqueCapacity;
+};
+
+static float aabb_area(const AABB* a) {                                                   /* aabb3.h:106-133 */
+    float w = a->hi.x - a->lo.x, h = a->hi.y - a->lo.y, d = a->hi.z - a->lo.z;
+    return 2.0f * (w * d + w * h + d * h);
+}
+static AABB aabb_combine(const AABB* a, const AABB* b) {                                 /* aabb3.h:238-244 */
+    AABB r;
+    r.lo = vmin(a->lo, b->lo);
+    r.hi = vmax(a->hi, b->hi);
+    return r;
+}
+static int aabb_contains(const AABB* a, const AABB* b) {                                 /* aabb3.h:156-167 */
+    return a->lo.x <= b->lo.x && b->lo.x <= a->hi.x && a->lo.y <= b->lo.y && b->lo.y <= a->hi.y &&
+           a->lo.z <= b->lo.z && b->lo.z <= a->hi.z && a->lo.x <= b->hi.x && b->hi.x <= a->hi.x &&
+           a->lo.y <= b->hi.y && b->hi.y <= a->hi.y && a->lo.z <= b->hi.z && b->hi.z <= a->hi.z;
+}
+static int dleaf(const DNode* n) { return n->child1 == B3O_NULL_NODE; }
+
+static void dt_add_to_free_list(b3o_broadphase* t, uint32_t node) {                    /* dynamic_tree.cpp:87-100 */
+    for (uint32_t i = node; i < t->nodeCapacity - 1; ++i) {
+        t->nodes[i].parent = i + 1;
+        t->nodes[i].height = -1;
+    }
+    t->nodes[t->nodeCapacity - 1].parent = B3O_NULL_NODE;
+    t->nodes[t->nodeCapacity - 1].height = -1;
+    t->freeList = node;
+}
+
+static uint32_t dt_allocate(b3o_broadphase* t) {                                       /* dynamic_tree.cpp:41-75 */
+    if (t->freeList == B3O_NULL_NODE) {
+        t->nodeCapacity *= 2;
+        t->nodes = (DNode*)realloc(t->nodes, sizeof(DNode) * t->nodeCapacity);
+        memset(t->nodes + t->nodeCount, 0, sizeof(DNode) * (t->nodeCapacity - t->nodeCount));
+        dt_add_to_free_list(t, t->nodeCount);
+    }
+    uint32_t node = t->freeList;
+    t->freeList = t->nodes[node].parent;
+    t->nodes[node].parent = B3O_NULL_NODE;
+    t->nodes[node].child1 = B3O_NULL_NODE;
+    t->nodes[node].child2 = B3O_NULL_NODE;
+    t->nodes[node].height = 0;
+    t->nodes[node].user = 0;
+    ++t->nodeCount;
+    return node;
+}
+
+static void dt_free(b3o_broadphase* t, uint32_t node) {                                /* dynamic_tree.cpp:77-84 */
+    t->nodes[node].parent = t->freeList;
+    t->nodes[node].height = -1;
+    t->freeList = node;
+    --t->nodeCount;
+}
+
+static uint32_t dt_find_best(const b3o_broadphase* t, const AABB* leaf) {              /* dynamic_tree.cpp:128-197 */
+    uint32_t index = t->root;
+    while (!dleaf(t->nodes + index)) {
+        const DNode* n = t->nodes + index;
+        float branchArea = aabb_area(&n->aabb);
+        AABB comb = aabb_combine(leaf, &n->aabb);
+        float combinedArea = aabb_area(&comb);
+        float branchCost = 2.0f * combinedArea;
+        float inheritanceCost = 2.0f * (combinedArea - branchArea);
+        uint32_t c1 = n->child1, c2 = n->child2;
+        float cost1, cost2;
+        AABB a1 = aabb_combine(leaf, &t->nodes[c1].aabb);
+        if (dleaf(t->nodes + c1)) cost1 = aabb_area(&a1);
+        else cost1 = (aabb_area(&a1) - aabb_area(&t->nodes[c1].aabb)) + inheritanceCost;
+        AABB a2 = aabb_combine(leaf, &t->nodes[c2].aabb);
+        if (dleaf(t->nodes + c2)) cost2 = aabb_area(&a2);
+        else cost2 = (aabb_area(&a2) - aabb_area(&t->nodes[c2].aabb)) + inheritanceCost;
+        if (branchCost < cost1 && branchCost < cost2) break;
+        index = cost1 < cost2 ? c1 : c2;
+    }
+    return index;
+}
+
+static void dt_walk_back(b3o_broadphase* t, uint32_t node) {                           /* dynamic_tree.cpp:294-312 */
+    while (node != B3O_NULL_NODE) {
+        DNode* n = t->nodes + node;
+        int32_t h1 = t->nodes[n->child1].height, h2 = t->nodes[n->child2].height;
+        n->height = 1 + (h1 > h2 ? h1 : h2);
+        n->aabb = aabb_combine(&t->nodes[n->child1].aabb, &t->nodes[n->child2].aabb);
+        node = n->parent;
+    }
+}
+
+static void dt_insert_leaf(b3o_broadphase* t, uint32_t leaf) {                         /* dynamic_tree.cpp:199-249 */
+    if (t->root == B3O_NULL_NODE) {
+        t->root = leaf;
+        t->nodes[leaf].parent = B3O_NULL_NODE;
+        return;
+    }
+    AABB leafAabb = t->nodes[leaf].aabb;
+    uint32_t sibling = dt_find_best(t, &leafAabb);
+    uint32_t oldParent = t->nodes[sibling].parent;
+    uint32_t newParent = dt_allocate(t);
+    DNode* p = t->nodes + newParent;
+    p->parent = oldParent;
+    p->child1 = sibling;
+    t->nodes[sibling].parent = newParent;
+    p->child2 = leaf;
+    t->nodes[leaf].parent = newParent;
+    p->user = 0;
+    p->aabb = aabb_combine(&leafAabb, &t->nodes[sibling].aabb);
+    p->height = t->nodes[sibling].height + 1;
+    if (oldParent != B3O_NULL_NODE) {
+        if (t->nodes[oldParent].child1 == sibling) t->nodes[oldParent].child1 = newParent;
+        else t->nodes[oldParent].child2 = newParent;
+    } else {
+        t->root = newParent;
+    }
+    dt_walk_back(t, newParent);
+}
+
+static void dt_remove_leaf(b3o_broadphase* t, uint32_t leaf) {                         /* dynamic_tree.cpp:251-292 */
+    if (leaf == t->root) {
+        t->root = B3O_NULL_NODE;
+        return;
+    }
+    uint32_t parent = t->nodes[leaf].parent;
+    uint32_t grand = t->nodes[parent].parent;
+    uint32_t sibling = t->nodes[parent].child1 == leaf ? t->nodes[parent].child2 : t->nodes[parent].child1;
+    if (grand != B3O_NULL_NODE) {
+        if (t->nodes[grand].child1 == parent) t->nodes[grand].child1 = sibling;
+        else t->nodes[grand].child2 = sibling;
+        t->nodes[sibling].parent = grand;
+        dt_free(t, parent);
+        dt_walk_back(t, grand);
+    } else {
+        t->root = sibling;
+        t->nodes[sibling].parent = B3O_NULL_NODE;
+        dt_free(t, parent);
+    }
+}
+
+static AABB aabb_of(const b3o_aabb* a) {
+    AABB r;
+    r.lo = v3(a->lower.x, a->lower.y, a->lower.z);
+    r.hi = v3(a->upper.x, a->upper.y, a->upper.z);
+    return r;
+}
+
+static void bp_buffer_move(b3o_broadphase* bp, uint32_t id) {                           /* broad_phase.cpp:41-59 */
+    if (bp->moveCount == bp->moveCapacity) {
+        bp->moveCapacity *= 2;
+        bp->move = (uint32_t*)realloc(bp->move, 4 * (size_t)bp->moveCapacity);
+    }
+    bp->move[bp->moveCount++] = id;
+}
+
+b3o_broadphase* b3o_bp_create(void) {                                                   /* dynamic_tree.cpp:22-35, broad_phase.cpp:21-34 */
+    b3o_broadphase* bp = (b3o_broadphase*)calloc(1, sizeof(b3o_broadphase));
+    bp->root = B3O_NULL_NODE;
+    bp->nodeCapacity = 32;
+    bp->nodes = (DNode*)calloc(bp->nodeCapacity, sizeof(DNode));
+    dt_add_to_free_list(bp, 0);
+    bp->moveCapacity = 16;
+    bp->move = (uint32_t*)calloc(bp->moveCapacity, 4);
+    bp->pairCapacity = 16;
+    bp->pairs = (uint32_t*)calloc(2 * (size_t)bp->pairCapacity, 4);
+    return bp;
+}
+
+void b3o_bp_destroy(b3o_broadphase* bp) {
+    if (!bp) return;
+    free(bp->nodes);
+    free(bp->move);
+    free(bp->pairs);
+    free(bp->unique);
+    free(bp);
+}
+
+uint32_t b3o_bp_create_proxy(b3o_broadphase* bp, const b3o_aabb* aabb, uint32_t user) {   /* broad_phase.cpp:78-90 */
+    AABB fat = aabb_of(aabb);
+    aabb_extend(&fat, 0.2f); /* B3_AABB_EXTENSION (settings.h:54) */
+    uint32_t id = dt_allocate(bp);                                                       /* InsertNode :102-114 */
+    bp->nodes[id].aabb = fat;
+    bp->nodes[id].user = user;
+    bp->nodes[id].height = 0;
+    dt_insert_leaf(bp, id);
+    ++bp->proxyCount;
+    bp_buffer_move(bp, id);
+    return id;
+}
+
+void b3o_bp_destroy_proxy(b3o_broadphase* bp, uint32_t id) {                             /* broad_phase.cpp:92-97 */
+    for (uint32_t i = 0; i < bp->moveCount; ++i)                                          /* UnbufferMove :61-70 */
+        if (bp->move[i] == id) bp->move[i] = B3O_NULL_NODE;
+    --bp->proxyCount;
+    dt_remove_leaf(bp, id);                                                               /* RemoveNode :116-122 */
+    dt_free(bp, id);
+}
+
+void b3o_bp_touch_proxy(b3o_broadphase* bp, uint32_t id) { bp_buffer_move(bp, id); }   /* broad_phase.cpp:151-154 */
+
+int b3o_bp_move_proxy(b3o_broadphase* bp, uint32_t id, const b3o_aabb* aabb, const b3o_vec3* displacement) { /* :99-149 */
+    AABB a = aabb_of(aabb);
+    if (aabb_contains(&bp->nodes[id].aabb, &a)) return 0;
+    AABB fat = a;
+    aabb_extend(&fat, 0.2f);
+    V3 d = scale(2.0f, v3(displacement->x, displacement->y, displacement->z)); /* B3_AABB_MULTIPLIER (settings.h:59) */
+    if (d.x < 0.0f) fat.lo.x += d.x; else fat.hi.x += d.x;
+    if (d.y < 0.0f) fat.lo.y += d.y; else fat.hi.y += d.y;
+    if (d.z < 0.0f) fat.lo.z += d.z; else fat.hi.z += d.z;
+    dt_remove_leaf(bp, id);                                                                /* UpdateNode :116-126 */
+    bp->nodes[id].aabb = fat;
+    dt_insert_leaf(bp, id);
+    bp_buffer_move(bp, id);
+    return 1;
+}
+
+static void bp_report(b3o_broadphase* bp, uint32_t id) {                                /* broad_phase.cpp:156-182 */
+    if (id == bp->queryId) return;
+    if (bp->pairCount == bp->pairCapacity) {
+        bp->pairCapacity *= 2;
+        bp->pairs = (uint32_t*)realloc(bp->pairs, 8 * (size_t)bp->pairCapacity);
+    }
+    bp->pairs[2 * bp->pairCount] = id < bp->queryId ? id : bp->queryId;
+    bp->pairs[2 * bp->pairCount + 1] = id > bp->queryId ? id : bp->queryId;
+    ++bp->pairCount;
+}
+
+static int pair_cmp(const void* a, const void* b) {                                     /* broad_phase.h:136-149 */
+    const uint32_t* p = (const uint32_t*)a;
+    const uint32_t* q = (const uint32_t*)b;
+    if (p[0] != q[0]) return p[0] < q[0] ? -1 : 1;
+    if (p[1] != q[1]) return p[1] < q[1] ? -1 : 1;
+    return 0;
+}
+
+uint32_t b3o_bp_find_pairs(b3o_broadphase* bp, uint32_t cap, uint32_t* out) {         /* broad_phase.h:160-208 */
+    bp->pairCount = 0;
+    uint32_t stackCap = 256, *stack = (uint32_t*)malloc(4 * (size_t)stackCap);
+    for (uint32_t i = 0; i < bp->moveCount; ++i) {
+        bp->queryId = bp->move[i];
+        if (bp->queryId == B3O_NULL_NODE) continue;
+        AABB q = bp->nodes[bp->queryId].aabb;
+        uint32_t top = 0;                                                                  /* dynamic_tree.h:150-182 */
+        stack[top++] = bp->root;
+        while (top > 0) {
+            uint32_t node = stack[--top];
+            if (node == B3O_NULL_NODE) continue;
+            const DNode* n = bp->nodes + node;
+            if (aabb_overlap(&n->aabb, &q)) {
+                if (dleaf(n)) {
+                    bp_report(bp, node);
+                } else {
+                    if (top + 2 > stackCap) {
+                        stackCap *= 2;
+                        stack = (uint32_t*)realloc(stack, 4 * (size_t)stackCap);
+                    }
+                    stack[top++] = n->child1;
+                    stack[top++] = n->child2;
+                }
+            }
+        }
+    }
+    free(stack);
+    bp->moveCount = 0;
+    qsort(bp->pairs, bp->pairCount, 8, pair_cmp);
+    bp->uniqueCount = 0;
+    uint32_t index = 0;
+    while (index < bp->pairCount) {
+        const uint32_t* primary = bp->pairs + 2 * index;
+        if (bp->uniqueCount == bp->uniqueCapacity) {
+            bp->uniqueCapacity = bp->uniqueCapacity ? 2 * bp->uniqueCapacity : 64;
+            bp->unique = (uint32_t*)realloc(bp->unique, 16 * (size_t)bp->uniqueCapacity);
+        }
+        uint32_t* u = bp->unique + 4 * (size_t)bp->uniqueCount++;
+        u[0] = primary[0];
+        u[1] = primary[1];
+        u[2] = bp->nodes[primary[0]].user;
+        u[3] = bp->nodes[primary[1]].user;
+        ++index;
+        while (index < bp->pairCount && bp->pairs[2 * index] == primary[0] && bp->pairs[2 * index + 1] == primary[1])
+            ++index;
+    }
+    if (out) memcpy(out, bp->unique, 16 * (size_t)(bp->uniqueCount < cap ? bp->uniqueCount : cap));
+    return bp->uniqueCount;
+}
+
+int b3o_bp_test_overlap(const b3o_broadphase* bp, uint32_t a, uint32_t b) {              /* broad_phase.cpp:73-76 */
+    return aabb_overlap(&bp->nodes[a].aabb, &bp->nodes[b].aabb);
+}
+
+void b3o_bp_get_aabb(const b3o_broadphase* bp, uint32_t id, b3o_aabb* out) {
+    const AABB* a = &bp->nodes[id].aabb;
+    out->lower.x = a->lo.x, out->lower.y = a->lo.y, out->lower.z = a->lo.z;
+    out->upper.x = a->hi.x, out->upper.y = a->hi.y, out->upper.z = a->hi.z;
+}
+
+/* b3Sweep::GetTransform(0.0f) (transform.h:224-234). */
+static XF sweep_transform0(const b3o_sweep* s) {
+    const float t = 0.0f, u = 1.0f - t;
+    V3 c = v3(u * s->worldCenter0.x + t * s->worldCenter.x, u * s->worldCenter0.y + t * s->worldCenter.y,
+              u * s->worldCenter0.z + t * s->worldCenter.z);
+    b3o_quat q = {u * s->orientation0.x + t * s->orientation.x, u * s->orientation0.y + t * s->orientation.y,
+                  u * s->orientation0.z + t * s->orientation.z, u * s->orientation0.w + t * s->orientation.w};
+    float len = sqrtf(q.x * q.x + q.y * q.y + q.z * q.z + q.w * q.w);                  /* quat.h:88-100 */
+    if (len > FLT_EPSILON) {
+        float k = 1.0f / len;
+        q.x *= k, q.y *= k, q.z *= k, q.w *= k;
+    }
+    XF xf;
+    M33 R = quat_mat33(q);
+    xf.c[0] = R.c[0], xf.c[1] = R.c[1], xf.c[2] = R.c[2];
+    V3 qv = v3(q.x, q.y, q.z), lc = v3(s->localCenter.x, s->localCenter.y, s->localCenter.z);
+    V3 tt = scale(2.0f, cross(qv, lc));                                                   /* b3Mul(q, v), quat.h:225-232 */
+    V3 r = add(add(lc, scale(q.w, tt)), cross(qv, tt));
+    xf.p = sub(c, r);
+    return xf;
+}
+
+void b3o_shape_aabbs(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n, const uint32_t* shapeIdx,
+                     const uint32_t* bodies, const b3o_xf* xfs, const b3o_sweep* sweeps, b3o_aabb* out,
+                     b3o_vec3* disp) {
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3o_shape* s = shapes + shapeIdx[i];
+        const XF* xf2 = xfs + bodies[i];
+        AABB a = shape_aabb(pool, s, xf2);
+        V3 d = v3(0.0f, 0.0f, 0.0f);
+        if (sweeps) {                                                                       /* body.cpp:181-202 */
+            XF xf1 = sweep_transform0(sweeps + bodies[i]);
+            AABB a1 = shape_aabb(pool, s, &xf1);
+            a = aabb_combine(&a1, &a);
+            d = sub(xf2->p, xf1.p);
+        }
+        out[i].lower.x = a.lo.x, out[i].lower.y = a.lo.y, out[i].lower.z = a.lo.z;
+        out[i].upper.x = a.hi.x, out[i].upper.y = a.hi.y, out[i].upper.z = a.hi.z;
+        if (disp) disp[i].x = d.x, disp[i].y = d.y, disp[i].z = d.z;
+    }
+}
+
+/* ---------------------------------------------------------------- the contact manager */
+/* b3ContactManager (contact_manager.cpp) over a world whose bodies carry one convex shape
+ * each and are moved by a script instead of the solver: AddPair (:34-142) from the
+ * broadphase's pairs, UpdateContacts (:167-231) with b3Contact::Update's narrowphase
+ * (contact.cpp:41-140, convex contacts only), Destroy (:270-311); b3Body::SynchronizeShapes
+ * (body.cpp:181-202) after each move.  Body b's shape is its proxy's user data. */
+typedef struct KContact KContact;
+typedef struct KEdge { KContact* contact; uint32_t other; struct KEdge *prev, *next; } KEdge;
+struct KContact {
+    uint32_t shapeA, shapeB;
+    KEdge edgeA, edgeB;
+    KContact *prev, *next;
+    b3o_convex_cache cache;
+    b3o_manifold manifold;
+};
+struct b3o_kworld {
+    b3o_hull_pool pool;
+    b3o_shape* shapes;
+    b3o_broadphase* bp;
+    uint32_t bodyCount, bodyCap;
+    uint32_t *shapeOf, *type, *proxy, *awake;
+    KEdge** edges;
+    b3o_xf* xf;
+    b3o_sweep* sweep;
+    KContact* head;
+    uint32_t contactCount;
+};
+
+b3o_kworld* b3o_kworld_create(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t shapeCount) {
+    b3o_kworld* w = (b3o_kworld*)calloc(1, sizeof(b3o_kworld));
+    w->pool = *pool;
+    w->shapes = (b3o_shape*)malloc(sizeof(b3o_shape) * (shapeCount ? shapeCount : 1));
+    memcpy(w->shapes, shapes, sizeof(b3o_shape) * shapeCount);
+    w->bp = b3o_bp_create();
+    return w;
+}
+
+static void kw_destroy_contact(b3o_kworld* w, KContact* c) {                          /* contact_manager.cpp:270-311 */
+    KEdge* lists[2] = {&c->edgeA, &c->edgeB};
+    uint32_t owners[2] = {c->shapeA, c->shapeB};
+    for (int k = 0; k < 2; ++k) {
+        KEdge* e = lists[k];
+        if (e->prev) e->prev->next = e->next;
+        else w->edges[owners[k]] = e->next;
+        if (e->next) e->next->prev = e->prev;
+    }
+    if (c->prev) c->prev->next = c->next;
+    else w->head = c->next;
+    if (c->next) c->next->prev = c->prev;
+    --w->contactCount;
+    free(c);
+}
+
+void b3o_kworld_destroy(b3o_kworld* w) {
+    if (!w) return;
+    while (w->head) kw_destroy_contact(w, w->head);
+    b3o_bp_destroy(w->bp);
+    free(w->shapes), free(w->shapeOf), free(w->type), free(w->proxy), free(w->awake), free(w->edges);
+    free(w->xf), free(w->sweep);
+    free(w);
+}
+
+void b3o_kworld_add_bodies(b3o_kworld* w, uint32_t n, const uint32_t* shapeIdx, const uint32_t* types,
+                           const b3o_xf* xfs, const b3o_sweep* sweeps, uint32_t* proxyIds) {
+    if (w->bodyCount + n > w->bodyCap) {
+        uint32_t cap = w->bodyCap ? w->bodyCap : 64;
+        while (cap < w->bodyCount + n) cap *= 2;
+        w->shapeOf = (uint32_t*)realloc(w->shapeOf, 4 * (size_t)cap);
+        w->type = (uint32_t*)realloc(w->type, 4 * (size_t)cap);
+        w->proxy = (uint32_t*)realloc(w->proxy, 4 * (size_t)cap);
+        w->awake = (uint32_t*)realloc(w->awake, 4 * (size_t)cap);
+        w->edges = (KEdge**)realloc(w->edges, sizeof(KEdge*) * cap);
+        w->xf = (b3o_xf*)realloc(w->xf, sizeof(b3o_xf) * cap);
+        w->sweep = (b3o_sweep*)realloc(w->sweep, sizeof(b3o_sweep) * cap);
+        w->bodyCap = cap;
+    }
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t b = w->bodyCount++;
+        w->shapeOf[b] = shapeIdx[i];
+        w->type[b] = types[i];
+        w->awake[b] = 1; /* b3BodyDef::awake (body.h:52) */
+        w->edges[b] = NULL;
+        w->xf[b] = xfs[i];
+        w->sweep[b] = sweeps[i];
+        /* b3Body::CreateShape: the AABB at m_xf, then CreateProxy (body.cpp:110-118) */
+        AABB a = shape_aabb(&w->pool, w->shapes + shapeIdx[i], xfs + i);
+        b3o_aabb box = {{a.lo.x, a.lo.y, a.lo.z}, {a.hi.x, a.hi.y, a.hi.z}};
+        w->proxy[b] = b3o_bp_create_proxy(w->bp, &box, b);
+        if (proxyIds) proxyIds[i] = w->proxy[b];
+    }
+}
+
+static int kw_should_collide(const b3o_kworld* w, uint32_t a, uint32_t b) {            /* body.cpp:204-225 */
+    return w->type[a] == 2u || w->type[b] == 2u; /* e_dynamicBody; no joints in this world */
+}
+
+static void kw_add_pair(b3o_kworld* w, uint32_t sA, uint32_t sB) {                      /* contact_manager.cpp:34-142 */
+    if (sA == sB) return;
+    for (KEdge* ce = w->edges[sB]; ce; ce = ce->next)
+        if (ce->other == sA) return; /* a contact already exists (either order) */
+    if (!kw_should_collide(w, sA, sB)) return;
+    /* Create: swap so that typeA <= typeB (:233-268) */
+    if (w->shapes[w->shapeOf[sA]].type > w->shapes[w->shapeOf[sB]].type) {
+        uint32_t t = sA;
+        sA = sB;
+        sB = t;
+    }
+    KContact* c = (KContact*)calloc(1, sizeof(KContact));
+    c->shapeA = sA;
+    c->shapeB = sB;
+    fresh_cache(&c->cache); /* convex_contact.cpp:31-36 */
+    c->edgeA.contact = c, c->edgeA.other = sB;
+    c->edgeA.next = w->edges[sA];
+    if (w->edges[sA]) w->edges[sA]->prev = &c->edgeA;
+    w->edges[sA] = &c->edgeA;
+    c->edgeB.contact = c, c->edgeB.other = sA;
+    c->edgeB.next = w->edges[sB];
+    if (w->edges[sB]) w->edges[sB]->prev = &c->edgeB;
+    w->edges[sB] = &c->edgeB;
+    w->awake[sA] = w->awake[sB] = 1;
+    c->next = w->head;
+    if (w->head) w->head->prev = c;
+    w->head = c;
+    ++w->contactCount;
+}
+
+uint32_t b3o_kworld_find_new_contacts(b3o_kworld* w) {                                  /* contact_manager.cpp:156-165 */
+    uint32_t n = b3o_bp_find_pairs(w->bp, 0, NULL);
+    uint32_t before = w->contactCount;
+    for (uint32_t i = 0; i < n; ++i) kw_add_pair(w, w->bp->unique[4 * i + 2], w->bp->unique[4 * i + 3]);
+    return w->contactCount - before;
+}
+
+void b3o_kworld_update_contacts(b3o_kworld* w, uint32_t stats[3]) {                     /* contact_manager.cpp:167-231 */
+    uint32_t destroyed = 0, collided = 0, skipped = 0;
+    KContact* c = w->head;
+    while (c) {
+        uint32_t a = c->shapeA, b = c->shapeB;
+        if (!kw_should_collide(w, a, b)) {
+            KContact* q = c;
+            c = c->next;
+            kw_destroy_contact(w, q);
+            ++destroyed;
+            continue;
+        }
+        int activeA = w->awake[a] && w->type[a] != 0u, activeB = w->awake[b] && w->type[b] != 0u;
+        if (!activeA && !activeB) {
+            c = c->next;
+            ++skipped;
+            continue;
+        }
+        if (!b3o_bp_test_overlap(w->bp, w->proxy[a], w->proxy[b])) {
+            KContact* q = c;
+            c = c->next;
+            kw_destroy_contact(w, q);
+            ++destroyed;
+            continue;
+        }
+        /* b3Contact::Update -> b3ConvexContact::Collide (contact.cpp:63-78, convex_contact.cpp:50-62) */
+        uint32_t pair[2] = {w->shapeOf[a], w->shapeOf[b]};
+        b3o_xf xfs[2] = {w->xf[a], w->xf[b]};
+        b3o_quat sw[4] = {w->sweep[a].orientation0, w->sweep[a].orientation, w->sweep[b].orientation0,
+                          w->sweep[b].orientation};
+        b3o_collide_pairs(&w->pool, w->shapes, 1, pair, xfs, sw, &c->cache, &c->manifold);
+        ++collided;
+        c = c->next;
+    }
+    if (stats) stats[0] = destroyed, stats[1] = collided, stats[2] = skipped;
+}
+
+void b3o_kworld_move_bodies(b3o_kworld* w, uint32_t n, const uint32_t* bodies, const b3o_xf* xfs,
+                            const b3o_sweep* sweeps) {
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t b = bodies[i];
+        w->xf[b] = xfs[i];
+        w->sweep[b] = sweeps[i];
+        uint32_t s = w->shapeOf[b];
+        b3o_aabb box;
+        b3o_vec3 d;
+        uint32_t zero = 0;
+        b3o_shape_aabbs(&w->pool, w->shapes, 1, &s, &zero, w->xf + b, w->sweep + b, &box, &d);
+        b3o_bp_move_proxy(w->bp, w->proxy[b], &box, &d);
+    }
+}
+
+uint32_t b3o_kworld_contacts(const b3o_kworld* w, uint32_t cap, uint32_t* bodyPairs, b3o_manifold* out,
+                             b3o_convex_cache* caches) {
+    uint32_t k = 0;
+    for (const KContact* c = w->head; c; c = c->next, ++k) {
+        if (k >= cap) continue;
+        if (bodyPairs) bodyPairs[2 * k] = c->shapeA, bodyPairs[2 * k + 1] = c->shapeB;
+        if (out) out[k] = c->manifold;
+        if (caches) caches[k] = c->cache;
+    }
+    return k;
+}
+
+void b3o_kworld_proxies(const b3o_kworld* w, uint32_t n, const uint32_t* bodies, uint32_t* ids, b3o_aabb* fat) {
+    for (uint32_t i = 0; i < n; ++i) {
+        if (ids) ids[i] = w->proxy[bodies[i]];
+        if (fat) b3o_bp_get_aabb(w->bp, w->proxy[bodies[i]], fat + i);
+    }
+}

@@ -234,6 +234,74 @@ uint64_t b3o_flop_count(void);
 uint64_t b3o_flop_count_phase(int phase);
 void b3o_flop_reset(void);
 
+/* ---- the step before the narrowphase: broadphase and contact manager ---- */
+typedef struct { b3o_vec3 lower, upper; } b3o_aabb;                       /* b3AABB3 aabb3.h:26-28 */
+typedef struct {                                                          /* b3Sweep transform.h:202-222 */
+    b3o_vec3 localCenter;
+    b3o_quat orientation0;
+    b3o_vec3 worldCenter0;
+    float t0;
+    b3o_quat orientation;
+    b3o_vec3 worldCenter;
+} b3o_sweep;
+
+/* b3BroadPhase over b3DynamicTree, restated (oracle) and the reference's own (b3ref_bp_*).
+ * find_pairs writes up to cap pairs of 4 words (proxy1, proxy2, user1, user2) in the order
+ * b3ContactManager::AddPair receives them and returns how many there are. */
+typedef struct b3o_broadphase b3o_broadphase;
+b3o_broadphase* b3o_bp_create(void);
+void b3o_bp_destroy(b3o_broadphase* bp);
+uint32_t b3o_bp_create_proxy(b3o_broadphase* bp, const b3o_aabb* aabb, uint32_t user);
+void b3o_bp_destroy_proxy(b3o_broadphase* bp, uint32_t id);
+void b3o_bp_touch_proxy(b3o_broadphase* bp, uint32_t id);
+int b3o_bp_move_proxy(b3o_broadphase* bp, uint32_t id, const b3o_aabb* aabb, const b3o_vec3* displacement);
+uint32_t b3o_bp_find_pairs(b3o_broadphase* bp, uint32_t cap, uint32_t* out);
+int b3o_bp_test_overlap(const b3o_broadphase* bp, uint32_t a, uint32_t b);
+void b3o_bp_get_aabb(const b3o_broadphase* bp, uint32_t id, b3o_aabb* out);
+void* b3ref_bp_create(void);
+void b3ref_bp_destroy(void* bp);
+uint32_t b3ref_bp_create_proxy(void* bp, const b3o_aabb* aabb, uint32_t user);
+void b3ref_bp_destroy_proxy(void* bp, uint32_t id);
+void b3ref_bp_touch_proxy(void* bp, uint32_t id);
+int b3ref_bp_move_proxy(void* bp, uint32_t id, const b3o_aabb* aabb, const b3o_vec3* displacement);
+uint32_t b3ref_bp_find_pairs(void* bp, uint32_t cap, uint32_t* out);
+int b3ref_bp_test_overlap(void* bp, uint32_t a, uint32_t b);
+void b3ref_bp_get_aabb(void* bp, uint32_t id, b3o_aabb* out);
+
+/* b3Body::SynchronizeShapes' AABB and displacement for shape shapeIdx[i] on body bodies[i]
+ * (body.cpp:181-202); with sweeps null, the AABB at xfs alone (b3Body::CreateShape). */
+void b3o_shape_aabbs(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t n, const uint32_t* shapeIdx,
+                     const uint32_t* bodies, const b3o_xf* xfs, const b3o_sweep* sweeps, b3o_aabb* out,
+                     b3o_vec3* disp);
+
+/* A world of bodies with one convex shape each, moved by a script (no solver):
+ * b3ContactManager::FindNewContacts / UpdateContacts and b3Body::SynchronizeShapes, restated
+ * (b3o_kworld_*) and the reference's own b3World (b3ref_kworld_*).  Body types follow
+ * b3BodyType (0 static, 1 kinematic, 2 dynamic).  stats = destroyed, collided, skipped. */
+typedef struct b3o_kworld b3o_kworld;
+b3o_kworld* b3o_kworld_create(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t shapeCount);
+void b3o_kworld_destroy(b3o_kworld* w);
+void b3o_kworld_add_bodies(b3o_kworld* w, uint32_t n, const uint32_t* shapeIdx, const uint32_t* types,
+                           const b3o_xf* xfs, const b3o_sweep* sweeps, uint32_t* proxyIds);
+uint32_t b3o_kworld_find_new_contacts(b3o_kworld* w);
+void b3o_kworld_update_contacts(b3o_kworld* w, uint32_t stats[3]);
+void b3o_kworld_move_bodies(b3o_kworld* w, uint32_t n, const uint32_t* bodies, const b3o_xf* xfs,
+                            const b3o_sweep* sweeps);
+uint32_t b3o_kworld_contacts(const b3o_kworld* w, uint32_t cap, uint32_t* bodyPairs, b3o_manifold* out,
+                             b3o_convex_cache* caches);
+void b3o_kworld_proxies(const b3o_kworld* w, uint32_t n, const uint32_t* bodies, uint32_t* ids, b3o_aabb* fat);
+void* b3ref_kworld_create(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t shapeCount);
+void b3ref_kworld_destroy(void* w);
+void b3ref_kworld_add_bodies(void* w, uint32_t n, const uint32_t* shapeIdx, const uint32_t* types,
+                             const b3o_xf* xfs, const b3o_sweep* sweeps, uint32_t* proxyIds);
+uint32_t b3ref_kworld_find_new_contacts(void* w);
+void b3ref_kworld_update_contacts(void* w, uint32_t stats[3]);
+void b3ref_kworld_move_bodies(void* w, uint32_t n, const uint32_t* bodies, const b3o_xf* xfs,
+                              const b3o_sweep* sweeps);
+uint32_t b3ref_kworld_contacts(void* w, uint32_t cap, uint32_t* bodyPairs, b3o_manifold* out,
+                               b3o_convex_cache* caches);
+void b3ref_kworld_proxies(void* w, uint32_t n, const uint32_t* bodies, uint32_t* ids, b3o_aabb* fat);
+
 /* ---- compiled reference driver (_ref/libb3ref.so) ---- */
 void b3ref_collide_pairs(const b3o_hull_pool* pool, const b3o_shape* shapes,
                          uint32_t n, const uint32_t* pairs, const b3o_xf* xfs,

@@ -935,3 +935,209 @@ extern "C" void b3ref_mesh_contacts_destroy(void* handle) {
     b3_gjkCacheHits = (u32)now[5];
     delete r;
 }
+
+// ---------------------------------------------------------------- the step before
+// The reference's own b3BroadPhase, driven call by call.  User data is carried as an integer
+// in the void* the reference stores; FindPairs' callback records the (userData1, userData2)
+// it would pass to b3ContactManager::AddPair, with the proxy ids of the pair.
+namespace {
+struct PairRecorder {
+    b3BroadPhase* bp;
+    std::vector<uint32_t> out;
+    void AddPair(void* a, void* b) {
+        out.push_back((uint32_t)(uintptr_t)a);
+        out.push_back((uint32_t)(uintptr_t)b);
+    }
+};
+b3AABB3 ref_aabb(const b3o_aabb* a) {
+    b3AABB3 r;
+    r.m_lower = b3Vec3(a->lower.x, a->lower.y, a->lower.z);
+    r.m_upper = b3Vec3(a->upper.x, a->upper.y, a->upper.z);
+    return r;
+}
+// user data 0 is a null pointer in the reference; keep every value distinct from it
+void* user_ptr(uint32_t u) { return (void*)((uintptr_t)u + 1u); }
+uint32_t user_of(void* p) { return (uint32_t)((uintptr_t)p - 1u); }
+}  // namespace
+
+extern "C" void* b3ref_bp_create(void) { return new b3BroadPhase(); }
+extern "C" void b3ref_bp_destroy(void* bp) { delete (b3BroadPhase*)bp; }
+extern "C" uint32_t b3ref_bp_create_proxy(void* bp, const b3o_aabb* aabb, uint32_t user) {
+    return ((b3BroadPhase*)bp)->CreateProxy(ref_aabb(aabb), user_ptr(user));
+}
+extern "C" void b3ref_bp_destroy_proxy(void* bp, uint32_t id) { ((b3BroadPhase*)bp)->DestroyProxy(id); }
+extern "C" void b3ref_bp_touch_proxy(void* bp, uint32_t id) { ((b3BroadPhase*)bp)->TouchProxy(id); }
+extern "C" int b3ref_bp_move_proxy(void* bp, uint32_t id, const b3o_aabb* aabb, const b3o_vec3* d) {
+    return ((b3BroadPhase*)bp)->MoveProxy(id, ref_aabb(aabb), b3Vec3(d->x, d->y, d->z)) ? 1 : 0;
+}
+extern "C" uint32_t b3ref_bp_find_pairs(void* handle, uint32_t cap, uint32_t* out) {
+    b3BroadPhase* bp = (b3BroadPhase*)handle;
+    PairRecorder rec;
+    rec.bp = bp;
+    bp->FindPairs(&rec);
+    // The proxy ids of each reported pair are in the broadphase's sorted, unique pair buffer
+    // (broad_phase.h:188-207), one entry per AddPair call, in the same order.
+    const uint32_t n = (uint32_t)(rec.out.size() / 2);
+    uint32_t k = 0, index = 0;
+    while (index < bp->m_pairCount && k < n) {
+        const b3Pair* p = bp->m_pairs + index;
+        if (k < cap) {
+            out[4 * k] = p->proxy1;
+            out[4 * k + 1] = p->proxy2;
+            out[4 * k + 2] = rec.out[2 * k] - 1u;
+            out[4 * k + 3] = rec.out[2 * k + 1] - 1u;
+        }
+        ++k;
+        ++index;
+        while (index < bp->m_pairCount && bp->m_pairs[index].proxy1 == p->proxy1 && bp->m_pairs[index].proxy2 == p->proxy2)
+            ++index;
+    }
+    return n;
+}
+extern "C" int b3ref_bp_test_overlap(void* bp, uint32_t a, uint32_t b) {
+    return ((b3BroadPhase*)bp)->TestOverlap(a, b) ? 1 : 0;
+}
+extern "C" void b3ref_bp_get_aabb(void* bp, uint32_t id, b3o_aabb* out) {
+    const b3AABB3& a = ((b3BroadPhase*)bp)->GetAABB(id);
+    std::memcpy(&out->lower, &a.m_lower, 12);
+    std::memcpy(&out->upper, &a.m_upper, 12);
+}
+
+// A b3World whose bodies (one convex shape each, density 0 so no mass update moves the
+// local centre) are moved by a script: add bodies (b3World::CreateBody + b3Body::CreateShape,
+// world.cpp:86-92, body.cpp:89-121), FindNewContacts and UpdateContacts of its own
+// b3ContactManager (contact_manager.cpp:156-231), and a move that sets m_xf and m_sweep and
+// runs b3Body::SynchronizeShapes (body.cpp:181-202) as b3World::Solve does after the solver
+// (world.cpp:308-336).
+namespace {
+struct KWorld {
+    b3World* world = nullptr;
+    std::vector<uint32_t> desc, edges, faces;
+    std::vector<float> centroids, vertices, planes;
+    std::vector<b3Hull> hulls;
+    std::vector<b3o_shape> shapes;
+    std::vector<b3Body*> bodies;
+    std::unordered_map<const b3Shape*, uint32_t> bodyOf;
+    uint64_t saved[6];
+};
+}  // namespace
+
+extern "C" void* b3ref_kworld_create(const b3o_hull_pool* pool, const b3o_shape* shapes, uint32_t shapeCount) {
+    KWorld* k = new KWorld();
+    b3ref_get_counters(k->saved);
+    uint32_t nv = 0, ne = 0, nf = 0;
+    for (uint32_t h = 0; h < pool->count; ++h) {
+        const uint32_t* d = pool->desc + 6 * h;
+        nv = std::max(nv, d[0] + d[1]);
+        ne = std::max(ne, d[2] + d[3]);
+        nf = std::max(nf, d[4] + d[5]);
+    }
+    k->desc.assign(pool->desc, pool->desc + 6 * (size_t)pool->count);
+    k->centroids.assign(pool->centroids, pool->centroids + 3 * (size_t)pool->count);
+    k->vertices.assign(pool->vertices, pool->vertices + 3 * (size_t)nv);
+    k->edges.assign(pool->edges, pool->edges + 4 * (size_t)ne);
+    k->faces.assign(pool->faces, pool->faces + nf);
+    k->planes.assign(pool->planes, pool->planes + 4 * (size_t)nf);
+    b3o_hull_pool own = {pool->count, k->desc.data(), k->centroids.data(), k->vertices.data(),
+                         k->edges.data(), k->faces.data(), k->planes.data()};
+    build_hulls(&own, k->hulls);
+    k->shapes.assign(shapes, shapes + shapeCount);
+    k->world = new b3World();
+    return k;
+}
+
+extern "C" void b3ref_kworld_destroy(void* h) {
+    KWorld* k = (KWorld*)h;
+    delete k->world;
+    b3_convexCalls = (u32)k->saved[0];
+    b3_convexCacheHits = (u32)k->saved[1];
+    b3_gjkCalls = (u32)k->saved[2];
+    b3_gjkIters = (u32)k->saved[3];
+    b3_gjkMaxIters = (u32)k->saved[4];
+    b3_gjkCacheHits = (u32)k->saved[5];
+    delete k;
+}
+
+extern "C" void b3ref_kworld_add_bodies(void* h, uint32_t n, const uint32_t* shapeIdx, const uint32_t* types,
+                                        const b3o_xf* xfs, const b3o_sweep* sweeps, uint32_t* proxyIds) {
+    KWorld* k = (KWorld*)h;
+    static_assert(sizeof(b3Sweep) == sizeof(b3o_sweep), "sweep layout");
+    for (uint32_t i = 0; i < n; ++i) {
+        b3BodyDef bd;
+        bd.type = (b3BodyType)types[i];
+        b3Body* b = k->world->CreateBody(bd);
+        std::memcpy(&b->m_xf, xfs + i, sizeof(b3Transform));
+        std::memcpy(&b->m_sweep, sweeps + i, sizeof(b3Sweep));
+        Scene tmp;  // a template shape; CreateShape clones it (b3Shape::Create, shape.cpp)
+        b3Shape* proto = make_shape(k->shapes[shapeIdx[i]], k->hulls);
+        b3ShapeDef sd;
+        sd.shape = proto;
+        b3Shape* s = b->CreateShape(sd);
+        tmp.shapes.push_back(proto);
+        k->bodyOf[s] = (uint32_t)k->bodies.size();
+        k->bodies.push_back(b);
+        if (proxyIds) proxyIds[i] = s->m_broadPhaseID;
+    }
+}
+
+extern "C" uint32_t b3ref_kworld_find_new_contacts(void* h) {
+    KWorld* k = (KWorld*)h;
+    const uint32_t before = k->world->m_contactMan.m_contactList.m_count;
+    k->world->m_contactMan.FindNewContacts();
+    k->world->m_flags &= ~b3World::e_shapeAddedFlag;
+    return k->world->m_contactMan.m_contactList.m_count - before;
+}
+
+extern "C" void b3ref_kworld_update_contacts(void* h, uint32_t stats[3]) {
+    KWorld* k = (KWorld*)h;
+    b3ContactManager& cm = k->world->m_contactMan;
+    uint32_t before = cm.m_contactList.m_count, active = 0, skipped = 0;
+    for (b3Contact* c = cm.m_contactList.m_head; c; c = c->m_next) {
+        b3Body* bA = c->m_pair.shapeA->m_body;
+        b3Body* bB = c->m_pair.shapeB->m_body;
+        const bool aA = bA->IsAwake() && bA->m_type != e_staticBody;
+        const bool aB = bB->IsAwake() && bB->m_type != e_staticBody;
+        if (bA->ShouldCollide(bB) && !aA && !aB) ++skipped;
+        (void)active;
+    }
+    cm.UpdateContacts();
+    const uint32_t after = cm.m_contactList.m_count;
+    if (stats) stats[0] = before - after, stats[1] = after - skipped, stats[2] = skipped;
+}
+
+extern "C" void b3ref_kworld_move_bodies(void* h, uint32_t n, const uint32_t* bodies, const b3o_xf* xfs,
+                                         const b3o_sweep* sweeps) {
+    KWorld* k = (KWorld*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        b3Body* b = k->bodies[bodies[i]];
+        std::memcpy(&b->m_xf, xfs + i, sizeof(b3Transform));
+        std::memcpy(&b->m_sweep, sweeps + i, sizeof(b3Sweep));
+        b->SynchronizeShapes();
+    }
+}
+
+extern "C" uint32_t b3ref_kworld_contacts(void* h, uint32_t cap, uint32_t* bodyPairs, b3o_manifold* out,
+                                          b3o_convex_cache* caches) {
+    KWorld* k = (KWorld*)h;
+    uint32_t i = 0;
+    for (b3Contact* c = k->world->m_contactMan.m_contactList.m_head; c; c = c->m_next, ++i) {
+        if (i >= cap) continue;
+        if (bodyPairs) {
+            bodyPairs[2 * i] = k->bodyOf[c->m_pair.shapeA];
+            bodyPairs[2 * i + 1] = k->bodyOf[c->m_pair.shapeB];
+        }
+        b3ConvexContact* cc = (b3ConvexContact*)c;
+        if (out) to_out(cc->m_stackManifold, out + i);
+        if (caches) std::memcpy(caches + i, &cc->m_cache, sizeof(b3o_convex_cache));
+    }
+    return i;
+}
+
+extern "C" void b3ref_kworld_proxies(void* h, uint32_t n, const uint32_t* bodies, uint32_t* ids, b3o_aabb* fat) {
+    KWorld* k = (KWorld*)h;
+    for (uint32_t i = 0; i < n; ++i) {
+        const b3Shape* s = k->bodies[bodies[i]]->m_shapeList.m_head;
+        if (ids) ids[i] = s->m_broadPhaseID;
+        if (fat) b3ref_bp_get_aabb(&k->world->m_contactMan.m_broadPhase, s->m_broadPhaseID, fat + i);
+    }
+}

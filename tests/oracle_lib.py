@@ -383,3 +383,154 @@ class RefMeshContacts:
         fat = np.zeros((self.n, 6), np.float32)
         self.f["read"](self.handle, self.cap, ptr(counts), ptr(lists), ptr(caches), ptr(fat))
         return counts, lists, caches, fat
+
+
+class BroadPhase:
+    """b3BroadPhase: the oracle's restatement (prefix "b3o_bp_") or the reference's own
+    (prefix "b3ref_bp_"), one proxy call at a time."""
+
+    def __init__(self, lib, prefix):
+        self.lib, self.p = lib, prefix
+        vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+        sig = {"create": (vp, []), "destroy": (None, [vp]), "create_proxy": (u32, [vp, vp, u32]),
+               "destroy_proxy": (None, [vp, u32]), "touch_proxy": (None, [vp, u32]),
+               "move_proxy": (ctypes.c_int, [vp, u32, vp, vp]), "find_pairs": (u32, [vp, u32, vp]),
+               "test_overlap": (ctypes.c_int, [vp, u32, u32]), "get_aabb": (None, [vp, u32, vp])}
+        self.f = {}
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, prefix + name)
+            fn.restype, fn.argtypes = res, args
+            self.f[name] = fn
+        self.h = ctypes.c_void_p(self.f["create"]())
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def create(self, aabb, user):
+        a = np.ascontiguousarray(aabb, np.float32).reshape(6)
+        return int(self.f["create_proxy"](self.h, ptr(a), int(user)))
+
+    def destroy(self, pid):
+        self.f["destroy_proxy"](self.h, int(pid))
+
+    def touch(self, pid):
+        self.f["touch_proxy"](self.h, int(pid))
+
+    def move(self, pid, aabb, disp):
+        a = np.ascontiguousarray(aabb, np.float32).reshape(6)
+        d = np.ascontiguousarray(disp, np.float32).reshape(3)
+        return int(self.f["move_proxy"](self.h, int(pid), ptr(a), ptr(d)))
+
+    def find_pairs(self, cap=1 << 16):
+        """FindPairs: [k, 4] uint32 (proxy1, proxy2, user1, user2) in AddPair order."""
+        out = np.zeros((cap, 4), np.uint32)
+        n = int(self.f["find_pairs"](self.h, cap, ptr(out)))
+        assert n <= cap, "pair capacity %d too small for %d pairs" % (cap, n)
+        return out[:n]
+
+    def test_overlap(self, a, b):
+        return int(self.f["test_overlap"](self.h, int(a), int(b)))
+
+    def aabb(self, pid):
+        out = np.zeros(6, np.float32)
+        self.f["get_aabb"](self.h, int(pid), ptr(out))
+        return out
+
+
+class KWorld:
+    """A world of scripted bodies with one convex shape each: the oracle's restated
+    b3ContactManager (prefix "b3o_kworld_") or the reference's own b3World (prefix
+    "b3ref_kworld_")."""
+
+    def __init__(self, lib, prefix, pool, shapes):
+        from bounce_amd.types import SWEEP_DTYPE
+        self.lib, self.p, self.pool = lib, prefix, pool
+        self.shapes = np.ascontiguousarray(shapes)
+        self._ps = pool.struct()
+        vp, u32 = ctypes.c_void_p, ctypes.c_uint32
+        sig = {"create": (vp, [vp, vp, u32]), "destroy": (None, [vp]),
+               "add_bodies": (None, [vp, u32, vp, vp, vp, vp, vp]), "find_new_contacts": (u32, [vp]),
+               "update_contacts": (None, [vp, vp]), "move_bodies": (None, [vp, u32, vp, vp, vp]),
+               "contacts": (u32, [vp, u32, vp, vp, vp]), "proxies": (None, [vp, u32, vp, vp, vp])}
+        self.f = {}
+        for name, (res, args) in sig.items():
+            fn = getattr(lib, prefix + name)
+            fn.restype, fn.argtypes = res, args
+            self.f[name] = fn
+        self.h = ctypes.c_void_p(self.f["create"](ctypes.byref(self._ps), ptr(self.shapes), len(self.shapes)))
+        self.bodies = 0
+        self._sweep_dtype = SWEEP_DTYPE
+
+    def close(self):
+        if self.h:
+            self.f["destroy"](self.h)
+            self.h = None
+
+    def __del__(self):
+        self.close()
+
+    def add_bodies(self, shape_idx, types, xfs, sweeps):
+        shape_idx = np.ascontiguousarray(shape_idx, np.uint32)
+        n = len(shape_idx)
+        types = np.ascontiguousarray(types, np.uint32)
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 12)
+        sweeps = np.ascontiguousarray(sweeps, self._sweep_dtype)
+        ids = np.zeros(n, np.uint32)
+        self.f["add_bodies"](self.h, n, ptr(shape_idx), ptr(types), ptr(xfs), ptr(sweeps), ptr(ids))
+        self.bodies += n
+        return ids
+
+    def find_new_contacts(self):
+        return int(self.f["find_new_contacts"](self.h))
+
+    def update_contacts(self):
+        st = np.zeros(3, np.uint32)
+        self.f["update_contacts"](self.h, ptr(st))
+        return st
+
+    def move_bodies(self, bodies, xfs, sweeps):
+        bodies = np.ascontiguousarray(bodies, np.uint32)
+        n = len(bodies)
+        xfs = np.ascontiguousarray(xfs, np.float32).reshape(n, 12)
+        sweeps = np.ascontiguousarray(sweeps, self._sweep_dtype)
+        self.f["move_bodies"](self.h, n, ptr(bodies), ptr(xfs), ptr(sweeps))
+
+    def contacts(self):
+        """(body pairs [n, 2], manifolds, caches) in m_contactList order."""
+        n = int(self.f["contacts"](self.h, 0, None, None, None))
+        pairs = np.zeros((n, 2), np.uint32)
+        out = np.zeros(n, MANIFOLD_DTYPE)
+        caches = np.zeros(n, CACHE_DTYPE)
+        self.f["contacts"](self.h, n, ptr(pairs), ptr(out), ptr(caches))
+        return pairs, out, caches
+
+    def proxies(self, bodies):
+        bodies = np.ascontiguousarray(bodies, np.uint32)
+        ids = np.zeros(len(bodies), np.uint32)
+        fat = np.zeros((len(bodies), 6), np.float32)
+        self.f["proxies"](self.h, len(bodies), ptr(bodies), ptr(ids), ptr(fat))
+        return ids, fat
+
+
+def shape_aabbs(pool, shapes, shape_idx, bodies, xfs, sweeps=None):
+    """The oracle's b3Body::SynchronizeShapes AABBs [n, 6] and displacements [n, 3]."""
+    lib = oracle().lib
+    f = lib.b3o_shape_aabbs
+    f.restype = None
+    f.argtypes = [ctypes.c_void_p] * 2 + [ctypes.c_uint32] + [ctypes.c_void_p] * 6
+    from bounce_amd.types import SWEEP_DTYPE
+    si = np.ascontiguousarray(shape_idx, np.uint32)
+    bo = np.ascontiguousarray(bodies, np.uint32)
+    x = np.ascontiguousarray(xfs, np.float32).reshape(-1, 12)
+    sw = None if sweeps is None else np.ascontiguousarray(sweeps, SWEEP_DTYPE)
+    out = np.zeros((len(si), 6), np.float32)
+    disp = np.zeros((len(si), 3), np.float32)
+    ps = pool.struct()
+    f(ctypes.byref(ps), ptr(np.ascontiguousarray(shapes)), len(si), ptr(si), ptr(bo), ptr(x), ptr(sw), ptr(out),
+      ptr(disp))
+    return out, disp

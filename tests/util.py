@@ -681,3 +681,201 @@ def contact_init_workload(lib, oracle, seed, n, M=1):
     old_impulses["motorImpulse"] = rng.normal(0, 1, (n, M))
     return dict(bodies=bodies, state=state, material=material, counts=counts, manifolds=manifolds,
                 old_counts=old_counts, old_manifolds=old_manifolds, old_impulses=old_impulses)
+
+
+# ---------------------------------------------------------------- broadphase scripts
+def broadphase_script(seed, n0=300, steps=20, extent=20.0, create_frac=0.03, destroy_frac=0.03,
+                      touch_frac=0.01, move_frac=0.5, big=2):
+    """A seeded sequence of b3BroadPhase calls on proxies named by creation order (handles):
+    ("create", aabbs [k, 6]), ("destroy", handles), ("touch", handles),
+    ("move", handles, aabbs [k, 6], displacements [k, 3]) and ("find",) once per step.
+    AABBs are float32 boxes of half-extent 0.1-1 m in an `extent` cube, plus `big` ground-like
+    slabs; a move translates a proxy's tight box by up to 0.3 m (many stay inside their fat
+    box) and 5 % of moves jump up to 3 m."""
+    rng = np.random.default_rng(seed)
+    tight = []  # tight AABB per handle
+    live = []
+    ops = []
+
+    def new_boxes(k):
+        c = rng.uniform(0.0, extent, (k, 3))
+        h = rng.uniform(0.1, 1.0, (k, 3))
+        return np.concatenate([c - h, c + h], 1).astype(np.float32)
+
+    first = new_boxes(n0)
+    for g in range(min(big, n0)):
+        first[g] = np.array([0.0, -0.5 + 3.0 * g, 0.0, extent, 0.5 + 3.0 * g, extent], np.float32)
+    ops.append(("create", first))
+    tight.extend(first)
+    live.extend(range(n0))
+    for _ in range(steps):
+        ops.append(("find",))
+        alive = np.array(live)
+        k = max(1, int(move_frac * len(alive)))
+        mv = rng.choice(alive, size=min(k, len(alive)), replace=False)
+        d = rng.uniform(-0.3, 0.3, (len(mv), 3))
+        jump = rng.random(len(mv)) < 0.05
+        d[jump] = rng.uniform(-3.0, 3.0, (int(jump.sum()), 3))
+        d = d.astype(np.float32)
+        boxes = np.stack([tight[h] for h in mv]) + np.concatenate([d, d], 1)
+        boxes = boxes.astype(np.float32)
+        for j, h in enumerate(mv):
+            tight[h] = boxes[j]
+        ops.append(("move", mv.astype(np.int64), boxes, d))
+        nd = int(destroy_frac * len(alive))
+        if nd:
+            dead = rng.choice(alive, size=nd, replace=False)
+            ops.append(("destroy", dead.astype(np.int64)))
+            live = [h for h in live if h not in set(dead.tolist())]
+        nt = int(touch_frac * len(live))
+        if nt:
+            ops.append(("touch", rng.choice(np.array(live), size=nt, replace=False).astype(np.int64)))
+        nc = int(create_frac * len(live)) + 1
+        boxes = new_boxes(nc)
+        ops.append(("create", boxes))
+        base = len(tight)
+        tight.extend(boxes)
+        live.extend(range(base, base + nc))
+    ops.append(("find",))
+    return ops
+
+
+def run_broadphase_script(bp, ops):
+    """Runs a script on an object with create(aabbs, users) -> ids, destroy(ids), touch(ids),
+    move(ids, aabbs, disps) -> moved flags and find_pairs() -> [k, 4]; returns the ids of
+    every handle, the moved flags of every move and the pairs of every find (with users =
+    handles)."""
+    ids = []
+    finds, moves = [], []
+    for op in ops:
+        if op[0] == "create":
+            users = np.arange(len(ids), len(ids) + len(op[1]), dtype=np.uint32)
+            ids.extend(int(x) for x in bp.create(op[1], users))
+        elif op[0] == "destroy":
+            bp.destroy(np.array([ids[h] for h in op[1]], np.uint32))
+        elif op[0] == "touch":
+            bp.touch(np.array([ids[h] for h in op[1]], np.uint32))
+        elif op[0] == "move":
+            moves.append(np.asarray(bp.move(np.array([ids[h] for h in op[1]], np.uint32), op[2], op[3])))
+        else:
+            finds.append(np.asarray(bp.find_pairs()))
+    return np.array(ids, np.uint32), moves, finds
+
+
+class ScalarBroadPhase:
+    """Adapter of oracle_lib.BroadPhase (one proxy per call) to the batched script runner."""
+
+    def __init__(self, bp):
+        self.bp = bp
+
+    def create(self, aabbs, users):
+        return [self.bp.create(a, u) for a, u in zip(aabbs, users)]
+
+    def destroy(self, ids):
+        for i in ids:
+            self.bp.destroy(i)
+
+    def touch(self, ids):
+        for i in ids:
+            self.bp.touch(i)
+
+    def move(self, ids, aabbs, disps):
+        return [self.bp.move(i, a, d) for i, a, d in zip(ids, aabbs, disps)]
+
+    def find_pairs(self):
+        return self.bp.find_pairs(1 << 20)
+
+
+# ---------------------------------------------------------------- scripted worlds
+def kinematic_world(lib, seed, n_dyn=120, steps=40, spawn_every=10, spawn_count=12, extent=8.0, dt=1.0 / 60.0,
+                    n_hulls=40, speed=2.5):
+    """A world of scripted bodies for the contact-manager tests: a static 60 x 1 x 60 m ground
+    box (top at y = 0) and dynamic bodies with 8-16-point hulls, moving with constant linear
+    and angular velocities and bouncing off the ground (kinematically; no solver).
+    Returns a dict: pool, shapes, spawns [(step, bodies, shape indices, types, xfs, sweeps)]
+    (step -1: before the first step), states[s] = (xfs, sweeps) of every body for the move
+    after step s (sweep from the step's state to the next)."""
+    from bounce_amd.shapes import box_hull
+    from bounce_amd.types import HullPool, SWEEP_DTYPE, SHAPE_DTYPE, HULL
+    from bounce_amd.workloads import gen_hull_pool, quat_xf_rotation, _quat_mul
+    rng = np.random.default_rng(seed)
+    hulls = gen_hull_pool(lib, seed, n_hulls, 8, 16, 0.25, 0.6)
+    pool = HullPool.join(HullPool.concat([box_hull(30.0, 0.5, 30.0)]), hulls)
+    shapes = np.zeros(pool.count, SHAPE_DTYPE)
+    shapes["type"] = HULL
+    shapes["hull"] = np.arange(pool.count)
+    n_total = 1 + n_dyn + spawn_count * (steps // spawn_every)
+    shape_of = np.concatenate([[0], 1 + rng.integers(0, n_hulls, n_total - 1)]).astype(np.uint32)
+    types = np.full(n_total, 2, np.uint32)
+    types[0] = 0
+    pos = np.zeros((n_total, 3), np.float32)
+    pos[1:] = rng.uniform([-extent, 0.6, -extent], [extent, 2.0 * extent, extent], (n_total - 1, 3))
+    vel = rng.uniform(-speed, speed, (n_total, 3)).astype(np.float32)
+    vel[0] = 0.0
+    q = rng.normal(size=(n_total, 4))
+    q /= np.linalg.norm(q, axis=1, keepdims=True)
+    q[0] = [0.0, 0.0, 0.0, 1.0]
+    q = q.astype(np.float32)
+    w = rng.uniform(-2.0, 2.0, (n_total, 3)).astype(np.float32)
+    w[0] = 0.0
+
+    def xf_of(p, qq):
+        x = np.zeros((len(p), 12), np.float32)
+        x[:, :9] = quat_xf_rotation(qq)
+        x[:, 9:] = p
+        return x
+
+    def sweep_of(p0, q0, p1, q1):
+        s = np.zeros(len(p0), SWEEP_DTYPE)
+        s["orientation0"], s["worldCenter0"], s["orientation"], s["worldCenter"] = q0, p0, q1, p1
+        return s
+
+    spawns = []
+    first = 1 + n_dyn
+    b = np.arange(first)
+    spawns.append((-1, b, shape_of[b], types[b], xf_of(pos[b], q[b]), sweep_of(pos[b], q[b], pos[b], q[b])))
+    live = first
+    states = []
+    for s in range(steps):
+        if s > 0 and s % spawn_every == 0 and live + spawn_count <= n_total:
+            b = np.arange(live, live + spawn_count)
+            spawns.append((s, b, shape_of[b], types[b], xf_of(pos[b], q[b]), sweep_of(pos[b], q[b], pos[b], q[b])))
+            live += spawn_count
+        p1 = (pos + dt * vel).astype(np.float32)
+        low = p1[:, 1] < 0.4
+        low[0] = False
+        vel[low, 1] = np.abs(vel[low, 1])
+        p1[0] = pos[0]
+        wq = np.concatenate([w, np.zeros((n_total, 1), np.float32)], 1)
+        dq = 0.5 * dt * _quat_mul(wq.astype(np.float64), q.astype(np.float64))
+        q1 = q.astype(np.float64) + dq
+        q1 /= np.linalg.norm(q1, axis=1, keepdims=True)
+        q1 = q1.astype(np.float32)
+        q1[0] = q[0]
+        b = np.arange(1, live)
+        states.append((b, xf_of(p1[b], q1[b]), sweep_of(pos[b], q[b], p1[b], q1[b])))
+        pos, q = p1, q1
+    return {"pool": pool, "shapes": shapes, "spawns": spawns, "states": states, "steps": steps}
+
+
+def run_kinematic_world(world, w, check=None):
+    """Steps a world object (KWorld-like: add_bodies, find_new_contacts, update_contacts,
+    move_bodies, contacts) through a kinematic_world script in b3World::Step's order: new
+    shapes -> FindNewContacts, UpdateContacts, the move with SynchronizeShapes, then
+    FindNewContacts (world.cpp:115-144, 308-336).  check(step, world) runs after each
+    UpdateContacts."""
+    spawns = {s[0]: s for s in w["spawns"]}
+    _, b, sh, ty, xf, sw = spawns[-1]
+    world.add_bodies(sh, ty, xf, sw)
+    world.find_new_contacts()
+    for s in range(w["steps"]):
+        if s in spawns:
+            _, b, sh, ty, xf, sw = spawns[s]
+            world.add_bodies(sh, ty, xf, sw)
+            world.find_new_contacts()
+        stats = world.update_contacts()
+        if check:
+            check(s, world, stats)
+        bodies, xfs, sweeps = w["states"][s]
+        world.move_bodies(bodies, xfs, sweeps)
+        world.find_new_contacts()
