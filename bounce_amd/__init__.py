@@ -100,6 +100,17 @@ def load_library():
         "b3amd_broadphase_info": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "b3amd_broadphase_read": (i32, [vp, u32, u32, vp, vp, vp]),
         "b3amd_shape_aabbs": (i32, [vp, u32, vp, vp, u32, vp, vp, vp, vp]),
+        "b3amd_contact_manager_create": (i32, [vp, vp, ctypes.POINTER(vp)]),
+        "b3amd_contact_manager_destroy": (i32, [vp]),
+        "b3amd_contact_manager_set_bodies": (i32, [vp, u32, vp]),
+        "b3amd_contact_manager_get_bodies": (i32, [vp, vp]),
+        "b3amd_contact_manager_add_shapes": (i32, [vp, u32, vp, vp, vp, vp]),
+        "b3amd_contact_manager_synchronize": (i32, [vp, u32, vp, vp, vp]),
+        "b3amd_contact_manager_find_new_contacts": (i32, [vp, ctypes.POINTER(u32)]),
+        "b3amd_contact_manager_update_contacts": (i32, [vp, vp, vp, vp]),
+        "b3amd_contact_manager_read": (i32, [vp, u32, ctypes.POINTER(u32), vp, vp, vp, vp]),
+        "b3amd_contact_manager_view": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(vp), ctypes.POINTER(vp),
+                                             ctypes.POINTER(vp)]),
         "b3amd_set_counting": (i32, [vp, i32]),
         "b3amd_get_counters": (i32, [vp, vp]),
         "b3amd_reset_counters": (i32, [vp]),
@@ -412,6 +423,77 @@ class Broadphase:
         return a, u, f
 
 
+class ContactManager:
+    """b3ContactManager for convex contacts on the device (b3amd_contact_manager_*), over a
+    Broadphase; bodies are flag words (b3BodyType | BODY_AWAKE)."""
+
+    def __init__(self, ctx, bp):
+        self.ctx, self.bp, self.lib = ctx, bp, ctx.lib
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_contact_manager_create(ctx.handle, bp.handle, ctypes.byref(h)))
+        self.handle = h
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.lib.b3amd_contact_manager_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def set_bodies(self, flags):
+        f = np.ascontiguousarray(flags, np.uint32)
+        _check(self.lib.b3amd_contact_manager_set_bodies(self.handle, len(f), ptr(f)))
+
+    def bodies(self, count):
+        out = np.zeros(count, np.uint32)
+        _check(self.lib.b3amd_contact_manager_get_bodies(self.handle, ptr(out)))
+        return out
+
+    def add_shapes(self, table_shapes, bodies, d_body_xfs):
+        ts = np.ascontiguousarray(table_shapes, np.uint32)
+        bo = np.ascontiguousarray(bodies, np.uint32)
+        ids = np.zeros(max(1, len(ts)), np.uint32)
+        _check(self.lib.b3amd_contact_manager_add_shapes(self.handle, len(ts), ptr(ts), ptr(bo), d_body_xfs.ptr, ptr(ids)))
+        return ids[:len(ts)]
+
+    def synchronize(self, n, d_shape_ids, d_body_xfs, d_body_sweeps):
+        _check(self.lib.b3amd_contact_manager_synchronize(self.handle, n, d_shape_ids.ptr, d_body_xfs.ptr,
+                                                          d_body_sweeps.ptr))
+
+    def find_new_contacts(self):
+        c = ctypes.c_uint32()
+        _check(self.lib.b3amd_contact_manager_find_new_contacts(self.handle, ctypes.byref(c)))
+        return c.value
+
+    def update_contacts(self, d_body_xfs, d_body_sweeps=None):
+        st = np.zeros(3, np.uint32)
+        _check(self.lib.b3amd_contact_manager_update_contacts(self.handle, d_body_xfs.ptr,
+                                                              None if d_body_sweeps is None else d_body_sweeps.ptr,
+                                                              ptr(st)))
+        return st
+
+    def count(self):
+        c = ctypes.c_uint32()
+        _check(self.lib.b3amd_contact_manager_read(self.handle, 0, ctypes.byref(c), None, None, None, None))
+        return c.value
+
+    def read(self):
+        """(shape pairs [n, 2], born [n], overlap [n], manifolds, caches) in list order."""
+        n = self.count()
+        pairs = np.zeros((max(n, 1), 2), np.uint32)
+        born = np.zeros(max(n, 1), np.uint32)
+        out = np.zeros(max(n, 1), MANIFOLD_DTYPE)
+        caches = empty_caches(max(n, 1))
+        c = ctypes.c_uint32()
+        _check(self.lib.b3amd_contact_manager_read(self.handle, n, ctypes.byref(c), ptr(pairs), ptr(born), ptr(out),
+                                                   ptr(caches)))
+        return pairs[:n], born[:n] & 0x7FFFFFFF, born[:n] >> 31, out[:n], caches[:n]
+
+
 class MeshContacts:
     """Persistent mesh contacts (b3amd_mesh_contacts_*): b3MeshContact across steps."""
 
@@ -535,6 +617,10 @@ class Context:
     def broadphase(self):
         """A device b3BroadPhase on this context."""
         return Broadphase(self)
+
+    def contact_manager(self, bp):
+        """A device b3ContactManager over the Broadphase `bp`."""
+        return ContactManager(self, bp)
 
     def shape_aabbs(self, shapes, bodies, body_xfs, body_sweeps=None):
         """b3Body::SynchronizeShapes' swept AABB and displacement of shape shapes[i] on body

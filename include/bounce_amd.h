@@ -292,6 +292,54 @@ int b3amd_shape_aabbs(b3amd_context* ctx, uint32_t n, const uint32_t* d_shapes, 
                       uint32_t body_count, const b3amd_transform* d_body_xfs, const b3amd_sweep* d_body_sweeps,
                       b3amd_aabb* d_aabbs, b3amd_vec3* d_displacements);
 
+/* ---- the step before: the contact manager ----
+ * b3ContactManager (contact_manager.cpp) for convex contacts with the contact list, each
+ * contact's b3ConvexCache and manifold in HBM, over a device broadphase.  Shapes are added
+ * with their body (b3Body::CreateShape, body.cpp:89-121: the proxy's AABB is the shape's at
+ * the body's m_xf, its user data the manager's shape index, returned in order from 0); body
+ * flags are b3BodyType (0 static, 1 kinematic, 2 dynamic) | B3AMD_BODY_AWAKE.  Joints,
+ * contact filters, sensors and mesh shapes are not modelled here (ShouldCollide is the
+ * body-type test, body.cpp:204-211).  Destroy a manager before its broadphase. */
+#define B3AMD_BODY_AWAKE 4u
+typedef struct b3amd_contact_manager b3amd_contact_manager;
+int b3amd_contact_manager_create(b3amd_context* ctx, b3amd_broadphase* bp, b3amd_contact_manager** out);
+int b3amd_contact_manager_destroy(b3amd_contact_manager* m);
+/* Replaces the body flags (host array of body_count words). */
+int b3amd_contact_manager_set_bodies(b3amd_contact_manager* m, uint32_t body_count, const uint32_t* flags);
+/* Current body flags (AddPair and touch changes set B3AMD_BODY_AWAKE, contact_manager.cpp:116-121,
+ * contact.cpp:94-99). */
+int b3amd_contact_manager_get_bodies(b3amd_contact_manager* m, uint32_t* flags);
+/* b3Body::CreateShape for n shapes: context shape-table entry table_shapes[i] on body
+ * bodies[i]; the AABB comes from d_body_xfs (device, one per body).  proxy_ids may be null. */
+int b3amd_contact_manager_add_shapes(b3amd_contact_manager* m, uint32_t n, const uint32_t* table_shapes,
+                                     const uint32_t* bodies, const b3amd_transform* d_body_xfs, uint32_t* proxy_ids);
+/* b3Body::SynchronizeShapes (body.cpp:181-202) for the manager shapes d_shape_ids[0..n):
+ * the swept AABB between m_sweep.GetTransform(0) and m_xf, then MoveProxy.  Enqueued; a bad
+ * shape id makes the next find_new_contacts return B3AMD_ERR_ARGUMENT. */
+int b3amd_contact_manager_synchronize(b3amd_contact_manager* m, uint32_t n, const uint32_t* d_shape_ids,
+                                      const b3amd_transform* d_body_xfs, const b3amd_sweep* d_body_sweeps);
+/* FindNewContacts (contact_manager.cpp:156-165): FindPairs, then AddPair for each pair in
+ * order (:34-142); new contacts start with a new b3ConvexContact's cache
+ * (convex_contact.cpp:31-36) and join the list front.  Synchronizes. */
+int b3amd_contact_manager_find_new_contacts(b3amd_contact_manager* m, uint32_t* created);
+/* UpdateContacts (contact_manager.cpp:167-231) on device body state (one transform and one
+ * orientation pair (orientation0, orientation) per body, 16-byte aligned; sweeps may be
+ * null): contacts whose bodies may not collide or whose fat AABBs stopped overlapping are
+ * destroyed, contacts with no active body are skipped, the others run the narrowphase with
+ * their cache (b3Contact::Update, contact.cpp:41-114).  stats (may be null) = destroyed,
+ * collided, skipped.  The narrowphase is left enqueued on the context stream. */
+int b3amd_contact_manager_update_contacts(b3amd_contact_manager* m, const b3amd_transform* d_body_xfs,
+                                          const b3amd_quat* d_body_sweeps, uint32_t stats[3]);
+/* The contact list in m_contactList order (host copies of up to cap entries; any output may
+ * be null): shape pairs (manager shape indices, typeA <= typeB), born[i] = the number of
+ * update_contacts calls before the contact was created | overlap flag << 31, manifolds and
+ * caches. */
+int b3amd_contact_manager_read(b3amd_contact_manager* m, uint32_t cap, uint32_t* count, uint32_t* shape_pairs,
+                               uint32_t* born, b3amd_manifold* manifolds, b3amd_convex_cache* caches);
+/* Device view: the list as slot indices, and the manifold and cache arrays by slot. */
+int b3amd_contact_manager_view(b3amd_contact_manager* m, uint32_t* count, const uint32_t** d_order,
+                               const b3amd_manifold** d_manifolds, const b3amd_convex_cache** d_caches);
+
 /* b3amd_collide_device without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops
  * and harnesses that forbid hipMalloc).  The batch's scratch lives in a file-scope device
