@@ -66,6 +66,7 @@ def workload(lib, n_pairs, n_pool):
 # ----------------------------------------------------------------- CPU baseline
 
 _CPU_JOB = {}  # set before forking: children inherit the arrays instead of unpickling them
+_RAIN_JOB = {}
 
 
 def _cpu_worker(chunk):
@@ -432,6 +433,267 @@ def warm_world_run(world, steps, cpu):
     return res
 
 
+# ----------------------------------------------------------------- C5: the rain-in world
+
+RAIN_PATCH = 6  # columns per side of each CPU-baseline patch world
+
+
+def _rain_patch_bodies(world, i0, j0, k):
+    """Bodies of the k x k columns from (i0, j0), every layer, plus the statics."""
+    C, ns = world["columns"], world["statics"]
+    n = len(world["body_shapes"]) - ns
+    cells = np.arange(n)
+    i, j = cells % C, (cells // C) % C
+    sel = cells[(i >= i0) & (i < i0 + k) & (j >= j0) & (j < j0 + k)] + ns
+    return np.concatenate([np.arange(ns), sel])
+
+
+def _rain_cpu_worker(patch):
+    """The reference's own b3World (contact manager, broadphase, narrowphase) stepping one
+    patch of the rain-in for every step; returns (start, end, narrowphase calls)."""
+    j = _RAIN_JOB
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from bounce_amd.workloads import rain_state
+    w = j["world"]
+    chk = oracle_lib.reference() if j["kind"] == "reference" else oracle_lib.oracle()
+    prefix = "b3ref_kworld_" if j["kind"] == "reference" else "b3o_kworld_"
+    bodies = _rain_patch_bodies(w, patch[0], patch[1], RAIN_PATCH)
+    spawn = w["spawn"][bodies]
+    kw = oracle_lib.KWorld(chk.lib, prefix, w["pool"], w["shapes"])
+    # inputs first (untimed): the states of the patch's bodies per step
+    steps = j["steps"]
+    states = [rain_state(w, s, bodies) for s in range(steps + 1)]
+    order = []
+    t0 = time.perf_counter()
+    calls = 0
+    st = np.nonzero(spawn < 0)[0]
+    kw.add_bodies(w["body_shapes"][bodies[st]], np.zeros(len(st), np.uint32), states[0][0][st], states[0][1][st])
+    order.extend(st.tolist())
+    kw.find_new_contacts()
+    for s in range(steps):
+        new = np.nonzero(spawn == s)[0]
+        if len(new):
+            kw.add_bodies(w["body_shapes"][bodies[new]], np.full(len(new), 2, np.uint32), states[s][0][new],
+                          states[s][1][new])
+            order.extend(new.tolist())
+            kw.find_new_contacts()
+        calls += int(kw.update_contacts()[1])
+        present = np.array(order)
+        mv = np.nonzero(spawn[present] >= 0)[0]
+        kw.move_bodies(mv, states[s + 1][0][present[mv]], states[s + 1][1][present[mv]])
+        kw.find_new_contacts()
+    t1 = time.perf_counter()
+    kw.close()
+    return t0, t1, calls
+
+
+def rain_cpu_baseline(world, steps):
+    """The reference's own b3World over the rain-in, restricted to disjoint patches of
+    RAIN_PATCH x RAIN_PATCH columns (every layer, with the container), one patch world per
+    process, all steps: UpdateContacts with its narrowphase, SynchronizeShapes and
+    FindNewContacts, as b3World::Step runs them.  Runs before the GPU is initialised."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    kind = "reference" if oracle_lib.reference() is not None else "port"
+    host = host_cpu()
+    cores = host["processes"]
+    C = world["columns"]
+    patches = [(i, j) for j in range(0, C - RAIN_PATCH + 1, RAIN_PATCH) for i in range(0, C - RAIN_PATCH + 1, RAIN_PATCH)]
+    patches = patches[:: max(1, len(patches) // cores)][:cores]
+    _RAIN_JOB.update(world=world, steps=steps, kind=kind)
+    with mp.get_context("fork").Pool(cores) as p:
+        res = p.map(_rain_cpu_worker, patches, chunksize=1)
+    wall = wall_span([(r[0], r[1]) for r in res])
+    calls = sum(r[2] for r in res)
+    bodies = sum(len(_rain_patch_bodies(world, i, j, RAIN_PATCH)) for i, j in patches)
+    return dict({"value": calls / wall, "unit": "narrowphase contacts/s (whole contact-manager step)",
+                 "cores": cores, "kind": kind,
+                 "sample": "%d patch worlds of %d x %d columns (%d bodies in all), %d steps each, one process per "
+                           "patch: the reference's b3World contact manager, broadphase and narrowphase; %d "
+                           "narrowphase calls, wall %.1f s" % (len(patches), RAIN_PATCH, RAIN_PATCH, bodies, steps,
+                                                               calls, wall)}, **host)
+
+
+def rain_world_run(lib, world, steps, cpu, sample=2000):
+    """The rain-in (configs[4] as specified): hulls spawn (b3Body::CreateShape) and fall into
+    the container; each step runs, on the device, what b3World::Step runs around the solver:
+    FindNewContacts for the new shapes, UpdateContacts (contacts destroyed when their fat
+    AABBs part, the narrowphase with each contact's cache for the others), the move with
+    SynchronizeShapes, and FindNewContacts (broadphase pairs, AddPair).  The kinematic motion
+    stands in for the solver: its states are computed on the host and uploaded outside the
+    timed regions."""
+    import bounce_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from util import defined_hull_cache, HULL_CACHE_FIELDS
+    from bounce_amd.types import SWEEP_DTYPE, MANIFOLD_DTYPE
+    from bounce_amd.workloads import rain_state
+    w = world
+    B, ns = len(w["body_shapes"]), w["statics"]
+    spawn = w["spawn"]
+    fall_steps = int(np.ceil(w["t_fall"] / w["dt"])) + 2
+    res = {"workload": "%d hulls of 8-64 points raining into a 50 x 50 m container (floor and 4 walls): each "
+                       "spawns %.0f m above its resting place in its column (layer k at step %d k + a column "
+                       "offset), falls under %.1f m/s^2 turning to its resting orientation, and rests; %d steps "
+                       "at 60 Hz; kinematic motion (no solver)" % (B - ns, w["fall_height"], w["interval"],
+                                                                   w["gravity"], steps),
+           "bodies": B, "steps": steps}
+    # Host-side body state, advanced per step for the bodies in motion only.
+    xf, sw, q = rain_state(w, 0)
+    changing = lambda s: np.nonzero((spawn >= 0) & (spawn <= s + 1) & (s <= spawn + fall_steps))[0]
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(w["pool"])
+        ctx.upload_shapes(w["shapes"])
+        d_xf, d_q, d_sw, d_ids = ctx.alloc(48 * B), ctx.alloc(32 * B), ctx.alloc(72 * B), ctx.alloc(4 * B)
+        with ctx.broadphase() as bp, ctx.contact_manager(bp) as m:
+            flags = np.where(np.arange(B) < ns, 0 | 4, 2 | 4).astype(np.uint32)
+            m.set_bodies(flags)
+            shape_body, shape_proxy = [], []
+            st = np.nonzero(spawn < 0)[0]
+            d_xf.upload(xf)
+            shape_proxy.extend(m.add_shapes(w["body_shapes"][st], st, d_xf).tolist())
+            shape_body.extend(st.tolist())
+            m.find_new_contacts()
+            ctx.reset_kernel_stats()
+            ctx.set_profiling(True)
+            step_ms, created, destroyed, collided, listed = [], [], [], [], []
+            ctx.set_counting(True)
+            ctx.reset_counters()
+            for s in range(steps):
+                d_xf.upload(xf)
+                d_q.upload(q)
+                ctx.synchronize()
+                t0 = time.perf_counter()
+                new = np.nonzero(spawn == s)[0]
+                c = 0
+                if len(new):
+                    shape_proxy.extend(m.add_shapes(w["body_shapes"][new], new, d_xf).tolist())
+                    shape_body.extend(new.tolist())
+                    c += m.find_new_contacts()
+                stats = m.update_contacts(d_xf, d_q)
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                listed.append(m.count())
+                ch = changing(s)
+                x1, s1, q1 = rain_state(w, s + 1, ch)
+                xf[ch], sw[ch], q[ch] = x1, s1, q1
+                d_xf.upload(xf)
+                d_sw.upload(sw)
+                ids = np.nonzero(spawn[np.array(shape_body)] >= 0)[0].astype(np.uint32)
+                d_ids.upload(ids)
+                ctx.synchronize()
+                t2 = time.perf_counter()
+                m.synchronize(len(ids), d_ids, d_xf, d_sw)
+                c += m.find_new_contacts()
+                ctx.synchronize()
+                t3 = time.perf_counter()
+                step_ms.append(((t1 - t0) + (t3 - t2)) * 1000.0)
+                created.append(c)
+                destroyed.append(int(stats[0]))
+                collided.append(int(stats[1]))
+            ctx.set_profiling(False)
+            counters = ctx.counters()
+            ctx.set_counting(False)
+            kern = {}
+            for k in ("bp_move", "bp_lists", "bp_query", "cm_sync", "cm_filter", "cm_create", "cm_classify",
+                      "cm_scatter", "cm_overlap", "gather", "cache_hull", "sat_hull", "contact_hull", "convex"):
+                n_, t_ = ctx.kernel_stats(k)
+                if n_:
+                    kern[k] = t_ / steps
+            pairs, born, overlap, out, caches = m.read()
+            fat = bp.read(0, bp.info()[1])[0][np.array(shape_proxy)]
+    step_ms = np.array(step_ms)
+    total_s = step_ms.sum() / 1000.0
+    res.update({
+        "ms_per_step": float(step_ms.mean()),
+        "ms_per_step_p50_p99_max": [float(np.percentile(step_ms, 50)), float(np.percentile(step_ms, 99)),
+                                    float(step_ms.max())],
+        "contacts_per_s": float(np.sum(collided) / total_s),
+        "steps_per_s": steps / total_s,
+        "contacts_created": int(np.sum(created)), "contacts_destroyed": int(np.sum(destroyed)),
+        "narrowphase_calls": int(np.sum(collided)),
+        "contacts_at_end": int(listed[-1]), "touching_at_end": int((out["pointCount"] > 0).sum()),
+        "cache_hit_fraction": counters["convex_cache_hits"] / max(1, counters["convex_calls"]),
+        "kernel_ms_per_step": kern,
+        "what": "per step: b3amd_contact_manager_add_shapes + find_new_contacts for the new shapes, "
+                "update_contacts (destroy, narrowphase with each contact's cache), synchronize (SynchronizeShapes "
+                "+ MoveProxy of every shape present) and find_new_contacts (FindPairs + AddPair), all on the "
+                "device, host wall clock including their synchronisations; body states uploaded outside it",
+    })
+    res["timeline"] = {"every": 50, "contacts": [int(x) for x in listed[::50]],
+                       "created": [int(x) for x in np.add.reduceat(created, np.arange(0, steps, 50))],
+                       "destroyed": [int(x) for x in np.add.reduceat(destroyed, np.arange(0, steps, 50))]}
+    # Sampled parity: chains of the reference's narrowphase from each sampled contact's
+    # creation to the last step, with its cache carried, against the device's.
+    if cpu is not None:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib
+        ref = oracle_lib.reference() if cpu["kind"] == "reference" else oracle_lib.oracle()
+        from bounce_amd.types import empty_caches
+        idx = np.arange(0, len(pairs), max(1, len(pairs) // sample))
+        sb = np.array(shape_body)
+        bodies = sb[pairs[idx]]
+        need = np.unique(bodies)
+        where = {b: i for i, b in enumerate(need)}
+        loc = np.vectorize(where.get)(bodies)
+        cch = empty_caches(len(idx))
+        got = np.zeros(len(idx), MANIFOLD_DTYPE)
+        b0 = born[idx]
+        for s_ in range(int(b0.min()), steps):
+            act = np.nonzero(b0 <= s_)[0]
+            xs, _, qs = rain_state(w, s_, need)
+            pr = w["body_shapes"][bodies[act]].astype(np.uint32)
+            pxf = np.ascontiguousarray(xs[loc[act]].reshape(-1, 2, 12))
+            psw = np.ascontiguousarray(qs[loc[act]].reshape(-1, 4, 4))
+            sub = np.ascontiguousarray(cch[act])
+            got[act] = ref.collide(w["pool"], w["shapes"], pr, pxf, sweeps=psw, caches=sub)
+            cch[act] = sub
+        man_eq = (got.view(np.uint8).reshape(len(idx), -1) == out[idx].view(np.uint8).reshape(len(idx), -1)).all(1)
+        dc, dw = defined_hull_cache(caches[idx]), defined_hull_cache(cch)
+        cache_eq = np.all([dc[f] == dw[f] for f in HULL_CACHE_FIELDS], axis=0)
+        res["sampled_parity"] = {"contacts": int(len(idx)), "step": steps - 1, "manifolds_equal": int(man_eq.sum()),
+                                 "caches_equal": int(cache_eq.sum()), "against": cpu["kind"],
+                                 "what": "each sampled contact's narrowphase chain replayed by the reference from "
+                                         "its creation step with its cache carried"}
+        res["cpu_baseline"] = cpu
+        res["gpu_over_cpu"] = res["contacts_per_s"] / cpu["value"]
+    # Contact set check: after the last UpdateContacts the list holds exactly the shape pairs
+    # whose fat AABBs overlap (not both static), from the broadphase's own fat AABBs.
+    res["contact_set_equals_fat_aabb_overlaps"] = rain_contact_set_check(fat, pairs, np.array(shape_body) < ns)
+    res["separation_error_fp64"] = rain_fp64_error(w, out, pairs, shape_body, steps - 1)
+    return res
+
+
+def rain_contact_set_check(fat, pairs, static):
+    """Whether the contact list's shape pairs are exactly the pairs of shapes whose fat AABBs
+    overlap (b3TestOverlap on float32 boxes), excluding static-static pairs; brute force
+    through a k-d tree over the box centres (the reach covers the largest box)."""
+    from scipy.spatial import cKDTree
+    lo, hi = fat[:, :3], fat[:, 3:]
+    dyn = np.nonzero(~static)[0]
+    c = 0.5 * (lo[dyn].astype(np.float64) + hi[dyn])
+    half = 0.5 * (hi[dyn].astype(np.float64) - lo[dyn]).max()
+    pr = cKDTree(c).query_pairs(2.0 * half + 1e-3, p=np.inf, output_type="ndarray")
+    pr = dyn[pr]
+    ok = np.all((lo[pr[:, 0]] <= hi[pr[:, 1]]) & (hi[pr[:, 0]] >= lo[pr[:, 1]]), axis=1)
+    want = [pr[ok]]
+    for a in np.nonzero(static)[0]:
+        m = np.all((lo[dyn] <= hi[a]) & (hi[dyn] >= lo[a]), axis=1)
+        want.append(np.stack([np.full(int(m.sum()), a), dyn[m]], 1))
+    want = np.sort(np.concatenate(want), axis=1)
+    got = np.sort(pairs.astype(np.int64), axis=1)
+    key = lambda p: np.unique(p[:, 0] * (1 << 32) + p[:, 1])
+    return bool(len(got) == len(key(got)) and np.array_equal(key(got), key(want)))
+
+
+def rain_fp64_error(world, out, pairs, shape_body, s):
+    from bounce_amd.workloads import rain_state
+    sb = np.array(shape_body)
+    bodies = sb[pairs]
+    xs, _, _ = rain_state(world, s)
+    return separation_error_fp64(out, np.ascontiguousarray(xs[bodies]))
+
+
 # ----------------------------------------------------------------- C1: the box pyramid
 
 def pyramid_run(lib):
@@ -626,6 +888,8 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the host-array (PCIe) drop-in timing")
     ap.add_argument("--no-warm", action="store_true", help="skip the large warm world")
     ap.add_argument("--warm-steps", type=int, default=600)
+    ap.add_argument("--no-rain", action="store_true", help="skip the rain-in world")
+    ap.add_argument("--rain-steps", type=int, default=600)
     ap.add_argument("--serving-worlds", type=int, default=4096)
     ap.add_argument("--serving-rate", type=float, default=240000.0)
     ap.add_argument("--serving-seconds", type=float, default=0.5)
@@ -668,6 +932,17 @@ def main():
         if not args.no_cpu:  # also before the GPU is initialised
             warm_cpu = warm_cpu_baseline(warm_world, 4)
             log("warm world cpu baseline: %.0f contacts/s warm (%s)" % (warm_cpu[0]["value"], warm_cpu[0]["kind"]))
+    rain_world = rain_cpu = None
+    if not args.no_rain:
+        from bounce_amd.workloads import gen_rain_world, gen_warm_world
+        t = time.perf_counter()
+        rain_world = gen_rain_world(lib, 11, base=warm_world if warm_world is not None else gen_warm_world(lib, 11))
+        log("rain world: %d bodies in %.1f s" % (len(rain_world["body_shapes"]), time.perf_counter() - t))
+        if not args.no_cpu:  # also before the GPU is initialised
+            t = time.perf_counter()
+            rain_cpu = rain_cpu_baseline(rain_world, args.rain_steps)
+            log("rain world cpu baseline: %.0f contacts/s (%s) in %.1f s"
+                % (rain_cpu["value"], rain_cpu["kind"], time.perf_counter() - t))
     work = algorithmic_work(pool, pairs, xfs, max(1, len(pairs) // 8192))
     log("algorithmic work: %.3g SAT flops, %.3g SAT bytes per batch" % (work["sat_flops"], work["sat_bytes"]))
 
@@ -786,6 +1061,14 @@ def main():
             "cache hits %.3f" % (result["warm_world"]["warm_contacts_per_s"], args.warm_steps - 1,
                                  result["warm_world"]["warm_ms_per_step"], result["warm_world"]["cold_step_ms"],
                                  result["warm_world"]["cache_hit_fraction"]))
+    if not args.no_rain:
+        t = time.perf_counter()
+        result["rain_world"] = rain_world_run(lib, rain_world, args.rain_steps, rain_cpu)
+        r = result["rain_world"]
+        log("rain world: %.0f contacts/s, %.3f ms per step, %d created, %d destroyed, %d contacts at the end, "
+            "parity %s in %.1f s" % (r["contacts_per_s"], r["ms_per_step"], r["contacts_created"],
+                                     r["contacts_destroyed"], r["contacts_at_end"], r.get("sampled_parity"),
+                                     time.perf_counter() - t))
     if not args.no_serving:
         result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     print(json.dumps(result))

@@ -549,3 +549,81 @@ def warm_world_pair_inputs(world, xfs, sweeps, contacts=None):
     pairs = world["body_shapes"][c].astype(np.uint32)
     return (np.ascontiguousarray(pairs), np.ascontiguousarray(xfs[c]),
             np.ascontiguousarray(sweeps[c].reshape(-1, 4, 4)))
+
+
+def gen_rain_world(lib, seed, n_hulls=200000, columns=62, half_width=25.0, interval=8, fall_height=10.0,
+                   spin=3.0, hz=60.0, gravity=9.8, base=None, **layout):
+    """A world whose hulls rain into a container: the resting layout of gen_warm_world
+    (stacks of 8-64-point hulls on a grid in a 50 x 50 m box of floor and four walls), but
+    each hull enters the world (b3Body::CreateShape) at its spawn step `fall_height` m above
+    its resting place and falls (height fall_height - g t^2 / 2 over its place, orientation
+    turning towards its resting one from up to spin * t_fall rad away) until it lands and
+    rests.  Layer k of column (i, j) spawns at step k * interval + a per-column offset in
+    [0, interval), so each hull lands on the one below after that one has come to rest.
+    The motion is kinematic (no solver runs), and contacts are created as hulls pass and
+    reach their neighbours and destroyed as they move apart.
+
+    Returns gen_warm_world's dict (or a copy of `base`, one made by gen_warm_world, whose
+    layout it reuses) plus: spawn [B] step of each body's shape (-1: statics, present from the
+    start), t_fall, spin axis and rate per body."""
+    if base is None:
+        w = gen_warm_world(lib, seed, n_hulls=n_hulls, columns=columns, half_width=half_width, **layout)
+    else:
+        w = dict(base)
+        columns = w["columns"]
+    rng = np.random.default_rng(seed ^ 0x5A1)
+    ns, B = w["statics"], len(w["body_shapes"])
+    n = B - ns
+    level = np.arange(n) // (columns * columns)
+    col = np.arange(n) % (columns * columns)
+    offset = rng.integers(0, interval, columns * columns)
+    spawn = np.full(B, -1, np.int64)
+    spawn[ns:] = level * interval + offset[col]
+    axis = rng.normal(size=(n, 3))
+    axis /= np.linalg.norm(axis, axis=1, keepdims=True)
+    w.update({"spawn": spawn, "interval": interval, "fall_height": fall_height, "gravity": gravity, "dt": 1.0 / hz,
+              "t_fall": float(np.sqrt(2.0 * fall_height / gravity)), "spin_axis": axis,
+              "spin": rng.uniform(-spin, spin, n)})
+    return w
+
+
+def rain_pose(world, s, bodies=None):
+    """Positions [m][3] and orientations [m][4] (float32) of the listed bodies (all by default)
+    at step s; a body before its spawn step holds its spawn pose."""
+    ns = world["statics"]
+    b = np.arange(len(world["body_shapes"])) if bodies is None else np.asarray(bodies, np.int64)
+    pos = world["pos0"][b].copy()
+    q = world["q0"][b].copy()
+    dyn = b >= ns
+    d = b[dyn] - ns
+    t = np.maximum(s - world["spawn"][b[dyn]], 0) * world["dt"]
+    h = np.maximum(world["fall_height"] - 0.5 * world["gravity"] * t * t, 0.0)
+    pos[dyn, 1] += h
+    psi = world["spin"][d] * np.maximum(world["t_fall"] - t, 0.0)
+    turning = psi != 0.0
+    r = np.concatenate([world["spin_axis"][d] * (0.5 * psi)[:, None], np.ones((len(d), 1))], 1)
+    qs = _quat_mul(world["q0"][b[dyn]], r)
+    qs /= np.linalg.norm(qs, axis=1, keepdims=True)
+    qd = q[dyn]
+    qd[turning] = qs[turning]  # exactly the resting orientation once landed
+    q[dyn] = qd
+    return pos.astype(np.float32), q.astype(np.float32)
+
+
+def rain_state(world, s, bodies=None):
+    """Body state at step s for the listed bodies: (xfs [m][12]: b3QuatMat33 of the
+    orientation, then the position; sweeps [m] SWEEP_DTYPE from the pose at s - 1 to the one
+    at s, both the spawn pose at a body's spawn step (b3Body's constructor); orientation
+    pairs [m][2][4] (orientation0, orientation))."""
+    from .types import SWEEP_DTYPE
+    p1, q1 = rain_pose(world, s, bodies)
+    p0, q0 = rain_pose(world, s - 1, bodies)
+    b = np.arange(len(world["body_shapes"])) if bodies is None else np.asarray(bodies, np.int64)
+    new = world["spawn"][b] >= s
+    p0[new], q0[new] = p1[new], q1[new]
+    xfs = np.empty((len(p1), 12), np.float32)
+    xfs[:, :9] = quat_xf_rotation(q1)
+    xfs[:, 9:] = p1
+    sw = np.zeros(len(p1), SWEEP_DTYPE)
+    sw["orientation0"], sw["worldCenter0"], sw["orientation"], sw["worldCenter"] = q0, p0, q1, p1
+    return xfs, sw, np.ascontiguousarray(np.stack([q0, q1], 1))

@@ -176,3 +176,45 @@ def test_kinematic_world_oracle_equals_reference():
     # contacts are created and destroyed throughout, and many touch
     assert sum(created) > 300 and sum(destroyed) > 100
     assert sum(c[1] for c in counts) > 200
+
+
+@needs_ref
+def test_rain_world_oracle_equals_reference():
+    """Hulls raining into a container (gen_rain_world, 6 x 6 columns of 8): the oracle's
+    contact manager equals the reference's b3World at every step, and contacts come and go as
+    the hulls fall past their neighbours and land."""
+    from bounce_amd import load_library
+    from bounce_amd.workloads import gen_rain_world
+    from util import rain_script
+    w = rain_script(gen_rain_world(load_library(), 3, n_hulls=288, columns=6, half_width=2.9, fall_height=3.0),
+                    130)
+    a = oracle_lib.KWorld(oracle_lib.oracle().lib, "b3o_kworld_", w["pool"], w["shapes"])
+    b = oracle_lib.KWorld(ref.lib, "b3ref_kworld_", w["pool"], w["shapes"])
+    created, destroyed = [], []
+
+    class Both:
+        bodies = 0
+
+        def add_bodies(self, *args):
+            assert np.array_equal(a.add_bodies(*args), b.add_bodies(*args))
+            self.bodies = a.bodies
+
+        def find_new_contacts(self):
+            x, y = a.find_new_contacts(), b.find_new_contacts()
+            assert x == y
+            created.append(x)
+
+        def update_contacts(self):
+            x, y = a.update_contacts(), b.update_contacts()
+            assert np.array_equal(x, y), (x, y)
+            destroyed.append(int(x[0]))
+            return x
+
+        def move_bodies(self, *args):
+            a.move_bodies(*args)
+            b.move_bodies(*args)
+
+    check, counts = _kworld_compare(w, a, b)
+    run_kinematic_world(Both(), w, check)
+    assert sum(created) > 500 and sum(destroyed) > 50
+    assert counts[-1][1] > 200  # resting contacts with points at the end

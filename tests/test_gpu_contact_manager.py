@@ -212,3 +212,22 @@ def test_gpu_contact_manager_body_types_and_errors(ctx):
             m.find_new_contacts()
         d_bad.free(), d_sw.free()
     d_xf.free()
+
+
+def test_gpu_rain_world_equals_reference(ctx):
+    """Hulls raining into a container (gen_rain_world, 6 x 6 columns of 8 hulls of 8-64
+    points, 130 steps): broadphase, contact manager and narrowphase on the device equal the
+    reference's own b3World (and the oracle) after every step: the list order as contacts are
+    created when hulls fall past and reach their neighbours and destroyed as they move apart,
+    every manifold and cache carried from each contact's creation."""
+    from bounce_amd.workloads import gen_rain_world
+    from util import rain_script
+    print("rain world seed 3, 288 hulls, 130 steps", flush=True)
+    w = rain_script(gen_rain_world(bounce_amd.load_library(), 3, n_hulls=288, columns=6, half_width=2.9,
+                                   fall_height=3.0), 130)
+    others = [oracle_lib.KWorld(oracle_lib.oracle().lib, "b3o_kworld_", w["pool"], w["shapes"])]
+    ref = oracle_lib.reference()
+    if ref is not None:
+        others.append(oracle_lib.KWorld(ref.lib, "b3ref_kworld_", w["pool"], w["shapes"]))
+    st = _run(ctx, w, others)
+    assert st["created"] > 8000 and st["destroyed"] > 4000 and st["touching"] > 15000

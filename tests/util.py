@@ -879,3 +879,31 @@ def run_kinematic_world(world, w, check=None):
         bodies, xfs, sweeps = w["states"][s]
         world.move_bodies(bodies, xfs, sweeps)
         world.find_new_contacts()
+
+
+def rain_script(world, steps):
+    """A gen_rain_world world as a run_kinematic_world script: bodies enter the script's world
+    at their spawn steps (statics first), in spawn order (body indices of the script world are
+    that order; `order` maps them back to rain bodies); after each step every body present
+    moves to its next pose."""
+    from bounce_amd.workloads import rain_state
+    spawn = world["spawn"]
+    order = []
+    spawns = []
+    st = np.nonzero(spawn < 0)[0]
+    xf, sw, _ = rain_state(world, 0, st)
+    spawns.append((-1, st, world["body_shapes"][st], np.zeros(len(st), np.uint32), xf, sw))
+    order.extend(st.tolist())
+    states = []
+    for s in range(steps):
+        b = np.nonzero(spawn == s)[0]
+        if len(b):
+            xf, sw, _ = rain_state(world, s, b)
+            spawns.append((s, b, world["body_shapes"][b], np.full(len(b), 2, np.uint32), xf, sw))
+            order.extend(b.tolist())
+        present = np.array(order)
+        moving = np.nonzero(spawn[present] >= 0)[0]
+        xf, sw, _ = rain_state(world, s + 1, present[moving])
+        states.append((moving, xf, sw))
+    return {"pool": world["pool"], "shapes": world["shapes"], "spawns": spawns, "states": states, "steps": steps,
+            "order": np.array(order)}
