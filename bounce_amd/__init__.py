@@ -111,6 +111,14 @@ def load_library():
         "b3amd_contact_manager_read": (i32, [vp, u32, ctypes.POINTER(u32), vp, vp, vp, vp]),
         "b3amd_contact_manager_view": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(vp), ctypes.POINTER(vp),
                                              ctypes.POINTER(vp)]),
+        "b3amd_host_world_create": (i32, [vp, ctypes.POINTER(vp)]),
+        "b3amd_host_world_destroy": (i32, [vp]),
+        "b3amd_host_world_settings": (i32, [vp, vp, i32, i32]),
+        "b3amd_host_world_create_body": (i32, [vp, vp, ctypes.POINTER(u32)]),
+        "b3amd_host_world_create_shape": (i32, [vp, u32, vp, ctypes.POINTER(u32)]),
+        "b3amd_host_world_step": (i32, [vp, ctypes.c_float, u32, u32]),
+        "b3amd_host_world_bodies": (i32, [vp, u32, u32, vp]),
+        "b3amd_host_world_contact_count": (i32, [vp, ctypes.POINTER(u32)]),
         "b3amd_set_counting": (i32, [vp, i32]),
         "b3amd_get_counters": (i32, [vp, vp]),
         "b3amd_reset_counters": (i32, [vp]),
@@ -494,6 +502,70 @@ class ContactManager:
         return pairs[:n], born[:n] & 0x7FFFFFFF, born[:n] >> 31, out[:n], caches[:n]
 
 
+class HostWorld:
+    """b3World stepped on the host around the device pipeline (b3amd_host_world_*)."""
+
+    def __init__(self, ctx):
+        self.ctx, self.lib = ctx, ctx.lib
+        h = ctypes.c_void_p()
+        _check(self.lib.b3amd_host_world_create(ctx.handle, ctypes.byref(h)))
+        self.handle = h
+        self.count = 0
+
+    def close(self):
+        if self.handle is not None:
+            _check(self.lib.b3amd_host_world_destroy(self.handle))
+            self.handle = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def settings(self, gravity=None, warm_starting=True, sleeping=False):
+        g = None if gravity is None else np.ascontiguousarray(gravity, np.float32)
+        _check(self.lib.b3amd_host_world_settings(self.handle, ptr(g), int(warm_starting), int(sleeping)))
+
+    def create_body(self, type=0, position=(0, 0, 0), orientation=(0, 0, 0, 1), awake=True, linear_velocity=(0, 0, 0),
+                    angular_velocity=(0, 0, 0), gravity_scale=1.0, linear_damping=0.0, angular_damping=0.0,
+                    fixed_rotation=0):
+        from .types import BODY_DEF_DTYPE
+        d = np.zeros(1, BODY_DEF_DTYPE)
+        d["type"], d["awake"], d["fixed_rotation"] = type, int(awake), fixed_rotation
+        d["position"], d["orientation"] = position, orientation
+        d["linear_velocity"], d["angular_velocity"] = linear_velocity, angular_velocity
+        d["gravity_scale"], d["linear_damping"], d["angular_damping"] = gravity_scale, linear_damping, angular_damping
+        b = ctypes.c_uint32()
+        _check(self.lib.b3amd_host_world_create_body(self.handle, ptr(d), ctypes.byref(b)))
+        self.count += 1
+        return b.value
+
+    def create_shape(self, body, table_shape, density=0.0, friction=0.3, restitution=0.0, sensor=False):
+        from .types import SHAPE_DEF_DTYPE
+        d = np.zeros(1, SHAPE_DEF_DTYPE)
+        d["table_shape"], d["density"], d["friction"], d["restitution"], d["sensor"] = (
+            table_shape, density, friction, restitution, int(sensor))
+        s = ctypes.c_uint32()
+        _check(self.lib.b3amd_host_world_create_shape(self.handle, body, ptr(d), ctypes.byref(s)))
+        return s.value
+
+    def step(self, dt, velocity_iterations=8, position_iterations=2):
+        _check(self.lib.b3amd_host_world_step(self.handle, ctypes.c_float(dt), velocity_iterations,
+                                              position_iterations))
+
+    def bodies(self):
+        from .types import BODY_STATE_DTYPE
+        out = np.zeros(max(1, self.count), BODY_STATE_DTYPE)
+        _check(self.lib.b3amd_host_world_bodies(self.handle, 0, self.count, ptr(out)))
+        return out[:self.count]
+
+    def contact_count(self):
+        n = ctypes.c_uint32()
+        _check(self.lib.b3amd_host_world_contact_count(self.handle, ctypes.byref(n)))
+        return n.value
+
+
 class MeshContacts:
     """Persistent mesh contacts (b3amd_mesh_contacts_*): b3MeshContact across steps."""
 
@@ -621,6 +693,10 @@ class Context:
     def contact_manager(self, bp):
         """A device b3ContactManager over the Broadphase `bp`."""
         return ContactManager(self, bp)
+
+    def host_world(self):
+        """A b3World stepped on the host around this context's device pipeline."""
+        return HostWorld(self)
 
     def shape_aabbs(self, shapes, bodies, body_xfs, body_sweeps=None):
         """b3Body::SynchronizeShapes' swept AABB and displacement of shape shapes[i] on body

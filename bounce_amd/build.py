@@ -20,7 +20,7 @@ LIB = os.path.join(HERE, "libbounce_amd.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
-SOURCES = ["narrowphase_hull.hip", "sat_persistent.hip", "serving.hip", "narrowphase_convex.hip", "narrowphase_mesh.hip", "contact_init.hip", "broadphase.hip", "contact_manager.hip", "api.cpp", "workload.cpp",
+SOURCES = ["narrowphase_hull.hip", "sat_persistent.hip", "serving.hip", "narrowphase_convex.hip", "narrowphase_mesh.hip", "contact_init.hip", "broadphase.hip", "contact_manager.hip", "host_world.cpp", "api.cpp", "workload.cpp",
            "static_tree.cpp", "quickhull.cpp", "hullfile.cpp"]
 
 COMMON = [

@@ -63,6 +63,19 @@ SWEEP_DTYPE = np.dtype([("localCenter", "<f4", (3,)), ("orientation0", "<f4", (4
                         ("t0", "<f4"), ("orientation", "<f4", (4,)), ("worldCenter", "<f4", (3,))])
 assert SWEEP_DTYPE.itemsize == 72
 
+# b3BodyDef / b3ShapeDef / b3Body state of the host step (b3amd_body_def, b3amd_shape_def,
+# b3amd_body_state in include/bounce_amd.h).
+BODY_DEF_DTYPE = np.dtype([("type", "<u4"), ("awake", "<u4"), ("fixed_rotation", "<u4"), ("position", "<f4", (3,)),
+                           ("orientation", "<f4", (4,)), ("linear_velocity", "<f4", (3,)),
+                           ("angular_velocity", "<f4", (3,)), ("gravity_scale", "<f4"), ("linear_damping", "<f4"),
+                           ("angular_damping", "<f4")])
+SHAPE_DEF_DTYPE = np.dtype([("table_shape", "<u4"), ("density", "<f4"), ("friction", "<f4"),
+                            ("restitution", "<f4"), ("sensor", "<u4")])
+BODY_STATE_DTYPE = np.dtype([("xf", "<f4", (12,)), ("sweep", SWEEP_DTYPE), ("linear_velocity", "<f4", (3,)),
+                             ("angular_velocity", "<f4", (3,)), ("mass", "<f4"), ("inv_mass", "<f4"),
+                             ("inertia", "<f4", (9,)), ("inv_inertia", "<f4", (9,)), ("world_inv_inertia", "<f4", (9,)),
+                             ("awake", "<u4"), ("sleep_time", "<f4")])
+
 SPHERE, CAPSULE, HULL, MESH = 0, 1, 2, 3
 # Manifold status bits (B3AMD_FLAG_* of bounce_amd.h).
 FLAG_POLY_OVERFLOW, FLAG_INVALID_PAIR, FLAG_TRIANGLE_OVERFLOW = 1, 2, 4

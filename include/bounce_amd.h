@@ -340,6 +340,52 @@ int b3amd_contact_manager_read(b3amd_contact_manager* m, uint32_t cap, uint32_t*
 int b3amd_contact_manager_view(b3amd_contact_manager* m, uint32_t* count, const uint32_t** d_order,
                                const b3amd_manifold** d_manifolds, const b3amd_convex_cache** d_caches);
 
+/* ---- a whole step: b3World on the host around the device pipeline ----
+ * b3World::Step (world.cpp:115-144): FindNewContacts for new shapes, UpdateContacts, and
+ * Solve (:146-337: depth-first islands, b3Island::Solve with b3ContactSolver, island.cpp:109-341,
+ * contact_solver.cpp:70-588, then SynchronizeShapes and FindNewContacts), bit for bit, with
+ * the broadphase, contact manager and narrowphase on the device and the islands and the
+ * sequential-impulse solver on the host.  World defaults as b3World's constructor
+ * (world.cpp:33-50): gravity (0, -9.8, 0), warm starting on, sleeping off.  Joints, sensors
+ * and mesh shapes are not stepped here. */
+typedef struct b3amd_host_world b3amd_host_world;
+typedef struct {                 /* b3BodyDef (body.h:47-100) */
+    uint32_t type;               /* b3BodyType */
+    uint32_t awake;
+    uint32_t fixed_rotation;     /* bit 0 x, bit 1 y, bit 2 z */
+    b3amd_vec3 position;
+    b3amd_quat orientation;
+    b3amd_vec3 linear_velocity, angular_velocity;
+    float gravity_scale, linear_damping, angular_damping;
+} b3amd_body_def;
+typedef struct {                 /* b3ShapeDef (shape.h:48-66): the geometry is a shape-table entry */
+    uint32_t table_shape;
+    float density, friction, restitution;
+    uint32_t sensor;             /* must be 0 */
+} b3amd_shape_def;
+typedef struct {                 /* b3Body state after a step (body.h) */
+    b3amd_transform xf;          /* m_xf */
+    b3amd_sweep sweep;           /* m_sweep */
+    b3amd_vec3 linear_velocity, angular_velocity;
+    float mass, inv_mass;
+    float inertia[9], inv_inertia[9], world_inv_inertia[9];  /* m_I, m_invI, m_worldInvI (columns) */
+    uint32_t awake;
+    float sleep_time;
+} b3amd_body_state;
+int b3amd_host_world_create(b3amd_context* ctx, b3amd_host_world** out);
+int b3amd_host_world_destroy(b3amd_host_world* w);
+/* m_gravity (null: unchanged), m_warmStarting, m_sleeping. */
+int b3amd_host_world_settings(b3amd_host_world* w, const b3amd_vec3* gravity, int warm_starting, int sleeping);
+/* b3World::CreateBody (world.cpp:86-92); bodies are numbered from 0 in creation order. */
+int b3amd_host_world_create_body(b3amd_host_world* w, const b3amd_body_def* def, uint32_t* body);
+/* b3Body::CreateShape (body.cpp:89-121): ResetMass with the shape's b3*Shape::ComputeMass when
+ * its density is not 0, then its proxy. */
+int b3amd_host_world_create_shape(b3amd_host_world* w, uint32_t body, const b3amd_shape_def* def, uint32_t* shape);
+/* b3World::Step(dt, velocity_iterations, position_iterations). */
+int b3amd_host_world_step(b3amd_host_world* w, float dt, uint32_t velocity_iterations, uint32_t position_iterations);
+int b3amd_host_world_bodies(b3amd_host_world* w, uint32_t first, uint32_t count, b3amd_body_state* out);
+int b3amd_host_world_contact_count(b3amd_host_world* w, uint32_t* count);
+
 /* b3amd_collide_device without any allocation.  No counterpart in the reference: it is for
  * callers that cannot allocate device memory (code under HIP graph capture, serving loops
  * and harnesses that forbid hipMalloc).  The batch's scratch lives in a file-scope device
