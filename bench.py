@@ -722,10 +722,63 @@ def pyramid_run(lib):
             equal += int(np.array_equal(step_digest(out, caches), t["digests"][s]))
             contacts += len(ids)
             points += int(out["pointCount"].sum())
-    return {"steps": PYRAMID_STEPS, "steps_equal_to_reference": equal, "contacts": contacts, "points": points,
-            "ms_per_step": wall * 1000.0 / PYRAMID_STEPS,
-            "what": "b3amd_collide per step on host arrays (H2D, kernels, D2H, synchronise); inputs from "
-                    "the reference world's trace (tests/golden/pyramid_trace.npz)"}
+    res = {"steps": PYRAMID_STEPS, "steps_equal_to_reference": equal, "contacts": contacts, "points": points,
+           "ms_per_step": wall * 1000.0 / PYRAMID_STEPS,
+           "what": "b3amd_collide per step on host arrays (H2D, kernels, D2H, synchronise); inputs from "
+                   "the reference world's trace (tests/golden/pyramid_trace.npz)"}
+    res["whole_step"] = pyramid_world_run(t)
+    return res
+
+
+def pyramid_world_run(t):
+    """The same pyramid stepped end to end by the library alone (b3amd_host_world: islands and
+    solver on the host, broadphase, contact manager and narrowphase on the device), each step
+    compared with the reference world's trace; beside it the reference's own b3World::Step
+    timed on one host core where it is built."""
+    import bounce_amd
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    from bounce_amd.types import SHAPE_DTYPE, HULL
+    from util import PYRAMID_STEPS
+    levels = 20
+    f32 = np.float32
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(t["pool"])
+        shapes = np.zeros(2, SHAPE_DTYPE)
+        shapes["type"] = HULL
+        shapes["hull"] = [0, 1]
+        ctx.upload_shapes(shapes)
+        with ctx.host_world() as w:
+            w.create_shape(w.create_body(type=0), 0)
+            for l in range(levels):
+                for j in range(levels - l):
+                    x = f32(2.1) * f32(j) + f32(l) * f32(1.0) - f32(0.5) * f32(levels) * f32(2.0)
+                    w.create_shape(w.create_body(type=2, position=(x, f32(3.0) * f32(l + 1), 0.0)), 1, density=1.0)
+            equal, ms = 0, []
+            for s in range(PYRAMID_STEPS - 1):
+                t0 = time.perf_counter()
+                w.step(1.0 / 60.0, 8, 2)
+                ms.append((time.perf_counter() - t0) * 1000.0)
+                st = w.bodies()
+                equal += int(np.array_equal(st["xf"].view(np.uint32), t["body_xfs"][s + 1].view(np.uint32)))
+            contacts = w.contact_count()
+    res = {"steps": PYRAMID_STEPS - 1, "steps_equal_to_reference": equal, "ms_per_step": float(np.mean(ms)),
+           "ms_per_step_p50_p99": [float(np.percentile(ms, 50)), float(np.percentile(ms, 99))],
+           "contacts_at_end": contacts,
+           "what": "b3amd_host_world_step: b3World::Step with the islands and sequential-impulse solver on the "
+                   "host and the broadphase, contact manager and narrowphase on the device; every body transform "
+                   "compared with the reference world's after each step"}
+    ref = oracle_lib.reference()
+    if ref is not None:
+        world = oracle_lib.RefPyramid(ref.lib)
+        t0 = time.perf_counter()
+        for _ in range(PYRAMID_STEPS - 1):
+            world.step()
+        res["reference_ms_per_step"] = (time.perf_counter() - t0) * 1000.0 / (PYRAMID_STEPS - 1)
+        res["reference_what"] = "the reference's own b3World::Step on one host core (its driver also records each " \
+                                "step's narrowphase inputs and outputs)"
+        world.close()
+    return res
 
 
 # ----------------------------------------------------------------- main
@@ -1052,9 +1105,11 @@ def main():
                result["mesh"]["host_arrays_contacts_per_s"]))
     if not args.no_pyramid:
         result["pyramid"] = pyramid_run(lib)
-        log("pyramid: %d/%d steps equal to the reference, %.3f ms per step"
+        log("pyramid: %d/%d steps equal to the reference, %.3f ms per step; whole step on the library: %d/%d "
+            "steps equal, %.3f ms per step"
             % (result["pyramid"]["steps_equal_to_reference"], result["pyramid"]["steps"],
-               result["pyramid"]["ms_per_step"]))
+               result["pyramid"]["ms_per_step"], result["pyramid"]["whole_step"]["steps_equal_to_reference"],
+               result["pyramid"]["whole_step"]["steps"], result["pyramid"]["whole_step"]["ms_per_step"]))
     if not args.no_warm:
         result["warm_world"] = warm_world_run(warm_world, args.warm_steps, warm_cpu)
         log("warm world: %.0f contacts/s over %d warm steps (%.3f ms per step), cold step %.3f ms, "
