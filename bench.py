@@ -1116,6 +1116,8 @@ def main():
             "cache hits %.3f" % (result["warm_world"]["warm_contacts_per_s"], args.warm_steps - 1,
                                  result["warm_world"]["warm_ms_per_step"], result["warm_world"]["cold_step_ms"],
                                  result["warm_world"]["cache_hit_fraction"]))
+    if not args.no_serving:
+        result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     if not args.no_rain:
         t = time.perf_counter()
         result["rain_world"] = rain_world_run(lib, rain_world, args.rain_steps, rain_cpu)
@@ -1124,8 +1126,6 @@ def main():
             "parity %s in %.1f s" % (r["contacts_per_s"], r["ms_per_step"], r["contacts_created"],
                                      r["contacts_destroyed"], r["contacts_at_end"], r.get("sampled_parity"),
                                      time.perf_counter() - t))
-    if not args.no_serving:
-        result["serving"] = serving_run(lib, args.serving_worlds, args.serving_rate, args.serving_seconds)
     print(json.dumps(result))
 
 
