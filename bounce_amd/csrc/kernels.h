@@ -315,8 +315,10 @@ struct MeshArgs {
     // Work items are grouped by the convex shape's collider so that a wave runs one of
     // them: segment 0 hulls, 1 capsules, 2 spheres, in that order in `temp`.
     uint32_t* contactOffset;    // [n] first work item of the contact within its segment
-    uint32_t* totals;           // [4] work items per segment (zeroed before the midphase)
-    uint32_t* pairContact;      // [3][n * cap] work item of a segment -> contact
+    uint32_t* totals;           // [4]: work items per segment, then the clustering's count of
+                                // contacts left to its second pass (zeroed before the midphase)
+    uint32_t* pairContact;      // [3][n * cap] work item of a segment -> contact; once the
+                                // narrowphase has run, the clustering's second-pass list [n]
     Manifold* temp;             // [n * cap] per-triangle manifolds, segments back to back
     Manifold* out;              // [n][3] clustered manifolds
     uint32_t* manifoldCounts;   // [n]

@@ -8,8 +8,9 @@
 //   mesh_narrow_kernel    one lane per (contact, triangle): b3TriangleHull built in
 //                         registers and b3CollideShapeAndShape with an empty cache, the
 //                         triangle index stamped into every point key (:238-266).
-//   mesh_cluster_kernel   one lane per mesh contact: b3ClusterSolver::Run over the
-//                         triangle manifolds in midphase order (contact_cluster.cpp:202-590).
+//   mesh_cluster_kernel   sixteen lanes per mesh contact: b3ClusterSolver::Run over the
+//                         triangle manifolds in midphase order (contact_cluster.cpp:202-590),
+//                         observations in LDS, every one of a contact's up to 256 points.
 #include <hip/hip_runtime.h>
 
 #include "b3_math.h"
@@ -324,13 +325,18 @@ __device__ __forceinline__ float dist_sq(V3 a, V3 b) {
     return d.x * d.x + d.y * d.y + d.z * d.z;  // b3DistanceSquared (vec3.h:207-211)
 }
 
+// The clusters of b3ClusterSolver (contact_cluster.cpp:336-377), held by every lane of a
+// contact's group with the same values.  InitializeClusters adds at most three.  The loops run
+// over all three entries with a bound test, so that the centroids stay in registers.
 struct Clusters {
-    V3 c[4];  // InitializeClusters adds at most three
+    V3 c[3];
     uint32_t n;
-    __device__ uint32_t find(V3 p) const {  // contact_cluster.cpp:361-377
+    __device__ uint32_t find(V3 p) const {  // FindCluster (:361-377)
         uint32_t best = 0;
         float bestValue = kMaxFloat;
-        for (uint32_t i = 0; i < n; ++i) {
+#pragma unroll
+        for (uint32_t i = 0; i < 3u; ++i) {
+            if (i >= n) break;
             float m = dist_sq(p, c[i]);
             if (m < bestValue) {
                 bestValue = m;
@@ -339,215 +345,482 @@ struct Clusters {
         }
         return best;
     }
-    __device__ void add(V3 centroid) {  // contact_cluster.cpp:336-359
+    __device__ void add(V3 centroid) {  // AddCluster (:336-359)
         if (n == 0) {
-            c[n++] = centroid;
+            c[0] = centroid;
+            n = 1;
             return;
         }
-        uint32_t b = find(centroid);
+        const uint32_t b = find(centroid);
         const float kTol = 0.05f;
-        if (dist_sq(centroid, c[b]) <= kTol * kTol) {
-            c[b] = c[b] + centroid;
-            normalize_in_place(c[b]);
-            return;
+#pragma unroll
+        for (uint32_t i = 0; i < 3u; ++i) {
+            if (i != b) continue;
+            if (dist_sq(centroid, c[i]) <= kTol * kTol) {
+                c[i] = c[i] + centroid;
+                normalize_in_place(c[i]);
+                return;
+            }
         }
-        c[n++] = centroid;
+#pragma unroll
+        for (uint32_t i = 0; i < 3u; ++i)
+            if (i == n) c[i] = centroid;
+        ++n;
     }
 };
 
-__global__ __launch_bounds__(64) void mesh_cluster_kernel(MeshArgs a) {
-    const uint32_t ci = blockIdx.x * blockDim.x + threadIdx.x;
-    if (ci >= a.n) return;
-    Manifold* out = a.out + 3 * (size_t)ci;
-    Contact none;
-    none.pointCount = 0;
-    none.overflow = false;
+// b3ClusterSolver::Run (contact_cluster.cpp:475-590) for one mesh contact by a group of
+// kClusterGroup lanes.  The observations sit in LDS in manifold then point order, each as
+// its b3WorldManifoldPoint (manifold.cpp:57-69: world normal, world point, separation,
+// evaluated once) and its manifold point; observation o is lane o % 16's.  Everything that
+// compares or selects runs across the lanes: the cluster assignment, the first maximum or
+// minimum of a selection loop (larger value wins, equal values the smaller observation
+// index, which is the sequential loop's strict comparison in index order), the member
+// lists.  Every floating-point sum keeps the reference's order: a cluster's centroid sum is
+// one lane's walk over the observations in index order, as the reference adds them.  The
+// polygon of a cluster is its members in observation order, so a member's polygon index is
+// never needed: a selection over the polygon is the same selection over the members.
+//
+// The contact's observations get Cap LDS slots.  The pass with Cap = kClusterLds (four
+// contacts in 8 KB per wave) leaves a contact with more observations to the second pass
+// (Cap = kMaxObs, every observation the triangle cap allows), which takes the listed
+// contacts from scratch.
+constexpr uint32_t kClusterGroup = 16;
+constexpr uint32_t kClusterBlock = 64;  // one wave, four contacts
+constexpr uint32_t kClusterLds = 64;
+constexpr uint32_t kClusterWideBlocks = 256;  // grid of the second pass (1,024 contacts at a time)
+constexpr uint32_t kNoIndex = 0xFFFFFFFFu;
+static_assert(kClusterLds % 8u == 0u && kMaxObs % 8u == 0u, "the centroid sums read eight observations at a time");
+
+// LDS written by one lane and read by another lane of the same wave: the fences order the
+// accesses (no code motion across, outstanding LDS operations completed).
+__device__ __forceinline__ void group_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The group's bits of a wave ballot (bit g: lane g of the group).
+__device__ __forceinline__ uint32_t group_ballot(bool p, uint32_t gbase) {
+    return (uint32_t)(__ballot(p) >> gbase) & 0xFFFFu;
+}
+
+// First maximum across the group: each lane brings its own first maximum (v, o), o = kNoIndex
+// when it has none (then v = -FLT_MAX, which no candidate of a strict `v > max` loop from
+// -FLT_MAX can equal).
+__device__ __forceinline__ void group_first_max(float& v, uint32_t& o) {
+#pragma unroll
+    for (int off = 1; off < (int)kClusterGroup; off <<= 1) {
+        const float ov = __shfl_xor(v, off, (int)kClusterGroup);
+        const uint32_t oo = (uint32_t)__shfl_xor((int)o, off, (int)kClusterGroup);
+        if (ov > v || (ov == v && oo < o)) {
+            v = ov;
+            o = oo;
+        }
+    }
+}
+
+// First minimum across the group (v = FLT_MAX, o = kNoIndex for none).
+__device__ __forceinline__ void group_first_min(float& v, uint32_t& o) {
+#pragma unroll
+    for (int off = 1; off < (int)kClusterGroup; off <<= 1) {
+        const float ov = __shfl_xor(v, off, (int)kClusterGroup);
+        const uint32_t oo = (uint32_t)__shfl_xor((int)o, off, (int)kClusterGroup);
+        if (ov < v || (ov == v && oo < o)) {
+            v = ov;
+            o = oo;
+        }
+    }
+}
+
+__device__ __forceinline__ uint32_t group_min_u32(uint32_t x) {
+#pragma unroll
+    for (int off = 1; off < (int)kClusterGroup; off <<= 1) {
+        const uint32_t y = (uint32_t)__shfl_xor((int)x, off, (int)kClusterGroup);
+        x = y < x ? y : x;
+    }
+    return x;
+}
+
+__device__ __forceinline__ V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
+__device__ __forceinline__ uint32_t obs_point(float4 v) { return __float_as_uint(v.w) & 0xFFu; }  // (i << 2) | j
+__device__ __forceinline__ uint32_t obs_cluster(float4 v) { return __float_as_uint(v.w) >> 8; }
+__device__ __forceinline__ uint32_t byte_at(uint32_t packed, uint32_t i) { return (packed >> (8u * i)) & 0xFFu; }
+
+// b3ClosestPointOnSegment (geometry.h:174-193) and the squared distance to it, as
+// InitializeClusters' third pick measures it (contact_cluster.cpp:303-330).
+__device__ __forceinline__ float segment_dist_sq(V3 A, V3 B, V3 Pt) {
+    float w[3];
+    barycentric(w, A, B, Pt);
+    V3 Q;
+    if (w[1] <= 0.0f) Q = A;
+    else if (w[0] <= 0.0f) Q = B;
+    else {
+        float sInv = 1.0f / w[2];
+        float wA = sInv * w[0], wB = sInv * w[1];
+        Q = wA * A + wB * B;
+    }
+    return dist_sq(Pt, Q);
+}
+
+// b3SortPolygon (contact_cluster.cpp:33-56) on up to four members (observation indices packed
+// one per byte of q), positions in LDS; the pivot B is read once per i, before any swap.
+__device__ __forceinline__ uint32_t sort_members(uint32_t q, uint32_t count, const float4* ob, V3 N) {
+    if (count == 0u) return q;
+    const V3 A = xyz(ob[byte_at(q, 0)]);
+    for (uint32_t i = 1; i < count; ++i) {
+        const V3 B = xyz(ob[byte_at(q, i)]);
+        for (uint32_t j = i + 1; j < count; ++j) {
+            const V3 C = xyz(ob[byte_at(q, j)]);
+            if (!is_ccw(A, B, C, N)) {
+                const uint32_t bi = byte_at(q, i), bj = byte_at(q, j);
+                q = (q & ~(0xFFu << (8u * i)) & ~(0xFFu << (8u * j))) | (bj << (8u * i)) | (bi << (8u * j));
+            }
+        }
+    }
+    return q;
+}
+
+// Clusters contact ci with Cap observation slots (obN: world normal and the packed manifold
+// point and cluster; obP: world point, later moved along the cluster normal, and the
+// separation).  With more than Cap observations the group lists the contact in `defer` and
+// writes nothing; returns whether it did.
+template <uint32_t Cap>
+__device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, float4* obN, float4* obP, uint32_t g,
+                                                uint32_t gbase, uint32_t* defer, uint32_t* deferCount) {
+    Manifold* const out = a.out + 3 * (size_t)ci;
     const uint32_t flags = a.contactFlags[ci];
     const uint32_t found = a.triCounts[ci];
     const uint32_t m = found < a.cap ? found : a.cap;
     uint32_t numOut = 0;
+    uint32_t qs0 = 0, qs1 = 0, qs2 = 0, qn0 = 0, qn1 = 0, qn2 = 0;  // each output's members and count
+    const Manifold* in = nullptr;
     if (!(flags & kFlagInvalidPair) && m > 0) {
         const ShapeRec s = a.shapes[a.shapeA[ci]];
         const uint32_t seg = segment_of(s.type);
         const uint32_t base = seg == 0u ? 0u : (seg == 1u ? a.totals[0] : a.totals[0] + a.totals[1]);
-        const Manifold* in = a.temp + base + a.contactOffset[ci];
+        in = a.temp + base + a.contactOffset[ci];
         const Xf xfA = load_xf(a.xfs + 24 * (size_t)ci);
         const Xf xfB = load_xf(a.xfs + 24 * (size_t)ci + 12);
         const float rA = s.radius, rB = 0.0f;
-        // Observations: world normals of every point, in manifold order (:480-495).
-        V3 obs[kMaxObs];
-        uint32_t om[kMaxObs], cl[kMaxObs];
+
+        // Observations (:480-495), manifold by manifold.  Lane g takes manifolds g, g + 16, ...
+        // and loads all four point records with the count (one round trip); an exclusive scan
+        // of the counts places each manifold's points.
         uint32_t nObs = 0;
-        for (uint32_t i = 0; i < m; ++i)
-            for (uint32_t j = 0; j < in[i].pointCount; ++j) {
-                obs[nObs] = world_point(in[i].points[j], rA, xfA, rB, xfB).normal;
-                om[nObs] = (i << 2) | j;
-                ++nObs;
+        for (uint32_t r = 0; r * kClusterGroup < m; ++r) {
+            const uint32_t i = r * kClusterGroup + g;
+            uint32_t pc = 0;
+            ManifoldPoint pts[4];
+            if (i < m) {
+                pc = in[i].pointCount;
+#pragma unroll
+                for (uint32_t j = 0; j < 4u; ++j) pts[j] = in[i].points[j];
             }
+            pc = pc < 4u ? pc : 4u;
+            uint32_t incl = pc;
+#pragma unroll
+            for (uint32_t d = 1; d < kClusterGroup; d <<= 1) {
+                const uint32_t t = (uint32_t)__shfl_up((int)incl, d, (int)kClusterGroup);
+                if (g >= d) incl += t;
+            }
+            const uint32_t total = (uint32_t)__shfl((int)incl, (int)kClusterGroup - 1, (int)kClusterGroup);
+            if (nObs + total > Cap) {  // only with Cap < kMaxObs: nObs <= 4 m <= kMaxObs
+                if (g == 0u && defer) defer[atomicAdd(deferCount, 1u)] = ci;
+                return true;
+            }
+            const uint32_t off = nObs + incl - pc;
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j)
+                if (j < pc) {
+                    const WPoint w = world_point(pts[j], rA, xfA, rB, xfB);
+                    obN[off + j] = make_float4(w.normal.x, w.normal.y, w.normal.z, __uint_as_float((i << 2) | j));
+                    obP[off + j] = make_float4(w.point.x, w.point.y, w.point.z, w.separation);
+                }
+            nObs += total;
+        }
+        group_lds_sync();
+
+        // InitializeClusters (:212-334).
         Clusters C;
         C.n = 0;
-        // InitializeClusters (:212-334).
-        if (nObs <= 3) {
-            for (uint32_t i = 0; i < nObs; ++i) C.add(obs[i]);
+        if (nObs <= 3u) {
+            for (uint32_t o = 0; o < nObs; ++o) C.add(xyz(obN[o]));
         } else {
-            uint64_t chosenLo = 0, chosenHi[3] = {0, 0, 0};  // bit set over nObs <= 256
-            auto chosen = [&](uint32_t i) -> bool {
-                return i < 64 ? ((chosenLo >> i) & 1ull) : ((chosenHi[(i - 64) >> 6] >> ((i - 64) & 63)) & 1ull);
-            };
-            auto choose = [&](uint32_t i) {
-                if (i < 64) chosenLo |= 1ull << i;
-                else chosenHi[(i - 64) >> 6] |= 1ull << ((i - 64) & 63);
-            };
-            C.add(obs[0]);
-            choose(0);
+            C.add(xyz(obN[0]));
+            uint32_t c1;
             {
-                V3 A = C.c[0];
-                uint32_t index = 0;
-                float mx = -kMaxFloat;
-                for (uint32_t i = 0; i < nObs; ++i) {
-                    if (chosen(i)) continue;
-                    float dd = dist_sq(A, obs[i]);
-                    if (dd > mx) {
-                        mx = dd;
-                        index = i;
+                const V3 A = C.c[0];
+                float v = -kMaxFloat;
+                uint32_t idx = kNoIndex;
+                for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                    if (o == 0u) continue;
+                    const float dd = dist_sq(A, xyz(obN[o]));
+                    if (dd > v) {
+                        v = dd;
+                        idx = o;
                     }
                 }
-                C.add(obs[index]);
-                choose(index);
+                group_first_max(v, idx);
+                c1 = idx == kNoIndex ? 0u : idx;
+                C.add(xyz(obN[c1]));
             }
-            if (C.n == 1) {
-                V3 A = C.c[0];
-                uint32_t index = 0;
-                float mx = -kMaxFloat;
-                for (uint32_t i = 0; i < nObs; ++i) {
-                    if (chosen(i)) continue;
-                    float dd = dist_sq(A, obs[i]);
-                    if (dd > mx) {
-                        mx = dd;
-                        index = i;
+            float v = -kMaxFloat;
+            uint32_t idx = kNoIndex;
+            if (C.n == 1u) {
+                const V3 A = C.c[0];
+                for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                    if (o == 0u || o == c1) continue;
+                    const float dd = dist_sq(A, xyz(obN[o]));
+                    if (dd > v) {
+                        v = dd;
+                        idx = o;
                     }
                 }
-                C.add(obs[index]);
-                choose(index);
             } else {
-                V3 A = C.c[0], B = C.c[1];
-                uint32_t index = 0;
-                float mx = -kMaxFloat;
-                for (uint32_t i = 0; i < nObs; ++i) {
-                    if (chosen(i)) continue;
-                    V3 Pt = obs[i];
-                    // b3ClosestPointOnSegment (geometry.h:174-193)
-                    float w[3];
-                    barycentric(w, A, B, Pt);
-                    V3 Q;
-                    if (w[1] <= 0.0f) Q = A;
-                    else if (w[0] <= 0.0f) Q = B;
-                    else {
-                        float sInv = 1.0f / w[2];
-                        float wA = sInv * w[0], wB = sInv * w[1];
-                        Q = wA * A + wB * B;
-                    }
-                    float dd = dist_sq(Pt, Q);
-                    if (dd > mx) {
-                        mx = dd;
-                        index = i;
+                const V3 A = C.c[0], B = C.c[1];
+                for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                    if (o == 0u || o == c1) continue;
+                    const float dd = segment_dist_sq(A, B, xyz(obN[o]));
+                    if (dd > v) {
+                        v = dd;
+                        idx = o;
                     }
                 }
-                C.add(obs[index]);
-                choose(index);
             }
+            group_first_max(v, idx);
+            C.add(xyz(obN[idx == kNoIndex ? 0u : idx]));
         }
-        // Ten Lloyd iterations (:404-444).  The reference assigns every observation, then sums
-        // each cluster's observations in observation order; one pass that assigns observation
-        // j and adds it to its cluster's sum adds the same terms in the same order to each
-        // sum, so the centroids are the same, and the observations are read once per
-        // iteration instead of once per cluster.
-        for (uint32_t iter = 0; iter < 10; ++iter) {
-            V3 sum[4] = {{0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}, {0.0f, 0.0f, 0.0f}};
-            uint32_t count[4] = {0u, 0u, 0u, 0u};
-            for (uint32_t j = 0; j < nObs; ++j) {
-                const V3 o = obs[j];
-                const uint32_t c = C.find(o);
-                if (iter == 9u) cl[j] = c;
+
+        // Ten Lloyd iterations (:404-444).  The assignment of observation o is written into
+        // its LDS word; lane k < n then sums cluster k's observations in index order (eight
+        // LDS reads in flight, the adds in order).  The sums and counts of the last
+        // iteration are the polygon normals' (:512-530 add the same observations in the same
+        // order from zero).  An iteration that assigns every observation as the one before
+        // is a fixed point: the same members give the same centroids (an empty cluster keeps
+        // its centroid either way), so every later iteration repeats it, and the loop stops
+        // there with the state the tenth iteration would leave.
+        V3 sum{0.0f, 0.0f, 0.0f};
+        uint32_t cnt = 0;
+        for (uint32_t iter = 0; iter < 10u; ++iter) {
+            bool changed = iter == 0u;
+            for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                const float4 v = obN[o];
+                const uint32_t c = C.find(xyz(v));
+                changed = changed || c != obs_cluster(v);
+                reinterpret_cast<uint32_t*>(obN + o)[3] = obs_point(v) | (c << 8);
+            }
+            if (group_ballot(changed, gbase) == 0u) break;
+            group_lds_sync();
+            sum = V3{0.0f, 0.0f, 0.0f};
+            cnt = 0;
+            if (g < C.n) {
+                for (uint32_t o = 0; o < nObs; o += 8u) {  // o + 7 < Cap: nObs <= Cap, o % 8 == 0
+                    float4 v[8];
 #pragma unroll
-                for (uint32_t k = 0; k < 4u; ++k)
-                    if (c == k) {
-                        sum[k] = sum[k] + o;
-                        ++count[k];
-                    }
+                    for (uint32_t i = 0; i < 8u; ++i) v[i] = obN[o + i];
+#pragma unroll
+                    for (uint32_t i = 0; i < 8u; ++i)
+                        if (o + i < nObs && obs_cluster(v[i]) == g) {
+                            sum = sum + xyz(v[i]);
+                            ++cnt;
+                        }
+                }
             }
 #pragma unroll
-            for (uint32_t i = 0; i < 4u; ++i)
-                if (i < C.n && count[i] > 0) {
-                    float sInv = 1.0f / (float)count[i];  // b3Vec3::operator/= (vec3.h:70-76)
-                    C.c[i] = V3{sum[i].x * sInv, sum[i].y * sInv, sum[i].z * sInv};
+            for (uint32_t k = 0; k < 3u; ++k) {
+                const float sx = __shfl(sum.x, (int)k, (int)kClusterGroup);
+                const float sy = __shfl(sum.y, (int)k, (int)kClusterGroup);
+                const float sz = __shfl(sum.z, (int)k, (int)kClusterGroup);
+                const uint32_t ck = (uint32_t)__shfl((int)cnt, (int)k, (int)kClusterGroup);
+                if (k < C.n && ck > 0u) {
+                    const float sInv = 1.0f / (float)ck;  // b3Vec3::operator/= (vec3.h:70-76)
+                    C.c[k] = V3{sx * sInv, sy * sInv, sz * sInv};
                 }
-        }
-        // Drop empty clusters, renumbering the rest in order (:449-472).
-        uint32_t used = 0;
-        for (uint32_t i = 0; i < C.n; ++i) {
-            uint32_t cnt = 0;
-            for (uint32_t j = 0; j < nObs; ++j) cnt += cl[j] == i;
-            if (cnt > 0) {
-                for (uint32_t j = 0; j < nObs; ++j)
-                    if (cl[j] == i) cl[j] = used;
-                C.c[used++] = C.c[i];
             }
+            group_lds_sync();
         }
-        C.n = used;
-        // Per cluster: deepest point, offset polygon, reduction, original points (:497-587).
-        for (uint32_t i = 0; i < C.n && numOut < 3; ++i) {
-            CPoly poly, quad;
-            poly.count = 0;
-            V3 normal{0.0f, 0.0f, 0.0f};
-            for (uint32_t j = 0; j < nObs; ++j) {
-                if (cl[j] != i) continue;
-                const ManifoldPoint& mp = in[om[j] >> 2].points[om[j] & 3u];
-                if (poly.count < (uint32_t)kPolyCap) {
-                    poly.v[poly.count].position = world_point(mp, rA, xfA, rB, xfB).point;
-                    poly.v[poly.count].clipIndex = j;
-                    ++poly.count;
-                }
-                normal = normal + obs[j];
-            }
-            if (poly.count == 0) continue;
+        group_lds_sync();
+
+        // Per non-empty cluster, in cluster order (the renumbering of :449-472 keeps the
+        // order): the members' world points moved along the normal, the deepest member and
+        // b3ReducePolygon (:58-200).
+        for (uint32_t k = 0; k < C.n && numOut < 3u; ++k) {
+            const uint32_t ck = (uint32_t)__shfl((int)cnt, (int)k, (int)kClusterGroup);
+            V3 normal{__shfl(sum.x, (int)k, (int)kClusterGroup), __shfl(sum.y, (int)k, (int)kClusterGroup),
+                      __shfl(sum.z, (int)k, (int)kClusterGroup)};
+            if (ck == 0u) continue;
             normalize_in_place(normal);
-            uint32_t minIndex = 0;
-            float minSep = kMaxFloat;
-            for (uint32_t j = 0; j < poly.count; ++j) {
-                const uint32_t o = poly.v[j].clipIndex;
-                WPoint w = world_point(in[om[o] >> 2].points[om[o] & 3u], rA, xfA, rB, xfB);
-                if (w.separation < minSep) {
-                    minIndex = j;
-                    minSep = w.separation;
+            float sepMin = kMaxFloat;
+            uint32_t minO = kNoIndex, first = kNoIndex;
+            for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                if (obs_cluster(obN[o]) != k) continue;
+                const float4 w = obP[o];
+                if (w.w < sepMin) {
+                    sepMin = w.w;
+                    minO = o;
                 }
-                poly.v[j].position = poly.v[j].position + w.separation * normal;
+                first = first < o ? first : o;
+                const V3 pos = xyz(w) + w.w * normal;
+                obP[o] = make_float4(pos.x, pos.y, pos.z, w.w);
             }
-            reduce_polygon(quad, poly, normal, minIndex);
-            Manifold mo;
-            mo.pointCount = quad.count;
-            mo.flags = flags;
-            mo.reserved[0] = mo.reserved[1] = 0;
-            for (uint32_t j = 0; j < 4; ++j) {
-                if (j < quad.count) {
-                    const uint32_t o = quad.v[j].clipIndex;
-                    mo.points[j] = in[om[o] >> 2].points[om[o] & 3u];
-                } else {
-                    mo.points[j].localNormal1 = V3{0.0f, 0.0f, 0.0f};
-                    mo.points[j].localPoint1 = V3{0.0f, 0.0f, 0.0f};
-                    mo.points[j].localPoint2 = V3{0.0f, 0.0f, 0.0f};
-                    mo.points[j].triangleKey = 0;
-                    mo.points[j].key[0] = mo.points[j].key[1] = mo.points[j].key[2] = mo.points[j].key[3] = 0;
+            group_first_min(sepMin, minO);
+            first = group_min_u32(first);
+            group_lds_sync();
+            // b3ReducePolygon: with four points or fewer all of them, sorted; else the
+            // deepest, then the farthest from it, the largest triangle, the most negative
+            // signed area, sorted.  Members are tested by their cluster; chosen members are
+            // skipped.  A selection with no candidate keeps index 0: the first member.
+            uint32_t q = 0, qn = 0;
+            if (ck <= 4u) {
+                for (uint32_t r = 0; r * kClusterGroup < nObs && qn < ck; ++r) {
+                    const uint32_t o = r * kClusterGroup + g;
+                    uint32_t bits = group_ballot(o < nObs && obs_cluster(obN[o]) == k, gbase);
+                    while (bits != 0u && qn < 4u) {
+                        q |= (r * kClusterGroup + (uint32_t)__builtin_ctz(bits)) << (8u * qn);
+                        ++qn;
+                        bits &= bits - 1u;
+                    }
                 }
+                q = sort_members(q, qn, obP, normal);
+            } else {
+                const uint32_t q0 = minO == kNoIndex ? first : minO;
+                q = q0;
+                qn = 1;
+                do {
+                    const V3 A = xyz(obP[q0]);
+                    float v = -kMaxFloat;
+                    uint32_t idx = kNoIndex;
+                    for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                        if (obs_cluster(obN[o]) != k || o == q0) continue;
+                        const V3 d = xyz(obP[o]) - A;
+                        const float dd = dot(d, d);
+                        if (dd > v) {
+                            v = dd;
+                            idx = o;
+                        }
+                    }
+                    group_first_max(v, idx);
+                    if (v < kEpsilon * kEpsilon) break;
+                    const uint32_t q1 = idx;
+                    q |= q1 << 8;
+                    qn = 2;
+                    const V3 B = xyz(obP[q1]);
+                    v = -kMaxFloat;
+                    idx = kNoIndex;
+                    for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                        if (obs_cluster(obN[o]) != k || o == q0 || o == q1) continue;
+                        const V3 n = cross(B - A, xyz(obP[o]) - A);
+                        const float a2 = fabsf(dot(n, normal));
+                        if (a2 > v) {
+                            v = a2;
+                            idx = o;
+                        }
+                    }
+                    group_first_max(v, idx);
+                    const float kAreaTol = 0.01f;
+                    if (v < 2.0f * kAreaTol) break;
+                    const uint32_t q2 = idx;
+                    if (!is_ccw(A, B, xyz(obP[q2]), normal)) q = q1 | (q0 << 8);
+                    q |= q2 << 16;
+                    qn = 3;
+                    const V3 A2 = xyz(obP[byte_at(q, 0)]), B2 = xyz(obP[byte_at(q, 1)]);
+                    v = kMaxFloat;
+                    idx = kNoIndex;
+                    for (uint32_t o = g; o < nObs; o += kClusterGroup) {
+                        if (obs_cluster(obN[o]) != k || o == q0 || o == q1 || o == q2) continue;
+                        const V3 n = cross(B2 - A2, xyz(obP[o]) - A2);
+                        const float sa2 = dot(n, normal);
+                        if (sa2 < v) {
+                            v = sa2;
+                            idx = o;
+                        }
+                    }
+                    group_first_min(v, idx);
+                    if (fabsf(v) < 2.0f * kAreaTol) break;
+                    q |= (idx == kNoIndex ? first : idx) << 24;
+                    qn = 4;
+                    q = sort_members(q, 4u, obP, normal);
+                } while (false);
             }
-            out[numOut++] = mo;
+            // The members' manifold points (packed (i << 2) | j, one byte each).
+            uint32_t pts = 0;
+#pragma unroll
+            for (uint32_t j = 0; j < 4u; ++j)
+                if (j < qn) pts |= obs_point(obN[byte_at(q, j)]) << (8u * j);
+            if (numOut == 0u) qs0 = pts, qn0 = qn;
+            else if (numOut == 1u) qs1 = pts, qn1 = qn;
+            else qs2 = pts, qn2 = qn;
+            ++numOut;
+            group_lds_sync();
         }
     }
-    for (uint32_t i = numOut; i < 3; ++i) {
-        write_manifold(out + i, none);
-        out[i].flags = flags;
+    // The output manifolds (:587-589): lane 4 t + j copies point j of output t (its original
+    // manifold point, or zeros), lane 12 + t writes output t's header; outputs past numOut
+    // are empty manifolds with the contact's flags, 15 16-byte words each.
+    static_assert(sizeof(Manifold) == 15 * 16, "manifold of 15 16-byte words");
+    if (g < 12u) {
+        const uint32_t t = g >> 2, j = g & 3u;
+        if (t < numOut) {
+            const uint32_t qsT = t == 0u ? qs0 : (t == 1u ? qs1 : qs2);
+            const uint32_t qnT = t == 0u ? qn0 : (t == 1u ? qn1 : qn2);
+            ManifoldPoint pt;
+            if (j < qnT) {
+                const uint32_t pi = byte_at(qsT, j);
+                pt = in[pi >> 2].points[pi & 3u];
+            } else {
+                pt.localNormal1 = V3{0.0f, 0.0f, 0.0f};
+                pt.localPoint1 = V3{0.0f, 0.0f, 0.0f};
+                pt.localPoint2 = V3{0.0f, 0.0f, 0.0f};
+                pt.triangleKey = 0;
+                pt.key[0] = pt.key[1] = pt.key[2] = pt.key[3] = 0;
+            }
+            out[t].points[j] = pt;
+        }
+    } else if (g < 15u) {
+        const uint32_t t = g - 12u;
+        if (t < numOut) {
+            const uint32_t qnT = t == 0u ? qn0 : (t == 1u ? qn1 : qn2);
+            *reinterpret_cast<uint4*>(out + t) = make_uint4(qnT, flags, 0u, 0u);
+        }
     }
-    a.manifoldCounts[ci] = numOut;
+    for (uint32_t t = numOut; t < 3u; ++t)
+        if (g < 15u)
+            reinterpret_cast<uint4*>(out + t)[g] = g == 0u ? make_uint4(0u, flags, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
+    if (g == 0u) a.manifoldCounts[ci] = numOut;
+    return false;
+}
+
+// Every contact, four per wave, kClusterLds observation slots each; contacts with more
+// observations are listed (pairContact, free once the narrowphase has run; count in
+// totals[3]) for mesh_cluster_wide_kernel.
+__global__ __launch_bounds__(kClusterBlock) void mesh_cluster_kernel(MeshArgs a) {
+    __shared__ float4 obsN[kClusterBlock / kClusterGroup][kClusterLds];
+    __shared__ float4 obsP[kClusterBlock / kClusterGroup][kClusterLds];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane & (kClusterGroup - 1u);
+    const uint32_t gbase = lane & ~(kClusterGroup - 1u);
+    const uint32_t ci = blockIdx.x * (kClusterBlock / kClusterGroup) + lane / kClusterGroup;
+    if (ci >= a.n) return;  // whole groups leave; nothing below waits on other groups
+    cluster_contact<kClusterLds>(a, ci, obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g, gbase,
+                                 a.pairContact, a.totals + 3);
+}
+
+// The listed contacts with every observation (kMaxObs slots each): group q of the grid takes
+// entries q, q + groups, ...
+__global__ __launch_bounds__(kClusterBlock) void mesh_cluster_wide_kernel(MeshArgs a) {
+    __shared__ float4 obsN[kClusterBlock / kClusterGroup][kMaxObs];
+    __shared__ float4 obsP[kClusterBlock / kClusterGroup][kMaxObs];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t g = lane & (kClusterGroup - 1u);
+    const uint32_t gbase = lane & ~(kClusterGroup - 1u);
+    const uint32_t groups = gridDim.x * (kClusterBlock / kClusterGroup);
+    const uint32_t count = a.totals[3];
+    for (uint32_t e = blockIdx.x * (kClusterBlock / kClusterGroup) + lane / kClusterGroup; e < count; e += groups) {
+        cluster_contact<kMaxObs>(a, a.pairContact[e], obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g,
+                                 gbase, nullptr, nullptr);
+        group_lds_sync();
+    }
 }
 
 }  // namespace
@@ -635,7 +908,11 @@ hipError_t launch_mesh_overlap(const MeshArgs& a, hipStream_t s) {
 
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
-    mesh_cluster_kernel<<<(a.n + 63) / 64, 64, 0, s>>>(a);
+    constexpr uint32_t perBlock = kClusterBlock / kClusterGroup;
+    mesh_cluster_kernel<<<(a.n + perBlock - 1) / perBlock, kClusterBlock, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    mesh_cluster_wide_kernel<<<kClusterWideBlocks, kClusterBlock, 0, s>>>(a);
     return hipGetLastError();
 }
 

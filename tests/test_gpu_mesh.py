@@ -4,7 +4,7 @@ import numpy as np
 import pytest
 
 import oracle_lib
-from util import assert_manifolds_equal, load_golden, mesh_workload
+from util import assert_manifolds_equal, load_golden, mesh_workload, plate_workload
 
 pytestmark = pytest.mark.gpu
 
@@ -39,6 +39,26 @@ def test_gpu_mesh_equals_oracle(ctx, lib, oracle, seed, size):
     np.testing.assert_array_equal(counts, co[0])
     np.testing.assert_array_equal(mcount, co[3])
     assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "mesh manifolds")
+
+
+@pytest.mark.parametrize("seed", [5, 6])
+def test_gpu_mesh_large_clusters_equal_oracle(ctx, oracle, seed):
+    """Flat boxes on a flat grid: each contact's one cluster gathers 40-100 points from up to
+    50 triangle face contacts, beyond the reference's 32-point inline polygon; the reduction
+    runs over all of them (contact_cluster.cpp:497-587)."""
+    print("case: plates on a flat grid, seed=%d" % seed, flush=True)
+    pool, shapes, v, t, xfs = plate_workload(seed, 40)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    nodes = ctx.mesh_tree(mid)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    counts, lists, out, mcount = ctx.collide_mesh(mid, sa, xfs)
+    assert (counts <= ctx.MESH_TRIANGLE_CAP).all()
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs)
+    np.testing.assert_array_equal(counts, co[0])
+    np.testing.assert_array_equal(mcount, co[3])
+    assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "plate manifolds")
 
 
 def test_gpu_mesh_heightfield_full_size(ctx, lib, oracle):

@@ -262,6 +262,24 @@ def test_oracle_mesh_equals_reference(lib, oracle, reference):
     assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "clustered mesh manifolds")
 
 
+@pytest.mark.parametrize("seed", [5, 6])
+def test_oracle_mesh_large_clusters_equal_reference(oracle, reference, seed):
+    """Clusters of 40-100 points (flat boxes on a flat grid): the reference's polygon grows
+    past its 32-point inline capacity; the restatement keeps every point.  A 32-point cap
+    changes about one manifold in nine here, so the case tells the two apart."""
+    import oracle_lib
+    from util import plate_workload
+    pool, shapes, v, t, xfs = plate_workload(seed, 40)
+    nodes = oracle_lib.build_tree(reference.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs)
+    cr = oracle_lib.collide_mesh(reference.lib, "b3ref_", pool, shapes, v, t, nodes, sa, xfs)
+    np.testing.assert_array_equal(co[0], cr[0])
+    np.testing.assert_array_equal(co[3], cr[3])
+    assert (cr[0] <= 64).all()  # triangle lists within the device cap
+    assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "plate manifolds")
+
+
 def test_oracle_mesh_golden(oracle):
     import oracle_lib
     g = load_golden("mesh_contacts.npz")

@@ -364,6 +364,38 @@ def mesh_workload(lib, seed, H, W, n_shapes, gap_sigma=0.05):
     return pool, shapes, verts, tris, xfs
 
 
+def plate_workload(seed, n_shapes, H=24, W=24, cell=0.3):
+    """Flat boxes (half-extents 0.18-0.25 m in x and z, 0.1 m high) lying on a flat grid of
+    0.3 m cells, turned about the vertical and tilted by at most 0.01 rad, sunk 1-10 mm
+    into the plane: every triangle under a box gives a face contact of up to four points,
+    all with one normal, so a contact's single cluster holds more than 32 points while its
+    triangle list (the fattened AABB's footprint) stays within 64.
+    Returns (pool, shapes, vertices, triangles, xfs [n][2][12] (box, mesh))."""
+    from bounce_amd.shapes import box_hull, identity_xf
+    from bounce_amd.types import HULL, SHAPE_DTYPE, HullPool
+    rng = np.random.default_rng(seed)
+    verts, tris = grid_mesh(H, W, cell=cell, amplitude=0.0, seed=seed)
+    ext = [(0.25, 0.2), (0.2, 0.2), (0.25, 0.25), (0.22, 0.18)]
+    pool = HullPool.concat([box_hull(ex, 0.1, ez) for ex, ez in ext])
+    shapes = np.zeros(n_shapes, SHAPE_DTYPE)
+    shapes["type"] = HULL
+    shapes["hull"] = rng.integers(0, len(ext), n_shapes)
+    xfs = np.zeros((n_shapes, 2, 12), np.float32)
+    half = 0.5 * cell * np.array([W, H])
+    for i in range(n_shapes):
+        yaw = rng.uniform(0.0, 2.0 * np.pi)
+        c, s = np.cos(yaw), np.sin(yaw)
+        R = np.array([[c, 0.0, s], [0.0, 1.0, 0.0], [-s, 0.0, c]])
+        tx, tz = rng.uniform(-0.01, 0.01, 2)
+        Rx = np.array([[1.0, 0.0, 0.0], [0.0, np.cos(tx), -np.sin(tx)], [0.0, np.sin(tx), np.cos(tx)]])
+        Rz = np.array([[np.cos(tz), -np.sin(tz), 0.0], [np.sin(tz), np.cos(tz), 0.0], [0.0, 0.0, 1.0]])
+        x, z = rng.uniform(-half[0] + 1.0, half[0] - 1.0, 2)
+        y = 0.1 - rng.uniform(0.001, 0.01)
+        xfs[i, 0] = identity_xf((x, y, z), Rx @ Rz @ R)
+        xfs[i, 1] = identity_xf((0, 0, 0))
+    return pool, shapes, verts, tris, xfs
+
+
 # ---- the box pyramid (210 unit boxes on a static ground) stepped by the reference world
 
 PYRAMID_STEPS = 300
