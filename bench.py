@@ -80,10 +80,11 @@ def _cpu_worker(chunk):
     return t, time.perf_counter(), int((m["pointCount"] > 0).sum())
 
 
-def cpu_baseline(pool, pairs, xfs, sample):
+def cpu_baseline(pool, pairs, xfs, sample, shapes=None):
     """The reference narrowphase compiled from its own sources (oracle/_ref), or the
     oracle restatement where that is absent, on a strided sample of the same pairs,
-    one process per core (the reference writes unsynchronised global counters)."""
+    one process per core (the reference writes unsynchronised global counters).  `shapes`
+    defaults to one hull shape per pool hull."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_lib
     kind = "reference" if oracle_lib.reference() is not None else "port"
@@ -92,7 +93,7 @@ def cpu_baseline(pool, pairs, xfs, sample):
     stride = max(1, len(pairs) // sample)
     idx = np.arange(0, len(pairs), stride)[:sample]
     chunks = np.array_split(idx, cores)
-    _CPU_JOB.update(kind=kind, pool=pool, shapes=pool.shapes(), pairs=pairs, xfs=xfs)
+    _CPU_JOB.update(kind=kind, pool=pool, shapes=pool.shapes() if shapes is None else shapes, pairs=pairs, xfs=xfs)
     ctx = mp.get_context("fork")
     with ctx.Pool(cores) as p:
         p.map(_cpu_worker, [c[:64] for c in chunks])  # load the library in every worker
@@ -213,6 +214,34 @@ def mesh_cpu_baseline(pool, shapes, verts, tris, xfs, sample):
                            "beforehand in each process)" % (len(idx), len(shapes), cores, cpu_s, wall)}, **host)
 
 
+def mesh_algorithmic_work(world, nodes, counts, stride=50):
+    """FP32 ops of the mesh path (tree query, per-triangle b3CollideShapeAndShape,
+    b3ClusterSolver::Run) from the op-counting oracle on every `stride`-th contact, scaled to
+    the batch; and the algorithmic HBM bytes of one batch: per contact its shape index,
+    shape record (48 B), transforms (96 B), triangle count and list words, three output
+    manifolds (720 B) and manifold count; per (contact, triangle) work item the item (8 B,
+    written and read), the triangle and its three vertices (64 B) and the per-triangle
+    manifold (240 B, written by the narrowphase and read by the clustering); per hull contact
+    the hull record once (12 V + 16 HE + 20 F + 24 B).  Tree nodes are not counted."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    pool, shapes, verts, tris, xfs = world
+    oc = oracle_lib.oracle_counting()
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    idx = sa[::stride]
+    oc.lib.b3o_flop_reset()
+    oracle_lib.collide_mesh(oc.lib, "b3o_", pool, shapes, verts, tris, nodes, idx, xfs[idx])
+    flops = float(oc.lib.b3o_flop_count()) * len(sa) / len(idx)
+    items = np.minimum(counts.astype(np.int64), 64)
+    d = pool.desc
+    hul = shapes["type"] == 2
+    h = shapes["hull"][hul].astype(np.int64)
+    hull_bytes = (12 * d[h, 1].astype(np.int64) + 16 * d[h, 3].astype(np.int64) + 20 * d[h, 5].astype(np.int64) + 24)
+    n = len(shapes)
+    byts = n * (4 + 48 + 96 + 4 + 720 + 4) + 4 * items.sum() + items.sum() * (16 + 64 + 480) + hull_bytes.sum()
+    return {"flops": flops, "bytes": float(byts), "sample": len(idx)}
+
+
 def mesh_run(world, steps, cpu):
     """Mixed shapes on a 256 x 256 heightfield (131,072 triangles): 100,000 fresh mesh
     contacts per batch (static-tree midphase, per-triangle narrowphase, clustering)."""
@@ -253,8 +282,20 @@ def mesh_run(world, steps, cpu):
         ctx.set_profiling(False)
         kern = {k: ctx.kernel_stats(k) for k in ("mesh_midphase", "mesh_narrow", "mesh_cluster")}
         same = bool(np.array_equal(do.download(np.zeros_like(out)).view(np.uint8), out.view(np.uint8)))
+        nodes = ctx.mesh_tree(mid)
     kms = {k: v[1] / max(1, v[0]) for k, v in kern.items()}
     dev_ms = sum(kms.values())
+    work = mesh_algorithmic_work(world, nodes, counts)
+    tflops = work["flops"] / (dev_ms / 1000.0) / 1e12
+    gbs = work["bytes"] / (dev_ms / 1000.0) / 1e9
+    res["roofline"] = {
+        "bound": "valu", "achieved": tflops, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+        "frac": tflops / VALU_PEAK_TFLOPS, "unfused_frac": tflops / VALU_UNFUSED_TFLOPS,
+        "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS},
+        "algorithmic_flops_per_batch": work["flops"], "algorithmic_bytes_per_batch": work["bytes"],
+        "what": "the three mesh kernels together (their summed HIP-event times); FP32 ops counted by "
+                "the op-counting oracle on every 50th contact, scaled; bytes as mesh_algorithmic_work "
+                "states (tree nodes not counted)"}
     res.update({"ms_per_batch": wall * 1000.0, "contacts_per_s": n / wall,
                 "host_arrays_ms_per_batch": host_wall * 1000.0, "host_arrays_contacts_per_s": n / host_wall,
                 "kernels_ms": kms, "kernels_sum_ms": dev_ms, "device_equals_host_path": same,
@@ -265,6 +306,74 @@ def mesh_run(world, steps, cpu):
                         "arrays beside it"})
     if "cpu_baseline" in res:
         res["gpu_over_cpu"] = res["contacts_per_s"] / res["cpu_baseline"]["value"]
+    return res
+
+
+# ----------------------------------------------------------------- sphere/capsule pairs
+
+CONVEX_PAIRS = 1 << 20
+
+
+def convex_run(world, steps, cpu):
+    """The convex narrowphase (b3CollideSphereAndSphere ... b3CollideCapsuleAndHull,
+    collide.cpp:172-223, GJK for the hull pairs) on 1,048,576 fresh sphere/capsule pairs of
+    gen_convex_pairs, inputs and outputs resident: pairs/s of the batch and the convex
+    kernel's time, with its roofline (FP32 ops from the op-counting oracle on every 64th
+    pair; bytes: pair, transforms, two shape records, the manifold, and 12 B per vertex of a
+    hull the GJK reads)."""
+    import bounce_amd
+    from bounce_amd.types import MANIFOLD_DTYPE
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_lib
+    pool, shapes, pairs, xfs = world
+    n = len(pairs)
+    res = {"workload": "1,048,576 fresh pairs, equal shares of sphere-sphere, sphere-capsule, sphere-hull, "
+                       "capsule-capsule, capsule-hull; spheres r 0.1-0.5, capsules r 0.1-0.3 L 0-1, hulls "
+                       "8-32 points 0.2-1 m; placed along a random direction at touching distance + "
+                       "N(-0.02, 0.08) m, 5 % nearly concentric", "pairs": n}
+    if cpu is not None:
+        res["cpu_baseline"] = cpu
+    with bounce_amd.Context(0) as ctx:
+        ctx.upload_hulls(pool)
+        ctx.upload_shapes(shapes)
+        dp, dx, do = ctx.alloc(pairs.nbytes), ctx.alloc(xfs.nbytes), ctx.alloc(n * MANIFOLD_DTYPE.itemsize)
+        dp.upload(pairs)
+        dx.upload(xfs)
+        ctx.collide_device(n, dp, dx, do)
+        ctx.synchronize()
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.collide_device(n, dp, dx, do)
+        ctx.synchronize()
+        wall = (time.perf_counter() - t0) / steps
+        ctx.set_profiling(False)
+        k = ctx.kernel_stats("convex")
+        out = do.download(np.zeros(n, MANIFOLD_DTYPE))
+    kms = k[1] / max(1, k[0])
+    idx = np.arange(0, n, 64)
+    oc = oracle_lib.oracle_counting()
+    oc.lib.b3o_flop_reset()
+    ref = oc.collide(pool, shapes, pairs[idx], xfs[idx])
+    flops = float(oc.lib.b3o_flop_count()) * n / len(idx)
+    same = bool(np.array_equal(out[idx].view(np.uint8), ref.view(np.uint8)))
+    hv = np.where(shapes["type"] == 2, pool.desc[shapes["hull"], 1].astype(np.int64), 0)
+    byts = float(n * (8 + 96 + 96 + 240) + 12 * hv.sum())
+    tflops = flops / (kms / 1000.0) / 1e12
+    gbs = byts / (kms / 1000.0) / 1e9
+    res.update({"ms_per_batch": wall * 1000.0, "pairs_per_s": n / wall, "convex_kernel_ms": kms,
+                "touching_fraction": float((out["pointCount"] > 0).mean()),
+                "sample_equals_oracle": same,
+                "roofline": {"bound": "valu", "achieved": tflops, "peak": VALU_PEAK_TFLOPS, "unit": "TFLOP/s",
+                             "frac": tflops / VALU_PEAK_TFLOPS, "unfused_frac": tflops / VALU_UNFUSED_TFLOPS,
+                             "hbm": {"achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                     "frac": gbs / HBM_PEAK_GBS},
+                             "algorithmic_flops_per_batch": flops, "algorithmic_bytes_per_batch": byts},
+                "what": "b3amd_collide_device on resident inputs; the convex kernel runs on its own stream "
+                        "beside the (empty) hull kernels"})
+    if "cpu_baseline" in res:
+        res["gpu_over_cpu"] = res["pairs_per_s"] / res["cpu_baseline"]["value"]
     return res
 
 
@@ -938,6 +1047,7 @@ def main():
     ap.add_argument("--no-serving", action="store_true")
     ap.add_argument("--no-pyramid", action="store_true")
     ap.add_argument("--no-mesh", action="store_true")
+    ap.add_argument("--no-convex", action="store_true", help="skip the sphere/capsule pair batch")
     ap.add_argument("--no-plain", action="store_true", help="skip the host-array (PCIe) drop-in timing")
     ap.add_argument("--no-warm", action="store_true", help="skip the large warm world")
     ap.add_argument("--warm-steps", type=int, default=600)
@@ -975,6 +1085,14 @@ def main():
         if not args.no_cpu:  # also before the GPU is initialised
             mesh_cpu = mesh_cpu_baseline(*mesh_world, 100000)
             log("mesh cpu baseline: %.0f contacts/s (%s)" % (mesh_cpu["value"], mesh_cpu["kind"]))
+    convex_world = convex_cpu = None
+    if not args.no_convex:
+        from bounce_amd.workloads import gen_convex_pairs
+        convex_world = gen_convex_pairs(lib, 5, CONVEX_PAIRS)
+        if not args.no_cpu:  # also before the GPU is initialised
+            convex_cpu = cpu_baseline(convex_world[0], convex_world[2], convex_world[3], CONVEX_PAIRS,
+                                      shapes=convex_world[1])
+            log("convex cpu baseline: %.0f pairs/s (%s)" % (convex_cpu["value"], convex_cpu["kind"]))
     warm_world = warm_cpu = None
     if not args.no_warm:
         from bounce_amd.workloads import gen_warm_world
@@ -1103,6 +1221,11 @@ def main():
         log("mesh: %.0f contacts/s resident (%.3f ms per batch), %.0f contacts/s from host arrays"
             % (result["mesh"]["contacts_per_s"], result["mesh"]["ms_per_batch"],
                result["mesh"]["host_arrays_contacts_per_s"]))
+    if not args.no_convex:
+        result["convex"] = convex_run(convex_world, 5, convex_cpu)
+        log("convex: %.0f pairs/s resident (%.3f ms per batch, convex kernel %.3f ms, %.3f of VALU peak)"
+            % (result["convex"]["pairs_per_s"], result["convex"]["ms_per_batch"],
+               result["convex"]["convex_kernel_ms"], result["convex"]["roofline"]["frac"]))
     if not args.no_pyramid:
         result["pyramid"] = pyramid_run(lib)
         log("pyramid: %d/%d steps equal to the reference, %.3f ms per step; whole step on the library: %d/%d "

@@ -997,7 +997,7 @@ struct MeshScratch {
     uint32_t* flags;
     uint32_t* offsets;
     uint32_t* totals;
-    uint32_t* pairContact;
+    uint2* pairContact;
     Manifold* out;
     uint32_t* manifoldCounts;
     Manifold* temp;
@@ -1009,7 +1009,7 @@ int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
     const uint64_t cap = kMeshTriangleCap;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
     const uint64_t sizes[13] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
-                                4ull * n,  256,       12ull * n * cap, 720ull * n, 4ull * n,
+                                4ull * n,  256,       24ull * n * cap, 720ull * n, 4ull * n,
                                 sizeof(Manifold) * n * cap, 64ull * n, 12ull * n};
     uint64_t total = 0;
     for (uint64_t b : sizes) total += al(b);
@@ -1028,7 +1028,7 @@ int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
         p += al(sizes[i]);
     }
     *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2],  (uint32_t*)parts[3],
-                     (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint32_t*)parts[7],
+                     (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint2*)parts[7],
                      (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10], (float*)parts[11],
                      (float*)parts[12]};
     return B3AMD_OK;

@@ -16,7 +16,8 @@ namespace b3amd {
 
 // ------------------------------------------------------------------ GJK
 
-// A GJK proxy (gjk_proxy.h:26-44): up to two local points, or the vertices of a hull.
+// A GJK proxy (gjk_proxy.h:26-44): up to three local points, or the vertices of a hull.
+// Every index into the local points is a selection, so that a proxy stays in registers.
 struct Proxy {
     V3 pts[3];
     uint32_t count;
@@ -26,13 +27,40 @@ struct Proxy {
             float4 q = hullVerts[i];
             return V3{q.x, q.y, q.z};
         }
-        return pts[i];
+        return i == 0u ? pts[0] : (i == 1u ? pts[1] : pts[2]);
     }
-    // gjk_proxy.h:57-71
+    // gjk_proxy.h:57-71.  A hull's vertices are read four at a time (the loads in flight
+    // together), then compared in index order.
     __device__ uint32_t support(V3 d) const {
         uint32_t maxIndex = 0;
         float maxProjection = dot(d, vertex(0));
-        for (uint32_t i = 1; i < count; ++i) {
+        if (!hullVerts) {
+#pragma unroll
+            for (uint32_t i = 1; i < 3u; ++i) {
+                if (i >= count) break;
+                float q = dot(d, pts[i]);
+                if (q > maxProjection) {
+                    maxIndex = i;
+                    maxProjection = q;
+                }
+            }
+            return maxIndex;
+        }
+        uint32_t i = 1;
+        for (; i + 4u <= count; i += 4u) {
+            float4 w[4];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) w[k] = hullVerts[i + k];
+#pragma unroll
+            for (uint32_t k = 0; k < 4u; ++k) {
+                float q = dot(d, V3{w[k].x, w[k].y, w[k].z});
+                if (q > maxProjection) {
+                    maxIndex = i + k;
+                    maxProjection = q;
+                }
+            }
+        }
+        for (; i < count; ++i) {
             float q = dot(d, vertex(i));
             if (q > maxProjection) {
                 maxIndex = i;
@@ -241,15 +269,25 @@ __device__ __forceinline__ void set_cache_byte(uint32_t* w, uint32_t k, uint32_t
 // (b3_gjkCacheHits).
 __device__ inline GJKOut gjk_cached(const Xf& xf1, const Proxy& p1, const Xf& xf2, const Proxy& p2, uint32_t* cache,
                                     bool& cacheHit) {
+    // Every loop over the simplex's vertices runs over all four slots with a bound test, so
+    // that the simplex stays in registers.
     Simplex s;
     cacheHit = false;
     s.count = 0;
+#pragma unroll
+    for (uint32_t i = 0; i < 4u; ++i) s.v[i].index1 = s.v[i].index2 = 0u;
     if (cache) {
         s.count = cache[2] & 0xFFu;  // (u8)count
         if (s.count > 4u) s.count = 0;
-        for (uint32_t i = 0; i < s.count; ++i)
-            if (cache_byte(cache, 10u + i) >= p1.count || cache_byte(cache, 14u + i) >= p2.count) s.count = 0;
-        for (uint32_t i = 0; i < s.count; ++i) {
+        bool inRange = true;
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i)
+            if (i < s.count && (cache_byte(cache, 10u + i) >= p1.count || cache_byte(cache, 14u + i) >= p2.count))
+                inRange = false;
+        if (!inRange) s.count = 0;
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) {
+            if (i >= s.count) break;
             SVert& v = s.v[i];
             v.index1 = cache_byte(cache, 10u + i);
             v.index2 = cache_byte(cache, 14u + i);
@@ -278,7 +316,8 @@ __device__ inline GJKOut gjk_cached(const Xf& xf1, const Proxy& p1, const Xf& xf
     uint32_t iter = 0;
     while (iter < 20) {
         const uint32_t saveCount = s.count;
-        for (uint32_t i = 0; i < saveCount; ++i) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) {
             save1[i] = s.v[i].index1;
             save2[i] = s.v[i].index2;
         }
@@ -301,17 +340,23 @@ __device__ inline GJKOut gjk_cached(const Xf& xf1, const Proxy& p1, const Xf& xf
             d = dot(n, AQ) > 0.0f ? n : -n;
         }
         if (dot(d, d) < kEpsilon * kEpsilon) break;
-        SVert& vx = s.v[s.count];
+        SVert vx;
         vx.index1 = p1.support(mulT(xf1.R, -d));
         vx.point1 = mul(xf1, p1.vertex(vx.index1));
         vx.index2 = p2.support(mulT(xf2.R, d));
         vx.point2 = mul(xf2, p2.vertex(vx.index2));
         vx.point = vx.point2 - vx.point1;
         vx.weight = 0.0f;
+        // The new vertex takes slot count (below four here), also when it repeats a saved one
+        // and the search stops: the output below reads only the first count slots.
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i)
+            if (i == s.count) s.v[i] = vx;
         ++iter;
         bool dup = false;
-        for (uint32_t i = 0; i < saveCount; ++i)
-            if (vx.index1 == save1[i] && vx.index2 == save2[i]) dup = true;
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i)
+            if (i < saveCount && vx.index1 == save1[i] && vx.index2 == save2[i]) dup = true;
         if (dup) break;
         ++s.count;
     }
@@ -337,7 +382,9 @@ __device__ inline GJKOut gjk_cached(const Xf& xf1, const Proxy& p1, const Xf& xf
     if (cache) {  // WriteCache: metric, count and the used indices; the rest of the record kept
         cache[0] = __float_as_uint(simplex_metric(s));
         cache[2] = (cache[2] & 0xFFFF0000u) | s.count;
-        for (uint32_t i = 0; i < s.count; ++i) {
+#pragma unroll
+        for (uint32_t i = 0; i < 4u; ++i) {
+            if (i >= s.count) break;
             set_cache_byte(cache, 10u + i, s.v[i].index1);
             set_cache_byte(cache, 14u + i, s.v[i].index2);
         }

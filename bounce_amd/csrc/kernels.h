@@ -317,8 +317,9 @@ struct MeshArgs {
     uint32_t* contactOffset;    // [n] first work item of the contact within its segment
     uint32_t* totals;           // [4]: work items per segment, then the clustering's count of
                                 // contacts left to its second pass (zeroed before the midphase)
-    uint32_t* pairContact;      // [3][n * cap] work item of a segment -> contact; once the
-                                // narrowphase has run, the clustering's second-pass list [n]
+    uint2* pairContact;         // [3][n * cap] work items of a segment: (contact, triangle); once
+                                // the narrowphase has run, the clustering's second-pass list
+                                // (uint32 [n])
     Manifold* temp;             // [n * cap] per-triangle manifolds, segments back to back
     Manifold* out;              // [n][3] clustered manifolds
     uint32_t* manifoldCounts;   // [n]

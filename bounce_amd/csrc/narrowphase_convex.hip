@@ -1,4 +1,4 @@
-// Sphere and capsule pairs for gfx950: one lane per pair.
+// Sphere and capsule pairs for gfx950: one lane per pair, a block's pairs grouped by collider.
 //
 //   sphere-sphere    collide_spheres.cpp:24-56           closed form
 //   sphere-capsule   collide_sphere_capsule.cpp:24-133    closed form
@@ -21,13 +21,40 @@
 namespace b3amd {
 namespace {
 
-// s_CollideMatrix (collide.cpp:172-177) for every valid pair that is not hull-hull.
-// With counting on, GJK calls and support iterations go to the b3_gjk* totals
-// (gjk.cpp:583,673,697).
+// The collider of a sphere/capsule pair (typeA <= typeB): 0 sphere-hull, 1 capsule-hull,
+// 2 sphere-sphere, 3 sphere-capsule, 4 capsule-capsule.
+__device__ __forceinline__ uint32_t convex_kind(uint32_t t1, uint32_t t2) {
+    if (t2 == 2u) return t1 == 0u ? 0u : 1u;
+    return t1 == 0u ? (t2 == 0u ? 2u : 3u) : 4u;
+}
+
+// s_CollideMatrix (collide.cpp:172-177) for every valid pair that is not hull-hull.  A block
+// takes 256 consecutive pairs and orders them by collider in LDS before colliding, so that a
+// wave runs one collider (two at a boundary) instead of all five; each pair writes its own
+// manifold, so the order inside the block changes nothing.  With counting on, GJK calls and
+// support iterations go to the b3_gjk* totals (gjk.cpp:583,673,697).
 __global__ __launch_bounds__(256) void convex_kernel(ConvexArgs a) {
-    const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ uint32_t order[256];
+    __shared__ uint32_t kindCount[8];
+    const uint32_t p0 = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t kind = 5u;
+    if (p0 < a.n && a.classify(p0) == 2u)
+        kind = convex_kind(a.shapes[a.pairs[2 * p0]].type, a.shapes[a.pairs[2 * p0 + 1]].type);
+    if (threadIdx.x < 8u) kindCount[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t rank = kind < 5u ? atomicAdd(&kindCount[kind], 1u) : 0u;
+    __syncthreads();
+    uint32_t base = 0, total = 0;
+#pragma unroll
+    for (uint32_t k = 0; k < 5u; ++k) {
+        if (k == kind) base = total;
+        total += kindCount[k];
+    }
+    if (kind < 5u) order[base + rank] = p0;
+    __syncthreads();
     uint32_t calls = 0, iters = 0;
-    if (p < a.n && a.classify(p) == 2u) {
+    if (threadIdx.x < total) {
+        const uint32_t p = order[threadIdx.x];
         Manifold* const out = a.out + a.slot_of(p);
         const ShapeRec s1 = a.shapes[a.pairs[2 * p]];
         const ShapeRec s2 = a.shapes[a.pairs[2 * p + 1]];

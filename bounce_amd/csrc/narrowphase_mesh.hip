@@ -5,9 +5,10 @@
 //                         order, child2 first (static_tree.h:100-142); triangle lists are
 //                         compacted into (contact, triangle) work items, grouped by the
 //                         convex shape's type so that each wave runs one collider.
-//   mesh_narrow_kernel    one lane per (contact, triangle): b3TriangleHull built in
-//                         registers and b3CollideShapeAndShape with an empty cache, the
-//                         triangle index stamped into every point key (:238-266).
+//   mesh_narrow_kernel    one lane per (contact, triangle) work item: b3TriangleHull built
+//                         in registers and b3CollideShapeAndShape (with the triangle's cache
+//                         for persistent contacts), the triangle index stamped into every
+//                         point key (:238-266).
 //   mesh_cluster_kernel   sixteen lanes per mesh contact: b3ClusterSolver::Run over the
 //                         triangle manifolds in midphase order (contact_cluster.cpp:202-590),
 //                         observations in LDS, every one of a contact's up to 256 points.
@@ -121,18 +122,20 @@ __device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, u
     return found;
 }
 
-// Appends contact c's m triangles as work items of its collider's segment and returns the
-// first item's offset in the segment (0 without items).  Every lane of the wave must call it
-// (one atomic per segment with a wave-uniform address, so each folds into a single
-// wave-level add; a per-lane segment address would serialise every add).
+// Appends contact c's m triangles as work items (contact, triangle) of its collider's
+// segment and returns the first item's offset in the segment (0 without items).  Every lane
+// of the wave must call it (one atomic per segment with a wave-uniform address, so each folds
+// into a single wave-level add; a per-lane segment address would serialise every add).  The
+// triangle indices come from the contact's list, which this lane has just written.
 __device__ __forceinline__ uint32_t list_work(const MeshArgs& a, uint32_t c, uint32_t seg, uint32_t m) {
     uint32_t off = 0;
     for (uint32_t g = 0; g < 3u; ++g) {
         const uint32_t o = atomicAdd(a.totals + g, seg == g ? m : 0u);
         if (seg == g) off = o;
     }
-    uint32_t* const items = a.pairContact + (size_t)(seg < 3u ? seg : 0u) * a.n * a.cap + off;
-    for (uint32_t k = 0; k < m; ++k) items[k] = c;
+    uint2* const items = a.pairContact + (size_t)(seg < 3u ? seg : 0u) * a.n * a.cap + off;
+    const uint32_t* const list = a.triLists + (size_t)c * a.cap;
+    for (uint32_t k = 0; k < m; ++k) items[k] = make_uint2(c, list[k]);
     return off;
 }
 
@@ -244,9 +247,8 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
     if (f < t0 + t1 + t2) {
         const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
         const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
-        const uint32_t c = a.pairContact[(size_t)seg * a.n * a.cap + local];
-        const uint32_t k = local - a.contactOffset[c];
-        const uint32_t ti = a.triLists[(size_t)c * a.cap + k];
+        const uint2 item = a.pairContact[(size_t)seg * a.n * a.cap + local];
+        const uint32_t c = item.x, ti = item.y;
         Manifold* const out = a.temp + f;
         const uint4 t = a.mesh.tris[ti];
         const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
@@ -262,6 +264,7 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
             if (a.triCaches) {
                 // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
                 // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.
+                const uint32_t k = local - a.contactOffset[c];
                 FeatureCache* const fc = &a.triCaches[(size_t)c * a.cap + k].feature;
                 Quat q[4];
                 for (int w = 0; w < 4; ++w) {
@@ -288,12 +291,12 @@ __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
         iters = m.gjkIters;
     }
     if (a.counters) {  // all lanes of the block reach here
-        Counters* const c = counter_slot(a.counters);
-        wave_count(&c->convexCalls, hullCalls);
-        wave_count(&c->convexCacheHits, hits);
-        wave_count(&c->gjkCalls, calls);
-        wave_count(&c->gjkIters, iters);
-        wave_max(&c->gjkMaxIters, iters);
+        Counters* const ct = counter_slot(a.counters);
+        wave_count(&ct->convexCalls, hullCalls);
+        wave_count(&ct->convexCacheHits, hits);
+        wave_count(&ct->gjkCalls, calls);
+        wave_count(&ct->gjkIters, iters);
+        wave_max(&ct->gjkMaxIters, iters);
     }
 }
 
@@ -803,7 +806,7 @@ __global__ __launch_bounds__(kClusterBlock) void mesh_cluster_kernel(MeshArgs a)
     const uint32_t ci = blockIdx.x * (kClusterBlock / kClusterGroup) + lane / kClusterGroup;
     if (ci >= a.n) return;  // whole groups leave; nothing below waits on other groups
     cluster_contact<kClusterLds>(a, ci, obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g, gbase,
-                                 a.pairContact, a.totals + 3);
+                                 reinterpret_cast<uint32_t*>(a.pairContact), a.totals + 3);
 }
 
 // The listed contacts with every observation (kMaxObs slots each): group q of the grid takes
@@ -817,7 +820,7 @@ __global__ __launch_bounds__(kClusterBlock) void mesh_cluster_wide_kernel(MeshAr
     const uint32_t groups = gridDim.x * (kClusterBlock / kClusterGroup);
     const uint32_t count = a.totals[3];
     for (uint32_t e = blockIdx.x * (kClusterBlock / kClusterGroup) + lane / kClusterGroup; e < count; e += groups) {
-        cluster_contact<kMaxObs>(a, a.pairContact[e], obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g,
+        cluster_contact<kMaxObs>(a, reinterpret_cast<const uint32_t*>(a.pairContact)[e], obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g,
                                  gbase, nullptr, nullptr);
         group_lds_sync();
     }

@@ -116,6 +116,65 @@ def place_along(lib, pool, ha, ra, hb, rb, dirs, gaps):
     return t
 
 
+def gen_convex_pairs(lib, seed, n, hull_pool=2048, gap_mean=-0.02, gap_sigma=0.08):
+    """n sphere/capsule pairs for the convex narrowphase, in equal shares of the five kinds
+    it takes: sphere-sphere, sphere-capsule, sphere-hull, capsule-capsule, capsule-hull (type
+    A <= type B, collide.cpp:172-177).  Spheres r ~ U[0.1, 0.5], offset from the body origin
+    by U[-0.2, 0.2] per axis in half of them; capsules r ~ U[0.1, 0.3] with segment length
+    U[0, 1] (a tenth of them zero length); hulls of 8-32 points, semi-axes U[0.2, 1] m.
+    Body A at a random point of a 10 m cube, B along a random direction u at the distance
+    where their supports along u touch, plus N(gap_mean, gap_sigma); 5 % nearly concentric.
+    Returns (pool, shapes [2n], pairs [n][2] (shape 2i, 2i+1), xfs [n][2][12])."""
+    from .types import CAPSULE, HULL, SHAPE_DTYPE, SPHERE
+    rng = np.random.default_rng(seed)
+    pool = gen_hull_pool(lib, seed, hull_pool, 8, 32, 0.2, 1.0)
+    kinds = np.array([(SPHERE, SPHERE), (SPHERE, CAPSULE), (SPHERE, HULL), (CAPSULE, CAPSULE), (CAPSULE, HULL)])
+    k = kinds[rng.integers(0, len(kinds), n)]
+    shapes = np.zeros(2 * n, SHAPE_DTYPE)
+    t = k.reshape(-1)
+    shapes["type"] = t
+    sph, cap, hul = t == SPHERE, t == CAPSULE, t == HULL
+    shapes["radius"][sph] = rng.uniform(0.1, 0.5, sph.sum())
+    off = rng.uniform(-0.2, 0.2, (2 * n, 3)) * (rng.random(2 * n) < 0.5)[:, None]
+    shapes["v"][sph, :3] = off[sph]
+    L = rng.uniform(0.0, 1.0, 2 * n) * (rng.random(2 * n) < 0.9)
+    shapes["radius"][cap] = rng.uniform(0.1, 0.3, cap.sum())
+    shapes["v"][cap, 1] = -L[cap] / 2
+    shapes["v"][cap, 4] = L[cap] / 2
+    shapes["hull"][hul] = rng.integers(0, pool.count, hul.sum())
+    R = _uniform_rotations(rng, 2 * n)
+    u = rng.normal(size=(n, 3))
+    u /= np.linalg.norm(u, axis=1, keepdims=True)
+    # Support of each shape along +u (A) or -u (B) in world orientation.
+    dirs = np.repeat(u, 2, axis=0) * np.tile([1.0, -1.0], n)[:, None]
+    loc = np.einsum("nji,nj->ni", R, dirs)  # direction in the shape's frame: R^T d
+    sup = np.zeros(2 * n)
+    sup[sph] = (off[sph] * loc[sph]).sum(1) + shapes["radius"][sph]
+    seg = np.abs(L[cap] / 2 * loc[cap, 1])
+    sup[cap] = seg + shapes["radius"][cap]
+    Vmax = int(pool.desc[:, 1].max())
+    vp = np.zeros((pool.count, Vmax, 3))
+    for h in range(pool.count):
+        v = pool.hull(h)[0].astype(np.float64)
+        vp[h, :len(v)] = v
+        vp[h, len(v):] = v[0]
+    hi = np.where(hul)[0]
+    for a in range(0, len(hi), 1 << 16):
+        b = hi[a:a + (1 << 16)]
+        sup[b] = np.einsum("nvk,nk->nv", vp[shapes["hull"][b]], loc[b]).max(1)
+    gap = sup[0::2] + sup[1::2] + rng.normal(gap_mean, gap_sigma, n)
+    deep = rng.random(n) < 0.05
+    gap[deep] = rng.uniform(0.0, 0.05, deep.sum())
+    pa = rng.uniform(-5.0, 5.0, (n, 3))
+    xfs = np.zeros((n, 2, 12), np.float32)
+    for j, p in ((0, pa), (1, pa + gap[:, None] * u)):
+        Rj = R[j::2]
+        xfs[:, j, 0:3], xfs[:, j, 3:6], xfs[:, j, 6:9] = Rj[:, :, 0], Rj[:, :, 1], Rj[:, :, 2]
+        xfs[:, j, 9:12] = p
+    pairs = np.arange(2 * n, dtype=np.uint32).reshape(n, 2)
+    return pool, shapes, pairs, xfs
+
+
 def _rotations(rng, n, tilt_sigma=0.05):
     """n rotation matrices: uniform yaw about y with small tilts about x and z."""
     yaw = rng.uniform(0.0, 2.0 * np.pi, n)
