@@ -414,22 +414,45 @@ __device__ inline void query_face_seq(const Xf& xf1, const H1& h1, const Xf& xf2
     Xf xf = mulT(xf2, xf1);
     index = 0;
     separation = -kMaxFloat;
-    for (uint32_t i = 0; i < h1.F; ++i) {
-        Plane pl = mul(xf, h1.plane(i));
-        V3 d = -pl.n;
-        uint32_t mi = 0;
-        float mp = dot(d, h2.vertex(0));
-        for (uint32_t k = 1; k < h2.V; ++k) {
-            float q = dot(d, h2.vertex(k));
-            if (q > mp) {
-                mp = q;
-                mi = k;
+    // Records are read four at a time (the loads in flight together) and used in index order.
+    for (uint32_t i0 = 0; i0 < h1.F; i0 += 4u) {
+        Plane raw[4];
+#pragma unroll
+        for (uint32_t t = 0; t < 4u; ++t) raw[t] = h1.plane(i0 + t < h1.F ? i0 + t : i0);
+#pragma unroll
+        for (uint32_t t = 0; t < 4u; ++t) {
+            const uint32_t i = i0 + t;
+            if (i >= h1.F) break;
+            Plane pl = mul(xf, raw[t]);
+            V3 d = -pl.n;
+            uint32_t mi = 0;
+            float mp = dot(d, h2.vertex(0));
+            uint32_t k = 1;
+            for (; k + 4u <= h2.V; k += 4u) {
+                V3 w[4];
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) w[u] = h2.vertex(k + u);
+#pragma unroll
+                for (uint32_t u = 0; u < 4u; ++u) {
+                    float q = dot(d, w[u]);
+                    if (q > mp) {
+                        mp = q;
+                        mi = k + u;
+                    }
+                }
             }
-        }
-        float s = distance(h2.vertex(mi), pl);
-        if (s > separation) {
-            index = i;
-            separation = s;
+            for (; k < h2.V; ++k) {
+                float q = dot(d, h2.vertex(k));
+                if (q > mp) {
+                    mp = q;
+                    mi = k;
+                }
+            }
+            float s = distance(h2.vertex(mi), pl);
+            if (s > separation) {
+                index = i;
+                separation = s;
+            }
         }
     }
 }
@@ -443,12 +466,22 @@ __device__ inline void query_edge_seq(const Xf& xf1, const H1& h1, const Xf& xf2
     index1 = 0;
     index2 = 0;
     separation = -kMaxFloat;
+    // Hull 1's edge records are read two edges at a time (their loads in flight together).
+    V3 nextP = h1.vertex(h1.origin(0)), nextQ = h1.vertex(h1.origin(1));
+    V3 nextU = h1.plane(h1.face(0)).n, nextV = h1.plane(h1.face(1)).n;
     for (uint32_t i = 0; i < h1.HE; i += 2) {
-        V3 P1 = mul(xf, h1.vertex(h1.origin(i)));
-        V3 Q1 = mul(xf, h1.vertex(h1.origin(i + 1)));
+        const V3 P1raw = nextP, Q1raw = nextQ, U1raw = nextU, V1raw = nextV;
+        if (i + 2 < h1.HE) {
+            nextP = h1.vertex(h1.origin(i + 2));
+            nextQ = h1.vertex(h1.origin(i + 3));
+            nextU = h1.plane(h1.face(i + 2)).n;
+            nextV = h1.plane(h1.face(i + 3)).n;
+        }
+        V3 P1 = mul(xf, P1raw);
+        V3 Q1 = mul(xf, Q1raw);
         V3 E1 = Q1 - P1;
-        V3 U1 = mul(xf.R, h1.plane(h1.face(i)).n);
-        V3 V1 = mul(xf.R, h1.plane(h1.face(i + 1)).n);
+        V3 U1 = mul(xf.R, U1raw);
+        V3 V1 = mul(xf.R, V1raw);
         V3 nE1 = -E1;
         float L1 = length(E1);
         V3 D1 = P1 - C1;
