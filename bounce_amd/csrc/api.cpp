@@ -454,6 +454,8 @@ int b3amd::enqueue_batch(b3amd_context* ctx, uint32_t n, const uint32_t* pairs, 
     cta.listCount = sc.counter + kCountContacts;
     cta.satList = ca.satList;
     cta.satCount = sc.counter + kCountSat;
+    cta.wideList = ca.satList;  // free once contact_split_kernel has walked the SAT list
+    cta.wideCount = sc.counter + kCountWide;
     const BatchStreams ss = streams ? *streams : BatchStreams{ctx->stream, ctx->side, ctx->forkEvent, ctx->joinEvent};
     hipStream_t s = ss.main;
     if (ctx->roundShapes) {
@@ -1003,14 +1005,15 @@ struct MeshScratch {
     Manifold* temp;
     float* sweeps;   // staged per-contact sweeps (persistent contacts)
     float* centers;  // staged body-A centres (persistent contacts)
+    uint32_t* wideItems;
 };
 
 int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
     const uint64_t cap = kMeshTriangleCap;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    const uint64_t sizes[13] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
+    const uint64_t sizes[14] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
                                 4ull * n,  256,       24ull * n * cap, 720ull * n, 4ull * n,
-                                sizeof(Manifold) * n * cap, 64ull * n, 12ull * n};
+                                sizeof(Manifold) * n * cap, 64ull * n, 12ull * n, 4ull * n * cap};
     uint64_t total = 0;
     for (uint64_t b : sizes) total += al(b);
     if (total > ctx->meshScratchCap) {
@@ -1022,15 +1025,15 @@ int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
         ctx->meshScratchCap = total;
     }
     char* p = static_cast<char*>(ctx->dMeshScratch);
-    void* parts[13];
-    for (int i = 0; i < 13; ++i) {
+    void* parts[14];
+    for (int i = 0; i < 14; ++i) {
         parts[i] = p;
         p += al(sizes[i]);
     }
     *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2],  (uint32_t*)parts[3],
                      (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint2*)parts[7],
                      (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10], (float*)parts[11],
-                     (float*)parts[12]};
+                     (float*)parts[12], (uint32_t*)parts[13]};
     return B3AMD_OK;
 }
 
@@ -1050,6 +1053,7 @@ int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t
     a.contactOffset = sc.offsets;
     a.totals = sc.totals;
     a.pairContact = sc.pairContact;
+    a.wideItems = sc.wideItems;
     a.out = out;
     a.manifoldCounts = manifoldCounts;
     a.temp = sc.temp;
@@ -1067,7 +1071,7 @@ int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t
     a.cap = kMeshTriangleCap;
     hipStream_t s = ctx->stream;
     int rc;
-    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 4, s); }))) return rc;
+    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 8, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
     return timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
@@ -1165,6 +1169,7 @@ MeshArgs mesh_set_args(b3amd_mesh_contacts* set, const MeshScratch& sc) {
     a.contactOffset = sc.offsets;
     a.totals = sc.totals;
     a.pairContact = sc.pairContact;
+    a.wideItems = sc.wideItems;
     a.temp = sc.temp;
     a.out = sc.out;
     a.manifoldCounts = sc.manifoldCounts;
@@ -1241,7 +1246,7 @@ int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* set, const b3amd_transform*
     if (sweeps) HIP_TRY(hipMemcpyAsync(sc.sweeps, sweeps, 64ull * n, hipMemcpyHostToDevice, s));
     MeshArgs a = mesh_set_args(set, sc);
     a.sweeps = sweeps ? sc.sweeps : nullptr;
-    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 4, s); }))) return rc;
+    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 8, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_list", [&] { return launch_mesh_list(a, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); }))) return rc;

@@ -31,30 +31,72 @@ __device__ inline float project_edges(V3 P1, V3 E1, float L1, V3 D1, V3 P2, V3 E
 
 // ------------------------------------------------------------------ contact building
 
-constexpr int kPolyCap = 32;  // b3StackArray<b3ClipVertex, 32> inline capacity (clip.cpp)
+// Clip polygons.  The lane paths hold b3StackArray's inline capacity (clip.cpp); a polygon
+// that outgrows it sets `overflow`, and the pair goes to the wide pass, whose polygons live
+// in LDS with room for kWidePolyCap vertices: a convex polygon clipped by a convex face
+// gains at most one vertex per plane, so two faces of at most 255 vertices (the hull limit)
+// stay within 510.
+constexpr uint32_t kPolyCap = 32;
+constexpr uint32_t kWidePolyCap = 512;
 
-struct Poly {
+template <uint32_t Cap>
+struct PolyArr {  // in the lane's own storage
+    static constexpr uint32_t cap = Cap;
     uint32_t count;
     bool overflow;
-    ClipV v[kPolyCap];
+    ClipV v[Cap];
     __device__ void push(const ClipV& x) {
-        if (count < kPolyCap) v[count++] = x;
+        if (count < Cap) v[count++] = x;
         else overflow = true;
     }
 };
+struct PolyView {  // over caller-provided storage (the wide pass's LDS)
+    uint32_t count;
+    bool overflow;
+    uint32_t cap;
+    ClipV* v;
+    __device__ void push(const ClipV& x) {
+        if (count < cap) v[count++] = x;
+        else overflow = true;
+    }
+};
+using Poly = PolyArr<kPolyCap>;
 
 struct CPV {
     V3 position;
     uint32_t clipIndex;
 };
-struct CPoly {
+template <uint32_t Cap>
+struct CPolyArr {
     uint32_t count;
-    CPV v[kPolyCap];
+    CPV v[Cap];
 };
+struct CPolyView {
+    uint32_t count;
+    CPV* v;
+};
+using CPoly = CPolyArr<kPolyCap>;
+
+// The workspace of b3BuildFaceContact: the clip polygon, its scratch and the points kept
+// within the radius.  FaceWs is the lane's own; WideFaceWs points into LDS.
+struct FaceWs {
+    Poly poly, scratch;
+    CPoly kept;
+};
+struct WideFaceWs {
+    PolyView poly, scratch;
+    CPolyView kept;
+};
+// The lane's own workspace, or the wide pass's.
+template <bool Wide>
+__device__ __forceinline__ auto select_ws(FaceWs* lane, WideFaceWs* wide) {
+    if constexpr (Wide) return wide;
+    else return lane;
+}
 
 // clip.cpp:33-57
-template <class H>
-__device__ inline void build_polygon(Poly& out, const Xf& xf, uint32_t index, const H& h) {
+template <class H, class P>
+__device__ inline void build_polygon(P& out, const Xf& xf, uint32_t index, const H& h) {
     out.count = 0;
     uint32_t begin = h.face_edge(index);
     uint32_t e = begin;
@@ -69,7 +111,8 @@ __device__ inline void build_polygon(Poly& out, const Xf& xf, uint32_t index, co
 }
 
 // clip.cpp:99-149
-__device__ inline void clip_to_plane(Poly& out, const Poly& in, const Plane& pl, uint32_t id) {
+template <class P>
+__device__ inline void clip_to_plane(P& out, const P& in, const Plane& pl, uint32_t id) {
     out.count = 0;
     ClipV v1 = in.v[in.count - 1];
     float d1 = distance(v1.position, pl);
@@ -95,13 +138,13 @@ __device__ inline void clip_to_plane(Poly& out, const Poly& in, const Plane& pl,
 
 // clip.cpp:243-280; returns the clipped polygon in `a` (b is scratch).  The two buffers
 // alternate between clip planes; the result is copied into `a` once at the end.
-template <class H>
-__device__ inline void clip_to_face(Poly& a, Poly& b, const Xf& xf, float r, uint32_t index, const H& h) {
+template <class H, class P>
+__device__ inline void clip_to_face(P& a, P& b, const Xf& xf, float r, uint32_t index, const H& h) {
     uint32_t begin = h.face_edge(index);
     uint32_t e = begin;
     uint32_t guard = 0;
-    Poly* src = &a;
-    Poly* dst = &b;
+    P* src = &a;
+    P* dst = &b;
     bool overflow = a.overflow;
     do {
         uint32_t edgeId = e;  // (edge->twin)->twin
@@ -111,7 +154,7 @@ __device__ inline void clip_to_face(Poly& a, Poly& b, const Xf& xf, float r, uin
         dst->overflow = false;
         clip_to_plane(*dst, *src, wp, edgeId);
         overflow = overflow || dst->overflow;
-        Poly* t = src;
+        P* t = src;
         src = dst;
         dst = t;
         if (src->count == 0) break;
@@ -131,7 +174,8 @@ __device__ __forceinline__ bool is_ccw(V3 A, V3 B, V3 C, V3 N) {
 }
 
 // contact_cluster.cpp:33-56 (the pivot B is read once per i, before any swap)
-__device__ inline void sort_polygon(CPoly& p, V3 N) {
+template <class CP>
+__device__ inline void sort_polygon(CP& p, V3 N) {
     V3 A = p.v[0].position;
     for (uint32_t i = 1; i < p.count; ++i) {
         V3 B = p.v[i].position;
@@ -146,8 +190,10 @@ __device__ inline void sort_polygon(CPoly& p, V3 N) {
     }
 }
 
-// contact_cluster.cpp:58-200: reduces `in` into `out`.
-__device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_t initial) {
+// contact_cluster.cpp:58-200: reduces `in` into `out` (at most four points).  The chosen
+// points (at most three are chosen before the last pick) are skipped by index.
+template <class CIn>
+__device__ inline void reduce_polygon(CPolyArr<4>& out, const CIn& in, V3 N, uint32_t initial) {
     out.count = 0;
     if (in.count <= 4) {
         out.count = in.count;
@@ -155,15 +201,15 @@ __device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_
         sort_polygon(out, N);
         return;
     }
-    uint32_t chosen = 0;  // bit mask over in.v (count <= 32)
+    const uint32_t none = 0xFFFFFFFFu;
+    uint32_t c1 = none, c2 = none;
     out.v[out.count++] = in.v[initial];
-    chosen |= 1u << initial;
     {
         V3 A = out.v[0].position;
         uint32_t index = 0;
         float mx = -kMaxFloat;
         for (uint32_t i = 0; i < in.count; ++i) {
-            if (chosen & (1u << i)) continue;
+            if (i == initial) continue;
             V3 d = in.v[i].position - A;
             float dd = dot(d, d);
             if (dd > mx) {
@@ -173,14 +219,14 @@ __device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_
         }
         if (mx < kEpsilon * kEpsilon) return;
         out.v[out.count++] = in.v[index];
-        chosen |= 1u << index;
+        c1 = index;
     }
     {
         V3 A = out.v[0].position, B = out.v[1].position;
         uint32_t index = 0;
         float mx = -kMaxFloat;
         for (uint32_t i = 0; i < in.count; ++i) {
-            if (chosen & (1u << i)) continue;
+            if (i == initial || i == c1) continue;
             V3 C = in.v[i].position;
             V3 n = cross(B - A, C - A);
             float a2 = fabsf(dot(n, N));
@@ -198,14 +244,14 @@ __device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_
             out.v[1] = t;
         }
         out.v[out.count++] = in.v[index];
-        chosen |= 1u << index;
+        c2 = index;
     }
     {
         V3 A = out.v[0].position, B = out.v[1].position;
         uint32_t index = 0;
         float mn = kMaxFloat;
         for (uint32_t i = 0; i < in.count; ++i) {
-            if (chosen & (1u << i)) continue;
+            if (i == initial || i == c1 || i == c2) continue;
             V3 D = in.v[i].position;
             V3 n = cross(B - A, D - A);
             float sa2 = dot(n, N);
@@ -217,7 +263,6 @@ __device__ inline void reduce_polygon(CPoly& out, const CPoly& in, V3 N, uint32_
         const float kAreaTol = 0.01f;
         if (fabsf(mn) < 2.0f * kAreaTol) return;
         out.v[out.count++] = in.v[index];
-        chosen |= 1u << index;
     }
     sort_polygon(out, N);
 }
@@ -268,11 +313,12 @@ __device__ inline void build_edge_contact(Contact& m, const Xf& xf1, uint32_t in
     m.key[0][3] = index2 + 1;  // outEdge2
 }
 
-// collide_hulls.cpp:93-204
-template <class H1, class H2>
+// collide_hulls.cpp:93-204, in the workspace ws (FaceWs or WideFaceWs).
+template <class H1, class H2, class WS>
 __device__ inline void build_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1, float r1,
-                                   const Xf& xf2, const H2& h2, float r2, bool flip) {
-    Poly poly, scratch;
+                                   const Xf& xf2, const H2& h2, float r2, bool flip, WS& ws) {
+    auto& poly = ws.poly;
+    auto& scratch = ws.scratch;
     poly.overflow = false;
     scratch.overflow = false;
     float totalRadius = r1 + r2;
@@ -293,7 +339,8 @@ __device__ inline void build_face_contact(Contact& m, const Xf& xf1, uint32_t in
     clip_to_face(poly, scratch, xf1, totalRadius, index1, h1);
     m.overflow = m.overflow || poly.overflow;
     if (poly.count == 0) return;
-    CPoly p1, red;
+    auto& p1 = ws.kept;
+    CPolyArr<4> red;
     p1.count = 0;
     uint32_t minIndex = 0;
     float minSep = kMaxFloat;
@@ -350,11 +397,11 @@ __device__ inline bool rebuild_face_allowed(Quat q10, Quat q1, Quat q20, Quat q2
 }
 
 // collide_hulls_cache.cpp:113-153
-template <class H1, class H2>
+template <class H1, class H2, class WS>
 __device__ inline void rebuild_face_contact(Contact& m, const Xf& xf1, uint32_t index1, const H1& h1, float r1,
                                      const Xf& xf2, const H2& h2, float r2, bool flip,
-                                     Quat q10, Quat q1, Quat q20, Quat q2) {
-    if (rebuild_face_allowed(q10, q1, q20, q2)) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip);
+                                     Quat q10, Quat q1, Quat q20, Quat q2, WS& ws) {
+    if (rebuild_face_allowed(q10, q1, q20, q2)) build_face_contact(m, xf1, index1, h1, r1, xf2, h2, r2, flip, ws);
 }
 
 // b3FeatureCache::ReadFace (sat.cpp:209-222)
@@ -514,10 +561,12 @@ __device__ inline void set_feature_cache(FeatureCache* c, uint32_t state, uint32
     c->index2 = i2;
 }
 
-// b3CollideCache (collide_hulls_cache.cpp:155-275) for one lane.
-template <class H1, class H2>
+// b3CollideCache (collide_hulls_cache.cpp:155-275) for one lane, face contacts built in ws.
+// A clip polygon that outgrows ws stops it at once (m.overflow set, the cache untouched
+// since), for the wide pass to run the pair again.
+template <class H1, class H2, class WS>
 __device__ inline void collide_cache_seq(Contact& m, const Xf& xf1, const H1& h1, float r1, const Xf& xf2,
-                                         const H2& h2, float r2, FeatureCache* fc) {
+                                         const H2& h2, float r2, FeatureCache* fc, WS& ws) {
     const float totalRadius = r1 + r2;
     uint32_t f1i, f2i, ei, ej;
     float f1s, f2s, es;
@@ -545,13 +594,15 @@ __device__ inline void collide_cache_seq(Contact& m, const Xf& xf1, const H1& h1
             return;
         }
     } else if (f1s + kTol > f2s) {
-        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false);
+        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false, ws);
+        if (m.overflow) return;
         if (m.pointCount > 0) {
             set_feature_cache(fc, kCacheOverlap, kFeatureFace1, f1i, f1i);
             return;
         }
     } else {
-        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true);
+        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true, ws);
+        if (m.overflow) return;
         if (m.pointCount > 0) {
             set_feature_cache(fc, kCacheOverlap, kFeatureFace2, f2i, f2i);
             return;
@@ -564,13 +615,15 @@ __device__ inline void collide_cache_seq(Contact& m, const Xf& xf1, const H1& h1
             return;
         }
     } else if (f1s > f2s) {
-        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false);
+        build_face_contact(m, xf1, f1i, h1, r1, xf2, h2, r2, false, ws);
+        if (m.overflow) return;
         if (m.pointCount > 0) {
             set_feature_cache(fc, kCacheOverlap, kFeatureFace1, f1i, f1i);
             return;
         }
     } else {
-        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true);
+        build_face_contact(m, xf2, f2i, h2, r2, xf1, h1, r1, true, ws);
+        if (m.overflow) return;
         if (m.pointCount > 0) {
             set_feature_cache(fc, kCacheOverlap, kFeatureFace2, f2i, f2i);
             return;
@@ -585,10 +638,12 @@ __device__ inline void collide_cache_seq(Contact& m, const Xf& xf1, const H1& h1
 // or an overlap whose contact can be rebuilt (with the bodies' sweeps for the face rebuild),
 // is a hit (b3_convexCacheHits); otherwise the cache is flushed and b3CollideCache runs.
 // A cached feature outside the hulls cannot come from this pair and is treated as a miss.
-template <class H1, class H2>
+// Face contacts are built in ws; a polygon that outgrows it stops the pair at once
+// (m.overflow set, the cache as it was at that point) for the wide pass.
+template <class H1, class H2, class WS>
 __device__ inline void collide_hulls_cached_seq(Contact& m, const Xf& xf1, const H1& h1, float r1, const Xf& xf2,
                                                 const H2& h2, float r2, FeatureCache* fc, Quat q10, Quat q1,
-                                                Quat q20, Quat q2, uint32_t& hit) {
+                                                Quat q20, Quat q2, uint32_t& hit, WS& ws) {
     hit = 0;
     const float totalRadius = r1 + r2;
     const uint32_t state0 = fc->state, type = fc->type, i1 = fc->index1, i2 = fc->index2;
@@ -608,9 +663,10 @@ __device__ inline void collide_hulls_cached_seq(Contact& m, const Xf& xf1, const
         if (state0 == kCacheOverlap && state1 == kCacheOverlap) {
             switch (type) {
                 case kFeatureEdge1: build_edge_contact(m, xf1, i1, h1, xf2, i2, h2, true); break;
-                case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2); break;
-                default: rebuild_face_contact(m, xf2, i1, h2, r2, xf1, h1, r1, true, q20, q2, q10, q1); break;
+                case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2, ws); break;
+                default: rebuild_face_contact(m, xf2, i1, h2, r2, xf1, h1, r1, true, q20, q2, q10, q1, ws); break;
             }
+            if (m.overflow) return;
             if (m.pointCount > 0) {
                 hit = 1;
                 return;
@@ -618,7 +674,7 @@ __device__ inline void collide_hulls_cached_seq(Contact& m, const Xf& xf1, const
         }
     }
     fc->state = kCacheEmpty;
-    collide_cache_seq(m, xf1, h1, r1, xf2, h2, r2, fc);
+    collide_cache_seq(m, xf1, h1, r1, xf2, h2, r2, fc, ws);
 }
 
 }  // namespace b3amd

@@ -181,8 +181,10 @@ __host__ __device__ inline uint32_t rebuild_capacity(uint32_t n) {
     const uint32_t blocks = (n + kReadPairsPerBlock - 1u) / kReadPairsPerBlock;
     return (blocks + kRebuildLists - 1u) / kRebuildLists * kReadPairsPerBlock;
 }
+// The wide passes' list lengths: pairs whose clip polygon outgrew the lane contact kernel's
+// 32 vertices (kCountWide) or the lane cache-rebuild kernel's (kCountCacheWide).
 enum : uint32_t {
-    kCountSatTail = 0, kCountContacts = 1, kCountSat = 2, kCountRebuild = 64,
+    kCountSatTail = 0, kCountContacts = 1, kCountSat = 2, kCountWide = 3, kCountCacheWide = 4, kCountRebuild = 64,
     kCountWords = kCountRebuild + kRebuildLists
 };
 // Scratch words for a capacity of n pairs: stage, SAT list, rebuild lists.
@@ -224,6 +226,9 @@ struct CacheArgs : PairInputs {
 // rebuilt clip polygon outgrows the group kernel (the lane-per-pair cache kernel takes it).
 constexpr uint32_t kStageCached = 7u;
 constexpr uint32_t kStageCacheFallback = 8u;
+// A lane-form result whose clip polygon outgrew the lane's 32 vertices: nothing was written
+// and the pair goes to a wide pass.
+constexpr uint32_t kStageWide = 9u;
 
 // What the cache and SAT kernels need of a hull pair, written into the pair's SatResult by
 // the listing kernel before any query: the hull record units (f1Index, f2Index) and the
@@ -263,6 +268,8 @@ struct ContactArgs : PairInputs {
     uint32_t* listCount;     // zeroed by the cache kernel
     const uint32_t* satList;   // the pairs the SAT kernel ran, *satCount of them
     const uint32_t* satCount;
+    uint32_t* wideList;        // [n] pairs for the wide contact pass (the SAT list's storage)
+    uint32_t* wideCount;       // zeroed with the counter words
 };
 
 // A pair whose clip polygon outgrows the group kernel's capacity: the lane-per-pair
@@ -315,8 +322,11 @@ struct MeshArgs {
     // Work items are grouped by the convex shape's collider so that a wave runs one of
     // them: segment 0 hulls, 1 capsules, 2 spheres, in that order in `temp`.
     uint32_t* contactOffset;    // [n] first work item of the contact within its segment
-    uint32_t* totals;           // [4]: work items per segment, then the clustering's count of
-                                // contacts left to its second pass (zeroed before the midphase)
+    uint32_t* totals;           // [8]: work items per segment, the clustering's count of
+                                // contacts left to its second pass, the wide items' count
+                                // (zeroed before the midphase)
+    uint32_t* wideItems;        // [n * cap] work items for the wide pass (clip polygons beyond
+                                // the lane's 32 vertices)
     uint2* pairContact;         // [3][n * cap] work items of a segment: (contact, triangle); once
                                 // the narrowphase has run, the clustering's second-pass list
                                 // (uint32 [n])

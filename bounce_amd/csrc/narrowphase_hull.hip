@@ -12,6 +12,8 @@
 //                        manifold and feature-cache write.
 //   contact_hull_kernel  one lane per listed pair, the same for the pairs whose clip
 //                        polygon outgrows the group kernel (kStageFallback).
+//   contact_wide_kernel  the pairs whose polygon outgrows the lane's 32 vertices, with the
+//                        polygons in LDS (the reference's b3StackArray grows on the heap).
 // All arithmetic keeps the reference's FP32 operation order (b3_math.h); the build uses
 // -ffp-contract=off and IEEE division/square root.
 #include <hip/hip_runtime.h>
@@ -32,86 +34,113 @@ __device__ __forceinline__ void set_feature(FeatureCache* c, uint32_t state, uin
     c->index2 = i2;
 }
 
-// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275), one lane per pair, for
-// the pairs the group kernel marked kStageFallback (polygons beyond its capacity).
-__global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
-    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= *a.listCount) return;
-    const uint32_t p = a.list[k];
-    SatResult r = a.sat[p];
-    if (r.stage != kStageFallback) return;  // written by the group kernel
-    r.stage = kStageOverlap;
-    const uint32_t slot = a.slot_of(p);
-    Manifold* const out = a.out + slot;
-    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
-    Contact m;
+// b3CollideCache after the queries (collide_hulls_cache.cpp:155-275) for pair p with the
+// query results r, face contacts built in ws.  Leaves the manifold in m and the feature the
+// cache records in f (state, type, index1, index2); returns false, with nothing decided, when
+// a clip polygon outgrew ws.
+struct FeatureOut {
+    uint32_t state, type, i1, i2;
+};
+
+template <class WS>
+__device__ bool contact_hull_pair(const ContactArgs& a, uint32_t p, const SatResult& r, WS& ws, Contact& m,
+                                  FeatureOut& f) {
     m.pointCount = 0;
     m.overflow = false;
-    if (r.stage != kStageOverlap) {
-        if (r.stage == kStageSepFace1) set_feature(fc, kCacheSeparation, kFeatureFace1, r.f1Index, r.f1Index);
-        else if (r.stage == kStageSepFace2) set_feature(fc, kCacheSeparation, kFeatureFace2, r.f2Index, r.f2Index);
-        else set_feature(fc, kCacheSeparation, kFeatureEdge1, r.eIndex1, r.eIndex2);
-        write_manifold(out, m);
-        return;
-    }
     const uint32_t sa = a.pairs[2 * p], sb = a.pairs[2 * p + 1];
     const float r1 = a.shapes[sa].radius, r2 = a.shapes[sb].radius;
     const Xf xf1 = load_xf(a.xf_ptr(p, 0));
     const Xf xf2 = load_xf(a.xf_ptr(p, 1));
     const DevHull h1 = DevHull::at_unit(a.pool, a.shapes[sa].hullUnit);
     const DevHull h2 = DevHull::at_unit(a.pool, a.shapes[sb].hullUnit);
-
     const float kTol = 0.1f * kLinearSlop;
     const float fmax12 = r.f1Sep > r.f2Sep ? r.f1Sep : r.f2Sep;  // b3Max
+    f = FeatureOut{kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2};
     if (r.eSep > fmax12 + kTol) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
-        if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(out, m);
-            return;
-        }
+        if (m.pointCount > 0) return true;
     } else if (r.f1Sep + kTol > r.f2Sep) {
-        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
+        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false, ws);
+        if (m.overflow) return false;
         if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(out, m);
-            return;
+            f = FeatureOut{kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index};
+            return true;
         }
     } else {
-        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
+        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true, ws);
+        if (m.overflow) return false;
         if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(out, m);
-            return;
+            f = FeatureOut{kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index};
+            return true;
         }
     }
     // Heuristic failed: fallback without the tolerance.
     if (r.eSep > fmax12) {
         build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
-        if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-            write_manifold(out, m);
-            return;
-        }
+        if (m.pointCount > 0) return true;
     } else if (r.f1Sep > r.f2Sep) {
-        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false);
+        build_face_contact(m, xf1, r.f1Index, h1, r1, xf2, h2, r2, false, ws);
+        if (m.overflow) return false;
         if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index);
-            write_manifold(out, m);
-            return;
+            f = FeatureOut{kCacheOverlap, kFeatureFace1, r.f1Index, r.f1Index};
+            return true;
         }
     } else {
-        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true);
+        build_face_contact(m, xf2, r.f2Index, h2, r2, xf1, h1, r1, true, ws);
+        if (m.overflow) return false;
         if (m.pointCount > 0) {
-            set_feature(fc, kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index);
-            write_manifold(out, m);
-            return;
+            f = FeatureOut{kCacheOverlap, kFeatureFace2, r.f2Index, r.f2Index};
+            return true;
         }
     }
     // Last resort: a contact between the edge lines.
     build_edge_contact(m, xf1, r.eIndex1, h1, xf2, r.eIndex2, h2, false);
-    set_feature(fc, kCacheOverlap, kFeatureEdge1, r.eIndex1, r.eIndex2);
-    write_manifold(out, m);
+    return true;
+}
+
+// One lane per pair for the pairs the group kernel marked kStageFallback (polygons beyond its
+// capacity), with the reference's 32-vertex inline polygon; a pair whose polygon outgrows
+// that is listed for contact_wide_kernel (in the SAT list's storage, which contact_split_kernel
+// has finished reading) and writes nothing here.
+__global__ __launch_bounds__(256) void contact_hull_kernel(ContactArgs a) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= *a.listCount) return;
+    const uint32_t p = a.list[k];
+    const SatResult r = a.sat[p];
+    if (r.stage != kStageFallback) return;  // written by the group kernel
+    const uint32_t slot = a.slot_of(p);
+    FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+    FaceWs ws;
+    Contact m;
+    FeatureOut f;
+    if (!contact_hull_pair(a, p, r, ws, m, f)) {
+        a.wideList[atomicAdd(a.wideCount, 1u)] = p;
+        return;
+    }
+    set_feature(fc, f.state, f.type, f.i1, f.i2);
+    write_manifold(a.out + slot, m);
+}
+
+// The pairs contact_hull_kernel listed, one per block, its first lane building the contact
+// with the polygons in LDS (room for kWidePolyCap vertices).
+constexpr uint32_t kWideBlocks = 64;
+__global__ __launch_bounds__(64) void contact_wide_kernel(ContactArgs a) {
+    __shared__ ClipV polyBuf[kWidePolyCap], scratchBuf[kWidePolyCap];
+    __shared__ CPV keptBuf[kWidePolyCap];
+    if (threadIdx.x != 0u) return;
+    const uint32_t count = *a.wideCount;
+    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+        const uint32_t p = a.wideList[e];
+        const uint32_t slot = a.slot_of(p);
+        FeatureCache* const fc = a.caches ? &a.caches[slot].feature : nullptr;
+        WideFaceWs ws{PolyView{0u, false, kWidePolyCap, polyBuf}, PolyView{0u, false, kWidePolyCap, scratchBuf},
+                      CPolyView{0u, keptBuf}};
+        Contact m;
+        FeatureOut f;
+        contact_hull_pair(a, p, a.sat[p], ws, m, f);  // beyond kWidePolyCap: flagged in the manifold
+        set_feature(fc, f.state, f.type, f.i1, f.i2);
+        write_manifold(a.out + slot, m);
+    }
 }
 
 // ------------------------------------------------------------------ group contact kernel
@@ -542,11 +571,13 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
 
 // ------------------------------------------------------------------ feature-cache kernel
 
-// b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush.
-// Writes the manifold through `out` and the cache through `fc` when the pair is resolved
-// here; returns the pair's stage (kStageOverlap when it needs the full SAT).
+// b3CollideHulls with a feature cache (collide_hulls_cache.cpp:279-344) up to the flush, face
+// rebuilds in ws.  Writes the manifold through `out` and the cache through `fc` when the pair
+// is resolved here; returns the pair's stage (kStageOverlap when it needs the full SAT,
+// kStageWide, with nothing written, when a rebuild's polygon outgrew ws).
+template <class WS>
 __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, uint32_t cls, Manifold* out,
-                                               FeatureCache* fc) {
+                                               FeatureCache* fc, WS& ws) {
     if (cls != 1u) {
         if (cls == 0u) {
             Contact m;
@@ -597,10 +628,11 @@ __device__ __forceinline__ uint32_t cache_pair(const CacheArgs& a, uint32_t p, u
         a.sweep(p, q10, q1, q20, q2);
         switch (type) {
             case kFeatureEdge1: build_edge_contact(m, xf1, i1, h1, xf2, i2, h2, true); break;
-            case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2); break;
-            case kFeatureFace2: rebuild_face_contact(m, xf2, i1, h2, r2, xf1, h1, r1, true, q20, q2, q10, q1); break;
+            case kFeatureFace1: rebuild_face_contact(m, xf1, i1, h1, r1, xf2, h2, r2, false, q10, q1, q20, q2, ws); break;
+            case kFeatureFace2: rebuild_face_contact(m, xf2, i1, h2, r2, xf1, h1, r1, true, q20, q2, q10, q1, ws); break;
             default: break;
         }
+        if (m.overflow) return kStageWide;  // nothing written: the wide pass runs the pair again
         if (m.pointCount > 0) {
             write_manifold(out, m);
             return kStageSkip;
@@ -952,7 +984,9 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4))) void c
 }
 
 // The pairs the group kernel marked kStageCacheFallback, one lane each, with the
-// reference's 32-vertex clip polygon.
+// reference's 32-vertex clip polygon; a pair whose polygon outgrows that is listed for
+// cache_wide_kernel (in the rebuild lists' storage, which cache_group_kernel has finished
+// reading) and writes nothing here.
 __global__ __launch_bounds__(256) void cache_fallback_kernel(CacheArgs a) {
     const uint32_t p = blockIdx.x * blockDim.x + threadIdx.x;
     bool miss = false, hit = false;
@@ -961,13 +995,47 @@ __global__ __launch_bounds__(256) void cache_fallback_kernel(CacheArgs a) {
         Manifold* const out = a.out + slot;
         FeatureCache* const fc = &a.caches[slot].feature;
         SatResult* const sat = a.sat + p;
-        const uint32_t st = cache_pair(a, p, 1u, out, fc);
-        hit = st == kStageSkip;
-        miss = !hit;
-        if (miss) sat->stage = kStageOverlap;
+        FaceWs ws;
+        const uint32_t st = cache_pair(a, p, 1u, out, fc, ws);
+        if (st == kStageWide) {
+            a.rebuildList[atomicAdd(a.counts + kCountCacheWide, 1u)] = p;
+        } else {
+            hit = st == kStageSkip;
+            miss = !hit;
+            if (miss) sat->stage = kStageOverlap;
+        }
     }
     wave_append(miss, p, a.satList, a.counts + kCountSat);
     if (a.counters) wave_count(&counter_slot(a.counters)->convexCacheHits, hit ? 1u : 0u);
+}
+
+// The pairs cache_fallback_kernel listed, one per block, its first lane rebuilding with the
+// polygons in LDS (room for kWidePolyCap vertices); a miss goes to the SAT list.
+__global__ __launch_bounds__(64) void cache_wide_kernel(CacheArgs a) {
+    __shared__ ClipV polyBuf[kWidePolyCap], scratchBuf[kWidePolyCap];
+    __shared__ CPV keptBuf[kWidePolyCap];
+    if (threadIdx.x != 0u) return;
+    const uint32_t count = a.counts[kCountCacheWide];
+    uint32_t hits = 0;
+    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+        const uint32_t p = a.rebuildList[e];
+        const uint32_t slot = a.slot_of(p);
+        WideFaceWs ws{PolyView{0u, false, kWidePolyCap, polyBuf}, PolyView{0u, false, kWidePolyCap, scratchBuf},
+                      CPolyView{0u, keptBuf}};
+        // Beyond kWidePolyCap the rebuild keeps its flagged polygon and the pair takes the SAT.
+        uint32_t st = cache_pair(a, p, 1u, a.out + slot, &a.caches[slot].feature, ws);
+        if (st == kStageWide) {
+            a.caches[slot].feature.state = kCacheEmpty;
+            st = kStageOverlap;
+        }
+        if (st == kStageSkip) {
+            ++hits;
+        } else {
+            a.sat[p].stage = kStageOverlap;
+            a.satList[atomicAdd(a.counts + kCountSat, 1u)] = p;
+        }
+    }
+    if (a.counters && hits) atomicAdd(&counter_slot(a.counters)->convexCacheHits, hits);
 }
 
 // ------------------------------------------------------------------ launchers
@@ -989,6 +1057,8 @@ hipError_t launch_cache_hull(const CacheArgs& a, hipStream_t s) {
     cache_group_kernel<<<(uint32_t)blocks, 256, 0, s>>>(a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     cache_fallback_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    cache_wide_kernel<<<kWideBlocks, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 
@@ -1002,6 +1072,8 @@ hipError_t launch_contact_hull(const ContactArgs& a, hipStream_t s) {
     contact_group_kernel<<<(uint32_t)((threads + 255) / 256), 256, 0, s>>>(a);
     if ((e = hipGetLastError()) != hipSuccess) return e;
     contact_hull_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    contact_wide_kernel<<<kWideBlocks, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

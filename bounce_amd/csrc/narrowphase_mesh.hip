@@ -237,66 +237,120 @@ __global__ __launch_bounds__(256) void mesh_list_kernel(MeshArgs a) {
     a.contactOffset[c] = list_work(a, c, m > 0 ? segment_of(a.shapes[a.shapeA[c]].type) : 3u, m);
 }
 
+// Statistics of the per-triangle collides a lane ran.
+struct MeshTally {
+    uint32_t hullCalls = 0, hits = 0, calls = 0, iters = 0, maxIters = 0;
+};
+
+// Work item f (of t0 + t1 + t2, segments back to back): b3CollideShapeAndShape of the
+// contact's convex shape and the triangle, hull-triangle face contacts built in ws, the
+// manifold into temp[f] with the triangle index in every point key.  Returns false, writing
+// nothing and counting nothing, when a clip polygon outgrew ws (the wide pass runs it again:
+// the triangle's cache is as it was, collide_hulls_cached_seq stops before writing it).
+template <bool Wide>
+__device__ __forceinline__ bool mesh_item(const MeshArgs& a, uint32_t f, uint32_t t0, uint32_t t1, WideFaceWs* wide,
+                                          TriHull& tri, MeshTally& tally) {
+    const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
+    const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
+    const uint2 item = a.pairContact[(size_t)seg * a.n * a.cap + local];
+    const uint32_t c = item.x, ti = item.y;
+    Manifold* const out = a.temp + f;
+    const uint4 t = a.mesh.tris[ti];
+    const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
+    tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
+    const ShapeRec s = a.shapes[a.shapeA[c]];
+    const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+    const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+    Contact m;
+    m.pointCount = 0;
+    m.overflow = false;
+    uint32_t hit = 0;
+    if (s.type == 2u) {
+        const DevHull h = DevHull::at_unit(a.pool, s.hullUnit);
+        FaceWs lane;
+        auto& ws = *select_ws<Wide>(&lane, wide);
+        if (a.triCaches) {
+            // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
+            // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.
+            const uint32_t k = local - a.contactOffset[c];
+            FeatureCache* const fc = &a.triCaches[(size_t)c * a.cap + k].feature;
+            Quat q[4];
+            for (int w = 0; w < 4; ++w) {
+                if (a.sweeps) {
+                    const float* qs = a.sweeps + 16 * (size_t)c + 4 * w;
+                    q[w] = Quat{qs[0], qs[1], qs[2], qs[3]};
+                } else {
+                    q[w] = Quat{0.0f, 0.0f, 0.0f, 1.0f};
+                }
+            }
+            collide_hulls_cached_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, fc, q[0], q[1], q[2], q[3], hit, ws);
+        } else {
+            collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr, ws);
+        }
+        if (m.overflow && !Wide) return false;
+        tally.hullCalls += 1;
+        tally.hits += hit;
+    } else if (s.type == 0u) {
+        collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
+    } else {
+        collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
+    }
+    write_manifold(out, m);
+    for (uint32_t j = 0; j < m.pointCount; ++j) out->points[j].triangleKey = ti;
+    tally.calls += m.gjkCalls;
+    tally.iters += m.gjkIters;
+    tally.maxIters = m.gjkIters > tally.maxIters ? m.gjkIters : tally.maxIters;
+    return true;
+}
+
 // With counting on, hull-triangle collides go to b3_convexCalls and the sphere/capsule
 // GJK calls to the b3_gjk* totals, as b3MeshContact::Collide's per-triangle
 // b3CollideShapeAndShape calls count them in the reference (mesh_contact.cpp:217-275).
+// Items whose clip polygon outgrows the lane's 32 vertices are listed (wideItems, count in
+// totals[4]) for mesh_wide_kernel.
 __global__ __launch_bounds__(256) void mesh_narrow_kernel(MeshArgs a) {
+    // Each lane's triangle hull in LDS: the collide routines take it by reference.
+    __shared__ TriHull triLds[256];
     const uint32_t f = blockIdx.x * blockDim.x + threadIdx.x;
     const uint32_t t0 = a.totals[0], t1 = a.totals[1], t2 = a.totals[2];
-    uint32_t hullCalls = 0, hits = 0, calls = 0, iters = 0;
+    MeshTally tally;
     if (f < t0 + t1 + t2) {
-        const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
-        const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
-        const uint2 item = a.pairContact[(size_t)seg * a.n * a.cap + local];
-        const uint32_t c = item.x, ti = item.y;
-        Manifold* const out = a.temp + f;
-        const uint4 t = a.mesh.tris[ti];
-        const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
-        const TriHull tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
-        const ShapeRec s = a.shapes[a.shapeA[c]];
-        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
-        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
-        Contact m;
-        m.pointCount = 0;
-        m.overflow = false;
-        if (s.type == 2u) {
-            const DevHull h = DevHull::at_unit(a.pool, s.hullUnit);
-            if (a.triCaches) {
-                // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
-                // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.
-                const uint32_t k = local - a.contactOffset[c];
-                FeatureCache* const fc = &a.triCaches[(size_t)c * a.cap + k].feature;
-                Quat q[4];
-                for (int w = 0; w < 4; ++w) {
-                    if (a.sweeps) {
-                        const float* qs = a.sweeps + 16 * (size_t)c + 4 * w;
-                        q[w] = Quat{qs[0], qs[1], qs[2], qs[3]};
-                    } else {
-                        q[w] = Quat{0.0f, 0.0f, 0.0f, 1.0f};
-                    }
-                }
-                collide_hulls_cached_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, fc, q[0], q[1], q[2], q[3], hits);
-            } else {
-                collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr);
-            }
-            hullCalls = 1;
-        } else if (s.type == 0u) {
-            collide_sphere_hull(m, xfA, s, xfB, tri, 0.0f);
-        } else {
-            collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
-        }
-        write_manifold(out, m);
-        for (uint32_t j = 0; j < m.pointCount; ++j) out->points[j].triangleKey = ti;
-        calls = m.gjkCalls;
-        iters = m.gjkIters;
+        if (!mesh_item<false>(a, f, t0, t1, nullptr, triLds[threadIdx.x], tally))
+            a.wideItems[atomicAdd(a.totals + 4, 1u)] = f;
     }
     if (a.counters) {  // all lanes of the block reach here
         Counters* const ct = counter_slot(a.counters);
-        wave_count(&ct->convexCalls, hullCalls);
-        wave_count(&ct->convexCacheHits, hits);
-        wave_count(&ct->gjkCalls, calls);
-        wave_count(&ct->gjkIters, iters);
-        wave_max(&ct->gjkMaxIters, iters);
+        wave_count(&ct->convexCalls, tally.hullCalls);
+        wave_count(&ct->convexCacheHits, tally.hits);
+        wave_count(&ct->gjkCalls, tally.calls);
+        wave_count(&ct->gjkIters, tally.iters);
+        wave_max(&ct->gjkMaxIters, tally.maxIters);
+    }
+}
+
+// The items mesh_narrow_kernel listed, one per block, its first lane colliding with the
+// polygons in LDS (room for kWidePolyCap vertices).
+constexpr uint32_t kMeshWideBlocks = 64;
+__global__ __launch_bounds__(64) void mesh_wide_kernel(MeshArgs a) {
+    __shared__ ClipV polyBuf[kWidePolyCap], scratchBuf[kWidePolyCap];
+    __shared__ CPV keptBuf[kWidePolyCap];
+    __shared__ TriHull tri;
+    MeshTally tally;
+    if (threadIdx.x == 0u) {
+        const uint32_t t0 = a.totals[0], t1 = a.totals[1], count = a.totals[4];
+        for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+            WideFaceWs ws{PolyView{0u, false, kWidePolyCap, polyBuf}, PolyView{0u, false, kWidePolyCap, scratchBuf},
+                          CPolyView{0u, keptBuf}};
+            mesh_item<true>(a, a.wideItems[e], t0, t1, &ws, tri, tally);  // beyond kWidePolyCap: flagged in the manifold
+        }
+    }
+    if (a.counters) {  // all lanes of the block reach here (lanes other than 0 add nothing)
+        Counters* const ct = counter_slot(a.counters);
+        wave_count(&ct->convexCalls, tally.hullCalls);
+        wave_count(&ct->convexCacheHits, tally.hits);
+        wave_count(&ct->gjkCalls, tally.calls);
+        wave_count(&ct->gjkIters, tally.iters);
+        wave_max(&ct->gjkMaxIters, tally.maxIters);
     }
 }
 
@@ -850,6 +904,9 @@ hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s) {
     const uint64_t bound = (uint64_t)a.n * a.cap;
     if (bound == 0) return hipSuccess;
     mesh_narrow_kernel<<<(uint32_t)((bound + 255) / 256), 256, 0, s>>>(a);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    mesh_wide_kernel<<<kMeshWideBlocks, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

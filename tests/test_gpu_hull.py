@@ -317,6 +317,39 @@ def test_gpu_large_faces_fallback(ctx, oracle):
     assert rebuilt_12gon >= 10, rebuilt_12gon
 
 
+def test_gpu_wide_polygons(ctx, oracle):
+    """Clip polygons of 48-120 vertices (24-, 40- and 60-gon prism caps stacked and turned),
+    beyond the lane kernels' 32: the wide pass builds them with the polygons in LDS, equal to
+    the oracle bit for bit and unflagged, cold and over three warm steps with caches and
+    sweeps (the cached face rebuilds of these caps take the wide cache pass)."""
+    from util import wide_prism_pairs
+    say("case: wide clip polygons (24/40/60-gon caps)")
+    rng = np.random.default_rng(23)
+    pool, pairs, xfs = wide_prism_pairs(3, 300)
+    n = len(pairs)
+    want = oracle.collide(pool, pool.shapes(), pairs, xfs)
+    got = run(ctx, pool, pool.shapes(), pairs, xfs)
+    assert (got["flags"] == 0).all()
+    assert_manifolds_equal(got, want, "wide prism caps")
+    say("case: wide clip polygons, warm steps with caches")
+    cg, co = empty_caches(n), empty_caches(n)
+    x = xfs.copy()
+    for step in range(4):
+        sweeps = np.zeros((n, 4, 4), np.float32)
+        sweeps[:, :, 3] = 1.0
+        if step > 0:
+            yaw = rng.uniform(0.0, 0.04, n).astype(np.float32)
+            sweeps[:, 3, 1] = yaw
+            sweeps[:, 3, 3] = np.sqrt(1.0 - yaw * yaw)
+        got = run(ctx, pool, pool.shapes(), pairs, x, sweeps=sweeps, caches=cg)
+        want = oracle.collide(pool, pool.shapes(), pairs, x, sweeps=sweeps, caches=co)
+        assert (got["flags"] == 0).all()
+        assert_manifolds_equal(got, want, "wide prism caps, warm step %d" % step)
+        assert_caches_equal(cg, co, "wide prism caps, warm step %d" % step)
+        x[:, 1, 9:12] += rng.normal(0.0, 0.0005, size=(n, 3)).astype(np.float32)
+    assert ((co["state"] == 1) & (co["ftype"] != 0)).sum() > 100  # face features cached and rebuilt
+
+
 def test_gpu_degenerate_hulls(ctx, oracle):
     """Flat two-face plates (edge arcs are half circles), knife edges, boxes at identity and
     at 45 degrees, identical placements and 12-gon prisms, against each other in every

@@ -61,6 +61,26 @@ def test_gpu_mesh_large_clusters_equal_oracle(ctx, oracle, seed):
     assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "plate manifolds")
 
 
+def test_gpu_mesh_wide_polygons_equal_oracle(ctx, oracle):
+    """40- and 60-gon prisms lying on large flat triangles: cap and triangle clip to polygons
+    of 40-63 vertices, beyond the lane narrowphase's 32; the wide pass builds them, equal to
+    the oracle bit for bit and unflagged."""
+    from util import wide_prism_mesh
+    print("case: wide polygons on a mesh", flush=True)
+    pool, shapes, v, t, xfs = wide_prism_mesh(4, 60)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    nodes = ctx.mesh_tree(mid)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    counts, lists, out, mcount = ctx.collide_mesh(mid, sa, xfs)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs)
+    np.testing.assert_array_equal(counts, co[0])
+    np.testing.assert_array_equal(mcount, co[3])
+    assert (out["flags"] == 0).all()
+    assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "wide prisms on a mesh")
+
+
 def test_gpu_mesh_heightfield_full_size(ctx, lib, oracle):
     """The mixed-shapes-on-terrain configuration at full size: a 256 x 256 heightfield
     (131,072 triangles) and 100,000 spheres, capsules and hulls.  All contacts run on the

@@ -280,6 +280,27 @@ def test_oracle_mesh_large_clusters_equal_reference(oracle, reference, seed):
     assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "plate manifolds")
 
 
+def test_oracle_wide_polygons_equal_reference(oracle, reference):
+    """Clip polygons of 48-120 vertices (prism caps stacked, and prisms on large mesh
+    triangles): the reference's polygon grows past its 32-vertex inline capacity; the
+    restatement keeps them whole.  A 32-vertex cut changes most of these manifolds."""
+    import oracle_lib
+    from util import wide_prism_mesh, wide_prism_pairs
+    pool, pairs, xfs = wide_prism_pairs(3, 300)
+    a = oracle.collide(pool, pool.shapes(), pairs, xfs)
+    b = reference.collide(pool, pool.shapes(), pairs, xfs)
+    assert (b["pointCount"] > 0).all()
+    assert_manifolds_equal(a, b, "wide prism caps")
+    pool, shapes, v, t, xfs = wide_prism_mesh(4, 60)
+    nodes = oracle_lib.build_tree(reference.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs)
+    cr = oracle_lib.collide_mesh(reference.lib, "b3ref_", pool, shapes, v, t, nodes, sa, xfs)
+    np.testing.assert_array_equal(co[0], cr[0])
+    np.testing.assert_array_equal(co[3], cr[3])
+    assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "wide prisms on a mesh")
+
+
 def test_oracle_mesh_golden(oracle):
     import oracle_lib
     g = load_golden("mesh_contacts.npz")

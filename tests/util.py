@@ -939,3 +939,62 @@ def rain_script(world, steps):
         states.append((moving, xf, sw))
     return {"pool": world["pool"], "shapes": world["shapes"], "spawns": spawns, "states": states, "steps": steps,
             "order": np.array(order)}
+
+
+# ---- clip polygons beyond the reference's 32-vertex inline capacity
+
+def prism_hull(n, r, h):
+    """Right prism over a regular n-gon (radius r, half-height h): two n-gon caps."""
+    from bounce_amd.shapes import polyhedron
+    a = 2 * np.pi * np.arange(n) / n
+    ring = np.stack([r * np.cos(a), np.zeros(n), r * np.sin(a)], 1)
+    v = np.concatenate([ring + (0, -h, 0), ring + (0, h, 0)])
+    faces = [list(range(n)), list(range(2 * n - 1, n - 1, -1))]  # outward: -y, +y
+    faces += [[i, n + i, n + (i + 1) % n, (i + 1) % n] for i in range(n)]
+    return polyhedron(v, faces)
+
+
+def wide_prism_pairs(seed, n):
+    """Prisms with 24-, 40- and 60-gon caps stacked cap to cap, the upper one turned about
+    the vertical, tilted by a few milliradians and sunk up to 2 cm: the incident cap clipped by
+    the reference cap's side planes gives polygons of up to twice the sides (48-120 vertices),
+    beyond the reference's 32-vertex inline polygon (clip.cpp's b3StackArray grows on the
+    heap).  Returns (pool, pairs, xfs)."""
+    from bounce_amd.shapes import identity_xf
+    from bounce_amd.types import HullPool
+    rng = np.random.default_rng(seed)
+    pool = HullPool.concat([prism_hull(k, r, 0.3) for k, r in ((24, 0.6), (40, 0.55), (60, 0.65))])
+    pairs = rng.integers(0, pool.count, (n, 2)).astype(np.uint32)
+    xfs = np.zeros((n, 2, 12), np.float32)
+    for i in range(n):
+        yaw = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(yaw), np.sin(yaw)
+        R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+        tilt = rng.normal(0, 0.002, 3)
+        R = R @ np.array([[1, -tilt[2], tilt[1]], [tilt[2], 1, -tilt[0]], [-tilt[1], tilt[0], 1]])
+        xfs[i, 0] = identity_xf((0, 0, 0))
+        xfs[i, 1] = identity_xf((rng.normal(0, 0.05), 0.6 - rng.uniform(0.0, 0.02), rng.normal(0, 0.05)), R)
+    return pool, pairs, xfs
+
+
+def wide_prism_mesh(seed, n_shapes):
+    """40- and 60-gon prisms lying on a flat grid of 4 m cells (each cap within one or two
+    large triangles), turned about the vertical, sunk 1-10 mm: the cap and the triangle clip
+    to polygons of 40-63 vertices.  Returns (pool, shapes, vertices, triangles, xfs)."""
+    from bounce_amd.shapes import identity_xf
+    from bounce_amd.types import HULL, SHAPE_DTYPE, HullPool
+    rng = np.random.default_rng(seed)
+    verts, tris = grid_mesh(8, 8, cell=4.0, amplitude=0.0, seed=seed)
+    pool = HullPool.concat([prism_hull(40, 0.5, 0.1), prism_hull(60, 0.6, 0.1)])
+    shapes = np.zeros(n_shapes, SHAPE_DTYPE)
+    shapes["type"] = HULL
+    shapes["hull"] = rng.integers(0, pool.count, n_shapes)
+    xfs = np.zeros((n_shapes, 2, 12), np.float32)
+    for i in range(n_shapes):
+        yaw = rng.uniform(0, 2 * np.pi)
+        c, s = np.cos(yaw), np.sin(yaw)
+        R = np.array([[c, 0, s], [0, 1, 0], [-s, 0, c]])
+        x, z = rng.uniform(-14.0, 14.0, 2)
+        xfs[i, 0] = identity_xf((x, 0.1 - rng.uniform(0.001, 0.01), z), R)
+        xfs[i, 1] = identity_xf((0, 0, 0))
+    return pool, shapes, verts, tris, xfs
