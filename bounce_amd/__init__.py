@@ -765,6 +765,7 @@ class Context:
         return nodes
 
     MESH_TRIANGLE_CAP = 64
+    MESH_WIDE_TRIANGLES = 2048  # B3AMD_MESH_WIDE_TRIANGLES: fresh contacts collided whole up to this
 
     def collide_mesh(self, mesh_id, shape_a, xfs):
         """Fresh mesh contacts: (triangle counts, triangle lists [n][64], manifolds [n][3],

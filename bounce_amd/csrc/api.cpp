@@ -49,6 +49,8 @@ int fail(int code, const char* fmt, ...) {
 // kernel at the start of every batch).  Host code never names the array: its address on
 // each device comes from hipGetSymbolAddress, taken once per device when the first context
 // on that device is created.
+static_assert(B3AMD_MESH_TRIANGLE_CAP == kMeshTriangleCap && B3AMD_MESH_WIDE_TRIANGLES == kWideTris,
+              "mesh list capacities of the public header");
 constexpr uint32_t kNoAllocMaxPairs = 262144u;
 constexpr uint64_t kNoAllocStageWords =
     2ull * kNoAllocMaxPairs +
@@ -1006,14 +1008,17 @@ struct MeshScratch {
     float* sweeps;   // staged per-contact sweeps (persistent contacts)
     float* centers;  // staged body-A centres (persistent contacts)
     uint32_t* wideItems;
+    uint32_t* wideContacts;
+    void* wideSlabs;
 };
 
 int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
     const uint64_t cap = kMeshTriangleCap;
     auto al = [](uint64_t x) { return (x + 255) & ~255ull; };
-    const uint64_t sizes[14] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
+    const uint64_t sizes[16] = {4ull * n,  96ull * n, 4ull * n,   4ull * n * cap, 4ull * n,
                                 4ull * n,  256,       24ull * n * cap, 720ull * n, 4ull * n,
-                                sizeof(Manifold) * n * cap, 64ull * n, 12ull * n, 4ull * n * cap};
+                                sizeof(Manifold) * n * cap, 64ull * n, 12ull * n, 4ull * n * cap,
+                                4ull * n, kWideSlabBytes * kWideContactBlocks};
     uint64_t total = 0;
     for (uint64_t b : sizes) total += al(b);
     if (total > ctx->meshScratchCap) {
@@ -1025,15 +1030,15 @@ int mesh_scratch(b3amd_context* ctx, uint32_t n, MeshScratch* m) {
         ctx->meshScratchCap = total;
     }
     char* p = static_cast<char*>(ctx->dMeshScratch);
-    void* parts[14];
-    for (int i = 0; i < 14; ++i) {
+    void* parts[16];
+    for (int i = 0; i < 16; ++i) {
         parts[i] = p;
         p += al(sizes[i]);
     }
     *m = MeshScratch{(uint32_t*)parts[0], (float*)parts[1], (uint32_t*)parts[2],  (uint32_t*)parts[3],
                      (uint32_t*)parts[4], (uint32_t*)parts[5], (uint32_t*)parts[6], (uint2*)parts[7],
                      (Manifold*)parts[8], (uint32_t*)parts[9], (Manifold*)parts[10], (float*)parts[11],
-                     (float*)parts[12], (uint32_t*)parts[13]};
+                     (float*)parts[12], (uint32_t*)parts[13], (uint32_t*)parts[14], parts[15]};
     return B3AMD_OK;
 }
 
@@ -1054,6 +1059,8 @@ int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t
     a.totals = sc.totals;
     a.pairContact = sc.pairContact;
     a.wideItems = sc.wideItems;
+    a.wideContacts = sc.wideContacts;
+    a.wideSlabs = sc.wideSlabs;
     a.out = out;
     a.manifoldCounts = manifoldCounts;
     a.temp = sc.temp;
@@ -1074,7 +1081,8 @@ int enqueue_mesh(b3amd_context* ctx, uint32_t meshId, uint32_t n, const uint32_t
     if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 8, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_midphase", [&] { return launch_mesh_midphase(a, s); }))) return rc;
     if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
-    return timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
+    if ((rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); }))) return rc;
+    return timed(ctx, s, "mesh_contact_wide", [&] { return launch_mesh_contact_wide(a, s); });
 }
 
 int check_mesh_call(b3amd_context* ctx, uint32_t meshId, uint32_t n) {

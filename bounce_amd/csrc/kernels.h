@@ -303,6 +303,14 @@ struct MeshDev {
 constexpr uint32_t kMeshTriangleCap = 64;
 constexpr uint32_t kFlagTriangleOverflow = 4u;
 constexpr uint32_t kFlagTreeStackOverflow = 8u;
+// A fresh contact with more triangles than the list holds: the wide-contact pass queries
+// again into a slab of kWideTris triangles and owns the contact's outputs (internal flag,
+// cleared before any output).
+constexpr uint32_t kFlagWideContact = 16u;
+constexpr uint32_t kWideContactBlocks = 32;
+constexpr uint32_t kWideTris = 2048;
+constexpr uint32_t kWideObs = 4 * kWideTris;
+constexpr uint64_t kWideSlabBytes = 16ull * 2 * kWideObs + 240ull * kWideTris + 4ull * kWideTris;
 
 struct MeshArgs {
     Counters* counters;         // statistics totals, or null when counting is off
@@ -327,6 +335,9 @@ struct MeshArgs {
                                 // (zeroed before the midphase)
     uint32_t* wideItems;        // [n * cap] work items for the wide pass (clip polygons beyond
                                 // the lane's 32 vertices)
+    uint32_t* wideContacts = nullptr;  // [n] fresh contacts beyond cap triangles (count in totals[5]),
+                                       // null for persistent contacts
+    void* wideSlabs = nullptr;         // [kWideContactBlocks] slabs of kWideSlabBytes
     uint2* pairContact;         // [3][n * cap] work items of a segment: (contact, triangle); once
                                 // the narrowphase has run, the clustering's second-pass list
                                 // (uint32 [n])
@@ -354,6 +365,8 @@ hipError_t launch_mesh_midphase(const MeshArgs& a, hipStream_t s);
 // The grid covers the n * cap bound: the work-item count stays on the device.
 hipError_t launch_mesh_narrow(const MeshArgs& a, hipStream_t s);
 hipError_t launch_mesh_cluster(const MeshArgs& a, hipStream_t s);
+// The fresh contacts the midphase listed in wideContacts (more triangles than cap).
+hipError_t launch_mesh_contact_wide(const MeshArgs& a, hipStream_t s);
 // b3MeshContact::TestOverlap of every persistent contact (one lane per contact).
 hipError_t launch_mesh_overlap(const MeshArgs& a, hipStream_t s);
 

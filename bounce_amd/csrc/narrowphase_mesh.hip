@@ -90,26 +90,26 @@ __device__ __forceinline__ bool mesh_pair_valid(const MeshArgs& a, uint32_t si) 
 constexpr uint32_t kLdsStack = 32;
 constexpr uint32_t kMidphaseBlock = 256;
 
-__device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, uint32_t& flags, uint32_t* lds) {
+__device__ uint32_t tree_query(const MeshArgs& a, uint32_t* list, uint32_t cap, const Aabb& box, uint32_t& flags,
+                               uint32_t* lds, uint32_t ldsStride = kMidphaseBlock) {
     uint32_t found = 0;
     uint32_t deep[kTreeStack - kLdsStack];
     uint32_t top = 0;
     auto push = [&](uint32_t v) {
-        if (top < kLdsStack) lds[kMidphaseBlock * top] = v;
+        if (top < kLdsStack) lds[ldsStride * top] = v;
         else deep[top - kLdsStack] = v;
         ++top;
     };
     if (a.mesh.nodeCount > 0) push(0u);
-    uint32_t* list = a.triLists + (size_t)c * a.cap;
     while (top) {
         --top;
-        const uint32_t ni = top < kLdsStack ? lds[kMidphaseBlock * top] : deep[top - kLdsStack];
+        const uint32_t ni = top < kLdsStack ? lds[ldsStride * top] : deep[top - kLdsStack];
         const float4* nd = reinterpret_cast<const float4*>(a.mesh.nodes + 8 * (size_t)ni);
         const float4 q0 = nd[0], q1 = nd[1];  // lower xyz, upper x | upper yz, child1, child2
         if (!overlaps(q0, q1, box)) continue;
         const uint32_t child1 = __float_as_uint(q1.z), child2 = __float_as_uint(q1.w);
         if (child1 == 0xFFFFFFFFu) {
-            if (found < a.cap) list[found] = child2;
+            if (found < cap) list[found] = child2;
             ++found;
         } else if (top + 2 <= kTreeStack) {
             push(child1);
@@ -118,8 +118,20 @@ __device__ uint32_t tree_query(const MeshArgs& a, uint32_t c, const Aabb& box, u
             flags |= kFlagTreeStackOverflow;
         }
     }
-    if (found > a.cap) flags |= kFlagTriangleOverflow;
+    if (found > cap) flags |= kFlagTriangleOverflow;
     return found;
+}
+
+// The fat AABB of contact c's shape A in the mesh frame (mesh_contact.cpp:30-60): the shape's
+// AABB in the frame, extended by B3_AABB_EXTENSION (settings.h:54).
+__device__ __forceinline__ Aabb fat_box(const MeshArgs& a, uint32_t c, const ShapeRec& s) {
+    const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
+    const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
+    Aabb box = shape_aabb(s, mulT(xfB, xfA), a.pool, a.offsets);
+    const V3 ext{0.2f, 0.2f, 0.2f};
+    box.lo = box.lo - ext;
+    box.hi = box.hi + ext;
+    return box;
 }
 
 // Appends contact c's m triangles as work items (contact, triangle) of its collider's
@@ -149,18 +161,18 @@ __global__ __launch_bounds__(kMidphaseBlock) void mesh_midphase_kernel(MeshArgs 
     const uint32_t si = a.shapeA[c];
     uint32_t found = 0;
     if (mesh_pair_valid(a, si)) {
-        const ShapeRec s = a.shapes[si];
-        const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
-        const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
-        Aabb box = shape_aabb(s, mulT(xfB, xfA), a.pool, a.offsets);
-        const V3 ext{0.2f, 0.2f, 0.2f};  // B3_AABB_EXTENSION (settings.h:54)
-        box.lo = box.lo - ext;
-        box.hi = box.hi + ext;
-        found = tree_query(a, c, box, flags, stackLds + threadIdx.x);
+        found = tree_query(a, a.triLists + (size_t)c * a.cap, a.cap, fat_box(a, c, a.shapes[si]), flags,
+                           stackLds + threadIdx.x);
     } else {
         flags |= kFlagInvalidPair;
     }
-    const uint32_t m = (flags & kFlagInvalidPair) ? 0u : (found < a.cap ? found : a.cap);
+    uint32_t m = (flags & kFlagInvalidPair) ? 0u : (found < a.cap ? found : a.cap);
+    if (a.wideContacts && (flags & kFlagTriangleOverflow) && !(flags & kFlagTreeStackOverflow)) {
+        // More triangles than the list holds: mesh_contact_wide_kernel takes the contact whole.
+        flags |= kFlagWideContact;
+        m = 0;
+        a.wideContacts[atomicAdd(a.totals + 5, 1u)] = c;
+    }
     a.triCounts[c] = found;
     a.contactFlags[c] = flags;
     a.contactOffset[c] = list_work(a, c, m > 0 ? segment_of(a.shapes[si].type) : 3u, m);
@@ -212,7 +224,7 @@ __global__ __launch_bounds__(kMidphaseBlock) void mesh_sync_kernel(MeshArgs a, u
     if (query) {
         fat[0] = box.lo.x, fat[1] = box.lo.y, fat[2] = box.lo.z;
         fat[3] = box.hi.x, fat[4] = box.hi.y, fat[5] = box.hi.z;
-        found = tree_query(a, c, box, flags, stackLds + threadIdx.x);
+        found = tree_query(a, a.triLists + (size_t)c * a.cap, a.cap, box, flags, stackLds + threadIdx.x);
         const uint32_t m = found < a.cap ? found : a.cap;
         for (uint32_t k = 0; k < m; ++k) {
             ConvexCache e;
@@ -242,26 +254,21 @@ struct MeshTally {
     uint32_t hullCalls = 0, hits = 0, calls = 0, iters = 0, maxIters = 0;
 };
 
-// Work item f (of t0 + t1 + t2, segments back to back): b3CollideShapeAndShape of the
-// contact's convex shape and the triangle, hull-triangle face contacts built in ws, the
-// manifold into temp[f] with the triangle index in every point key.  Returns false, writing
-// nothing and counting nothing, when a clip polygon outgrew ws (the wide pass runs it again:
-// the triangle's cache is as it was, collide_hulls_cached_seq stops before writing it).
+// b3CollideShapeAndShape of contact c's convex shape s and triangle ti into m (mesh_contact.cpp:
+// 238-266), hull-triangle face contacts built in the lane's workspace or the wide pass's; fc
+// is the triangle's cache (persistent contacts) or null, q its four sweep orientations.
+// Returns false, with nothing written and nothing counted, when a clip polygon outgrew the
+// lane's workspace (the triangle's cache is as it was: collide_hulls_cached_seq stops before
+// writing it).
 template <bool Wide>
-__device__ __forceinline__ bool mesh_item(const MeshArgs& a, uint32_t f, uint32_t t0, uint32_t t1, WideFaceWs* wide,
-                                          TriHull& tri, MeshTally& tally) {
-    const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
-    const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
-    const uint2 item = a.pairContact[(size_t)seg * a.n * a.cap + local];
-    const uint32_t c = item.x, ti = item.y;
-    Manifold* const out = a.temp + f;
+__device__ __forceinline__ bool collide_tri(const MeshArgs& a, uint32_t c, const ShapeRec& s, uint32_t ti,
+                                            FeatureCache* fc, const Quat* q, WideFaceWs* wide, TriHull& tri,
+                                            Contact& m, MeshTally& tally) {
     const uint4 t = a.mesh.tris[ti];
     const float4 A = a.mesh.verts[t.x], B = a.mesh.verts[t.y], C = a.mesh.verts[t.z];
     tri = TriHull::make(V3{A.x, A.y, A.z}, V3{B.x, B.y, B.z}, V3{C.x, C.y, C.z});
-    const ShapeRec s = a.shapes[a.shapeA[c]];
     const Xf xfA = load_xf(a.xfs + 24 * (size_t)c);
     const Xf xfB = load_xf(a.xfs + 24 * (size_t)c + 12);
-    Contact m;
     m.pointCount = 0;
     m.overflow = false;
     uint32_t hit = 0;
@@ -269,24 +276,8 @@ __device__ __forceinline__ bool mesh_item(const MeshArgs& a, uint32_t f, uint32_
         const DevHull h = DevHull::at_unit(a.pool, s.hullUnit);
         FaceWs lane;
         auto& ws = *select_ws<Wide>(&lane, wide);
-        if (a.triCaches) {
-            // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257);
-            // the triangle hull belongs to the mesh body, whose sweep the rebuild reads.
-            const uint32_t k = local - a.contactOffset[c];
-            FeatureCache* const fc = &a.triCaches[(size_t)c * a.cap + k].feature;
-            Quat q[4];
-            for (int w = 0; w < 4; ++w) {
-                if (a.sweeps) {
-                    const float* qs = a.sweeps + 16 * (size_t)c + 4 * w;
-                    q[w] = Quat{qs[0], qs[1], qs[2], qs[3]};
-                } else {
-                    q[w] = Quat{0.0f, 0.0f, 0.0f, 1.0f};
-                }
-            }
-            collide_hulls_cached_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, fc, q[0], q[1], q[2], q[3], hit, ws);
-        } else {
-            collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr, ws);
-        }
+        if (fc) collide_hulls_cached_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, fc, q[0], q[1], q[2], q[3], hit, ws);
+        else collide_cache_seq(m, xfA, h, s.radius, xfB, tri, 0.0f, nullptr, ws);
         if (m.overflow && !Wide) return false;
         tally.hullCalls += 1;
         tally.hits += hit;
@@ -295,11 +286,46 @@ __device__ __forceinline__ bool mesh_item(const MeshArgs& a, uint32_t f, uint32_
     } else {
         collide_capsule_hull(m, xfA, s, xfB, tri, 0.0f);
     }
-    write_manifold(out, m);
-    for (uint32_t j = 0; j < m.pointCount; ++j) out->points[j].triangleKey = ti;
     tally.calls += m.gjkCalls;
     tally.iters += m.gjkIters;
     tally.maxIters = m.gjkIters > tally.maxIters ? m.gjkIters : tally.maxIters;
+    return true;
+}
+
+// The manifold of a triangle collide with the triangle index in every point key.
+__device__ __forceinline__ void write_tri_manifold(Manifold* out, const Contact& m, uint32_t ti) {
+    write_manifold(out, m);
+    for (uint32_t j = 0; j < m.pointCount; ++j) out->points[j].triangleKey = ti;
+}
+
+// Work item f (of t0 + t1 + t2, segments back to back) into temp[f]; false as collide_tri.
+template <bool Wide>
+__device__ __forceinline__ bool mesh_item(const MeshArgs& a, uint32_t f, uint32_t t0, uint32_t t1, WideFaceWs* wide,
+                                          TriHull& tri, MeshTally& tally) {
+    const uint32_t seg = f < t0 ? 0u : (f < t0 + t1 ? 1u : 2u);
+    const uint32_t local = f - (seg == 0u ? 0u : (seg == 1u ? t0 : t0 + t1));
+    const uint2 item = a.pairContact[(size_t)seg * a.n * a.cap + local];
+    const uint32_t c = item.x, ti = item.y;
+    const ShapeRec s = a.shapes[a.shapeA[c]];
+    FeatureCache* fc = nullptr;
+    Quat q[4];
+    if (s.type == 2u && a.triCaches) {
+        // b3MeshContact::Collide with the triangle's cache (mesh_contact.cpp:238-257); the
+        // triangle hull belongs to the mesh body, whose sweep the rebuild reads.
+        const uint32_t k = local - a.contactOffset[c];
+        fc = &a.triCaches[(size_t)c * a.cap + k].feature;
+        for (int w = 0; w < 4; ++w) {
+            if (a.sweeps) {
+                const float* qs = a.sweeps + 16 * (size_t)c + 4 * w;
+                q[w] = Quat{qs[0], qs[1], qs[2], qs[3]};
+            } else {
+                q[w] = Quat{0.0f, 0.0f, 0.0f, 1.0f};
+            }
+        }
+    }
+    Contact m;
+    if (!collide_tri<Wide>(a, c, s, ti, fc, q, wide, tri, m, tally)) return false;
+    write_tri_manifold(a.temp + f, m, ti);
     return true;
 }
 
@@ -500,9 +526,13 @@ __device__ __forceinline__ uint32_t group_min_u32(uint32_t x) {
 }
 
 __device__ __forceinline__ V3 xyz(float4 v) { return V3{v.x, v.y, v.z}; }
-__device__ __forceinline__ uint32_t obs_point(float4 v) { return __float_as_uint(v.w) & 0xFFu; }  // (i << 2) | j
-__device__ __forceinline__ uint32_t obs_cluster(float4 v) { return __float_as_uint(v.w) >> 8; }
-__device__ __forceinline__ uint32_t byte_at(uint32_t packed, uint32_t i) { return (packed >> (8u * i)) & 0xFFu; }
+// An observation's fourth word: its manifold point (i << 2) | j in the low 16 bits, its
+// cluster above.  Member lists pack up to four observation indices in 16-bit fields.
+__device__ __forceinline__ uint32_t obs_point(float4 v) { return __float_as_uint(v.w) & 0xFFFFu; }
+__device__ __forceinline__ uint32_t obs_cluster(float4 v) { return __float_as_uint(v.w) >> 16; }
+__device__ __forceinline__ uint32_t field_at(uint64_t packed, uint32_t i) {
+    return (uint32_t)(packed >> (16u * i)) & 0xFFFFu;
+}
 
 // b3ClosestPointOnSegment (geometry.h:174-193) and the squared distance to it, as
 // InitializeClusters' third pick measures it (contact_cluster.cpp:303-330).
@@ -521,42 +551,39 @@ __device__ __forceinline__ float segment_dist_sq(V3 A, V3 B, V3 Pt) {
 }
 
 // b3SortPolygon (contact_cluster.cpp:33-56) on up to four members (observation indices packed
-// one per byte of q), positions in LDS; the pivot B is read once per i, before any swap.
-__device__ __forceinline__ uint32_t sort_members(uint32_t q, uint32_t count, const float4* ob, V3 N) {
+// in the 16-bit fields of q), positions in ob; the pivot B is read once per i, before any swap.
+__device__ __forceinline__ uint64_t sort_members(uint64_t q, uint32_t count, const float4* ob, V3 N) {
     if (count == 0u) return q;
-    const V3 A = xyz(ob[byte_at(q, 0)]);
+    const V3 A = xyz(ob[field_at(q, 0)]);
     for (uint32_t i = 1; i < count; ++i) {
-        const V3 B = xyz(ob[byte_at(q, i)]);
+        const V3 B = xyz(ob[field_at(q, i)]);
         for (uint32_t j = i + 1; j < count; ++j) {
-            const V3 C = xyz(ob[byte_at(q, j)]);
+            const V3 C = xyz(ob[field_at(q, j)]);
             if (!is_ccw(A, B, C, N)) {
-                const uint32_t bi = byte_at(q, i), bj = byte_at(q, j);
-                q = (q & ~(0xFFu << (8u * i)) & ~(0xFFu << (8u * j))) | (bj << (8u * i)) | (bi << (8u * j));
+                const uint64_t bi = field_at(q, i), bj = field_at(q, j);
+                q = (q & ~(0xFFFFull << (16u * i)) & ~(0xFFFFull << (16u * j))) | (bj << (16u * i)) |
+                    (bi << (16u * j));
             }
         }
     }
     return q;
 }
 
-// Clusters contact ci with Cap observation slots (obN: world normal and the packed manifold
-// point and cluster; obP: world point, later moved along the cluster normal, and the
-// separation).  With more than Cap observations the group lists the contact in `defer` and
-// writes nothing; returns whether it did.
+// Clusters contact ci's m triangle manifolds `in` (midphase order) with Cap observation slots
+// (obN: world normal and the packed manifold point and cluster; obP: world point, later
+// moved along the cluster normal, and the separation), the outputs carrying `flags`.  With
+// more than Cap observations the group lists the contact in `defer` and writes nothing;
+// returns whether it did.
 template <uint32_t Cap>
-__device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, float4* obN, float4* obP, uint32_t g,
-                                                uint32_t gbase, uint32_t* defer, uint32_t* deferCount) {
+__device__ __forceinline__ bool cluster_manifolds(const MeshArgs& a, uint32_t ci, const Manifold* in, uint32_t m,
+                                                  uint32_t flags, float4* obN, float4* obP, uint32_t g,
+                                                  uint32_t gbase, uint32_t* defer, uint32_t* deferCount) {
     Manifold* const out = a.out + 3 * (size_t)ci;
-    const uint32_t flags = a.contactFlags[ci];
-    const uint32_t found = a.triCounts[ci];
-    const uint32_t m = found < a.cap ? found : a.cap;
     uint32_t numOut = 0;
-    uint32_t qs0 = 0, qs1 = 0, qs2 = 0, qn0 = 0, qn1 = 0, qn2 = 0;  // each output's members and count
-    const Manifold* in = nullptr;
+    uint64_t qs0 = 0, qs1 = 0, qs2 = 0;  // each output's manifold points (16-bit fields)
+    uint32_t qn0 = 0, qn1 = 0, qn2 = 0;  // and their counts
     if (!(flags & kFlagInvalidPair) && m > 0) {
         const ShapeRec s = a.shapes[a.shapeA[ci]];
-        const uint32_t seg = segment_of(s.type);
-        const uint32_t base = seg == 0u ? 0u : (seg == 1u ? a.totals[0] : a.totals[0] + a.totals[1]);
-        in = a.temp + base + a.contactOffset[ci];
         const Xf xfA = load_xf(a.xfs + 24 * (size_t)ci);
         const Xf xfB = load_xf(a.xfs + 24 * (size_t)ci + 12);
         const float rA = s.radius, rB = 0.0f;
@@ -665,7 +692,7 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
                 const float4 v = obN[o];
                 const uint32_t c = C.find(xyz(v));
                 changed = changed || c != obs_cluster(v);
-                reinterpret_cast<uint32_t*>(obN + o)[3] = obs_point(v) | (c << 8);
+                reinterpret_cast<uint32_t*>(obN + o)[3] = obs_point(v) | (c << 16);
             }
             if (group_ballot(changed, gbase) == 0u) break;
             group_lds_sync();
@@ -728,13 +755,14 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
             // deepest, then the farthest from it, the largest triangle, the most negative
             // signed area, sorted.  Members are tested by their cluster; chosen members are
             // skipped.  A selection with no candidate keeps index 0: the first member.
-            uint32_t q = 0, qn = 0;
+            uint64_t q = 0;
+            uint32_t qn = 0;
             if (ck <= 4u) {
                 for (uint32_t r = 0; r * kClusterGroup < nObs && qn < ck; ++r) {
                     const uint32_t o = r * kClusterGroup + g;
                     uint32_t bits = group_ballot(o < nObs && obs_cluster(obN[o]) == k, gbase);
                     while (bits != 0u && qn < 4u) {
-                        q |= (r * kClusterGroup + (uint32_t)__builtin_ctz(bits)) << (8u * qn);
+                        q |= (uint64_t)(r * kClusterGroup + (uint32_t)__builtin_ctz(bits)) << (16u * qn);
                         ++qn;
                         bits &= bits - 1u;
                     }
@@ -760,7 +788,7 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
                     group_first_max(v, idx);
                     if (v < kEpsilon * kEpsilon) break;
                     const uint32_t q1 = idx;
-                    q |= q1 << 8;
+                    q |= (uint64_t)q1 << 16;
                     qn = 2;
                     const V3 B = xyz(obP[q1]);
                     v = -kMaxFloat;
@@ -778,10 +806,10 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
                     const float kAreaTol = 0.01f;
                     if (v < 2.0f * kAreaTol) break;
                     const uint32_t q2 = idx;
-                    if (!is_ccw(A, B, xyz(obP[q2]), normal)) q = q1 | (q0 << 8);
-                    q |= q2 << 16;
+                    if (!is_ccw(A, B, xyz(obP[q2]), normal)) q = (uint64_t)q1 | ((uint64_t)q0 << 16);
+                    q |= (uint64_t)q2 << 32;
                     qn = 3;
-                    const V3 A2 = xyz(obP[byte_at(q, 0)]), B2 = xyz(obP[byte_at(q, 1)]);
+                    const V3 A2 = xyz(obP[field_at(q, 0)]), B2 = xyz(obP[field_at(q, 1)]);
                     v = kMaxFloat;
                     idx = kNoIndex;
                     for (uint32_t o = g; o < nObs; o += kClusterGroup) {
@@ -795,16 +823,16 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
                     }
                     group_first_min(v, idx);
                     if (fabsf(v) < 2.0f * kAreaTol) break;
-                    q |= (idx == kNoIndex ? first : idx) << 24;
+                    q |= (uint64_t)(idx == kNoIndex ? first : idx) << 48;
                     qn = 4;
                     q = sort_members(q, 4u, obP, normal);
                 } while (false);
             }
-            // The members' manifold points (packed (i << 2) | j, one byte each).
-            uint32_t pts = 0;
+            // The members' manifold points (i << 2) | j.
+            uint64_t pts = 0;
 #pragma unroll
             for (uint32_t j = 0; j < 4u; ++j)
-                if (j < qn) pts |= obs_point(obN[byte_at(q, j)]) << (8u * j);
+                if (j < qn) pts |= (uint64_t)obs_point(obN[field_at(q, j)]) << (16u * j);
             if (numOut == 0u) qs0 = pts, qn0 = qn;
             else if (numOut == 1u) qs1 = pts, qn1 = qn;
             else qs2 = pts, qn2 = qn;
@@ -819,11 +847,11 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
     if (g < 12u) {
         const uint32_t t = g >> 2, j = g & 3u;
         if (t < numOut) {
-            const uint32_t qsT = t == 0u ? qs0 : (t == 1u ? qs1 : qs2);
+            const uint64_t qsT = t == 0u ? qs0 : (t == 1u ? qs1 : qs2);
             const uint32_t qnT = t == 0u ? qn0 : (t == 1u ? qn1 : qn2);
             ManifoldPoint pt;
             if (j < qnT) {
-                const uint32_t pi = byte_at(qsT, j);
+                const uint32_t pi = field_at(qsT, j);
                 pt = in[pi >> 2].points[pi & 3u];
             } else {
                 pt.localNormal1 = V3{0.0f, 0.0f, 0.0f};
@@ -846,6 +874,24 @@ __device__ __forceinline__ bool cluster_contact(const MeshArgs& a, uint32_t ci, 
             reinterpret_cast<uint4*>(out + t)[g] = g == 0u ? make_uint4(0u, flags, 0u, 0u) : make_uint4(0u, 0u, 0u, 0u);
     if (g == 0u) a.manifoldCounts[ci] = numOut;
     return false;
+}
+
+// Contact ci from the narrowphase's work items (its manifolds sit at its segment's offset in
+// temp); a contact the wide-contact pass owns is left to it.
+template <uint32_t Cap>
+__device__ __forceinline__ void cluster_contact(const MeshArgs& a, uint32_t ci, float4* obN, float4* obP, uint32_t g,
+                                                uint32_t gbase, uint32_t* defer, uint32_t* deferCount) {
+    const uint32_t flags = a.contactFlags[ci];
+    if (flags & kFlagWideContact) return;
+    const uint32_t found = a.triCounts[ci];
+    const uint32_t m = found < a.cap ? found : a.cap;
+    const Manifold* in = nullptr;
+    if (!(flags & kFlagInvalidPair) && m > 0) {
+        const uint32_t seg = segment_of(a.shapes[a.shapeA[ci]].type);
+        const uint32_t base = seg == 0u ? 0u : (seg == 1u ? a.totals[0] : a.totals[0] + a.totals[1]);
+        in = a.temp + base + a.contactOffset[ci];
+    }
+    cluster_manifolds<Cap>(a, ci, in, m, flags, obN, obP, g, gbase, defer, deferCount);
 }
 
 // Every contact, four per wave, kClusterLds observation slots each; contacts with more
@@ -877,6 +923,79 @@ __global__ __launch_bounds__(kClusterBlock) void mesh_cluster_wide_kernel(MeshAr
         cluster_contact<kMaxObs>(a, reinterpret_cast<const uint32_t*>(a.pairContact)[e], obsN[lane / kClusterGroup], obsP[lane / kClusterGroup], g,
                                  gbase, nullptr, nullptr);
         group_lds_sync();
+    }
+}
+
+// Contacts whose tree query found more triangles than the lists hold (the reference's
+// b3MeshContact grows m_triangles without bound, mesh_contact.cpp:166-189), one per block
+// from the midphase's list: the query again into the block's slab (up to kWideTris
+// triangles), the per-triangle collides across the block's lanes (a clip polygon beyond the
+// lane's 32 vertices is redone by the first lane with the polygons in LDS), then the
+// clustering by the first 16 lanes with the observations in the slab.
+__global__ __launch_bounds__(64) void mesh_contact_wide_kernel(MeshArgs a) {
+    __shared__ ClipV polyBuf[kWidePolyCap], scratchBuf[kWidePolyCap];
+    __shared__ CPV keptBuf[kWidePolyCap];
+    __shared__ TriHull triLds[64];
+    __shared__ uint32_t stack[kLdsStack];
+    __shared__ uint32_t redo[64];
+    __shared__ uint32_t sh[2];  // triangles found, items to redo
+    const uint32_t lane = threadIdx.x;
+    char* const slab = static_cast<char*>(a.wideSlabs) + kWideSlabBytes * blockIdx.x;
+    float4* const obN = reinterpret_cast<float4*>(slab);
+    float4* const obP = obN + kWideObs;
+    Manifold* const mans = reinterpret_cast<Manifold*>(obP + kWideObs);
+    uint32_t* const list = reinterpret_cast<uint32_t*>(mans + kWideTris);
+    MeshTally tally;
+    const uint32_t count = a.totals[5];
+    for (uint32_t e = blockIdx.x; e < count; e += gridDim.x) {
+        const uint32_t c = a.wideContacts[e];
+        const ShapeRec s = a.shapes[a.shapeA[c]];
+        if (lane == 0u) {
+            uint32_t flags = 0;
+            sh[0] = tree_query(a, list, kWideTris, fat_box(a, c, s), flags, stack, 1u);
+            sh[1] = 0u;
+        }
+        __syncthreads();
+        const uint32_t found = sh[0];
+        const uint32_t mt = found < kWideTris ? found : kWideTris;
+        for (uint32_t k0 = 0; k0 < mt; k0 += 64u) {
+            const uint32_t k = k0 + lane;
+            if (k < mt) {
+                Contact m;
+                if (collide_tri<false>(a, c, s, list[k], nullptr, nullptr, nullptr, triLds[lane], m, tally))
+                    write_tri_manifold(mans + k, m, list[k]);
+                else
+                    redo[atomicAdd(&sh[1], 1u)] = k;
+            }
+            __syncthreads();
+            if (lane == 0u) {
+                for (uint32_t r = 0; r < sh[1]; ++r) {
+                    WideFaceWs ws{PolyView{0u, false, kWidePolyCap, polyBuf},
+                                  PolyView{0u, false, kWidePolyCap, scratchBuf}, CPolyView{0u, keptBuf}};
+                    Contact m;
+                    const uint32_t kr = redo[r];
+                    collide_tri<true>(a, c, s, list[kr], nullptr, nullptr, &ws, triLds[0], m, tally);
+                    write_tri_manifold(mans + kr, m, list[kr]);
+                }
+                sh[1] = 0u;
+            }
+            __syncthreads();
+        }
+        if (lane < kClusterGroup) {
+            // Every triangle within the slab: the contact's result is the reference's, unflagged.
+            const uint32_t keep = found <= kWideTris ? ~(kFlagWideContact | kFlagTriangleOverflow) : ~kFlagWideContact;
+            cluster_manifolds<kWideObs>(a, c, mans, mt, a.contactFlags[c] & keep, obN, obP, lane, 0u, nullptr,
+                                        nullptr);
+        }
+        __syncthreads();
+    }
+    if (a.counters) {  // all lanes of the block reach here
+        Counters* const ct = counter_slot(a.counters);
+        wave_count(&ct->convexCalls, tally.hullCalls);
+        wave_count(&ct->convexCacheHits, tally.hits);
+        wave_count(&ct->gjkCalls, tally.calls);
+        wave_count(&ct->gjkIters, tally.iters);
+        wave_max(&ct->gjkMaxIters, tally.maxIters);
     }
 }
 
@@ -963,6 +1082,12 @@ __global__ __launch_bounds__(256) void mesh_overlap_kernel(MeshArgs a) {
 hipError_t launch_mesh_overlap(const MeshArgs& a, hipStream_t s) {
     if (a.n == 0) return hipSuccess;
     mesh_overlap_kernel<<<(a.n + 255) / 256, 256, 0, s>>>(a);
+    return hipGetLastError();
+}
+
+hipError_t launch_mesh_contact_wide(const MeshArgs& a, hipStream_t s) {
+    if (a.n == 0 || !a.wideContacts) return hipSuccess;
+    mesh_contact_wide_kernel<<<kWideContactBlocks, 64, 0, s>>>(a);
     return hipGetLastError();
 }
 

@@ -550,9 +550,18 @@ int b3amd_build_static_tree(uint32_t vertexCount, const float* vertices, uint32_
  * child1, child2 or triangle; child1 = 0xFFFFFFFF for a leaf), nodesOut may be NULL. */
 int b3amd_mesh_tree(b3amd_context* ctx, uint32_t meshId, uint32_t* nodeCount, float* nodesOut);
 
+/* Triangle lists hold B3AMD_MESH_TRIANGLE_CAP entries per contact.  A fresh contact
+ * (b3amd_collide_mesh*) over more triangles is collided whole by a second pass (up to
+ * B3AMD_MESH_WIDE_TRIANGLES triangles, as the reference's growing m_triangles): its
+ * manifolds are the reference's and unflagged; its list returns the first
+ * B3AMD_MESH_TRIANGLE_CAP triangles and its count all of them.  B3AMD_FLAG_TRIANGLE_OVERFLOW
+ * marks the manifolds of a fresh contact beyond B3AMD_MESH_WIDE_TRIANGLES and of a
+ * persistent contact (b3amd_mesh_contacts_*) beyond B3AMD_MESH_TRIANGLE_CAP, whose lists and
+ * caches are cut there. */
 #define B3AMD_MESH_TRIANGLE_CAP 64u
+#define B3AMD_MESH_WIDE_TRIANGLES 2048u
 #define B3AMD_FLAG_INVALID_PAIR 2u
-#define B3AMD_FLAG_TRIANGLE_OVERFLOW 4u  /* more than B3AMD_MESH_TRIANGLE_CAP triangles overlapped */
+#define B3AMD_FLAG_TRIANGLE_OVERFLOW 4u
 
 /* n freshly created mesh contacts between convex shapes[shapeA[i]] (body transform
  * xfs[2i]) and mesh meshId (body transform xfs[2i+1]): what b3MeshContact does from

@@ -52,7 +52,7 @@ void b3o_flop_reset(void) {}
 #define NULL_EDGE 0xFFFFFFFFu       /* clip.h:25 */
 #define NULL_TRIANGLE 0xFFFFFFFFu   /* manifold.h:25 */
 #define MAX_POINTS 4                /* settings.h:81 */
-#define POLY_CAP 256                /* b3StackArray<...,32> grows on the heap; 256 bounds it here */
+#define POLY_CAP 65536              /* b3StackArray<...,32> grows on the heap; 65536 bounds it here */
 
 typedef b3o_vec3 V3;
 typedef b3o_xf XF;
@@ -438,8 +438,8 @@ static void sort_polygon(CPoly* out, const CPoly* in, V3 N) {                   
 static void reduce_polygon(CPoly* out, const CPoly* in, V3 N, uint32_t initial) {      /* contact_cluster.cpp:58-200 */
     out->count = 0;
     if (in->count <= MAX_POINTS) { sort_polygon(out, in, N); return; }
-    char chosen[POLY_CAP];
-    memset(chosen, 0, sizeof(chosen));
+    static char chosen[POLY_CAP];
+    memset(chosen, 0, in->count);
     out->v[out->count++] = in->v[initial];
     chosen[initial] = 1;
     {
@@ -1776,7 +1776,7 @@ static WPoint world_point(const b3o_mpoint* p, float rA, const XF* xfA, float rB
 }
 
 /* ---------------------------------------------------------------- contact_cluster.cpp:202-590 */
-#define OBS_CAP 1024
+#define OBS_CAP 65536
 typedef struct { V3 point; uint32_t cluster, manifold, manifoldPoint; } Obs;
 typedef struct { Obs obs[OBS_CAP]; uint32_t nObs; V3 clusters[OBS_CAP]; uint32_t nClusters; } Cluster;
 

@@ -81,6 +81,30 @@ def test_gpu_mesh_wide_polygons_equal_oracle(ctx, oracle):
     assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "wide prisms on a mesh")
 
 
+def test_gpu_mesh_long_triangle_lists_equal_oracle(ctx, oracle):
+    """Fresh contacts over 98-288 triangles (beyond the device lists' 64): the wide-contact
+    pass queries them again into its slab, collides every triangle and clusters hundreds of
+    points; manifolds equal the oracle's bit for bit, without the overflow flag, and the
+    lists' first 64 entries and the counts equal the oracle's too.  Ordinary contacts in the
+    same batch take the list path."""
+    from util import plate_workload, wide_plate_workload
+    print("case: long triangle lists (large plates on a fine grid)", flush=True)
+    pool, shapes, v, t, xfs = wide_plate_workload(8, 30)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    nodes = ctx.mesh_tree(mid)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    counts, lists, out, mcount = ctx.collide_mesh(mid, sa, xfs)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs, cap=512)
+    assert (co[0] > ctx.MESH_TRIANGLE_CAP).all()
+    np.testing.assert_array_equal(counts, co[0])
+    np.testing.assert_array_equal(lists, co[1][:, :ctx.MESH_TRIANGLE_CAP])
+    np.testing.assert_array_equal(mcount, co[3])
+    assert (out["flags"] == 0).all()
+    assert_manifolds_equal(out.reshape(-1), co[2].reshape(-1), "long triangle lists")
+
+
 def test_gpu_mesh_heightfield_full_size(ctx, lib, oracle):
     """The mixed-shapes-on-terrain configuration at full size: a 256 x 256 heightfield
     (131,072 triangles) and 100,000 spheres, capsules and hulls.  All contacts run on the

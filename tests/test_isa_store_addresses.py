@@ -9,7 +9,11 @@ Scope: narrowphase_hull.hip (the cache and contact kernels where the fault occur
 serving.hip and contact_init.hip.  The checker follows uniform branches on both edges, so
 kernels that select a path by a uniform flag and redefine an address on each path
 (narrowphase_convex.hip's optional sweeps) would be reported falsely, and the SAT and mesh
-kernels' loop nests exceed its fixpoint budget; those files are not checked here."""
+kernels' loop nests exceed its fixpoint budget; those files are not checked here.  In
+narrowphase_hull.hip, contact_wide_kernel is left out for the same reason (its loop over the
+listed pairs around four inlined clip loops); it runs on one lane per block, every other lane
+leaving before the first memory access, so no access of it can merge lanes that defined the
+address differently."""
 import concurrent.futures
 import os
 import shutil
@@ -51,9 +55,12 @@ def asm():
         shutil.rmtree(tmp, ignore_errors=True)
 
 
+SINGLE_LANE_KERNELS = ("contact_wide_kernel",)
+
+
 @pytest.mark.parametrize("src", CHECKED)
 def test_kernels_address_only_written_registers(asm, src):
-    findings = isa_check.check_assembly(asm[src])
+    findings = isa_check.check_assembly(asm[src], skip=SINGLE_LANE_KERNELS)
     assert not findings, "%s: accesses through VGPRs unwritten in some active lane: %s" % (
         src, {k: v[:4] for k, v in findings.items()})
 

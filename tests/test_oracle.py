@@ -301,6 +301,24 @@ def test_oracle_wide_polygons_equal_reference(oracle, reference):
     assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "wide prisms on a mesh")
 
 
+def test_oracle_mesh_long_triangle_lists_equal_reference(oracle, reference):
+    """Contacts over 98-288 triangles with clusters of hundreds of points (large flat boxes on
+    a fine flat grid): the reference's triangle list and cluster polygon grow without bound;
+    the restatement keeps them whole."""
+    import oracle_lib
+    from util import wide_plate_workload
+    pool, shapes, v, t, xfs = wide_plate_workload(8, 30)
+    nodes = oracle_lib.build_tree(reference.lib, "b3ref_", v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    co = oracle_lib.collide_mesh(oracle.lib, "b3o_", pool, shapes, v, t, nodes, sa, xfs, cap=512)
+    cr = oracle_lib.collide_mesh(reference.lib, "b3ref_", pool, shapes, v, t, nodes, sa, xfs, cap=512)
+    np.testing.assert_array_equal(co[0], cr[0])
+    np.testing.assert_array_equal(co[1], cr[1])
+    np.testing.assert_array_equal(co[3], cr[3])
+    assert (cr[0] > 64).all()
+    assert_manifolds_equal(co[2].reshape(-1), cr[2].reshape(-1), "long triangle lists")
+
+
 def test_oracle_mesh_golden(oracle):
     import oracle_lib
     g = load_golden("mesh_contacts.npz")

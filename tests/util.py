@@ -364,18 +364,19 @@ def mesh_workload(lib, seed, H, W, n_shapes, gap_sigma=0.05):
     return pool, shapes, verts, tris, xfs
 
 
-def plate_workload(seed, n_shapes, H=24, W=24, cell=0.3):
+def plate_workload(seed, n_shapes, H=24, W=24, cell=0.3, extents=None):
     """Flat boxes (half-extents 0.18-0.25 m in x and z, 0.1 m high) lying on a flat grid of
     0.3 m cells, turned about the vertical and tilted by at most 0.01 rad, sunk 1-10 mm
     into the plane: every triangle under a box gives a face contact of up to four points,
     all with one normal, so a contact's single cluster holds more than 32 points while its
-    triangle list (the fattened AABB's footprint) stays within 64.
+    triangle list (the fattened AABB's footprint) stays within 64.  `extents` replaces the
+    four (x, z) half-extents.
     Returns (pool, shapes, vertices, triangles, xfs [n][2][12] (box, mesh))."""
     from bounce_amd.shapes import box_hull, identity_xf
     from bounce_amd.types import HULL, SHAPE_DTYPE, HullPool
     rng = np.random.default_rng(seed)
     verts, tris = grid_mesh(H, W, cell=cell, amplitude=0.0, seed=seed)
-    ext = [(0.25, 0.2), (0.2, 0.2), (0.25, 0.25), (0.22, 0.18)]
+    ext = extents or [(0.25, 0.2), (0.2, 0.2), (0.25, 0.25), (0.22, 0.18)]
     pool = HullPool.concat([box_hull(ex, 0.1, ez) for ex, ez in ext])
     shapes = np.zeros(n_shapes, SHAPE_DTYPE)
     shapes["type"] = HULL
@@ -998,3 +999,10 @@ def wide_prism_mesh(seed, n_shapes):
         xfs[i, 0] = identity_xf((x, 0.1 - rng.uniform(0.001, 0.01), z), R)
         xfs[i, 1] = identity_xf((0, 0, 0))
     return pool, shapes, verts, tris, xfs
+
+
+def wide_plate_workload(seed, n_shapes):
+    """Flat boxes of 1-1.8 m on a flat grid of 0.25 m cells: 70-200 triangles under each fat
+    AABB (beyond the device lists' 64) and single clusters of 150-400 points."""
+    return plate_workload(seed, n_shapes, H=32, W=32, cell=0.25,
+                          extents=[(0.5, 0.5), (0.9, 0.6), (0.7, 0.7), (0.8, 0.9)])
