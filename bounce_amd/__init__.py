@@ -73,6 +73,9 @@ def load_library():
         "b3amd_mesh_contacts_collide": (i32, [vp, vp, vp, vp, vp]),
         "b3amd_mesh_contacts_synchronize": (i32, [vp, vp, vp]),
         "b3amd_mesh_contacts_read": (i32, [vp, vp, vp, vp, vp]),
+        "b3amd_mesh_contacts_collide_device": (i32, [vp, vp, vp, vp, vp]),
+        "b3amd_mesh_contacts_test_overlap_device": (i32, [vp, vp, vp]),
+        "b3amd_mesh_contacts_synchronize_device": (i32, [vp, vp, vp]),
         "b3amd_set_sat_tuning": (i32, [vp, u32, u32]),
         "b3amd_get_sat_tuning": (i32, [vp, ctypes.POINTER(u32), ctypes.POINTER(u32)]),
         "b3amd_autotune": (i32, [vp, u32, vp, vp, vp, u32, ctypes.POINTER(ctypes.c_float)]),
@@ -613,6 +616,17 @@ class MeshContacts:
         xfs = np.ascontiguousarray(xfs, np.float32).reshape(self.n, 2, 12)
         c0 = np.ascontiguousarray(centers0, np.float32).reshape(self.n, 3)
         _check(self.lib.b3amd_mesh_contacts_synchronize(self.handle, ptr(xfs), ptr(c0)))
+
+    def collide_device(self, d_xfs, d_out, d_counts, d_sweeps=None):
+        """b3amd_mesh_contacts_collide_device on resident buffers (enqueued, not synchronised)."""
+        _check(self.lib.b3amd_mesh_contacts_collide_device(self.handle, d_xfs.ptr, None if d_sweeps is None else
+                                                          d_sweeps.ptr, d_out.ptr, d_counts.ptr))
+
+    def test_overlap_device(self, d_xfs, d_overlap):
+        _check(self.lib.b3amd_mesh_contacts_test_overlap_device(self.handle, d_xfs.ptr, d_overlap.ptr))
+
+    def synchronize_device(self, d_xfs, d_centers0):
+        _check(self.lib.b3amd_mesh_contacts_synchronize_device(self.handle, d_xfs.ptr, d_centers0.ptr))
 
     def read(self):
         """(triangle counts, lists [n][64], caches [n][64], fat AABBs [n][6])."""

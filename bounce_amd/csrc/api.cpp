@@ -1240,6 +1240,48 @@ int b3amd_mesh_contacts_destroy(b3amd_mesh_contacts* set) {
     return B3AMD_OK;
 }
 
+namespace {
+
+// b3MeshContact::Collide of every contact of the set on device-resident transforms (and
+// sweeps, or null), manifolds and counts into device buffers; enqueued, not synchronised.
+int enqueue_mesh_contacts_collide(b3amd_mesh_contacts* set, const float* d_xfs, const float* d_sweeps, Manifold* d_out,
+                                  uint32_t* d_counts, const MeshScratch& sc) {
+    b3amd_context* ctx = set->ctx;
+    hipStream_t s = ctx->stream;
+    MeshArgs a = mesh_set_args(set, sc);
+    a.xfs = d_xfs;
+    a.sweeps = d_sweeps;
+    a.out = d_out;
+    a.manifoldCounts = d_counts;
+    int rc;
+    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 8, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_list", [&] { return launch_mesh_list(a, s); }))) return rc;
+    if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
+    return timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); });
+}
+
+int enqueue_mesh_contacts_overlap(b3amd_mesh_contacts* set, const float* d_xfs, uint32_t* d_overlap,
+                                  const MeshScratch& sc) {
+    b3amd_context* ctx = set->ctx;
+    hipStream_t s = ctx->stream;
+    MeshArgs a = mesh_set_args(set, sc);
+    a.xfs = d_xfs;
+    a.overlap = d_overlap;
+    return timed(ctx, s, "mesh_overlap", [&] { return launch_mesh_overlap(a, s); });
+}
+
+int enqueue_mesh_contacts_sync(b3amd_mesh_contacts* set, const float* d_xfs, const float* d_centers0,
+                               const MeshScratch& sc) {
+    b3amd_context* ctx = set->ctx;
+    hipStream_t s = ctx->stream;
+    MeshArgs a = mesh_set_args(set, sc);
+    a.xfs = d_xfs;
+    a.centers0 = d_centers0;
+    return timed(ctx, s, "mesh_sync", [&] { return launch_mesh_sync(a, kMeshSync, s); });
+}
+
+}  // namespace
+
 int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* set, const b3amd_transform* xfs, const b3amd_quat* sweeps,
                                 b3amd_manifold* out, uint32_t* manifoldCounts) {
     if (!set || (set->n && (!xfs || !out || !manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
@@ -1252,17 +1294,25 @@ int b3amd_mesh_contacts_collide(b3amd_mesh_contacts* set, const b3amd_transform*
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
     if (sweeps) HIP_TRY(hipMemcpyAsync(sc.sweeps, sweeps, 64ull * n, hipMemcpyHostToDevice, s));
-    MeshArgs a = mesh_set_args(set, sc);
-    a.sweeps = sweeps ? sc.sweeps : nullptr;
-    if ((rc = timed(ctx, s, "zero", [&] { return launch_zero_words(a.totals, 8, s); }))) return rc;
-    if ((rc = timed(ctx, s, "mesh_list", [&] { return launch_mesh_list(a, s); }))) return rc;
-    if ((rc = timed(ctx, s, "mesh_narrow", [&] { return launch_mesh_narrow(a, s); }))) return rc;
-    if ((rc = timed(ctx, s, "mesh_cluster", [&] { return launch_mesh_cluster(a, s); }))) return rc;
+    if ((rc = enqueue_mesh_contacts_collide(set, sc.xfs, sweeps ? sc.sweeps : nullptr, sc.out, sc.manifoldCounts, sc)))
+        return rc;
     hipError_t e = hipMemcpyAsync(out, sc.out, 720ull * n, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipMemcpyAsync(manifoldCounts, sc.manifoldCounts, 4ull * n, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return fail(B3AMD_ERR_DEVICE, "mesh readback: %s", hipGetErrorString(e));
     return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_collide_device(b3amd_mesh_contacts* set, const b3amd_transform* d_xfs,
+                                       const b3amd_quat* d_sweeps, b3amd_manifold* d_out, uint32_t* d_manifoldCounts) {
+    if (!set || (set->n && (!d_xfs || !d_out || !d_manifoldCounts))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (set->n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(set->ctx, set->n, &sc))) return rc;
+    return enqueue_mesh_contacts_collide(set, reinterpret_cast<const float*>(d_xfs),
+                                         reinterpret_cast<const float*>(d_sweeps), reinterpret_cast<Manifold*>(d_out),
+                                         d_manifoldCounts, sc);
 }
 
 int b3amd_mesh_contacts_test_overlap(b3amd_mesh_contacts* set, const b3amd_transform* xfs, uint32_t* overlap) {
@@ -1275,12 +1325,21 @@ int b3amd_mesh_contacts_test_overlap(b3amd_mesh_contacts* set, const b3amd_trans
     if ((rc = mesh_scratch(ctx, n, &sc))) return rc;
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
-    MeshArgs a = mesh_set_args(set, sc);
-    a.overlap = sc.manifoldCounts;  // [n] words of the scratch, free outside Collide
-    if ((rc = timed(ctx, s, "mesh_overlap", [&] { return launch_mesh_overlap(a, s); }))) return rc;
-    HIP_TRY(hipMemcpyAsync(overlap, a.overlap, 4ull * n, hipMemcpyDeviceToHost, s));
+    // The result goes to [n] words of the scratch that only Collide uses.
+    if ((rc = enqueue_mesh_contacts_overlap(set, sc.xfs, sc.manifoldCounts, sc))) return rc;
+    HIP_TRY(hipMemcpyAsync(overlap, sc.manifoldCounts, 4ull * n, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_test_overlap_device(b3amd_mesh_contacts* set, const b3amd_transform* d_xfs,
+                                            uint32_t* d_overlap) {
+    if (!set || (set->n && (!d_xfs || !d_overlap))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (set->n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(set->ctx, set->n, &sc))) return rc;
+    return enqueue_mesh_contacts_overlap(set, reinterpret_cast<const float*>(d_xfs), d_overlap, sc);
 }
 
 int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* set, const b3amd_transform* xfs, const float* centers0) {
@@ -1294,11 +1353,19 @@ int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* set, const b3amd_transf
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(sc.xfs, xfs, 96ull * n, hipMemcpyHostToDevice, s));
     HIP_TRY(hipMemcpyAsync(sc.centers, centers0, 12ull * n, hipMemcpyHostToDevice, s));
-    MeshArgs a = mesh_set_args(set, sc);
-    a.centers0 = sc.centers;
-    if ((rc = timed(ctx, s, "mesh_sync", [&] { return launch_mesh_sync(a, kMeshSync, s); }))) return rc;
+    if ((rc = enqueue_mesh_contacts_sync(set, sc.xfs, sc.centers, sc))) return rc;
     HIP_TRY(hipStreamSynchronize(s));
     return ctx->resolve_timings();
+}
+
+int b3amd_mesh_contacts_synchronize_device(b3amd_mesh_contacts* set, const b3amd_transform* d_xfs,
+                                           const float* d_centers0) {
+    if (!set || (set->n && (!d_xfs || !d_centers0))) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    if (set->n == 0) return B3AMD_OK;
+    int rc;
+    MeshScratch sc;
+    if ((rc = mesh_scratch(set->ctx, set->n, &sc))) return rc;
+    return enqueue_mesh_contacts_sync(set, reinterpret_cast<const float*>(d_xfs), d_centers0, sc);
 }
 
 int b3amd_mesh_contacts_read(b3amd_mesh_contacts* set, uint32_t* triangleCounts, uint32_t* triangleLists,

@@ -614,6 +614,16 @@ int b3amd_mesh_contacts_test_overlap(b3amd_mesh_contacts* contacts, const b3amd_
 int b3amd_mesh_contacts_synchronize(b3amd_mesh_contacts* contacts, const b3amd_transform* xfs, const float* centers0);
 int b3amd_mesh_contacts_read(b3amd_mesh_contacts* contacts, uint32_t* triangleCounts, uint32_t* triangleLists,
                              b3amd_convex_cache* caches, float* fatAabbs);
+/* The same three steps on device-resident buffers (b3amd_device_alloc), enqueued on the
+ * context stream without synchronizing, for a world whose body state stays on the device:
+ * collide (mesh_contact.cpp:217-275; d_sweeps may be NULL), test_overlap (:191-215) and
+ * synchronize (:62-160), with the layouts of the host forms above. */
+int b3amd_mesh_contacts_collide_device(b3amd_mesh_contacts* contacts, const b3amd_transform* d_xfs,
+                                       const b3amd_quat* d_sweeps, b3amd_manifold* d_out, uint32_t* d_manifoldCounts);
+int b3amd_mesh_contacts_test_overlap_device(b3amd_mesh_contacts* contacts, const b3amd_transform* d_xfs,
+                                            uint32_t* d_overlap);
+int b3amd_mesh_contacts_synchronize_device(b3amd_mesh_contacts* contacts, const b3amd_transform* d_xfs,
+                                           const float* d_centers0);
 
 /* ---- device memory helpers (callers need no other GPU runtime) ---- */
 int b3amd_device_alloc(b3amd_context* ctx, uint64_t bytes, void** ptr);

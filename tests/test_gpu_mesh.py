@@ -224,6 +224,61 @@ def test_gpu_mesh_contacts_persistent_equal_reference(ctx, lib, reference):
     ref.close()
 
 
+def test_gpu_mesh_contacts_device_entry_points_equal_host_forms(ctx, lib):
+    """Two sets of the same persistent mesh contacts stepped eight times side by side, one
+    through the host-array entry points, one through collide/test_overlap/synchronize
+    _device on resident buffers: manifolds, counts, overlap flags and the kept state (lists,
+    caches, fat boxes) are identical after every step."""
+    import bounce_amd
+    from bounce_amd.types import MANIFOLD_DTYPE
+    from util import moving_mesh_steps
+    print("case: persistent mesh contacts, device entry points", flush=True)
+    pool, shapes, v, t, xfs = mesh_workload(lib, 93, 48, 48, 300)
+    ctx.upload_hulls(pool)
+    ctx.upload_shapes(shapes)
+    mid = ctx.upload_mesh(v, t)
+    sa = np.arange(len(shapes), dtype=np.uint32)
+    n = len(sa)
+    steps = moving_mesh_steps(xfs, 9, 5)
+    dx, ds, dc0 = ctx.alloc(96 * n), ctx.alloc(64 * n), ctx.alloc(12 * n)
+    do, dm, dov = ctx.alloc(720 * n), ctx.alloc(4 * n), ctx.alloc(4 * n)
+    try:
+        with bounce_amd.MeshContacts(ctx, mid, sa, steps[0][0]) as host, \
+                bounce_amd.MeshContacts(ctx, mid, sa, steps[0][0]) as dev:
+            for s in range(8):
+                x, sw, p = steps[s]
+                want, wc = host.collide(x, sw)
+                dx.upload(np.ascontiguousarray(x, np.float32))
+                ds.upload(np.ascontiguousarray(sw, np.float32))
+                dev.collide_device(dx, do, dm, ds)
+                ctx.synchronize()
+                got = do.download(np.zeros((n, 3), MANIFOLD_DTYPE))
+                gc = dm.download(np.zeros(n, np.uint32))
+                np.testing.assert_array_equal(gc, wc, "step %d counts" % s)
+                assert_manifolds_equal(got.reshape(-1), want.reshape(-1), "step %d manifolds" % s)
+                ov_want = host.test_overlap(x)
+                dev.test_overlap_device(dx, dov)
+                ctx.synchronize()
+                np.testing.assert_array_equal(dov.download(np.zeros(n, np.uint32)), ov_want, "step %d overlap" % s)
+                host.synchronize(steps[s + 1][0], p)
+                dx.upload(np.ascontiguousarray(steps[s + 1][0], np.float32))
+                dc0.upload(np.ascontiguousarray(p, np.float32))
+                dev.synchronize_device(dx, dc0)
+                ctx.synchronize()
+                hc, hl, hca, hf = host.read()
+                dc, dl, dca, df = dev.read()
+                np.testing.assert_array_equal(hc, dc, "step %d counts" % s)
+                np.testing.assert_array_equal(hf.view(np.uint32), df.view(np.uint32), "step %d fat boxes" % s)
+                for i in range(n):  # entries past a contact's count are not part of its state
+                    k = min(int(hc[i]), ctx.MESH_TRIANGLE_CAP)
+                    np.testing.assert_array_equal(hl[i, :k], dl[i, :k], "step %d lists" % s)
+                    np.testing.assert_array_equal(hca[i, :k].view(np.uint8), dca[i, :k].view(np.uint8),
+                                                  "step %d caches" % s)
+    finally:
+        for b in (dx, ds, dc0, do, dm, dov):
+            b.free()
+
+
 def test_gpu_mesh_contacts_persistent_golden(ctx):
     """The persistent mesh-contact fixture written by the reference's own b3MeshContact,
     replayed through b3amd_mesh_contacts_*: six steps, bit-exact."""
