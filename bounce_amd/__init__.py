@@ -15,7 +15,10 @@ from .types import (CACHE_DTYPE, MANIFOLD_DTYPE, SHAPE_DTYPE, HullPool, empty_ca
 __all__ = ["Context", "Graph", "Server", "HullPool", "load_library", "empty_caches", "BounceError",
            "MANIFOLD_DTYPE", "CACHE_DTYPE", "SHAPE_DTYPE", "SPHERE", "CAPSULE", "HULL", "MESH"]
 
-LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbounce_amd.so")
+# The in-tree build; B3AMD_LIBRARY names another build of the same library (a variant built
+# for comparison) to load instead.
+LIB_PATH = os.environ.get("B3AMD_LIBRARY") or os.path.join(os.path.dirname(os.path.abspath(__file__)),
+                                                           "libbounce_amd.so")
 _lib = None
 
 OK, ERR_ARGUMENT, ERR_DEVICE, ERR_BUDGET, ERR_UNSUPPORTED, ERR_NO_DEVICE, ERR_OVERFLOW = 0, -1, -2, -3, -4, -5, -6
