@@ -312,6 +312,8 @@ def mesh_run(world, steps, cpu):
 # ----------------------------------------------------------------- sphere/capsule pairs
 
 CONVEX_PAIRS = 1 << 20
+SERVING_WARMUP = 0.1  # seconds of arrivals before each measured Poisson run (see serving_run)
+SERVING_RUNS = 3      # measured Poisson runs per rate; the run of median p99 is reported
 
 
 def convex_run(world, steps, cpu):
@@ -913,7 +915,15 @@ def serving_run(lib, worlds_n, rate, duration, seed=4096):
     request.  Measures the flood capacity (two warm steps of every world submitted at once)
     with one lane and with three overlapping lanes, the latency of Poisson arrivals at
     `rate` steps/s and at half the capacity for `duration` seconds each, and beside them the
-    plain drop-in: one b3amd_collide call on host arrays per world step."""
+    plain drop-in: one b3amd_collide call on host arrays per world step.  The measured
+    Poisson runs follow one unmeasured run of 3 x SERVING_WARMUP seconds at `rate` (the first
+    run after the floods pays one-time costs at its start: its request records' host memory
+    is touched for the first time), and each opens with SERVING_WARMUP seconds of arrivals at
+    the same rate whose latencies are not counted; the warm-up run's and the prefixes' p99 and
+    max are reported beside the steady-state figures.  Each rate runs SERVING_RUNS times: a
+    stall of several milliseconds now and then (the box's GPU or host, not any one batch's
+    work) lifts one run's p99 above the others, so the run of median p99 is the figure and all
+    runs' p50/p99/max are listed."""
     import bounce_amd
     from bounce_amd.types import empty_caches
     from bounce_amd.workloads import gen_serving_worlds, world_pair_inputs
@@ -947,29 +957,49 @@ def serving_run(lib, worlds_n, rate, duration, seed=4096):
                 log("serving capacity, %d lane(s): %.0f steps/s" % (lanes, capacity))
             out["capacity_steps_per_s"] = capacity
             out["capacity_contacts_per_s"] = capacity * total_pairs / worlds_n
+            # One unmeasured Poisson run first: the first run after the floods pays one-time
+            # costs of its request records (host memory first touched), milliseconds that land
+            # at its start.
+            nwu = int(rate * 3 * SERVING_WARMUP)
+            twu = np.cumsum(rng.exponential(1.0 / rate, nwu))
+            wwu, swu = rng.integers(0, worlds_n, nwu), rng.integers(1, 4, nwu)
+            lat_wu = srv.replay(ids[wwu], twu, [worlds[w]["xfs"][s] for w, s in zip(wwu, swu)])
+            out["warmup_run"] = {"rate_steps_per_s": rate, "requests": nwu,
+                                 "p99_ms": float(np.percentile(lat_wu, 99)), "max_ms": float(lat_wu.max())}
             for name, r in (("at_rate", rate), ("at_half_capacity", 0.5 * capacity)):
-                n = int(r * duration)
-                times = np.cumsum(rng.exponential(1.0 / r, n))
-                wsel = rng.integers(0, worlds_n, n)
-                ssel = rng.integers(1, 4, n)
-                before = srv.stats()
-                lat = srv.replay(ids[wsel], times, [worlds[w]["xfs"][s] for w, s in zip(wsel, ssel)])
-                after = srv.stats()
-                batches = after["batches"] - before["batches"]
-                out[name] = {
-                    "rate_steps_per_s": r,
-                    "requests": n,
-                    "p50_ms": float(np.percentile(lat, 50)),
-                    "p99_ms": float(np.percentile(lat, 99)),
-                    "max_ms": float(lat.max()),
-                    "achieved_steps_per_s": n / float((times + lat / 1000.0).max()),
-                    "batches": batches,
-                    "mean_batch_contacts": (after["pairs"] - before["pairs"]) / max(1, batches),
-                    "device_ms_per_batch": (after["gpu_ms"] - before["gpu_ms"]) / max(1, batches),
-                    "pack_ms_per_batch": (after["pack_ms"] - before["pack_ms"]) / max(1, batches),
-                }
-                log("serving %s: %.0f steps/s, p50 %.3f ms, p99 %.3f ms, %d batches"
-                    % (name, r, out[name]["p50_ms"], out[name]["p99_ms"], batches))
+                runs = []
+                for _ in range(SERVING_RUNS):
+                    nw = int(r * SERVING_WARMUP)
+                    n = int(r * duration)
+                    times = np.cumsum(rng.exponential(1.0 / r, nw + n))
+                    wsel = rng.integers(0, worlds_n, nw + n)
+                    ssel = rng.integers(1, 4, nw + n)
+                    before = srv.stats()
+                    lat_all = srv.replay(ids[wsel], times, [worlds[w]["xfs"][s] for w, s in zip(wsel, ssel)])
+                    after = srv.stats()
+                    batches = after["batches"] - before["batches"]
+                    lat, t_run = lat_all[nw:], times[nw:] - times[nw]
+                    runs.append({
+                        "requests": n,
+                        "p50_ms": float(np.percentile(lat, 50)),
+                        "p99_ms": float(np.percentile(lat, 99)),
+                        "max_ms": float(lat.max()),
+                        "prefix": {"seconds": SERVING_WARMUP, "requests": nw,
+                                   "p99_ms": float(np.percentile(lat_all[:nw], 99)) if nw else None,
+                                   "max_ms": float(lat_all[:nw].max()) if nw else None},
+                        "achieved_steps_per_s": n / float((t_run + lat / 1000.0).max()),
+                        "batches": batches,
+                        "mean_batch_contacts": (after["pairs"] - before["pairs"]) / max(1, batches),
+                        "device_ms_per_batch": (after["gpu_ms"] - before["gpu_ms"]) / max(1, batches),
+                        "pack_ms_per_batch": (after["pack_ms"] - before["pack_ms"]) / max(1, batches),
+                    })
+                # The run of median p99 stands for the rate; every run is listed.
+                med = sorted(runs, key=lambda x: x["p99_ms"])[len(runs) // 2]
+                out[name] = dict(med, rate_steps_per_s=r, runs=[{k: x[k] for k in ("p50_ms", "p99_ms", "max_ms")}
+                                                                 for x in runs])
+                log("serving %s: %.0f steps/s, p50 %.3f ms, p99 %.3f ms (runs' p99: %s), %d batches"
+                    % (name, r, out[name]["p50_ms"], out[name]["p99_ms"],
+                       ", ".join("%.3f" % x["p99_ms"] for x in runs), med["batches"]))
         # Plain drop-in: one synchronous b3amd_collide per world step on host arrays, caches
         # carried per world, for 256 sampled worlds (warm steps 1-3).
         pick = rng.choice(worlds_n, 256, replace=False)

@@ -154,6 +154,11 @@ struct Lane {
     float* hBodyXfs = nullptr;
     float* hBodySweeps = nullptr;
     ReqDesc* hDescs = nullptr;
+    // Recorded around each batch the lane runs: created once with the lane (event creation
+    // and destruction per batch cost milliseconds now and then in the runtime).  A lane runs
+    // one batch at a time, and a later batch's wait on `done` is enqueued while the batch that
+    // recorded it is still in flight.
+    hipEvent_t start = nullptr, done = nullptr;
     bool busy = false;
 };
 
@@ -389,8 +394,6 @@ void retire(b3amd_server* s, std::deque<Batch>::iterator it) {
     }
     float ms = 0.0f;
     if (hipEventElapsedTime(&ms, b.start, b.done) == hipSuccess) s->stats.gpu_ms += ms;
-    hipEventDestroy(b.start);
-    hipEventDestroy(b.done);
     s->lanes[b.lane].busy = false;
     s->inflight.erase(it);
 }
@@ -417,17 +420,9 @@ void worker_main(b3amd_server* s) {
             Batch b;
             while (s->lanes[b.lane].busy) ++b.lane;  // a free lane exists: inflight < maxInflight <= lanes
             b.seq = s->nextBatch++;
-            if (hipEventCreateWithFlags(&b.start, 0) != hipSuccess ||
-                hipEventCreateWithFlags(&b.done, 0) != hipSuccess) {
-                s->workerError = B3AMD_ERR_DEVICE;
-                s->workerMessage = "hipEventCreate failed";
-                continue;
-            }
-            if (select_batch(s, b) == 0) {
-                hipEventDestroy(b.start);
-                hipEventDestroy(b.done);
-                continue;
-            }
+            b.start = s->lanes[b.lane].start;
+            b.done = s->lanes[b.lane].done;
+            if (select_batch(s, b) == 0) continue;
             s->lanes[b.lane].busy = true;
             s->inflight.push_back(b);
             Batch& queued = s->inflight.back();  // deque: stays valid while others are pushed or erased
@@ -459,7 +454,8 @@ int alloc_lane(b3amd_server* s, Lane& L) {
     if (hipStreamCreateWithFlags(&L.streams.main, hipStreamNonBlocking) != hipSuccess ||
         hipStreamCreateWithFlags(&L.streams.side, hipStreamNonBlocking) != hipSuccess ||
         hipEventCreateWithFlags(&L.streams.fork, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&L.streams.join, hipEventDisableTiming) != hipSuccess)
+        hipEventCreateWithFlags(&L.streams.join, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&L.start, 0) != hipSuccess || hipEventCreateWithFlags(&L.done, 0) != hipSuccess)
         return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
     int rc = B3AMD_OK;
     auto A = [&](uint64_t bytes, void** p) {
@@ -495,6 +491,8 @@ void free_lane(b3amd_server* s, Lane& L) {
     if (L.hDescs) hipHostFree(L.hDescs);
     if (L.streams.fork) hipEventDestroy(L.streams.fork);
     if (L.streams.join) hipEventDestroy(L.streams.join);
+    if (L.start) hipEventDestroy(L.start);
+    if (L.done) hipEventDestroy(L.done);
     if (L.streams.main) hipStreamDestroy(L.streams.main);
     if (L.streams.side) hipStreamDestroy(L.streams.side);
     L = Lane();
@@ -754,6 +752,13 @@ int b3amd_server_replay(b3amd_server* s, uint32_t count, const uint32_t* worlds,
     if (!s || !worlds || !times_s || !body_xfs || !latency_ms) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     std::vector<uint64_t> tickets(count);
     std::vector<Clock::time_point> due(count);
+    {
+        // Every request stays in the ticket map until the waits at the end: room for all of
+        // them now, so that no insertion rehashes the map under the lock (milliseconds with
+        // a few hundred thousand entries, while the worker cannot take the lock to launch).
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->requests.reserve(s->requests.size() + count);
+    }
     const Clock::time_point t0 = Clock::now();
     for (uint32_t i = 0; i < count; ++i) {
         due[i] = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(times_s[i]));
