@@ -383,6 +383,7 @@ def convex_run(world, steps, cpu):
 
 _WARM_JOB = {}
 WARM_CPU_STEPS = 8
+WARM_PROFILE_STEPS = 60
 
 
 def _warm_cpu_worker(chunk):
@@ -511,16 +512,21 @@ def warm_world_run(world, steps, cpu):
         step(0)
         ctx.synchronize()
         t1 = time.perf_counter()
-        ctx.reset_kernel_stats()
-        ctx.set_profiling(True)
         t2 = time.perf_counter()
         for s in range(1, steps):
             step(s)
         ctx.synchronize()
         t3 = time.perf_counter()
+        last = fp64_error(steps - 1)
+        # Per-kernel times from further warm steps with events around each launch (kept out
+        # of the timed steps: the events add host work and gaps between launches).
+        ctx.reset_kernel_stats()
+        ctx.set_profiling(True)
+        for s in range(steps, steps + WARM_PROFILE_STEPS):
+            step(s)
+        ctx.synchronize()
         ctx.set_profiling(False)
         kern = {k: ctx.kernel_stats(k) for k in ("gather", "cache_hull", "sat_hull", "contact_hull")}
-        last = fp64_error(steps - 1)
     warm_s = t3 - t2
     calls, hits = warm["convex_calls"], warm["convex_cache_hits"]
     res.update({
@@ -534,7 +540,8 @@ def warm_world_run(world, steps, cpu):
         "touching_fraction_last_step": float((last["pointCount"] > 0).mean()),
         "points_last_step": int(last["pointCount"].sum()),
         "what": "b3amd_update_contacts per step on device-resident body transforms and sweeps; caches and "
-                "manifolds stay in HBM; statistics counters off in the timed steps"})
+                "manifolds stay in HBM; statistics counters and kernel events off in the timed steps "
+                "(kernels_ms from %d further steps with events)" % WARM_PROFILE_STEPS})
     outs = np.concatenate([o for o, _ in errs])
     xfs = np.concatenate([x for _, x in errs])
     res["separation_error_fp64"] = separation_error_fp64(outs, xfs)
@@ -626,6 +633,9 @@ def rain_cpu_baseline(world, steps):
                                                                calls, wall)}, **host)
 
 
+RAIN_PROFILE_EVERY = 10
+
+
 def rain_world_run(lib, world, steps, cpu, sample=2000):
     """The rain-in (configs[4] as specified): hulls spawn (b3Body::CreateShape) and fall into
     the container; each step runs, on the device, what b3World::Step runs around the solver:
@@ -666,11 +676,15 @@ def rain_world_run(lib, world, steps, cpu, sample=2000):
             shape_body.extend(st.tolist())
             m.find_new_contacts()
             ctx.reset_kernel_stats()
-            ctx.set_profiling(True)
-            step_ms, created, destroyed, collided, listed = [], [], [], [], []
+            step_ms, created, destroyed, collided, listed, profiled = [], [], [], [], [], []
             ctx.set_counting(True)
             ctx.reset_counters()
             for s in range(steps):
+                # Every RAIN_PROFILE_EVERY-th step brackets its launches with events for the
+                # per-kernel times; the step times and rates come from the other steps.
+                prof = s % RAIN_PROFILE_EVERY == 0
+                ctx.set_profiling(prof)
+                profiled.append(prof)
                 d_xf.upload(xf)
                 d_q.upload(q)
                 ctx.synchronize()
@@ -710,17 +724,19 @@ def rain_world_run(lib, world, steps, cpu, sample=2000):
                       "cm_scatter", "cm_overlap", "gather", "cache_hull", "sat_hull", "contact_hull", "convex"):
                 n_, t_ = ctx.kernel_stats(k)
                 if n_:
-                    kern[k] = t_ / steps
+                    kern[k] = t_ / max(1, int(np.sum(profiled)))
             pairs, born, overlap, out, caches = m.read()
             fat = bp.read(0, bp.info()[1])[0][np.array(shape_proxy)]
-    step_ms = np.array(step_ms)
+    timed_steps = ~np.array(profiled)
+    step_ms = np.array(step_ms)[timed_steps]
     total_s = step_ms.sum() / 1000.0
     res.update({
         "ms_per_step": float(step_ms.mean()),
         "ms_per_step_p50_p99_max": [float(np.percentile(step_ms, 50)), float(np.percentile(step_ms, 99)),
                                     float(step_ms.max())],
-        "contacts_per_s": float(np.sum(collided) / total_s),
-        "steps_per_s": steps / total_s,
+        "contacts_per_s": float(np.sum(np.array(collided)[timed_steps]) / total_s),
+        "steps_per_s": len(step_ms) / total_s,
+        "timed_steps": int(len(step_ms)),
         "contacts_created": int(np.sum(created)), "contacts_destroyed": int(np.sum(destroyed)),
         "narrowphase_calls": int(np.sum(collided)),
         "contacts_at_end": int(listed[-1]), "touching_at_end": int((out["pointCount"] > 0).sum()),
@@ -729,7 +745,9 @@ def rain_world_run(lib, world, steps, cpu, sample=2000):
         "what": "per step: b3amd_contact_manager_add_shapes + find_new_contacts for the new shapes, "
                 "update_contacts (destroy, narrowphase with each contact's cache), synchronize (SynchronizeShapes "
                 "+ MoveProxy of every shape present) and find_new_contacts (FindPairs + AddPair), all on the "
-                "device, host wall clock including their synchronisations; body states uploaded outside it",
+                "device, host wall clock including their synchronisations; body states uploaded outside it; "
+                "rates over the steps run without kernel events, kernel_ms_per_step from every %dth step, "
+                "which records them" % RAIN_PROFILE_EVERY,
     })
     res["timeline"] = {"every": 50, "contacts": [int(x) for x in listed[::50]],
                        "created": [int(x) for x in np.add.reduceat(created, np.arange(0, steps, 50))],

@@ -514,6 +514,11 @@ int b3amd_create(int device, uint64_t hbm_budget_bytes, b3amd_context** out) {
         b3amd_destroy(ctx);
         return fail(B3AMD_ERR_DEVICE, "stream or event creation failed");
     }
+    if (hipHostMalloc((void**)&ctx->hWords, 4ull * b3amd_context::kHostWords, 0) != hipSuccess) {
+        ctx->hWords = nullptr;
+        b3amd_destroy(ctx);
+        return fail(B3AMD_ERR_DEVICE, "pinned host allocation failed");
+    }
     if (scratch_address(device, &ctx->noAllocScratch) != B3AMD_OK) {
         std::string msg = b3amd_last_error();
         b3amd_destroy(ctx);
@@ -545,6 +550,7 @@ int b3amd_destroy(b3amd_context* ctx) {
     if (ctx->joinEvent) hipEventDestroy(ctx->joinEvent);
     if (ctx->side) hipStreamDestroy(ctx->side);
     if (ctx->stream) hipStreamDestroy(ctx->stream);
+    if (ctx->hWords) hipHostFree(ctx->hWords);
     delete ctx;
     return B3AMD_OK;
 }

@@ -274,44 +274,116 @@ __global__ __launch_bounds__(256) void bp_escape_kernel(Lbvh t) {
     t.escape[x] = e;
 }
 
-// One lane per moved proxy: every overlapping leaf p != q, emitted once per unordered pair.
+// A pair (p, q) of the moved proxy q and the leaf's proxy p, kept once per unordered pair:
+// from the lower id when both moved.
+__device__ __forceinline__ bool query_pair(const BpState& st, uint32_t p, uint32_t q, uint32_t bits, uint64_t& key) {
+    if (p == q || ((st.flags[p] & kMoved) && p < q)) return false;
+    key = ((uint64_t)min(p, q) << bits) | max(p, q);
+    return true;
+}
+
+// The subtree of `root` walked by one lane over the escape links (the walk leaves the
+// subtree at escape[root]), pairs appended one atomic each: the way out for a group whose
+// stack of pending nodes is full.
+__device__ void query_subtree(const Lbvh& t, const BpState& st, uint32_t root, uint32_t q, float4 ql, float4 qu,
+                              uint32_t bits, uint64_t* out, uint32_t cap, uint32_t* count, uint32_t* status) {
+    const uint32_t firstLeaf = t.n - 1u, end = t.escape[root];
+    uint32_t node = root;
+    for (uint32_t steps = 0; node != end; ++steps) {
+        if (node == kNone || steps >= 2u * t.n) {  // a malformed tree: stop and report
+            atomicMax(status, 1u);
+            return;
+        }
+        float4 l, u;
+        node_box(t, st, node, l, u);
+        const bool hit = overlap(l, u, ql, qu);
+        if (node >= firstLeaf) {
+            uint64_t key;
+            if (hit && query_pair(st, (uint32_t)t.keys[node - firstLeaf], q, bits, key)) {
+                const uint32_t slot = atomicAdd(count, 1u);
+                if (slot < cap) out[slot] = key;
+            }
+            node = t.escape[node];
+        } else {
+            node = hit ? t.left[node] : t.escape[node];
+        }
+    }
+}
+
+// kQueryLanes lanes per moved proxy q: every overlapping leaf p != q, emitted once per
+// unordered pair.  The group keeps the nodes still to test on a stack in LDS; each round
+// its lanes pop up to kQueryLanes of them, test their boxes, and push the children of the
+// internal nodes hit, so a walk takes about as many rounds as the tree is deep instead of
+// one round per node visited.  Children that do not fit on the stack are walked by their
+// lane alone.  The pairs found do not depend on the order of the walk.
+constexpr uint32_t kQueryLanes = 16;
+constexpr uint32_t kQueryGroups = 256 / kQueryLanes;  // moved proxies per block
+constexpr uint32_t kQueryStack = 128;                 // pending nodes per moved proxy
+static_assert(kQueryLanes < 32 && 64 % kQueryLanes == 0 && kQueryStack >= 3 * kQueryLanes, "query groups");
+
+__device__ __forceinline__ void query_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
 __global__ __launch_bounds__(256) void bp_query_kernel(Lbvh t, BpState st, uint32_t nq, const uint32_t* queries,
                                                        uint32_t bits, uint64_t* out, uint32_t cap,
                                                        uint32_t* count, uint32_t* status) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const bool active = i < nq;
-    const uint32_t q = active ? queries[i] : 0u;
+    __shared__ uint32_t stacks[kQueryGroups][kQueryStack];
+    const uint32_t g = threadIdx.x / kQueryLanes, r = threadIdx.x % kQueryLanes;
+    const uint32_t gbase = (threadIdx.x % 64u) - r;  // the group's first lane in its wave
+    const uint32_t qi = blockIdx.x * kQueryGroups + g;
+    const bool active = qi < nq;
+    const uint32_t q = active ? queries[qi] : 0u;
     float4 ql = make_float4(0, 0, 0, 0), qu = ql;
     if (active) ql = st.lower[q], qu = st.upper[q];
-    // Node 0 is the root (the single leaf when n == 1).  A stackless depth-first walk
-    // visits each node at most once, so 2n - 1 steps bound it.
-    uint32_t node = active ? 0u : kNone;
+    uint32_t* stk = stacks[g];
+    if (r == 0) stk[0] = 0u;  // node 0: the root (the single leaf when n == 1)
+    uint32_t top = active ? 1u : 0u;  // the same on every lane of the group
     const uint32_t firstLeaf = t.n - 1u;
-    for (uint32_t steps = 0; __ballot(node != kNone); ++steps) {
-        if (steps >= 2u * t.n) {  // a malformed tree: stop and report
-            if (node != kNone) atomicMax(status, 1u);
+    query_lds_sync();
+    // Every node is pushed at most once per proxy, so 2n - 1 rounds bound the walk.
+    for (uint32_t round = 0; __ballot(top != 0u); ++round) {
+        if (round >= 2u * t.n) {  // a malformed tree: stop and report
+            if (top != 0u && r == 0) atomicMax(status, 1u);
             break;
         }
-        bool emit = false;
+        const uint32_t k = top < kQueryLanes ? top : kQueryLanes;
+        const uint32_t node = r < k ? stk[top - 1u - r] : kNone;
+        top -= k;
+        bool emit = false, push = false;
         uint64_t key = 0;
+        uint32_t lc = 0, rc = 0;
         if (node != kNone) {
             float4 l, u;
             node_box(t, st, node, l, u);
             const bool hit = overlap(l, u, ql, qu);
             if (node >= firstLeaf) {
-                const uint32_t p = (uint32_t)t.keys[node - firstLeaf];
-                if (hit && p != q && (!(st.flags[p] & kMoved) || q < p)) {
-                    emit = true;
-                    const uint32_t a = min(p, q), b = max(p, q);
-                    key = ((uint64_t)a << bits) | b;
-                }
-                node = t.escape[node];
-            } else {
-                node = hit ? t.left[node] : t.escape[node];
+                emit = hit && query_pair(st, (uint32_t)t.keys[node - firstLeaf], q, bits, key);
+            } else if (hit) {
+                push = true;
+                lc = t.left[node];
+                rc = t.right[node];
             }
         }
+        const uint32_t gm = (uint32_t)(__ballot(push) >> gbase) & ((1u << kQueryLanes) - 1u);
+        const uint32_t before = (uint32_t)__popc(gm & ((1u << r) - 1u)), total = (uint32_t)__popc(gm);
+        const uint32_t room = (kQueryStack - top) / 2u;
+        const uint32_t fit = total < room ? total : room;
+        query_lds_sync();  // this round's pops are read before its pushes overwrite them
+        if (push && before < fit) {
+            stk[top + 2u * before] = rc;
+            stk[top + 2u * before + 1u] = lc;
+        }
+        top += 2u * fit;
         const uint32_t slot = wave_append(count, emit);
         if (emit && slot < cap) out[slot] = key;
+        if (push && before >= fit) {
+            query_subtree(t, st, lc, q, ql, qu, bits, out, cap, count, status);
+            query_subtree(t, st, rc, q, ql, qu, bits, out, cap, count, status);
+        }
+        query_lds_sync();
     }
 }
 
@@ -339,6 +411,13 @@ __global__ __launch_bounds__(256) void bp_test_overlap_kernel(uint32_t n, const 
         return;
     }
     out[i] = overlap(st.lower[a], st.upper[a], st.lower[b], st.upper[b]) ? 1u : 0u;
+}
+
+// The find-pairs count words: leaf, query and pair counts 0, the Morton bounds' minimum
+// all ones and maximum 0 (word 9, the move status, is kept).
+__global__ void bp_init_counts_kernel(uint32_t* counts) {
+    const uint32_t i = threadIdx.x;
+    if (i < 9u) counts[i] = (i >= 3u && i < 6u) ? 0xFFFFFFFFu : 0u;
 }
 
 __global__ void bp_fill_kernel(uint32_t n, uint32_t* p, uint32_t v) {
@@ -640,9 +719,8 @@ int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, cons
     bp->lastPairs = 0;
     if (bp->count == 0) return B3AMD_OK;
     if ((rc = bp_grow_leaves(bp, bp->count))) return rc;
-    // counts[0..2] = 0, bounds min = all ones, max = 0 (the move status word 9 is kept)
-    const uint32_t init[9] = {0u, 0u, 0u, 0xFFFFFFFFu, 0xFFFFFFFFu, 0xFFFFFFFFu, 0u, 0u, 0u};
-    HIP_TRY(hipMemcpyAsync(bp->dCounts, init, sizeof(init), hipMemcpyHostToDevice, s));
+    bp_init_counts_kernel<<<1, 64, 0, s>>>(bp->dCounts);
+    HIP_TRY(hipGetLastError());
     rc = timed(ctx, s, "bp_lists", [&] {
         bp_lists_kernel<<<blocks(bp->idCap), 256, 0, s>>>(bp->idCap, bp->st, bp->dLeaves, bp->dQueries,
                                                           bp->leafCap, bp->dCounts, bp->dCounts + 3);
@@ -650,8 +728,7 @@ int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, cons
     });
     if (rc) return rc;
     uint32_t counts[10];
-    HIP_TRY(hipMemcpyAsync(counts, bp->dCounts, sizeof(counts), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if ((rc = ctx->read_back(bp->dCounts, counts, sizeof(counts)))) return rc;
     const uint32_t n = counts[0], nq = counts[1];
     if (counts[9]) {
         const uint32_t zero = 0u;
@@ -685,17 +762,16 @@ int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, cons
     uint32_t np = 0;
     for (int attempt = 0; attempt < 2; ++attempt) {
         if ((rc = bp_grow_pairs(bp, std::max<uint32_t>(np, 4u * nq)))) return rc;
-        const uint32_t zero = 0u;
-        HIP_TRY(hipMemcpyAsync(bp->dCounts + 2, &zero, 4, hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_zero_words(bp->dCounts + 2, 1, s));
         rc = timed(ctx, s, "bp_query", [&] {
-            bp_query_kernel<<<blocks(nq), 256, 0, s>>>(t, bp->st, nq, bp->dQueries, bits, bp->dPairKeys, bp->pairCap,
-                                                       bp->dCounts + 2, bp->dCounts + 10);
+            const uint32_t qb = (nq + kQueryGroups - 1u) / kQueryGroups;
+            bp_query_kernel<<<qb, 256, 0, s>>>(t, bp->st, nq, bp->dQueries, bits, bp->dPairKeys, bp->pairCap,
+                                               bp->dCounts + 2, bp->dCounts + 10);
             return hipGetLastError();
         });
         if (rc) return rc;
         uint32_t qs[9];
-        HIP_TRY(hipMemcpyAsync(qs, bp->dCounts + 2, sizeof(qs), hipMemcpyDeviceToHost, s));
-        HIP_TRY(hipStreamSynchronize(s));
+        if ((rc = ctx->read_back(bp->dCounts + 2, qs, sizeof(qs)))) return rc;
         np = qs[0];
         if (qs[8]) return fail(B3AMD_ERR_DEVICE, "broadphase tree walk did not end (internal)");
         if (np <= bp->pairCap) break;

@@ -367,8 +367,19 @@ int cm_reserve_pairs(b3amd_contact_manager* m, uint32_t need) {
 
 template <class T>
 int read1(b3amd_contact_manager* m, const T* d, T* h) {
-    HIP_TRY(hipMemcpyAsync(h, d, sizeof(T), hipMemcpyDeviceToHost, m->ctx->stream));
-    HIP_TRY(hipStreamSynchronize(m->ctx->stream));
+    return m->ctx->read_back(d, h, sizeof(T));
+}
+
+// The last element of an exclusive scan and of its input (their sum is the total), read
+// back together through the context's pinned words.
+template <class T>
+int read_scan_total(b3amd_contact_manager* m, const T* pos, const T* in, uint32_t n, T* total) {
+    b3amd_context* ctx = m->ctx;
+    T* h = reinterpret_cast<T*>(ctx->hWords);
+    HIP_TRY(hipMemcpyAsync(h, pos + n - 1, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipMemcpyAsync(h + 1, in + n - 1, sizeof(T), hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    *total = h[0] + h[1];
     return B3AMD_OK;
 }
 
@@ -549,11 +560,8 @@ int b3amd_contact_manager_find_new_contacts(b3amd_contact_manager* m, uint32_t* 
     size_t have = m->tmpBytes;
     if (rocprim::exclusive_scan(m->dTmp, have, m->dKeep, m->dKeepPos, 0u, P, rocprim::plus<uint32_t>(), s) != hipSuccess)
         return fail(B3AMD_ERR_DEVICE, "scan failed");
-    uint32_t lastPos = 0, lastKeep = 0;
-    HIP_TRY(hipMemcpyAsync(&lastPos, m->dKeepPos + P - 1, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&lastKeep, m->dKeep + P - 1, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const uint32_t K = lastPos + lastKeep;
+    uint32_t K = 0;
+    if ((rc = read_scan_total(m, m->dKeepPos, m->dKeep, P, &K))) return rc;
     if (K == 0) return B3AMD_OK;
     const uint32_t fresh = K > m->freeCount ? K - m->freeCount : 0u;
     if ((rc = cm_reserve(m, std::max(m->slotHigh + fresh, m->L + K)))) return rc;
@@ -639,11 +647,8 @@ int b3amd_contact_manager_update_contacts(b3amd_contact_manager* m, const b3amd_
     if (rocprim::exclusive_scan(m->dTmp, have, m->dV, m->dPos, (uint64_t)0, L, rocprim::plus<uint64_t>(), s) !=
         hipSuccess)
         return fail(B3AMD_ERR_DEVICE, "scan failed");
-    uint64_t lastPos = 0, lastV = 0;
-    HIP_TRY(hipMemcpyAsync(&lastPos, m->dPos + L - 1, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&lastV, m->dV + L - 1, 8, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    const uint64_t tot = lastPos + lastV;
+    uint64_t tot = 0;
+    if ((rc = read_scan_total(m, m->dPos, m->dV, L, &tot))) return rc;
     const uint32_t kept = (uint32_t)(tot >> 32), collided = (uint32_t)tot, destroyed = L - kept;
     const int nxt = 1 - m->cur;
     CmScatterArgs a;

@@ -6,6 +6,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 #include <map>
 #include <string>
 #include <unordered_map>
@@ -113,6 +114,10 @@ struct b3amd_context {
     uint64_t dInCap = 0;
     // Status words of entry points that check device-side inputs (16 words, on demand).
     uint32_t* dStatus = nullptr;
+    // Pinned host words for the counts that host code reads back between launches
+    // (kHostWords words; a copy into pageable memory goes through a staging copy).
+    static constexpr uint32_t kHostWords = 64;
+    uint32_t* hWords = nullptr;
     // Shape pairs of the contacts of a world step (b3amd_update_contacts).
     void* dGather = nullptr;
     uint32_t gatherCap = 0;
@@ -172,6 +177,15 @@ struct b3amd_context {
         hipEvent_t e;
         if (hipEventCreate(&e) != hipSuccess) return nullptr;
         return e;
+    }
+    // Copies `bytes` (at most 4 * kHostWords) of device memory to `h` after the work queued
+    // on the stream so far, through the pinned words.
+    int read_back(const void* d, void* h, size_t bytes) {
+        if (bytes > 4ull * kHostWords) return fail(B3AMD_ERR_ARGUMENT, "read-back of %zu bytes", bytes);
+        HIP_TRY(hipMemcpyAsync(hWords, d, bytes, hipMemcpyDeviceToHost, stream));
+        HIP_TRY(hipStreamSynchronize(stream));
+        std::memcpy(h, hWords, bytes);
+        return B3AMD_OK;
     }
     int resolve_timings() {
         if (pending.empty()) return B3AMD_OK;
