@@ -134,8 +134,8 @@ struct b3amd_broadphase {
     uint32_t* dQueries = nullptr;
     uint64_t* dKeys = nullptr;     // [leafCap] unsorted / sorted Morton keys
     uint64_t* dKeys2 = nullptr;
-    uint32_t* dNodes = nullptr;    // left, right, parent, escape, visits
-    float4* dBoxes = nullptr;      // internal node boxes
+    uint32_t* dNodes = nullptr;    // left, right, parent, escape, first, last
+    float4* dBoxes = nullptr;      // internal node boxes, then the refit's range tables
     uint32_t* dCounts = nullptr;   // [0] leaves, [1] queries, [2] pairs, [3..8] bounds, [9] status
     uint32_t pairCap = 0;
     uint64_t* dPairKeys = nullptr;

@@ -167,7 +167,14 @@ struct Lbvh {
     uint32_t* escape;      // [2n-1]
     float4* lo;            // [n-1] internal node boxes
     float4* hi;
-    uint32_t* visits;      // [n-1]
+    uint32_t* first;       // [n-1] first and last sorted leaf under each internal node
+    uint32_t* last;
+    // Refit scratch: per sorted leaf, the box of the leaves from its refit block's start to
+    // it (pre) and from it to the block's end (suf); a sparse table over the blocks' boxes
+    // (level k, block j: blocks j .. j + 2^k - 1) at blkLo/blkHi[k * blkStride + j].
+    float4 *preLo, *preHi, *sufLo, *sufHi;
+    float4 *blkLo, *blkHi;
+    uint32_t blkStride;
 };
 
 __device__ __forceinline__ int common_prefix(const uint64_t* k, uint32_t n, int i, int j) {
@@ -204,7 +211,8 @@ __global__ __launch_bounds__(256) void bp_build_kernel(Lbvh t) {
     t.right[i] = rc;
     t.parent[lc] = (uint32_t)i;
     t.parent[rc] = (uint32_t)i;
-    t.visits[i] = 0u;
+    t.first[i] = first;
+    t.last[i] = last;
     if (i == 0) t.parent[0] = kNone;
 }
 
@@ -219,43 +227,113 @@ __device__ __forceinline__ void node_box(const Lbvh& t, const BpState& st, uint3
     }
 }
 
-// Box of a node written earlier in the same refit launch, possibly by another CU: read at
-// device scope so that no stale line of this CU's vector cache is used.
-__device__ __forceinline__ float4 load_fresh(const float4* p) {
-    const uint32_t* w = reinterpret_cast<const uint32_t*>(p);
-    return make_float4(__uint_as_float(__hip_atomic_load(w + 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)),
-                       __uint_as_float(__hip_atomic_load(w + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)),
-                       __uint_as_float(__hip_atomic_load(w + 2, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)), 0.0f);
+// Refit: an internal node's box is the exact min/max over the leaves of its range, which
+// is order-free, so it is read off range-minimum tables instead of climbing the tree (a
+// climb passes every level through device-scope atomics and fences).  Leaves are taken
+// in blocks of kRefitBlock in sorted order:
+//   * bp_refit_block_kernel: a sparse table of the block's leaf boxes in LDS gives the box
+//     of every internal node whose range lies in the block, the box of each leaf's prefix
+//     and suffix within the block, and the block's box;
+//   * bp_refit_table_kernel: a sparse table over the blocks' boxes (one workgroup);
+//   * bp_refit_cross_kernel: a node whose range crosses blocks combines the suffix in its
+//     first block, the whole blocks between (two table entries) and the prefix in its last.
+constexpr uint32_t kRefitBlock = 256;
+constexpr uint32_t kRefitLevels = 8;  // in-block table levels: spans 1 .. 128
+
+struct Box6 {
+    float4 l, u;
+};
+
+__device__ __forceinline__ Box6 join(const Box6& a, const Box6& b) {
+    return Box6{make_float4(fminf(a.l.x, b.l.x), fminf(a.l.y, b.l.y), fminf(a.l.z, b.l.z), 0.0f),
+                make_float4(fmaxf(a.u.x, b.u.x), fmaxf(a.u.y, b.u.y), fmaxf(a.u.z, b.u.z), 0.0f)};
 }
 
-__device__ __forceinline__ void refit_child_box(const Lbvh& t, const BpState& st, uint32_t node, float4& l,
-                                                float4& u) {
-    if (node >= t.n - 1u) {
-        const uint32_t id = (uint32_t)t.keys[node - (t.n - 1u)];
-        l = st.lower[id];
-        u = st.upper[id];
-    } else {
-        l = load_fresh(t.lo + node);
-        u = load_fresh(t.hi + node);
+__device__ __forceinline__ uint32_t floor_log2(uint32_t v) { return 31u - (uint32_t)__clz(v); }
+
+__global__ __launch_bounds__(kRefitBlock) void bp_refit_block_kernel(Lbvh t, BpState st) {
+    __shared__ float tab[kRefitLevels][6][kRefitBlock];
+    const uint32_t tid = threadIdx.x, b0 = blockIdx.x * kRefitBlock;
+    const uint32_t cnt = t.n - b0 < kRefitBlock ? t.n - b0 : kRefitBlock;
+    if (tid < cnt) {
+        const uint32_t id = (uint32_t)t.keys[b0 + tid];
+        const float4 l = st.lower[id], u = st.upper[id];
+        tab[0][0][tid] = l.x, tab[0][1][tid] = l.y, tab[0][2][tid] = l.z;
+        tab[0][3][tid] = u.x, tab[0][4][tid] = u.y, tab[0][5][tid] = u.z;
+    }
+    for (uint32_t k = 1; k < kRefitLevels; ++k) {
+        __syncthreads();
+        const uint32_t half = 1u << (k - 1u);
+        if (tid + 2u * half <= cnt) {
+#pragma unroll
+            for (int c = 0; c < 3; ++c) tab[k][c][tid] = fminf(tab[k - 1][c][tid], tab[k - 1][c][tid + half]);
+#pragma unroll
+            for (int c = 3; c < 6; ++c) tab[k][c][tid] = fmaxf(tab[k - 1][c][tid], tab[k - 1][c][tid + half]);
+        }
+    }
+    __syncthreads();
+    // Box of block-relative leaves f .. l (f <= l < cnt): two windows of the largest span
+    // that fits cover the range.
+    auto range = [&](uint32_t f, uint32_t l) {
+        uint32_t k = floor_log2(l - f + 1u);
+        if (k >= kRefitLevels) k = kRefitLevels - 1u;
+        const uint32_t g = l + 1u - (1u << k);
+        Box6 r;
+        r.l = make_float4(fminf(tab[k][0][f], tab[k][0][g]), fminf(tab[k][1][f], tab[k][1][g]),
+                          fminf(tab[k][2][f], tab[k][2][g]), 0.0f);
+        r.u = make_float4(fmaxf(tab[k][3][f], tab[k][3][g]), fmaxf(tab[k][4][f], tab[k][4][g]),
+                          fmaxf(tab[k][5][f], tab[k][5][g]), 0.0f);
+        return r;
+    };
+    if (tid < cnt) {
+        const Box6 p = range(0u, tid), q = range(tid, cnt - 1u);
+        t.preLo[b0 + tid] = p.l, t.preHi[b0 + tid] = p.u;
+        t.sufLo[b0 + tid] = q.l, t.sufHi[b0 + tid] = q.u;
+    }
+    if (tid == 0) {
+        const Box6 a = range(0u, cnt - 1u);
+        t.blkLo[blockIdx.x] = a.l, t.blkHi[blockIdx.x] = a.u;
+    }
+    // Internal node i has i at one end of its range, so a range inside this block has its
+    // node here.
+    const uint32_t i = b0 + tid;
+    if (i + 1u < t.n) {
+        const uint32_t f = t.first[i], l = t.last[i];
+        if (f >= b0 && l < b0 + cnt) {
+            const Box6 r = range(f - b0, l - b0);
+            t.lo[i] = r.l, t.hi[i] = r.u;
+        }
     }
 }
 
-// Bottom-up boxes: the second child to finish combines both (exact min/max).
-__global__ __launch_bounds__(256) void bp_refit_kernel(Lbvh t, BpState st) {
-    const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
-    if (j >= t.n || t.n < 2u) return;
-    uint32_t node = t.parent[t.n - 1u + j];
-    for (uint32_t up = 0; node != kNone && up < t.n; ++up) {
-        __threadfence();
-        if (atomicAdd(t.visits + node, 1u) == 0u) return;
-        __threadfence();
-        float4 a, b, c, d;
-        refit_child_box(t, st, t.left[node], a, b);
-        refit_child_box(t, st, t.right[node], c, d);
-        t.lo[node] = make_float4(fminf(a.x, c.x), fminf(a.y, c.y), fminf(a.z, c.z), 0.0f);
-        t.hi[node] = make_float4(fmaxf(b.x, d.x), fmaxf(b.y, d.y), fmaxf(b.z, d.z), 0.0f);
-        node = t.parent[node];
+__global__ __launch_bounds__(1024) void bp_refit_table_kernel(Lbvh t, uint32_t nb) {
+    for (uint32_t k = 1; (1u << k) <= nb; ++k) {
+        __syncthreads();
+        const uint32_t half = 1u << (k - 1u);
+        const float4* pl = t.blkLo + (k - 1u) * t.blkStride;
+        const float4* pu = t.blkHi + (k - 1u) * t.blkStride;
+        for (uint32_t j = threadIdx.x; j + 2u * half <= nb; j += blockDim.x) {
+            const Box6 r = join(Box6{pl[j], pu[j]}, Box6{pl[j + half], pu[j + half]});
+            t.blkLo[k * t.blkStride + j] = r.l;
+            t.blkHi[k * t.blkStride + j] = r.u;
+        }
     }
+}
+
+__global__ __launch_bounds__(256) void bp_refit_cross_kernel(Lbvh t) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i + 1u >= t.n) return;
+    const uint32_t f = t.first[i], l = t.last[i];
+    const uint32_t bf = f / kRefitBlock, bl = l / kRefitBlock;
+    if (bf == bl) return;
+    Box6 r = join(Box6{t.sufLo[f], t.sufHi[f]}, Box6{t.preLo[l], t.preHi[l]});
+    if (bl > bf + 1u) {
+        const uint32_t a = bf + 1u, c = bl - 1u, k = floor_log2(c - a + 1u), g = c + 1u - (1u << k);
+        const uint32_t o = k * t.blkStride;
+        r = join(r, join(Box6{t.blkLo[o + a], t.blkHi[o + a]}, Box6{t.blkLo[o + g], t.blkHi[o + g]}));
+    }
+    t.lo[i] = r.l;
+    t.hi[i] = r.u;
 }
 
 // escape[x]: the node after x's subtree in depth-first order (kNone after the last).
@@ -478,6 +556,9 @@ __global__ __launch_bounds__(256) void shape_aabb_kernel(ShapeAabbArgs a) {
 }
 
 inline uint32_t blocks(uint64_t n) { return (uint32_t)((n + 255) / 256); }
+// Refit blocks of `cap` leaves (the row length of the block table, whose 32 levels cover
+// any count below 2^32).
+inline uint32_t refit_blocks(uint32_t cap) { return (cap + kRefitBlock - 1u) / kRefitBlock; }
 
 }  // namespace b3amd
 
@@ -541,7 +622,7 @@ int bp_grow_leaves(b3amd_broadphase* bp, uint32_t need) {
     int rc;
     if ((rc = ctx->alloc(8ull * cap, (void**)&bp->dLeaves)) || (rc = ctx->alloc(8ull * cap, (void**)&bp->dKeys)) ||
         (rc = ctx->alloc(8ull * cap, (void**)&bp->dKeys2)) || (rc = ctx->alloc(4ull * 8 * cap, (void**)&bp->dNodes)) ||
-        (rc = ctx->alloc(32ull * cap, (void**)&bp->dBoxes)))
+        (rc = ctx->alloc(16ull * (6ull * cap + 64ull * refit_blocks(cap)), (void**)&bp->dBoxes)))
         return rc;
     bp->dQueries = bp->dLeaves + cap;
     bp->leafCap = cap;
@@ -746,15 +827,28 @@ int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, cons
     t.right = bp->dNodes + bp->leafCap;
     t.parent = bp->dNodes + 2ull * bp->leafCap;
     t.escape = bp->dNodes + 4ull * bp->leafCap;
-    t.visits = bp->dNodes + 6ull * bp->leafCap;
+    t.first = bp->dNodes + 6ull * bp->leafCap;
+    t.last = bp->dNodes + 7ull * bp->leafCap;
     t.lo = bp->dBoxes;
     t.hi = bp->dBoxes + bp->leafCap;
+    t.preLo = bp->dBoxes + 2ull * bp->leafCap;
+    t.preHi = bp->dBoxes + 3ull * bp->leafCap;
+    t.sufLo = bp->dBoxes + 4ull * bp->leafCap;
+    t.sufHi = bp->dBoxes + 5ull * bp->leafCap;
+    t.blkStride = refit_blocks(bp->leafCap);
+    t.blkLo = bp->dBoxes + 6ull * bp->leafCap;
+    t.blkHi = t.blkLo + 32ull * t.blkStride;
     bp_morton_kernel<<<blocks(n), 256, 0, s>>>(n, bp->dLeaves, bp->st, bp->dCounts + 3, bp->dKeys);
     HIP_TRY(hipGetLastError());
     if ((rc = bp_sort(bp, bp->dKeys, bp->dKeys2, n, 64u))) return rc;
     if (n > 1u) {
         bp_build_kernel<<<blocks(n - 1u), 256, 0, s>>>(t);
-        bp_refit_kernel<<<blocks(n), 256, 0, s>>>(t, bp->st);
+        const uint32_t nb = (n + kRefitBlock - 1u) / kRefitBlock;
+        bp_refit_block_kernel<<<nb, kRefitBlock, 0, s>>>(t, bp->st);
+        if (nb > 1u) {
+            bp_refit_table_kernel<<<1, 1024, 0, s>>>(t, nb);
+            bp_refit_cross_kernel<<<blocks(n - 1u), 256, 0, s>>>(t);
+        }
     }
     bp_escape_kernel<<<blocks(2ull * n - 1u), 256, 0, s>>>(t);
     HIP_TRY(hipGetLastError());
