@@ -116,6 +116,20 @@ __device__ void shape_aabb(const ShapeRec& s, const float4* pool, const Xf& xf, 
     }
 }
 
+// b3amd_broadphase_create_proxies for AABBs already on the device (d_aabbs, [n][6]): the
+// ids come from the host's model of the node pool as there, the fat AABBs are made on the
+// device.  Enqueued on the context stream: `records` ([n][2] host words: id, user data) is
+// copied to d_records, so the caller keeps both until it synchronises the stream.
+int bp_create_proxies_device(b3amd_broadphase* bp, uint32_t n, const float* d_aabbs, const uint32_t* users,
+                             uint32_t* ids, uint32_t* records, uint32_t* d_records);
+// b3amd_broadphase_find_pairs; with sync false the last launches (the pair sort and the
+// move-buffer reset) are left enqueued on the context stream.
+int bp_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, const uint32_t** d_pairs, bool sync);
+// b3amd_shape_aabbs without the status read-back, for inputs the caller has checked:
+// enqueued on the context stream.
+int launch_shape_aabbs(b3amd_context* ctx, uint32_t n, const uint32_t* d_shapes, const uint32_t* d_bodies,
+                       uint32_t body_count, const b3amd_transform* d_body_xfs, float* d_aabbs);
+
 }  // namespace b3amd
 
 struct b3amd_broadphase {

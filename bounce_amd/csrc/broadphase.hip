@@ -53,6 +53,20 @@ __global__ __launch_bounds__(256) void bp_edit_kernel(uint32_t n, const BpEdit* 
     }
 }
 
+// CreateProxy (broad_phase.cpp:78-90) of proxy records[2i] with user data records[2i+1]
+// and the AABB aabbs[i] ([n][6]), fattened by b3_aabbExtension (aabb3.h:85-90).
+__global__ __launch_bounds__(256) void bp_create_kernel(uint32_t n, const uint32_t* records, const float* aabbs,
+                                                        BpState st) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = records[2 * i];
+    const float* a = aabbs + 6ull * i;
+    st.lower[id] = make_float4(a[0] - kAabbExtension, a[1] - kAabbExtension, a[2] - kAabbExtension, 0.0f);
+    st.upper[id] = make_float4(a[3] + kAabbExtension, a[4] + kAabbExtension, a[5] + kAabbExtension, 0.0f);
+    st.user[id] = records[2 * i + 1];
+    st.flags[id] = kLive | kMoved;  // BufferMove (broad_phase.cpp:87)
+}
+
 // MoveProxy (broad_phase.cpp:99-149) for proxies ids[i].
 __global__ __launch_bounds__(256) void bp_move_kernel(uint32_t n, const uint32_t* ids, const float* aabbs,
                                                       const float* disp, BpState st, uint32_t idCap,
@@ -591,6 +605,28 @@ int bp_grow_ids(b3amd_broadphase* bp, uint32_t need) {
     return B3AMD_OK;
 }
 
+// The proxy id of the next create (b3DynamicTree::InsertNode: AllocateNode for the leaf,
+// then InsertLeaf allocates a parent unless the tree is empty; dynamic_tree.cpp:102-114,
+// 203-214), marked live.
+int bp_take_id(b3amd_broadphase* bp, uint32_t* out) {
+    uint32_t id;
+    if (!bp->freeGroups.empty()) {
+        const std::pair<uint32_t, bool> g = bp->freeGroups.back();
+        if (g.second != (bp->count >= 1u)) return fail(B3AMD_ERR_ARGUMENT, "node pool out of step (internal)");
+        bp->freeGroups.pop_back();
+        id = g.first;
+    } else {
+        id = bp->fresh++;
+        if (bp->count >= 1u) bp->fresh++;  // the new parent node
+    }
+    int rc = bp_grow_ids(bp, id + 1u);
+    if (rc) return rc;
+    bp->live[id] = 1u;
+    bp->count++;
+    *out = id;
+    return B3AMD_OK;
+}
+
 int bp_apply_edits(b3amd_broadphase* bp, const std::vector<BpEdit>& edits) {
     if (edits.empty()) return B3AMD_OK;
     b3amd_context* ctx = bp->ctx;
@@ -672,125 +708,24 @@ uint32_t id_bits(uint32_t idCap) {
 
 }  // namespace
 
-extern "C" {
+namespace b3amd {
 
-int b3amd_broadphase_create(b3amd_context* ctx, b3amd_broadphase** out) {
-    if (!ctx || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    b3amd_broadphase* bp = new b3amd_broadphase();
-    bp->ctx = ctx;
-    int rc;
-    if ((rc = ctx->alloc(16 * 4, (void**)&bp->dCounts)) || (rc = bp_grow_ids(bp, 1024u))) {
-        b3amd_broadphase_destroy(bp);
-        return rc;
-    }
-    bp_fill_kernel<<<1, 64, 0, ctx->stream>>>(16u, bp->dCounts, 0u);
-    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
-        b3amd_broadphase_destroy(bp);
-        return fail(B3AMD_ERR_DEVICE, "broadphase set-up failed");
-    }
-    *out = bp;
-    return B3AMD_OK;
-}
-
-int b3amd_broadphase_destroy(b3amd_broadphase* bp) {
-    if (!bp) return B3AMD_OK;
-    b3amd_context* ctx = bp->ctx;
-    hipStreamSynchronize(ctx->stream);
-    void* ptrs[] = {bp->st.lower, bp->st.upper, bp->st.user, bp->st.flags, bp->dLeaves, bp->dKeys, bp->dKeys2,
-                    bp->dNodes, bp->dBoxes, bp->dCounts, bp->dPairKeys, bp->dPairKeys2, bp->dPairs,
-                    bp->dSortTmp, bp->dEdits};
-    for (void* p : ptrs) ctx->release(p);
-    delete bp;
-    return B3AMD_OK;
-}
-
-int b3amd_broadphase_create_proxies(b3amd_broadphase* bp, uint32_t n, const b3amd_aabb* aabbs,
-                                    const uint32_t* user_data, uint32_t* proxy_ids) {
-    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
-    if (n == 0) return B3AMD_OK;
-    if (!aabbs || !proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    std::vector<BpEdit> edits(n);
+int bp_create_proxies_device(b3amd_broadphase* bp, uint32_t n, const float* d_aabbs, const uint32_t* users,
+                             uint32_t* ids, uint32_t* records, uint32_t* d_records) {
     for (uint32_t i = 0; i < n; ++i) {
-        // b3DynamicTree::InsertNode: AllocateNode for the leaf, then InsertLeaf allocates a
-        // parent unless the tree is empty (dynamic_tree.cpp:102-114, 203-214).
-        uint32_t id;
-        if (!bp->freeGroups.empty()) {
-            const std::pair<uint32_t, bool> g = bp->freeGroups.back();
-            if (g.second != (bp->count >= 1u))
-                return fail(B3AMD_ERR_ARGUMENT, "node pool out of step (internal)");
-            bp->freeGroups.pop_back();
-            id = g.first;
-        } else {
-            id = bp->fresh++;
-            if (bp->count >= 1u) bp->fresh++;  // the new parent node
-        }
-        int rc = bp_grow_ids(bp, id + 1u);
+        int rc = bp_take_id(bp, ids + i);
         if (rc) return rc;
-        bp->live[id] = 1u;
-        bp->count++;
-        proxy_ids[i] = id;
-        BpEdit& e = edits[i];
-        e.id = id;
-        e.mode = kEditCreate;
-        e.user = user_data ? user_data[i] : i;
-        e.pad = 0u;
-        // fatAABB = aabb; fatAABB.Extend(b3_aabbExtension) (broad_phase.cpp:80-81, aabb3.h:85-90)
-        const b3amd_aabb& a = aabbs[i];
-        e.lower[0] = a.lower.x - kAabbExtension, e.lower[1] = a.lower.y - kAabbExtension,
-        e.lower[2] = a.lower.z - kAabbExtension;
-        e.upper[0] = a.upper.x + kAabbExtension, e.upper[1] = a.upper.y + kAabbExtension,
-        e.upper[2] = a.upper.z + kAabbExtension;
+        records[2 * i] = ids[i];
+        records[2 * i + 1] = users[i];
     }
-    return bp_apply_edits(bp, edits);
-}
-
-int b3amd_broadphase_destroy_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids) {
-    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
-    if (n == 0) return B3AMD_OK;
-    if (!proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    std::vector<BpEdit> edits(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t id = proxy_ids[i];
-        if (id >= bp->idCap || !bp->live[id]) return fail(B3AMD_ERR_ARGUMENT, "proxy %u is not live", id);
-        // RemoveLeaf frees the parent when the leaf is not the root, then FreeNode the leaf
-        // (dynamic_tree.cpp:72-79, 251-292).
-        bp->freeGroups.emplace_back(id, bp->count >= 2u);
-        bp->count--;
-        bp->live[id] = 0u;
-        edits[i] = BpEdit{id, kEditDestroy, 0u, 0u, {0, 0, 0}, {0, 0, 0}};
-    }
-    return bp_apply_edits(bp, edits);
-}
-
-int b3amd_broadphase_touch_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids) {
-    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
-    if (n == 0) return B3AMD_OK;
-    if (!proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
-    std::vector<BpEdit> edits(n);
-    for (uint32_t i = 0; i < n; ++i) {
-        const uint32_t id = proxy_ids[i];
-        if (id >= bp->idCap || !bp->live[id]) return fail(B3AMD_ERR_ARGUMENT, "proxy %u is not live", id);
-        edits[i] = BpEdit{id, kEditTouch, 0u, 0u, {0, 0, 0}, {0, 0, 0}};
-    }
-    return bp_apply_edits(bp, edits);
-}
-
-int b3amd_broadphase_move_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* d_proxy_ids,
-                                  const b3amd_aabb* d_aabbs, const b3amd_vec3* d_displacements, uint32_t* d_moved) {
-    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
-    if (n == 0) return B3AMD_OK;
-    if (!d_proxy_ids || !d_aabbs || !d_displacements) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     hipStream_t s = bp->ctx->stream;
-    // A bad id is counted in the status word and reported by the next find_pairs.
-    return timed(bp->ctx, s, "bp_move", [&] {
-        bp_move_kernel<<<blocks(n), 256, 0, s>>>(n, d_proxy_ids, reinterpret_cast<const float*>(d_aabbs),
-                                                 reinterpret_cast<const float*>(d_displacements), bp->st, bp->idCap,
-                                                 d_moved, bp->dCounts + 9);
-        return hipGetLastError();
-    });
+    HIP_TRY(hipMemcpyAsync(d_records, records, 8ull * n, hipMemcpyHostToDevice, s));
+    bp_create_kernel<<<blocks(n), 256, 0, s>>>(n, d_records, d_aabbs, bp->st);
+    HIP_TRY(hipGetLastError());
+    return B3AMD_OK;
 }
 
-int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, const uint32_t** d_pairs) {
+int bp_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, const uint32_t** d_pairs, bool sync) {
     if (!bp || !pair_count) return fail(B3AMD_ERR_ARGUMENT, "null argument");
     b3amd_context* ctx = bp->ctx;
     hipStream_t s = ctx->stream;
@@ -878,11 +813,145 @@ int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, cons
     // The move buffer is reset for the next step (broad_phase.h:185-186).
     bp_clear_moved_kernel<<<blocks(nq), 256, 0, s>>>(nq, bp->dQueries, bp->st.flags);
     HIP_TRY(hipGetLastError());
-    HIP_TRY(hipStreamSynchronize(s));
+    if (sync) HIP_TRY(hipStreamSynchronize(s));
     bp->lastPairs = np;
     *pair_count = np;
     if (d_pairs) *d_pairs = bp->dPairs;
     return B3AMD_OK;
+}
+
+int launch_shape_aabbs(b3amd_context* ctx, uint32_t n, const uint32_t* d_shapes, const uint32_t* d_bodies,
+                       uint32_t body_count, const b3amd_transform* d_body_xfs, float* d_aabbs) {
+    hipStream_t s = ctx->stream;
+    int rc;
+    if (!ctx->dStatus && (rc = ctx->alloc(64, (void**)&ctx->dStatus))) return rc;
+    ShapeAabbArgs a;
+    a.n = n;
+    a.bodyCount = body_count;
+    a.shapes = d_shapes;
+    a.bodies = d_bodies;
+    a.xfs = reinterpret_cast<const float*>(d_body_xfs);
+    a.sweeps = nullptr;
+    a.table = ctx->dShapes;
+    a.shapeCount = ctx->shapeCount;
+    a.pool = ctx->dPool;
+    a.aabbs = d_aabbs;
+    a.disp = nullptr;
+    a.status = ctx->dStatus;
+    return timed(ctx, s, "shape_aabb", [&] {
+        shape_aabb_kernel<<<blocks(n), 256, 0, s>>>(a);
+        return hipGetLastError();
+    });
+}
+
+}  // namespace b3amd
+
+extern "C" {
+
+int b3amd_broadphase_create(b3amd_context* ctx, b3amd_broadphase** out) {
+    if (!ctx || !out) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    b3amd_broadphase* bp = new b3amd_broadphase();
+    bp->ctx = ctx;
+    int rc;
+    if ((rc = ctx->alloc(16 * 4, (void**)&bp->dCounts)) || (rc = bp_grow_ids(bp, 1024u))) {
+        b3amd_broadphase_destroy(bp);
+        return rc;
+    }
+    bp_fill_kernel<<<1, 64, 0, ctx->stream>>>(16u, bp->dCounts, 0u);
+    if (hipGetLastError() != hipSuccess || hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        b3amd_broadphase_destroy(bp);
+        return fail(B3AMD_ERR_DEVICE, "broadphase set-up failed");
+    }
+    *out = bp;
+    return B3AMD_OK;
+}
+
+int b3amd_broadphase_destroy(b3amd_broadphase* bp) {
+    if (!bp) return B3AMD_OK;
+    b3amd_context* ctx = bp->ctx;
+    hipStreamSynchronize(ctx->stream);
+    void* ptrs[] = {bp->st.lower, bp->st.upper, bp->st.user, bp->st.flags, bp->dLeaves, bp->dKeys, bp->dKeys2,
+                    bp->dNodes, bp->dBoxes, bp->dCounts, bp->dPairKeys, bp->dPairKeys2, bp->dPairs,
+                    bp->dSortTmp, bp->dEdits};
+    for (void* p : ptrs) ctx->release(p);
+    delete bp;
+    return B3AMD_OK;
+}
+
+int b3amd_broadphase_create_proxies(b3amd_broadphase* bp, uint32_t n, const b3amd_aabb* aabbs,
+                                    const uint32_t* user_data, uint32_t* proxy_ids) {
+    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
+    if (n == 0) return B3AMD_OK;
+    if (!aabbs || !proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<BpEdit> edits(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        uint32_t id;
+        int rc = bp_take_id(bp, &id);
+        if (rc) return rc;
+        proxy_ids[i] = id;
+        BpEdit& e = edits[i];
+        e.id = id;
+        e.mode = kEditCreate;
+        e.user = user_data ? user_data[i] : i;
+        e.pad = 0u;
+        // fatAABB = aabb; fatAABB.Extend(b3_aabbExtension) (broad_phase.cpp:80-81, aabb3.h:85-90)
+        const b3amd_aabb& a = aabbs[i];
+        e.lower[0] = a.lower.x - kAabbExtension, e.lower[1] = a.lower.y - kAabbExtension,
+        e.lower[2] = a.lower.z - kAabbExtension;
+        e.upper[0] = a.upper.x + kAabbExtension, e.upper[1] = a.upper.y + kAabbExtension,
+        e.upper[2] = a.upper.z + kAabbExtension;
+    }
+    return bp_apply_edits(bp, edits);
+}
+
+int b3amd_broadphase_destroy_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids) {
+    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
+    if (n == 0) return B3AMD_OK;
+    if (!proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<BpEdit> edits(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t id = proxy_ids[i];
+        if (id >= bp->idCap || !bp->live[id]) return fail(B3AMD_ERR_ARGUMENT, "proxy %u is not live", id);
+        // RemoveLeaf frees the parent when the leaf is not the root, then FreeNode the leaf
+        // (dynamic_tree.cpp:72-79, 251-292).
+        bp->freeGroups.emplace_back(id, bp->count >= 2u);
+        bp->count--;
+        bp->live[id] = 0u;
+        edits[i] = BpEdit{id, kEditDestroy, 0u, 0u, {0, 0, 0}, {0, 0, 0}};
+    }
+    return bp_apply_edits(bp, edits);
+}
+
+int b3amd_broadphase_touch_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* proxy_ids) {
+    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
+    if (n == 0) return B3AMD_OK;
+    if (!proxy_ids) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    std::vector<BpEdit> edits(n);
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t id = proxy_ids[i];
+        if (id >= bp->idCap || !bp->live[id]) return fail(B3AMD_ERR_ARGUMENT, "proxy %u is not live", id);
+        edits[i] = BpEdit{id, kEditTouch, 0u, 0u, {0, 0, 0}, {0, 0, 0}};
+    }
+    return bp_apply_edits(bp, edits);
+}
+
+int b3amd_broadphase_move_proxies(b3amd_broadphase* bp, uint32_t n, const uint32_t* d_proxy_ids,
+                                  const b3amd_aabb* d_aabbs, const b3amd_vec3* d_displacements, uint32_t* d_moved) {
+    if (!bp) return fail(B3AMD_ERR_ARGUMENT, "null broadphase");
+    if (n == 0) return B3AMD_OK;
+    if (!d_proxy_ids || !d_aabbs || !d_displacements) return fail(B3AMD_ERR_ARGUMENT, "null argument");
+    hipStream_t s = bp->ctx->stream;
+    // A bad id is counted in the status word and reported by the next find_pairs.
+    return timed(bp->ctx, s, "bp_move", [&] {
+        bp_move_kernel<<<blocks(n), 256, 0, s>>>(n, d_proxy_ids, reinterpret_cast<const float*>(d_aabbs),
+                                                 reinterpret_cast<const float*>(d_displacements), bp->st, bp->idCap,
+                                                 d_moved, bp->dCounts + 9);
+        return hipGetLastError();
+    });
+}
+
+int b3amd_broadphase_find_pairs(b3amd_broadphase* bp, uint32_t* pair_count, const uint32_t** d_pairs) {
+    return bp_find_pairs(bp, pair_count, d_pairs, true);
 }
 
 int b3amd_broadphase_test_overlap(b3amd_broadphase* bp, uint32_t n, const uint32_t* d_proxy_pairs,
@@ -944,16 +1013,14 @@ int b3amd_shape_aabbs(b3amd_context* ctx, uint32_t n, const uint32_t* d_shapes, 
     int rc;
     if (!ctx->dStatus && (rc = ctx->alloc(64, (void**)&ctx->dStatus))) return rc;
     a.status = ctx->dStatus;
-    const uint32_t none = 0u;
-    HIP_TRY(hipMemcpyAsync(a.status, &none, 4, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_zero_words(a.status, 1, s));
     rc = timed(ctx, s, "shape_aabb", [&] {
         shape_aabb_kernel<<<blocks(n), 256, 0, s>>>(a);
         return hipGetLastError();
     });
     if (rc) return rc;
     uint32_t status = 0;
-    HIP_TRY(hipMemcpyAsync(&status, a.status, 4, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
+    if ((rc = ctx->read_back(a.status, &status, 4))) return rc;
     if (status)
         return fail(B3AMD_ERR_ARGUMENT, "shape entry %u: not a convex shape of the table or body beyond %u",
                     status - 1u, body_count);

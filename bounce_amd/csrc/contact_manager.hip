@@ -273,6 +273,9 @@ struct b3amd_contact_manager {
     std::vector<CmShape> shapes;
     uint32_t shapeCap = 0;
     CmShape* dShapes = nullptr;
+    // add_shapes scratch: table shapes, bodies, proxy records and AABBs of addCap shapes
+    uint32_t addCap = 0;
+    uint32_t* dAdd = nullptr;
     // contacts
     uint32_t L = 0, freeCount = 0, slotHigh = 0, cap = 0, updates = 0;
     CmRec* dRecs = nullptr;
@@ -411,7 +414,7 @@ int b3amd_contact_manager_destroy(b3amd_contact_manager* m) {
     void* ptrs[] = {m->dBodyFlags, m->dShapes, m->dRecs, m->dCaches, m->dManifolds, m->dDead, m->dOrder[0],
                     m->dOrder[1], m->dKeys[0], m->dKeys[1], m->dVals[0], m->dVals[1], m->dFree, m->dV, m->dPos,
                     m->dColSlots, m->dColPairs, m->dColBodies, m->dKeep, m->dKeepPos, m->dNewKeys, m->dNewVals,
-                    m->dStatus, m->dTmp};
+                    m->dStatus, m->dTmp, m->dAdd};
     for (void* p : ptrs) ctx->release(p);
     delete m;
     return B3AMD_OK;
@@ -459,32 +462,28 @@ int b3amd_contact_manager_add_shapes(b3amd_contact_manager* m, uint32_t n, const
         if (bodies[i] >= m->bodyCount) return fail(B3AMD_ERR_ARGUMENT, "entry %u: body %u beyond %u", i, bodies[i], m->bodyCount);
     }
     // b3Body::CreateShape: the shape's AABB at the body's m_xf, then CreateProxy with the
-    // shape as user data (body.cpp:110-115).
-    uint32_t *dS = nullptr, *dB = nullptr;
-    b3amd_aabb* dA = nullptr;
+    // shape as user data (body.cpp:110-115); the AABBs stay on the device.
     int rc;
-    if ((rc = ctx->alloc(4ull * n, (void**)&dS)) || (rc = ctx->alloc(4ull * n, (void**)&dB)) ||
-        (rc = ctx->alloc(24ull * n, (void**)&dA))) {
-        ctx->release(dS), ctx->release(dB), ctx->release(dA);
+    if (n > m->addCap) {
+        const uint32_t cap = std::max<uint32_t>(1024u, std::max(n, 2u * m->addCap));
+        if ((rc = cm_regrow(ctx, &m->dAdd, 0, 12ull * cap, false))) return rc;
+        m->addCap = cap;
+    }
+    uint32_t* dS = m->dAdd;                    // [n] table shapes
+    uint32_t* dB = dS + m->addCap;             // [n] bodies
+    uint32_t* dR = dB + m->addCap;             // [n][2] proxy records
+    float* dA = reinterpret_cast<float*>(dR + 2ull * m->addCap);  // [n][6] AABBs
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(dS, table_shapes, 4ull * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(dB, bodies, 4ull * n, hipMemcpyHostToDevice, s));
+    if ((rc = launch_shape_aabbs(ctx, n, dS, dB, m->bodyCount, d_body_xfs, dA))) return rc;
+    const uint32_t first = (uint32_t)m->shapes.size();
+    std::vector<uint32_t> users(n), ids(n), records(2ull * n);  // kept until the synchronisation below
+    for (uint32_t i = 0; i < n; ++i) users[i] = first + i;
+    if ((rc = bp_create_proxies_device(m->bp, n, dA, users.data(), ids.data(), records.data(), dR))) {
+        hipStreamSynchronize(s);
         return rc;
     }
-    std::vector<b3amd_aabb> aabbs(n);
-    std::vector<uint32_t> users(n);
-    hipStream_t s = ctx->stream;
-    rc = B3AMD_OK;
-    if (hipMemcpyAsync(dS, table_shapes, 4ull * n, hipMemcpyHostToDevice, s) != hipSuccess ||
-        hipMemcpyAsync(dB, bodies, 4ull * n, hipMemcpyHostToDevice, s) != hipSuccess)
-        rc = fail(B3AMD_ERR_DEVICE, "copy failed");
-    if (!rc) rc = b3amd_shape_aabbs(ctx, n, dS, dB, m->bodyCount, d_body_xfs, nullptr, dA, nullptr);
-    if (!rc && (hipMemcpyAsync(aabbs.data(), dA, 24ull * n, hipMemcpyDeviceToHost, s) != hipSuccess ||
-                hipStreamSynchronize(s) != hipSuccess))
-        rc = fail(B3AMD_ERR_DEVICE, "copy failed");
-    ctx->release(dS), ctx->release(dB), ctx->release(dA);
-    if (rc) return rc;
-    const uint32_t first = (uint32_t)m->shapes.size();
-    for (uint32_t i = 0; i < n; ++i) users[i] = first + i;
-    std::vector<uint32_t> ids(n);
-    if ((rc = b3amd_broadphase_create_proxies(m->bp, n, aabbs.data(), users.data(), ids.data()))) return rc;
     for (uint32_t i = 0; i < n; ++i) {
         m->shapes.push_back(CmShape{table_shapes[i], bodies[i], ids[i], ctx->shapes[table_shapes[i]].type});
         if (proxy_ids) proxy_ids[i] = ids[i];
@@ -543,8 +542,11 @@ int b3amd_contact_manager_find_new_contacts(b3amd_contact_manager* m, uint32_t* 
     }
     uint32_t P = 0;
     const uint32_t* dPairs = nullptr;
-    if ((rc = b3amd_broadphase_find_pairs(m->bp, &P, &dPairs))) return rc;
-    if (P == 0) return B3AMD_OK;
+    if ((rc = bp_find_pairs(m->bp, &P, &dPairs, false))) return rc;
+    if (P == 0) {
+        HIP_TRY(hipStreamSynchronize(s));  // the broadphase's move-buffer reset
+        return B3AMD_OK;
+    }
     if ((rc = cm_reserve_pairs(m, P))) return rc;
     const uint4* pairs = reinterpret_cast<const uint4*>(dPairs);
     rc = timed(ctx, s, "cm_filter", [&] {
