@@ -368,6 +368,17 @@ int cm_reserve_pairs(b3amd_contact_manager* m, uint32_t need) {
     return B3AMD_OK;
 }
 
+// add_shapes scratch for `need` shapes (12 words each).
+int cm_reserve_add(b3amd_contact_manager* m, uint32_t need) {
+    if (need <= m->addCap) return B3AMD_OK;
+    uint32_t cap = std::max<uint32_t>(m->addCap ? m->addCap : 4096u, 4096u);
+    while (cap < need) cap *= 2u;
+    int rc = cm_regrow(m->ctx, &m->dAdd, 0, 12ull * cap, false);
+    if (rc) return rc;
+    m->addCap = cap;
+    return B3AMD_OK;
+}
+
 template <class T>
 int read1(b3amd_contact_manager* m, const T* d, T* h) {
     return m->ctx->read_back(d, h, sizeof(T));
@@ -397,7 +408,8 @@ int b3amd_contact_manager_create(b3amd_context* ctx, b3amd_broadphase* bp, b3amd
     m->ctx = ctx;
     m->bp = bp;
     int rc;
-    if ((rc = ctx->alloc(64, (void**)&m->dStatus)) || (rc = cm_reserve(m, 4096u)) || (rc = cm_reserve_pairs(m, 4096u))) {
+    if ((rc = ctx->alloc(64, (void**)&m->dStatus)) || (rc = cm_reserve(m, 4096u)) || (rc = cm_reserve_pairs(m, 4096u)) ||
+        (rc = cm_reserve_add(m, 4096u))) {
         b3amd_contact_manager_destroy(m);
         return rc;
     }
@@ -464,11 +476,7 @@ int b3amd_contact_manager_add_shapes(b3amd_contact_manager* m, uint32_t n, const
     // b3Body::CreateShape: the shape's AABB at the body's m_xf, then CreateProxy with the
     // shape as user data (body.cpp:110-115); the AABBs stay on the device.
     int rc;
-    if (n > m->addCap) {
-        const uint32_t cap = std::max<uint32_t>(1024u, std::max(n, 2u * m->addCap));
-        if ((rc = cm_regrow(ctx, &m->dAdd, 0, 12ull * cap, false))) return rc;
-        m->addCap = cap;
-    }
+    if ((rc = cm_reserve_add(m, n))) return rc;
     uint32_t* dS = m->dAdd;                    // [n] table shapes
     uint32_t* dB = dS + m->addCap;             // [n] bodies
     uint32_t* dR = dB + m->addCap;             // [n][2] proxy records
