@@ -93,6 +93,26 @@ def test_gpu_broadphase_edge_cases(ctx):
         assert len(bp.find_pairs()) == 0
 
 
+def test_gpu_broadphase_query_overflowing_its_stack(ctx):
+    """One moved proxy whose box covers 4,000 others: the query's group of lanes meets more
+    pending nodes than its LDS stack holds, and the overflow is walked lane by lane."""
+    print("broadphase query over 4,000 covered proxies", flush=True)
+    ref = _oracle_bp()
+    rng = np.random.default_rng(9)
+    lo = rng.uniform(0.0, 20.0, (4000, 3)).astype(np.float32)
+    boxes = np.concatenate([lo, lo + 0.1], axis=1).astype(np.float32)
+    big = np.array([[-1, -1, -1, 21, 21, 21]], np.float32)
+    with ctx.broadphase() as bp:
+        ids = bp.create(boxes, np.arange(4000))
+        assert list(ids) == [ref.create(b, i) for i, b in enumerate(boxes)]
+        assert np.array_equal(bp.find_pairs(), ref.find_pairs())
+        c = bp.create(big, [4000])
+        assert c[0] == ref.create(big[0], 4000)
+        got, want = bp.find_pairs(), ref.find_pairs()
+        assert len(got) == 4000
+        assert np.array_equal(got, want)
+
+
 def test_gpu_shape_aabbs_equal_oracle(ctx):
     from bounce_amd.types import SWEEP_DTYPE
     from bounce_amd.workloads import quat_xf_rotation
